@@ -1,0 +1,220 @@
+/*
+ * vl1.h -- C ABI of the MI355X list-1 (near-field) evaluation library.
+ *
+ * Plain C types only: device buffers are raw pointers (allocated by the
+ * caller, e.g. torch tensors), streams are opaque `void*` hipStream_t
+ * handles (NULL = default stream).  No entry point throws; every one
+ * returns a vl1_status and records a message for vl1_last_error().
+ *
+ * Reference interfaces each entry point replaces (volumential):
+ *   vl1_payload_create   expansion_wrangler_fpnd.py:1805-2066,2939-3158
+ *                        (_prepare_table_data_and_entry_map,
+ *                         _get_cached_nearfield_payload: device upload of
+ *                         table_data_combined and the arithmetic-orbit
+ *                         case descriptors)
+ *   vl1_near_field_f64   list1.py:1260-1487 (NearFieldFromCSR.__call__ and
+ *                        the loopy `near_field` kernel, list1.py:1057-1200)
+ *   vl1_plan_create /
+ *   vl1_plan_eval        expansion_wrangler_fpnd.py:3222-3489 (eval_direct
+ *                        called once per RHS by volume_fmm.py:983-989);
+ *                        batched over RHS, FP64 or split-precision form
+ *   vl1_duffy_build      nearfield_potential_table.py:2034-2213,2328-2463
+ *                        (duffy_invariant_fused_table kernel and its driver)
+ *   vl1_orbit_rows_host  orbit_arithmetic.py:566-634 (host twin of the
+ *                        device compact-row function, for CPU-side checks)
+ */
+#ifndef VL1_H
+#define VL1_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VL1_ABI_VERSION 1
+
+typedef enum vl1_status {
+  VL1_OK = 0,
+  VL1_E_ARG = -1,         /* invalid argument (ValueError in the reference) */
+  VL1_E_LAYOUT = -2,      /* per-box node layout violation (ValueError) */
+  VL1_E_LEVEL = -3,       /* single-table level mismatch (RuntimeError) */
+  VL1_E_COVERAGE = -4,    /* table levels do not cover the tree (RuntimeError) */
+  VL1_E_TABLE = -5,       /* missing / non-finite table data (RuntimeError) */
+  VL1_E_NONFINITE = -6,   /* non-finite output (RuntimeError) */
+  VL1_E_HIP = -7,         /* HIP runtime failure */
+  VL1_E_UNSUPPORTED = -8  /* configuration this build does not implement */
+} vl1_status;
+
+typedef enum vl1_kernel_kind {
+  VL1_KERNEL_CONSTANT = 0,  /* G = 1 */
+  VL1_KERNEL_LAPLACE = 1,   /* 2D: -log|d|/(2 pi); 3D: 1/(4 pi |d|) */
+  VL1_KERNEL_HELMHOLTZ = 2  /* 2D: (i/4) H0^(1)(k|d|); 3D: e^{ik|d|}/(4 pi |d|) */
+} vl1_kernel_kind;
+
+/* list1.py:361-437 (get_kernel_scaling_policy) */
+typedef enum vl1_scaling_mode {
+  VL1_SCALING_CANONICAL = 0,  /* table level 0, scaling (h/H)^power [+ 2D log term] */
+  VL1_SCALING_FIXED = 1,      /* table level 0, scaling 1 */
+  VL1_SCALING_PER_LEVEL = 2   /* table level from source level, scaling 1 */
+} vl1_scaling_mode;
+
+/* Last error message of the calling thread ("" if none). */
+const char* vl1_last_error(void);
+int vl1_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Payload: device-resident table data + orbit descriptors.            */
+/* ------------------------------------------------------------------ */
+typedef struct vl1_payload vl1_payload;
+
+typedef struct vl1_payload_desc {
+  int dim;            /* 2 or 3 */
+  int quad_order;     /* q; n_q_points = q^dim */
+  int n_cases;        /* 33 (2D) / 139 (3D) */
+  int n_case_orbits;  /* canonical cases */
+  int n_tables;       /* table levels */
+  int is_complex;     /* 0: float64 values, 1: complex128 (interleaved re,im) */
+  int64_t n_table_entries;  /* compact rows per level */
+  /* host pointers, copied into the payload */
+  const double* table_data;            /* [n_tables][n_table_entries] (x2 if complex) */
+  const double* mode_nmlz;             /* [n_tables][n_q_points] (x2 if complex) */
+  const int64_t* case_indices;         /* [13^dim] */
+  const uint16_t* case_orbit_ranks;    /* [n_cases] */
+  const uint8_t* case_axis_perm;       /* [n_cases][dim] */
+  const int8_t* case_axis_sign;        /* [n_cases][dim] */
+  const uint8_t* case_axis_group;      /* [n_cases][dim] */
+  const int32_t* case_value_offsets;   /* [n_case_orbits + 1] */
+} vl1_payload_desc;
+
+int vl1_payload_create(vl1_payload** out, const vl1_payload_desc* desc, void* stream);
+int vl1_payload_destroy(vl1_payload* p);
+/* Bytes of device memory the payload holds. */
+int64_t vl1_payload_device_bytes(const vl1_payload* p);
+
+/* ------------------------------------------------------------------ */
+/* NearFieldFromCSR: general CSR evaluation, reference semantics.      */
+/* ------------------------------------------------------------------ */
+typedef struct vl1_near_field_args {
+  int scaling_mode;        /* vl1_scaling_mode */
+  int scaling_power;       /* canonical: exponent of (h/H) (2 for Laplace, d for constant) */
+  int log_displacement;    /* canonical 2D Laplace: add -(1/2pi) s log(h/H) nmlz */
+  double root_extent;
+  double table_root_extent;
+  int table_starting_level;
+  double encoding_shift;   /* 6 */
+  int encoding_base;       /* 13 */
+  double case_encoding_bias; /* 1e-10 */
+  int n_tgt_boxes;
+  int aligned_nboxes;
+  int n_rhs;               /* number of densities (>= 1) */
+  int64_t ld_coefs;        /* stride between densities, in elements */
+  int64_t ld_result;       /* stride between outputs, in elements */
+  /* device pointers */
+  double* result;                          /* [n_rhs][ld_result] (x2 if complex) */
+  const double* mode_coefs;                /* [n_rhs][ld_coefs] (x2 if complex) */
+  const double* box_centers;               /* [dim][aligned_nboxes] */
+  const uint8_t* box_levels;               /* [aligned_nboxes] */
+  const int32_t* box_source_counts_nonchild;
+  const int32_t* box_source_starts;
+  const int32_t* box_target_counts_nonchild;
+  const int32_t* box_target_starts;
+  const int32_t* neighbor_source_boxes_starts; /* [n_tgt_boxes + 1] */
+  const int32_t* neighbor_source_boxes_lists;
+  const int32_t* target_boxes;             /* [n_tgt_boxes] */
+  const int32_t* source_mode_ids;          /* per source particle */
+  const int32_t* target_point_ids;         /* per target particle */
+  int32_t* error_flags;                    /* optional [1], OR-ed: 1 = invalid case id */
+} vl1_near_field_args;
+
+/* Overwrites result[target] for every target of the listed boxes. */
+int vl1_near_field_f64(const vl1_payload* p, const vl1_near_field_args* a, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Plan: batched evaluation over many densities (RHS).                 */
+/* ------------------------------------------------------------------ */
+typedef struct vl1_plan vl1_plan;
+
+typedef enum vl1_eval_form {
+  VL1_FORM_F64 = 0,         /* dense per-case blocks, FP64 MFMA */
+  VL1_FORM_SPLIT_BF16 = 1   /* Ozaki split: BF16 MFMA, FP32 accumulate, FP64 sums */
+} vl1_eval_form;
+
+typedef struct vl1_plan_desc {
+  /* host pointers describing the tree and its list-1 CSR */
+  int dim;
+  int n_boxes;                 /* aligned_nboxes */
+  int n_tgt_boxes;
+  int64_t n_points;            /* particles (tree order) */
+  const double* box_centers;   /* [dim][n_boxes] */
+  const uint8_t* box_levels;   /* [n_boxes] */
+  const int32_t* box_starts;   /* [n_boxes] first particle of each box */
+  const int32_t* box_counts;   /* [n_boxes] particles of each box */
+  const int32_t* target_boxes; /* [n_tgt_boxes] */
+  const int32_t* nbr_starts;   /* [n_tgt_boxes + 1] */
+  const int32_t* nbr_lists;
+  const int32_t* local_ids;    /* [n_points] box-local node id of each particle */
+  double root_extent;
+  double table_root_extent;
+  int table_starting_level;
+  int scaling_mode;
+  int scaling_power;
+  int log_displacement;
+  double case_encoding_bias;
+  int boxes_per_tile;          /* 0 = choose */
+} vl1_plan_desc;
+
+typedef struct vl1_split_opts {
+  int n_slices;      /* BF16 slices per operand (1..6) */
+  int n_products;    /* 0 = all slice products with i + j < n_slices */
+} vl1_split_opts;
+
+int vl1_plan_create(vl1_plan** out, const vl1_payload* p, const vl1_plan_desc* d, void* stream);
+int vl1_plan_destroy(vl1_plan* plan);
+/* f: [n_rhs][ldf] densities (tree order), y: [n_rhs][ldy] outputs;
+ * every target of the plan's target boxes is overwritten. */
+int vl1_plan_eval(vl1_plan* plan, int form, const double* f, int64_t ldf, int n_rhs,
+                  double* y, int64_t ldy, const vl1_split_opts* opts, void* stream);
+/* Plan statistics: [0] list-1 pairs, [1] tiles, [2] (tile, case) steps,
+ * [3] distinct (case, level) blocks, [4] device bytes. */
+int vl1_plan_stats(const vl1_plan* plan, int64_t* out5);
+
+/* ------------------------------------------------------------------ */
+/* Duffy table build (one launch over all canonical entries).          */
+/* ------------------------------------------------------------------ */
+typedef struct vl1_duffy_args {
+  int dim;
+  int quad_order;
+  int kernel_kind;          /* vl1_kernel_kind */
+  double helmholtz_k;
+  int64_t n_entries;
+  int64_t n_nodes;
+  int64_t nodes_per_sign;   /* nodes per sign vector (perm x tail x radial) */
+  double source_box_extent;
+  /* device pointers */
+  const double* target_points;   /* [dim][n_entries] */
+  const double* decomposition;   /* [dim][n_entries] clamp(target, 0, H) */
+  const int32_t* mode_axes;      /* [dim][n_entries] */
+  const double* node_u;          /* [dim][n_nodes] */
+  const double* node_sign;       /* [dim][n_nodes] */
+  const double* node_jac;        /* [n_nodes] */
+  const double* interp_nodes;    /* [quad_order] */
+  const double* bary_w;          /* [quad_order] */
+  double* result;                /* [n_entries] (x2 interleaved if Helmholtz) */
+} vl1_duffy_args;
+
+int vl1_duffy_build(const vl1_duffy_args* a, void* stream);
+
+/* Host evaluation of the compact-row function the device kernels use
+ * (orbit_arithmetic.py:566-634 _evaluate_compact_arithmetic_orbit_entry,
+ * row relative to the orbit's first row): out[i] = row(s[i], t[i]) for one
+ * case's descriptors perm/sign/group [dim]. */
+int vl1_orbit_rows_host(int dim, int q, const uint8_t* perm, const int8_t* sign,
+                        const uint8_t* group, const int32_t* s, const int32_t* t,
+                        int64_t count, int64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VL1_H */

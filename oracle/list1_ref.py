@@ -1,0 +1,132 @@
+"""Oracle: the ``near_field`` list-1 kernel semantics in numpy.
+
+Restates volumential/list1.py:1057-1148 (kernel body), 439-459 and
+1083-1091 (float case-id encoding with bias and +0.5 truncation), 278-437
+(scaling policies) and 604-902 (arithmetic orbit reconstruction, via
+``orbit_ref.rows_for_case``).  Loops over (target box, source box) pairs in
+Python and contracts each pair's q^d x q^d block with numpy, so it is meant
+for small trees.  ``mode_coefs`` may hold several densities ``[R, n]``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import orbit_ref
+
+
+def case_id_float(ct, cs, level, root_extent, case_indices, bias=1e-10, shift=6.0, base=13):
+    sext = root_extent * (1.0 / (2**int(level)))
+    code = 0.0
+    for d in range(len(ct)):
+        code = code * base + ((ct[d] - cs[d]) / sext * 4.0 + shift + bias)
+    vid = int(code + 0.5)
+    if vid < 0 or vid >= case_indices.size:
+        return -1, sext
+    return int(case_indices[vid]), sext
+
+
+def case_id_exact(ct, cs, level, root_extent, case_indices):
+    """Integer case vector from rounded offsets (exact for dyadic trees)."""
+    sext = root_extent / 2**int(level)
+    vec = np.rint((np.asarray(ct) - np.asarray(cs)) / sext * 4.0).astype(int)
+    return int(case_indices[orbit_ref.encode(vec)])
+
+
+def near_field(
+    *,
+    dim,
+    q,
+    box_centers,
+    box_levels,
+    box_starts,
+    box_counts,
+    target_boxes,
+    nbr_starts,
+    nbr_lists,
+    source_mode_ids,
+    target_point_ids,
+    table_data,
+    mode_nmlz,
+    meta,
+    mode,
+    scaling_power=2,
+    log_displacement=False,
+    root_extent,
+    table_root_extent,
+    table_starting_level=0,
+    mode_coefs,
+    result=None,
+    bias=1e-10,
+):
+    """``mode``: "canonical", "fixed" or "per_level" (list1.py:361-437)."""
+    coefs = np.atleast_2d(np.asarray(mode_coefs))
+    R, npts = coefs.shape
+    table = np.atleast_2d(np.asarray(table_data))
+    nmlz = np.atleast_2d(np.asarray(mode_nmlz))
+    n_tables = table.shape[0]
+    out = np.zeros((R, npts), dtype=np.result_type(coefs, table)) if result is None else (
+        np.atleast_2d(np.array(result, copy=True))
+    )
+    case_indices = np.asarray(meta["case_indices"])
+    rows_cache = {}
+    for i, tb in enumerate(np.asarray(target_boxes)):
+        tbeg, nt = int(box_starts[tb]), int(box_counts[tb])
+        tmodes = np.asarray(target_point_ids[tbeg : tbeg + nt])
+        ct = box_centers[:, tb]
+        acc = np.zeros((R, nt), dtype=out.dtype)
+        for sb in np.asarray(nbr_lists[nbr_starts[i] : nbr_starts[i + 1]]):
+            lvl = int(box_levels[sb])
+            cid, sext = case_id_float(ct, box_centers[:, sb], lvl, root_extent, case_indices, bias)
+            if cid < 0:
+                raise RuntimeError("invalid list-1 case")
+            if mode == "canonical":
+                num = sext**scaling_power
+                den = table_root_extent**scaling_power
+                scal = num / den
+                lev = 0
+            elif mode == "fixed":
+                scal, lev = 1.0, 0
+            else:
+                scal = 1.0
+                rel = lvl - table_starting_level
+                lev = 0 if lvl < table_starting_level else (rel if rel < n_tables else n_tables - 1)
+            sbeg, ns = int(box_starts[sb]), int(box_counts[sb])
+            smodes = np.asarray(source_mode_ids[sbeg : sbeg + ns])
+            if cid not in rows_cache:
+                rows_cache[cid] = orbit_ref.rows_for_case(meta, cid, q, dim)
+            rows = rows_cache[cid][np.ix_(smodes, tmodes)]  # [ns, nt]
+            integ = table[lev][rows] * scal
+            if mode == "canonical" and log_displacement:
+                disp = -0.5 / np.pi * scal * np.log(sext / table_root_extent) * nmlz[lev, :ns]
+                integ = integ + disp[:, None]
+            acc += coefs[:, sbeg : sbeg + ns] @ integ
+        out[:, tbeg : tbeg + nt] = acc
+    return out
+
+
+def near_field_from_tree(tree, table_data, mode_nmlz, meta, mode_coefs, *, mode, scaling_power=2,
+                         log_displacement=False, table_root_extent, table_starting_level=0):
+    return near_field(
+        dim=tree.dim,
+        q=tree.quad_order,
+        box_centers=tree.box_centers,
+        box_levels=tree.box_levels,
+        box_starts=tree.box_target_starts,
+        box_counts=tree.box_target_counts_nonchild,
+        target_boxes=tree.target_boxes,
+        nbr_starts=tree.neighbor_source_boxes_starts,
+        nbr_lists=tree.neighbor_source_boxes_lists,
+        source_mode_ids=tree.local_ids,
+        target_point_ids=tree.local_ids,
+        table_data=table_data,
+        mode_nmlz=mode_nmlz,
+        meta=meta,
+        mode=mode,
+        scaling_power=scaling_power,
+        log_displacement=log_displacement,
+        root_extent=tree.root_extent,
+        table_root_extent=table_root_extent,
+        table_starting_level=table_starting_level,
+        mode_coefs=mode_coefs,
+    )

@@ -1,0 +1,163 @@
+"""GPU parity of the plain FP64 list-1 evaluation (vl1_near_field_f64)
+against the CPU oracles, through the wrangler's eval_direct drop-in.
+
+Tolerance (north star, FP64 form): rel-L2 and rel-Linf <= 1e-12 per RHS.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu as ocpu
+from oracle import list1_ref, orbit_ref
+from tests.gpu_helpers import adaptive, rel_errors, synthetic_table_levels
+from volumential_amd import densities, tree as vtree
+from volumential_amd.wrangler import NearFieldWrangler
+
+TOL = 1e-12
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(tr, tables, coefs, **kw):
+    w = NearFieldWrangler(tr, tables if len(tables) > 1 else tables[0], **kw)
+    c = torch.as_tensor(coefs, device="cuda")
+    if c.dim() == 1:
+        pot, _ = w.eval_direct(tr.target_boxes, tr.neighbor_source_boxes_starts,
+                               tr.neighbor_source_boxes_lists, c)
+    else:
+        pot, _ = w.eval_direct_batched(tr.target_boxes, tr.neighbor_source_boxes_starts,
+                                       tr.neighbor_source_boxes_lists, c)
+    return pot[0].cpu().numpy()
+
+
+def _oracle(tr, tables, coefs, mode, power=2, logd=False):
+    meta = orbit_ref.case_metadata(tr.dim, tr.quad_order)
+    data = np.stack([t.compact_values() for t in tables])
+    nm = np.stack([t.mode_normalizers for t in tables])
+    tsl = int(round(np.log2(tr.root_extent / tables[0].source_box_extent)))
+    return list1_ref.near_field_from_tree(
+        tr, data, nm, meta, coefs, mode=mode, scaling_power=power, log_displacement=logd,
+        table_root_extent=tables[0].source_box_extent, table_starting_level=tsl,
+    )
+
+
+def test_laplace2d_q4_level5_log_term(cuda):
+    """C1 geometry: 2D, q=4, uniform level 5 (1024 boxes), canonical scaling
+    with the FP64 log displacement, single RHS."""
+    print("case: 2D q=4 level 5, canonical + log", flush=True)
+    tr = vtree.uniform_tree(2, 4, 5)
+    tables = synthetic_table_levels(2, 4)
+    f = np.exp(-np.sum(tr.points**2, axis=0) / 0.01)
+    got = _run(tr, tables, f)
+    ref = _oracle(tr, tables, f, "canonical", 2, True)[0]
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL and linf <= TOL, (l2, linf)
+
+
+@pytest.mark.parametrize("n_rhs", [1, 3, 6, 20])
+def test_laplace3d_q6_level2_batched(cuda, n_rhs):
+    print(f"case: 3D q=6 level 2, R={n_rhs}", flush=True)
+    tr = vtree.uniform_tree(3, 6, 2)
+    tables = synthetic_table_levels(3, 6, seed=1)
+    f = densities.gaussian_mixture_batch(tr.points, range(1000, 1000 + n_rhs))
+    got = _run(tr, tables, f if n_rhs > 1 else f[0])
+    meta = orbit_ref.case_metadata(3, 6)
+    ref = ocpu.near_field(tr, tables[0].compact_values(), tables[0].mode_normalizers, meta, f,
+                          mode="canonical", table_root_extent=2.0)
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL and linf <= TOL, (l2, linf)
+
+
+@pytest.mark.parametrize("dim,q", [(2, 3), (2, 6), (3, 2), (3, 5)])
+def test_adaptive_per_level_tables(cuda, dim, q):
+    print(f"case: adaptive {dim}D q={q} per-level tables", flush=True)
+    tr = adaptive(dim, q)
+    tables = synthetic_table_levels(dim, q, n_tables=tr.nlevels, seed=5)
+    f = densities.gaussian_mixture_batch(tr.points, [11, 12])
+    got = _run(tr, tables, f)
+    ref = _oracle(tr, tables, f, "per_level")
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL and linf <= TOL, (l2, linf)
+
+
+@pytest.mark.parametrize("dim,q", [(2, 4), (3, 3)])
+def test_adaptive_canonical_scaling(cuda, dim, q):
+    print(f"case: adaptive {dim}D q={q} canonical", flush=True)
+    tr = adaptive(dim, q, max_level=5)
+    tables = synthetic_table_levels(dim, q, seed=6)
+    f = densities.gaussian_mixture_batch(tr.points, [21, 22, 23])
+    got = _run(tr, tables, f)
+    ref = _oracle(tr, tables, f, "canonical", 2, dim == 2)
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL and linf <= TOL, (l2, linf)
+
+
+def test_permuted_box_local_ids(cuda):
+    """Particles inside a box in any order (source_mode_ids/target_point_ids)."""
+    print("case: permuted local ids", flush=True)
+    tr = vtree.uniform_tree(2, 3, 3)
+    rng = np.random.default_rng(9)
+    n = tr.n_q_points
+    perm = np.concatenate([rng.permutation(n) for _ in range(tr.nboxes)]).astype(np.int32)
+    tr.local_ids = perm
+    tables = synthetic_table_levels(2, 3, seed=2)
+    f = densities.gaussian_mixture_batch(tr.points, [31, 32])
+    got = _run(tr, tables, f)
+    ref = _oracle(tr, tables, f, "canonical", 2, True)
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL and linf <= TOL, (l2, linf)
+
+
+def test_only_listed_targets_overwritten(cuda):
+    """Targets of unlisted boxes keep the caller's zeros."""
+    from volumential_amd.list1 import NearFieldFromCSR  # noqa: F401
+
+    print("case: partial target list", flush=True)
+    tr = vtree.uniform_tree(2, 3, 3)
+    tables = synthetic_table_levels(2, 3, seed=3)
+    f = densities.gaussian_mixture(tr.points, 41)
+    w = NearFieldWrangler(tr, tables[0])
+    sel = np.arange(0, tr.nboxes, 3)
+    starts = np.zeros(sel.size + 1, np.int32)
+    lists = []
+    for k, b in enumerate(sel):
+        seg = tr.neighbor_source_boxes_lists[
+            tr.neighbor_source_boxes_starts[b] : tr.neighbor_source_boxes_starts[b + 1]
+        ]
+        lists.extend(seg)
+        starts[k + 1] = starts[k] + len(seg)
+    pot, _ = w.eval_direct(sel.astype(np.int32), starts, np.asarray(lists, np.int32),
+                           torch.as_tensor(f, device="cuda"))
+    got = pot[0].cpu().numpy()
+    full = _oracle(tr, tables, f, "canonical", 2, True)[0]
+    n = tr.n_q_points
+    mask = np.zeros(tr.n_points, bool)
+    for b in sel:
+        mask[b * n : (b + 1) * n] = True
+    assert np.all(got[~mask] == 0.0)
+    l2, linf = rel_errors(got[mask], full[mask])
+    assert l2 <= TOL and linf <= TOL
+
+
+def test_error_paths(cuda):
+    print("case: error paths", flush=True)
+    tr = vtree.uniform_tree(2, 3, 2)
+    tables = synthetic_table_levels(2, 3)
+    f = torch.ones(tr.n_points, dtype=torch.float64, device="cuda")
+    # a source box far away in the negative direction -> invalid case code
+    w = NearFieldWrangler(tr, tables[0])
+    bad_lists = np.array([tr.nboxes - 1], np.int32)
+    with pytest.raises(RuntimeError, match="invalid case id"):
+        w.eval_direct(np.array([0], np.int32), np.array([0, 1], np.int32), bad_lists, f)
+    # fixed single table (no inferred scaling) at a level that does not match
+    w2 = NearFieldWrangler(tr, tables[0], list1_extra_kwargs={"infer_kernel_scaling": False})
+    with pytest.raises(RuntimeError, match="level mismatch"):
+        w2.eval_direct(tr.target_boxes, tr.neighbor_source_boxes_starts,
+                       tr.neighbor_source_boxes_lists, f)
+    # layout: a box with the wrong node count
+    tr.box_target_counts_nonchild = tr.box_target_counts_nonchild.copy()
+    tr.box_target_counts_nonchild[3] = 5
+    with pytest.raises(ValueError, match="quadrature points per active"):
+        w.eval_direct(tr.target_boxes, tr.neighbor_source_boxes_starts,
+                      tr.neighbor_source_boxes_lists, f)

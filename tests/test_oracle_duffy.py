@@ -1,0 +1,105 @@
+"""Pins of the Duffy-quadrature oracle (CPU).
+
+* node rule: tanh-sinh-fast integrates smooth functions on (0,1), GL tail
+  weights sum to 1, the Duffy node set of the product equals the oracle's;
+* constant kernel: each entry equals the product of GL weights, whatever the
+  target point (reference: test_nearfield_potential_table.py:107-132 gives
+  1 at q=1 and 0.25 at q=2 in 2D);
+* Laplace 2D/3D: the sum over source modes equals the closed-form potential
+  of a uniform box at the target point;
+* symmetry: a non-canonical entry integrated directly equals the canonical
+  value at its compact row (pins target-point geometry + orbit arithmetic);
+* 2D: the triangle form and the n-D form agree.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import duffy_ref as D
+from volumential_amd import orbit, quadrature
+
+
+def test_tanh_sinh_fast_rule():
+    x, w = D.tanh_sinh_fast(61)
+    assert np.all((x > 0) & (x < 1))
+    assert abs(np.sum(w) - 1.0) < 1e-13
+    assert abs(np.dot(w, np.sqrt(x)) - 2.0 / 3.0) < 1e-12
+    assert abs(np.dot(w, np.log(x)) + 1.0) < 1e-10
+    xp, wp = quadrature.tanh_sinh_fast(61)
+    assert np.array_equal(x, xp) and np.array_equal(w, wp)
+
+
+@pytest.mark.parametrize("dim,reg,rad", [(2, 6, 41), (3, 4, 41)])
+def test_duffy_node_set_matches_oracle_construction(dim, reg, rad):
+    ns = quadrature.duffy_node_set(dim, reg, rad)
+    # integrate 1 and a polynomial over the unit box with the singular
+    # vertex at a generic interior point: sum over sign blocks
+    v = np.full(dim, 0.3)
+    H = 1.0
+    length = np.where(ns.node_sign < 0, v[:, None], H - v[:, None])
+    y = v[:, None] + ns.node_sign * length * ns.node_u
+    w = ns.node_jac * np.prod(length, axis=0)
+    assert abs(np.sum(w) - 1.0) < 1e-12
+    f = np.prod(y**2, axis=0)
+    assert abs(np.dot(w, f) - (1.0 / 3.0) ** dim) < 1e-10
+    total = D.box_quad_duffy_nd(lambda *x: np.prod(np.array(x) ** 2, axis=0), [(0, 1)] * dim, v, reg, rad)
+    assert abs(total - np.dot(w, f)) < 1e-13
+
+
+@pytest.mark.parametrize("dim,q", [(2, 1), (2, 2), (2, 3), (3, 2)])
+def test_constant_kernel_is_gl_weight(dim, q):
+    g = D.TableGeometry(dim, q, H=1.0)
+    w = np.polynomial.legendre.leggauss(q)[1] * 0.5
+    n = g.n
+    rng = np.random.default_rng(0)
+    for e in rng.integers(0, len(g.vecs) * n * n, 6):
+        s = (e % (n * n)) // n
+        axes = [(s // q ** (dim - 1 - k)) % q for k in range(dim)]
+        expect = np.prod([w[a] for a in axes])
+        val = g.entry(int(e), D.constant(dim), 12, 31)
+        assert abs(val - expect) < 1e-13
+
+
+@pytest.mark.parametrize("case", [0, 69, 100, 138])
+def test_laplace3d_mode_sum_is_box_potential(case):
+    g = D.TableGeometry(3, 2, H=1.0)
+    for t in (0, 7):
+        x = g.target_point(t, case)
+        tot = sum(g.entry(case * 64 + s * 8 + t, D.laplace(3), 20, 61) for s in range(8))
+        cf = D.box_potential_laplace3d(x, [0, 0, 0], [1, 1, 1])
+        assert abs(tot - cf) <= 1e-12 * abs(cf)
+
+
+@pytest.mark.parametrize("case", [0, 5, 16, 27])
+def test_laplace2d_mode_sum_is_box_potential(case):
+    g = D.TableGeometry(2, 3, H=2.0)
+    for t in (0, 4, 8):
+        x = g.target_point(t, case)
+        tot = sum(g.entry(case * 81 + s * 9 + t, D.laplace(2), 30, 91) for s in range(9))
+        cf = D.box_potential_laplace2d(x, [0, 0], [2, 2])
+        assert abs(tot - cf) <= 1e-12 * max(1.0, abs(cf))
+
+
+def test_laplace2d_triangle_form_agrees():
+    g = D.TableGeometry(2, 3, H=1.0)
+    for e in (5 * 81 + 3 * 9 + 1, 16 * 81 + 4 * 9 + 4, 30 * 81 + 8 * 9 + 2):
+        a = g.entry(e, D.laplace(2), 20, 61)
+        b = g.entry(e, D.laplace(2), 30, 81, triangles_2d=True)
+        assert abs(a - b) <= 1e-9 * max(1.0, abs(a))
+
+
+@pytest.mark.parametrize("dim,q", [(2, 3), (3, 2)])
+def test_symmetric_entries_equal_canonical(dim, q):
+    """T(case, s, t) computed directly equals T at its canonical row."""
+    g = D.TableGeometry(dim, q, H=1.0)
+    m = orbit.build_case_metadata(dim, q)
+    reps = orbit.enumerate_representatives(m)
+    n = q**dim
+    rng = np.random.default_rng(1)
+    knl = D.laplace(dim)
+    for e in rng.integers(0, m.n_cases * n * n, 5):
+        c, s, t = e // (n * n), (e % (n * n)) // n, e % n
+        row = int(orbit.entry_rows(m, c, s, t))
+        direct = g.entry(int(e), knl, 20, 61)
+        canon = g.entry(int(reps[row]), knl, 20, 61)
+        assert abs(direct - canon) <= 1e-12 * max(1.0, abs(canon))

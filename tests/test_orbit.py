@@ -1,0 +1,100 @@
+"""Gallery and arithmetic orbit reduction: product vs oracle vs known answers.
+
+Known-answer values from the reference's own tests:
+* test_list1_gallery.py:27-64 -- 7 cases in 1D with 7 distinct components,
+  33 in 2D with 11 distinct components, all in -6..6;
+* test_nearfield_potential_table.py:2633-2670 -- q=3 3D: 2510 canonical
+  rows, 7290 dense arithmetic layout rows, 1576 metadata bytes;
+* test_nearfield_potential_table.py:2361-2383 -- sorting network row.
+Brute force: every entry's compact row must be the position of the
+smallest full id of its orbit under all 2^d d! signed axis permutations.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import orbit_ref
+from volumential_amd import gallery, orbit
+
+
+@pytest.mark.parametrize("dim,count,distinct", [(1, 7, 7), (2, 33, 11), (3, 139, 11)])
+def test_gallery_counts(dim, count, distinct):
+    vecs, encode, case_indices = gallery.generate_list1_gallery(dim)
+    assert len(vecs) == count
+    comps = {c for v in vecs for c in v}
+    assert len(comps) == distinct and min(comps) == -6 and max(comps) == 6
+    assert case_indices.size == 13**dim
+    for cid, v in enumerate(vecs):
+        assert case_indices[encode(v)] == cid
+    assert np.count_nonzero(case_indices >= 0) == count
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_gallery_matches_tree_walk(dim):
+    vecs, _, ci = gallery.generate_list1_gallery(dim)
+    vr, cir = orbit_ref.gallery(dim)
+    assert vecs == [tuple(v) for v in vr]
+    assert np.array_equal(ci, cir)
+
+
+def test_q3_3d_payload_known_answers():
+    m = orbit.build_case_metadata(3, 3)
+    assert m.n_compact_entries == 2510
+    assert m.n_orbits * 27 * 27 == 7290
+    assert m.metadata_bytes == 1576
+    reps = orbit.enumerate_representatives(m)
+    assert reps.size == np.unique(reps).size == 2510
+
+
+def test_sorting_network_known_answer():
+    q = 3
+    meta = {
+        "case_orbit_ranks": np.array([0], np.uint16),
+        "case_axis_perm": np.array([[0, 1, 2]], np.uint8),
+        "case_axis_sign": np.array([[1, 1, 1]], np.int8),
+        "case_axis_group": np.array([[1, 0, 1]], np.uint8),
+    }
+    row, sign = orbit_ref.arithmetic_entry(0, 2 * q * q, 2 * q * q, meta, q, 3)
+    assert row == 2 * 27 + 2 and sign == 1
+
+
+@pytest.mark.parametrize("dim,q", [(2, 1), (2, 2), (2, 3), (2, 4), (3, 1), (3, 2), (3, 3)])
+def test_rows_match_bruteforce_orbits(dim, q):
+    m = orbit.build_case_metadata(dim, q)
+    reps = orbit.enumerate_representatives(m)
+    mins = orbit_ref.orbit_min_ids(dim, q)
+    assert np.array_equal(np.sort(reps), np.unique(mins))
+    n = q**dim
+    c, s, t = np.meshgrid(np.arange(m.n_cases), np.arange(n), np.arange(n), indexing="ij")
+    rows = orbit.entry_rows(m, c.ravel(), s.ravel(), t.ravel())
+    pos = np.empty(int(reps.max()) + 1, dtype=np.int64)
+    pos[reps] = np.arange(reps.size)
+    assert np.array_equal(rows, pos[mins])
+    mo = orbit_ref.case_metadata(dim, q)
+    assert np.array_equal(reps, orbit_ref.representatives(mo, q, dim))
+    rows_o = np.concatenate([orbit_ref.rows_for_case(mo, cc, q, dim).ravel() for cc in range(m.n_cases)])
+    assert np.array_equal(rows_o, rows)
+
+
+@pytest.mark.parametrize("dim,q,expected", [(3, 6, 140496), (3, 5, 48230), (2, 4, 1084), (2, 6, 5409)])
+def test_compact_counts(dim, q, expected):
+    m = orbit.build_case_metadata(dim, q)
+    assert m.n_compact_entries == expected
+    # the compact count equals the number of orbits: sum over canonical cases
+    # of the multiset counts, checked against the scalar oracle formula
+    mo = orbit_ref.case_metadata(dim, q)
+    assert int(mo["n_compact_entries"]) == expected
+
+
+def test_scalar_compact_entry_matches_vectorised():
+    rng = np.random.default_rng(3)
+    for dim, q in [(2, 5), (3, 4)]:
+        m = orbit.build_case_metadata(dim, q)
+        mo = orbit_ref.case_metadata(dim, q)
+        n = q**dim
+        c = rng.integers(0, m.n_cases, 300)
+        s = rng.integers(0, n, 300)
+        t = rng.integers(0, n, 300)
+        rows = orbit.entry_rows(m, c, s, t)
+        ref = [orbit_ref.compact_entry(int(a), int(b), int(d), mo, q, dim)[0] for a, b, d in zip(c, s, t)]
+        assert np.array_equal(rows, ref)

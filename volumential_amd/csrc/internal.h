@@ -1,0 +1,36 @@
+// Internal object layouts shared by the translation units of libvl1.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+struct vl1_payload {
+  int dim = 0, q = 0, n = 0, n_cases = 0, n_orbits = 0, n_tables = 0, is_complex = 0;
+  int64_t n_entries = 0;   // compact rows per table level
+  int rowmap_u16 = 1;      // rowmap element type: 1 = uint16, 0 = int32
+  int max_orbit_count = 0; // largest canonical orbit (rows)
+  // host copies of the small descriptors
+  std::vector<int64_t> case_indices;
+  std::vector<uint16_t> orbit_ranks;
+  std::vector<uint8_t> perm, group;
+  std::vector<int8_t> sign;
+  std::vector<int32_t> offsets;
+  // device buffers
+  vl1::DevBuf table;       // [n_tables][n_entries] (x2 complex)
+  vl1::DevBuf nmlz;        // [n_tables][n] (x2 complex)
+  vl1::DevBuf case_idx;    // int64 [13^dim]
+  vl1::DevBuf ranks;       // uint16 [n_cases]
+  vl1::DevBuf offs;        // int32 [n_orbits+1]
+  vl1::DevBuf rowmap;      // [n_cases][n (s)][n (t)] relative compact rows
+  int64_t device_bytes() const {
+    return (int64_t)(table.bytes + nmlz.bytes + case_idx.bytes + ranks.bytes + offs.bytes +
+                     rowmap.bytes);
+  }
+};
+
+namespace vl1 {
+// Launchers implemented in the .hip translation units.
+hipError_t launch_rowmap(const vl1_payload& p, const uint8_t* d_perm, const int8_t* d_sign,
+                         const uint8_t* d_group, hipStream_t s);
+}  // namespace vl1

@@ -1,0 +1,286 @@
+// Plain FP64 list-1 evaluation with the reference's general CSR semantics.
+//
+// Semantics follow the loopy `near_field` kernel of
+// volumential/list1.py:1057-1148: for every target box of `target_boxes`
+// (CSR order) and every target particle `tid` of the box,
+//   result[tbeg + tid] = sum_{sbox in list1} sum_{sid}
+//       coef[sbeg + sid] * (T[lev, row(case, s, t)] * scaling + displacement)
+// with case = case_indices[int(sum_d ((c_t - c_s)/h_s*4 + shift + bias) base^k + 0.5)]
+// (list1.py:439-459, 1083-1091), s/t the box-local mode ids of the particles,
+// and the scaling/level policy of list1.py:361-437.
+//
+// GPU mapping: one workgroup per target box, taken in CSR order.  For every
+// source box the canonical orbit block of the table (all compact rows of the
+// case's orbit) is staged in LDS when it fits, together with the source
+// densities of up to RC right-hand sides; each lane owns one target node and
+// walks the source modes, gathering T through the precomputed per-case row
+// map (coalesced over target nodes).  For boxes with fewer than 256 nodes the
+// lanes are split into P parts over the source modes and reduced through LDS
+// at the end ("wavefront reduction over list-1 sources").
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "internal.h"
+
+namespace vl1 {
+
+struct NFParams {
+  const double* centers;
+  int nboxes;
+  const uint8_t* levels;
+  const int32_t* src_counts;
+  const int32_t* src_starts;
+  const int32_t* tgt_counts;
+  const int32_t* tgt_starts;
+  const int32_t* nbr_starts;
+  const int32_t* nbr_lists;
+  const int32_t* target_boxes;
+  const int32_t* src_ids;
+  const int32_t* tgt_ids;
+  const int64_t* case_indices;
+  int n_codes;
+  const uint16_t* ranks;
+  const int32_t* offs;
+  const double* table;
+  int64_t n_entries;
+  int n_tables;
+  const double* nmlz;
+  const void* rowmap;
+  int dim, n;
+  int scaling_mode, scaling_power, log_disp;
+  double root_extent, table_root_extent;
+  int table_starting_level;
+  double enc_shift, enc_bias;
+  int enc_base;
+  int n_rhs;
+  int64_t ldf, ldy;
+  const double* f;
+  double* y;
+  int32_t* err;
+  int tbuf;   // doubles of LDS for the staged orbit block
+  int nt;     // lanes per target set (power of two)
+};
+
+constexpr int kNFBlock = 256;
+
+template <typename IdxT, int RC>
+__global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams p) {
+  extern __shared__ double smem[];
+  const int n = p.n;
+  const int P = kNFBlock / p.nt;
+  double* sT = smem;
+  double* sF = sT + p.tbuf;
+  double* sD = sF + (size_t)n * RC;
+  double* red = sD + n;
+  int* sMode = (int*)(red + (P > 1 ? kNFBlock * RC : 0));
+
+  const int tid = threadIdx.x;
+  const int lane_t = tid % p.nt;
+  const int part = tid / p.nt;
+
+  const int tbox = blockIdx.x;
+  const int tb = p.target_boxes[tbox];
+  const int tbeg = p.tgt_starts[tb];
+  const int ntb = min(p.tgt_counts[tb], n);
+  const int l0 = p.nbr_starts[tbox];
+  const int l1 = p.nbr_starts[tbox + 1];
+  double ct[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < p.dim; ++d) ct[d] = p.centers[(int64_t)d * p.nboxes + tb];
+
+  for (int r0 = 0; r0 < p.n_rhs; r0 += RC) {
+    const int rc = min(RC, p.n_rhs - r0);
+    for (int t0 = 0; t0 < ntb; t0 += p.nt) {
+      const int ti = t0 + lane_t;
+      const int tm_raw = ti < ntb ? p.tgt_ids[tbeg + ti] : -1;
+      const bool tact = (unsigned)tm_raw < (unsigned)n;
+      const int tmode = tact ? tm_raw : 0;
+      double acc[RC];
+#pragma unroll
+      for (int r = 0; r < RC; ++r) acc[r] = 0.0;
+
+      for (int li = l0; li < l1; ++li) {
+        const int sb = p.nbr_lists[li];
+        const int lvl = p.levels[sb];
+        const double sext = p.root_extent * (1.0 / (double)(1 << lvl));
+        double code = 0.0;
+        for (int d = 0; d < p.dim; ++d) {
+          const double cs = p.centers[(int64_t)d * p.nboxes + sb];
+          code = code * (double)p.enc_base + ((ct[d] - cs) / sext * 4.0 + p.enc_shift + p.enc_bias);
+        }
+        const int64_t vid = (int64_t)(code + 0.5);
+        const int cid = (vid >= 0 && vid < p.n_codes) ? (int)p.case_indices[vid] : -1;
+        __syncthreads();  // previous source box done with LDS
+        if (cid < 0) {    // uniform over the workgroup
+          if (tid == 0 && p.err) atomicOr(p.err, 1);
+          continue;
+        }
+        int tlev = 0;
+        double scal = 1.0, dscale = 0.0;
+        if (p.scaling_mode == VL1_SCALING_CANONICAL) {
+          double num = 1.0, den = 1.0;
+          for (int k = 0; k < p.scaling_power; ++k) {
+            num *= sext;
+            den *= p.table_root_extent;
+          }
+          scal = num / den;
+          if (p.log_disp) dscale = -0.5 / M_PI * scal * log(sext / p.table_root_extent);
+        } else if (p.scaling_mode == VL1_SCALING_PER_LEVEL) {
+          const int rel = lvl - p.table_starting_level;
+          tlev = lvl < p.table_starting_level ? 0 : (rel < p.n_tables ? rel : p.n_tables - 1);
+        }
+        const int rank = p.ranks[cid];
+        const int off = p.offs[rank];
+        const int cnt = p.offs[rank + 1] - off;
+        const double* gT = p.table + (int64_t)tlev * p.n_entries + off;
+        const bool staged = cnt <= p.tbuf;
+        const int sbeg = p.src_starts[sb];
+        const int ns = min(p.src_counts[sb], n);
+        if (staged)
+          for (int i = tid; i < cnt; i += kNFBlock) sT[i] = gT[i];
+        for (int i = tid; i < ns * RC; i += kNFBlock) {
+          const int r = i / ns, sid = i % ns;
+          sF[sid * RC + r] = r < rc ? p.f[(int64_t)(r0 + r) * p.ldf + sbeg + sid] : 0.0;
+        }
+        for (int i = tid; i < ns; i += kNFBlock) {
+          sMode[i] = p.src_ids[sbeg + i];
+          sD[i] = p.log_disp ? dscale * p.nmlz[(int64_t)tlev * n + i] : 0.0;
+        }
+        __syncthreads();
+        if (tact) {
+          const IdxT* rm = static_cast<const IdxT*>(p.rowmap) + (int64_t)cid * n * n + tmode;
+          for (int sid = part; sid < ns; sid += P) {
+            const int s = sMode[sid];
+            if ((unsigned)s >= (unsigned)n) continue;  // malformed mode id: no contribution
+            const int rel = (int)rm[(int64_t)s * n];
+            const double tv = staged ? sT[rel] : gT[rel];
+            const double v = tv * scal + sD[sid];
+            const double* fs = sF + sid * RC;
+#pragma unroll
+            for (int r = 0; r < RC; ++r) acc[r] = fma(fs[r], v, acc[r]);
+          }
+        }
+      }
+
+      if (P > 1) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RC; ++r) red[(part * p.nt + lane_t) * RC + r] = acc[r];
+        __syncthreads();
+        if (part == 0) {
+          for (int j = 1; j < P; ++j)
+#pragma unroll
+            for (int r = 0; r < RC; ++r) acc[r] += red[(j * p.nt + lane_t) * RC + r];
+        }
+      }
+      if (part == 0 && tact) {
+        for (int r = 0; r < rc; ++r) p.y[(int64_t)(r0 + r) * p.ldy + tbeg + ti] = acc[r];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename IdxT, int RC>
+static hipError_t launch_nf(const NFParams& prm, int grid, size_t smem, hipStream_t s) {
+  auto fn = near_field_f64_kernel<IdxT, RC>;
+  hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)smem);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(kNFBlock), smem, s, prm);
+  return hipGetLastError();
+}
+
+}  // namespace vl1
+
+using namespace vl1;
+
+extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_args* a,
+                                  void* stream) {
+  clear_error();
+  if (!pl || !a) return fail(VL1_E_ARG, "vl1_near_field_f64: null argument");
+  if (pl->is_complex) return fail(VL1_E_UNSUPPORTED, "vl1_near_field_f64: complex payload");
+  if (a->n_tgt_boxes < 0 || a->n_rhs < 1 || a->aligned_nboxes < 1)
+    return fail(VL1_E_ARG, "vl1_near_field_f64: bad sizes");
+  if (a->scaling_mode < 0 || a->scaling_mode > 2)
+    return fail(VL1_E_ARG, "vl1_near_field_f64: bad scaling mode");
+  if (a->scaling_mode == VL1_SCALING_CANONICAL && (a->scaling_power < 0 || a->scaling_power > 3))
+    return fail(VL1_E_ARG, "vl1_near_field_f64: bad scaling power");
+  if (a->n_tgt_boxes == 0) return VL1_OK;
+  if (!a->result || !a->mode_coefs || !a->box_centers || !a->box_levels ||
+      !a->box_source_counts_nonchild || !a->box_source_starts ||
+      !a->box_target_counts_nonchild || !a->box_target_starts ||
+      !a->neighbor_source_boxes_starts || !a->neighbor_source_boxes_lists || !a->target_boxes ||
+      !a->source_mode_ids || !a->target_point_ids)
+    return fail(VL1_E_ARG, "vl1_near_field_f64: null device array");
+  if (a->n_rhs > 1 && (a->ld_coefs < 1 || a->ld_result < 1))
+    return fail(VL1_E_ARG, "vl1_near_field_f64: bad leading dimension");
+
+  NFParams prm{};
+  prm.centers = a->box_centers;
+  prm.nboxes = a->aligned_nboxes;
+  prm.levels = a->box_levels;
+  prm.src_counts = a->box_source_counts_nonchild;
+  prm.src_starts = a->box_source_starts;
+  prm.tgt_counts = a->box_target_counts_nonchild;
+  prm.tgt_starts = a->box_target_starts;
+  prm.nbr_starts = a->neighbor_source_boxes_starts;
+  prm.nbr_lists = a->neighbor_source_boxes_lists;
+  prm.target_boxes = a->target_boxes;
+  prm.src_ids = a->source_mode_ids;
+  prm.tgt_ids = a->target_point_ids;
+  prm.case_indices = pl->case_idx.as<int64_t>();
+  prm.n_codes = (int)pl->case_indices.size();
+  prm.ranks = pl->ranks.as<uint16_t>();
+  prm.offs = pl->offs.as<int32_t>();
+  prm.table = pl->table.as<double>();
+  prm.n_entries = pl->n_entries;
+  prm.n_tables = pl->n_tables;
+  prm.nmlz = pl->nmlz.as<double>();
+  prm.rowmap = pl->rowmap.ptr;
+  prm.dim = pl->dim;
+  prm.n = pl->n;
+  prm.scaling_mode = a->scaling_mode;
+  prm.scaling_power = a->scaling_power;
+  prm.log_disp = a->log_displacement ? 1 : 0;
+  prm.root_extent = a->root_extent;
+  prm.table_root_extent = a->table_root_extent;
+  prm.table_starting_level = a->table_starting_level;
+  prm.enc_shift = a->encoding_shift;
+  prm.enc_bias = a->case_encoding_bias;
+  prm.enc_base = a->encoding_base;
+  prm.n_rhs = a->n_rhs;
+  prm.ldf = a->ld_coefs;
+  prm.ldy = a->ld_result;
+  prm.f = a->mode_coefs;
+  prm.y = a->result;
+  prm.err = a->error_flags;
+  if (prm.enc_base != 13) return fail(VL1_E_ARG, "encoding_base must be 13");
+
+  int nt = 1;
+  while (nt < pl->n && nt < kNFBlock) nt *= 2;
+  prm.nt = nt;
+  const int P = kNFBlock / nt;
+  const int RC = a->n_rhs == 1 ? 1 : (a->n_rhs <= 4 ? 4 : 16);
+  const size_t lds_cap = 150 * 1024;
+  const size_t fixed = sizeof(double) * ((size_t)pl->n * RC + pl->n + (P > 1 ? kNFBlock * RC : 0)) +
+                       sizeof(int) * pl->n;
+  if (fixed > lds_cap) return fail(VL1_E_UNSUPPORTED, "vl1_near_field_f64: box too large for LDS");
+  size_t tbuf = std::min<size_t>((size_t)pl->max_orbit_count, (lds_cap - fixed) / sizeof(double));
+  prm.tbuf = (int)tbuf;
+  const size_t smem = fixed + sizeof(double) * tbuf;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  const bool u16 = pl->rowmap_u16 != 0;
+  const int grid = a->n_tgt_boxes;
+  if (RC == 1)
+    e = u16 ? launch_nf<uint16_t, 1>(prm, grid, smem, s) : launch_nf<int32_t, 1>(prm, grid, smem, s);
+  else if (RC == 4)
+    e = u16 ? launch_nf<uint16_t, 4>(prm, grid, smem, s) : launch_nf<int32_t, 4>(prm, grid, smem, s);
+  else
+    e = u16 ? launch_nf<uint16_t, 16>(prm, grid, smem, s)
+            : launch_nf<int32_t, 16>(prm, grid, smem, s);
+  if (e != hipSuccess)
+    return fail(VL1_E_HIP, std::string("near_field_f64 launch: ") + hipGetErrorString(e));
+  return VL1_OK;
+}

@@ -1,0 +1,34 @@
+"""Synthetic densities: seeded sums of Gaussians.
+
+``f(x) = sum_i a_i exp(-|x - c_i|^2 / w_i^2)`` with centres uniform in
+``[-0.5, 0.5]^d``, widths log-uniform in ``[0.02, 0.1]`` and amplitudes
+``N(0, 1)`` (numpy ``default_rng(seed)``), the recipe of the benchmark
+configurations (cf. ``volumential/gaussian.py:51-133``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def gaussian_mixture_params(dim, seed, n_terms=8, lo=0.02, hi=0.1, box=0.5):
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-box, box, size=(n_terms, dim))
+    widths = np.exp(rng.uniform(np.log(lo), np.log(hi), size=n_terms))
+    amps = rng.standard_normal(n_terms)
+    return centers, widths, amps
+
+
+def gaussian_mixture(points, seed, n_terms=8):
+    """Evaluate at ``points[dim, n]``."""
+    dim = points.shape[0]
+    c, w, a = gaussian_mixture_params(dim, seed, n_terms)
+    out = np.zeros(points.shape[1])
+    for i in range(n_terms):
+        r2 = np.sum((points - c[i][:, None]) ** 2, axis=0)
+        out += a[i] * np.exp(-r2 / w[i] ** 2)
+    return out
+
+
+def gaussian_mixture_batch(points, seeds, n_terms=8):
+    return np.stack([gaussian_mixture(points, s, n_terms) for s in seeds])

@@ -1,0 +1,66 @@
+"""Device driver of the Duffy table build (``vl1_duffy_build``).
+
+Host side of ``_batched_duffy_values_for_local_indices``
+(volumential/nearfield_potential_table.py:2328-2463): gathers per-entry
+geometry, uploads the node set once, launches one kernel over all entries.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _capi, quadrature
+
+
+def _dev(a: np.ndarray, dtype=None) -> torch.Tensor:
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda", non_blocking=False)
+
+
+def duffy_values(table, entry_ids, regular_order, radial_order, stream=None) -> np.ndarray:
+    if not torch.cuda.is_available():
+        raise RuntimeError("the Duffy table build runs on the GPU; no GPU is visible")
+    L = _capi.lib()
+    nodes = quadrature.duffy_node_set(table.dim, int(regular_order), int(radial_order))
+    tp, dv, axes = table.duffy_inputs(entry_ids)
+    xi = table.interp_nodes
+    bw = quadrature.barycentric_weights(xi)
+    n_entries = int(tp.shape[1])
+    cplx = bool(table.integral_knl.is_complex_valued)
+    d_tp, d_dv, d_ax = _dev(tp), _dev(dv), _dev(axes)
+    d_u, d_s, d_j = _dev(nodes.node_u), _dev(nodes.node_sign), _dev(nodes.node_jac)
+    d_xi, d_bw = _dev(xi), _dev(bw)
+    out = torch.empty(n_entries * (2 if cplx else 1), dtype=torch.float64, device="cuda")
+    import math
+
+    nodes_per_sign = nodes.n_nodes // (2**table.dim)
+    assert nodes_per_sign * 2**table.dim == nodes.n_nodes
+    assert nodes_per_sign % math.factorial(table.dim) == 0
+    args = _capi.DuffyArgs(
+        dim=table.dim,
+        quad_order=table.quad_order,
+        kernel_kind=int(table.integral_knl.kind),
+        helmholtz_k=float(getattr(table.integral_knl, "k", 0.0)),
+        n_entries=n_entries,
+        n_nodes=nodes.n_nodes,
+        nodes_per_sign=nodes_per_sign,
+        source_box_extent=table.source_box_extent,
+        target_points=d_tp.data_ptr(),
+        decomposition=d_dv.data_ptr(),
+        mode_axes=d_ax.data_ptr(),
+        node_u=d_u.data_ptr(),
+        node_sign=d_s.data_ptr(),
+        node_jac=d_j.data_ptr(),
+        interp_nodes=d_xi.data_ptr(),
+        bary_w=d_bw.data_ptr(),
+        result=out.data_ptr(),
+    )
+    _capi.check(L.vl1_duffy_build(args, _capi.stream_handle(stream)), "vl1_duffy_build")
+    torch.cuda.synchronize()
+    res = out.cpu().numpy()
+    if cplx:
+        res = res[0::2] + 1j * res[1::2]
+    return res

@@ -1,0 +1,295 @@
+"""``NearFieldFromCSR``: the list-1 evaluator operator.
+
+Same constructor and call convention as volumential's
+``NearFieldFromCSR`` (volumential/list1.py:91-215, 223-437, 1260-1487):
+``NearFieldFromCSR(integral_kernel, table_data_shapes, potential_kind=1,
+**list1_extra_kwargs)`` and ``__call__(queue, result=..., box_centers=...,
+...)`` returning ``(result, event)``.  Arrays may be torch tensors (device or
+host) or numpy arrays; the evaluation always runs on the GPU through
+``vl1_near_field_f64``.  User scaling/displacement *code strings* are not
+supported (this build has no runtime code generation); the inferred,
+fixed-single-table and per-level-table policies are.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _capi, gallery, orbit
+from .kernels import inferred_scaling, supports_inferred_scaling
+from .payload import DevicePayload, PayloadCache, fingerprint
+
+CASE_ENCODING_BIAS_DEFAULT = 1.0e-10
+
+
+@dataclass(frozen=True)
+class KernelScalingPolicy:
+    """``list1.py:49-61``."""
+
+    kernel_name: str
+    mode: str
+    infer_kernel_scaling: bool
+    single_table_scaling_supported: bool
+    reference_table_level: int | None
+    scaling_power: int
+    log_displacement: bool
+    notes: str
+
+
+_PAYLOADS = PayloadCache(16)
+
+
+def _to_device(a, dtype):
+    if isinstance(a, torch.Tensor):
+        t = a
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(a)))
+    t = t.to(dtype=dtype)
+    if t.device.type != "cuda":
+        t = t.to("cuda")
+    return t.contiguous()
+
+
+def _to_host(a):
+    if isinstance(a, torch.Tensor):
+        return a.detach().cpu().numpy()
+    if hasattr(a, "get"):
+        return a.get()
+    return np.asarray(a)
+
+
+class NearFieldFromCSR:
+    default_name = "near_field_from_csr"
+
+    def __init__(self, integral_kernel, table_data_shapes, potential_kind=1, **kwargs):
+        if potential_kind != 1:
+            raise NotImplementedError(
+                "potential_kind=2 (exterior normalisers) is not part of this build"
+            )
+        self.integral_kernel = integral_kernel
+        self.potential_kind = potential_kind
+        self.n_tables = int(table_data_shapes["n_tables"])
+        self.n_q_points = int(table_data_shapes["n_q_points"])
+        self.n_table_entries = int(table_data_shapes["n_table_entries"])
+        self.n_cases = int(table_data_shapes.get("n_cases", 0))
+        self.reconstruction_kind = table_data_shapes.get(
+            "reconstruction_kind", "scalar-arithmetic-orbit"
+        )
+        if self.reconstruction_kind != "scalar-arithmetic-orbit":
+            raise NotImplementedError(
+                f"reconstruction kind {self.reconstruction_kind!r} is not supported; "
+                "scalar kernels use 'scalar-arithmetic-orbit'"
+            )
+        self.dim = integral_kernel.dim
+        self.quad_order = int(
+            table_data_shapes.get("quad_order", round(self.n_q_points ** (1.0 / self.dim)))
+        )
+        self.extra_kwargs = dict(kwargs)
+        self.extra_kwargs.setdefault("case_encoding_bias", CASE_ENCODING_BIAS_DEFAULT)
+        self.extra_kwargs["case_encoding_bias"] = float(self.extra_kwargs["case_encoding_bias"])
+        if "kernel_scaling_code" in kwargs or "kernel_displacement_code" in kwargs:
+            raise NotImplementedError(
+                "custom kernel_scaling_code/kernel_displacement_code strings are not "
+                "supported; use per-level tables or inferred scaling"
+            )
+        if "infer_kernel_scaling" not in self.extra_kwargs:
+            self.extra_kwargs["infer_kernel_scaling"] = (
+                self.n_tables == 1 and supports_inferred_scaling(integral_kernel)
+            )
+        if self.extra_kwargs["infer_kernel_scaling"] and not supports_inferred_scaling(
+            integral_kernel
+        ):
+            raise RuntimeError(
+                "infer_kernel_scaling is unsupported for kernel "
+                f"{integral_kernel!r}. For Helmholtz, K_k(h r) = h^{{-1}} K_{{k h}}(r): "
+                "exact table scaling requires changing the effective wave number with box "
+                "size, which a single fixed-k table cannot represent. Use multi-level "
+                "tables."
+            )
+        bs = int(self.extra_kwargs.get("list1_target_batch_size", 1))
+        if bs < 1:
+            raise ValueError("list1_target_batch_size must be positive")
+        self._level_check_key = None
+
+    # -------------------------------------------------------------- policy
+    def get_kernel_scaling_policy(self) -> KernelScalingPolicy:
+        """``list1.py:361-437``."""
+        kname = repr(self.integral_kernel)
+        if self.extra_kwargs.get("infer_kernel_scaling", False):
+            p, logd = inferred_scaling(self.integral_kernel)
+            return KernelScalingPolicy(
+                kname, "canonical_single_table", True, True, 0, p, logd,
+                "Uses table level 0 for all source-box levels with inferred exact scaling.",
+            )
+        if self.n_tables == 1:
+            return KernelScalingPolicy(
+                kname, "fixed_single_table", False, False, 0, 0, False,
+                "Uses the only cached table level without kernel scaling; runtime checks "
+                "reject mixed source levels or a table level mismatch.",
+            )
+        return KernelScalingPolicy(
+            kname, "per_level_tables", False, False, None, 0, False,
+            "Selects one cached table level per source-box level without kernel scaling.",
+        )
+
+    def _mode_code(self, policy):
+        return {
+            "canonical_single_table": _capi.SCALING_CANONICAL,
+            "fixed_single_table": _capi.SCALING_FIXED,
+            "per_level_tables": _capi.SCALING_PER_LEVEL,
+        }[policy.mode]
+
+    # ------------------------------------------------------------ payload
+    def _payload(self, kw):
+        table_data = _to_host(kw["table_data_combined"])
+        key = (
+            repr(self.integral_kernel),
+            self.quad_order,
+            fingerprint(table_data),
+            fingerprint(_to_host(kw["mode_nmlz_combined"])),
+        )
+
+        def make():
+            meta = orbit.ArithmeticCaseMetadata(
+                dim=self.dim,
+                quad_order=self.quad_order,
+                canonical_case_ids=np.zeros(
+                    int(np.asarray(_to_host(kw["arithmetic_case_value_offsets"])).size) - 1,
+                    dtype=np.int32,
+                ),
+                case_orbit_ranks=np.asarray(_to_host(kw["arithmetic_case_orbit_ranks"]), np.uint16),
+                case_axis_perm=np.asarray(_to_host(kw["arithmetic_case_axis_perm"]), np.uint8),
+                case_axis_sign=np.asarray(_to_host(kw["arithmetic_case_axis_sign"]), np.int8),
+                case_axis_group=np.asarray(_to_host(kw["arithmetic_case_axis_group"]), np.uint8),
+                case_value_offsets=np.asarray(
+                    _to_host(kw["arithmetic_case_value_offsets"]), np.int32
+                ),
+                axis_sign_power=np.zeros(self.dim, np.uint8),
+            )
+            if np.any(np.asarray(_to_host(kw.get("arithmetic_axis_sign_power", 0))) != 0):
+                raise NotImplementedError("signed (derivative) orbit payloads are not supported")
+            return DevicePayload(
+                self.dim,
+                self.quad_order,
+                table_data.reshape(self.n_tables, -1),
+                _to_host(kw["mode_nmlz_combined"]).reshape(self.n_tables, -1),
+                _to_host(kw["case_indices"]),
+                meta,
+            )
+
+        return _PAYLOADS.get(key, make)
+
+    # ---------------------------------------------------------------- call
+    def __call__(self, queue=None, **kw):
+        policy = self.get_kernel_scaling_policy()
+        result = kw.pop("result")
+        mode_coefs = kw.pop("mode_coefs")
+        payload = self._payload(kw)
+        if payload.is_complex:
+            raise NotImplementedError("complex list-1 evaluation goes through eval_direct plans")
+        if int(kw.get("encoding_base", gallery.ENCODING_BASE)) != gallery.ENCODING_BASE:
+            raise ValueError("encoding_base must be 13")
+
+        target_boxes = kw["target_boxes"]
+        lists = kw["neighbor_source_boxes_lists"]
+        box_levels = kw["box_levels"]
+        tsl = int(kw["table_starting_level"])
+        if policy.mode == "fixed_single_table":
+            key = (id(lists), id(box_levels), tsl)
+            if key != self._level_check_key:
+                src = _to_host(lists).astype(np.int64)
+                if src.size:
+                    lv = _to_host(box_levels)[src]
+                    lo, hi = int(lv.min()), int(lv.max())
+                    if lo != hi:
+                        raise RuntimeError(
+                            "Single near-field table without scaling cannot be used with "
+                            f"source boxes on multiple levels ({lo}..{hi}); build per-level "
+                            "tables or provide explicit scaling rules."
+                        )
+                    if lo != tsl:
+                        raise RuntimeError(
+                            "Single near-field table level mismatch: source level "
+                            f"{lo} but table_starting_level is {tsl}. Build the table at "
+                            "the active source level or provide scaling rules."
+                        )
+                self._level_check_key = key
+
+        coefs_2d = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
+            np.asarray(mode_coefs)
+        )
+        host_result = not (isinstance(result, torch.Tensor) and result.device.type == "cuda")
+        d_res = _to_device(result, torch.float64)
+        d_coef = _to_device(coefs_2d, torch.float64)
+        n_rhs = 1 if d_coef.dim() == 1 else int(d_coef.shape[0])
+        centers = _to_device(kw["box_centers"], torch.float64)
+        arrs = {
+            k: _to_device(kw[k], torch.int32)
+            for k in (
+                "box_source_counts_nonchild",
+                "box_source_starts",
+                "box_target_counts_nonchild",
+                "box_target_starts",
+                "neighbor_source_boxes_starts",
+                "neighbor_source_boxes_lists",
+                "target_boxes",
+                "source_mode_ids",
+                "target_point_ids",
+            )
+        }
+        d_levels = _to_device(box_levels, torch.uint8)
+        err = torch.zeros(1, dtype=torch.int32, device="cuda")
+        n_tgt = int(target_boxes.shape[0] if hasattr(target_boxes, "shape") else len(target_boxes))
+        args = _capi.NearFieldArgs(
+            scaling_mode=self._mode_code(policy),
+            scaling_power=int(policy.scaling_power),
+            log_displacement=int(policy.log_displacement),
+            root_extent=float(kw["root_extent"]),
+            table_root_extent=float(kw["table_root_extent"]),
+            table_starting_level=tsl,
+            encoding_shift=float(kw.get("encoding_shift", gallery.ENCODING_SHIFT)),
+            encoding_base=gallery.ENCODING_BASE,
+            case_encoding_bias=self.extra_kwargs["case_encoding_bias"],
+            n_tgt_boxes=n_tgt,
+            aligned_nboxes=int(centers.shape[1]),
+            n_rhs=n_rhs,
+            ld_coefs=int(d_coef.shape[-1]),
+            ld_result=int(d_res.shape[-1]),
+            result=d_res.data_ptr(),
+            mode_coefs=d_coef.data_ptr(),
+            box_centers=centers.data_ptr(),
+            box_levels=d_levels.data_ptr(),
+            box_source_counts_nonchild=arrs["box_source_counts_nonchild"].data_ptr(),
+            box_source_starts=arrs["box_source_starts"].data_ptr(),
+            box_target_counts_nonchild=arrs["box_target_counts_nonchild"].data_ptr(),
+            box_target_starts=arrs["box_target_starts"].data_ptr(),
+            neighbor_source_boxes_starts=arrs["neighbor_source_boxes_starts"].data_ptr(),
+            neighbor_source_boxes_lists=arrs["neighbor_source_boxes_lists"].data_ptr(),
+            target_boxes=arrs["target_boxes"].data_ptr(),
+            source_mode_ids=arrs["source_mode_ids"].data_ptr(),
+            target_point_ids=arrs["target_point_ids"].data_ptr(),
+            error_flags=err.data_ptr(),
+        )
+        L = _capi.lib()
+        _capi.check(
+            L.vl1_near_field_f64(payload.handle, args, _capi.stream_handle()),
+            "vl1_near_field_f64",
+        )
+        torch.cuda.current_stream().synchronize()
+        if int(err.item()) != 0:
+            raise RuntimeError(
+                "list-1 case encoding produced an invalid case id (box geometry is not a "
+                "2:1-balanced list-1 neighbourhood)"
+            )
+        if host_result:
+            out = d_res.cpu().numpy()
+            if isinstance(result, np.ndarray):
+                result[...] = out.reshape(result.shape)
+                return result, None
+            return out, None
+        if d_res.data_ptr() != result.data_ptr():
+            result.copy_(d_res)
+        return result, None

@@ -1,0 +1,229 @@
+"""Drop-in list-1 part of the FPND expansion wrangler.
+
+``NearFieldWrangler.eval_direct(target_boxes, neighbor_source_boxes_starts,
+neighbor_source_boxes_lists, mode_coefs)`` has the signature and return
+shape of ``ExpansionWranglerInterface.eval_direct``
+(volumential/expansion_wrangler_interface.py:115-128) as implemented by
+``FPNDSumpyExpansionWrangler.eval_direct`` (expansion_wrangler_fpnd.py:
+3222-3489): it returns ``(pot, timing_future)`` with ``pot`` an object array
+holding one potential per output kernel, each overwritten on the targets of
+the listed boxes.  Table handling (single table / per-level list / dict by
+kernel repr, ``table_starting_level``, coverage and extent checks) follows
+2368-2472 and 3254-3273; layout validation follows 408-463.
+
+``eval_direct_batched`` evaluates many densities ``mode_coefs[R, n_points]``
+in one call (the per-RHS loop of ``volume_fmm.py:983-1031`` collapsed), and
+``eval_direct_plan`` runs the batched plan forms (dense FP64 or Ozaki split).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _capi, gallery
+from .list1 import NearFieldFromCSR
+from .payload import prepare_table_data_and_entry_map
+
+
+class TimingFuture:
+    def __init__(self, seconds=None):
+        self.seconds = seconds
+
+    def result(self):
+        return {"wall_elapsed": self.seconds, "process_elapsed": None}
+
+    def done(self):
+        return True
+
+
+def _validate_layout(tree, target_boxes, source_boxes, n_q_points):
+    """``expansion_wrangler_fpnd.py:427-463``."""
+    counts = np.asarray(tree.box_target_counts_nonchild)
+    tc = counts[np.asarray(target_boxes, dtype=np.int64)]
+    sc = counts[np.asarray(source_boxes, dtype=np.int64)]
+    if np.all(tc == n_q_points) and np.all(sc == n_q_points):
+        return
+    bad_t = np.unique(tc[tc != n_q_points])
+    bad_s = np.unique(sc[sc != n_q_points])
+    raise ValueError(
+        "table-based near-field evaluation requires exactly "
+        f"{n_q_points} quadrature points per active source/target box; "
+        f"found target counts {bad_t.tolist()} and source counts {bad_s.tolist()}."
+    )
+
+
+def _validate_local_ids(tree, boxes, n):
+    """Each listed box's node ids must be a permutation of 0..n-1."""
+    starts = np.asarray(tree.box_target_starts, dtype=np.int64)[boxes]
+    ids = np.asarray(tree.local_ids)
+    blk = ids[starts[:, None] + np.arange(n)[None, :]]
+    if not np.array_equal(np.sort(blk, axis=1), np.broadcast_to(np.arange(n), blk.shape)):
+        raise ValueError("box-local node ids must be a permutation of 0..q^d-1 in every box")
+
+
+class NearFieldWrangler:
+    def __init__(
+        self,
+        tree,
+        near_field_table,
+        target_kernels=None,
+        dtype=np.float64,
+        list1_extra_kwargs=None,
+        helmholtz_split=False,
+    ):
+        if not getattr(tree, "sources_are_targets", False):
+            raise ValueError(
+                "table-based near-field evaluation requires sources and targets "
+                "to coincide (tree.sources_are_targets=True)"
+            )
+        self.tree = tree
+        self.dtype = np.dtype(dtype)
+        if helmholtz_split:
+            raise NotImplementedError(
+                "the Helmholtz split correction is not part of this build; "
+                "pass helmholtz_split=False with per-level Helmholtz tables"
+            )
+        if isinstance(near_field_table, dict):
+            tables = {k: self._as_list(v) for k, v in near_field_table.items()}
+        else:
+            tables = None
+            lst = self._as_list(near_field_table)
+        if target_kernels is None:
+            if tables is not None:
+                raise ValueError("target_kernels is required with a dict of tables")
+            target_kernels = [lst[0].integral_knl]
+        self.target_kernels = list(target_kernels)
+        if tables is None:
+            tables = {repr(k): lst for k in self.target_kernels}
+        self.near_field_table = tables
+        self.list1_extra_kwargs = dict(list1_extra_kwargs or {})
+        for kname, lst in tables.items():
+            t0 = lst[0]
+            if t0.n_q_points != tree.n_q_points:
+                raise ValueError("table order does not match the tree's nodes per box")
+            lvl0 = self.table_starting_level(lst)
+            for lev, t in enumerate(lst):
+                if not np.isclose(t.source_box_extent * 2**lev, t0.source_box_extent):
+                    raise RuntimeError(
+                        f"table level {lev} extent {t.source_box_extent} is inconsistent with "
+                        f"level-0 extent {t0.source_box_extent}"
+                    )
+            if len(lst) > 1 and tree.nlevels > len(lst) + lvl0:
+                raise RuntimeError(
+                    f"Insufficient list of tables: the tree has {tree.nlevels} levels but "
+                    f"only {len(lst)} tables starting at level {lvl0} were given."
+                )
+        self._evaluators = {}
+        self._plans = {}
+
+    @staticmethod
+    def _as_list(t):
+        return list(t) if isinstance(t, (list, tuple)) else [t]
+
+    def table_starting_level(self, tables):
+        return int(round(np.log2(self.tree.root_extent / tables[0].source_box_extent)))
+
+    def output_zeros(self, n_rhs=None):
+        shape = (self.tree.n_points,) if n_rhs is None else (n_rhs, self.tree.n_points)
+        pot = np.empty(len(self.target_kernels), dtype=object)
+        for i in range(len(self.target_kernels)):
+            pot[i] = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        return pot
+
+    # -------------------------------------------------------------- eval
+    def _eval_single(self, out_pot, out_kernel, target_boxes, starts, lists, mode_coefs):
+        kname = repr(out_kernel)
+        tables = self.near_field_table[kname]
+        t0 = tables[0]
+        tsl = self.table_starting_level(tables)
+        if len(tables) > 1:
+            min_lev = int(np.min(np.asarray(self.tree.box_levels)[np.asarray(target_boxes)]))
+            largest = self.tree.root_extent * 0.5**min_lev
+            if not t0.source_box_extent >= largest - 1e-15:
+                raise RuntimeError(
+                    "Insufficient list of tables: the coarsest level mesh cells at level "
+                    f"{min_lev} are not covered."
+                )
+        data, nmlz, _, _, _, info = prepare_table_data_and_entry_map(tables)
+        shapes = {
+            "n_tables": len(tables),
+            "n_q_points": t0.n_q_points,
+            "n_table_entries": data.shape[1],
+            "n_cases": t0.n_cases,
+            "quad_order": t0.quad_order,
+            "reconstruction_kind": info["kind"],
+            "n_arithmetic_case_orbits": info["n_case_orbits"],
+        }
+        key = (kname, len(tables))
+        if key not in self._evaluators:
+            self._evaluators[key] = NearFieldFromCSR(out_kernel, shapes, **self.list1_extra_kwargs)
+        nf = self._evaluators[key]
+        tb = np.asarray(_host(target_boxes), dtype=np.int64)
+        ls = np.asarray(_host(lists), dtype=np.int64)
+        _validate_layout(self.tree, tb, ls, t0.n_q_points)
+        _validate_local_ids(self.tree, np.unique(np.concatenate([tb, ls])), t0.n_q_points)
+        tree = self.tree
+        res, _ = nf(
+            None,
+            result=out_pot,
+            box_centers=tree.box_centers,
+            box_levels=tree.box_levels,
+            box_source_counts_nonchild=tree.box_target_counts_nonchild,
+            box_source_starts=tree.box_target_starts,
+            box_target_counts_nonchild=tree.box_target_counts_nonchild,
+            box_target_starts=tree.box_target_starts,
+            case_indices=t0.case_indices,
+            encoding_base=gallery.ENCODING_BASE,
+            encoding_shift=gallery.ENCODING_SHIFT,
+            mode_nmlz_combined=nmlz,
+            neighbor_source_boxes_starts=starts,
+            root_extent=tree.root_extent,
+            neighbor_source_boxes_lists=lists,
+            mode_coefs=mode_coefs,
+            source_mode_ids=tree.local_ids,
+            table_data_combined=data,
+            target_boxes=target_boxes,
+            target_point_ids=tree.local_ids,
+            table_root_extent=t0.source_box_extent,
+            table_starting_level=tsl,
+            arithmetic_case_orbit_ranks=info["case_orbit_ranks"],
+            arithmetic_case_axis_perm=info["case_axis_perm"],
+            arithmetic_case_axis_sign=info["case_axis_sign"],
+            arithmetic_case_axis_group=info["case_axis_group"],
+            arithmetic_case_value_offsets=info["case_value_offsets"],
+            arithmetic_axis_sign_power=info["axis_sign_power"],
+        )
+        return res
+
+    def eval_direct(self, target_boxes, neighbor_source_boxes_starts, neighbor_source_boxes_lists,
+                    mode_coefs):
+        pot = self.output_zeros()
+        for i, knl in enumerate(self.target_kernels):
+            pot[i] = self._eval_single(
+                pot[i], knl, target_boxes, neighbor_source_boxes_starts,
+                neighbor_source_boxes_lists, mode_coefs,
+            )
+        return pot, TimingFuture()
+
+    def eval_direct_batched(self, target_boxes, neighbor_source_boxes_starts,
+                            neighbor_source_boxes_lists, mode_coefs):
+        """``mode_coefs[R, n_points]`` -> ``pot[i][R, n_points]``."""
+        mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
+            np.asarray(mode_coefs)
+        )
+        if mc.dim() != 2:
+            raise ValueError("mode_coefs must have shape (n_rhs, n_points)")
+        pot = self.output_zeros(int(mc.shape[0]))
+        for i, knl in enumerate(self.target_kernels):
+            pot[i] = self._eval_single(
+                pot[i], knl, target_boxes, neighbor_source_boxes_starts,
+                neighbor_source_boxes_lists, mc,
+            )
+        return pot, TimingFuture()
+
+
+def _host(a):
+    if isinstance(a, torch.Tensor):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)

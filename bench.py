@@ -1,0 +1,295 @@
+"""List-1 near-field evaluation benchmark (one MI355X).
+
+Workload (default): 3D Laplace, domain [-1,1]^3, uniform octree level 5
+(32768 leaves), q = 6 (216 nodes per box, 7,077,888 target points),
+R = 64 densities, each a seeded sum of 8 Gaussians (seeds 1000..1063);
+canonical FP64 table (140,496 orbit-reduced values) built on the GPU.
+A "step" is one evaluation of all 64 densities on all targets.
+
+Forms:
+  f64    plain FP64 kernel (vl1_near_field_f64): CSR target boxes, canonical
+         orbit block staged in LDS, lanes over target nodes.
+  split  (when built) Ozaki-split BF16 MFMA batched plan.
+
+Prints one JSON line: metric/value (target points x RHS per second, 1e9),
+timing, dtype, roofline of the dominant kernel (HIP events on the stream the
+kernel runs on) and the CPU baseline (C/OpenMP oracle on a bounded sample).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--level", type=int, default=5)
+    ap.add_argument("--q", type=int, default=6)
+    ap.add_argument("--rhs", type=int, default=64)
+    ap.add_argument("--form", default="auto", choices=["auto", "f64", "split"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gaussian_densities_gpu(points_dev, seeds):
+    """Seeded 8-Gaussian sums evaluated on the device (same recipe as
+    volumential_amd.densities.gaussian_mixture)."""
+    import torch
+
+    from volumential_amd.densities import gaussian_mixture_params
+
+    dim = points_dev.shape[0]
+    out = torch.empty((len(seeds), points_dev.shape[1]), dtype=torch.float64, device="cuda")
+    for i, s in enumerate(seeds):
+        c, w, a = gaussian_mixture_params(dim, s)
+        acc = torch.zeros(points_dev.shape[1], dtype=torch.float64, device="cuda")
+        for k in range(c.shape[0]):
+            r2 = torch.zeros_like(acc)
+            for d in range(dim):
+                r2 += (points_dev[d] - float(c[k, d])) ** 2
+            acc += float(a[k]) * torch.exp(-r2 / float(w[k]) ** 2)
+        out[i] = acc
+    return out
+
+
+class F64Runner:
+    """Direct C-ABI driver of vl1_near_field_f64 with device-resident inputs."""
+
+    def __init__(self, tree, table, f_dev, y_dev):
+        import torch
+
+        from volumential_amd import _capi, gallery
+        from volumential_amd.payload import DevicePayload, prepare_table_data_and_entry_map
+
+        data, nmlz, _, _, _, info = prepare_table_data_and_entry_map([table])
+        meta = table.arithmetic_metadata()
+        self.payload = DevicePayload(tree.dim, tree.quad_order, data, nmlz, table.case_indices, meta)
+
+        def dev(a, dt):
+            return torch.as_tensor(np.ascontiguousarray(a)).to(dt).cuda().contiguous()
+
+        self.keep = dict(
+            centers=dev(tree.box_centers, torch.float64),
+            levels=dev(tree.box_levels, torch.uint8),
+            counts=dev(tree.box_target_counts_nonchild, torch.int32),
+            starts=dev(tree.box_target_starts, torch.int32),
+            nstarts=dev(tree.neighbor_source_boxes_starts, torch.int32),
+            nlists=dev(tree.neighbor_source_boxes_lists, torch.int32),
+            tboxes=dev(tree.target_boxes, torch.int32),
+            ids=dev(tree.local_ids, torch.int32),
+        )
+        k = self.keep
+        self.f, self.y = f_dev, y_dev
+        self.args = _capi.NearFieldArgs(
+            scaling_mode=_capi.SCALING_CANONICAL,
+            scaling_power=2,
+            log_displacement=int(tree.dim == 2),
+            root_extent=tree.root_extent,
+            table_root_extent=table.source_box_extent,
+            table_starting_level=0,
+            encoding_shift=float(gallery.ENCODING_SHIFT),
+            encoding_base=gallery.ENCODING_BASE,
+            case_encoding_bias=1e-10,
+            n_tgt_boxes=int(tree.target_boxes.size),
+            aligned_nboxes=int(tree.nboxes),
+            n_rhs=int(f_dev.shape[0]),
+            ld_coefs=int(f_dev.shape[1]),
+            ld_result=int(y_dev.shape[1]),
+            result=y_dev.data_ptr(),
+            mode_coefs=f_dev.data_ptr(),
+            box_centers=k["centers"].data_ptr(),
+            box_levels=k["levels"].data_ptr(),
+            box_source_counts_nonchild=k["counts"].data_ptr(),
+            box_source_starts=k["starts"].data_ptr(),
+            box_target_counts_nonchild=k["counts"].data_ptr(),
+            box_target_starts=k["starts"].data_ptr(),
+            neighbor_source_boxes_starts=k["nstarts"].data_ptr(),
+            neighbor_source_boxes_lists=k["nlists"].data_ptr(),
+            target_boxes=k["tboxes"].data_ptr(),
+            source_mode_ids=k["ids"].data_ptr(),
+            target_point_ids=k["ids"].data_ptr(),
+            error_flags=0,
+        )
+        self.L = _capi.lib()
+        self.capi = _capi
+
+    def __call__(self, stream_handle):
+        self.capi.check(self.L.vl1_near_field_f64(self.payload.handle, self.args, stream_handle))
+
+
+def cpu_baseline(tree, table, f_host_rows, seconds):
+    """C/OpenMP oracle (reference semantics, one evaluation per RHS) on a
+    bounded sample of interior target boxes; returns points x RHS / s."""
+    import copy
+
+    from oracle import cpu as ocpu
+    from oracle import orbit_ref
+
+    meta = orbit_ref.case_metadata(tree.dim, tree.quad_order)
+    data = table.compact_values()[None]
+    nm = table.mode_normalizers[None]
+    counts = np.diff(tree.neighbor_source_boxes_starts)
+    full = np.flatnonzero(counts == counts.max())
+    cores = min(16, os.cpu_count() or 1)
+
+    def run(nb):
+        sub = copy.copy(tree)
+        sel = full[:: max(1, full.size // nb)][:nb]
+        starts = np.zeros(sel.size + 1, np.int32)
+        lists = []
+        for i, b in enumerate(sel):
+            seg = tree.neighbor_source_boxes_lists[
+                tree.neighbor_source_boxes_starts[b] : tree.neighbor_source_boxes_starts[b + 1]
+            ]
+            lists.append(seg)
+            starts[i + 1] = starts[i] + seg.size
+        sub.target_boxes = sel.astype(np.int32)
+        sub.neighbor_source_boxes_starts = starts
+        sub.neighbor_source_boxes_lists = np.concatenate(lists).astype(np.int32)
+        t0 = time.perf_counter()
+        ocpu.near_field(sub, data, nm, meta, f_host_rows, mode="canonical", scaling_power=2,
+                        log_displacement=(tree.dim == 2), table_root_extent=table.source_box_extent,
+                        nthreads=cores)
+        return time.perf_counter() - t0, sel.size
+
+    ocpu.lib()
+    t, nb = run(max(cores, 8))
+    per_box = t / nb
+    nb2 = int(min(full.size, max(cores, seconds / max(per_box, 1e-9))))
+    t, nb = run(nb2)
+    rhs = f_host_rows.shape[0]
+    value = nb * tree.n_q_points * rhs / t / 1e9
+    return {
+        "value": value,
+        "unit": "Gpts*RHS/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{nb} interior target boxes x {rhs} RHS of the same tree/table "
+                  f"({nb * tree.n_q_points * rhs} target-point evaluations, {t:.1f} s)",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+
+    from volumential_amd import _capi, tree as vtree
+    from volumential_amd.kernels import LaplaceKernel
+    from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(2)
+    _capi.lib()
+    dim, q, level, R = 3, args.q, args.level, args.rhs
+    log(f"tree: {dim}D q={q} uniform level {level}")
+    tr = vtree.uniform_tree(dim, q, level)
+    n_pts = tr.n_points
+    pairs = tr.n_list1_pairs
+    log(f"points {n_pts}, boxes {tr.nboxes}, list-1 pairs {pairs}")
+    t0 = time.perf_counter()
+    table = NearFieldInteractionTable(q, dim, LaplaceKernel(dim), source_box_extent=2.0)
+    table.build_table(DuffyBuildConfig(regular_quad_order=10, radial_quad_order=31))
+    log(f"table built on GPU in {time.perf_counter() - t0:.2f} s ({table.reduced_entry_ids.size} values)")
+
+    pts = torch.as_tensor(tr.points).cuda()
+    seeds = list(range(1000, 1000 + R))
+    f = gaussian_densities_gpu(pts, seeds).contiguous()
+    y = torch.zeros_like(f)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    sh = int(stream.cuda_stream)
+
+    runner = F64Runner(tr, table, f, y)
+    form = "f64"
+    for _ in range(args.warmup):
+        runner(sh)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        runner(sh)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    value = n_pts * R / (ms * 1e-3) / 1e9
+    n = tr.n_q_points
+    flops = 2.0 * R * pairs * n * n
+    bytes_alg = 2.0 * 8 * n_pts * R + table.reduced_entry_ids.size * 8 + 4 * (pairs + tr.nboxes)
+    achieved_tf = flops / (ms * 1e-3) / 1e12
+    hbm_gbs = bytes_alg / (ms * 1e-3) / 1e9
+    log(f"f64: {ms:.2f} ms/step, {value:.3f} Gpts*RHS/s, {achieved_tf:.2f} TFLOP/s, {hbm_gbs:.1f} GB/s")
+
+    res = {
+        "metric": "list-1 eval Gpts*RHS/s at <=1e-10 rel err; split-MFMA vs FP64 GPU kernel",
+        "value": value,
+        "unit": "Gpts*RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 8-Gaussian densities, GPU-built Laplace table)",
+        "config": {
+            "workload": f"3D Laplace uniform level {level}, q={q}, {R} RHS",
+            "boxes": int(tr.nboxes),
+            "target_points": int(n_pts),
+            "rhs": R,
+            "list1_pairs": int(pairs),
+            "form": form,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved_tf,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tf / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "kernel": "near_field_f64_kernel",
+            "kernel_ms": ms,
+            "flops_per_launch": flops,
+            "hbm_gbs_alg": hbm_gbs,
+        },
+        "forms": {"f64": {"ms": ms, "value": value, "tflops": achieved_tf}},
+    }
+    if not args.no_cpu_baseline:
+        rows = min(R, 1)
+        f_host = f[:rows].cpu().numpy()
+        log("cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline(tr, table, f_host, args.cpu_seconds)
+        log(f"cpu: {res['cpu_baseline']}")
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -26,19 +26,19 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("q,expected", [(1, 1.0), (2, 0.25)])
 def test_constant_order_1_2(cuda, q, expected):
     t = NearFieldInteractionTable(q, 2, ConstantKernel(2))
-    t.build_table(DuffyBuildConfig(regular_quad_order=8, radial_quad_order=21))
+    t.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=61))
     _, vals = t.get_reduced_table_data()
-    assert np.allclose(vals, expected, rtol=0, atol=1e-13)
+    assert np.allclose(vals, expected, rtol=0, atol=1e-12)
 
 
 def test_constant_3d_gl_weights(cuda):
     t = NearFieldInteractionTable(3, 3, ConstantKernel(3))
-    t.build_table(DuffyBuildConfig(regular_quad_order=6, radial_quad_order=21))
+    t.build_table(DuffyBuildConfig(regular_quad_order=10, radial_quad_order=61))
     ids, vals = t.get_reduced_table_data()
     w = np.polynomial.legendre.leggauss(3)[1] * 0.5
     s = (ids % t.n_pairs) // t.n_q_points
     expect = w[s // 9] * w[(s // 3) % 3] * w[s % 3]
-    assert np.allclose(vals, expect, rtol=0, atol=1e-13)
+    assert np.allclose(vals, expect, rtol=0, atol=1e-12)
 
 
 @pytest.mark.parametrize("dim,q,extent", [(2, 4, 2.0), (3, 2, 1.0), (3, 3, 2.0)])
@@ -58,14 +58,16 @@ def test_laplace_entries_match_oracle(cuda, dim, q, extent):
 def test_laplace3d_mode_sums_are_box_potentials(cuda):
     q = 3
     t = NearFieldInteractionTable(q, 3, LaplaceKernel(3), source_box_extent=1.0)
-    t.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=61))
+    # (20, 61) leaves 6.5e-11 at a corner-adjacent self-case node (the oracle
+    # shows the same); (30, 91) resolves every sum to ~1e-15
+    t.build_table(DuffyBuildConfig(regular_quad_order=30, radial_quad_order=91))
     g = D.TableGeometry(3, q, H=1.0)
     n = t.n_q_points
     for case in (0, 69, 100):
         for tt in (0, 13):
             tot = sum(t.get_entry_data(case * n * n + s * n + tt) for s in range(n))
             cf = D.box_potential_laplace3d(g.target_point(tt, case), [0, 0, 0], [1, 1, 1])
-            assert abs(tot - cf) <= 1e-11 * abs(cf)
+            assert abs(tot - cf) <= 1e-12 * abs(cf)
 
 
 def test_gpu_table_end_to_end(cuda):

@@ -14,7 +14,7 @@ from . import _capi, quadrature
 
 
 def _dev(a: np.ndarray, dtype=None) -> torch.Tensor:
-    t = torch.from_numpy(np.ascontiguousarray(a))
+    t = torch.from_numpy(np.array(a, copy=True, order="C"))
     if dtype is not None:
         t = t.to(dtype)
     return t.to("cuda", non_blocking=False)

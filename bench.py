@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--form", default="auto", choices=["auto", "f64", "split"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--slices", type=int, default=5)
     return ap.parse_args()
 
 
@@ -225,26 +226,90 @@ def main():
     stream = torch.cuda.current_stream()
     sh = int(stream.cuda_stream)
 
-    runner = F64Runner(tr, table, f, y)
-    form = "f64"
-    for _ in range(args.warmup):
-        runner(sh)
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(args.steps):
-        runner(sh)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / args.steps
-    value = n_pts * R / (ms * 1e-3) / 1e9
+    def timed(fn, steps, warmup, inner=None):
+        """Average ms per step over `steps` (after `warmup`), events on the
+        current stream; `inner` optionally returns (ev_a, ev_b) pairs
+        bracketing the dominant kernel inside each step."""
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        pairs = []
+        e0.record(stream)
+        for _ in range(steps):
+            r = fn()
+            if r is not None:
+                pairs.append(r)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        step_ms = e0.elapsed_time(e1) / steps
+        kern_ms = sum(a.elapsed_time(b) for a, b in pairs) / len(pairs) if pairs else None
+        return step_ms, kern_ms
+
     n = tr.n_q_points
     flops = 2.0 * R * pairs * n * n
     bytes_alg = 2.0 * 8 * n_pts * R + table.reduced_entry_ids.size * 8 + 4 * (pairs + tr.nboxes)
-    achieved_tf = flops / (ms * 1e-3) / 1e12
-    hbm_gbs = bytes_alg / (ms * 1e-3) / 1e9
-    log(f"f64: {ms:.2f} ms/step, {value:.3f} Gpts*RHS/s, {achieved_tf:.2f} TFLOP/s, {hbm_gbs:.1f} GB/s")
+    forms = {}
+
+    # plain FP64 kernel (reference semantics, CSR target boxes)
+    runner = F64Runner(tr, table, f, y)
+    ms_plain, _ = timed(lambda: runner(sh), args.steps, args.warmup)
+    y_plain = y.clone()
+    forms["plain_f64"] = {"ms": ms_plain, "value": n_pts * R / (ms_plain * 1e-3) / 1e9,
+                          "tflops": flops / (ms_plain * 1e-3) / 1e12}
+    log(f"plain f64: {ms_plain:.2f} ms/step, {forms['plain_f64']['value']:.3f} Gpts*RHS/s, "
+        f"{forms['plain_f64']['tflops']:.2f} TFLOP/s")
+    del runner
+
+    # batched plan: dense FP64 MFMA and Ozaki split BF16
+    from volumential_amd.payload import DevicePayload, prepare_table_data_and_entry_map
+    from volumential_amd.plan import FORMS, List1Plan
+
+    data, nmlz, _, _, _, _ = prepare_table_data_and_entry_map([table])
+    payload = DevicePayload(dim, q, data, nmlz, table.case_indices, table.arithmetic_metadata())
+    t0 = time.perf_counter()
+    plan = List1Plan(tr, payload, scaling_mode=_capi.SCALING_CANONICAL, scaling_power=2,
+                     log_displacement=False, table_root_extent=table.source_box_extent)
+    log(f"plan built in {time.perf_counter() - t0:.2f} s")
+    L = _capi.lib()
+
+    def plan_step(form, slices, y_out):
+        opts = _capi.SplitOpts(n_slices=slices, n_products=0)
+        fm = FORMS[form]
+        _capi.check(L.vl1_plan_stage(plan.handle, fm, f.data_ptr(), f.stride(0), R, opts, sh))
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        _capi.check(L.vl1_plan_contract(plan.handle, fm, R, y_out.data_ptr(), y_out.stride(0), opts, sh))
+        b.record(stream)
+        return a, b
+
+    y64 = torch.zeros_like(f)
+    ms64, k64 = timed(lambda: plan_step("f64", 0, y64), args.steps, args.warmup)
+    forms["dense_f64_mfma"] = {"ms": ms64, "kernel_ms": k64, "value": n_pts * R / (ms64 * 1e-3) / 1e9,
+                               "tflops": flops / (k64 * 1e-3) / 1e12}
+    log(f"dense f64: {ms64:.2f} ms/step (contract {k64:.2f} ms, "
+        f"{forms['dense_f64_mfma']['tflops']:.2f} TFLOP/s)")
+    d_plain = (y64 - y_plain).norm(dim=1) / y64.norm(dim=1)
+    log(f"plain vs dense f64 max rel L2 {float(d_plain.max()):.2e}")
+
+    S = args.slices
+    ysp = torch.zeros_like(f)
+    mss, ks = timed(lambda: plan_step("split", S, ysp), args.steps, args.warmup)
+    err_l2 = ((ysp - y64).norm(dim=1) / y64.norm(dim=1)).cpu().numpy()
+    err_inf = ((ysp - y64).abs().amax(dim=1) / y64.abs().amax(dim=1)).cpu().numpy()
+    nprod = S * (S + 1) // 2
+    mfma_flops = nprod * flops
+    forms["split_bf16"] = {"ms": mss, "kernel_ms": ks, "value": n_pts * R / (mss * 1e-3) / 1e9,
+                           "slices": S, "slice_products": nprod,
+                           "tflops_fp64_equiv": flops / (ks * 1e-3) / 1e12,
+                           "rel_l2_max": float(err_l2.max()), "rel_linf_max": float(err_inf.max())}
+    log(f"split S={S}: {mss:.2f} ms/step (contract {ks:.2f} ms), rel L2 max {err_l2.max():.2e}, "
+        f"Linf max {err_inf.max():.2e}")
+    value = forms["split_bf16"]["value"]
+    ms = mss
+    achieved = mfma_flops / (ks * 1e-3) / 1e12
 
     res = {
         "metric": "list-1 eval Gpts*RHS/s at <=1e-10 rel err; split-MFMA vs FP64 GPU kernel",
@@ -257,7 +322,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16 slices (MFMA) + f64 sums",
         "data": "synthetic (seeded 8-Gaussian densities, GPU-built Laplace table)",
         "config": {
             "workload": f"3D Laplace uniform level {level}, q={q}, {R} RHS",
@@ -265,22 +330,26 @@ def main():
             "target_points": int(n_pts),
             "rhs": R,
             "list1_pairs": int(pairs),
-            "form": form,
+            "form": f"split_bf16 S={S}",
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "mfma",
-            "achieved": achieved_tf,
-            "peak": PEAK_FP64_TFLOPS,
+            "achieved": achieved,
+            "peak": PEAK_BF16_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved_tf / PEAK_FP64_TFLOPS,
+            "frac": achieved / PEAK_BF16_TFLOPS,
             "traffic": None,
-            "kernel": "near_field_f64_kernel",
-            "kernel_ms": ms,
-            "flops_per_launch": flops,
-            "hbm_gbs_alg": hbm_gbs,
+            "kernel": f"split_bf16_kernel<{S}>",
+            "kernel_ms": ks,
+            "flops_per_launch": mfma_flops,
+            "hbm_gbs_alg": bytes_alg / (ks * 1e-3) / 1e9,
         },
-        "forms": {"f64": {"ms": ms, "value": value, "tflops": achieved_tf}},
+        "error_vs_fp64": {"rel_l2_per_rhs_max": float(err_l2.max()),
+                          "rel_linf_per_rhs_max": float(err_inf.max())},
+        "speedup_vs_plain_f64": ms_plain / mss,
+        "speedup_vs_dense_f64_mfma": ms64 / mss,
+        "forms": forms,
     }
     if not args.no_cpu_baseline:
         rows = min(R, 1)

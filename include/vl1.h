@@ -175,6 +175,14 @@ int vl1_plan_destroy(vl1_plan* plan);
  * every target of the plan's target boxes is overwritten. */
 int vl1_plan_eval(vl1_plan* plan, int form, const double* f, int64_t ldf, int n_rhs,
                   double* y, int64_t ldy, const vl1_split_opts* opts, void* stream);
+/* The two halves of vl1_plan_eval: stage the densities (gather into mode
+ * order, split into slices for the split form), then contract them into y.
+ * vl1_plan_contract requires a preceding vl1_plan_stage with the same form,
+ * RHS count and slice count on the same stream. */
+int vl1_plan_stage(vl1_plan* plan, int form, const double* f, int64_t ldf, int n_rhs,
+                   const vl1_split_opts* opts, void* stream);
+int vl1_plan_contract(vl1_plan* plan, int form, int n_rhs, double* y, int64_t ldy,
+                      const vl1_split_opts* opts, void* stream);
 /* Plan statistics: [0] list-1 pairs, [1] tiles, [2] (tile, case) steps,
  * [3] distinct (case, level) blocks, [4] device bytes. */
 int vl1_plan_stats(const vl1_plan* plan, int64_t* out5);

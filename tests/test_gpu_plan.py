@@ -1,0 +1,156 @@
+"""GPU parity of the batched plan forms (vl1_plan_eval).
+
+* FP64 dense MFMA form vs the CPU oracles: rel-L2 and rel-Linf <= 1e-12 per
+  RHS (uniform, adaptive canonical / per-level, 2D log term, permuted ids);
+* Ozaki-split BF16 form vs the CPU oracle: rel-L2 and rel-Linf <= 1e-10 per
+  RHS with 5 slices, error decreasing with the slice count;
+* the split form at a larger size vs the FP64 form (itself pinned above).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu as ocpu
+from oracle import list1_ref, orbit_ref
+from tests.gpu_helpers import adaptive, rel_errors, synthetic_table_levels
+from volumential_amd import densities, tree as vtree
+from volumential_amd.kernels import LaplaceKernel
+from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+from volumential_amd.wrangler import NearFieldWrangler
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOLSPLIT = 1e-10
+
+
+def _plan_eval(tr, tables, f, form, n_slices=5):
+    w = NearFieldWrangler(tr, tables if len(tables) > 1 else tables[0])
+    pot, _ = w.eval_direct_plan(tr.target_boxes, tr.neighbor_source_boxes_starts,
+                                tr.neighbor_source_boxes_lists,
+                                torch.as_tensor(np.atleast_2d(f), device="cuda"),
+                                form=form, n_slices=n_slices)
+    return pot[0].cpu().numpy()
+
+
+def _oracle(tr, tables, f, mode, power=2, logd=False):
+    meta = orbit_ref.case_metadata(tr.dim, tr.quad_order)
+    data = np.stack([t.compact_values() for t in tables])
+    nm = np.stack([t.mode_normalizers for t in tables])
+    tsl = int(round(np.log2(tr.root_extent / tables[0].source_box_extent)))
+    return list1_ref.near_field_from_tree(
+        tr, data, nm, meta, f, mode=mode, scaling_power=power, log_displacement=logd,
+        table_root_extent=tables[0].source_box_extent, table_starting_level=tsl,
+    )
+
+
+def _laplace_table(dim, q, extent=2.0, rule=(10, 31)):
+    t = NearFieldInteractionTable(q, dim, LaplaceKernel(dim), source_box_extent=extent)
+    return t.build_table(DuffyBuildConfig(regular_quad_order=rule[0], radial_quad_order=rule[1]))
+
+
+@pytest.mark.parametrize("n_rhs", [1, 5, 64])
+def test_f64_uniform_3d_q6(cuda, n_rhs):
+    print(f"case: plan f64 3D q=6 level 2 R={n_rhs}", flush=True)
+    tr = vtree.uniform_tree(3, 6, 2)
+    tables = synthetic_table_levels(3, 6, seed=1)
+    f = densities.gaussian_mixture_batch(tr.points, range(1000, 1000 + n_rhs))
+    got = _plan_eval(tr, tables, f, "f64")
+    meta = orbit_ref.case_metadata(3, 6)
+    ref = ocpu.near_field(tr, tables[0].compact_values(), tables[0].mode_normalizers, meta, f,
+                          mode="canonical", table_root_extent=2.0)
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL64 and linf <= TOL64, (l2, linf)
+
+
+def test_f64_rhs_multiple_of_64(cuda):
+    print("case: plan f64 3D q=3 level 2 R=128", flush=True)
+    tr = vtree.uniform_tree(3, 3, 2)
+    tables = synthetic_table_levels(3, 3, seed=2)
+    f = densities.gaussian_mixture_batch(tr.points, range(128))
+    got = _plan_eval(tr, tables, f, "f64")
+    ref = _oracle(tr, tables, f, "canonical")
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL64 and linf <= TOL64, (l2, linf)
+
+
+def test_f64_2d_log_term(cuda):
+    print("case: plan f64 2D q=4 level 5 log", flush=True)
+    tr = vtree.uniform_tree(2, 4, 5)
+    tables = synthetic_table_levels(2, 4, seed=3)
+    f = np.stack([np.exp(-np.sum(tr.points**2, axis=0) / 0.01),
+                  densities.gaussian_mixture(tr.points, 5)])
+    got = _plan_eval(tr, tables, f, "f64")
+    ref = _oracle(tr, tables, f, "canonical", 2, True)
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL64 and linf <= TOL64, (l2, linf)
+
+
+@pytest.mark.parametrize("dim,q,mode", [(2, 6, "per_level"), (3, 5, "per_level"),
+                                        (3, 4, "canonical"), (2, 3, "canonical")])
+def test_f64_adaptive(cuda, dim, q, mode):
+    print(f"case: plan f64 adaptive {dim}D q={q} {mode}", flush=True)
+    tr = adaptive(dim, q)
+    n_tables = tr.nlevels if mode == "per_level" else 1
+    tables = synthetic_table_levels(dim, q, n_tables=n_tables, seed=4)
+    f = densities.gaussian_mixture_batch(tr.points, [7, 8, 9])
+    got = _plan_eval(tr, tables, f, "f64")
+    ref = _oracle(tr, tables, f, mode, 2, dim == 2 and mode == "canonical")
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL64 and linf <= TOL64, (l2, linf)
+
+
+def test_f64_permuted_ids(cuda):
+    print("case: plan f64 permuted ids", flush=True)
+    tr = vtree.uniform_tree(2, 3, 3)
+    rng = np.random.default_rng(9)
+    n = tr.n_q_points
+    tr.local_ids = np.concatenate([rng.permutation(n) for _ in range(tr.nboxes)]).astype(np.int32)
+    tables = synthetic_table_levels(2, 3, seed=2)
+    f = densities.gaussian_mixture_batch(tr.points, [31, 32])
+    got = _plan_eval(tr, tables, f, "f64")
+    ref = _oracle(tr, tables, f, "canonical", 2, True)
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= TOL64 and linf <= TOL64, (l2, linf)
+
+
+@pytest.mark.parametrize("dim,q,level", [(3, 6, 2), (3, 5, 2), (2, 6, 4)])
+def test_split_vs_oracle(cuda, dim, q, level):
+    print(f"case: plan split {dim}D q={q} level {level}", flush=True)
+    tr = vtree.uniform_tree(dim, q, level)
+    tables = [_laplace_table(dim, q)]
+    f = densities.gaussian_mixture_batch(tr.points, range(1000, 1008))
+    got = _plan_eval(tr, tables, f, "split", 5)
+    ref = _oracle(tr, tables, f, "canonical", 2, dim == 2)
+    l2, linf = rel_errors(got, ref)
+    print(f"split S=5 rel L2 {l2:.2e} Linf {linf:.2e}")
+    assert l2 <= TOLSPLIT and linf <= TOLSPLIT, (l2, linf)
+
+
+def test_split_error_decreases_with_slices(cuda):
+    print("case: plan split slices 1..5", flush=True)
+    tr = vtree.uniform_tree(3, 4, 3)
+    tables = [_laplace_table(3, 4)]
+    f = densities.gaussian_mixture_batch(tr.points, range(4))
+    ref = _plan_eval(tr, tables, f, "f64")
+    errs = []
+    for s in range(1, 6):
+        got = _plan_eval(tr, tables, f, "split", s)
+        errs.append(rel_errors(got, ref)[0])
+    print("rel L2 by slices:", errs)
+    assert all(errs[i + 1] < errs[i] * 1e-1 for i in range(3)), errs
+    assert errs[4] <= TOLSPLIT
+
+
+def test_split_adaptive_per_level(cuda):
+    print("case: plan split adaptive per-level", flush=True)
+    tr = adaptive(3, 4)
+    tables = [_laplace_table(3, 4, extent=2.0 / 2**lev) for lev in range(tr.nlevels)]
+    f = densities.gaussian_mixture_batch(tr.points, range(6))
+    got = _plan_eval(tr, tables, f, "split", 5)
+    ref = _plan_eval(tr, tables, f, "f64")
+    ref_o = _oracle(tr, tables, f, "per_level")
+    assert rel_errors(ref, ref_o)[0] <= TOL64
+    l2, linf = rel_errors(got, ref_o)
+    assert l2 <= TOLSPLIT and linf <= TOLSPLIT, (l2, linf)

@@ -45,6 +45,8 @@ EXPORTED_SYMBOLS = (
     "vl1_plan_create",
     "vl1_plan_destroy",
     "vl1_plan_eval",
+    "vl1_plan_stage",
+    "vl1_plan_contract",
     "vl1_plan_stats",
     "vl1_duffy_build",
     "vl1_orbit_rows_host",
@@ -183,6 +185,8 @@ def lib():
     L.vl1_plan_create.argtypes = [C.POINTER(vp), vp, C.POINTER(PlanDesc), vp]
     L.vl1_plan_destroy.argtypes = [vp]
     L.vl1_plan_eval.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, i64, C.POINTER(SplitOpts), vp]
+    L.vl1_plan_stage.argtypes = [vp, C.c_int, vp, i64, C.c_int, C.POINTER(SplitOpts), vp]
+    L.vl1_plan_contract.argtypes = [vp, C.c_int, C.c_int, vp, i64, C.POINTER(SplitOpts), vp]
     L.vl1_plan_stats.argtypes = [vp, C.POINTER(i64)]
     L.vl1_duffy_build.argtypes = [C.POINTER(DuffyArgs), vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]

@@ -1,20 +1,704 @@
-// Batched list-1 evaluation plans (tile schedule + dense / split forms).
+// Batched list-1 evaluation plans: tile schedule, dense FP64-MFMA form and
+// Ozaki-split BF16-MFMA form.
+//
+// The reference evaluates list 1 once per density (volume_fmm.py:983-989,
+// expansion_wrangler_fpnd.py:3222-3489).  Here the densities are the GEMM N
+// dimension:  for every target box tb and list-1 source box sb,
+//     Y[:, (tb, r)] += alpha(sb) * M_{case(tb,sb), lev(sb)} . F[:, (sb, r)]
+//                      + disp(sb) * (nmlz . F[:, (sb, r)])           (2D log term)
+// where M_c[t][s] is the dense per-case block of the orbit-reduced table
+// (expanded once per plan).  Target boxes are grouped into tiles of B boxes
+// (B * R_chunk <= 64 columns) sorted by their case signature; a tile is
+// processed by one workgroup that keeps its FP64 output block in registers
+// and walks the distinct (case, level) blocks of its boxes ("steps").
+//
+// Split form (Ozaki scheme I): rows of M_c and columns of F are scaled by
+// powers of two into (-1, 1) and cut into S BF16 slices of 8 bits each
+// (slice i stored as integer * 2^-8i, exact in BF16).  Slice products
+// A_i X_j with i + j < S are accumulated by v_mfma_f32_16x16x32_bf16 into
+// three FP32 accumulator sets (i+j = 0, 1, >= 2): the first two hold exact
+// integer sums (|a||x| <= 2^14, K <= 256, at most two products), the third
+// carries the small tail terms.  After every step the three sets are summed
+// in FP64, scaled by the row/column exponents and the pair scaling, and
+// added to the FP64 output ("cross-box sums in FP64").
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
 #include "internal.h"
+
+namespace vl1 {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float floatx4;
+typedef __attribute__((ext_vector_type(4))) double doublex4;
+
+constexpr int kCols = 64;      // output columns per tile (boxes x RHS)
+constexpr int kMaxSlices = 6;
+
+static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// ------------------------------------------------------------------ setup
+// M[b][t][s] (FP64, [n_blocks][MP][KP], zero padded) from the compact table.
+__global__ void expand_blocks_kernel(int dim, int q, int n, int MP, int KP, int n_blocks,
+                                     const int32_t* __restrict__ blk_case,
+                                     const int32_t* __restrict__ blk_lev,
+                                     const uint8_t* __restrict__ perm,
+                                     const int8_t* __restrict__ sign,
+                                     const uint8_t* __restrict__ group,
+                                     const uint16_t* __restrict__ ranks,
+                                     const int32_t* __restrict__ offs,
+                                     const double* __restrict__ table, int64_t n_entries,
+                                     double* __restrict__ M) {
+  const int64_t total = (int64_t)n_blocks * MP * KP;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(i % KP);
+    const int t = (int)((i / KP) % MP);
+    const int b = (int)(i / ((int64_t)KP * MP));
+    double v = 0.0;
+    if (s < n && t < n) {
+      const int c = blk_case[b];
+      const int64_t rel = compact_rel_row(dim, q, perm + c * dim, sign + c * dim, group + c * dim, s, t);
+      v = table[(int64_t)blk_lev[b] * n_entries + offs[ranks[c]] + rel];
+    }
+    M[i] = v;
+  }
+}
+
+// Per block row: exponent e with |row| * 2^-e < 1, BF16 slices
+// A[b][i][t][s] = a_i * 2^(-8 i), a_i integer in [-128, 128], and the row
+// scale 2^(e - 7).  One wavefront per row.
+__global__ void split_rows_kernel(int MP, int KP, int n_blocks, int S,
+                                  const double* __restrict__ M, __bf16* __restrict__ A,
+                                  double* __restrict__ rowscale) {
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (row >= n_blocks * MP) return;
+  const int b = row / MP, t = row % MP;
+  const double* m = M + (int64_t)row * KP;
+  double mx = 0.0;
+  for (int s = lane; s < KP; s += 64) mx = fmax(mx, fabs(m[s]));
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  int e = 0;
+  if (mx > 0.0) frexp(mx, &e);  // mx = f 2^e, f in [0.5, 1)
+  if (lane == 0) rowscale[row] = mx > 0.0 ? ldexp(1.0, e - 7) : 0.0;
+  for (int s = lane; s < KP; s += 64) {
+    double r = mx > 0.0 ? ldexp(m[s], -e) : 0.0;  // |r| < 1
+    for (int i = 0; i < S; ++i) {
+      const double d = rint(ldexp(r, 7 + 8 * i));  // integer in [-128, 128]
+      r -= ldexp(d, -7 - 8 * i);
+      A[(((int64_t)b * S + i) * MP + t) * KP + s] = (__bf16)(float)ldexp(d, -8 * i);
+    }
+  }
+}
+
+// Densities -> per (box, rhs) column: BF16 slices X[box][j][r][k] (mode
+// order, zero padded to KP), column scale 2^(e - 7) and, for the 2D log term,
+// nm[box][r] = sum_sid nmlz[sid] f[sbeg + sid].  One wavefront per column.
+__global__ void split_cols_kernel(int n, int KP, int nboxes, int R, int S,
+                                  const double* __restrict__ f, int64_t ldf,
+                                  const int32_t* __restrict__ mode_point,
+                                  const int32_t* __restrict__ box_starts,
+                                  const double* __restrict__ nmlz, int log_disp,
+                                  __bf16* __restrict__ X, double* __restrict__ colscale,
+                                  double* __restrict__ nmdot) {
+  const int col = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (col >= nboxes * R) return;
+  const int box = col / R, r = col % R;
+  const double* fr = f + (int64_t)r * ldf;
+  double mx = 0.0, dot = 0.0;
+  for (int s = lane; s < n; s += 64) {
+    mx = fmax(mx, fabs(fr[mode_point[(int64_t)box * n + s]]));
+    if (log_disp) dot += nmlz[s] * fr[box_starts[box] + s];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o));
+    dot += __shfl_xor(dot, o);
+  }
+  int e = 0;
+  if (mx > 0.0) frexp(mx, &e);
+  if (lane == 0) {
+    colscale[col] = mx > 0.0 ? ldexp(1.0, e - 7) : 0.0;
+    nmdot[col] = dot;
+  }
+  for (int s = lane; s < KP; s += 64) {
+    double v = (s < n && mx > 0.0) ? ldexp(fr[mode_point[(int64_t)box * n + s]], -e) : 0.0;
+    for (int j = 0; j < S; ++j) {
+      const double d = rint(ldexp(v, 7 + 8 * j));
+      v -= ldexp(d, -7 - 8 * j);
+      X[(((int64_t)box * S + j) * R + r) * KP + s] = (__bf16)(float)ldexp(d, -8 * j);
+    }
+  }
+}
+
+// Densities in mode order for the FP64 form: XF[box][r][k] (zero padded).
+__global__ void gather_cols_f64_kernel(int n, int KP, int nboxes, int R,
+                                       const double* __restrict__ f, int64_t ldf,
+                                       const int32_t* __restrict__ mode_point,
+                                       const int32_t* __restrict__ box_starts,
+                                       const double* __restrict__ nmlz, int log_disp,
+                                       double* __restrict__ XF, double* __restrict__ nmdot) {
+  const int col = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (col >= nboxes * R) return;
+  const int box = col / R, r = col % R;
+  const double* fr = f + (int64_t)r * ldf;
+  double dot = 0.0;
+  for (int s = lane; s < KP; s += 64) {
+    XF[((int64_t)box * R + r) * KP + s] = s < n ? fr[mode_point[(int64_t)box * n + s]] : 0.0;
+    if (log_disp && s < n) dot += nmlz[s] * fr[box_starts[box] + s];
+  }
+  for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
+  if (lane == 0) nmdot[col] = dot;
+}
+
+struct TileParams {
+  int n, MP, KP, S;
+  int B, Rc, R;          // boxes per tile, RHS per tile, total RHS in the staged arrays
+  int r0;                // first RHS of this launch
+  int n_tiles;
+  const int32_t* tile_step_start;  // [n_tiles + 1]
+  const int32_t* tile_boxes;       // [n_tiles][B] target box or -1
+  const int32_t* step_block;       // [n_steps]
+  const int32_t* step_src;         // [n_steps][B] source box or -1
+  const double* step_alpha;        // [n_steps][B]
+  const double* step_disp;         // [n_steps][B]
+  const int32_t* mode_point;       // [nboxes][n]
+  const __bf16* A;                 // split form
+  const double* rowscale;          // [n_blocks][MP]
+  const __bf16* X;                 // [nboxes][S][R][KP]
+  const double* colscale;          // [nboxes][R]
+  const double* M;                 // dense form [n_blocks][MP][KP]
+  const double* XF;                // [nboxes][R][KP]
+  const double* nmdot;             // [nboxes][R]
+  double* y;
+  int64_t ldy;
+};
+
+// XCD-aware tile order: consecutive workgroup ids go round-robin over the 8
+// XCDs, so give each XCD a contiguous range of (spatially sorted) tiles.
+__device__ inline int xcd_tile(int wg, int n) {
+  const int per = (n + 7) / 8;
+  const int xcd = wg % 8, k = wg / 8;
+  const int t = xcd * per + k;
+  return t;
+}
+
+template <int S>
+__global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int KPL = p.KP + 8;  // LDS row pitch (bf16): conflict-free b128 reads
+  __bf16* sX = reinterpret_cast<__bf16*>(smem_raw);
+  double* sCol = reinterpret_cast<double*>(sX + (size_t)S * kCols * KPL);
+  double* sDsp = sCol + kCols;
+
+  const int tile = xcd_tile(blockIdx.x, p.n_tiles);
+  if (tile >= p.n_tiles) return;  // uniform over the workgroup
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int l16 = lane % 16, lg = lane / 16;
+  const int nthreads = blockDim.x;
+  const int ncols = p.B * p.Rc;
+
+  double yacc[2][4][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) yacc[a][b][e] = 0.0;
+
+  const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
+  const int vec_per_col = p.KP / 8;  // 16-byte vectors per column and slice
+  for (int st = st0; st < st1; ++st) {
+    const int blk = p.step_block[st];
+    __syncthreads();
+    // stage X slices of the step's source boxes
+    for (int v = threadIdx.x; v < S * kCols * vec_per_col; v += nthreads) {
+      const int kv = v % vec_per_col;
+      const int c = (v / vec_per_col) % kCols;
+      const int j = v / (vec_per_col * kCols);
+      uint4 val = make_uint4(0, 0, 0, 0);
+      if (c < ncols) {
+        const int b = c / p.Rc, r = c % p.Rc;
+        const int sb = p.step_src[st * p.B + b];
+        if (sb >= 0)
+          val = *reinterpret_cast<const uint4*>(
+              p.X + (((int64_t)sb * S + j) * p.R + p.r0 + r) * p.KP + kv * 8);
+      }
+      *reinterpret_cast<uint4*>(sX + ((size_t)j * kCols + c) * KPL + kv * 8) = val;
+    }
+    for (int c = threadIdx.x; c < kCols; c += nthreads) {
+      double cs = 0.0, ds = 0.0;
+      if (c < ncols) {
+        const int b = c / p.Rc, r = c % p.Rc;
+        const int sb = p.step_src[st * p.B + b];
+        if (sb >= 0) {
+          const int64_t ci = (int64_t)sb * p.R + p.r0 + r;
+          cs = p.colscale[ci] * p.step_alpha[st * p.B + b];
+          ds = p.step_disp[st * p.B + b] * p.nmdot[ci];
+        }
+      }
+      sCol[c] = cs;
+      sDsp[c] = ds;
+    }
+    __syncthreads();
+
+    floatx4 acc[3][2][4];
+#pragma unroll
+    for (int g = 0; g < 3; ++g)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[g][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const __bf16* Ab = p.A + (int64_t)blk * S * p.MP * p.KP;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const __bf16* Ai = Ab + (int64_t)i * p.MP * p.KP;
+      for (int kc = 0; kc < p.KP; kc += 32) {
+        bf16x8 a[2];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) {
+          const int row = 32 * wave + 16 * rt + l16;
+          a[rt] = *reinterpret_cast<const bf16x8*>(Ai + (int64_t)row * p.KP + kc + 8 * lg);
+        }
+#pragma unroll
+        for (int j = 0; j < S - i; ++j) {
+          constexpr int dummy = 0;
+          (void)dummy;
+          const int g = (i + j) < 2 ? (i + j) : 2;
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct) {
+            const bf16x8 bx = *reinterpret_cast<const bf16x8*>(
+                sX + ((size_t)j * kCols + 16 * ct + l16) * KPL + kc + 8 * lg);
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+              acc[g][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], bx, acc[g][rt][ct], 0, 0, 0);
+          }
+        }
+      }
+    }
+    // FP64 flush
+    const double* rs = p.rowscale + (int64_t)blk * p.MP;
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 32 * wave + 16 * rt + 4 * lg + e;
+        const double rsc = rs[row];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+          const int c = 16 * ct + l16;
+          const double v = ((double)acc[0][rt][ct][e] + (double)acc[1][rt][ct][e]) +
+                           (double)acc[2][rt][ct][e];
+          yacc[rt][ct][e] = fma(v * rsc, sCol[c], yacc[rt][ct][e] + sDsp[c]);
+        }
+      }
+  }
+  // epilogue: write targets of the tile's boxes
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 32 * wave + 16 * rt + 4 * lg + e;
+      if (row >= p.n) continue;
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        const int c = 16 * ct + l16;
+        if (c >= ncols) continue;
+        const int b = c / p.Rc, r = c % p.Rc;
+        const int tb = p.tile_boxes[tile * p.B + b];
+        if (tb < 0) continue;
+        p.y[(int64_t)(p.r0 + r) * p.ldy + p.mode_point[(int64_t)tb * p.n + row]] = yacc[rt][ct][e];
+      }
+    }
+}
+
+// Dense FP64 form: same schedule, v_mfma_f64_16x16x4f64 on the FP64 blocks.
+// Output lane layout of the f64 16x16x4 MFMA: lane l, register r holds
+// row l/16 + 4 r, column l % 16.
+__global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int KPL = p.KP + 1;  // doubles; odd pitch spreads columns over banks
+  double* sX = reinterpret_cast<double*>(smem_raw);
+  double* sCol = sX + (size_t)kCols * KPL;
+  double* sDsp = sCol + kCols;
+
+  const int tile = xcd_tile(blockIdx.x, p.n_tiles);
+  if (tile >= p.n_tiles) return;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int l16 = lane % 16, lg = lane / 16;
+  const int nthreads = blockDim.x;
+  const int ncols = p.B * p.Rc;
+
+  doublex4 yacc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) yacc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+
+  const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
+  for (int st = st0; st < st1; ++st) {
+    const int blk = p.step_block[st];
+    __syncthreads();
+    for (int v = threadIdx.x; v < kCols * p.KP; v += nthreads) {
+      const int k = v % p.KP, c = v / p.KP;
+      double val = 0.0;
+      if (c < ncols) {
+        const int b = c / p.Rc, r = c % p.Rc;
+        const int sb = p.step_src[st * p.B + b];
+        if (sb >= 0) val = p.XF[((int64_t)sb * p.R + p.r0 + r) * p.KP + k] * p.step_alpha[st * p.B + b];
+      }
+      sX[(size_t)c * KPL + k] = val;
+    }
+    for (int c = threadIdx.x; c < kCols; c += nthreads) {
+      double ds = 0.0;
+      if (c < ncols) {
+        const int b = c / p.Rc, r = c % p.Rc;
+        const int sb = p.step_src[st * p.B + b];
+        if (sb >= 0) ds = p.step_disp[st * p.B + b] * p.nmdot[(int64_t)sb * p.R + p.r0 + r];
+      }
+      sDsp[c] = ds;
+    }
+    __syncthreads();
+    const double* Mb = p.M + (int64_t)blk * p.MP * p.KP;
+    for (int kc = 0; kc < p.KP; kc += 4) {
+      double a[2];
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt) a[rt] = Mb[(int64_t)(32 * wave + 16 * rt + l16) * p.KP + kc + lg];
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        const double bx = sX[(size_t)(16 * ct + l16) * KPL + kc + lg];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+          yacc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], bx, yacc[rt][ct], 0, 0, 0);
+      }
+    }
+    // displacement term (same value for every row of a column)
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+      const double ds = sDsp[16 * ct + l16];
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) yacc[rt][ct][e] += ds;
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 32 * wave + 16 * rt + lg + 4 * e;
+      if (row >= p.n) continue;
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        const int c = 16 * ct + l16;
+        if (c >= ncols) continue;
+        const int b = c / p.Rc, r = c % p.Rc;
+        const int tb = p.tile_boxes[tile * p.B + b];
+        if (tb < 0) continue;
+        p.y[(int64_t)(p.r0 + r) * p.ldy + p.mode_point[(int64_t)tb * p.n + row]] = yacc[rt][ct][e];
+      }
+    }
+}
+
+}  // namespace vl1
 
 using namespace vl1;
 
-struct vl1_plan {
-  int dummy = 0;
+// ------------------------------------------------------------------- plan
+struct TileSchedule {
+  int B = 0, Rc = 0, n_tiles = 0;
+  DevBuf tile_step_start, tile_boxes, step_block, step_src, step_alpha, step_disp;
+  int64_t n_steps = 0;
 };
+
+struct vl1_plan {
+  const vl1_payload* payload = nullptr;
+  int dim = 0, n = 0, MP = 0, KP = 0;
+  int nboxes = 0, n_tgt = 0;
+  int64_t n_points = 0, n_pairs = 0;
+  int log_disp = 0;
+  // host pair data
+  std::vector<int32_t> tgt_boxes, pair_tgt, pair_src, pair_block;
+  std::vector<double> pair_alpha, pair_disp;
+  int n_blocks = 0;
+  std::vector<int32_t> blk_case, blk_lev;
+  // device
+  DevBuf mode_point, box_starts, M, rowscale, A;
+  int A_slices = 0;
+  std::map<int, TileSchedule*> schedules;  // by Rc
+  // staging buffers (grown on demand)
+  DevBuf X, colscale, nmdot, XF;
+  int staged_form = -1, staged_rhs = 0, staged_slices = -1;
+  ~vl1_plan() {
+    for (auto& kv : schedules) delete kv.second;
+  }
+};
+
+static int build_schedule(vl1_plan* pl, int Rc, TileSchedule** out, hipStream_t s) {
+  auto it = pl->schedules.find(Rc);
+  if (it != pl->schedules.end()) {
+    *out = it->second;
+    return VL1_OK;
+  }
+  const int B = std::max(1, kCols / Rc);
+  // per target: sorted list of (block, pair)
+  std::vector<std::vector<std::pair<int, int>>> per_tgt(pl->n_tgt);
+  for (int64_t k = 0; k < pl->n_pairs; ++k) per_tgt[pl->pair_tgt[k]].push_back({pl->pair_block[k], (int)k});
+  for (auto& v : per_tgt) std::sort(v.begin(), v.end());
+  // sort targets by signature (block list), stable in box order
+  std::vector<int> order(pl->n_tgt);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const auto& A = per_tgt[a];
+    const auto& Bv = per_tgt[b];
+    if (A.size() != Bv.size()) return A.size() > Bv.size();
+    for (size_t i = 0; i < A.size(); ++i)
+      if (A[i].first != Bv[i].first) return A[i].first < Bv[i].first;
+    return false;
+  });
+  auto sch = new TileSchedule();
+  sch->B = B;
+  sch->Rc = Rc;
+  sch->n_tiles = (pl->n_tgt + B - 1) / B;
+  std::vector<int32_t> tss(sch->n_tiles + 1, 0), tboxes((size_t)sch->n_tiles * B, -1);
+  std::vector<int32_t> sblk, ssrc;
+  std::vector<double> salpha, sdisp;
+  for (int t = 0; t < sch->n_tiles; ++t) {
+    std::vector<int> blocks;
+    for (int b = 0; b < B; ++b) {
+      const int idx = t * B + b;
+      if (idx >= pl->n_tgt) break;
+      const int tg = order[idx];
+      tboxes[(size_t)t * B + b] = pl->tgt_boxes[tg];
+      for (auto& pr : per_tgt[tg]) blocks.push_back(pr.first);
+    }
+    std::sort(blocks.begin(), blocks.end());
+    blocks.erase(std::unique(blocks.begin(), blocks.end()), blocks.end());
+    for (int blk : blocks) {
+      sblk.push_back(blk);
+      for (int b = 0; b < B; ++b) {
+        const int idx = t * B + b;
+        int src = -1;
+        double al = 0.0, ds = 0.0;
+        if (idx < pl->n_tgt) {
+          const auto& v = per_tgt[order[idx]];
+          auto f = std::lower_bound(v.begin(), v.end(), std::make_pair(blk, -1));
+          if (f != v.end() && f->first == blk) {
+            src = pl->pair_src[f->second];
+            al = pl->pair_alpha[f->second];
+            ds = pl->pair_disp[f->second];
+          }
+        }
+        ssrc.push_back(src);
+        salpha.push_back(al);
+        sdisp.push_back(ds);
+      }
+    }
+    tss[t + 1] = (int32_t)sblk.size();
+  }
+  sch->n_steps = (int64_t)sblk.size();
+  hipError_t e = upload(sch->tile_step_start, tss.data(), tss.size() * 4, s);
+  if (e == hipSuccess) e = upload(sch->tile_boxes, tboxes.data(), tboxes.size() * 4, s);
+  if (e == hipSuccess) e = upload(sch->step_block, sblk.data(), sblk.size() * 4, s);
+  if (e == hipSuccess) e = upload(sch->step_src, ssrc.data(), ssrc.size() * 4, s);
+  if (e == hipSuccess) e = upload(sch->step_alpha, salpha.data(), salpha.size() * 8, s);
+  if (e == hipSuccess) e = upload(sch->step_disp, sdisp.data(), sdisp.size() * 8, s);
+  if (e != hipSuccess) {
+    delete sch;
+    return fail(VL1_E_HIP, std::string("plan schedule upload: ") + hipGetErrorString(e));
+  }
+  pl->schedules[Rc] = sch;
+  *out = sch;
+  return VL1_OK;
+}
+
+static int ensure_A(vl1_plan* pl, int S, hipStream_t s) {
+  if (pl->A_slices == S) return VL1_OK;
+  hipError_t e = pl->A.alloc((size_t)pl->n_blocks * S * pl->MP * pl->KP * sizeof(__bf16));
+  if (e == hipSuccess && pl->rowscale.bytes == 0)
+    e = pl->rowscale.alloc((size_t)pl->n_blocks * pl->MP * sizeof(double));
+  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("plan A alloc: ") + hipGetErrorString(e));
+  const int rows = pl->n_blocks * pl->MP;
+  const int wpb = 4;
+  hipLaunchKernelGGL(split_rows_kernel, dim3((rows + wpb - 1) / wpb), dim3(64 * wpb), 0, s, pl->MP,
+                     pl->KP, pl->n_blocks, S, pl->M.as<double>(), pl->A.as<__bf16>(),
+                     pl->rowscale.as<double>());
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split_rows: ") + hipGetErrorString(e));
+  pl->A_slices = S;
+  return VL1_OK;
+}
 
 extern "C" {
 
-int vl1_plan_create(vl1_plan** out, const vl1_payload* p, const vl1_plan_desc* d, void* stream) {
-  (void)p;
-  (void)d;
-  (void)stream;
-  if (out) *out = nullptr;
-  return fail(VL1_E_UNSUPPORTED, "vl1_plan_create: not built yet");
+int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc* d, void* stream) {
+  clear_error();
+  if (!out || !pay || !d) return fail(VL1_E_ARG, "vl1_plan_create: null argument");
+  *out = nullptr;
+  if (pay->is_complex) return fail(VL1_E_UNSUPPORTED, "vl1_plan_create: complex payload");
+  if (d->dim != pay->dim) return fail(VL1_E_ARG, "vl1_plan_create: dimension mismatch");
+  if (d->n_boxes < 1 || d->n_tgt_boxes < 0 || d->n_points < 1)
+    return fail(VL1_E_ARG, "vl1_plan_create: bad sizes");
+  if (!d->box_centers || !d->box_levels || !d->box_starts || !d->box_counts || !d->target_boxes ||
+      !d->nbr_starts || !d->nbr_lists || !d->local_ids)
+    return fail(VL1_E_ARG, "vl1_plan_create: null host array");
+  if (d->scaling_mode < 0 || d->scaling_mode > 2)
+    return fail(VL1_E_ARG, "vl1_plan_create: bad scaling mode");
+  hipStream_t s = (hipStream_t)stream;
+  const int n = pay->n, dim = pay->dim;
+  auto pl = new vl1_plan();
+  pl->payload = pay;
+  pl->dim = dim;
+  pl->n = n;
+  pl->MP = round_up(n, 32);
+  pl->KP = round_up(n, 32);
+  pl->nboxes = d->n_boxes;
+  pl->n_tgt = d->n_tgt_boxes;
+  pl->n_points = d->n_points;
+  pl->log_disp = (d->scaling_mode == VL1_SCALING_CANONICAL && d->log_displacement) ? 1 : 0;
+  if (pl->MP / 32 > 8) {
+    delete pl;
+    return fail(VL1_E_UNSUPPORTED, "vl1_plan_create: more than 256 nodes per box");
+  }
+
+  // layout: every listed box holds exactly n particles whose local ids are a
+  // permutation of 0..n-1 (expansion_wrangler_fpnd.py:427-463)
+  std::vector<int32_t> mode_point((size_t)d->n_boxes * n, -1);
+  std::vector<char> used(d->n_boxes, 0);
+  for (int i = 0; i < d->n_tgt_boxes; ++i) used[d->target_boxes[i]] = 1;
+  for (int64_t k = 0; k < d->nbr_starts[d->n_tgt_boxes]; ++k) {
+    const int b = d->nbr_lists[k];
+    if (b < 0 || b >= d->n_boxes) {
+      delete pl;
+      return fail(VL1_E_ARG, "vl1_plan_create: source box id out of range");
+    }
+    used[b] = 1;
+  }
+  for (int b = 0; b < d->n_boxes; ++b) {
+    if (!used[b]) continue;
+    if (d->box_counts[b] != n) {
+      delete pl;
+      return fail(VL1_E_LAYOUT, "table-based near-field evaluation requires exactly " +
+                                    std::to_string(n) + " quadrature points per active box");
+    }
+    for (int k = 0; k < n; ++k) {
+      const int64_t pt = (int64_t)d->box_starts[b] + k;
+      if (pt < 0 || pt >= d->n_points) {
+        delete pl;
+        return fail(VL1_E_LAYOUT, "box particle range outside the point array");
+      }
+      const int m = d->local_ids[pt];
+      if (m < 0 || m >= n || mode_point[(size_t)b * n + m] >= 0) {
+        delete pl;
+        return fail(VL1_E_LAYOUT, "box-local node ids must be a permutation of 0..q^d-1");
+      }
+      mode_point[(size_t)b * n + m] = (int32_t)pt;
+    }
+  }
+  for (auto& v : mode_point)
+    if (v < 0) v = 0;  // unused boxes: never read
+
+  // pairs: case id by the reference's float encoding, level, scaling
+  const int64_t n_pairs = d->nbr_starts[d->n_tgt_boxes];
+  pl->n_pairs = n_pairs;
+  std::map<std::pair<int, int>, int> blk_of;
+  const double H = d->table_root_extent;
+  for (int i = 0; i < d->n_tgt_boxes; ++i) {
+    const int tb = d->target_boxes[i];
+    pl->tgt_boxes.push_back(tb);
+    for (int64_t k = d->nbr_starts[i]; k < d->nbr_starts[i + 1]; ++k) {
+      const int sb = d->nbr_lists[k];
+      const int lvl = d->box_levels[sb];
+      const double sext = d->root_extent * (1.0 / (double)(1 << lvl));
+      double code = 0.0;
+      for (int a = 0; a < dim; ++a)
+        code = code * 13.0 + ((d->box_centers[(int64_t)a * d->n_boxes + tb] -
+                               d->box_centers[(int64_t)a * d->n_boxes + sb]) /
+                                  sext * 4.0 +
+                              6.0 + d->case_encoding_bias);
+      const int64_t vid = (int64_t)(code + 0.5);
+      const int cid = (vid >= 0 && vid < (int64_t)pay->case_indices.size()) ? (int)pay->case_indices[vid] : -1;
+      if (cid < 0) {
+        delete pl;
+        return fail(VL1_E_ARG, "list-1 case encoding produced an invalid case id (target box " +
+                                   std::to_string(tb) + ", source box " + std::to_string(sb) + ")");
+      }
+      int lev = 0;
+      double alpha = 1.0, disp = 0.0;
+      if (d->scaling_mode == VL1_SCALING_CANONICAL) {
+        double num = 1.0, den = 1.0;
+        for (int p = 0; p < d->scaling_power; ++p) {
+          num *= sext;
+          den *= H;
+        }
+        alpha = num / den;
+        if (pl->log_disp) disp = -0.5 / M_PI * alpha * log(sext / H);
+      } else if (d->scaling_mode == VL1_SCALING_PER_LEVEL) {
+        const int rel = lvl - d->table_starting_level;
+        lev = lvl < d->table_starting_level ? 0 : (rel < pay->n_tables ? rel : pay->n_tables - 1);
+      } else if (lvl != d->table_starting_level) {
+        delete pl;
+        return fail(VL1_E_LEVEL, "Single near-field table level mismatch: source level " +
+                                     std::to_string(lvl) + " but table_starting_level is " +
+                                     std::to_string(d->table_starting_level));
+      }
+      auto key = std::make_pair(cid, lev);
+      auto f = blk_of.find(key);
+      int blk;
+      if (f == blk_of.end()) {
+        blk = (int)pl->blk_case.size();
+        blk_of[key] = blk;
+        pl->blk_case.push_back(cid);
+        pl->blk_lev.push_back(lev);
+      } else {
+        blk = f->second;
+      }
+      pl->pair_tgt.push_back(i);
+      pl->pair_src.push_back(sb);
+      pl->pair_block.push_back(blk);
+      pl->pair_alpha.push_back(alpha);
+      pl->pair_disp.push_back(disp);
+    }
+  }
+  pl->n_blocks = (int)pl->blk_case.size();
+
+  hipError_t e = upload(pl->mode_point, mode_point.data(), mode_point.size() * 4, s);
+  std::vector<int32_t> bstarts(d->box_starts, d->box_starts + d->n_boxes);
+  if (e == hipSuccess) e = upload(pl->box_starts, bstarts.data(), bstarts.size() * 4, s);
+  DevBuf dcase, dlev, dperm, dsign, dgroup;
+  if (e == hipSuccess && pl->n_blocks > 0) {
+    e = upload(dcase, pl->blk_case.data(), pl->blk_case.size() * 4, s);
+    if (e == hipSuccess) e = upload(dlev, pl->blk_lev.data(), pl->blk_lev.size() * 4, s);
+    if (e == hipSuccess) e = upload(dperm, pay->perm.data(), pay->perm.size(), s);
+    if (e == hipSuccess) e = upload(dsign, pay->sign.data(), pay->sign.size(), s);
+    if (e == hipSuccess) e = upload(dgroup, pay->group.data(), pay->group.size(), s);
+    if (e == hipSuccess) e = pl->M.alloc((size_t)pl->n_blocks * pl->MP * pl->KP * sizeof(double));
+    if (e == hipSuccess) {
+      const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
+      const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+      hipLaunchKernelGGL(expand_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n, pl->MP,
+                         pl->KP, pl->n_blocks, dcase.as<int32_t>(), dlev.as<int32_t>(),
+                         dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
+                         pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(), pay->table.as<double>(),
+                         pay->n_entries, pl->M.as<double>());
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (e != hipSuccess) {
+    delete pl;
+    return fail(VL1_E_HIP, std::string("vl1_plan_create: ") + hipGetErrorString(e));
+  }
+  *out = pl;
+  return VL1_OK;
 }
 
 int vl1_plan_destroy(vl1_plan* plan) {
@@ -22,17 +706,174 @@ int vl1_plan_destroy(vl1_plan* plan) {
   return VL1_OK;
 }
 
-int vl1_plan_eval(vl1_plan* plan, int form, const double* f, int64_t ldf, int n_rhs, double* y,
-                  int64_t ldy, const vl1_split_opts* opts, void* stream) {
-  (void)plan; (void)form; (void)f; (void)ldf; (void)n_rhs; (void)y; (void)ldy; (void)opts;
-  (void)stream;
-  return fail(VL1_E_UNSUPPORTED, "vl1_plan_eval: not built yet");
+static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts* opts,
+                        hipStream_t s, int* S_out, TileSchedule** sch_out) {
+  if (form != VL1_FORM_F64 && form != VL1_FORM_SPLIT_BF16)
+    return fail(VL1_E_ARG, "vl1_plan: unknown form");
+  if (n_rhs < 1) return fail(VL1_E_ARG, "vl1_plan: n_rhs must be >= 1");
+  if (n_rhs > kCols && n_rhs % kCols != 0)
+    return fail(VL1_E_UNSUPPORTED, "vl1_plan: RHS count must be <= 64 or a multiple of 64");
+  int S = 0;
+  if (form == VL1_FORM_SPLIT_BF16) {
+    S = opts ? opts->n_slices : 5;
+    if (S < 1 || S > kMaxSlices) return fail(VL1_E_ARG, "vl1_plan: n_slices must be 1..6");
+    const size_t lds = (size_t)S * kCols * (pl->KP + 8) * 2 + 2 * kCols * 8;
+    if (lds > 160 * 1024)
+      return fail(VL1_E_UNSUPPORTED, "vl1_plan: too many slices for the box size (LDS)");
+    int rc = ensure_A(pl, S, s);
+    if (rc != VL1_OK) return rc;
+  }
+  *S_out = S;
+  return build_schedule(pl, std::min(n_rhs, kCols), sch_out, s);
 }
 
-int vl1_plan_stats(const vl1_plan* plan, int64_t* out5) {
-  (void)plan;
-  (void)out5;
-  return fail(VL1_E_UNSUPPORTED, "vl1_plan_stats: not built yet");
+int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs,
+                   const vl1_split_opts* opts, void* stream) {
+  clear_error();
+  if (!pl || !f) return fail(VL1_E_ARG, "vl1_plan_stage: null argument");
+  if (ldf < pl->n_points) return fail(VL1_E_ARG, "vl1_plan_stage: bad leading dimension");
+  if (pl->n_tgt == 0 || pl->n_blocks == 0) return VL1_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int S = 0;
+  TileSchedule* sch = nullptr;
+  int rc = plan_prepare(pl, form, n_rhs, opts, s, &S, &sch);
+  if (rc != VL1_OK) return rc;
+  const vl1_payload* pay = pl->payload;
+  const int R = n_rhs;
+  hipError_t e = hipSuccess;
+  const int ncolumns = pl->nboxes * R;
+  if (pl->nmdot.bytes < (size_t)ncolumns * 8) e = pl->nmdot.alloc((size_t)ncolumns * 8);
+  const int wpb = 4;
+  const double* nmlz = pay->nmlz.as<double>();  // level-0 normalisers
+  if (form == VL1_FORM_SPLIT_BF16) {
+    const size_t xb = (size_t)pl->nboxes * S * R * pl->KP * 2;
+    if (e == hipSuccess && pl->X.bytes < xb) e = pl->X.alloc(xb);
+    if (e == hipSuccess && pl->colscale.bytes < (size_t)ncolumns * 8)
+      e = pl->colscale.alloc((size_t)ncolumns * 8);
+    if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
+    hipLaunchKernelGGL(split_cols_kernel, dim3((ncolumns + wpb - 1) / wpb), dim3(64 * wpb), 0, s, pl->n,
+                       pl->KP, pl->nboxes, R, S, f, ldf, pl->mode_point.as<int32_t>(),
+                       pl->box_starts.as<int32_t>(), nmlz, pl->log_disp, pl->X.as<__bf16>(),
+                       pl->colscale.as<double>(), pl->nmdot.as<double>());
+  } else {
+    const size_t xb = (size_t)pl->nboxes * R * pl->KP * 8;
+    if (e == hipSuccess && pl->XF.bytes < xb) e = pl->XF.alloc(xb);
+    if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
+    hipLaunchKernelGGL(gather_cols_f64_kernel, dim3((ncolumns + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
+                       pl->n, pl->KP, pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),
+                       pl->box_starts.as<int32_t>(), nmlz, pl->log_disp, pl->XF.as<double>(),
+                       pl->nmdot.as<double>());
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("staging launch: ") + hipGetErrorString(e));
+  pl->staged_form = form;
+  pl->staged_rhs = R;
+  pl->staged_slices = S;
+  return VL1_OK;
+}
+
+int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
+                      const vl1_split_opts* opts, void* stream) {
+  clear_error();
+  if (!pl || !y) return fail(VL1_E_ARG, "vl1_plan_contract: null argument");
+  if (ldy < pl->n_points) return fail(VL1_E_ARG, "vl1_plan_contract: bad leading dimension");
+  if (pl->n_tgt == 0 || pl->n_blocks == 0) return VL1_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int S = 0;
+  TileSchedule* sch = nullptr;
+  int rc = plan_prepare(pl, form, n_rhs, opts, s, &S, &sch);
+  if (rc != VL1_OK) return rc;
+  if (pl->staged_form != form || pl->staged_rhs != n_rhs || pl->staged_slices != S)
+    return fail(VL1_E_ARG, "vl1_plan_contract: densities were not staged for this form/RHS/slices");
+  const int R = n_rhs;
+  const int Rc = std::min(R, kCols);
+  hipError_t e;
+  TileParams prm{};
+  prm.n = pl->n;
+  prm.MP = pl->MP;
+  prm.KP = pl->KP;
+  prm.S = S;
+  prm.B = sch->B;
+  prm.Rc = Rc;
+  prm.R = R;
+  prm.n_tiles = sch->n_tiles;
+  prm.tile_step_start = sch->tile_step_start.as<int32_t>();
+  prm.tile_boxes = sch->tile_boxes.as<int32_t>();
+  prm.step_block = sch->step_block.as<int32_t>();
+  prm.step_src = sch->step_src.as<int32_t>();
+  prm.step_alpha = sch->step_alpha.as<double>();
+  prm.step_disp = sch->step_disp.as<double>();
+  prm.mode_point = pl->mode_point.as<int32_t>();
+  prm.A = pl->A.as<__bf16>();
+  prm.rowscale = pl->rowscale.as<double>();
+  prm.X = pl->X.as<__bf16>();
+  prm.colscale = pl->colscale.as<double>();
+  prm.M = pl->M.as<double>();
+  prm.XF = pl->XF.as<double>();
+  prm.nmdot = pl->nmdot.as<double>();
+  prm.y = y;
+  prm.ldy = ldy;
+  const int waves = pl->MP / 32;
+  const int grid = ((sch->n_tiles + 7) / 8) * 8;
+  for (int r0 = 0; r0 < R; r0 += Rc) {
+    prm.r0 = r0;
+    if (form == VL1_FORM_SPLIT_BF16) {
+      const size_t lds = (size_t)S * kCols * (pl->KP + 8) * 2 + 2 * kCols * 8;
+      const void* fn = nullptr;
+      switch (S) {
+        case 1: fn = (const void*)split_bf16_kernel<1>; break;
+        case 2: fn = (const void*)split_bf16_kernel<2>; break;
+        case 3: fn = (const void*)split_bf16_kernel<3>; break;
+        case 4: fn = (const void*)split_bf16_kernel<4>; break;
+        case 5: fn = (const void*)split_bf16_kernel<5>; break;
+        default: fn = (const void*)split_bf16_kernel<6>; break;
+      }
+      e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split attr: ") + hipGetErrorString(e));
+      switch (S) {
+        case 1: hipLaunchKernelGGL(split_bf16_kernel<1>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
+        case 2: hipLaunchKernelGGL(split_bf16_kernel<2>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
+        case 3: hipLaunchKernelGGL(split_bf16_kernel<3>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
+        case 4: hipLaunchKernelGGL(split_bf16_kernel<4>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
+        case 5: hipLaunchKernelGGL(split_bf16_kernel<5>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
+        default: hipLaunchKernelGGL(split_bf16_kernel<6>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
+      }
+    } else {
+      const size_t lds = ((size_t)kCols * (pl->KP + 1) + 2 * kCols) * 8;
+      e = hipFuncSetAttribute((const void*)dense_f64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+      if (e != hipSuccess) return fail(VL1_E_HIP, std::string("dense attr: ") + hipGetErrorString(e));
+      hipLaunchKernelGGL(dense_f64_kernel, dim3(grid), dim3(64 * waves), lds, s, prm);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(VL1_E_HIP, std::string("tile kernel launch: ") + hipGetErrorString(e));
+  }
+  return VL1_OK;
+}
+
+int vl1_plan_eval(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs, double* y,
+                  int64_t ldy, const vl1_split_opts* opts, void* stream) {
+  int rc = vl1_plan_stage(pl, form, f, ldf, n_rhs, opts, stream);
+  if (rc != VL1_OK) return rc;
+  return vl1_plan_contract(pl, form, n_rhs, y, ldy, opts, stream);
+}
+
+int vl1_plan_stats(const vl1_plan* pl, int64_t* out5) {
+  clear_error();
+  if (!pl || !out5) return fail(VL1_E_ARG, "vl1_plan_stats: null argument");
+  out5[0] = pl->n_pairs;
+  int64_t tiles = 0, steps = 0;
+  if (!pl->schedules.empty()) {
+    const TileSchedule* sch = pl->schedules.begin()->second;
+    tiles = sch->n_tiles;
+    steps = sch->n_steps;
+  }
+  out5[1] = tiles;
+  out5[2] = steps;
+  out5[3] = pl->n_blocks;
+  out5[4] = (int64_t)(pl->M.bytes + pl->A.bytes + pl->rowscale.bytes + pl->X.bytes + pl->XF.bytes +
+                      pl->colscale.bytes + pl->nmdot.bytes + pl->mode_point.bytes);
+  return VL1_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,116 @@
+"""Batched list-1 evaluation plans (``vl1_plan_*``).
+
+A plan binds one tree + list-1 CSR to one device payload: it computes the
+case / level / scaling of every (target box, source box) pair once, expands
+the per-case dense blocks, and then evaluates many densities per call
+(``mode_coefs[R, n_points]``) with
+
+* ``form="f64"``   dense FP64 blocks on FP64 MFMA (rel. error ~1e-15);
+* ``form="split"`` Ozaki-split BF16 MFMA with FP32 accumulation and FP64
+  cross-box sums (``n_slices`` BF16 pieces per operand).
+
+This is the batched counterpart of the per-RHS ``eval_direct`` loop of
+``volume_fmm.py:983-1031``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _capi
+
+FORMS = {"f64": _capi.FORM_F64, "split": _capi.FORM_SPLIT_BF16}
+
+
+class List1Plan:
+    def __init__(self, tree, payload, *, scaling_mode, scaling_power=2, log_displacement=False,
+                 table_root_extent, table_starting_level=0, case_encoding_bias=1e-10,
+                 target_boxes=None, starts=None, lists=None, stream=None):
+        L = _capi.lib()
+        tb = np.ascontiguousarray(tree.target_boxes if target_boxes is None else target_boxes,
+                                  dtype=np.int32)
+        st = np.ascontiguousarray(
+            tree.neighbor_source_boxes_starts if starts is None else starts, dtype=np.int32)
+        ls = np.ascontiguousarray(
+            tree.neighbor_source_boxes_lists if lists is None else lists, dtype=np.int32)
+        self._keep = dict(
+            centers=np.ascontiguousarray(tree.box_centers, np.float64),
+            levels=np.ascontiguousarray(tree.box_levels, np.uint8),
+            starts=np.ascontiguousarray(tree.box_target_starts, np.int32),
+            counts=np.ascontiguousarray(tree.box_target_counts_nonchild, np.int32),
+            tb=tb,
+            st=st,
+            ls=ls,
+            ids=np.ascontiguousarray(tree.local_ids, np.int32),
+        )
+        k = self._keep
+        desc = _capi.PlanDesc(
+            dim=int(tree.dim),
+            n_boxes=int(k["centers"].shape[1]),
+            n_tgt_boxes=int(tb.size),
+            n_points=int(k["ids"].size),
+            box_centers=k["centers"].ctypes.data,
+            box_levels=k["levels"].ctypes.data,
+            box_starts=k["starts"].ctypes.data,
+            box_counts=k["counts"].ctypes.data,
+            target_boxes=tb.ctypes.data,
+            nbr_starts=st.ctypes.data,
+            nbr_lists=ls.ctypes.data,
+            local_ids=k["ids"].ctypes.data,
+            root_extent=float(tree.root_extent),
+            table_root_extent=float(table_root_extent),
+            table_starting_level=int(table_starting_level),
+            scaling_mode=int(scaling_mode),
+            scaling_power=int(scaling_power),
+            log_displacement=int(bool(log_displacement)),
+            case_encoding_bias=float(case_encoding_bias),
+            boxes_per_tile=0,
+        )
+        handle = _capi.vp()
+        _capi.check(
+            L.vl1_plan_create(_capi.C.byref(handle), payload.handle, desc, _capi.stream_handle(stream)),
+            "vl1_plan_create",
+        )
+        self.handle = handle
+        self.payload = payload  # keep alive
+        self.n_points = int(k["ids"].size)
+
+    def eval(self, mode_coefs: torch.Tensor, out: torch.Tensor | None = None, form="split",
+             n_slices=5, stream=None) -> torch.Tensor:
+        """Evaluate ``mode_coefs[R, n_points]`` (CUDA float64) into ``out``
+        (targets of the plan's boxes overwritten; allocated if None)."""
+        f = mode_coefs
+        if f.dim() == 1:
+            f = f.unsqueeze(0)
+        if f.dtype != torch.float64 or f.device.type != "cuda" or f.stride(-1) != 1:
+            raise ValueError("mode_coefs must be a contiguous-rows float64 CUDA tensor")
+        if f.shape[1] != self.n_points:
+            raise ValueError("mode_coefs has the wrong number of points")
+        if out is None:
+            out = torch.zeros_like(f)
+        opts = _capi.SplitOpts(n_slices=int(n_slices), n_products=0)
+        _capi.check(
+            _capi.lib().vl1_plan_eval(
+                self.handle, FORMS[form], f.data_ptr(), int(f.stride(0)), int(f.shape[0]),
+                out.data_ptr(), int(out.stride(0)), opts, _capi.stream_handle(stream),
+            ),
+            "vl1_plan_eval",
+        )
+        return out
+
+    def stats(self):
+        arr = (_capi.i64 * 5)()
+        _capi.check(_capi.lib().vl1_plan_stats(self.handle, arr), "vl1_plan_stats")
+        return dict(zip(["pairs", "tiles", "steps", "blocks", "device_bytes"], list(arr)))
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _capi.lib().vl1_plan_destroy(self.handle)
+            self.handle = _capi.vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

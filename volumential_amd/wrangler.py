@@ -223,6 +223,63 @@ class NearFieldWrangler:
         return pot, TimingFuture()
 
 
+    # ------------------------------------------------------------ plans
+    def get_plan(self, out_kernel, target_boxes, neighbor_source_boxes_starts,
+                 neighbor_source_boxes_lists):
+        """Plan for one output kernel and one list-1 CSR (cached)."""
+        from .payload import DevicePayload
+        from .plan import List1Plan
+
+        tb = np.ascontiguousarray(_host(target_boxes), dtype=np.int32)
+        st = np.ascontiguousarray(_host(neighbor_source_boxes_starts), dtype=np.int32)
+        ls = np.ascontiguousarray(_host(neighbor_source_boxes_lists), dtype=np.int32)
+        key = (repr(out_kernel), tb.tobytes(), st.tobytes(), ls.tobytes())
+        if key in self._plans:
+            return self._plans[key]
+        tables = self.near_field_table[repr(out_kernel)]
+        t0 = tables[0]
+        data, nmlz, _, _, _, _ = prepare_table_data_and_entry_map(tables)
+        payload = DevicePayload(t0.dim, t0.quad_order, data, nmlz, t0.case_indices,
+                                t0.arithmetic_metadata())
+        nf = NearFieldFromCSR(
+            out_kernel,
+            {"n_tables": len(tables), "n_q_points": t0.n_q_points,
+             "n_table_entries": data.shape[1], "n_cases": t0.n_cases,
+             "quad_order": t0.quad_order},
+            **self.list1_extra_kwargs,
+        )
+        pol = nf.get_kernel_scaling_policy()
+        plan = List1Plan(
+            self.tree, payload,
+            scaling_mode=nf._mode_code(pol),
+            scaling_power=pol.scaling_power,
+            log_displacement=pol.log_displacement,
+            table_root_extent=t0.source_box_extent,
+            table_starting_level=self.table_starting_level(tables),
+            case_encoding_bias=nf.extra_kwargs["case_encoding_bias"],
+            target_boxes=tb, starts=st, lists=ls,
+        )
+        self._plans[key] = plan
+        return plan
+
+    def eval_direct_plan(self, target_boxes, neighbor_source_boxes_starts,
+                         neighbor_source_boxes_lists, mode_coefs, form="split", n_slices=5):
+        """Batched evaluation of ``mode_coefs[R, n_points]`` with a plan form
+        ("f64" or "split"); returns ``(pot, timing_future)`` like eval_direct."""
+        mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
+            np.asarray(mode_coefs))
+        mc = mc.to(device="cuda", dtype=torch.float64)
+        if mc.dim() == 1:
+            mc = mc.unsqueeze(0)
+        mc = mc.contiguous()
+        pot = self.output_zeros(int(mc.shape[0]))
+        for i, knl in enumerate(self.target_kernels):
+            plan = self.get_plan(knl, target_boxes, neighbor_source_boxes_starts,
+                                 neighbor_source_boxes_lists)
+            plan.eval(mc, out=pot[i], form=form, n_slices=n_slices)
+        return pot, TimingFuture()
+
+
 def _host(a):
     if isinstance(a, torch.Tensor):
         return a.detach().cpu().numpy()

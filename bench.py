@@ -47,6 +47,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--slices", type=int, default=5)
+    ap.add_argument("--only-split", action="store_true",
+                    help="time only the split form (profiling runs)")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="do not time the plain FP64 kernel (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (byte-counter calibration)")
     return ap.parse_args()
 
 
@@ -207,6 +213,14 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(2)
     _capi.lib()
+    if args.copy_calibration:
+        x = torch.ones(2**27, dtype=torch.float64, device="cuda")
+        z = torch.empty_like(x)
+        torch.cuda.synchronize()
+        z.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": z.numel() * 8}))
+        return
     dim, q, level, R = 3, args.q, args.level, args.rhs
     log(f"tree: {dim}D q={q} uniform level {level}")
     tr = vtree.uniform_tree(dim, q, level)
@@ -253,14 +267,17 @@ def main():
     forms = {}
 
     # plain FP64 kernel (reference semantics, CSR target boxes)
-    runner = F64Runner(tr, table, f, y)
-    ms_plain, _ = timed(lambda: runner(sh), args.steps, args.warmup)
-    y_plain = y.clone()
-    forms["plain_f64"] = {"ms": ms_plain, "value": n_pts * R / (ms_plain * 1e-3) / 1e9,
-                          "tflops": flops / (ms_plain * 1e-3) / 1e12}
-    log(f"plain f64: {ms_plain:.2f} ms/step, {forms['plain_f64']['value']:.3f} Gpts*RHS/s, "
-        f"{forms['plain_f64']['tflops']:.2f} TFLOP/s")
-    del runner
+    ms_plain, y_plain = None, None
+    if not args.skip_plain:
+        runner = F64Runner(tr, table, f, y)
+        ms_plain, _ = timed(lambda: runner(sh), args.steps, args.warmup)
+        y_plain = y.clone()
+        forms["plain_f64"] = {"ms": ms_plain, "value": n_pts * R / (ms_plain * 1e-3) / 1e9,
+                              "tflops": flops / (ms_plain * 1e-3) / 1e12,
+                              "hbm_gbs_alg": bytes_alg / (ms_plain * 1e-3) / 1e9}
+        log(f"plain f64: {ms_plain:.2f} ms/step, {forms['plain_f64']['value']:.3f} Gpts*RHS/s, "
+            f"{forms['plain_f64']['tflops']:.2f} TFLOP/s")
+        del runner
 
     # batched plan: dense FP64 MFMA and Ozaki split BF16
     from volumential_amd.payload import DevicePayload, prepare_table_data_and_entry_map
@@ -286,13 +303,19 @@ def main():
         return a, b
 
     y64 = torch.zeros_like(f)
-    ms64, k64 = timed(lambda: plan_step("f64", 0, y64), args.steps, args.warmup)
+    if args.only_split:
+        args.steps, args.warmup = max(1, args.steps), 0
+    ms64, k64 = timed(lambda: plan_step("f64", 0, y64), 1 if args.only_split else args.steps,
+                      0 if args.only_split else args.warmup)
     forms["dense_f64_mfma"] = {"ms": ms64, "kernel_ms": k64, "value": n_pts * R / (ms64 * 1e-3) / 1e9,
                                "tflops": flops / (k64 * 1e-3) / 1e12}
     log(f"dense f64: {ms64:.2f} ms/step (contract {k64:.2f} ms, "
         f"{forms['dense_f64_mfma']['tflops']:.2f} TFLOP/s)")
-    d_plain = (y64 - y_plain).norm(dim=1) / y64.norm(dim=1)
-    log(f"plain vs dense f64 max rel L2 {float(d_plain.max()):.2e}")
+    if y_plain is not None:
+        d_plain = (y64 - y_plain).norm(dim=1) / y64.norm(dim=1)
+        forms["plain_f64"]["rel_l2_vs_dense_f64"] = float(d_plain.max())
+        log(f"plain vs dense f64 max rel L2 {float(d_plain.max()):.2e}")
+        del y_plain
 
     S = args.slices
     ysp = torch.zeros_like(f)
@@ -347,7 +370,7 @@ def main():
         },
         "error_vs_fp64": {"rel_l2_per_rhs_max": float(err_l2.max()),
                           "rel_linf_per_rhs_max": float(err_inf.max())},
-        "speedup_vs_plain_f64": ms_plain / mss,
+        "speedup_vs_plain_f64": (ms_plain / mss) if ms_plain else None,
         "speedup_vs_dense_f64_mfma": ms64 / mss,
         "forms": forms,
     }

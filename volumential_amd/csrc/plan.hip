@@ -27,6 +27,7 @@
 #include <map>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -71,8 +72,12 @@ __global__ void expand_blocks_kernel(int dim, int q, int n, int MP, int KP, int 
 }
 
 // Per block row: exponent e with |row| * 2^-e < 1, BF16 slices
-// A[b][i][t][s] = a_i * 2^(-8 i), a_i integer in [-128, 128], and the row
-// scale 2^(e - 7).  One wavefront per row.
+// a_i * 2^(-8 i), a_i integer in [-128, 128], and the row scale 2^(e - 7).
+// The slices are stored in MFMA fragment order: for row tile rt (16 rows),
+// K chunk kc (32 columns) and slice i, the 64 lanes' 8-element operands of
+// v_mfma_f32_16x16x32_bf16 (lane l: row l % 16, columns 8 (l / 16) + 0..7)
+// form one contiguous 1 KiB block, [b][rt][kc][i][lane][8], so a wave's
+// fragment load is a single fully coalesced access.  One wavefront per row.
 __global__ void split_rows_kernel(int MP, int KP, int n_blocks, int S,
                                   const double* __restrict__ M, __bf16* __restrict__ A,
                                   double* __restrict__ rowscale) {
@@ -92,8 +97,28 @@ __global__ void split_rows_kernel(int MP, int KP, int n_blocks, int S,
     for (int i = 0; i < S; ++i) {
       const double d = rint(ldexp(r, 7 + 8 * i));  // integer in [-128, 128]
       r -= ldexp(d, -7 - 8 * i);
-      A[(((int64_t)b * S + i) * MP + t) * KP + s] = (__bf16)(float)ldexp(d, -8 * i);
+      const int rt = t / 16, lane_ = (s % 32) / 8 * 16 + t % 16, kc = s / 32;
+      A[((((int64_t)b * (MP / 16) + rt) * (KP / 32) + kc) * S + i) * 512 + lane_ * 8 + s % 8] =
+          (__bf16)(float)ldexp(d, -8 * i);
     }
+  }
+}
+
+// FP64 blocks in the fragment order of v_mfma_f64_16x16x4f64 (lane l holds
+// row l % 16, column 4 k4 + l / 16): MF[b][rt][k4][lane].
+__global__ void frag_f64_kernel(int MP, int KP, int n_blocks, const double* __restrict__ M,
+                                double* __restrict__ MF) {
+  const int64_t total = (int64_t)n_blocks * MP * KP;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+       o += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(o % 64);
+    const int64_t g = o / 64;
+    const int k4 = (int)(g % (KP / 4));
+    const int64_t bt = g / (KP / 4);
+    const int rt = (int)(bt % (MP / 16));
+    const int b = (int)(bt / (MP / 16));
+    const int t = 16 * rt + lane % 16, k = 4 * k4 + lane / 16;
+    MF[o] = M[((int64_t)b * MP + t) * KP + k];
   }
 }
 
@@ -174,12 +199,32 @@ struct TileParams {
   const double* rowscale;          // [n_blocks][MP]
   const __bf16* X;                 // [nboxes][S][R][KP]
   const double* colscale;          // [nboxes][R]
-  const double* M;                 // dense form [n_blocks][MP][KP]
+  const double* M;                 // dense blocks [n_blocks][MP][KP]
+  const double* MF;                // dense blocks in FP64 MFMA fragment order
   const double* XF;                // [nboxes][R][KP]
   const double* nmdot;             // [nboxes][R]
   double* y;
   int64_t ldy;
+  int32_t* phase;                  // [8] block each XCD is working on (hint)
 };
+
+// Phase alignment of the step loop.  The steps of a tile may run in any
+// order (their sum is order independent), and every tile walks the same
+// sorted block list, so a tile starts at the block its XCD's workgroups are
+// currently on: concurrently running workgroups then share one table block
+// in that XCD's L2 instead of spreading over all of them.  The phase word is
+// only a hint (plain racy stores); correctness does not depend on it.
+__device__ inline int phase_start(const TileParams& p, int st0, int st1, int xcd) {
+  const int nst = st1 - st0;
+  if (nst <= 1) return 0;
+  const int want = __builtin_nontemporal_load(p.phase + xcd);
+  int lo = st0, hi = st1;  // first step with block >= want (blocks sorted)
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (p.step_block[mid] < want) lo = mid + 1; else hi = mid;
+  }
+  return lo < st1 ? lo - st0 : 0;
+}
 
 // XCD-aware tile order: consecutive workgroup ids go round-robin over the 8
 // XCDs, so give each XCD a contiguous range of (spatially sorted) tiles.
@@ -190,51 +235,141 @@ __device__ inline int xcd_tile(int wg, int n) {
   return t;
 }
 
+// Stage the step's density image into LDS: every thread first issues all
+// of its global loads (so they are in flight together), then stores them.
+// The image is V vectors of 16 bytes; thread t owns vectors t, t + T, ...
+template <int NV, typename LoadFn, typename StoreFn>
+__device__ inline void staged_copy(int nvec, LoadFn load, StoreFn store) {
+  // rounds of up to 8 vectors per thread keep the in-flight buffer small
+  constexpr int kRound = NV < 8 ? NV : 8;
+#pragma unroll
+  for (int u0 = 0; u0 < NV; u0 += kRound) {
+    uint4 buf[kRound];
+#pragma unroll
+    for (int u = 0; u < kRound; ++u) {
+      const int v = threadIdx.x + (u0 + u) * blockDim.x;
+      buf[u] = (u0 + u < NV && v < nvec) ? load(v) : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kRound; ++u) {
+      const int v = threadIdx.x + (u0 + u) * blockDim.x;
+      if (u0 + u < NV && v < nvec) store(v, buf[u]);
+    }
+  }
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [0, N).
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// One pass of slice products over the K dimension: products (i, j) with
+// GLO <= i + j <= GMAX accumulate into acc[min(i + j - GLO, NSETS - 1)].
+// A wave owns row tiles w, w+4, w+8, w+12 (those below MP/16) and all four
+// column tiles.  Per K chunk the pass's table fragments sit in registers;
+// each density slice is read from LDS once and applied to every table slice
+// it pairs with.  A table slot is refilled for the next chunk right after
+// its last use.  Slice loops are compile-time unrolled (static_for).
+template <int S, int GLO, int GMAX, int NSETS>
+__device__ inline void split_pass(floatx4 (&acc)[NSETS][4][4], const __bf16* const (&Ar)[4],
+                                  int nrt, int KP, const __bf16* sX, int KPL, int l16, int lg) {
+  static_assert(GMAX <= S - 1, "diagonal beyond the slice count");
+  constexpr int IMAX = GMAX;
+  // Ar[rt] points at this lane's operand of (kc = 0, slice 0); fragment
+  // (kc, i) is 512 elements * (kc / 32 * S + i) further
+  bf16x8 af[IMAX + 1][4];
+  static_for<0, IMAX + 1>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+      if (rt < nrt) af[i][rt] = *reinterpret_cast<const bf16x8*>(Ar[rt] + i * 512);
+  });
+#pragma unroll 1
+  for (int kc = 0; kc < KP; kc += 32) {
+    const bool more = kc + 32 < KP;
+    static_for<0, IMAX + 1>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      bf16x8 xf[4];
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+        xf[ct] = *reinterpret_cast<const bf16x8*>(
+            sX + ((size_t)j * kCols + 16 * ct + l16) * KPL + kc + 8 * lg);
+      static_for<0, IMAX + 1>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if constexpr (i + j >= GLO && i + j <= GMAX) {
+          constexpr int g = (i + j - GLO < NSETS - 1) ? i + j - GLO : NSETS - 1;
+#pragma unroll
+          for (int rt = 0; rt < 4; ++rt)
+            if (rt < nrt) {
+#pragma unroll
+              for (int ct = 0; ct < 4; ++ct)
+                acc[g][rt][ct] =
+                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][rt], xf[ct], acc[g][rt][ct], 0, 0, 0);
+            }
+        }
+        constexpr int jlast = (GMAX - i) < IMAX ? (GMAX - i) : IMAX;
+        if constexpr (j == jlast) {
+          if (more) {
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt)
+              if (rt < nrt)
+                af[i][rt] = *reinterpret_cast<const bf16x8*>(Ar[rt] + ((kc / 32 + 1) * S + i) * 512);
+          }
+        }
+      });
+    });
+  }
+}
+
+// Split form.  Workgroup = 4 waves (one per SIMD, so a wave may use the whole
+// register file: accumulators in AGPRs, the FP64 output block in VGPRs).
+// Wave w owns row tiles w, w+4, w+8, w+12 below MP/16 and all 64 columns, so
+// every table fragment is fetched once per workgroup.  The slice products
+// run in two passes -- diagonals i+j = 0 and 1 (exact FP32 sums), then
+// i+j >= 2 -- each flushed to FP64 with the row/column scales.
 template <int S>
-__global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
+__global__ void __launch_bounds__(256) split_bf16_kernel(const TileParams p) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int KPL = p.KP + 8;  // LDS row pitch (bf16): conflict-free b128 reads
   __bf16* sX = reinterpret_cast<__bf16*>(smem_raw);
   double* sCol = reinterpret_cast<double*>(sX + (size_t)S * kCols * KPL);
   double* sDsp = sCol + kCols;
+  int64_t* sBase = reinterpret_cast<int64_t*>(sDsp + kCols);
 
   const int tile = xcd_tile(blockIdx.x, p.n_tiles);
   if (tile >= p.n_tiles) return;  // uniform over the workgroup
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int l16 = lane % 16, lg = lane / 16;
-  const int nthreads = blockDim.x;
   const int ncols = p.B * p.Rc;
+  const int vec_per_col = p.KP / 8;
+  const int nrt_total = p.MP / 16;
+  const int nrt = (nrt_total - wave + 3) / 4;  // row tiles of this wave
 
-  double yacc[2][4][4];
+  double yacc[4][4][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int e = 0; e < 4; ++e) yacc[a][b][e] = 0.0;
 
   const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
-  const int vec_per_col = p.KP / 8;  // 16-byte vectors per column and slice
-  for (int st = st0; st < st1; ++st) {
+  const int xcd = blockIdx.x % 8;
+  const int rot = phase_start(p, st0, st1, xcd);
+  for (int k = 0; k < st1 - st0; ++k) {
+    int st = st0 + rot + k;
+    if (st >= st1) st -= st1 - st0;
     const int blk = p.step_block[st];
-    __syncthreads();
-    // stage X slices of the step's source boxes
-    for (int v = threadIdx.x; v < S * kCols * vec_per_col; v += nthreads) {
-      const int kv = v % vec_per_col;
-      const int c = (v / vec_per_col) % kCols;
-      const int j = v / (vec_per_col * kCols);
-      uint4 val = make_uint4(0, 0, 0, 0);
-      if (c < ncols) {
-        const int b = c / p.Rc, r = c % p.Rc;
-        const int sb = p.step_src[st * p.B + b];
-        if (sb >= 0)
-          val = *reinterpret_cast<const uint4*>(
-              p.X + (((int64_t)sb * S + j) * p.R + p.r0 + r) * p.KP + kv * 8);
-      }
-      *reinterpret_cast<uint4*>(sX + ((size_t)j * kCols + c) * KPL + kv * 8) = val;
-    }
-    for (int c = threadIdx.x; c < kCols; c += nthreads) {
+    if (threadIdx.x == 0) __builtin_nontemporal_store(blk, p.phase + xcd);
+    __syncthreads();  // previous step done with sX
+    if (threadIdx.x < kCols) {
+      const int c = threadIdx.x;
       double cs = 0.0, ds = 0.0;
+      int64_t base = -1;
       if (c < ncols) {
         const int b = c / p.Rc, r = c % p.Rc;
         const int sb = p.step_src[st * p.B + b];
@@ -242,71 +377,108 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
           const int64_t ci = (int64_t)sb * p.R + p.r0 + r;
           cs = p.colscale[ci] * p.step_alpha[st * p.B + b];
           ds = p.step_disp[st * p.B + b] * p.nmdot[ci];
+          base = ((int64_t)sb * S * p.R + p.r0 + r) * p.KP;
         }
       }
       sCol[c] = cs;
       sDsp[c] = ds;
+      sBase[c] = base;
+    }
+    __syncthreads();
+    {
+      // image: S slices x 64 columns x KP/8 vectors; a thread's vectors
+      // u = 0..(8 KP / 256)-1 of every slice use the same columns
+      const int64_t jstride = (int64_t)p.R * p.KP;
+      const int per_slice = kCols * vec_per_col;
+      for (int w0 = 0; w0 < per_slice; w0 += 4 * blockDim.x) {
+        int col[4], kv8[4];
+        int64_t src[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int w = w0 + threadIdx.x + u * blockDim.x;
+          col[u] = w < per_slice ? w / vec_per_col : 0;
+          kv8[u] = (w % vec_per_col) * 8;
+          src[u] = w < per_slice ? sBase[col[u]] : -2;
+        }
+        uint4 buf[S][4];
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            buf[j][u] = src[u] >= 0 ? *reinterpret_cast<const uint4*>(p.X + src[u] + j * jstride + kv8[u])
+                                    : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (src[u] != -2)
+              *reinterpret_cast<uint4*>(sX + ((size_t)j * kCols + col[u]) * KPL + kv8[u]) = buf[j][u];
+      }
     }
     __syncthreads();
 
-    floatx4 acc[3][2][4];
+    const __bf16* Ar[4];
 #pragma unroll
-    for (int g = 0; g < 3; ++g)
+    for (int rt = 0; rt < 4; ++rt) {
+      const int rt16 = rt < nrt ? wave + 4 * rt : 0;
+      Ar[rt] = p.A + ((int64_t)blk * (p.MP / 16) + rt16) * (p.KP / 32) * S * 512 + lane * 8;
+    }
+    const double* rs = p.rowscale + (int64_t)blk * p.MP;
+    {
+      // pass 1: exact diagonals 0 and 1
+      floatx4 acc[2][4][4];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int g = 0; g < 2; ++g)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[g][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-    const __bf16* Ab = p.A + (int64_t)blk * S * p.MP * p.KP;
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int i = 0; i < S; ++i) {
-      const __bf16* Ai = Ab + (int64_t)i * p.MP * p.KP;
-      for (int kc = 0; kc < p.KP; kc += 32) {
-        bf16x8 a[2];
+          for (int b = 0; b < 4; ++b) acc[g][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+      split_pass<S, 0, (S > 1 ? 1 : 0), 2>(acc, Ar, nrt, p.KP, sX, KPL, l16, lg);
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt) {
-          const int row = 32 * wave + 16 * rt + l16;
-          a[rt] = *reinterpret_cast<const bf16x8*>(Ai + (int64_t)row * p.KP + kc + 8 * lg);
-        }
+      for (int rt = 0; rt < 4; ++rt) {
+        if (rt >= nrt) continue;
 #pragma unroll
-        for (int j = 0; j < S - i; ++j) {
-          constexpr int dummy = 0;
-          (void)dummy;
-          const int g = (i + j) < 2 ? (i + j) : 2;
+        for (int e = 0; e < 4; ++e) {
+          const double rsc = rs[16 * (wave + 4 * rt) + 4 * lg + e];
 #pragma unroll
           for (int ct = 0; ct < 4; ++ct) {
-            const bf16x8 bx = *reinterpret_cast<const bf16x8*>(
-                sX + ((size_t)j * kCols + 16 * ct + l16) * KPL + kc + 8 * lg);
-#pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
-              acc[g][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], bx, acc[g][rt][ct], 0, 0, 0);
+            const int c = 16 * ct + l16;
+            const double v = (double)acc[0][rt][ct][e] + (double)acc[1][rt][ct][e];
+            yacc[rt][ct][e] = fma(v * rsc, sCol[c], yacc[rt][ct][e] + sDsp[c]);
           }
         }
       }
     }
-    // FP64 flush
-    const double* rs = p.rowscale + (int64_t)blk * p.MP;
+    if constexpr (S > 2) {
+      // pass 2: the tail diagonals i+j >= 2 in one set
+      floatx4 acc[1][4][4];
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = 32 * wave + 16 * rt + 4 * lg + e;
-        const double rsc = rs[row];
+        for (int b = 0; b < 4; ++b) acc[0][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+      split_pass<S, 2, S - 1, 1>(acc, Ar, nrt, p.KP, sX, KPL, l16, lg);
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-          const int c = 16 * ct + l16;
-          const double v = ((double)acc[0][rt][ct][e] + (double)acc[1][rt][ct][e]) +
-                           (double)acc[2][rt][ct][e];
-          yacc[rt][ct][e] = fma(v * rsc, sCol[c], yacc[rt][ct][e] + sDsp[c]);
+      for (int rt = 0; rt < 4; ++rt) {
+        if (rt >= nrt) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double rsc = rs[16 * (wave + 4 * rt) + 4 * lg + e];
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct) {
+            const int c = 16 * ct + l16;
+            yacc[rt][ct][e] = fma((double)acc[0][rt][ct][e] * rsc, sCol[c], yacc[rt][ct][e]);
+          }
         }
       }
+    }
   }
   // epilogue: write targets of the tile's boxes
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int rt = 0; rt < 4; ++rt) {
+    if (rt >= nrt) continue;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int row = 32 * wave + 16 * rt + 4 * lg + e;
+      const int row = 16 * (wave + 4 * rt) + 4 * lg + e;
       if (row >= p.n) continue;
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
@@ -318,24 +490,29 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
         p.y[(int64_t)(p.r0 + r) * p.ldy + p.mode_point[(int64_t)tb * p.n + row]] = yacc[rt][ct][e];
       }
     }
+  }
 }
 
 // Dense FP64 form: same schedule, v_mfma_f64_16x16x4f64 on the FP64 blocks.
 // Output lane layout of the f64 16x16x4 MFMA: lane l, register r holds
-// row l/16 + 4 r, column l % 16.
+// row l/16 + 4 r, column l % 16.  Wave = 32 rows x 64 columns; the density
+// image (pre-multiplied by the pair scaling) is staged with all loads in
+// flight, the table fragments of the next K chunk are loaded while the
+// current one feeds the MFMAs.
 __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int KPL = p.KP + 1;  // doubles; odd pitch spreads columns over banks
+  const int KPL = p.KP + 2;  // doubles; pitch keeps 16-byte alignment
   double* sX = reinterpret_cast<double*>(smem_raw);
-  double* sCol = sX + (size_t)kCols * KPL;
-  double* sDsp = sCol + kCols;
+  double* sAl = sX + (size_t)kCols * KPL;
+  double* sDsp = sAl + kCols;
 
   const int tile = xcd_tile(blockIdx.x, p.n_tiles);
   if (tile >= p.n_tiles) return;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int l16 = lane % 16, lg = lane / 16;
-  const int nthreads = blockDim.x;
   const int ncols = p.B * p.Rc;
+  const int vec_per_col = p.KP / 2;  // 16-byte vectors (2 doubles) per column
+  const int nvec = kCols * vec_per_col;
 
   doublex4 yacc[2][4];
 #pragma unroll
@@ -344,43 +521,83 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
     for (int b = 0; b < 4; ++b) yacc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
 
   const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
-  for (int st = st0; st < st1; ++st) {
+  const int xcd = blockIdx.x % 8;
+  const int rot = phase_start(p, st0, st1, xcd);
+  for (int k = 0; k < st1 - st0; ++k) {
+    int st = st0 + rot + k;
+    if (st >= st1) st -= st1 - st0;
     const int blk = p.step_block[st];
+    if (threadIdx.x == 0) __builtin_nontemporal_store(blk, p.phase + xcd);
     __syncthreads();
-    for (int v = threadIdx.x; v < kCols * p.KP; v += nthreads) {
-      const int k = v % p.KP, c = v / p.KP;
-      double val = 0.0;
+    if (threadIdx.x < kCols) {
+      const int c = threadIdx.x;
+      double al = 0.0, ds = 0.0;
       if (c < ncols) {
         const int b = c / p.Rc, r = c % p.Rc;
         const int sb = p.step_src[st * p.B + b];
-        if (sb >= 0) val = p.XF[((int64_t)sb * p.R + p.r0 + r) * p.KP + k] * p.step_alpha[st * p.B + b];
+        if (sb >= 0) {
+          al = p.step_alpha[st * p.B + b];
+          ds = p.step_disp[st * p.B + b] * p.nmdot[(int64_t)sb * p.R + p.r0 + r];
+        }
       }
-      sX[(size_t)c * KPL + k] = val;
-    }
-    for (int c = threadIdx.x; c < kCols; c += nthreads) {
-      double ds = 0.0;
-      if (c < ncols) {
-        const int b = c / p.Rc, r = c % p.Rc;
-        const int sb = p.step_src[st * p.B + b];
-        if (sb >= 0) ds = p.step_disp[st * p.B + b] * p.nmdot[(int64_t)sb * p.R + p.r0 + r];
-      }
+      sAl[c] = al;
       sDsp[c] = ds;
     }
     __syncthreads();
-    const double* Mb = p.M + (int64_t)blk * p.MP * p.KP;
-    for (int kc = 0; kc < p.KP; kc += 4) {
-      double a[2];
-#pragma unroll
-      for (int rt = 0; rt < 2; ++rt) a[rt] = Mb[(int64_t)(32 * wave + 16 * rt + l16) * p.KP + kc + lg];
-#pragma unroll
-      for (int ct = 0; ct < 4; ++ct) {
-        const double bx = sX[(size_t)(16 * ct + l16) * KPL + kc + lg];
-#pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
-          yacc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], bx, yacc[rt][ct], 0, 0, 0);
+    auto ld = [&](int v) -> uint4 {
+      const int kv = v % vec_per_col, c = v / vec_per_col;
+      if (c < ncols) {
+        const int b = c / p.Rc, r = c % p.Rc;
+        const int sb = p.step_src[st * p.B + b];
+        if (sb >= 0)
+          return *reinterpret_cast<const uint4*>(p.XF + ((int64_t)sb * p.R + p.r0 + r) * p.KP + 2 * kv);
       }
+      return make_uint4(0, 0, 0, 0);
+    };
+    auto stv = [&](int v, uint4 val) {
+      const int kv = v % vec_per_col, c = v / vec_per_col;
+      double2 d = *reinterpret_cast<double2*>(&val);
+      const double al = sAl[c];
+      d.x *= al;
+      d.y *= al;
+      *reinterpret_cast<double2*>(sX + (size_t)c * KPL + 2 * kv) = d;
+    };
+    staged_copy<16>(nvec, ld, stv);  // 64 KP / 2 vectors over 2 MP threads = 16 each
+    __syncthreads();
+    // fragment-ordered block: operand of (row tile rt16, k group k4) for
+    // lane l is MF[((blk * MP/16 + rt16) * KP/4 + k4) * 64 + l]
+    const double* Mrow[2];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+      Mrow[rt] = p.MF + ((int64_t)blk * (p.MP / 16) + 2 * wave + rt) * (p.KP / 4) * 64 + lane;
+    double a_cur[2][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a_cur[rt][u] = Mrow[rt][u * 64];
+    for (int kc = 0; kc < p.KP; kc += 16) {
+      double a_nxt[2][4];
+      const bool more = kc + 16 < p.KP;
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          a_nxt[rt][u] = more ? Mrow[rt][(kc / 4 + 4 + u) * 64] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+          const double bx = sX[(size_t)(16 * ct + l16) * KPL + kc + 4 * u + lg];
+#pragma unroll
+          for (int rt = 0; rt < 2; ++rt)
+            yacc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_cur[rt][u], bx, yacc[rt][ct], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a_cur[rt][u] = a_nxt[rt][u];
     }
-    // displacement term (same value for every row of a column)
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
       const double ds = sDsp[16 * ct + l16];
@@ -427,15 +644,17 @@ struct vl1_plan {
   int log_disp = 0;
   // host pair data
   std::vector<int32_t> tgt_boxes, pair_tgt, pair_src, pair_block;
+  std::vector<uint64_t> tgt_morton;
   std::vector<double> pair_alpha, pair_disp;
   int n_blocks = 0;
   std::vector<int32_t> blk_case, blk_lev;
   // device
-  DevBuf mode_point, box_starts, M, rowscale, A;
+  DevBuf mode_point, box_starts, M, MF, rowscale, A;
   int A_slices = 0;
   std::map<int, TileSchedule*> schedules;  // by Rc
   // staging buffers (grown on demand)
   DevBuf X, colscale, nmdot, XF;
+  DevBuf phase;  // int32 [8]
   int staged_form = -1, staged_rhs = 0, staged_slices = -1;
   ~vl1_plan() {
     for (auto& kv : schedules) delete kv.second;
@@ -453,17 +672,12 @@ static int build_schedule(vl1_plan* pl, int Rc, TileSchedule** out, hipStream_t 
   std::vector<std::vector<std::pair<int, int>>> per_tgt(pl->n_tgt);
   for (int64_t k = 0; k < pl->n_pairs; ++k) per_tgt[pl->pair_tgt[k]].push_back({pl->pair_block[k], (int)k});
   for (auto& v : per_tgt) std::sort(v.begin(), v.end());
-  // sort targets by signature (block list), stable in box order
+  // tiles follow the Morton order of the target boxes: concurrently running
+  // tiles then share most source boxes (L2 reuse of their density slices)
   std::vector<int> order(pl->n_tgt);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    const auto& A = per_tgt[a];
-    const auto& Bv = per_tgt[b];
-    if (A.size() != Bv.size()) return A.size() > Bv.size();
-    for (size_t i = 0; i < A.size(); ++i)
-      if (A[i].first != Bv[i].first) return A[i].first < Bv[i].first;
-    return false;
-  });
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return pl->tgt_morton[a] < pl->tgt_morton[b]; });
   auto sch = new TileSchedule();
   sch->B = B;
   sch->Rc = Rc;
@@ -611,6 +825,28 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
   pl->n_pairs = n_pairs;
   std::map<std::pair<int, int>, int> blk_of;
   const double H = d->table_root_extent;
+  {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < d->n_tgt_boxes; ++i)
+      for (int a = 0; a < dim; ++a) {
+        const double c = d->box_centers[(int64_t)a * d->n_boxes + d->target_boxes[i]];
+        lo[a] = std::min(lo[a], c);
+        hi[a] = std::max(hi[a], c);
+      }
+    pl->tgt_morton.resize(d->n_tgt_boxes);
+    for (int i = 0; i < d->n_tgt_boxes; ++i) {
+      uint64_t code = 0;
+      uint32_t u[3] = {0, 0, 0};
+      for (int a = 0; a < dim; ++a) {
+        const double c = d->box_centers[(int64_t)a * d->n_boxes + d->target_boxes[i]];
+        const double w = hi[a] > lo[a] ? (c - lo[a]) / (hi[a] - lo[a]) : 0.0;
+        u[a] = (uint32_t)std::min(1048575.0, std::floor(w * 1048575.0));
+      }
+      for (int bit = 19; bit >= 0; --bit)
+        for (int a = 0; a < dim; ++a) code = (code << 1) | ((u[a] >> bit) & 1u);
+      pl->tgt_morton[i] = code;
+    }
+  }
   for (int i = 0; i < d->n_tgt_boxes; ++i) {
     const int tb = d->target_boxes[i];
     pl->tgt_boxes.push_back(tb);
@@ -671,6 +907,10 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
   pl->n_blocks = (int)pl->blk_case.size();
 
   hipError_t e = upload(pl->mode_point, mode_point.data(), mode_point.size() * 4, s);
+  if (e == hipSuccess) {
+    const int32_t zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    e = upload(pl->phase, zeros, sizeof(zeros), s);
+  }
   std::vector<int32_t> bstarts(d->box_starts, d->box_starts + d->n_boxes);
   if (e == hipSuccess) e = upload(pl->box_starts, bstarts.data(), bstarts.size() * 4, s);
   DevBuf dcase, dlev, dperm, dsign, dgroup;
@@ -689,6 +929,14 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
                          dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
                          pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(), pay->table.as<double>(),
                          pay->n_entries, pl->M.as<double>());
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = pl->MF.alloc(pl->M.bytes);
+    if (e == hipSuccess) {
+      const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
+      const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+      hipLaunchKernelGGL(frag_f64_kernel, dim3(grid), dim3(256), 0, s, pl->MP, pl->KP, pl->n_blocks,
+                         pl->M.as<double>(), pl->MF.as<double>());
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -717,7 +965,7 @@ static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts*
   if (form == VL1_FORM_SPLIT_BF16) {
     S = opts ? opts->n_slices : 5;
     if (S < 1 || S > kMaxSlices) return fail(VL1_E_ARG, "vl1_plan: n_slices must be 1..6");
-    const size_t lds = (size_t)S * kCols * (pl->KP + 8) * 2 + 2 * kCols * 8;
+    const size_t lds = (size_t)S * kCols * (pl->KP + 8) * 2 + 3 * kCols * 8;
     if (lds > 160 * 1024)
       return fail(VL1_E_UNSUPPORTED, "vl1_plan: too many slices for the box size (LDS)");
     int rc = ensure_A(pl, S, s);
@@ -809,16 +1057,18 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
   prm.X = pl->X.as<__bf16>();
   prm.colscale = pl->colscale.as<double>();
   prm.M = pl->M.as<double>();
+  prm.MF = pl->MF.as<double>();
   prm.XF = pl->XF.as<double>();
   prm.nmdot = pl->nmdot.as<double>();
   prm.y = y;
   prm.ldy = ldy;
+  prm.phase = pl->phase.as<int32_t>();
   const int waves = pl->MP / 32;
   const int grid = ((sch->n_tiles + 7) / 8) * 8;
   for (int r0 = 0; r0 < R; r0 += Rc) {
     prm.r0 = r0;
     if (form == VL1_FORM_SPLIT_BF16) {
-      const size_t lds = (size_t)S * kCols * (pl->KP + 8) * 2 + 2 * kCols * 8;
+      const size_t lds = (size_t)S * kCols * (pl->KP + 8) * 2 + 3 * kCols * 8;
       const void* fn = nullptr;
       switch (S) {
         case 1: fn = (const void*)split_bf16_kernel<1>; break;
@@ -831,15 +1081,15 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
       e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split attr: ") + hipGetErrorString(e));
       switch (S) {
-        case 1: hipLaunchKernelGGL(split_bf16_kernel<1>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
-        case 2: hipLaunchKernelGGL(split_bf16_kernel<2>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
-        case 3: hipLaunchKernelGGL(split_bf16_kernel<3>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
-        case 4: hipLaunchKernelGGL(split_bf16_kernel<4>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
-        case 5: hipLaunchKernelGGL(split_bf16_kernel<5>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
-        default: hipLaunchKernelGGL(split_bf16_kernel<6>, dim3(grid), dim3(64 * waves), lds, s, prm); break;
+        case 1: hipLaunchKernelGGL(split_bf16_kernel<1>, dim3(grid), dim3(256), lds, s, prm); break;
+        case 2: hipLaunchKernelGGL(split_bf16_kernel<2>, dim3(grid), dim3(256), lds, s, prm); break;
+        case 3: hipLaunchKernelGGL(split_bf16_kernel<3>, dim3(grid), dim3(256), lds, s, prm); break;
+        case 4: hipLaunchKernelGGL(split_bf16_kernel<4>, dim3(grid), dim3(256), lds, s, prm); break;
+        case 5: hipLaunchKernelGGL(split_bf16_kernel<5>, dim3(grid), dim3(256), lds, s, prm); break;
+        default: hipLaunchKernelGGL(split_bf16_kernel<6>, dim3(grid), dim3(256), lds, s, prm); break;
       }
     } else {
-      const size_t lds = ((size_t)kCols * (pl->KP + 1) + 2 * kCols) * 8;
+      const size_t lds = ((size_t)kCols * (pl->KP + 2) + 2 * kCols) * 8;
       e = hipFuncSetAttribute((const void*)dense_f64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
       if (e != hipSuccess) return fail(VL1_E_HIP, std::string("dense attr: ") + hipGetErrorString(e));
@@ -871,7 +1121,7 @@ int vl1_plan_stats(const vl1_plan* pl, int64_t* out5) {
   out5[1] = tiles;
   out5[2] = steps;
   out5[3] = pl->n_blocks;
-  out5[4] = (int64_t)(pl->M.bytes + pl->A.bytes + pl->rowscale.bytes + pl->X.bytes + pl->XF.bytes +
+  out5[4] = (int64_t)(pl->M.bytes + pl->MF.bytes + pl->A.bytes + pl->rowscale.bytes + pl->X.bytes + pl->XF.bytes +
                       pl->colscale.bytes + pl->nmdot.bytes + pl->mode_point.bytes);
   return VL1_OK;
 }

@@ -267,105 +267,56 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// One pass of slice products over the K dimension: products (i, j) with
-// GLO <= i + j <= GMAX accumulate into acc[min(i + j - GLO, NSETS - 1)].
-// A wave owns row tiles w, w+4, w+8, w+12 (those below MP/16) and all four
-// column tiles.  Per K chunk the pass's table fragments sit in registers;
-// each density slice is read from LDS once and applied to every table slice
-// it pairs with.  A table slot is refilled for the next chunk right after
-// its last use.  Slice loops are compile-time unrolled (static_for).
-template <int S, int GLO, int GMAX, int NSETS>
-__device__ inline void split_pass(floatx4 (&acc)[NSETS][4][4], const __bf16* const (&Ar)[4],
-                                  int nrt, int KP, const __bf16* sX, int KPL, int l16, int lg) {
-  static_assert(GMAX <= S - 1, "diagonal beyond the slice count");
-  constexpr int IMAX = GMAX;
-  // Ar[rt] points at this lane's operand of (kc = 0, slice 0); fragment
-  // (kc, i) is 512 elements * (kc / 32 * S + i) further
-  bf16x8 af[IMAX + 1][4];
-  static_for<0, IMAX + 1>([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-      if (rt < nrt) af[i][rt] = *reinterpret_cast<const bf16x8*>(Ar[rt] + i * 512);
-  });
-#pragma unroll 1
-  for (int kc = 0; kc < KP; kc += 32) {
-    const bool more = kc + 32 < KP;
-    static_for<0, IMAX + 1>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      bf16x8 xf[4];
-#pragma unroll
-      for (int ct = 0; ct < 4; ++ct)
-        xf[ct] = *reinterpret_cast<const bf16x8*>(
-            sX + ((size_t)j * kCols + 16 * ct + l16) * KPL + kc + 8 * lg);
-      static_for<0, IMAX + 1>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        if constexpr (i + j >= GLO && i + j <= GMAX) {
-          constexpr int g = (i + j - GLO < NSETS - 1) ? i + j - GLO : NSETS - 1;
-#pragma unroll
-          for (int rt = 0; rt < 4; ++rt)
-            if (rt < nrt) {
-#pragma unroll
-              for (int ct = 0; ct < 4; ++ct)
-                acc[g][rt][ct] =
-                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][rt], xf[ct], acc[g][rt][ct], 0, 0, 0);
-            }
-        }
-        constexpr int jlast = (GMAX - i) < IMAX ? (GMAX - i) : IMAX;
-        if constexpr (j == jlast) {
-          if (more) {
-#pragma unroll
-            for (int rt = 0; rt < 4; ++rt)
-              if (rt < nrt)
-                af[i][rt] = *reinterpret_cast<const bf16x8*>(Ar[rt] + ((kc / 32 + 1) * S + i) * 512);
-          }
-        }
-      });
-    });
-  }
+// Split form, software-pipelined.  Workgroup = 8 waves, two per SIMD.  Wave
+// w owns row tiles w and w + 8 below MP/16 and all 64 columns, so every
+// table fragment is fetched once per workgroup.
+//
+// The (step, K chunk) iterations of a tile form one pipeline.  Densities
+// stream through a double-buffered LDS ring one K chunk (S slices x 64
+// columns x 32) at a time: the chunk for the next iteration -- which may
+// belong to the next step -- is fetched into registers before the current
+// chunk's MFMAs are issued and written to the other ring slot afterwards.
+// Table fragments (MFMA fragment order, 1 KiB per wave load) for the next
+// chunk are issued right after their last use in the current one.
+// Slice products (i, j), i + j < S, accumulate into set min(i + j, 2):
+// sets 0 and 1 are exact integer sums; all three are added in FP64, scaled
+// by the row/column exponents and the pair scaling, at the end of a step
+// into the FP64 output block, which lives in LDS (each lane owns its own
+// words there, in MFMA output order) so that registers hold only the
+// accumulator sets and operands.
+//
+// LDS: ring [2][S][64 columns][32] bf16, 16-byte vector v of column c stored
+// at v ^ ((c >> 1) & 3) (conflict-free b128 reads without padding); output
+// block [MP/16][4][4][64] doubles; column descriptors [2][64] x 3.
+inline size_t split_lds_bytes(int S, int MP) {
+  return (size_t)2 * S * kCols * 32 * 2 + (size_t)(MP / 16) * 1024 * 8 + 6 * kCols * 8;
 }
 
-// Split form.  Workgroup = 4 waves (one per SIMD, so a wave may use the whole
-// register file: accumulators in AGPRs, the FP64 output block in VGPRs).
-// Wave w owns row tiles w, w+4, w+8, w+12 below MP/16 and all 64 columns, so
-// every table fragment is fetched once per workgroup.  The slice products
-// run in two passes -- diagonals i+j = 0 and 1 (exact FP32 sums), then
-// i+j >= 2 -- each flushed to FP64 with the row/column scales.
 template <int S>
-__global__ void __launch_bounds__(256) split_bf16_kernel(const TileParams p) {
+__global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int KPL = p.KP + 8;  // LDS row pitch (bf16): conflict-free b128 reads
-  __bf16* sX = reinterpret_cast<__bf16*>(smem_raw);
-  double* sCol = reinterpret_cast<double*>(sX + (size_t)S * kCols * KPL);
-  double* sDsp = sCol + kCols;
-  int64_t* sBase = reinterpret_cast<int64_t*>(sDsp + kCols);
+  constexpr int RING = S * kCols * 32;  // bf16 per ring slot
+  __bf16* sX = reinterpret_cast<__bf16*>(smem_raw);             // [2][S][64][32]
+  double* sY = reinterpret_cast<double*>(sX + 2 * RING);         // [MP/16][4][4][64]
+  double* sCol = sY + (size_t)(p.MP / 16) * 1024;                // [2][64]
+  double* sDsp = sCol + 2 * kCols;                               // [2][64]
+  int64_t* sBase = reinterpret_cast<int64_t*>(sDsp + 2 * kCols);  // [2][64]
 
   const int tile = xcd_tile(blockIdx.x, p.n_tiles);
   if (tile >= p.n_tiles) return;  // uniform over the workgroup
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int l16 = lane % 16, lg = lane / 16;
   const int ncols = p.B * p.Rc;
-  const int vec_per_col = p.KP / 8;
-  const int nrt_total = p.MP / 16;
-  const int nrt = (nrt_total - wave + 3) / 4;  // row tiles of this wave
-
-  double yacc[4][4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) yacc[a][b][e] = 0.0;
-
-  const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
+  const int nrt = (p.MP / 16 - wave + 7) / 8;  // row tiles of this wave (<= 2)
+  const int nkc = p.KP / 32;
   const int xcd = blockIdx.x % 8;
+  const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
+  const int nst = st1 - st0;
   const int rot = phase_start(p, st0, st1, xcd);
-  for (int k = 0; k < st1 - st0; ++k) {
-    int st = st0 + rot + k;
-    if (st >= st1) st -= st1 - st0;
-    const int blk = p.step_block[st];
-    if (threadIdx.x == 0) __builtin_nontemporal_store(blk, p.phase + xcd);
-    __syncthreads();  // previous step done with sX
+  auto step_of = [&](int k) { int st = st0 + rot + k; return st >= st1 ? st - nst : st; };
+
+  // column descriptors of a step into slot b: scale, displacement, base
+  auto load_cols = [&](int st, int slot) {
     if (threadIdx.x < kCols) {
       const int c = threadIdx.x;
       double cs = 0.0, ds = 0.0;
@@ -380,105 +331,171 @@ __global__ void __launch_bounds__(256) split_bf16_kernel(const TileParams p) {
           base = ((int64_t)sb * S * p.R + p.r0 + r) * p.KP;
         }
       }
-      sCol[c] = cs;
-      sDsp[c] = ds;
-      sBase[c] = base;
+      sCol[slot * kCols + c] = cs;
+      sDsp[slot * kCols + c] = ds;
+      sBase[slot * kCols + c] = base;
     }
-    __syncthreads();
-    {
-      // image: S slices x 64 columns x KP/8 vectors; a thread's vectors
-      // u = 0..(8 KP / 256)-1 of every slice use the same columns
-      const int64_t jstride = (int64_t)p.R * p.KP;
-      const int per_slice = kCols * vec_per_col;
-      for (int w0 = 0; w0 < per_slice; w0 += 4 * blockDim.x) {
-        int col[4], kv8[4];
-        int64_t src[4];
+  };
+  // chunk kc of the density image of the step whose columns are in slot cs:
+  // S x 64 columns x 4 vectors = 256 S vectors over 512 threads
+  constexpr int NV = (S + 1) / 2;
+  const int64_t jstride = (int64_t)p.R * p.KP;
+  auto fetch_chunk = [&](int cs, int kc, uint4 (&buf)[NV]) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int w = w0 + threadIdx.x + u * blockDim.x;
-          col[u] = w < per_slice ? w / vec_per_col : 0;
-          kv8[u] = (w % vec_per_col) * 8;
-          src[u] = w < per_slice ? sBase[col[u]] : -2;
-        }
-        uint4 buf[S][4];
-#pragma unroll
-        for (int j = 0; j < S; ++j)
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            buf[j][u] = src[u] >= 0 ? *reinterpret_cast<const uint4*>(p.X + src[u] + j * jstride + kv8[u])
-                                    : make_uint4(0, 0, 0, 0);
-#pragma unroll
-        for (int j = 0; j < S; ++j)
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (src[u] != -2)
-              *reinterpret_cast<uint4*>(sX + ((size_t)j * kCols + col[u]) * KPL + kv8[u]) = buf[j][u];
-      }
+    for (int m = 0; m < NV; ++m) {
+      const int u = threadIdx.x + 512 * m;
+      const int j = u / 256, col = (u % 256) / 4, v = u % 4;
+      const int64_t base = j < S ? sBase[cs * kCols + col] : -1;
+      buf[m] = base >= 0 ? *reinterpret_cast<const uint4*>(p.X + base + j * jstride + kc * 32 + v * 8)
+                         : make_uint4(0, 0, 0, 0);
     }
-    __syncthreads();
+  };
+  auto store_chunk = [&](int ring, const uint4 (&buf)[NV]) {
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      const int u = threadIdx.x + 512 * m;
+      const int j = u / 256, col = (u % 256) / 4, v = u % 4;
+      if (j < S)
+        *reinterpret_cast<uint4*>(sX + ring * RING + ((size_t)j * kCols + col) * 32 +
+                                  ((v ^ ((col >> 1) & 3)) * 8)) = buf[m];
+    }
+  };
+  // this lane's word of output element (row tile rt16, column tile ct, reg e)
+  auto yslot = [&](int rt16, int ct, int e) { return sY + ((rt16 * 4 + ct) * 4 + e) * 64 + lane; };
 
-    const __bf16* Ar[4];
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt) {
-      const int rt16 = rt < nrt ? wave + 4 * rt : 0;
-      Ar[rt] = p.A + ((int64_t)blk * (p.MP / 16) + rt16) * (p.KP / 32) * S * 512 + lane * 8;
+  for (int rt = 0; rt < 2; ++rt)
+    if (rt < nrt)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) *yslot(wave + 8 * rt, ct, e) = 0.0;
+  floatx4 acc[3][2][4];
+#pragma unroll
+  for (int g = 0; g < 3; ++g)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[g][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  if (nst > 0) {
+    // prologue: descriptors of the first step, its chunk 0 into ring 0
+    load_cols(step_of(0), 0);
+    __syncthreads();
+    uint4 buf[NV];
+    fetch_chunk(0, 0, buf);
+    store_chunk(0, buf);
+  }
+  const int64_t frag_tile = (int64_t)nkc * S * 512;  // bf16 per (block, row tile)
+  auto arow = [&](int blk, int rt) {
+    const int rt16 = rt < nrt ? wave + 8 * rt : 0;
+    return p.A + ((int64_t)blk * (p.MP / 16) + rt16) * frag_tile + lane * 8;
+  };
+  bf16x8 af[S][2];
+  if (nst > 0) {
+    const int blk = p.step_block[step_of(0)];
+    if (threadIdx.x == 0) __builtin_nontemporal_store(blk, p.phase + xcd);
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+      const __bf16* ar = arow(blk, rt);
+#pragma unroll
+      for (int i = 0; i < S; ++i)
+        if (rt < nrt) af[i][rt] = *reinterpret_cast<const bf16x8*>(ar + i * 512);
     }
-    const double* rs = p.rowscale + (int64_t)blk * p.MP;
-    {
-      // pass 1: exact diagonals 0 and 1
-      floatx4 acc[2][4][4];
+  }
+  __syncthreads();
+
+  const int total = nst * nkc;
+  for (int it = 0; it < total; ++it) {
+    const int k = it / nkc, kc = it % nkc;
+    const int ring = it & 1, cslot = k & 1;
+    const int st = step_of(k);
+    const int blk = p.step_block[st];
+    // where the next iteration's data comes from
+    const bool has_next = it + 1 < total;
+    const bool next_step = kc + 1 == nkc;
+    const int nkc_ = next_step ? 0 : kc + 1;
+    int nblk = blk;
+    if (has_next && next_step) {
+      nblk = p.step_block[step_of(k + 1)];
+      load_cols(step_of(k + 1), cslot ^ 1);  // read after the barrier below
+    }
+    uint4 xnext[NV];
+    if (has_next && !next_step) fetch_chunk(cslot, nkc_, xnext);
+
+    // MFMAs of this chunk: density slice j from LDS, table slices from regs
+    const __bf16* rX = sX + ring * RING;
+    static_for<0, S>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
 #pragma unroll
-      for (int g = 0; g < 2; ++g)
+      for (int ct = 0; ct < 4; ++ct) {
+        const int col = 16 * ct + l16;
+        const bf16x8 xf = *reinterpret_cast<const bf16x8*>(
+            rX + ((size_t)j * kCols + col) * 32 + ((lg ^ ((col >> 1) & 3)) * 8));
+        static_for<0, S - j>([&](auto ic) {
+          constexpr int i = S - j - 1 - decltype(ic)::value;
+          constexpr int g = (i + j) < 2 ? (i + j) : 2;
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+          for (int rt = 0; rt < 2; ++rt)
+            if (rt < nrt)
+              acc[g][rt][ct] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][rt], xf, acc[g][rt][ct], 0, 0, 0);
+        });
+      }
+      // slot i = S-1-j had its last partner: refill it for the next chunk
+      if (has_next) {
+        constexpr int i = S - 1 - j;
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+          if (rt < nrt)
+            af[i][rt] = *reinterpret_cast<const bf16x8*>(arow(nblk, rt) + ((int64_t)nkc_ * S + i) * 512);
+      }
+    });
+
+    if (next_step) {
+      // step done: add the three sets, scaled, to the FP64 block
+      const double* rs = p.rowscale + (int64_t)blk * p.MP;
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt) {
+        if (rt >= nrt) continue;
+        const int rt16 = wave + 8 * rt;
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+          const int c = 16 * ct + l16;
+          const double cs = sCol[cslot * kCols + c], ds = sDsp[cslot * kCols + c];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const double rsc = rs[16 * rt16 + 4 * lg + e];
+            const double v = ((double)acc[0][rt][ct][e] + (double)acc[1][rt][ct][e]) +
+                             (double)acc[2][rt][ct][e];
+            double* ys = yslot(rt16, ct, e);
+            *ys = fma(v * rsc, cs, *ys + ds);
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < 3; ++g)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 4; ++b) acc[g][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-      split_pass<S, 0, (S > 1 ? 1 : 0), 2>(acc, Ar, nrt, p.KP, sX, KPL, l16, lg);
-#pragma unroll
-      for (int rt = 0; rt < 4; ++rt) {
-        if (rt >= nrt) continue;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const double rsc = rs[16 * (wave + 4 * rt) + 4 * lg + e];
-#pragma unroll
-          for (int ct = 0; ct < 4; ++ct) {
-            const int c = 16 * ct + l16;
-            const double v = (double)acc[0][rt][ct][e] + (double)acc[1][rt][ct][e];
-            yacc[rt][ct][e] = fma(v * rsc, sCol[c], yacc[rt][ct][e] + sDsp[c]);
-          }
-        }
-      }
+      if (has_next && threadIdx.x == 0) __builtin_nontemporal_store(nblk, p.phase + xcd);
     }
-    if constexpr (S > 2) {
-      // pass 2: the tail diagonals i+j >= 2 in one set
-      floatx4 acc[1][4][4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[0][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-      split_pass<S, 2, S - 1, 1>(acc, Ar, nrt, p.KP, sX, KPL, l16, lg);
-#pragma unroll
-      for (int rt = 0; rt < 4; ++rt) {
-        if (rt >= nrt) continue;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const double rsc = rs[16 * (wave + 4 * rt) + 4 * lg + e];
-#pragma unroll
-          for (int ct = 0; ct < 4; ++ct) {
-            const int c = 16 * ct + l16;
-            yacc[rt][ct][e] = fma((double)acc[0][rt][ct][e] * rsc, sCol[c], yacc[rt][ct][e]);
-          }
-        }
-      }
+    __syncthreads();  // ring[ring ^ 1] free, next descriptors visible
+    if (has_next) {
+      if (next_step) fetch_chunk(cslot ^ 1, 0, xnext);
+      store_chunk(ring ^ 1, xnext);
     }
+    __syncthreads();  // next chunk visible
   }
   // epilogue: write targets of the tile's boxes
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt) {
+  for (int rt = 0; rt < 2; ++rt) {
     if (rt >= nrt) continue;
+    const int rt16 = wave + 8 * rt;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int row = 16 * (wave + 4 * rt) + 4 * lg + e;
+      const int row = 16 * rt16 + 4 * lg + e;
       if (row >= p.n) continue;
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
@@ -487,7 +504,7 @@ __global__ void __launch_bounds__(256) split_bf16_kernel(const TileParams p) {
         const int b = c / p.Rc, r = c % p.Rc;
         const int tb = p.tile_boxes[tile * p.B + b];
         if (tb < 0) continue;
-        p.y[(int64_t)(p.r0 + r) * p.ldy + p.mode_point[(int64_t)tb * p.n + row]] = yacc[rt][ct][e];
+        p.y[(int64_t)(p.r0 + r) * p.ldy + p.mode_point[(int64_t)tb * p.n + row]] = *yslot(rt16, ct, e);
       }
     }
   }
@@ -965,8 +982,7 @@ static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts*
   if (form == VL1_FORM_SPLIT_BF16) {
     S = opts ? opts->n_slices : 5;
     if (S < 1 || S > kMaxSlices) return fail(VL1_E_ARG, "vl1_plan: n_slices must be 1..6");
-    const size_t lds = (size_t)S * kCols * (pl->KP + 8) * 2 + 3 * kCols * 8;
-    if (lds > 160 * 1024)
+    if (split_lds_bytes(S, pl->MP) > 160 * 1024)
       return fail(VL1_E_UNSUPPORTED, "vl1_plan: too many slices for the box size (LDS)");
     int rc = ensure_A(pl, S, s);
     if (rc != VL1_OK) return rc;
@@ -1068,7 +1084,7 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
   for (int r0 = 0; r0 < R; r0 += Rc) {
     prm.r0 = r0;
     if (form == VL1_FORM_SPLIT_BF16) {
-      const size_t lds = (size_t)S * kCols * (pl->KP + 8) * 2 + 3 * kCols * 8;
+      const size_t lds = split_lds_bytes(S, pl->MP);
       const void* fn = nullptr;
       switch (S) {
         case 1: fn = (const void*)split_bf16_kernel<1>; break;
@@ -1081,12 +1097,12 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
       e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split attr: ") + hipGetErrorString(e));
       switch (S) {
-        case 1: hipLaunchKernelGGL(split_bf16_kernel<1>, dim3(grid), dim3(256), lds, s, prm); break;
-        case 2: hipLaunchKernelGGL(split_bf16_kernel<2>, dim3(grid), dim3(256), lds, s, prm); break;
-        case 3: hipLaunchKernelGGL(split_bf16_kernel<3>, dim3(grid), dim3(256), lds, s, prm); break;
-        case 4: hipLaunchKernelGGL(split_bf16_kernel<4>, dim3(grid), dim3(256), lds, s, prm); break;
-        case 5: hipLaunchKernelGGL(split_bf16_kernel<5>, dim3(grid), dim3(256), lds, s, prm); break;
-        default: hipLaunchKernelGGL(split_bf16_kernel<6>, dim3(grid), dim3(256), lds, s, prm); break;
+        case 1: hipLaunchKernelGGL(split_bf16_kernel<1>, dim3(grid), dim3(512), lds, s, prm); break;
+        case 2: hipLaunchKernelGGL(split_bf16_kernel<2>, dim3(grid), dim3(512), lds, s, prm); break;
+        case 3: hipLaunchKernelGGL(split_bf16_kernel<3>, dim3(grid), dim3(512), lds, s, prm); break;
+        case 4: hipLaunchKernelGGL(split_bf16_kernel<4>, dim3(grid), dim3(512), lds, s, prm); break;
+        case 5: hipLaunchKernelGGL(split_bf16_kernel<5>, dim3(grid), dim3(512), lds, s, prm); break;
+        default: hipLaunchKernelGGL(split_bf16_kernel<6>, dim3(grid), dim3(512), lds, s, prm); break;
       }
     } else {
       const size_t lds = ((size_t)kCols * (pl->KP + 2) + 2 * kCols) * 8;

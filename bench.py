@@ -146,6 +146,23 @@ class F64Runner:
         self.capi.check(self.L.vl1_near_field_f64(self.payload.handle, self.args, stream_handle))
 
 
+def measured_traffic(kernel, level, q, rhs):
+    """HBM-side bytes per launch of `kernel` on the default workload, from the
+    committed counter passes (profiles/c3_traffic.json: FETCH_SIZE and
+    WRITE_SIZE in KiB per dispatch, scaled by a 1 GiB copy's readings).
+    Counters cannot be read from inside this process, so other workloads
+    report None."""
+    if (level, q, rhs) != (5, 6, 64):
+        return None, None
+    path = os.path.join(ROOT, "profiles", "c3_traffic.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return float(d["kernels"][kernel]["traffic_bytes"]), "profiles/c3_traffic.json"
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def cpu_baseline(tree, table, f_host_rows, seconds):
     """C/OpenMP oracle (reference semantics, one evaluation per RHS) on a
     bounded sample of interior target boxes; returns points x RHS / s."""
@@ -333,6 +350,7 @@ def main():
     value = forms["split_bf16"]["value"]
     ms = mss
     achieved = mfma_flops / (ks * 1e-3) / 1e12
+    traffic, traffic_note = measured_traffic(f"split_bf16_kernel<{S}>", level, q, R)
 
     res = {
         "metric": "list-1 eval Gpts*RHS/s at <=1e-10 rel err; split-MFMA vs FP64 GPU kernel",
@@ -362,7 +380,8 @@ def main():
             "peak": PEAK_BF16_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_BF16_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_note,
             "kernel": f"split_bf16_kernel<{S}>",
             "kernel_ms": ks,
             "flops_per_launch": mfma_flops,

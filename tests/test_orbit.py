@@ -10,19 +10,26 @@ Brute force: every entry's compact row must be the position of the
 smallest full id of its orbit under all 2^d d! signed axis permutations.
 """
 
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
 from oracle import orbit_ref
 from volumential_amd import gallery, orbit
 
+KAT = json.loads((Path(__file__).parent / "golden" / "reference_known_answers.json").read_text())
 
-@pytest.mark.parametrize("dim,count,distinct", [(1, 7, 7), (2, 33, 11), (3, 139, 11)])
-def test_gallery_counts(dim, count, distinct):
+
+@pytest.mark.parametrize("kat", KAT["gallery"], ids=lambda k: f"{k['dim']}d")
+def test_gallery_counts(kat):
+    dim, count = kat["dim"], kat["n_cases"]
     vecs, encode, case_indices = gallery.generate_list1_gallery(dim)
     assert len(vecs) == count
     comps = {c for v in vecs for c in v}
-    assert len(comps) == distinct and min(comps) == -6 and max(comps) == 6
+    assert len(comps) == kat["distinct_components"]
+    assert min(comps) == kat["min"] and max(comps) == kat["max"]
     assert case_indices.size == 13**dim
     for cid, v in enumerate(vecs):
         assert case_indices[encode(v)] == cid
@@ -38,24 +45,26 @@ def test_gallery_matches_tree_walk(dim):
 
 
 def test_q3_3d_payload_known_answers():
+    kat = KAT["q3_3d_arithmetic_payload"]
     m = orbit.build_case_metadata(3, 3)
-    assert m.n_compact_entries == 2510
-    assert m.n_orbits * 27 * 27 == 7290
-    assert m.metadata_bytes == 1576
+    assert m.n_compact_entries == kat["canonical_rows"]
+    assert m.n_orbits * 27 * 27 == kat["dense_layout_rows"]
+    assert m.metadata_bytes == kat["metadata_bytes"]
     reps = orbit.enumerate_representatives(m)
-    assert reps.size == np.unique(reps).size == 2510
+    assert reps.size == np.unique(reps).size == kat["canonical_rows"]
 
 
 def test_sorting_network_known_answer():
-    q = 3
+    kat = KAT["sorting_network_row"]
     meta = {
-        "case_orbit_ranks": np.array([0], np.uint16),
-        "case_axis_perm": np.array([[0, 1, 2]], np.uint8),
-        "case_axis_sign": np.array([[1, 1, 1]], np.int8),
-        "case_axis_group": np.array([[1, 0, 1]], np.uint8),
+        "case_orbit_ranks": np.array(kat["case_orbit_ranks"], np.uint16),
+        "case_axis_perm": np.array(kat["case_axis_perm"], np.uint8),
+        "case_axis_sign": np.array(kat["case_axis_sign"], np.int8),
+        "case_axis_group": np.array(kat["case_axis_group"], np.uint8),
     }
-    row, sign = orbit_ref.arithmetic_entry(0, 2 * q * q, 2 * q * q, meta, q, 3)
-    assert row == 2 * 27 + 2 and sign == 1
+    row, sign = orbit_ref.arithmetic_entry(kat["case"], kat["source_mode"], kat["target_node"],
+                                           meta, kat["q"], kat["dim"])
+    assert row == kat["row"] and sign == kat["sign"]
 
 
 @pytest.mark.parametrize("dim,q", [(2, 1), (2, 2), (2, 3), (2, 4), (3, 1), (3, 2), (3, 3)])

@@ -51,6 +51,8 @@ def parse():
                     help="time only the split form (profiling runs)")
     ap.add_argument("--skip-plain", action="store_true",
                     help="do not time the plain FP64 kernel (profiling runs)")
+    ap.add_argument("--no-table-build", action="store_true",
+                    help="skip the near-field table build record (3D Laplace q=6, all cases)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (byte-counter calibration)")
     return ap.parse_args()
@@ -144,6 +146,33 @@ class F64Runner:
 
     def __call__(self, stream_handle):
         self.capi.check(self.L.vl1_near_field_f64(self.payload.handle, self.args, stream_handle))
+
+
+FLOP_PER_NODE_3D = 71.0  # declared FP64 work per (entry, node), 3D Laplace q=6
+
+
+def table_build_record(rules):
+    """Near-field table build, 3D Laplace q=6: every canonical entry of all
+    139 cases (140,496 entries) in one launch of vl1_duffy_build per rule
+    (GL tail order, tanh-sinh-fast order); kernel time from HIP events."""
+    from volumential_amd.kernels import LaplaceKernel
+    from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+
+    out = []
+    for reg, rad in rules:
+        t = NearFieldInteractionTable(6, 3, LaplaceKernel(3), source_box_extent=2.0)
+        t.build_table(DuffyBuildConfig(regular_quad_order=reg, radial_quad_order=rad))
+        ms = t.last_build_kernel_ms
+        ne = t.last_build_node_evals
+        rate = ne / (ms * 1e-3)
+        out.append({"rule": [reg, rad], "entries": int(t.reduced_entry_ids.size),
+                    "kernel_ms": ms, "build_s": t.last_build_seconds,
+                    "node_evals": ne, "node_evals_per_s": rate,
+                    "fp64_tflops_declared": rate * FLOP_PER_NODE_3D / 1e12,
+                    "fp64_frac": rate * FLOP_PER_NODE_3D / 1e12 / PEAK_FP64_TFLOPS})
+        log(f"table build rule {reg}/{rad}: kernel {ms:.1f} ms, {rate:.3e} node-evals/s, "
+            f"{out[-1]['fp64_frac'] * 100:.1f} % of FP64 peak (declared 71 flop/node)")
+    return out
 
 
 def measured_traffic(kernel, level, q, rhs):
@@ -393,6 +422,8 @@ def main():
         "speedup_vs_dense_f64_mfma": ms64 / mss,
         "forms": forms,
     }
+    if not args.no_table_build:
+        res["table_build"] = table_build_record([(20, 20), (40, 40)])
     if not args.no_cpu_baseline:
         rows = min(R, 1)
         f_host = f[:rows].cpu().numpy()

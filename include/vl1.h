@@ -207,7 +207,8 @@ typedef struct vl1_duffy_args {
   const double* node_sign;       /* [dim][n_nodes] */
   const double* node_jac;        /* [n_nodes] */
   const double* interp_nodes;    /* [quad_order] */
-  const double* bary_w;          /* [quad_order] */
+  const double* bary_w;          /* [quad_order] barycentric weights (validated; the kernel
+                                    evaluates the basis in product form) */
   double* result;                /* [n_entries] (x2 interleaved if Helmholtz) */
 } vl1_duffy_args;
 

@@ -99,6 +99,18 @@ def helmholtz3d(k):
     return g
 
 
+def helmholtz2d(k):
+    """(i/4) H0^(1)(k r), the sumpy 2D Helmholtz kernel (scipy's Hankel
+    function: a library routine, so the kernel value itself needs no pin)."""
+    from scipy.special import hankel1
+
+    def g(*d):
+        r = np.sqrt(sum(c * c for c in d))
+        return 0.25j * hankel1(0, k * r)
+
+    return g
+
+
 # ---------------------------------------------------------- quadratures
 def box_quad_duffy_nd(func, bounds, singular_point, deg_regular, radial_order):
     """``singular_integral_2d.py:856-949`` (tanh-sinh-fast), vectorised."""

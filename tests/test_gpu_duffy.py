@@ -5,7 +5,12 @@
 * Laplace 2D/3D: sampled canonical entries vs the oracle's per-entry Duffy
   quadrature with the same node rule, |diff| / max(1, |T|) <= 1e-12 (the
   north star's entrywise bar); mode sums vs closed-form box potentials;
-* end to end: a GPU-built table evaluated on the GPU vs the oracle.
+* end to end: a GPU-built table evaluated on the GPU vs the oracle;
+* the full 3D Laplace q=6 build (all 139 cases, 140,496 entries, one
+  launch) at GL tail / tanh-sinh orders (20, 20) and (40, 40): sampled
+  entries of every case kind vs the oracle at the same orders, <= 1e-12;
+* Helmholtz 2D (k=40, the level-4 extent of [-1,1]^2) and 3D: sampled
+  entries vs the oracle (scipy's Hankel function / exp for the kernel).
 """
 
 import numpy as np
@@ -16,7 +21,7 @@ from oracle import duffy_ref as D
 from oracle import list1_ref, orbit_ref
 from tests.gpu_helpers import rel_errors
 from volumential_amd import densities, tree as vtree
-from volumential_amd.kernels import ConstantKernel, LaplaceKernel
+from volumential_amd.kernels import ConstantKernel, HelmholtzKernel, LaplaceKernel
 from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
 from volumential_amd.wrangler import NearFieldWrangler
 
@@ -87,3 +92,55 @@ def test_gpu_table_end_to_end(cuda):
     l2, linf = rel_errors(pot[0].cpu().numpy(), ref[0])
     assert l2 <= 1e-12 and linf <= 1e-12
     assert np.all(np.isfinite(pot[0].cpu().numpy()))
+
+
+def _sample_ids(ids, n, n_cases, rng, per_kind=3):
+    """Entries from the self case, same-level, coarser and finer cases."""
+    case = ids // (n * n)
+    pick = [0, ids.size - 1]
+    from oracle import orbit_ref
+
+    vecs, _ = orbit_ref.gallery(3)
+    kinds = {}
+    for c in np.unique(case):
+        m = int(np.max(np.abs(vecs[c])))
+        kinds.setdefault({0: "self", 4: "same", 3: "coarse", 1: "coarse", 2: "fine", 6: "fine"}[m], []).append(c)
+    for cs in kinds.values():
+        sel = np.flatnonzero(np.isin(case, cs))
+        pick.extend(rng.choice(sel, size=min(per_kind, sel.size), replace=False).tolist())
+    return np.unique(pick)
+
+
+@pytest.mark.parametrize("rule", [(20, 20), (40, 40)])
+def test_c5_laplace3d_q6_full_build(cuda, rule):
+    print(f"case: C5 Laplace 3D q=6 full build rule {rule}", flush=True)
+    q = 6
+    t = NearFieldInteractionTable(q, 3, LaplaceKernel(3), source_box_extent=2.0)
+    t.build_table(DuffyBuildConfig(regular_quad_order=rule[0], radial_quad_order=rule[1]))
+    ids, vals = t.get_reduced_table_data()
+    assert ids.size == 140496 and np.all(np.isfinite(vals))
+    g = D.TableGeometry(3, q, H=2.0)
+    rng = np.random.default_rng(rule[0])
+    for i in _sample_ids(ids, q**3, 139, rng):
+        ref = g.entry(int(ids[i]), D.laplace(3), *rule)
+        assert abs(vals[i] - ref) / max(1.0, abs(ref)) <= 1e-12, (ids[i], vals[i], ref)
+
+
+@pytest.mark.parametrize("dim,q,k,extent", [(2, 6, 40.0, 0.125), (2, 4, 40.0, 2.0 / 2**7),
+                                            (3, 3, 5.0, 1.0)])
+def test_helmholtz_entries_match_oracle(cuda, dim, q, k, extent):
+    print(f"case: Helmholtz {dim}D q={q} k={k} H={extent}", flush=True)
+    reg, rad = 12, 41
+    t = NearFieldInteractionTable(q, dim, HelmholtzKernel(dim, k), source_box_extent=extent)
+    t.build_table(DuffyBuildConfig(regular_quad_order=reg, radial_quad_order=rad))
+    ids, vals = t.get_reduced_table_data()
+    assert np.iscomplexobj(vals) and np.all(np.isfinite(vals))
+    g = D.TableGeometry(dim, q, H=extent)
+    kern = D.helmholtz2d(k) if dim == 2 else D.helmholtz3d(k)
+    rng = np.random.default_rng(dim * 11 + q)
+    pick = np.unique(np.concatenate([[0, ids.size - 1], rng.integers(0, ids.size, 16)]))
+    scale = np.abs(vals).max()
+    for i in pick:
+        ref = g.entry(int(ids[i]), kern, reg, rad)
+        assert abs(vals[i] - ref) / max(1.0, abs(ref)) <= 1e-12, (ids[i], vals[i], ref)
+        assert abs(vals[i] - ref) <= 1e-11 * scale, (ids[i], vals[i], ref, scale)

@@ -37,27 +37,32 @@ struct DuffyParams {
   double* out;
 };
 
-__device__ inline double lagrange_basis(double y, int m, int q, const double* xi, const double* bw,
+// Lagrange basis l_m(y) on the q interpolation nodes, product form:
+//   l_m(y) = c_m * prod_{j != m} (y - xi_j),  c_m = 1 / prod_{j != m} (xi_m - xi_j)
+// (no divisions per node), with the reference's node-hit rule: if y lies
+// within tol of some node, l_m(y) is 1 at that node and 0 elsewhere
+// (lagrange.py:45-92, nearfield_potential_table.py:2172-2178).
+template <int Q>
+__device__ inline double lagrange_basis(double y, int m, int q, double cm, const double* xi,
                                         double tol) {
-  if (q == 1) return 1.0;
-  int hit_any = 0, hit_mode = 0;
-  double den = 0.0, num = 0.0;
-  for (int j = 0; j < q; ++j) {
-    const double diff = y - xi[j];
-    const bool hit = fabs(diff) < tol;
-    hit_any |= hit;
-    if (j == m) hit_mode = hit;
-    if (!hit) {
-      const double term = bw[j] / diff;
-      den += term;
-      if (j == m) num = term;
-    }
+  const int qq = Q > 0 ? Q : q;
+  bool hit_any = false, hit_m = false;
+  double pr = cm;
+#pragma unroll
+  for (int j = 0; j < (Q > 0 ? Q : 16); ++j) {
+    if (Q == 0 && j >= qq) break;
+    const double d = y - xi[j];
+    const bool h = fabs(d) < tol;
+    hit_any = hit_any || h;
+    hit_m = hit_m || (h && j == m);
+    pr *= (j == m) ? 1.0 : d;
   }
-  return hit_any ? (double)hit_mode : num / den;
+  return hit_any ? (hit_m ? 1.0 : 0.0) : pr;
 }
 
 // Kernel values, sumpy conventions (expression times global scaling):
-// Laplace 2D -log(r)/(2 pi), 3D 1/(4 pi r); Helmholtz 3D e^{ikr}/(4 pi r).
+// Laplace 2D -log(r)/(2 pi), 3D 1/(4 pi r); Helmholtz 2D (i/4) H0^(1)(k r)
+// = (-Y0(kr) + i J0(kr)) / 4, 3D e^{ikr}/(4 pi r).
 template <int DIM, int KIND>
 __device__ inline void kernel_value(double r2, double k, double& re, double& im) {
   if (KIND == VL1_KERNEL_CONSTANT) {
@@ -67,35 +72,49 @@ __device__ inline void kernel_value(double r2, double k, double& re, double& im)
     if (DIM == 2) {
       re = -0.25 / M_PI * log(r2);
     } else {
-      re = 0.25 / M_PI / sqrt(r2);
+      re = 0.25 / M_PI * rsqrt(r2);
     }
     im = 0.0;
+  } else if (DIM == 2) {
+    const double kr = k * sqrt(r2);
+    re = -0.25 * y0(kr);
+    im = 0.25 * j0(kr);
   } else {
-    const double r = sqrt(r2);
+    const double ir = rsqrt(r2);
     double sn, cs;
-    sincos(k * r, &sn, &cs);
-    re = 0.25 / M_PI * cs / r;
-    im = 0.25 / M_PI * sn / r;
+    sincos(k * (r2 * ir), &sn, &cs);
+    re = 0.25 / M_PI * cs * ir;
+    im = 0.25 / M_PI * sn * ir;
   }
 }
 
-template <int DIM, int KIND>
+// One lane per entry; every lane of a wave walks the same node sequence, so
+// node data are wave-uniform loads.
+template <int DIM, int KIND, int Q>
 __global__ void __launch_bounds__(256) duffy_kernel(const DuffyParams p) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= p.n_entries) return;
   constexpr int kMaxQ = 16;
-  double xi[kMaxQ], bw[kMaxQ];
-  for (int j = 0; j < p.q; ++j) {
-    xi[j] = p.xi[j];
-    bw[j] = p.bw[j];
-  }
-  double x[DIM], v[DIM];
+  const int q = Q > 0 ? Q : p.q;
+  const double* __restrict__ nu = p.u;
+  const double* __restrict__ nsg = p.sg;
+  const double* __restrict__ njac = p.jac;
+  double xi[kMaxQ];
+#pragma unroll
+  for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j)
+    if (Q > 0 || j < q) xi[j] = p.xi[j];
+  double x[DIM], v[DIM], cm[DIM];
   int m[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     x[a] = p.xt[a * p.n_entries + e];
     v[a] = p.vt[a * p.n_entries + e];
     m[a] = p.mode[a * p.n_entries + e];
+    double den = 1.0;
+#pragma unroll
+    for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j)
+      if (Q > 0 || j < q) den *= (j == m[a]) ? 1.0 : (xi[m[a]] - xi[j]);
+    cm[a] = 1.0 / den;
   }
   const double tol = 8.0 * 2.220446049250313e-16 * p.H;
   double acc_re = 0.0, acc_im = 0.0;
@@ -107,33 +126,35 @@ __global__ void __launch_bounds__(256) duffy_kernel(const DuffyParams p) {
     bool pos = true;
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
-      sgn[a] = p.sg[a * p.n_nodes + base];
+      sgn[a] = nsg[a * p.n_nodes + base];
       len[a] = sgn[a] < 0.0 ? v[a] : p.H - v[a];
       pos = pos && (len[a] > 0.0);
     }
-    if (!pos) continue;
+    if (!pos) continue;  // empty sub-box: every node of the block inactive
+    double sl[DIM], lenprod = 1.0;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      sl[a] = sgn[a] * len[a];
+      lenprod *= len[a];
+    }
     for (int64_t i = 0; i < p.nodes_per_sign; ++i) {
       const int64_t node = base + i;
       double r2 = 0.0;
       double yk[DIM];
 #pragma unroll
       for (int a = 0; a < DIM; ++a) {
-        yk[a] = v[a] + sgn[a] * len[a] * p.u[a * p.n_nodes + node];
+        yk[a] = v[a] + sl[a] * nu[a * p.n_nodes + node];
         const double d = x[a] - yk[a];
         r2 += d * d;
       }
       if (!(r2 > 0.0)) continue;
-      double w = p.jac[node];
+      double basis = njac[node] * lenprod;
 #pragma unroll
-      for (int a = 0; a < DIM; ++a) w *= len[a];
-      double basis = 1.0;
-#pragma unroll
-      for (int a = 0; a < DIM; ++a) basis *= lagrange_basis(yk[a], m[a], p.q, xi, bw, tol);
+      for (int a = 0; a < DIM; ++a) basis *= lagrange_basis<Q>(yk[a], m[a], q, cm[a], xi, tol);
       double kr, ki;
       kernel_value<DIM, KIND>(r2, p.k, kr, ki);
-      const double wb = w * basis;
-      acc_re = fma(kr, wb, acc_re);
-      if (KIND == VL1_KERNEL_HELMHOLTZ) acc_im = fma(ki, wb, acc_im);
+      acc_re = fma(kr, basis, acc_re);
+      if (KIND == VL1_KERNEL_HELMHOLTZ) acc_im = fma(ki, basis, acc_im);
     }
   }
   if (KIND == VL1_KERNEL_HELMHOLTZ) {
@@ -144,12 +165,26 @@ __global__ void __launch_bounds__(256) duffy_kernel(const DuffyParams p) {
   }
 }
 
-template <int DIM, int KIND>
-static hipError_t launch_duffy(const DuffyParams& prm, hipStream_t s) {
+template <int DIM, int KIND, int Q>
+static hipError_t launch_duffy_q(const DuffyParams& prm, hipStream_t s) {
   const int block = 256;
   const int64_t grid = (prm.n_entries + block - 1) / block;
-  hipLaunchKernelGGL((duffy_kernel<DIM, KIND>), dim3((unsigned)grid), dim3(block), 0, s, prm);
+  hipLaunchKernelGGL((duffy_kernel<DIM, KIND, Q>), dim3((unsigned)grid), dim3(block), 0, s, prm);
   return hipGetLastError();
+}
+
+template <int DIM, int KIND>
+static hipError_t launch_duffy(const DuffyParams& prm, hipStream_t s) {
+  switch (prm.q) {
+    case 1: return launch_duffy_q<DIM, KIND, 1>(prm, s);
+    case 2: return launch_duffy_q<DIM, KIND, 2>(prm, s);
+    case 3: return launch_duffy_q<DIM, KIND, 3>(prm, s);
+    case 4: return launch_duffy_q<DIM, KIND, 4>(prm, s);
+    case 5: return launch_duffy_q<DIM, KIND, 5>(prm, s);
+    case 6: return launch_duffy_q<DIM, KIND, 6>(prm, s);
+    case 8: return launch_duffy_q<DIM, KIND, 8>(prm, s);
+    default: return launch_duffy_q<DIM, KIND, 0>(prm, s);
+  }
 }
 
 }  // namespace vl1
@@ -165,8 +200,6 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
       a->n_nodes % a->nodes_per_sign != 0)
     return fail(VL1_E_ARG, "vl1_duffy_build: bad node counts");
   if (!(a->source_box_extent > 0)) return fail(VL1_E_ARG, "vl1_duffy_build: extent must be > 0");
-  if (a->kernel_kind == VL1_KERNEL_HELMHOLTZ && a->dim == 2)
-    return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: 2D Helmholtz tables are not implemented");
   if (a->n_entries == 0) return VL1_OK;
   if (!a->target_points || !a->decomposition || !a->mode_axes || !a->node_u || !a->node_sign ||
       !a->node_jac || !a->interp_nodes || !a->bary_w || !a->result)
@@ -181,6 +214,7 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
   if (a->dim == 2) {
     if (kind == VL1_KERNEL_CONSTANT) e = launch_duffy<2, VL1_KERNEL_CONSTANT>(prm, s);
     else if (kind == VL1_KERNEL_LAPLACE) e = launch_duffy<2, VL1_KERNEL_LAPLACE>(prm, s);
+    else if (kind == VL1_KERNEL_HELMHOLTZ) e = launch_duffy<2, VL1_KERNEL_HELMHOLTZ>(prm, s);
     else return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: kernel kind");
   } else {
     if (kind == VL1_KERNEL_CONSTANT) e = launch_duffy<3, VL1_KERNEL_CONSTANT>(prm, s);

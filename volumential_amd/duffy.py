@@ -58,8 +58,15 @@ def duffy_values(table, entry_ids, regular_order, radial_order, stream=None) -> 
         bary_w=d_bw.data_ptr(),
         result=out.data_ptr(),
     )
-    _capi.check(L.vl1_duffy_build(args, _capi.stream_handle(stream)), "vl1_duffy_build")
+    st = stream if stream is not None else torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(st)
+    _capi.check(L.vl1_duffy_build(args, _capi.stream_handle(st)), "vl1_duffy_build")
+    ev1.record(st)
     torch.cuda.synchronize()
+    table.last_build_kernel_ms = ev0.elapsed_time(ev1)
+    table.last_build_node_evals = int(n_entries) * int(nodes.n_nodes)
     res = out.cpu().numpy()
     if cplx:
         res = res[0::2] + 1j * res[1::2]

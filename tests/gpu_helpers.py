@@ -39,3 +39,33 @@ def adaptive(dim, q, max_level=4, point=0.23):
         return np.linalg.norm(c - point) < 1.2 * h
 
     return vtree.adaptive_tree(dim, q, refine, min_level=1, max_level=max_level)
+
+
+def beam_forest(seeds, q=6, tol=1e-5, min_level=4, max_level=6, k=40.0):
+    """Adaptive quadtrees refined around Gaussian-beam densities (one per
+    seed), merged into one forest."""
+    from volumential_amd import densities
+    from volumential_amd import tree as vtree
+
+    trees = []
+    for s in seeds:
+        f = (lambda P, s=s: densities.helmholtz_beams(P, s, k=k))
+        trees.append(vtree.adaptive_tree_batched(
+            2, q, vtree.interpolation_refiner(f, 2, q, tol, relative_to=4.0),
+            min_level=min_level, max_level=max_level))
+    return vtree.Forest(trees)
+
+
+def helmholtz_tables(dim, q, k, first_level, n_levels, root_extent=2.0, rule=(10, 31)):
+    """Per-level Helmholtz tables built on the GPU, source extents
+    root_extent * 2^-(first_level + lev)."""
+    from volumential_amd.kernels import HelmholtzKernel
+    from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+
+    out = []
+    for lev in range(n_levels):
+        t = NearFieldInteractionTable(q, dim, HelmholtzKernel(dim, k),
+                                      source_box_extent=root_extent / 2 ** (first_level + lev))
+        t.build_table(DuffyBuildConfig(regular_quad_order=rule[0], radial_quad_order=rule[1]))
+        out.append(t)
+    return out

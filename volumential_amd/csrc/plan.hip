@@ -45,7 +45,11 @@ static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 // ------------------------------------------------------------------ setup
 // M[b][t][s] (FP64, [n_blocks][MP][KP], zero padded) from the compact table.
-__global__ void expand_blocks_kernel(int dim, int q, int n, int MP, int KP, int n_blocks,
+// Complex tables (interleaved re, im) expand to the real 2n x 2n block
+// [[Re T, -Im T], [Im T, Re T]] acting on [Re f; Im f] and giving
+// [Re y; Im y], so both contraction kernels run unchanged on complex data
+// (8 real flops per complex multiply-add, as the complex product itself).
+__global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int MP, int KP, int n_blocks,
                                      const int32_t* __restrict__ blk_case,
                                      const int32_t* __restrict__ blk_lev,
                                      const uint8_t* __restrict__ perm,
@@ -62,10 +66,19 @@ __global__ void expand_blocks_kernel(int dim, int q, int n, int MP, int KP, int 
     const int t = (int)((i / KP) % MP);
     const int b = (int)(i / ((int64_t)KP * MP));
     double v = 0.0;
-    if (s < n && t < n) {
+    const int rows = cplx ? 2 * n : n;
+    if (s < rows && t < rows) {
       const int c = blk_case[b];
-      const int64_t rel = compact_rel_row(dim, q, perm + c * dim, sign + c * dim, group + c * dim, s, t);
-      v = table[(int64_t)blk_lev[b] * n_entries + offs[ranks[c]] + rel];
+      const int ss = s % n, tt = t % n;
+      const int64_t rel = compact_rel_row(dim, q, perm + c * dim, sign + c * dim, group + c * dim, ss, tt);
+      const int64_t at = (int64_t)blk_lev[b] * n_entries + offs[ranks[c]] + rel;
+      if (!cplx) {
+        v = table[at];
+      } else {
+        const bool tr = t < n, sr = s < n;
+        const double re = table[2 * at], im = table[2 * at + 1];
+        v = (tr == sr) ? re : (tr ? -im : im);
+      }
     }
     M[i] = v;
   }
@@ -125,7 +138,7 @@ __global__ void frag_f64_kernel(int MP, int KP, int n_blocks, const double* __re
 // Densities -> per (box, rhs) column: BF16 slices X[box][j][r][k] (mode
 // order, zero padded to KP), column scale 2^(e - 7) and, for the 2D log term,
 // nm[box][r] = sum_sid nmlz[sid] f[sbeg + sid].  One wavefront per column.
-__global__ void split_cols_kernel(int n, int KP, int nboxes, int R, int S,
+__global__ void split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, int S,
                                   const double* __restrict__ f, int64_t ldf,
                                   const int32_t* __restrict__ mode_point,
                                   const int32_t* __restrict__ box_starts,
@@ -137,9 +150,14 @@ __global__ void split_cols_kernel(int n, int KP, int nboxes, int R, int S,
   if (col >= nboxes * R) return;
   const int box = col / R, r = col % R;
   const double* fr = f + (int64_t)r * ldf;
+  const int rows = cplx ? 2 * n : n;
+  auto val = [&](int s) {  // component s of the column: [Re f; Im f] if complex
+    const int pt = mode_point[(int64_t)box * n + s % n];
+    return cplx ? fr[2 * pt + s / n] : fr[pt];
+  };
   double mx = 0.0, dot = 0.0;
-  for (int s = lane; s < n; s += 64) {
-    mx = fmax(mx, fabs(fr[mode_point[(int64_t)box * n + s]]));
+  for (int s = lane; s < rows; s += 64) {
+    mx = fmax(mx, fabs(val(s)));
     if (log_disp) dot += nmlz[s] * fr[box_starts[box] + s];
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -153,7 +171,7 @@ __global__ void split_cols_kernel(int n, int KP, int nboxes, int R, int S,
     nmdot[col] = dot;
   }
   for (int s = lane; s < KP; s += 64) {
-    double v = (s < n && mx > 0.0) ? ldexp(fr[mode_point[(int64_t)box * n + s]], -e) : 0.0;
+    double v = (s < rows && mx > 0.0) ? ldexp(val(s), -e) : 0.0;
     for (int j = 0; j < S; ++j) {
       const double d = rint(ldexp(v, 7 + 8 * j));
       v -= ldexp(d, -7 - 8 * j);
@@ -163,7 +181,7 @@ __global__ void split_cols_kernel(int n, int KP, int nboxes, int R, int S,
 }
 
 // Densities in mode order for the FP64 form: XF[box][r][k] (zero padded).
-__global__ void gather_cols_f64_kernel(int n, int KP, int nboxes, int R,
+__global__ void gather_cols_f64_kernel(int n, int cplx, int KP, int nboxes, int R,
                                        const double* __restrict__ f, int64_t ldf,
                                        const int32_t* __restrict__ mode_point,
                                        const int32_t* __restrict__ box_starts,
@@ -174,9 +192,15 @@ __global__ void gather_cols_f64_kernel(int n, int KP, int nboxes, int R,
   if (col >= nboxes * R) return;
   const int box = col / R, r = col % R;
   const double* fr = f + (int64_t)r * ldf;
+  const int rows = cplx ? 2 * n : n;
   double dot = 0.0;
   for (int s = lane; s < KP; s += 64) {
-    XF[((int64_t)box * R + r) * KP + s] = s < n ? fr[mode_point[(int64_t)box * n + s]] : 0.0;
+    double v = 0.0;
+    if (s < rows) {
+      const int pt = mode_point[(int64_t)box * n + s % n];
+      v = cplx ? fr[2 * pt + s / n] : fr[pt];
+    }
+    XF[((int64_t)box * R + r) * KP + s] = v;
     if (log_disp && s < n) dot += nmlz[s] * fr[box_starts[box] + s];
   }
   for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
@@ -184,7 +208,8 @@ __global__ void gather_cols_f64_kernel(int n, int KP, int nboxes, int R,
 }
 
 struct TileParams {
-  int n, MP, KP, S;
+  int n, MP, KP, S;      // n: block rows in use (2 x modes for complex data)
+  int nm, cplx;          // modes per box; complex data (interleaved re, im)
   int B, Rc, R;          // boxes per tile, RHS per tile, total RHS in the staged arrays
   int r0;                // first RHS of this launch
   int n_tiles;
@@ -207,6 +232,13 @@ struct TileParams {
   int64_t ldy;
   int32_t* phase;                  // [8] block each XCD is working on (hint)
 };
+
+// Output element of block row `row` of target box `tb`: the row's mode is
+// row % nm, and for complex data row / nm selects the real or imaginary part.
+__device__ inline int64_t out_index(const TileParams& p, int tb, int row) {
+  const int mp = p.mode_point[(int64_t)tb * p.nm + row % p.nm];
+  return p.cplx ? 2 * (int64_t)mp + row / p.nm : (int64_t)mp;
+}
 
 // Phase alignment of the step loop.  The steps of a tile may run in any
 // order (their sum is order independent), and every tile walks the same
@@ -504,7 +536,7 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
         const int b = c / p.Rc, r = c % p.Rc;
         const int tb = p.tile_boxes[tile * p.B + b];
         if (tb < 0) continue;
-        p.y[(int64_t)(p.r0 + r) * p.ldy + p.mode_point[(int64_t)tb * p.n + row]] = *yslot(rt16, ct, e);
+        p.y[(int64_t)(p.r0 + r) * p.ldy + out_index(p, tb, row)] = *yslot(rt16, ct, e);
       }
     }
   }
@@ -637,7 +669,7 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
         const int b = c / p.Rc, r = c % p.Rc;
         const int tb = p.tile_boxes[tile * p.B + b];
         if (tb < 0) continue;
-        p.y[(int64_t)(p.r0 + r) * p.ldy + p.mode_point[(int64_t)tb * p.n + row]] = yacc[rt][ct][e];
+        p.y[(int64_t)(p.r0 + r) * p.ldy + out_index(p, tb, row)] = yacc[rt][ct][e];
       }
     }
 }
@@ -656,6 +688,7 @@ struct TileSchedule {
 struct vl1_plan {
   const vl1_payload* payload = nullptr;
   int dim = 0, n = 0, MP = 0, KP = 0;
+  int cplx = 0, rows = 0;  // complex data; block rows in use (n, or 2n if complex)
   int nboxes = 0, n_tgt = 0;
   int64_t n_points = 0, n_pairs = 0;
   int log_disp = 0;
@@ -774,7 +807,6 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
   clear_error();
   if (!out || !pay || !d) return fail(VL1_E_ARG, "vl1_plan_create: null argument");
   *out = nullptr;
-  if (pay->is_complex) return fail(VL1_E_UNSUPPORTED, "vl1_plan_create: complex payload");
   if (d->dim != pay->dim) return fail(VL1_E_ARG, "vl1_plan_create: dimension mismatch");
   if (d->n_boxes < 1 || d->n_tgt_boxes < 0 || d->n_points < 1)
     return fail(VL1_E_ARG, "vl1_plan_create: bad sizes");
@@ -789,15 +821,22 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
   pl->payload = pay;
   pl->dim = dim;
   pl->n = n;
-  pl->MP = round_up(n, 32);
-  pl->KP = round_up(n, 32);
+  pl->cplx = pay->is_complex ? 1 : 0;
+  pl->rows = pl->cplx ? 2 * n : n;
+  pl->MP = round_up(pl->rows, 32);
+  pl->KP = round_up(pl->rows, 32);
   pl->nboxes = d->n_boxes;
   pl->n_tgt = d->n_tgt_boxes;
   pl->n_points = d->n_points;
   pl->log_disp = (d->scaling_mode == VL1_SCALING_CANONICAL && d->log_displacement) ? 1 : 0;
   if (pl->MP / 32 > 8) {
     delete pl;
-    return fail(VL1_E_UNSUPPORTED, "vl1_plan_create: more than 256 nodes per box");
+    return fail(VL1_E_UNSUPPORTED, pl->cplx ? "vl1_plan_create: more than 128 nodes per box (complex)"
+                                            : "vl1_plan_create: more than 256 nodes per box");
+  }
+  if (pl->cplx && pl->log_disp) {
+    delete pl;
+    return fail(VL1_E_UNSUPPORTED, "vl1_plan_create: the 2D log displacement applies to real tables only");
   }
 
   // layout: every listed box holds exactly n particles whose local ids are a
@@ -941,7 +980,7 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
     if (e == hipSuccess) {
       const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
       const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
-      hipLaunchKernelGGL(expand_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n, pl->MP,
+      hipLaunchKernelGGL(expand_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n, pl->cplx, pl->MP,
                          pl->KP, pl->n_blocks, dcase.as<int32_t>(), dlev.as<int32_t>(),
                          dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
                          pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(), pay->table.as<double>(),
@@ -995,7 +1034,8 @@ int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_r
                    const vl1_split_opts* opts, void* stream) {
   clear_error();
   if (!pl || !f) return fail(VL1_E_ARG, "vl1_plan_stage: null argument");
-  if (ldf < pl->n_points) return fail(VL1_E_ARG, "vl1_plan_stage: bad leading dimension");
+  if (ldf < pl->n_points * (pl->cplx ? 2 : 1))
+    return fail(VL1_E_ARG, "vl1_plan_stage: bad leading dimension");
   if (pl->n_tgt == 0 || pl->n_blocks == 0) return VL1_OK;
   hipStream_t s = (hipStream_t)stream;
   int S = 0;
@@ -1016,7 +1056,7 @@ int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_r
       e = pl->colscale.alloc((size_t)ncolumns * 8);
     if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
     hipLaunchKernelGGL(split_cols_kernel, dim3((ncolumns + wpb - 1) / wpb), dim3(64 * wpb), 0, s, pl->n,
-                       pl->KP, pl->nboxes, R, S, f, ldf, pl->mode_point.as<int32_t>(),
+                       pl->cplx, pl->KP, pl->nboxes, R, S, f, ldf, pl->mode_point.as<int32_t>(),
                        pl->box_starts.as<int32_t>(), nmlz, pl->log_disp, pl->X.as<__bf16>(),
                        pl->colscale.as<double>(), pl->nmdot.as<double>());
   } else {
@@ -1024,7 +1064,7 @@ int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_r
     if (e == hipSuccess && pl->XF.bytes < xb) e = pl->XF.alloc(xb);
     if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
     hipLaunchKernelGGL(gather_cols_f64_kernel, dim3((ncolumns + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
-                       pl->n, pl->KP, pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),
+                       pl->n, pl->cplx, pl->KP, pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),
                        pl->box_starts.as<int32_t>(), nmlz, pl->log_disp, pl->XF.as<double>(),
                        pl->nmdot.as<double>());
   }
@@ -1040,7 +1080,8 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
                       const vl1_split_opts* opts, void* stream) {
   clear_error();
   if (!pl || !y) return fail(VL1_E_ARG, "vl1_plan_contract: null argument");
-  if (ldy < pl->n_points) return fail(VL1_E_ARG, "vl1_plan_contract: bad leading dimension");
+  if (ldy < pl->n_points * (pl->cplx ? 2 : 1))
+    return fail(VL1_E_ARG, "vl1_plan_contract: bad leading dimension");
   if (pl->n_tgt == 0 || pl->n_blocks == 0) return VL1_OK;
   hipStream_t s = (hipStream_t)stream;
   int S = 0;
@@ -1053,7 +1094,9 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
   const int Rc = std::min(R, kCols);
   hipError_t e;
   TileParams prm{};
-  prm.n = pl->n;
+  prm.n = pl->rows;
+  prm.nm = pl->n;
+  prm.cplx = pl->cplx;
   prm.MP = pl->MP;
   prm.KP = pl->KP;
   prm.S = S;

@@ -32,3 +32,26 @@ def gaussian_mixture(points, seed, n_terms=8):
 
 def gaussian_mixture_batch(points, seeds, n_terms=8):
     return np.stack([gaussian_mixture(points, s, n_terms) for s in seeds])
+
+
+def helmholtz_beam_params(dim, seed, n_beams=16, box=0.8):
+    """Random-phase Gaussian beams: amplitudes a + ib with a, b ~ N(0, 1),
+    centres uniform in ``[-box, box]^d``, unit propagation directions."""
+    rng = np.random.default_rng(seed)
+    amps = rng.standard_normal(n_beams) + 1j * rng.standard_normal(n_beams)
+    centers = rng.uniform(-box, box, size=(n_beams, dim))
+    dirs = rng.standard_normal((n_beams, dim))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    return amps, centers, dirs
+
+
+def helmholtz_beams(points, seed, k=40.0, width=0.03, n_beams=16):
+    """``f(x) = sum_b A_b exp(-|x - c_b|^2 / w^2) exp(i k d_b . x)`` at
+    ``points[dim, n]`` (complex128)."""
+    dim = points.shape[0]
+    amps, c, d = helmholtz_beam_params(dim, seed, n_beams)
+    out = np.zeros(points.shape[1], dtype=np.complex128)
+    for b in range(n_beams):
+        r2 = np.sum((points - c[b][:, None]) ** 2, axis=0)
+        out += amps[b] * np.exp(-r2 / width**2) * np.exp(1j * k * (d[b] @ points))
+    return out

@@ -75,25 +75,35 @@ class List1Plan:
         self.handle = handle
         self.payload = payload  # keep alive
         self.n_points = int(k["ids"].size)
+        self.is_complex = bool(payload.is_complex)
 
     def eval(self, mode_coefs: torch.Tensor, out: torch.Tensor | None = None, form="split",
              n_slices=5, stream=None) -> torch.Tensor:
-        """Evaluate ``mode_coefs[R, n_points]`` (CUDA float64) into ``out``
-        (targets of the plan's boxes overwritten; allocated if None)."""
+        """Evaluate ``mode_coefs[R, n_points]`` into ``out`` (targets of the
+        plan's boxes overwritten; allocated if None).  float64 CUDA data for
+        real tables, complex128 for complex (Helmholtz) tables; the split
+        form then slices the real and imaginary parts alike."""
         f = mode_coefs
         if f.dim() == 1:
             f = f.unsqueeze(0)
-        if f.dtype != torch.float64 or f.device.type != "cuda" or f.stride(-1) != 1:
-            raise ValueError("mode_coefs must be a contiguous-rows float64 CUDA tensor")
+        want = torch.complex128 if self.is_complex else torch.float64
+        if f.dtype != want or f.device.type != "cuda" or f.stride(-1) != 1:
+            raise ValueError(f"mode_coefs must be a contiguous-rows {want} CUDA tensor")
         if f.shape[1] != self.n_points:
             raise ValueError("mode_coefs has the wrong number of points")
         if out is None:
             out = torch.zeros_like(f)
+        if out.dtype != want or out.stride(-1) != 1 or out.shape != f.shape:
+            raise ValueError("out must match mode_coefs in shape and dtype")
+        # complex data go through the C ABI as interleaved doubles
+        fr = torch.view_as_real(f) if self.is_complex else f
+        yr = torch.view_as_real(out) if self.is_complex else out
+        scale = 2 if self.is_complex else 1
         opts = _capi.SplitOpts(n_slices=int(n_slices), n_products=0)
         _capi.check(
             _capi.lib().vl1_plan_eval(
-                self.handle, FORMS[form], f.data_ptr(), int(f.stride(0)), int(f.shape[0]),
-                out.data_ptr(), int(out.stride(0)), opts, _capi.stream_handle(stream),
+                self.handle, FORMS[form], fr.data_ptr(), int(f.stride(0)) * scale, int(f.shape[0]),
+                yr.data_ptr(), int(out.stride(0)) * scale, opts, _capi.stream_handle(stream),
             ),
             "vl1_plan_eval",
         )

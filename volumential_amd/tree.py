@@ -217,6 +217,120 @@ def adaptive_tree(dim, quad_order, refine, min_level=1, max_level=6, a=-1.0, b=1
     )
 
 
+def adaptive_tree_batched(dim, quad_order, refine_batch, min_level=1, max_level=6, a=-1.0,
+                          b=1.0) -> BoxTree:
+    """Level-by-level refinement with a vectorised criterion
+    ``refine_batch(level, lowers[N, dim], h) -> bool[N]``; then 2:1-balance."""
+    root_vertex = np.full(dim, float(a))
+    extent = float(b - a)
+    done = []  # (level, index) leaves that stay
+    cur = np.zeros((1, dim), dtype=np.int64)
+    for lv in range(0, max_level + 1):
+        if cur.shape[0] == 0:
+            break
+        h = extent / 2**lv
+        if lv >= max_level:
+            ref = np.zeros(cur.shape[0], dtype=bool)
+        elif lv < min_level:
+            ref = np.ones(cur.shape[0], dtype=bool)
+        else:
+            ref = np.asarray(refine_batch(lv, root_vertex[None, :] + cur * h, h), dtype=bool)
+        done.extend((lv, tuple(map(int, ix))) for ix in cur[~ref])
+        kids = cur[ref]
+        ch = np.array(list(itertools.product((0, 1), repeat=dim)), dtype=np.int64)
+        cur = (2 * kids[:, None, :] + ch[None, :, :]).reshape(-1, dim)
+    leaves = _balance(dim, set(done))
+    items = sorted(leaves)
+    return BoxTree(
+        dim=dim,
+        quad_order=quad_order,
+        root_vertex=root_vertex,
+        root_extent=extent,
+        leaf_levels=np.array([lv for lv, _ in items], dtype=np.uint8),
+        leaf_index=np.array([ix for _, ix in items], dtype=np.int64).reshape(-1, dim),
+    )
+
+
+def interpolation_refiner(func, dim, quad_order, tol, relative_to=None):
+    """Batched criterion: refine a box while ``max |f - I_q f| > tol`` on the
+    ``(q+1)^d`` Gauss-Legendre check grid, ``I_q`` the tensor Lagrange
+    interpolant on the box's ``q^d`` nodes; ``func(points[dim, m])``.
+    With ``relative_to`` the tolerance is ``tol * relative_to``."""
+    x_q, _ = quadrature.gauss_legendre_01(quad_order)
+    x_c, _ = quadrature.gauss_legendre_01(quad_order + 1)
+    w = quadrature.barycentric_weights(x_q)
+    # P[c, j] = l_j(x_c): 1D interpolation matrix from q nodes to q+1 checks
+    diff = x_c[:, None] - x_q[None, :]
+    P = (w[None, :] / diff) / np.sum(w[None, :] / diff, axis=1, keepdims=True)
+    ref_q = quadrature.tensor_q_points(quad_order, dim, 1.0)
+    ref_c = quadrature.tensor_q_points(quad_order + 1, dim, 1.0)
+    thr = tol * (1.0 if relative_to is None else float(relative_to))
+
+    def refine_batch(level, lowers, h):
+        N = lowers.shape[0]
+        out = np.zeros(N, dtype=bool)
+        for b0 in range(0, N, 4096):
+            lw = lowers[b0:b0 + 4096]
+            pq = lw[:, None, :] + h * ref_q[None, :, :]
+            pc = lw[:, None, :] + h * ref_c[None, :, :]
+            fq = func(pq.reshape(-1, dim).T).reshape(lw.shape[0], *([quad_order] * dim))
+            fc = func(pc.reshape(-1, dim).T).reshape(lw.shape[0], *([quad_order + 1] * dim))
+            interp = fq
+            for ax in range(dim):
+                interp = np.moveaxis(np.tensordot(interp, P, axes=([1 + ax], [1])), -1, 1 + ax)
+            err = np.abs(fc - interp).reshape(lw.shape[0], -1).max(axis=1)
+            out[b0:b0 + 4096] = err > thr
+        return out
+
+    return refine_batch
+
+
+class Forest:
+    """Several trees on the same root box as one set of box arrays (box and
+    point ids offset tree by tree), so one plan -- one launch per eval --
+    covers all of them.  List 1 never crosses trees."""
+
+    sources_are_targets = True
+
+    def __init__(self, trees):
+        t0 = trees[0]
+        for t in trees:
+            if (t.dim, t.quad_order, t.root_extent) != (t0.dim, t0.quad_order, t0.root_extent) or \
+                    not np.array_equal(t.root_vertex, t0.root_vertex):
+                raise ValueError("forest trees must share dimension, order and root box")
+        self.trees = list(trees)
+        self.dim, self.quad_order = t0.dim, t0.quad_order
+        self.root_vertex, self.root_extent = t0.root_vertex, t0.root_extent
+        boff = np.cumsum([0] + [t.nboxes for t in trees])
+        poff = np.cumsum([0] + [t.n_points for t in trees])
+        self.tree_box_offsets, self.tree_point_offsets = boff, poff
+        cat = np.concatenate
+        self.box_centers = np.ascontiguousarray(cat([t.box_centers for t in trees], axis=1))
+        self.box_levels = cat([t.box_levels for t in trees])
+        self.box_target_starts = cat([t.box_target_starts.astype(np.int64) + poff[i]
+                                      for i, t in enumerate(trees)]).astype(np.int32)
+        self.box_target_counts_nonchild = cat([t.box_target_counts_nonchild for t in trees])
+        self.target_boxes = cat([t.target_boxes.astype(np.int64) + boff[i]
+                                 for i, t in enumerate(trees)]).astype(np.int32)
+        starts, lists, acc = [np.zeros(1, np.int64)], [], 0
+        for i, t in enumerate(trees):
+            starts.append(t.neighbor_source_boxes_starts[1:].astype(np.int64) + acc)
+            lists.append(t.neighbor_source_boxes_lists.astype(np.int64) + boff[i])
+            acc += int(t.neighbor_source_boxes_starts[-1])
+        self.neighbor_source_boxes_starts = cat(starts).astype(np.int32)
+        self.neighbor_source_boxes_lists = cat(lists).astype(np.int32)
+        self.points = np.ascontiguousarray(cat([t.points for t in trees], axis=1))
+        self.weights = cat([t.weights for t in trees])
+        self.local_ids = cat([t.local_ids for t in trees])
+
+    n_q_points = property(lambda self: self.quad_order**self.dim)
+    nboxes = property(lambda self: int(self.box_levels.size))
+    n_points = property(lambda self: int(self.local_ids.size))
+    nlevels = property(lambda self: int(self.box_levels.max()) + 1)
+    n_list1_pairs = property(lambda self: int(self.neighbor_source_boxes_starts[-1]))
+    ntargets = n_points
+
+
 def _balance(dim, leaves):
     leaves = set(leaves)
     offs = [o for o in itertools.product((-1, 0, 1), repeat=dim) if any(o)]

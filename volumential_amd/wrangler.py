@@ -124,11 +124,16 @@ class NearFieldWrangler:
     def table_starting_level(self, tables):
         return int(round(np.log2(self.tree.root_extent / tables[0].source_box_extent)))
 
+    @staticmethod
+    def _dtype(kernel):
+        """complex128 for complex kernels (``expansion_wrangler_fpnd.py:3283-3297``)."""
+        return torch.complex128 if getattr(kernel, "is_complex_valued", False) else torch.float64
+
     def output_zeros(self, n_rhs=None):
         shape = (self.tree.n_points,) if n_rhs is None else (n_rhs, self.tree.n_points)
         pot = np.empty(len(self.target_kernels), dtype=object)
-        for i in range(len(self.target_kernels)):
-            pot[i] = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        for i, knl in enumerate(self.target_kernels):
+            pot[i] = torch.zeros(shape, dtype=self._dtype(knl), device="cuda")
         return pot
 
     # -------------------------------------------------------------- eval
@@ -145,6 +150,18 @@ class NearFieldWrangler:
                     "Insufficient list of tables: the coarsest level mesh cells at level "
                     f"{min_lev} are not covered."
                 )
+        if getattr(out_kernel, "is_complex_valued", False):
+            # complex tables evaluate through a plan in FP64 (the real 2n x 2n
+            # embedding of each complex block), cached per CSR
+            mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
+                np.asarray(mode_coefs))
+            mc = mc.to(device="cuda", dtype=torch.complex128)
+            one = mc.dim() == 1
+            mc = (mc.unsqueeze(0) if one else mc).contiguous()
+            out = out_pot.unsqueeze(0) if one else out_pot
+            plan = self.get_plan(out_kernel, target_boxes, starts, lists)
+            plan.eval(mc, out=out, form="f64")
+            return out_pot
         data, nmlz, _, _, _, info = prepare_table_data_and_entry_map(tables)
         shapes = {
             "n_tables": len(tables),
@@ -266,14 +283,13 @@ class NearFieldWrangler:
                          neighbor_source_boxes_lists, mode_coefs, form="split", n_slices=5):
         """Batched evaluation of ``mode_coefs[R, n_points]`` with a plan form
         ("f64" or "split"); returns ``(pot, timing_future)`` like eval_direct."""
-        mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
+        mc0 = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
             np.asarray(mode_coefs))
-        mc = mc.to(device="cuda", dtype=torch.float64)
-        if mc.dim() == 1:
-            mc = mc.unsqueeze(0)
-        mc = mc.contiguous()
-        pot = self.output_zeros(int(mc.shape[0]))
+        if mc0.dim() == 1:
+            mc0 = mc0.unsqueeze(0)
+        pot = self.output_zeros(int(mc0.shape[0]))
         for i, knl in enumerate(self.target_kernels):
+            mc = mc0.to(device="cuda", dtype=self._dtype(knl)).contiguous()
             plan = self.get_plan(knl, target_boxes, neighbor_source_boxes_starts,
                                  neighbor_source_boxes_lists)
             plan.eval(mc, out=pot[i], form=form, n_slices=n_slices)

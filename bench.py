@@ -53,6 +53,11 @@ def parse():
                     help="do not time the plain FP64 kernel (profiling runs)")
     ap.add_argument("--no-table-build", action="store_true",
                     help="skip the near-field table build record (3D Laplace q=6, all cases)")
+    ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c2"],
+                    help="c3 (default): 3D Laplace uniform level 5 q=6, 64 RHS; "
+                         "c4: 2D Helmholtz k=40, 8 adaptive trees depth 4..10, q=6, 16 complex "
+                         "RHS in one plan; c2: 3D Laplace adaptive (interpolation error 1e-8, "
+                         "depth 3..7), q=5, 1 RHS")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (byte-counter calibration)")
     return ap.parse_args()
@@ -205,6 +210,8 @@ def cpu_baseline(tree, table, f_host_rows, seconds):
     nm = table.mode_normalizers[None]
     counts = np.diff(tree.neighbor_source_boxes_starts)
     full = np.flatnonzero(counts == counts.max())
+    if full.size < 1024:  # adaptive trees: a seeded sample of all target boxes
+        full = np.random.default_rng(0).permutation(counts.size)
     cores = min(16, os.cpu_count() or 1)
 
     def run(nb):
@@ -239,9 +246,225 @@ def cpu_baseline(tree, table, f_host_rows, seconds):
         "unit": "Gpts*RHS/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{nb} interior target boxes x {rhs} RHS of the same tree/table "
+        "sample": f"{nb} target boxes x {rhs} RHS of the same tree/table "
                   f"({nb * tree.n_q_points * rhs} target-point evaluations, {t:.1f} s)",
     }
+
+
+def _build_beam_tree(seed):
+    from volumential_amd import densities
+    from volumential_amd import tree as vtree
+
+    f = (lambda P: densities.helmholtz_beams(P, seed, k=40.0))
+    return vtree.adaptive_tree_batched(2, 6, vtree.interpolation_refiner(f, 2, 6, 1e-10, 4.0),
+                                       min_level=4, max_level=10)
+
+
+def _plan_forms(plan, f, R, n_pts, flops, steps, warmup, slices, complex_data):
+    """Time stage + contract of the FP64 and split plan forms; per-RHS error
+    of the split form against the FP64 form."""
+    import torch
+
+    from volumential_amd import _capi
+    from volumential_amd.plan import FORMS
+
+    L = _capi.lib()
+    stream = torch.cuda.current_stream()
+    sh = int(stream.cuda_stream)
+    fr = torch.view_as_real(f).reshape(R, -1) if complex_data else f
+    ldf = fr.shape[1]  # rows are contiguous (a one-row tensor may report stride 0)
+    res = {}
+    ys = {}
+    for form, S in (("f64", 0), ("split", slices)):
+        y = torch.zeros_like(f)
+        yr = torch.view_as_real(y).reshape(R, -1) if complex_data else y
+        opts = _capi.SplitOpts(n_slices=S, n_products=0)
+        kern = []
+
+        def step():
+            _capi.check(L.vl1_plan_stage(plan.handle, FORMS[form], fr.data_ptr(), ldf, R, opts, sh))
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            _capi.check(L.vl1_plan_contract(plan.handle, FORMS[form], R, yr.data_ptr(), ldf, opts, sh))
+            b.record(stream)
+            kern.append((a, b))
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        kern.clear()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        kms = sum(a.elapsed_time(b) for a, b in kern) / len(kern)
+        res[form] = {"ms": ms, "kernel_ms": kms, "value": n_pts * R / (ms * 1e-3) / 1e9,
+                     "tflops": flops / (kms * 1e-3) / 1e12}
+        ys[form] = y
+        log(f"{form}: {ms:.3f} ms/step (contract {kms:.3f} ms), "
+            f"{res[form]['value']:.3f} Gpts*RHS/s, {res[form]['tflops']:.2f} TFLOP/s")
+    d = ys["split"] - ys["f64"]
+    l2 = (d.abs().pow(2).sum(dim=1).sqrt() / ys["f64"].abs().pow(2).sum(dim=1).sqrt()).cpu().numpy()
+    li = (d.abs().amax(dim=1) / ys["f64"].abs().amax(dim=1)).cpu().numpy()
+    res["split"].update({"slices": slices, "rel_l2_max": float(l2.max()),
+                         "rel_linf_max": float(li.max())})
+    log(f"split vs f64: rel L2 max {l2.max():.2e}, Linf max {li.max():.2e}")
+    return res
+
+
+def _line(metric_value, ms, steps, warmup, dtype, data, config, roofline, forms, extra=None):
+    out = {
+        "metric": "list-1 eval Gpts*RHS/s at <=1e-10 rel err; split-MFMA vs FP64 GPU kernel",
+        "value": metric_value, "unit": "Gpts*RHS/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype, "data": data, "config": config,
+        "roofline": roofline, "forms": forms,
+    }
+    out.update(extra or {})
+    return out
+
+
+def run_c4(args):
+    """2D Helmholtz k=40: eight adaptive quadtrees (seeds 0..7, depth 4..10,
+    refined until the q=6 interpolant of each tree's beam density is within
+    1e-10 x 4), per-level complex tables for levels 4..10 built on the GPU,
+    16 complex beam densities (seeds 100..115) on every tree, one plan over
+    the forest (one launch per eval)."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=8) as ex:  # before any GPU call
+        trees = list(ex.map(_build_beam_tree, range(8)))
+    log(f"8 trees built in {time.perf_counter() - t0:.1f} s")
+    import torch
+
+    from volumential_amd import densities, tree as vtree
+    from volumential_amd.kernels import HelmholtzKernel
+    from volumential_amd.payload import DevicePayload, prepare_table_data_and_entry_map
+    from volumential_amd.plan import List1Plan
+    from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(2)
+    fo = vtree.Forest(trees)
+    lv = np.bincount(fo.box_levels)
+    log(f"forest: {fo.nboxes} boxes, {fo.n_points} points, {fo.n_list1_pairs} pairs, levels {lv.tolist()}")
+    tables = []
+    t0 = time.perf_counter()
+    for lev in range(4, fo.nlevels):
+        t = NearFieldInteractionTable(6, 2, HelmholtzKernel(2, 40.0), source_box_extent=2.0 / 2**lev)
+        t.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=41))
+        tables.append(t)
+    log(f"{len(tables)} complex tables built on the GPU in {time.perf_counter() - t0:.2f} s")
+    data, nmlz, _, _, _, _ = prepare_table_data_and_entry_map(tables)
+    t0_ = tables[0]
+    payload = DevicePayload(2, 6, data, nmlz, t0_.case_indices, t0_.arithmetic_metadata())
+    plan = List1Plan(fo, payload, scaling_mode=2, scaling_power=0, table_root_extent=t0_.source_box_extent,
+                     table_starting_level=4)
+    R = 16
+    f = torch.as_tensor(np.stack([densities.helmholtz_beams(fo.points, 100 + r) for r in range(R)]),
+                        device="cuda").contiguous()
+    n = 36
+    flops = 8.0 * R * fo.n_list1_pairs * n * n
+    forms = _plan_forms(plan, f, R, fo.n_points, flops, args.steps, args.warmup, args.slices, True)
+    sp = forms["split"]
+    nprod = args.slices * (args.slices + 1) // 2
+    achieved = nprod * flops / (sp["kernel_ms"] * 1e-3) / 1e12
+    return _line(
+        sp["value"], sp["ms"], args.steps, args.warmup, "bf16 slices (MFMA) + f64 sums, complex128 I/O",
+        "synthetic (seeded Gaussian beams, GPU-built per-level Helmholtz tables)",
+        {"workload": "2D Helmholtz k=40, 8 adaptive quadtrees depth 4..10, q=6, 16 complex RHS",
+         "boxes": fo.nboxes, "target_points": fo.n_points, "rhs": R, "list1_pairs": fo.n_list1_pairs,
+         "levels": lv.tolist(), "form": f"split_bf16 S={args.slices}", "parallelism": "single GPU"},
+        {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+         "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+         "kernel": f"split_bf16_kernel<{args.slices}>", "kernel_ms": sp["kernel_ms"],
+         "flops_per_launch": nprod * flops},
+        forms,
+        {"error_vs_fp64": {"rel_l2_per_rhs_max": sp["rel_l2_max"],
+                           "rel_linf_per_rhs_max": sp["rel_linf_max"]},
+         "speedup_vs_dense_f64_mfma": forms["f64"]["ms"] / sp["ms"]})
+
+
+def run_c2(args):
+    """3D Laplace, adaptive: refine while max|f - I_5 f| > 1e-8 on the 6^3
+    check grid (f = the seed-0 8-Gaussian sum), depth 3..7, 2:1 balanced;
+    q = 5; one Laplace table (H = 2, canonical scaling); one RHS."""
+    from volumential_amd import densities, tree as vtree
+
+    t0 = time.perf_counter()
+    fden = (lambda P: densities.gaussian_mixture(P, 0))
+    tr = vtree.adaptive_tree_batched(3, 5, vtree.interpolation_refiner(fden, 3, 5, 1e-8),
+                                     min_level=3, max_level=7)
+    log(f"tree: {tr.nboxes} leaves, {tr.n_points} points, {tr.n_list1_pairs} pairs, "
+        f"levels {np.bincount(tr.box_levels).tolist()} ({time.perf_counter() - t0:.1f} s)")
+    import torch
+
+    from volumential_amd import _capi
+    from volumential_amd.kernels import LaplaceKernel
+    from volumential_amd.payload import DevicePayload, prepare_table_data_and_entry_map
+    from volumential_amd.plan import List1Plan
+    from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(2)
+    table = NearFieldInteractionTable(5, 3, LaplaceKernel(3), source_box_extent=2.0)
+    table.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=20))
+    f = torch.as_tensor(densities.gaussian_mixture(tr.points, 0)[None, :], device="cuda").contiguous()
+    y = torch.zeros_like(f)
+    R = 1
+    n = 125
+    flops = 2.0 * R * tr.n_list1_pairs * n * n
+    stream = torch.cuda.current_stream()
+    sh = int(stream.cuda_stream)
+    runner = F64Runner(tr, table, f, y)
+    for _ in range(args.warmup):
+        runner(sh)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        runner(sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_plain = e0.elapsed_time(e1) / args.steps
+    log(f"plain f64: {ms_plain:.3f} ms/step, {flops / (ms_plain * 1e-3) / 1e12:.2f} TFLOP/s")
+    data, nmlz, _, _, _, _ = prepare_table_data_and_entry_map([table])
+    payload = DevicePayload(3, 5, data, nmlz, table.case_indices, table.arithmetic_metadata())
+    plan = List1Plan(tr, payload, scaling_mode=_capi.SCALING_CANONICAL, scaling_power=2,
+                     table_root_extent=2.0)
+    forms = _plan_forms(plan, f, R, tr.n_points, flops, args.steps, args.warmup, args.slices, False)
+    forms["plain_f64"] = {"ms": ms_plain, "value": tr.n_points * R / (ms_plain * 1e-3) / 1e9,
+                          "tflops": flops / (ms_plain * 1e-3) / 1e12}
+    sp = forms["split"]
+    nprod = args.slices * (args.slices + 1) // 2
+    achieved = nprod * flops / (sp["kernel_ms"] * 1e-3) / 1e12
+    extra = {"error_vs_fp64": {"rel_l2_per_rhs_max": sp["rel_l2_max"],
+                               "rel_linf_per_rhs_max": sp["rel_linf_max"]},
+             "speedup_vs_plain_f64": ms_plain / sp["ms"],
+             "speedup_vs_dense_f64_mfma": forms["f64"]["ms"] / sp["ms"]}
+    if not args.no_cpu_baseline:
+        extra["cpu_baseline"] = cpu_baseline(tr, table, f.cpu().numpy(), args.cpu_seconds)
+    return _line(
+        sp["value"], sp["ms"], args.steps, args.warmup, "bf16 slices (MFMA) + f64 sums",
+        "synthetic (seed-0 8-Gaussian density, GPU-built Laplace table)",
+        {"workload": "3D Laplace adaptive (interp. error 1e-8, depth 3..7), q=5, 1 RHS",
+         "boxes": tr.nboxes, "target_points": tr.n_points, "rhs": R,
+         "list1_pairs": tr.n_list1_pairs, "form": f"split_bf16 S={args.slices}",
+         "parallelism": "single GPU"},
+        {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+         "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+         "kernel": f"split_bf16_kernel<{args.slices}>", "kernel_ms": sp["kernel_ms"],
+         "flops_per_launch": nprod * flops},
+        forms, extra)
 
 
 def main():
@@ -249,6 +472,12 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "c4":
+        print(json.dumps(run_c4(args)), flush=True)
+        return
+    if args.workload == "c2":
+        print(json.dumps(run_c2(args)), flush=True)
+        return
     import torch
 
     from volumential_amd import _capi, tree as vtree

@@ -324,7 +324,7 @@ inline size_t split_lds_bytes(int S, int MP) {
   return (size_t)2 * S * kCols * 32 * 2 + (size_t)(MP / 16) * 1024 * 8 + 6 * kCols * 8;
 }
 
-template <int S>
+template <int S, int NRT>
 __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int RING = S * kCols * 32;  // bf16 per ring slot
@@ -336,10 +336,15 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
 
   const int tile = xcd_tile(blockIdx.x, p.n_tiles);
   if (tile >= p.n_tiles) return;  // uniform over the workgroup
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  // NRT = 2: wave w owns row tiles w and w + 8 (blocks of up to 256 rows);
+  // NRT = 1: wave w owns row tile w (up to 128 rows), and a wave without
+  // one skips the MFMA block as a whole (wave-uniform branch), so small
+  // blocks issue no MFMAs on empty row-tile slots.
+  const int wave = NRT == 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x) / 64 : threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
   const int l16 = lane % 16, lg = lane / 16;
   const int ncols = p.B * p.Rc;
-  const int nrt = (p.MP / 16 - wave + 7) / 8;  // row tiles of this wave (<= 2)
+  const int nrt = NRT == 2 ? (p.MP / 16 - wave + 7) / 8 : (wave < p.MP / 16 ? 1 : 0);
   const int nkc = p.KP / 32;
   const int xcd = blockIdx.x % 8;
   const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
@@ -396,17 +401,17 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
   auto yslot = [&](int rt16, int ct, int e) { return sY + ((rt16 * 4 + ct) * 4 + e) * 64 + lane; };
 
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int rt = 0; rt < NRT; ++rt)
     if (rt < nrt)
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
         for (int e = 0; e < 4; ++e) *yslot(wave + 8 * rt, ct, e) = 0.0;
-  floatx4 acc[3][2][4];
+  floatx4 acc[3][NRT][4];
 #pragma unroll
   for (int g = 0; g < 3; ++g)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NRT; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) acc[g][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
 
@@ -423,12 +428,12 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
     const int rt16 = rt < nrt ? wave + 8 * rt : 0;
     return p.A + ((int64_t)blk * (p.MP / 16) + rt16) * frag_tile + lane * 8;
   };
-  bf16x8 af[S][2];
+  bf16x8 af[S][NRT];
   if (nst > 0) {
     const int blk = p.step_block[step_of(0)];
     if (threadIdx.x == 0) __builtin_nontemporal_store(blk, p.phase + xcd);
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt) {
+    for (int rt = 0; rt < NRT; ++rt) {
       const __bf16* ar = arow(blk, rt);
 #pragma unroll
       for (int i = 0; i < S; ++i)
@@ -457,7 +462,7 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
 
     // MFMAs of this chunk: density slice j from LDS, table slices from regs
     const __bf16* rX = sX + ring * RING;
-    static_for<0, S>([&](auto jc) {
+    if (NRT == 2 || nrt > 0) static_for<0, S>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
@@ -468,7 +473,7 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
           constexpr int i = S - j - 1 - decltype(ic)::value;
           constexpr int g = (i + j) < 2 ? (i + j) : 2;
 #pragma unroll
-          for (int rt = 0; rt < 2; ++rt)
+          for (int rt = 0; rt < NRT; ++rt)
             if (rt < nrt)
               acc[g][rt][ct] =
                   __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][rt], xf, acc[g][rt][ct], 0, 0, 0);
@@ -478,7 +483,7 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
       if (has_next) {
         constexpr int i = S - 1 - j;
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int rt = 0; rt < NRT; ++rt)
           if (rt < nrt)
             af[i][rt] = *reinterpret_cast<const bf16x8*>(arow(nblk, rt) + ((int64_t)nkc_ * S + i) * 512);
       }
@@ -488,7 +493,7 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
       // step done: add the three sets, scaled, to the FP64 block
       const double* rs = p.rowscale + (int64_t)blk * p.MP;
 #pragma unroll
-      for (int rt = 0; rt < 2; ++rt) {
+      for (int rt = 0; rt < NRT; ++rt) {
         if (rt >= nrt) continue;
         const int rt16 = wave + 8 * rt;
 #pragma unroll
@@ -508,7 +513,7 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
 #pragma unroll
       for (int g = 0; g < 3; ++g)
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < NRT; ++a)
 #pragma unroll
           for (int b = 0; b < 4; ++b) acc[g][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
       if (has_next && threadIdx.x == 0) __builtin_nontemporal_store(nblk, p.phase + xcd);
@@ -522,7 +527,7 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
   }
   // epilogue: write targets of the tile's boxes
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt) {
+  for (int rt = 0; rt < NRT; ++rt) {
     if (rt >= nrt) continue;
     const int rt16 = wave + 8 * rt;
 #pragma unroll
@@ -685,6 +690,33 @@ struct TileSchedule {
   int64_t n_steps = 0;
 };
 
+template <int S, int NRT>
+static hipError_t launch_split_t(dim3 grid, size_t lds, hipStream_t s, const TileParams& prm) {
+  hipError_t e = hipFuncSetAttribute((const void*)split_bf16_kernel<S, NRT>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((split_bf16_kernel<S, NRT>), grid, dim3(512), lds, s, prm);
+  return hipGetLastError();
+}
+
+static hipError_t launch_split(int S, bool one, dim3 grid, size_t lds, hipStream_t s,
+                               const TileParams& prm) {
+  switch (S * 2 + (one ? 1 : 0)) {
+    case 2: return launch_split_t<1, 2>(grid, lds, s, prm);
+    case 3: return launch_split_t<1, 1>(grid, lds, s, prm);
+    case 4: return launch_split_t<2, 2>(grid, lds, s, prm);
+    case 5: return launch_split_t<2, 1>(grid, lds, s, prm);
+    case 6: return launch_split_t<3, 2>(grid, lds, s, prm);
+    case 7: return launch_split_t<3, 1>(grid, lds, s, prm);
+    case 8: return launch_split_t<4, 2>(grid, lds, s, prm);
+    case 9: return launch_split_t<4, 1>(grid, lds, s, prm);
+    case 10: return launch_split_t<5, 2>(grid, lds, s, prm);
+    case 11: return launch_split_t<5, 1>(grid, lds, s, prm);
+    case 12: return launch_split_t<6, 2>(grid, lds, s, prm);
+    default: return launch_split_t<6, 1>(grid, lds, s, prm);
+  }
+}
+
 struct vl1_plan {
   const vl1_payload* payload = nullptr;
   int dim = 0, n = 0, MP = 0, KP = 0;
@@ -722,12 +754,29 @@ static int build_schedule(vl1_plan* pl, int Rc, TileSchedule** out, hipStream_t 
   std::vector<std::vector<std::pair<int, int>>> per_tgt(pl->n_tgt);
   for (int64_t k = 0; k < pl->n_pairs; ++k) per_tgt[pl->pair_tgt[k]].push_back({pl->pair_block[k], (int)k});
   for (auto& v : per_tgt) std::sort(v.begin(), v.end());
-  // tiles follow the Morton order of the target boxes: concurrently running
-  // tiles then share most source boxes (L2 reuse of their density slices)
+  // Tiles follow the Morton order of the target boxes: concurrently running
+  // tiles then share most source boxes (L2 reuse of their density slices).
+  // With several boxes per tile, boxes are first grouped by their block
+  // signature (the sorted list of their (case, level) blocks), so the boxes
+  // of a tile need the same blocks and every step fills all their columns
+  // (on adaptive trees Morton neighbours often differ in level or case).
+  std::vector<uint64_t> sig(pl->n_tgt, 0);
+  if (B > 1) {
+    for (int i = 0; i < pl->n_tgt; ++i) {
+      uint64_t h = 1469598103934665603ull;  // FNV-1a over the block ids
+      for (auto& pr : per_tgt[i]) {
+        h ^= (uint64_t)(uint32_t)pr.first;
+        h *= 1099511628211ull;
+      }
+      sig[i] = h;
+    }
+  }
   std::vector<int> order(pl->n_tgt);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return pl->tgt_morton[a] < pl->tgt_morton[b]; });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (sig[a] != sig[b]) return sig[a] < sig[b];
+    return pl->tgt_morton[a] < pl->tgt_morton[b];
+  });
   auto sch = new TileSchedule();
   sch->B = B;
   sch->Rc = Rc;
@@ -1128,25 +1177,10 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
     prm.r0 = r0;
     if (form == VL1_FORM_SPLIT_BF16) {
       const size_t lds = split_lds_bytes(S, pl->MP);
-      const void* fn = nullptr;
-      switch (S) {
-        case 1: fn = (const void*)split_bf16_kernel<1>; break;
-        case 2: fn = (const void*)split_bf16_kernel<2>; break;
-        case 3: fn = (const void*)split_bf16_kernel<3>; break;
-        case 4: fn = (const void*)split_bf16_kernel<4>; break;
-        case 5: fn = (const void*)split_bf16_kernel<5>; break;
-        default: fn = (const void*)split_bf16_kernel<6>; break;
-      }
-      e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split attr: ") + hipGetErrorString(e));
-      switch (S) {
-        case 1: hipLaunchKernelGGL(split_bf16_kernel<1>, dim3(grid), dim3(512), lds, s, prm); break;
-        case 2: hipLaunchKernelGGL(split_bf16_kernel<2>, dim3(grid), dim3(512), lds, s, prm); break;
-        case 3: hipLaunchKernelGGL(split_bf16_kernel<3>, dim3(grid), dim3(512), lds, s, prm); break;
-        case 4: hipLaunchKernelGGL(split_bf16_kernel<4>, dim3(grid), dim3(512), lds, s, prm); break;
-        case 5: hipLaunchKernelGGL(split_bf16_kernel<5>, dim3(grid), dim3(512), lds, s, prm); break;
-        default: hipLaunchKernelGGL(split_bf16_kernel<6>, dim3(grid), dim3(512), lds, s, prm); break;
-      }
+      // one row tile per wave when the block has at most 8 of them
+      const bool one = pl->MP <= 128;
+      e = launch_split(S, one, dim3(grid), lds, s, prm);
+      if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split launch: ") + hipGetErrorString(e));
     } else {
       const size_t lds = ((size_t)kCols * (pl->KP + 2) + 2 * kCols) * 8;
       e = hipFuncSetAttribute((const void*)dense_f64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -95,6 +95,9 @@ class List1Plan:
             out = torch.zeros_like(f)
         if out.dtype != want or out.stride(-1) != 1 or out.shape != f.shape:
             raise ValueError("out must match mode_coefs in shape and dtype")
+        # a one-row tensor may report any stride for its first dimension
+        ldf_el = f.stride(0) if f.shape[0] > 1 else f.shape[1]
+        ldy_el = out.stride(0) if out.shape[0] > 1 else out.shape[1]
         # complex data go through the C ABI as interleaved doubles
         fr = torch.view_as_real(f) if self.is_complex else f
         yr = torch.view_as_real(out) if self.is_complex else out
@@ -102,8 +105,8 @@ class List1Plan:
         opts = _capi.SplitOpts(n_slices=int(n_slices), n_products=0)
         _capi.check(
             _capi.lib().vl1_plan_eval(
-                self.handle, FORMS[form], fr.data_ptr(), int(f.stride(0)) * scale, int(f.shape[0]),
-                yr.data_ptr(), int(out.stride(0)) * scale, opts, _capi.stream_handle(stream),
+                self.handle, FORMS[form], fr.data_ptr(), int(ldf_el) * scale, int(f.shape[0]),
+                yr.data_ptr(), int(ldy_el) * scale, opts, _capi.stream_handle(stream),
             ),
             "vl1_plan_eval",
         )

@@ -1,0 +1,67 @@
+"""HIP-graph capture of a plan evaluation (List1Plan.capture).
+
+The captured graph (density staging + tile kernel, no allocation, no
+memset) is replayed several times with new densities written between the
+replays by ordinary eager work; every replay must match an eager evaluation
+of the same densities to 1e-13 relative (same kernels and inputs; the
+order of a tile's steps follows a racy L2 hint, so FP64 sums may differ in
+the last bits), for both forms and for real and complex data.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from tests.gpu_helpers import beam_forest, helmholtz_tables, synthetic_table_levels
+from volumential_amd import densities, tree as vtree
+from volumential_amd.wrangler import NearFieldWrangler
+
+pytestmark = pytest.mark.gpu
+
+
+def _replays(plan, make_f, dtype, form):
+    f = make_f(0)
+    out = torch.zeros_like(f)
+    graph = plan.capture(f, out, form=form, n_slices=5)
+    for it in range(1, 4):
+        f.copy_(make_f(it))          # eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        ref = plan.eval(f.clone(), form=form, n_slices=5)
+        torch.cuda.synchronize()
+        err = (out - ref).abs().max() / ref.abs().max()
+        assert err <= 1e-13, f"replay {it} differs from the eager evaluation ({err})"
+        assert out.abs().max() > 0
+
+
+@pytest.mark.parametrize("form", ["f64", "split"])
+def test_graph_replay_real(cuda, form):
+    print(f"case: graph replay real 3D q=6 level 2 form={form}", flush=True)
+    tr = vtree.uniform_tree(3, 6, 2)
+    tables = synthetic_table_levels(3, 6, seed=3)
+    w = NearFieldWrangler(tr, tables[0])
+    plan = w.get_plan(w.target_kernels[0], tr.target_boxes, tr.neighbor_source_boxes_starts,
+                      tr.neighbor_source_boxes_lists)
+
+    def make_f(it):
+        return torch.as_tensor(densities.gaussian_mixture_batch(tr.points, range(10 * it, 10 * it + 8)),
+                               device="cuda").contiguous()
+
+    _replays(plan, make_f, torch.float64, form)
+
+
+@pytest.mark.parametrize("form", ["f64", "split"])
+def test_graph_replay_complex(cuda, form):
+    print(f"case: graph replay complex 2D forest form={form}", flush=True)
+    fo = beam_forest([3], q=6, tol=1e-4, min_level=4, max_level=5)
+    tables = helmholtz_tables(2, 6, 40.0, 4, fo.nlevels - 4)
+    w = NearFieldWrangler(fo, tables)
+    plan = w.get_plan(w.target_kernels[0], fo.target_boxes, fo.neighbor_source_boxes_starts,
+                      fo.neighbor_source_boxes_lists)
+
+    def make_f(it):
+        return torch.as_tensor(np.stack([densities.helmholtz_beams(fo.points, 50 + 4 * it + r)
+                                         for r in range(4)]), device="cuda").contiguous()
+
+    _replays(plan, make_f, torch.complex128, form)

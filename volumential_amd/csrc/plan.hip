@@ -22,6 +22,7 @@
 // in FP64, scaled by the row/column exponents and the pair scaling, and
 // added to the FP64 output ("cross-box sums in FP64").
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -690,10 +691,24 @@ struct TileSchedule {
   int64_t n_steps = 0;
 };
 
+// The dynamic-LDS limit of a tile kernel is raised to the full 160 KiB once
+// per device (bit d of `done`), so later launches -- including ones recorded
+// into a HIP graph -- make no attribute calls.
+static hipError_t allow_full_lds(const void* fn, std::atomic<unsigned>& done) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const unsigned bit = 1u << (dev & 31);
+  if (done.load(std::memory_order_acquire) & bit) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e == hipSuccess) done.fetch_or(bit, std::memory_order_acq_rel);
+  return e;
+}
+
 template <int S, int NRT>
 static hipError_t launch_split_t(dim3 grid, size_t lds, hipStream_t s, const TileParams& prm) {
-  hipError_t e = hipFuncSetAttribute((const void*)split_bf16_kernel<S, NRT>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  static std::atomic<unsigned> attr_done{0};
+  hipError_t e = allow_full_lds((const void*)split_bf16_kernel<S, NRT>, attr_done);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((split_bf16_kernel<S, NRT>), grid, dim3(512), lds, s, prm);
   return hipGetLastError();
@@ -1183,8 +1198,8 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
       if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split launch: ") + hipGetErrorString(e));
     } else {
       const size_t lds = ((size_t)kCols * (pl->KP + 2) + 2 * kCols) * 8;
-      e = hipFuncSetAttribute((const void*)dense_f64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
+      static std::atomic<unsigned> dense_attr_done{0};
+      e = allow_full_lds((const void*)dense_f64_kernel, dense_attr_done);
       if (e != hipSuccess) return fail(VL1_E_HIP, std::string("dense attr: ") + hipGetErrorString(e));
       hipLaunchKernelGGL(dense_f64_kernel, dim3(grid), dim3(64 * waves), lds, s, prm);
     }

@@ -112,6 +112,19 @@ class List1Plan:
         )
         return out
 
+    def capture(self, mode_coefs: torch.Tensor, out: torch.Tensor, form="split", n_slices=5):
+        """Record one evaluation (density staging + tile contraction) of
+        ``mode_coefs`` into ``out`` as a HIP graph and return it; each
+        ``graph.replay()`` re-evaluates whatever ``mode_coefs`` then holds.
+        One eager evaluation runs first: it allocates the staging buffers and
+        uploads the tile schedule, so the captured launches allocate nothing."""
+        self.eval(mode_coefs, out=out, form=form, n_slices=n_slices)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            self.eval(mode_coefs, out=out, form=form, n_slices=n_slices)
+        return graph
+
     def stats(self):
         arr = (_capi.i64 * 5)()
         _capi.check(_capi.lib().vl1_plan_stats(self.handle, arr), "vl1_plan_stats")

@@ -58,6 +58,8 @@ def parse():
                          "c4: 2D Helmholtz k=40, 8 adaptive trees depth 4..10, q=6, 16 complex "
                          "RHS in one plan; c2: 3D Laplace adaptive (interpolation error 1e-8, "
                          "depth 3..7), q=5, 1 RHS")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the plan autotuner (vl1_plan_tune) before timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (byte-counter calibration)")
     return ap.parse_args()
@@ -276,6 +278,8 @@ def _plan_forms(plan, f, R, n_pts, flops, steps, warmup, slices, complex_data):
     res = {}
     ys = {}
     for form, S in (("f64", 0), ("split", slices)):
+        tuned = plan.tune(f, form=form, n_slices=S, reps=2)
+        log(f"autotune {form}: variant {tuned['variant']} {tuned['flags']} times {tuned['times_ms']}")
         y = torch.zeros_like(f)
         yr = torch.view_as_real(y).reshape(R, -1) if complex_data else y
         opts = _capi.SplitOpts(n_slices=S, n_products=0)
@@ -304,7 +308,7 @@ def _plan_forms(plan, f, R, n_pts, flops, steps, warmup, slices, complex_data):
         ms = e0.elapsed_time(e1) / steps
         kms = sum(a.elapsed_time(b) for a, b in kern) / len(kern)
         res[form] = {"ms": ms, "kernel_ms": kms, "value": n_pts * R / (ms * 1e-3) / 1e9,
-                     "tflops": flops / (kms * 1e-3) / 1e12}
+                     "tflops": flops / (kms * 1e-3) / 1e12, "autotune": tuned}
         ys[form] = y
         log(f"{form}: {ms:.3f} ms/step (contract {kms:.3f} ms), "
             f"{res[form]['value']:.3f} Gpts*RHS/s, {res[form]['tflops']:.2f} TFLOP/s")
@@ -577,6 +581,12 @@ def main():
         b.record(stream)
         return a, b
 
+    tuned = {}
+    if not args.no_tune:
+        for form, S_ in (("f64", 0), ("split", args.slices)):
+            tuned[form] = plan.tune(f, form=form, n_slices=S_, reps=2)
+            log(f"autotune {form}: variant {tuned[form]['variant']} {tuned[form]['flags']} "
+                f"times {tuned[form]['times_ms']}")
     y64 = torch.zeros_like(f)
     if args.only_split:
         args.steps, args.warmup = max(1, args.steps), 0
@@ -649,6 +659,7 @@ def main():
                           "rel_linf_per_rhs_max": float(err_inf.max())},
         "speedup_vs_plain_f64": (ms_plain / mss) if ms_plain else None,
         "speedup_vs_dense_f64_mfma": ms64 / mss,
+        "autotune": tuned,
         "forms": forms,
     }
     if not args.no_table_build:

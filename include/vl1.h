@@ -15,7 +15,8 @@
  *   vl1_near_field_f64   list1.py:1260-1487 (NearFieldFromCSR.__call__ and
  *                        the loopy `near_field` kernel, list1.py:1057-1200)
  *   vl1_plan_create /
- *   vl1_plan_eval        expansion_wrangler_fpnd.py:3222-3489 (eval_direct
+ *   vl1_plan_eval /
+ *   vl1_plan_tune        expansion_wrangler_fpnd.py:3222-3489 (eval_direct
  *                        called once per RHS by volume_fmm.py:983-989);
  *                        batched over RHS, FP64 or split-precision form
  *   vl1_duffy_build      nearfield_potential_table.py:2034-2213,2328-2463
@@ -183,6 +184,19 @@ int vl1_plan_stage(vl1_plan* plan, int form, const double* f, int64_t ldf, int n
                    const vl1_split_opts* opts, void* stream);
 int vl1_plan_contract(vl1_plan* plan, int form, int n_rhs, double* y, int64_t ldy,
                       const vl1_split_opts* opts, void* stream);
+/* Autotuner (the reference's counterpart is the list1_target_batch_size
+ * knob, list1.py / doc/source/nearfield_symmetry.rst:99-106, chosen by hand):
+ * stages f, then times every valid tile-kernel variant -- bit 0: one 16-row
+ * tile per wave (split form, blocks of at most 128 rows); bit 1: tiles in
+ * plain Morton order instead of grouped by block signature (several boxes
+ * per tile only); bit 2: no XCD phase alignment -- over `reps` contractions
+ * each, and keeps the fastest for this (form, slices, RHS per tile); later
+ * vl1_plan_contract / vl1_plan_eval calls use it.  y receives a valid
+ * result.  Synchronous.  chosen (optional): the variant; times_ms
+ * (optional, [8]): ms per contraction by variant, -1 if not valid. */
+int vl1_plan_tune(vl1_plan* plan, int form, const double* f, int64_t ldf, int n_rhs, double* y,
+                  int64_t ldy, const vl1_split_opts* opts, int reps, int32_t* chosen,
+                  double* times_ms, void* stream);
 /* Plan statistics: [0] list-1 pairs, [1] tiles, [2] (tile, case) steps,
  * [3] distinct (case, level) blocks, [4] device bytes. */
 int vl1_plan_stats(const vl1_plan* plan, int64_t* out5);

@@ -154,3 +154,28 @@ def test_split_adaptive_per_level(cuda):
     assert rel_errors(ref, ref_o)[0] <= TOL64
     l2, linf = rel_errors(got, ref_o)
     assert l2 <= TOLSPLIT and linf <= TOLSPLIT, (l2, linf)
+
+
+@pytest.mark.parametrize("form", ["f64", "split"])
+def test_autotune_keeps_results(cuda, form):
+    """vl1_plan_tune times every valid variant (several boxes per tile here,
+    so both tile orders are candidates) and later evaluations use the pick;
+    results stay at the oracle's bar."""
+    print(f"case: autotune adaptive 2D q=6 form={form}", flush=True)
+    tr = adaptive(2, 6)
+    tables = synthetic_table_levels(2, 6, n_tables=tr.nlevels, seed=8)
+    f = densities.gaussian_mixture_batch(tr.points, [21, 22, 23, 24])
+    w = NearFieldWrangler(tr, tables)
+    plan = w.get_plan(w.target_kernels[0], tr.target_boxes, tr.neighbor_source_boxes_starts,
+                      tr.neighbor_source_boxes_lists)
+    fd = torch.as_tensor(f, device="cuda").contiguous()
+    info = plan.tune(fd, form=form, n_slices=5, reps=2)
+    print("tuned:", info)
+    valid = {0, 2, 4, 6} | ({1, 3, 5, 7} if form == "split" else set())
+    assert set(info["times_ms"]) <= valid and info["variant"] in info["times_ms"]
+    assert len(info["times_ms"]) >= 4
+    got = plan.eval(fd, form=form, n_slices=5).cpu().numpy()
+    ref = _oracle(tr, tables, f, "per_level")
+    l2, linf = rel_errors(got, ref)
+    tol = TOL64 if form == "f64" else TOLSPLIT
+    assert l2 <= tol and linf <= tol, (l2, linf)

@@ -48,6 +48,7 @@ EXPORTED_SYMBOLS = (
     "vl1_plan_stage",
     "vl1_plan_contract",
     "vl1_plan_stats",
+    "vl1_plan_tune",
     "vl1_duffy_build",
     "vl1_orbit_rows_host",
 )
@@ -188,6 +189,8 @@ def lib():
     L.vl1_plan_stage.argtypes = [vp, C.c_int, vp, i64, C.c_int, C.POINTER(SplitOpts), vp]
     L.vl1_plan_contract.argtypes = [vp, C.c_int, C.c_int, vp, i64, C.POINTER(SplitOpts), vp]
     L.vl1_plan_stats.argtypes = [vp, C.POINTER(i64)]
+    L.vl1_plan_tune.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, i64, C.POINTER(SplitOpts),
+                                C.c_int, C.POINTER(i32), C.POINTER(f64), vp]
     L.vl1_duffy_build.argtypes = [C.POINTER(DuffyArgs), vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     if L.vl1_abi_version() != 1:

@@ -232,6 +232,7 @@ struct TileParams {
   double* y;
   int64_t ldy;
   int32_t* phase;                  // [8] block each XCD is working on (hint)
+  int use_phase;                   // start tiles at their XCD's current block
 };
 
 // Output element of block row `row` of target box `tb`: the row's mode is
@@ -249,7 +250,7 @@ __device__ inline int64_t out_index(const TileParams& p, int tb, int row) {
 // only a hint (plain racy stores); correctness does not depend on it.
 __device__ inline int phase_start(const TileParams& p, int st0, int st1, int xcd) {
   const int nst = st1 - st0;
-  if (nst <= 1) return 0;
+  if (nst <= 1 || !p.use_phase) return 0;
   const int want = __builtin_nontemporal_load(p.phase + xcd);
   int lo = st0, hi = st1;  // first step with block >= want (blocks sorted)
   while (lo < hi) {
@@ -748,7 +749,9 @@ struct vl1_plan {
   // device
   DevBuf mode_point, box_starts, M, MF, rowscale, A;
   int A_slices = 0;
-  std::map<int, TileSchedule*> schedules;  // by Rc
+  std::map<int, TileSchedule*> schedules;  // by 2 Rc + (Morton-only order)
+  std::map<int, int> tuned;                 // variant by (form, slices, Rc), from vl1_plan_tune
+  int forced_variant = -1;                  // candidate under test in vl1_plan_tune
   // staging buffers (grown on demand)
   DevBuf X, colscale, nmdot, XF;
   DevBuf phase;  // int32 [8]
@@ -758,8 +761,10 @@ struct vl1_plan {
   }
 };
 
-static int build_schedule(vl1_plan* pl, int Rc, TileSchedule** out, hipStream_t s) {
-  auto it = pl->schedules.find(Rc);
+static int build_schedule(vl1_plan* pl, int Rc, bool morton_only, TileSchedule** out,
+                          hipStream_t s) {
+  const int key = Rc * 2 + (morton_only ? 1 : 0);
+  auto it = pl->schedules.find(key);
   if (it != pl->schedules.end()) {
     *out = it->second;
     return VL1_OK;
@@ -776,7 +781,7 @@ static int build_schedule(vl1_plan* pl, int Rc, TileSchedule** out, hipStream_t 
   // of a tile need the same blocks and every step fills all their columns
   // (on adaptive trees Morton neighbours often differ in level or case).
   std::vector<uint64_t> sig(pl->n_tgt, 0);
-  if (B > 1) {
+  if (B > 1 && !morton_only) {
     for (int i = 0; i < pl->n_tgt; ++i) {
       uint64_t h = 1469598103934665603ull;  // FNV-1a over the block ids
       for (auto& pr : per_tgt[i]) {
@@ -843,7 +848,7 @@ static int build_schedule(vl1_plan* pl, int Rc, TileSchedule** out, hipStream_t 
     delete sch;
     return fail(VL1_E_HIP, std::string("plan schedule upload: ") + hipGetErrorString(e));
   }
-  pl->schedules[Rc] = sch;
+  pl->schedules[key] = sch;
   *out = sch;
   return VL1_OK;
 }
@@ -1074,6 +1079,24 @@ int vl1_plan_destroy(vl1_plan* plan) {
   return VL1_OK;
 }
 
+// Tile-kernel variants (bit set): kVarOneRow = one 16-row tile per wave
+// (split form, blocks of at most 128 rows); kVarMorton = tiles in plain Morton
+// order instead of grouped by block signature (only matters with several
+// boxes per tile); kVarNoPhase = no XCD phase alignment of the step order.
+constexpr int kVarOneRow = 1, kVarMorton = 2, kVarNoPhase = 4;
+
+static int tune_key(int form, int S, int Rc) { return (form * 8 + S) * 1024 + Rc; }
+
+static int default_variant(const vl1_plan* pl, int form) {
+  return (form == VL1_FORM_SPLIT_BF16 && pl->MP <= 128) ? kVarOneRow : 0;
+}
+
+static int plan_variant(const vl1_plan* pl, int form, int S, int Rc) {
+  if (pl->forced_variant >= 0) return pl->forced_variant;
+  auto it = pl->tuned.find(tune_key(form, S, Rc));
+  return it != pl->tuned.end() ? it->second : default_variant(pl, form);
+}
+
 static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts* opts,
                         hipStream_t s, int* S_out, TileSchedule** sch_out) {
   if (form != VL1_FORM_F64 && form != VL1_FORM_SPLIT_BF16)
@@ -1091,7 +1114,8 @@ static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts*
     if (rc != VL1_OK) return rc;
   }
   *S_out = S;
-  return build_schedule(pl, std::min(n_rhs, kCols), sch_out, s);
+  const int Rc = std::min(n_rhs, kCols);
+  return build_schedule(pl, Rc, (plan_variant(pl, form, S, Rc) & kVarMorton) != 0, sch_out, s);
 }
 
 int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs,
@@ -1186,14 +1210,16 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
   prm.y = y;
   prm.ldy = ldy;
   prm.phase = pl->phase.as<int32_t>();
+  const int variant = plan_variant(pl, form, S, Rc);
+  prm.use_phase = (variant & kVarNoPhase) ? 0 : 1;
   const int waves = pl->MP / 32;
   const int grid = ((sch->n_tiles + 7) / 8) * 8;
   for (int r0 = 0; r0 < R; r0 += Rc) {
     prm.r0 = r0;
     if (form == VL1_FORM_SPLIT_BF16) {
       const size_t lds = split_lds_bytes(S, pl->MP);
-      // one row tile per wave when the block has at most 8 of them
-      const bool one = pl->MP <= 128;
+      // one row tile per wave (blocks of at most 8 row tiles only)
+      const bool one = (variant & kVarOneRow) && pl->MP <= 128;
       e = launch_split(S, one, dim3(grid), lds, s, prm);
       if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split launch: ") + hipGetErrorString(e));
     } else {
@@ -1214,6 +1240,62 @@ int vl1_plan_eval(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rh
   int rc = vl1_plan_stage(pl, form, f, ldf, n_rhs, opts, stream);
   if (rc != VL1_OK) return rc;
   return vl1_plan_contract(pl, form, n_rhs, y, ldy, opts, stream);
+}
+
+// Autotuning: time every valid tile-kernel variant on the caller's data and
+// keep the fastest for this (form, slices, RHS-per-tile) -- see the variant
+// bits above.  Synchronous: the host waits for the timing runs.
+int vl1_plan_tune(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs, double* y,
+                  int64_t ldy, const vl1_split_opts* opts, int reps, int32_t* chosen,
+                  double* times_ms, void* stream) {
+  clear_error();
+  if (!pl || !f || !y) return fail(VL1_E_ARG, "vl1_plan_tune: null argument");
+  if (reps < 1) return fail(VL1_E_ARG, "vl1_plan_tune: reps must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  if (times_ms)
+    for (int v = 0; v < 8; ++v) times_ms[v] = -1.0;
+  int rc = vl1_plan_stage(pl, form, f, ldf, n_rhs, opts, stream);
+  if (rc != VL1_OK) return rc;
+  const int S = pl->staged_slices, Rc = std::min(n_rhs, kCols);
+  if (pl->n_tgt == 0 || pl->n_blocks == 0) {
+    if (chosen) *chosen = default_variant(pl, form);
+    return VL1_OK;
+  }
+  const bool multi_box = kCols / Rc > 1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int best = -1;
+  float best_ms = 0.f;
+  for (int v = 0; v < 8 && e == hipSuccess; ++v) {
+    if ((v & kVarOneRow) && (form != VL1_FORM_SPLIT_BF16 || pl->MP > 128)) continue;
+    if ((v & kVarMorton) && !multi_box) continue;
+    pl->forced_variant = v;
+    rc = vl1_plan_contract(pl, form, n_rhs, y, ldy, opts, stream);  // warm (and schedule)
+    if (rc != VL1_OK) break;
+    e = hipEventRecord(e0, s);
+    for (int r = 0; r < reps && rc == VL1_OK && e == hipSuccess; ++r)
+      rc = vl1_plan_contract(pl, form, n_rhs, y, ldy, opts, stream);
+    if (rc != VL1_OK) break;
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    ms /= reps;
+    if (times_ms) times_ms[v] = ms;
+    if (best < 0 || ms < best_ms) {
+      best = v;
+      best_ms = ms;
+    }
+  }
+  pl->forced_variant = -1;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (rc != VL1_OK) return rc;
+  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_tune: ") + hipGetErrorString(e));
+  pl->tuned[tune_key(form, S, Rc)] = best;
+  if (chosen) *chosen = best;
+  return VL1_OK;
 }
 
 int vl1_plan_stats(const vl1_plan* pl, int64_t* out5) {

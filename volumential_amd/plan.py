@@ -112,6 +112,41 @@ class List1Plan:
         )
         return out
 
+    VARIANT_BITS = {1: "one_row_tile_per_wave", 2: "morton_order", 4: "no_phase_alignment"}
+
+    def tune(self, mode_coefs: torch.Tensor, form="split", n_slices=5, reps=3, stream=None):
+        """Time every valid tile-kernel variant on ``mode_coefs`` and keep the
+        fastest for later evaluations of this form / slice count / RHS count.
+        Returns ``{"variant": v, "flags": [...], "times_ms": {v: ms}}``."""
+        f, fr, ldf, scale = self._prep(mode_coefs)
+        out = torch.zeros(f.shape, dtype=f.dtype, device=f.device)  # compact rows
+        yr = torch.view_as_real(out) if self.is_complex else out
+        ldy = int(f.shape[1])
+        opts = _capi.SplitOpts(n_slices=int(n_slices), n_products=0)
+        chosen = _capi.i32(0)
+        times = (_capi.f64 * 8)()
+        _capi.check(
+            _capi.lib().vl1_plan_tune(
+                self.handle, FORMS[form], fr.data_ptr(), ldf * scale, int(f.shape[0]),
+                yr.data_ptr(), ldy * scale, opts, int(reps), _capi.C.byref(chosen), times,
+                _capi.stream_handle(stream)),
+            "vl1_plan_tune",
+        )
+        v = int(chosen.value)
+        return {"variant": v, "flags": [n for b, n in self.VARIANT_BITS.items() if v & b],
+                "times_ms": {i: float(t) for i, t in enumerate(times) if t >= 0}}
+
+    def _prep(self, mode_coefs):
+        f = mode_coefs if mode_coefs.dim() == 2 else mode_coefs.unsqueeze(0)
+        want = torch.complex128 if self.is_complex else torch.float64
+        if f.dtype != want or f.device.type != "cuda" or f.stride(-1) != 1:
+            raise ValueError(f"mode_coefs must be a contiguous-rows {want} CUDA tensor")
+        if f.shape[1] != self.n_points:
+            raise ValueError("mode_coefs has the wrong number of points")
+        fr = torch.view_as_real(f) if self.is_complex else f
+        ldf = f.stride(0) if f.shape[0] > 1 else f.shape[1]
+        return f, fr, int(ldf), 2 if self.is_complex else 1
+
     def capture(self, mode_coefs: torch.Tensor, out: torch.Tensor, form="split", n_slices=5):
         """Record one evaluation (density staging + tile contraction) of
         ``mode_coefs`` into ``out`` as a HIP graph and return it; each

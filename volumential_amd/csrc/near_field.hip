@@ -64,16 +64,19 @@ struct NFParams {
 
 constexpr int kNFBlock = 256;
 
-template <typename IdxT, int RC>
+// CPLX: complex table, densities and result (interleaved re, im doubles;
+// leading dimensions in doubles); products are complex, the scaling real.
+template <typename IdxT, int RC, bool CPLX>
 __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams p) {
   extern __shared__ double smem[];
+  constexpr int VW = CPLX ? 2 : 1;  // doubles per value
   const int n = p.n;
   const int P = kNFBlock / p.nt;
   double* sT = smem;
   double* sF = sT + p.tbuf;
-  double* sD = sF + (size_t)n * RC;
+  double* sD = sF + (size_t)n * RC * VW;
   double* red = sD + n;
-  int* sMode = (int*)(red + (P > 1 ? kNFBlock * RC : 0));
+  int* sMode = (int*)(red + (P > 1 ? kNFBlock * RC * VW : 0));
 
   const int tid = threadIdx.x;
   const int lane_t = tid % p.nt;
@@ -95,9 +98,9 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
       const int tm_raw = ti < ntb ? p.tgt_ids[tbeg + ti] : -1;
       const bool tact = (unsigned)tm_raw < (unsigned)n;
       const int tmode = tact ? tm_raw : 0;
-      double acc[RC];
+      double acc[RC], aci[RC];  // aci: imaginary parts (complex data only)
 #pragma unroll
-      for (int r = 0; r < RC; ++r) acc[r] = 0.0;
+      for (int r = 0; r < RC; ++r) acc[r] = aci[r] = 0.0;
 
       for (int li = l0; li < l1; ++li) {
         const int sb = p.nbr_lists[li];
@@ -132,15 +135,17 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
         const int rank = p.ranks[cid];
         const int off = p.offs[rank];
         const int cnt = p.offs[rank + 1] - off;
-        const double* gT = p.table + (int64_t)tlev * p.n_entries + off;
-        const bool staged = cnt <= p.tbuf;
+        const double* gT = p.table + ((int64_t)tlev * p.n_entries + off) * VW;
+        const bool staged = cnt * VW <= p.tbuf;
         const int sbeg = p.src_starts[sb];
         const int ns = min(p.src_counts[sb], n);
         if (staged)
-          for (int i = tid; i < cnt; i += kNFBlock) sT[i] = gT[i];
-        for (int i = tid; i < ns * RC; i += kNFBlock) {
-          const int r = i / ns, sid = i % ns;
-          sF[sid * RC + r] = r < rc ? p.f[(int64_t)(r0 + r) * p.ldf + sbeg + sid] : 0.0;
+          for (int i = tid; i < cnt * VW; i += kNFBlock) sT[i] = gT[i];
+        for (int i = tid; i < ns * RC * VW; i += kNFBlock) {
+          const int c = i % VW, rs = i / VW;
+          const int r = rs / ns, sid = rs % ns;
+          sF[(sid * RC + r) * VW + c] =
+              r < rc ? p.f[(int64_t)(r0 + r) * p.ldf + (int64_t)(sbeg + sid) * VW + c] : 0.0;
         }
         for (int i = tid; i < ns; i += kNFBlock) {
           sMode[i] = p.src_ids[sbeg + i];
@@ -153,11 +158,22 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
             const int s = sMode[sid];
             if ((unsigned)s >= (unsigned)n) continue;  // malformed mode id: no contribution
             const int rel = (int)rm[(int64_t)s * n];
-            const double tv = staged ? sT[rel] : gT[rel];
-            const double v = tv * scal + sD[sid];
-            const double* fs = sF + sid * RC;
+            const double* fs = sF + sid * RC * VW;
+            if constexpr (!CPLX) {
+              const double tv = staged ? sT[rel] : gT[rel];
+              const double v = tv * scal + sD[sid];
 #pragma unroll
-            for (int r = 0; r < RC; ++r) acc[r] = fma(fs[r], v, acc[r]);
+              for (int r = 0; r < RC; ++r) acc[r] = fma(fs[r], v, acc[r]);
+            } else {
+              const double vr = (staged ? sT[2 * rel] : gT[2 * rel]) * scal;
+              const double vi = (staged ? sT[2 * rel + 1] : gT[2 * rel + 1]) * scal;
+#pragma unroll
+              for (int r = 0; r < RC; ++r) {
+                const double fr = fs[2 * r], fi = fs[2 * r + 1];
+                acc[r] = fma(fr, vr, fma(-fi, vi, acc[r]));
+                aci[r] = fma(fr, vi, fma(fi, vr, aci[r]));
+              }
+            }
           }
         }
       }
@@ -165,25 +181,34 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
       if (P > 1) {
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < RC; ++r) red[(part * p.nt + lane_t) * RC + r] = acc[r];
+        for (int r = 0; r < RC; ++r) {
+          red[((part * p.nt + lane_t) * RC + r) * VW] = acc[r];
+          if (CPLX) red[((part * p.nt + lane_t) * RC + r) * VW + 1] = aci[r];
+        }
         __syncthreads();
         if (part == 0) {
           for (int j = 1; j < P; ++j)
 #pragma unroll
-            for (int r = 0; r < RC; ++r) acc[r] += red[(j * p.nt + lane_t) * RC + r];
+            for (int r = 0; r < RC; ++r) {
+              acc[r] += red[((j * p.nt + lane_t) * RC + r) * VW];
+              if (CPLX) aci[r] += red[((j * p.nt + lane_t) * RC + r) * VW + 1];
+            }
         }
       }
       if (part == 0 && tact) {
-        for (int r = 0; r < rc; ++r) p.y[(int64_t)(r0 + r) * p.ldy + tbeg + ti] = acc[r];
+        for (int r = 0; r < rc; ++r) {
+          p.y[(int64_t)(r0 + r) * p.ldy + (int64_t)(tbeg + ti) * VW] = acc[r];
+          if (CPLX) p.y[(int64_t)(r0 + r) * p.ldy + (int64_t)(tbeg + ti) * VW + 1] = aci[r];
+        }
       }
       __syncthreads();
     }
   }
 }
 
-template <typename IdxT, int RC>
+template <typename IdxT, int RC, bool CPLX>
 static hipError_t launch_nf(const NFParams& prm, int grid, size_t smem, hipStream_t s) {
-  auto fn = near_field_f64_kernel<IdxT, RC>;
+  auto fn = near_field_f64_kernel<IdxT, RC, CPLX>;
   hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)smem);
   if (e != hipSuccess) return e;
@@ -199,7 +224,8 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
                                   void* stream) {
   clear_error();
   if (!pl || !a) return fail(VL1_E_ARG, "vl1_near_field_f64: null argument");
-  if (pl->is_complex) return fail(VL1_E_UNSUPPORTED, "vl1_near_field_f64: complex payload");
+  if (pl->is_complex && a->log_displacement && a->scaling_mode == VL1_SCALING_CANONICAL)
+    return fail(VL1_E_UNSUPPORTED, "vl1_near_field_f64: the 2D log displacement applies to real tables");
   if (a->n_tgt_boxes < 0 || a->n_rhs < 1 || a->aligned_nboxes < 1)
     return fail(VL1_E_ARG, "vl1_near_field_f64: bad sizes");
   if (a->scaling_mode < 0 || a->scaling_mode > 2)
@@ -263,23 +289,35 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
   const int P = kNFBlock / nt;
   const int RC = a->n_rhs == 1 ? 1 : (a->n_rhs <= 4 ? 4 : 16);
   const size_t lds_cap = 150 * 1024;
-  const size_t fixed = sizeof(double) * ((size_t)pl->n * RC + pl->n + (P > 1 ? kNFBlock * RC : 0)) +
-                       sizeof(int) * pl->n;
+  const size_t vw = pl->is_complex ? 2 : 1;
+  const size_t fixed =
+      sizeof(double) * ((size_t)pl->n * RC * vw + pl->n + (P > 1 ? kNFBlock * RC * vw : 0)) +
+      sizeof(int) * pl->n;
   if (fixed > lds_cap) return fail(VL1_E_UNSUPPORTED, "vl1_near_field_f64: box too large for LDS");
-  size_t tbuf = std::min<size_t>((size_t)pl->max_orbit_count, (lds_cap - fixed) / sizeof(double));
+  size_t tbuf = std::min<size_t>((size_t)pl->max_orbit_count * vw, (lds_cap - fixed) / sizeof(double));
   prm.tbuf = (int)tbuf;
   const size_t smem = fixed + sizeof(double) * tbuf;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   const bool u16 = pl->rowmap_u16 != 0;
   const int grid = a->n_tgt_boxes;
-  if (RC == 1)
-    e = u16 ? launch_nf<uint16_t, 1>(prm, grid, smem, s) : launch_nf<int32_t, 1>(prm, grid, smem, s);
-  else if (RC == 4)
-    e = u16 ? launch_nf<uint16_t, 4>(prm, grid, smem, s) : launch_nf<int32_t, 4>(prm, grid, smem, s);
-  else
-    e = u16 ? launch_nf<uint16_t, 16>(prm, grid, smem, s)
-            : launch_nf<int32_t, 16>(prm, grid, smem, s);
+  if (pl->is_complex) {
+    if (RC == 1)
+      e = u16 ? launch_nf<uint16_t, 1, true>(prm, grid, smem, s) : launch_nf<int32_t, 1, true>(prm, grid, smem, s);
+    else if (RC == 4)
+      e = u16 ? launch_nf<uint16_t, 4, true>(prm, grid, smem, s) : launch_nf<int32_t, 4, true>(prm, grid, smem, s);
+    else
+      e = u16 ? launch_nf<uint16_t, 16, true>(prm, grid, smem, s)
+              : launch_nf<int32_t, 16, true>(prm, grid, smem, s);
+  } else {
+    if (RC == 1)
+      e = u16 ? launch_nf<uint16_t, 1, false>(prm, grid, smem, s) : launch_nf<int32_t, 1, false>(prm, grid, smem, s);
+    else if (RC == 4)
+      e = u16 ? launch_nf<uint16_t, 4, false>(prm, grid, smem, s) : launch_nf<int32_t, 4, false>(prm, grid, smem, s);
+    else
+      e = u16 ? launch_nf<uint16_t, 16, false>(prm, grid, smem, s)
+              : launch_nf<int32_t, 16, false>(prm, grid, smem, s);
+  }
   if (e != hipSuccess)
     return fail(VL1_E_HIP, std::string("near_field_f64 launch: ") + hipGetErrorString(e));
   return VL1_OK;

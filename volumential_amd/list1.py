@@ -188,8 +188,6 @@ class NearFieldFromCSR:
         result = kw.pop("result")
         mode_coefs = kw.pop("mode_coefs")
         payload = self._payload(kw)
-        if payload.is_complex:
-            raise NotImplementedError("complex list-1 evaluation goes through eval_direct plans")
         if int(kw.get("encoding_base", gallery.ENCODING_BASE)) != gallery.ENCODING_BASE:
             raise ValueError("encoding_base must be 13")
 
@@ -222,9 +220,15 @@ class NearFieldFromCSR:
             np.asarray(mode_coefs)
         )
         host_result = not (isinstance(result, torch.Tensor) and result.device.type == "cuda")
-        d_res = _to_device(result, torch.float64)
-        d_coef = _to_device(coefs_2d, torch.float64)
+        # complex tables evaluate in complex128 (expansion_wrangler_fpnd.py:3283-3297);
+        # the C ABI takes interleaved (re, im) doubles
+        vdt = torch.complex128 if payload.is_complex else torch.float64
+        d_res = _to_device(result, vdt)
+        d_coef = _to_device(coefs_2d, vdt)
         n_rhs = 1 if d_coef.dim() == 1 else int(d_coef.shape[0])
+        vw = 2 if payload.is_complex else 1
+        p_res = torch.view_as_real(d_res) if payload.is_complex else d_res
+        p_coef = torch.view_as_real(d_coef) if payload.is_complex else d_coef
         centers = _to_device(kw["box_centers"], torch.float64)
         arrs = {
             k: _to_device(kw[k], torch.int32)
@@ -256,10 +260,10 @@ class NearFieldFromCSR:
             n_tgt_boxes=n_tgt,
             aligned_nboxes=int(centers.shape[1]),
             n_rhs=n_rhs,
-            ld_coefs=int(d_coef.shape[-1]),
-            ld_result=int(d_res.shape[-1]),
-            result=d_res.data_ptr(),
-            mode_coefs=d_coef.data_ptr(),
+            ld_coefs=int(d_coef.shape[-1]) * vw,
+            ld_result=int(d_res.shape[-1]) * vw,
+            result=p_res.data_ptr(),
+            mode_coefs=p_coef.data_ptr(),
             box_centers=centers.data_ptr(),
             box_levels=d_levels.data_ptr(),
             box_source_counts_nonchild=arrs["box_source_counts_nonchild"].data_ptr(),

@@ -150,18 +150,6 @@ class NearFieldWrangler:
                     "Insufficient list of tables: the coarsest level mesh cells at level "
                     f"{min_lev} are not covered."
                 )
-        if getattr(out_kernel, "is_complex_valued", False):
-            # complex tables evaluate through a plan in FP64 (the real 2n x 2n
-            # embedding of each complex block), cached per CSR
-            mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
-                np.asarray(mode_coefs))
-            mc = mc.to(device="cuda", dtype=torch.complex128)
-            one = mc.dim() == 1
-            mc = (mc.unsqueeze(0) if one else mc).contiguous()
-            out = out_pot.unsqueeze(0) if one else out_pot
-            plan = self.get_plan(out_kernel, target_boxes, starts, lists)
-            plan.eval(mc, out=out, form="f64")
-            return out_pot
         data, nmlz, _, _, _, info = prepare_table_data_and_entry_map(tables)
         shapes = {
             "n_tables": len(tables),

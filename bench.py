@@ -182,6 +182,13 @@ def table_build_record(rules):
     return out
 
 
+def split_rows_per_wave(form_record):
+    """Template argument NRT of the split kernel that ran: 1 when the tuned
+    (or default) variant has one row tile per wave, else 2."""
+    tuned = form_record.get("autotune") or {}
+    return 1 if (tuned.get("variant", 0) & 1) else 2
+
+
 def measured_traffic(kernel, level, q, rhs):
     """HBM-side bytes per launch of `kernel` on the default workload, from the
     committed counter passes (profiles/c3_traffic.json: FETCH_SIZE and
@@ -388,7 +395,8 @@ def run_c4(args):
          "levels": lv.tolist(), "form": f"split_bf16 S={args.slices}", "parallelism": "single GPU"},
         {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
          "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
-         "kernel": f"split_bf16_kernel<{args.slices}>", "kernel_ms": sp["kernel_ms"],
+         "kernel": f"split_bf16_kernel<{args.slices}, {split_rows_per_wave(sp)}>",
+         "kernel_ms": sp["kernel_ms"],
          "flops_per_launch": nprod * flops},
         forms,
         {"error_vs_fp64": {"rel_l2_per_rhs_max": sp["rel_l2_max"],
@@ -419,6 +427,18 @@ def run_c2(args):
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(2)
+    # list-1 CSR of the same leaves on the GPU (vl1_csr_builder_*) vs the host builder
+    t0 = time.perf_counter()
+    st_h, ls_h = vtree._list1_csr(3, tr.leaf_levels, tr.leaf_index)
+    csr_host_s = time.perf_counter() - t0
+    vtree.list1_csr_gpu(3, tr.leaf_levels, tr.leaf_index)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st_g, ls_g = vtree.list1_csr_gpu(3, tr.leaf_levels, tr.leaf_index)
+    torch.cuda.synchronize()
+    csr_gpu_s = time.perf_counter() - t0
+    csr_same = bool(np.array_equal(st_g.cpu().numpy(), st_h) and np.array_equal(ls_g.cpu().numpy(), ls_h))
+    log(f"list-1 CSR: host {csr_host_s:.2f} s, GPU {csr_gpu_s * 1e3:.1f} ms, identical {csr_same}")
     table = NearFieldInteractionTable(5, 3, LaplaceKernel(3), source_box_extent=2.0)
     table.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=20))
     f = torch.as_tensor(densities.gaussian_mixture(tr.points, 0)[None, :], device="cuda").contiguous()
@@ -454,7 +474,8 @@ def run_c2(args):
     extra = {"error_vs_fp64": {"rel_l2_per_rhs_max": sp["rel_l2_max"],
                                "rel_linf_per_rhs_max": sp["rel_linf_max"]},
              "speedup_vs_plain_f64": ms_plain / sp["ms"],
-             "speedup_vs_dense_f64_mfma": forms["f64"]["ms"] / sp["ms"]}
+             "speedup_vs_dense_f64_mfma": forms["f64"]["ms"] / sp["ms"],
+             "list1_csr": {"host_s": csr_host_s, "gpu_s": csr_gpu_s, "identical": csr_same}}
     if not args.no_cpu_baseline:
         extra["cpu_baseline"] = cpu_baseline(tr, table, f.cpu().numpy(), args.cpu_seconds)
     return _line(
@@ -466,7 +487,8 @@ def run_c2(args):
          "parallelism": "single GPU"},
         {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
          "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
-         "kernel": f"split_bf16_kernel<{args.slices}>", "kernel_ms": sp["kernel_ms"],
+         "kernel": f"split_bf16_kernel<{args.slices}, {split_rows_per_wave(sp)}>",
+         "kernel_ms": sp["kernel_ms"],
          "flops_per_launch": nprod * flops},
         forms, extra)
 
@@ -618,7 +640,9 @@ def main():
     value = forms["split_bf16"]["value"]
     ms = mss
     achieved = mfma_flops / (ks * 1e-3) / 1e12
-    traffic, traffic_note = measured_traffic(f"split_bf16_kernel<{S}>", level, q, R)
+    forms["split_bf16"]["autotune"] = tuned.get("split")
+    kname = f"split_bf16_kernel<{S}, {split_rows_per_wave(forms['split_bf16'])}>"
+    traffic, traffic_note = measured_traffic(kname, level, q, R)
 
     res = {
         "metric": "list-1 eval Gpts*RHS/s at <=1e-10 rel err; split-MFMA vs FP64 GPU kernel",
@@ -650,7 +674,7 @@ def main():
             "frac": achieved / PEAK_BF16_TFLOPS,
             "traffic": traffic,
             "traffic_source": traffic_note,
-            "kernel": f"split_bf16_kernel<{S}>",
+            "kernel": kname,
             "kernel_ms": ks,
             "flops_per_launch": mfma_flops,
             "hbm_gbs_alg": bytes_alg / (ks * 1e-3) / 1e9,

@@ -228,6 +228,24 @@ typedef struct vl1_duffy_args {
 
 int vl1_duffy_build(const vl1_duffy_args* a, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* List-1 CSR of a 2:1-balanced tree of leaves, built on the GPU.      */
+/* Replaces boxtree's list-1 traversal used by meshgen.py:391-444 /    */
+/* tree_interactive_build.py:202-470 for the leaf-only trees here.     */
+/* ------------------------------------------------------------------ */
+typedef struct vl1_csr_builder vl1_csr_builder;
+/* levels [n_leaves] and index [n_leaves][dim] (integer cell index of each
+ * leaf at its level) are HOST arrays.  Sorts the leaves and counts every
+ * leaf's neighbours (leaves touching it, itself included) on the device;
+ * *n_pairs receives the total.  VL1_E_LAYOUT if the leaves are not a
+ * 2:1-balanced tiling of the cube. */
+int vl1_csr_builder_create(vl1_csr_builder** out, int dim, int n_leaves, const uint8_t* levels,
+                           const int32_t* index, int64_t* n_pairs, void* stream);
+/* Writes starts [n_leaves + 1] and lists [n_pairs] (device, caller-owned);
+ * each list sorted by leaf id. */
+int vl1_csr_builder_fill(vl1_csr_builder* b, int32_t* starts, int32_t* lists, void* stream);
+int vl1_csr_builder_destroy(vl1_csr_builder* b);
+
 /* Host evaluation of the compact-row function the device kernels use
  * (orbit_arithmetic.py:566-634 _evaluate_compact_arithmetic_orbit_entry,
  * row relative to the orbit's first row): out[i] = row(s[i], t[i]) for one

@@ -49,6 +49,9 @@ EXPORTED_SYMBOLS = (
     "vl1_plan_contract",
     "vl1_plan_stats",
     "vl1_plan_tune",
+    "vl1_csr_builder_create",
+    "vl1_csr_builder_fill",
+    "vl1_csr_builder_destroy",
     "vl1_duffy_build",
     "vl1_orbit_rows_host",
 )
@@ -192,6 +195,9 @@ def lib():
     L.vl1_plan_tune.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, i64, C.POINTER(SplitOpts),
                                 C.c_int, C.POINTER(i32), C.POINTER(f64), vp]
     L.vl1_duffy_build.argtypes = [C.POINTER(DuffyArgs), vp]
+    L.vl1_csr_builder_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, vp, vp, C.POINTER(i64), vp]
+    L.vl1_csr_builder_fill.argtypes = [vp, vp, vp, vp]
+    L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     if L.vl1_abi_version() != 1:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")

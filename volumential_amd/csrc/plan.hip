@@ -409,9 +409,13 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
       for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
         for (int e = 0; e < 4; ++e) *yslot(wave + 8 * rt, ct, e) = 0.0;
-  floatx4 acc[3][NRT][4];
+  // sets 0, 1: exact diagonal sums; 2 (and 3 when NRT = 1): the tail, split
+  // over two sets for one row tile per wave so that consecutive tail products
+  // into one accumulator stay 8 MFMAs apart
+  constexpr int NSET = NRT == 1 ? 4 : 3;
+  floatx4 acc[NSET][NRT][4];
 #pragma unroll
-  for (int g = 0; g < 3; ++g)
+  for (int g = 0; g < NSET; ++g)
 #pragma unroll
     for (int a = 0; a < NRT; ++a)
 #pragma unroll
@@ -466,21 +470,26 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
     const __bf16* rX = sX + ring * RING;
     if (NRT == 2 || nrt > 0) static_for<0, S>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
+      bf16x8 xf[4];
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
         const int col = 16 * ct + l16;
-        const bf16x8 xf = *reinterpret_cast<const bf16x8*>(
+        xf[ct] = *reinterpret_cast<const bf16x8*>(
             rX + ((size_t)j * kCols + col) * 32 + ((lg ^ ((col >> 1) & 3)) * 8));
-        static_for<0, S - j>([&](auto ic) {
-          constexpr int i = S - j - 1 - decltype(ic)::value;
-          constexpr int g = (i + j) < 2 ? (i + j) : 2;
-#pragma unroll
-          for (int rt = 0; rt < NRT; ++rt)
-            if (rt < nrt)
-              acc[g][rt][ct] =
-                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][rt], xf, acc[g][rt][ct], 0, 0, 0);
-        });
       }
+      // column tiles innermost: consecutive products into one accumulator
+      // are 4 NRT MFMAs apart (closer dependent MFMAs measurably stall)
+      static_for<0, S - j>([&](auto ic) {
+        constexpr int i = S - j - 1 - decltype(ic)::value;
+        constexpr int g = (i + j) < 2 ? (i + j) : (NSET == 4 ? 2 + (i & 1) : 2);
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+          if (rt < nrt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+              acc[g][rt][ct] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][rt], xf[ct], acc[g][rt][ct], 0, 0, 0);
+      });
       // slot i = S-1-j had its last partner: refill it for the next chunk
       if (has_next) {
         constexpr int i = S - 1 - j;
@@ -505,15 +514,16 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const double rsc = rs[16 * rt16 + 4 * lg + e];
-            const double v = ((double)acc[0][rt][ct][e] + (double)acc[1][rt][ct][e]) +
-                             (double)acc[2][rt][ct][e];
+            double v = ((double)acc[0][rt][ct][e] + (double)acc[1][rt][ct][e]) +
+                       (double)acc[2][rt][ct][e];
+            if constexpr (NSET == 4) v += (double)acc[3][rt][ct][e];
             double* ys = yslot(rt16, ct, e);
             *ys = fma(v * rsc, cs, *ys + ds);
           }
         }
       }
 #pragma unroll
-      for (int g = 0; g < 3; ++g)
+      for (int g = 0; g < NSET; ++g)
 #pragma unroll
         for (int a = 0; a < NRT; ++a)
 #pragma unroll
@@ -549,125 +559,181 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
   }
 }
 
-// Dense FP64 form: same schedule, v_mfma_f64_16x16x4f64 on the FP64 blocks.
-// Output lane layout of the f64 16x16x4 MFMA: lane l, register r holds
-// row l/16 + 4 r, column l % 16.  Wave = 32 rows x 64 columns; the density
-// image (pre-multiplied by the pair scaling) is staged with all loads in
-// flight, the table fragments of the next K chunk are loaded while the
-// current one feeds the MFMAs.
+// Dense FP64 form, software-pipelined like the split kernel: 8 waves, wave
+// w owns row tiles w and w + 8 (NRT = 2) or w (NRT = 1) and all 64 columns;
+// FP64 accumulation in the MFMA accumulators (v_mfma_f64_16x16x4f64; lane l,
+// register r of a 16 x 16 output tile holds row l/16 + 4 r, column l % 16).
+// Densities stream through a double-buffered LDS ring one K chunk of 32 at
+// a time, scaled by the pair scaling as they are stored; the column pitch is
+// 33 doubles, so the 16 columns of a b64 read pass start 8 bytes apart mod
+// 128 (conflict free).  Table fragments (fragment-ordered FP64 blocks, 512 B
+// per wave load) for the next chunk are double-buffered in registers and
+// loaded at the top of the chunk.  Each step ends by adding the column's
+// 2D log displacement.
+constexpr int kDPitch = 33;
+
+template <int NRT>
 __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int KPL = p.KP + 2;  // doubles; pitch keeps 16-byte alignment
-  double* sX = reinterpret_cast<double*>(smem_raw);
-  double* sAl = sX + (size_t)kCols * KPL;
-  double* sDsp = sAl + kCols;
+  double* sX = reinterpret_cast<double*>(smem_raw);          // [2][64][kDPitch]
+  double* sAl = sX + 2 * kCols * kDPitch;                    // [2][64]
+  double* sDsp = sAl + 2 * kCols;                            // [2][64]
+  int64_t* sBase = reinterpret_cast<int64_t*>(sDsp + 2 * kCols);  // [2][64]
 
   const int tile = xcd_tile(blockIdx.x, p.n_tiles);
   if (tile >= p.n_tiles) return;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int wave = NRT == 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x) / 64 : threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
   const int l16 = lane % 16, lg = lane / 16;
   const int ncols = p.B * p.Rc;
-  const int vec_per_col = p.KP / 2;  // 16-byte vectors (2 doubles) per column
-  const int nvec = kCols * vec_per_col;
-
-  doublex4 yacc[2][4];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) yacc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
-
-  const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
+  const int nrt = NRT == 2 ? (p.MP / 16 - wave + 7) / 8 : (wave < p.MP / 16 ? 1 : 0);
+  const int nkc = p.KP / 32;
   const int xcd = blockIdx.x % 8;
+  const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
+  const int nst = st1 - st0;
   const int rot = phase_start(p, st0, st1, xcd);
-  for (int k = 0; k < st1 - st0; ++k) {
-    int st = st0 + rot + k;
-    if (st >= st1) st -= st1 - st0;
-    const int blk = p.step_block[st];
-    if (threadIdx.x == 0) __builtin_nontemporal_store(blk, p.phase + xcd);
-    __syncthreads();
+  auto step_of = [&](int k) { int st = st0 + rot + k; return st >= st1 ? st - nst : st; };
+
+  auto load_cols = [&](int st, int slot) {
     if (threadIdx.x < kCols) {
       const int c = threadIdx.x;
       double al = 0.0, ds = 0.0;
+      int64_t base = -1;
       if (c < ncols) {
         const int b = c / p.Rc, r = c % p.Rc;
         const int sb = p.step_src[st * p.B + b];
         if (sb >= 0) {
+          const int64_t ci = (int64_t)sb * p.R + p.r0 + r;
           al = p.step_alpha[st * p.B + b];
-          ds = p.step_disp[st * p.B + b] * p.nmdot[(int64_t)sb * p.R + p.r0 + r];
+          ds = p.step_disp[st * p.B + b] * p.nmdot[ci];
+          base = ci * p.KP;
         }
       }
-      sAl[c] = al;
-      sDsp[c] = ds;
+      sAl[slot * kCols + c] = al;
+      sDsp[slot * kCols + c] = ds;
+      sBase[slot * kCols + c] = base;
+    }
+  };
+  // chunk kc of the step in descriptor slot cs: 64 columns x 32 doubles =
+  // 1024 16-byte vectors, 2 per thread
+  auto fetch_chunk = [&](int cs, int kc, uint4 (&buf)[2]) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int u = threadIdx.x + 512 * m;
+      const int col = u / 16, v = u % 16;
+      const int64_t base = sBase[cs * kCols + col];
+      buf[m] = base >= 0 ? *reinterpret_cast<const uint4*>(p.XF + base + kc * 32 + 2 * v)
+                         : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store_chunk = [&](int ring, int cs, const uint4 (&buf)[2]) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int u = threadIdx.x + 512 * m;
+      const int col = u / 16, v = u % 16;
+      const double2 d = *reinterpret_cast<const double2*>(&buf[m]);
+      const double al = sAl[cs * kCols + col];
+      double* dst = sX + ((size_t)ring * kCols + col) * kDPitch + 2 * v;
+      dst[0] = d.x * al;
+      dst[1] = d.y * al;
+    }
+  };
+
+  doublex4 yacc[NRT][4];
+#pragma unroll
+  for (int a = 0; a < NRT; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) yacc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+
+  // fragment of (block, row tile slot rt, k group k4) for this lane; slots
+  // past the wave's count read row tile 0 and are not used
+  auto frag = [&](int blk, int rt, int k4) {
+    const int rt16 = rt < nrt ? wave + 8 * rt : 0;
+    return p.MF + (((int64_t)blk * (p.MP / 16) + rt16) * (p.KP / 4) + k4) * 64 + lane;
+  };
+  double ac[NRT][8], an[NRT][8];  // current / next chunk's table fragments
+  if (nst > 0) {
+    load_cols(step_of(0), 0);
+    __syncthreads();
+    uint4 buf[2];
+    fetch_chunk(0, 0, buf);
+    store_chunk(0, 0, buf);
+    const int blk = p.step_block[step_of(0)];
+    if (threadIdx.x == 0) __builtin_nontemporal_store(blk, p.phase + xcd);
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) ac[rt][u] = *frag(blk, rt, u);
+  }
+  __syncthreads();
+
+  const int total = nst * nkc;
+  for (int it = 0; it < total; ++it) {
+    const int k = it / nkc, kc = it % nkc;
+    const int ring = it & 1, cslot = k & 1;
+    const int blk = p.step_block[step_of(k)];
+    const bool has_next = it + 1 < total;
+    const bool next_step = kc + 1 == nkc;
+    const int nkc_ = next_step ? 0 : kc + 1;
+    int nblk = blk;
+    if (has_next && next_step) {
+      nblk = p.step_block[step_of(k + 1)];
+      load_cols(step_of(k + 1), cslot ^ 1);  // read after the barrier below
+    }
+    uint4 xnext[2];
+    if (has_next && !next_step) fetch_chunk(cslot, nkc_, xnext);
+    if (has_next) {
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) an[rt][u] = *frag(nblk, rt, nkc_ * 8 + u);
+    }
+    const double* rX = sX + (size_t)ring * kCols * kDPitch;
+    if (NRT == 2 || nrt > 0) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        double bx[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) bx[ct] = rX[(16 * ct + l16) * kDPitch + 4 * u + lg];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+          if (rt < nrt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+              yacc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[rt][u], bx[ct], yacc[rt][ct], 0, 0, 0);
+      }
+    }
+    if (has_next) {
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ac[rt][u] = an[rt][u];
+    }
+    if (next_step) {
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        const double ds = sDsp[cslot * kCols + 16 * ct + l16];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) yacc[rt][ct][e] += ds;
+      }
+      if (has_next && threadIdx.x == 0) __builtin_nontemporal_store(nblk, p.phase + xcd);
+    }
+    __syncthreads();  // ring[ring ^ 1] free, next descriptors visible
+    if (has_next) {
+      if (next_step) fetch_chunk(cslot ^ 1, 0, xnext);
+      store_chunk(ring ^ 1, next_step ? cslot ^ 1 : cslot, xnext);
     }
     __syncthreads();
-    auto ld = [&](int v) -> uint4 {
-      const int kv = v % vec_per_col, c = v / vec_per_col;
-      if (c < ncols) {
-        const int b = c / p.Rc, r = c % p.Rc;
-        const int sb = p.step_src[st * p.B + b];
-        if (sb >= 0)
-          return *reinterpret_cast<const uint4*>(p.XF + ((int64_t)sb * p.R + p.r0 + r) * p.KP + 2 * kv);
-      }
-      return make_uint4(0, 0, 0, 0);
-    };
-    auto stv = [&](int v, uint4 val) {
-      const int kv = v % vec_per_col, c = v / vec_per_col;
-      double2 d = *reinterpret_cast<double2*>(&val);
-      const double al = sAl[c];
-      d.x *= al;
-      d.y *= al;
-      *reinterpret_cast<double2*>(sX + (size_t)c * KPL + 2 * kv) = d;
-    };
-    staged_copy<16>(nvec, ld, stv);  // 64 KP / 2 vectors over 2 MP threads = 16 each
-    __syncthreads();
-    // fragment-ordered block: operand of (row tile rt16, k group k4) for
-    // lane l is MF[((blk * MP/16 + rt16) * KP/4 + k4) * 64 + l]
-    const double* Mrow[2];
-#pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
-      Mrow[rt] = p.MF + ((int64_t)blk * (p.MP / 16) + 2 * wave + rt) * (p.KP / 4) * 64 + lane;
-    double a_cur[2][4];
-#pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-      for (int u = 0; u < 4; ++u) a_cur[rt][u] = Mrow[rt][u * 64];
-    for (int kc = 0; kc < p.KP; kc += 16) {
-      double a_nxt[2][4];
-      const bool more = kc + 16 < p.KP;
-#pragma unroll
-      for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          a_nxt[rt][u] = more ? Mrow[rt][(kc / 4 + 4 + u) * 64] : 0.0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-#pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-          const double bx = sX[(size_t)(16 * ct + l16) * KPL + kc + 4 * u + lg];
-#pragma unroll
-          for (int rt = 0; rt < 2; ++rt)
-            yacc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_cur[rt][u], bx, yacc[rt][ct], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) a_cur[rt][u] = a_nxt[rt][u];
-    }
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
-      const double ds = sDsp[16 * ct + l16];
-#pragma unroll
-      for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) yacc[rt][ct][e] += ds;
-    }
   }
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int rt = 0; rt < NRT; ++rt) {
+    if (rt >= nrt) continue;
+    const int rt16 = wave + 8 * rt;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int row = 32 * wave + 16 * rt + lg + 4 * e;
+      const int row = 16 * rt16 + lg + 4 * e;
       if (row >= p.n) continue;
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
@@ -679,6 +745,7 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
         p.y[(int64_t)(p.r0 + r) * p.ldy + out_index(p, tb, row)] = yacc[rt][ct][e];
       }
     }
+  }
 }
 
 }  // namespace vl1
@@ -1088,7 +1155,8 @@ constexpr int kVarOneRow = 1, kVarMorton = 2, kVarNoPhase = 4;
 static int tune_key(int form, int S, int Rc) { return (form * 8 + S) * 1024 + Rc; }
 
 static int default_variant(const vl1_plan* pl, int form) {
-  return (form == VL1_FORM_SPLIT_BF16 && pl->MP <= 128) ? kVarOneRow : 0;
+  (void)form;
+  return pl->MP <= 128 ? kVarOneRow : 0;
 }
 
 static int plan_variant(const vl1_plan* pl, int form, int S, int Rc) {
@@ -1212,7 +1280,6 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
   prm.phase = pl->phase.as<int32_t>();
   const int variant = plan_variant(pl, form, S, Rc);
   prm.use_phase = (variant & kVarNoPhase) ? 0 : 1;
-  const int waves = pl->MP / 32;
   const int grid = ((sch->n_tiles + 7) / 8) * 8;
   for (int r0 = 0; r0 < R; r0 += Rc) {
     prm.r0 = r0;
@@ -1223,11 +1290,17 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
       e = launch_split(S, one, dim3(grid), lds, s, prm);
       if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split launch: ") + hipGetErrorString(e));
     } else {
-      const size_t lds = ((size_t)kCols * (pl->KP + 2) + 2 * kCols) * 8;
-      static std::atomic<unsigned> dense_attr_done{0};
-      e = allow_full_lds((const void*)dense_f64_kernel, dense_attr_done);
+      const size_t lds = (size_t)2 * kCols * kDPitch * 8 + 6 * kCols * 8;
+      static std::atomic<unsigned> dense_attr_done[2];
+      const bool one = (variant & kVarOneRow) && pl->MP <= 128;
+      if (one) {
+        e = allow_full_lds((const void*)dense_f64_kernel<1>, dense_attr_done[1]);
+        if (e == hipSuccess) hipLaunchKernelGGL(dense_f64_kernel<1>, dim3(grid), dim3(512), lds, s, prm);
+      } else {
+        e = allow_full_lds((const void*)dense_f64_kernel<2>, dense_attr_done[0]);
+        if (e == hipSuccess) hipLaunchKernelGGL(dense_f64_kernel<2>, dim3(grid), dim3(512), lds, s, prm);
+      }
       if (e != hipSuccess) return fail(VL1_E_HIP, std::string("dense attr: ") + hipGetErrorString(e));
-      hipLaunchKernelGGL(dense_f64_kernel, dim3(grid), dim3(64 * waves), lds, s, prm);
     }
     e = hipGetLastError();
     if (e != hipSuccess) return fail(VL1_E_HIP, std::string("tile kernel launch: ") + hipGetErrorString(e));
@@ -1268,7 +1341,7 @@ int vl1_plan_tune(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rh
   int best = -1;
   float best_ms = 0.f;
   for (int v = 0; v < 8 && e == hipSuccess; ++v) {
-    if ((v & kVarOneRow) && (form != VL1_FORM_SPLIT_BF16 || pl->MP > 128)) continue;
+    if ((v & kVarOneRow) && pl->MP > 128) continue;
     if ((v & kVarMorton) && !multi_box) continue;
     pl->forced_variant = v;
     rc = vl1_plan_contract(pl, form, n_rhs, y, ldy, opts, stream);  // warm (and schedule)

@@ -367,3 +367,31 @@ def _balance(dim, leaves):
             leaves.discard((lv, ix))
             for ch in itertools.product((0, 1), repeat=dim):
                 leaves.add((lv + 1, tuple(2 * i + c for i, c in zip(ix, ch))))
+
+
+def list1_csr_gpu(dim, leaf_levels, leaf_index, stream=None):
+    """List-1 CSR (leaves touching each leaf, itself included, sorted by id)
+    built on the GPU by ``vl1_csr_builder_*``; returns CUDA int32 tensors
+    ``(starts[n + 1], lists[n_pairs])``.  Same result as the host builder of
+    :class:`BoxTree` (``_list1_csr``)."""
+    import torch
+
+    from . import _capi
+
+    L = _capi.lib()
+    lv = np.ascontiguousarray(leaf_levels, dtype=np.uint8)
+    ix = np.ascontiguousarray(leaf_index, dtype=np.int32).reshape(lv.size, dim)
+    handle = _capi.vp()
+    npairs = _capi.i64(0)
+    sh = _capi.stream_handle(stream)
+    _capi.check(L.vl1_csr_builder_create(_capi.C.byref(handle), int(dim), int(lv.size),
+                                         lv.ctypes.data, ix.ctypes.data, _capi.C.byref(npairs), sh),
+                "vl1_csr_builder_create")
+    try:
+        starts = torch.empty(lv.size + 1, dtype=torch.int32, device="cuda")
+        lists = torch.empty(max(int(npairs.value), 1), dtype=torch.int32, device="cuda")
+        _capi.check(L.vl1_csr_builder_fill(handle, starts.data_ptr(), lists.data_ptr(), sh),
+                    "vl1_csr_builder_fill")
+    finally:
+        L.vl1_csr_builder_destroy(handle)
+    return starts, lists[: int(npairs.value)]

@@ -171,9 +171,9 @@ def test_autotune_keeps_results(cuda, form):
     fd = torch.as_tensor(f, device="cuda").contiguous()
     info = plan.tune(fd, form=form, n_slices=5, reps=2)
     print("tuned:", info)
-    valid = {0, 2, 4, 6} | ({1, 3, 5, 7} if form == "split" else set())
-    assert set(info["times_ms"]) <= valid and info["variant"] in info["times_ms"]
-    assert len(info["times_ms"]) >= 4
+    # 36 nodes -> 64-row blocks: one row tile per wave is a candidate in both
+    # forms, and several boxes per tile make both tile orders candidates
+    assert set(info["times_ms"]) == set(range(8)) and info["variant"] in info["times_ms"]
     got = plan.eval(fd, form=form, n_slices=5).cpu().numpy()
     ref = _oracle(tr, tables, f, "per_level")
     l2, linf = rel_errors(got, ref)

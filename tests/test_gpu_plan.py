@@ -179,3 +179,22 @@ def test_autotune_keeps_results(cuda, form):
     l2, linf = rel_errors(got, ref)
     tol = TOL64 if form == "f64" else TOLSPLIT
     assert l2 <= tol and linf <= tol, (l2, linf)
+
+
+@pytest.mark.parametrize("form,tol", [("f64", TOL64), ("split", TOLSPLIT)])
+def test_c2_style_adaptive_canonical(cuda, form, tol):
+    """A small instance of the adaptive 3D workload: q=5, refined while the
+    interpolation error of the seed-0 8-Gaussian density exceeds 1e-4 (depth
+    2..4), 2:1 balanced, one canonical Laplace table; GPU-built table."""
+    print(f"case: C2-style adaptive 3D q=5 form={form}", flush=True)
+    fden = (lambda P: densities.gaussian_mixture(P, 0))
+    tr = vtree.adaptive_tree_batched(3, 5, vtree.interpolation_refiner(fden, 3, 5, 1e-4),
+                                     min_level=2, max_level=4)
+    assert tr.nlevels >= 4
+    tables = [_laplace_table(3, 5)]
+    f = densities.gaussian_mixture_batch(tr.points, [0, 1])
+    got = _plan_eval(tr, tables, f, form, 5)
+    ref = _oracle(tr, tables, f, "canonical")
+    l2, linf = rel_errors(got, ref)
+    print(f"{form}: rel L2 {l2:.2e} Linf {linf:.2e}")
+    assert l2 <= tol and linf <= tol, (l2, linf)

@@ -570,15 +570,13 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
 // per wave load) for the next chunk are double-buffered in registers and
 // loaded at the top of the chunk.  Each step ends by adding the column's
 // 2D log displacement.
-constexpr int kDPitch = 33;
+constexpr int kDPitch = 34;
 
 template <int NRT>
 __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* sX = reinterpret_cast<double*>(smem_raw);          // [2][64][kDPitch]
-  double* sAl = sX + 2 * kCols * kDPitch;                    // [2][64]
-  double* sDsp = sAl + 2 * kCols;                            // [2][64]
-  int64_t* sBase = reinterpret_cast<int64_t*>(sDsp + 2 * kCols);  // [2][64]
+  double* sX = reinterpret_cast<double*>(smem_raw);  // [2][64][kDPitch]
+  double* sDsp = sX + 2 * kCols * kDPitch;           // [2][64]
 
   const int tile = xcd_tile(blockIdx.x, p.n_tiles);
   if (tile >= p.n_tiles) return;
@@ -594,48 +592,57 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
   const int rot = phase_start(p, st0, st1, xcd);
   auto step_of = [&](int k) { int st = st0 + rot + k; return st >= st1 ? st - nst : st; };
 
-  auto load_cols = [&](int st, int slot) {
+  // step displacements per column (2D log term), read at the step's end
+  auto load_disp = [&](int st, int slot) {
     if (threadIdx.x < kCols) {
       const int c = threadIdx.x;
-      double al = 0.0, ds = 0.0;
-      int64_t base = -1;
+      double ds = 0.0;
       if (c < ncols) {
         const int b = c / p.Rc, r = c % p.Rc;
         const int sb = p.step_src[st * p.B + b];
-        if (sb >= 0) {
-          const int64_t ci = (int64_t)sb * p.R + p.r0 + r;
-          al = p.step_alpha[st * p.B + b];
-          ds = p.step_disp[st * p.B + b] * p.nmdot[ci];
-          base = ci * p.KP;
-        }
+        if (sb >= 0) ds = p.step_disp[st * p.B + b] * p.nmdot[(int64_t)sb * p.R + p.r0 + r];
       }
-      sAl[slot * kCols + c] = al;
       sDsp[slot * kCols + c] = ds;
-      sBase[slot * kCols + c] = base;
     }
   };
-  // chunk kc of the step in descriptor slot cs: 64 columns x 32 doubles =
-  // 1024 16-byte vectors, 2 per thread
-  auto fetch_chunk = [&](int cs, int kc, uint4 (&buf)[2]) {
+  // A thread moves 2 of the 1024 16-byte vectors of a chunk (64 columns x
+  // 32 doubles).  It keeps its columns' density row offset and pair scaling
+  // for the current step and, loaded one step ahead, for the next step in
+  // registers, so a chunk fetch -- the next step's first one included --
+  // never waits on a schedule load and needs no barrier.
+  auto col_desc = [&](int st, int64_t (&off)[2], double (&al)[2]) {
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      const int u = threadIdx.x + 512 * m;
-      const int col = u / 16, v = u % 16;
-      const int64_t base = sBase[cs * kCols + col];
-      buf[m] = base >= 0 ? *reinterpret_cast<const uint4*>(p.XF + base + kc * 32 + 2 * v)
-                         : make_uint4(0, 0, 0, 0);
+      const int col = (threadIdx.x + 512 * m) / 16;
+      off[m] = -1;
+      al[m] = 0.0;
+      if (col < ncols) {
+        const int b = col / p.Rc, r = col % p.Rc;
+        const int sb = p.step_src[st * p.B + b];
+        if (sb >= 0) {
+          off[m] = ((int64_t)sb * p.R + p.r0 + r) * p.KP;
+          al[m] = p.step_alpha[st * p.B + b];
+        }
+      }
     }
   };
-  auto store_chunk = [&](int ring, int cs, const uint4 (&buf)[2]) {
+  auto fetch_chunk = [&](const int64_t (&off)[2], int kc, uint4 (&buf)[2]) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int v = (threadIdx.x + 512 * m) % 16;
+      buf[m] = off[m] >= 0 ? *reinterpret_cast<const uint4*>(p.XF + off[m] + kc * 32 + 2 * v)
+                           : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store_chunk = [&](int ring, const uint4 (&buf)[2], const double (&al)[2]) {
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
       const int u = threadIdx.x + 512 * m;
       const int col = u / 16, v = u % 16;
       const double2 d = *reinterpret_cast<const double2*>(&buf[m]);
-      const double al = sAl[cs * kCols + col];
       double* dst = sX + ((size_t)ring * kCols + col) * kDPitch + 2 * v;
-      dst[0] = d.x * al;
-      dst[1] = d.y * al;
+      dst[0] = d.x * al[m];
+      dst[1] = d.y * al[m];
     }
   };
 
@@ -652,12 +659,15 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
     return p.MF + (((int64_t)blk * (p.MP / 16) + rt16) * (p.KP / 4) + k4) * 64 + lane;
   };
   double ac[NRT][8], an[NRT][8];  // current / next chunk's table fragments
+  int64_t offC[2] = {-1, -1}, offN[2] = {-1, -1};
+  double alC[2] = {0.0, 0.0}, alN[2] = {0.0, 0.0};
   if (nst > 0) {
-    load_cols(step_of(0), 0);
-    __syncthreads();
+    load_disp(step_of(0), 0);
+    col_desc(step_of(0), offC, alC);
+    if (nst > 1) col_desc(step_of(1), offN, alN);
     uint4 buf[2];
-    fetch_chunk(0, 0, buf);
-    store_chunk(0, 0, buf);
+    fetch_chunk(offC, 0, buf);
+    store_chunk(0, buf, alC);
     const int blk = p.step_block[step_of(0)];
     if (threadIdx.x == 0) __builtin_nontemporal_store(blk, p.phase + xcd);
 #pragma unroll
@@ -678,10 +688,18 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
     int nblk = blk;
     if (has_next && next_step) {
       nblk = p.step_block[step_of(k + 1)];
-      load_cols(step_of(k + 1), cslot ^ 1);  // read after the barrier below
+      load_disp(step_of(k + 1), cslot ^ 1);  // read at the end of step k + 1
+    }
+    if (kc == 0 && k > 0) {  // step k begins: its descriptors, then load step k+1's
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        offC[m] = offN[m];
+        alC[m] = alN[m];
+      }
+      if (k + 1 < nst) col_desc(step_of(k + 1), offN, alN);
     }
     uint4 xnext[2];
-    if (has_next && !next_step) fetch_chunk(cslot, nkc_, xnext);
+    if (has_next) fetch_chunk(next_step ? offN : offC, nkc_, xnext);
     if (has_next) {
 #pragma unroll
       for (int rt = 0; rt < NRT; ++rt)
@@ -720,11 +738,9 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
       }
       if (has_next && threadIdx.x == 0) __builtin_nontemporal_store(nblk, p.phase + xcd);
     }
-    __syncthreads();  // ring[ring ^ 1] free, next descriptors visible
-    if (has_next) {
-      if (next_step) fetch_chunk(cslot ^ 1, 0, xnext);
-      store_chunk(ring ^ 1, next_step ? cslot ^ 1 : cslot, xnext);
-    }
+    // the other ring slot was last read in the previous iteration, which
+    // every wave has left: write the next chunk, then one barrier
+    if (has_next) store_chunk(ring ^ 1, xnext, next_step ? alN : alC);
     __syncthreads();
   }
 #pragma unroll

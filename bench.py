@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--rhs", type=int, default=64)
     ap.add_argument("--form", default="auto", choices=["auto", "f64", "split"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--slices", type=int, default=5)
     ap.add_argument("--only-split", action="store_true",
                     help="time only the split form (profiling runs)")
@@ -53,11 +53,12 @@ def parse():
                     help="do not time the plain FP64 kernel (profiling runs)")
     ap.add_argument("--no-table-build", action="store_true",
                     help="skip the near-field table build record (3D Laplace q=6, all cases)")
-    ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c2"],
+    ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c2", "c1"],
                     help="c3 (default): 3D Laplace uniform level 5 q=6, 64 RHS; "
                          "c4: 2D Helmholtz k=40, 8 adaptive trees depth 4..10, q=6, 16 complex "
                          "RHS in one plan; c2: 3D Laplace adaptive (interpolation error 1e-8, "
-                         "depth 3..7), q=5, 1 RHS")
+                         "depth 3..7), q=5, 1 RHS; c1: 2D Laplace uniform level 5, q=4, "
+                         "exp(-|x|^2/0.01), 1 RHS")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan autotuner (vl1_plan_tune) before timing")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -493,6 +494,67 @@ def run_c2(args):
         forms, extra)
 
 
+def run_c1(args):
+    """2D Laplace on [-1,1]^2, uniform level 5 (1,024 leaves), q=4 (16,384
+    points), f = exp(-|x|^2/0.01), one canonical table with the 2D log term,
+    one RHS.  Launch-latency bound (4.5 MFLOP); reported, not rooflined."""
+    import torch
+
+    from volumential_amd import _capi, tree as vtree
+    from volumential_amd.kernels import LaplaceKernel
+    from volumential_amd.payload import DevicePayload, prepare_table_data_and_entry_map
+    from volumential_amd.plan import List1Plan
+    from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(2)
+    tr = vtree.uniform_tree(2, 4, 5)
+    table = NearFieldInteractionTable(4, 2, LaplaceKernel(2), source_box_extent=2.0)
+    table.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=41))
+    f = torch.as_tensor(np.exp(-np.sum(tr.points**2, axis=0) / 0.01)[None, :], device="cuda").contiguous()
+    y = torch.zeros_like(f)
+    n = 16
+    flops = 2.0 * tr.n_list1_pairs * n * n
+    stream = torch.cuda.current_stream()
+    sh = int(stream.cuda_stream)
+    runner = F64Runner(tr, table, f, y)
+    for _ in range(max(args.warmup, 1)):
+        runner(sh)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = max(args.steps, 20)
+    e0.record(stream)
+    for _ in range(reps):
+        runner(sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_plain = e0.elapsed_time(e1) / reps
+    log(f"plain f64: {ms_plain * 1e3:.1f} us/eval, {flops / (ms_plain * 1e-3) / 1e9:.1f} GFLOP/s")
+    data, nmlz, _, _, _, _ = prepare_table_data_and_entry_map([table])
+    payload = DevicePayload(2, 4, data, nmlz, table.case_indices, table.arithmetic_metadata())
+    plan = List1Plan(tr, payload, scaling_mode=_capi.SCALING_CANONICAL, scaling_power=2,
+                     log_displacement=True, table_root_extent=2.0)
+    forms = _plan_forms(plan, f, 1, tr.n_points, flops, reps, max(args.warmup, 1), args.slices, False)
+    forms["plain_f64"] = {"ms": ms_plain, "value": tr.n_points / (ms_plain * 1e-3) / 1e9,
+                          "gflops": flops / (ms_plain * 1e-3) / 1e9}
+    extra = {"speedup_plain_vs_split": forms["split"]["ms"] / ms_plain}
+    if not args.no_cpu_baseline:
+        extra["cpu_baseline"] = cpu_baseline(tr, table, f.cpu().numpy(), min(args.cpu_seconds, 5.0))
+    return _line(
+        forms["plain_f64"]["value"], ms_plain, reps, args.warmup, "f64",
+        "synthetic (exp(-|x|^2/0.01) on the nodes, GPU-built Laplace 2D table)",
+        {"workload": "2D Laplace uniform level 5, q=4, 1 RHS (plain FP64)", "boxes": tr.nboxes,
+         "target_points": tr.n_points, "rhs": 1, "list1_pairs": tr.n_list1_pairs,
+         "form": "plain_f64", "parallelism": "single GPU"},
+        {"bound": "latency", "achieved": flops / (ms_plain * 1e-3) / 1e12, "peak": PEAK_FP64_TFLOPS,
+         "unit": "TFLOP/s", "frac": flops / (ms_plain * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+         "traffic": None, "kernel": "near_field_f64_kernel<unsigned short, 1, false>",
+         "kernel_ms": ms_plain, "flops_per_launch": flops},
+        forms, extra)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -503,6 +565,9 @@ def main():
         return
     if args.workload == "c2":
         print(json.dumps(run_c2(args)), flush=True)
+        return
+    if args.workload == "c1":
+        print(json.dumps(run_c1(args)), flush=True)
         return
     import torch
 

@@ -1,5 +1,6 @@
-"""GPU parity of the plain FP64 list-1 evaluation (vl1_near_field_f64)
-against the CPU oracles, through the wrangler's eval_direct drop-in.
+"""GPU parity of the wrangler's eval_direct drop-in against the CPU oracles,
+through both list-1 routes: the cached FP64 plan (default, vl1_plan_* with
+FORM_F64) and the plain FP64 kernel (vl1_near_field_f64).
 
 Tolerance (north star, FP64 form): rel-L2 and rel-Linf <= 1e-12 per RHS.
 """
@@ -19,8 +20,14 @@ TOL = 1e-12
 pytestmark = pytest.mark.gpu
 
 
-def _run(tr, tables, coefs, **kw):
-    w = NearFieldWrangler(tr, tables if len(tables) > 1 else tables[0], **kw)
+@pytest.fixture(params=["plan", "plain"])
+def method(request):
+    return request.param
+
+
+def _run(tr, tables, coefs, method, **kw):
+    w = NearFieldWrangler(tr, tables if len(tables) > 1 else tables[0], direct_method=method,
+                          **kw)
     c = torch.as_tensor(coefs, device="cuda")
     if c.dim() == 1:
         pot, _ = w.eval_direct(tr.target_boxes, tr.neighbor_source_boxes_starts,
@@ -42,26 +49,26 @@ def _oracle(tr, tables, coefs, mode, power=2, logd=False):
     )
 
 
-def test_laplace2d_q4_level5_log_term(cuda):
+def test_laplace2d_q4_level5_log_term(cuda, method):
     """C1 geometry: 2D, q=4, uniform level 5 (1024 boxes), canonical scaling
     with the FP64 log displacement, single RHS."""
-    print("case: 2D q=4 level 5, canonical + log", flush=True)
+    print(f"case: 2D q=4 level 5, canonical + log, {method}", flush=True)
     tr = vtree.uniform_tree(2, 4, 5)
     tables = synthetic_table_levels(2, 4)
     f = np.exp(-np.sum(tr.points**2, axis=0) / 0.01)
-    got = _run(tr, tables, f)
+    got = _run(tr, tables, f, method)
     ref = _oracle(tr, tables, f, "canonical", 2, True)[0]
     l2, linf = rel_errors(got, ref)
     assert l2 <= TOL and linf <= TOL, (l2, linf)
 
 
 @pytest.mark.parametrize("n_rhs", [1, 3, 6, 20])
-def test_laplace3d_q6_level2_batched(cuda, n_rhs):
-    print(f"case: 3D q=6 level 2, R={n_rhs}", flush=True)
+def test_laplace3d_q6_level2_batched(cuda, n_rhs, method):
+    print(f"case: 3D q=6 level 2, R={n_rhs}, {method}", flush=True)
     tr = vtree.uniform_tree(3, 6, 2)
     tables = synthetic_table_levels(3, 6, seed=1)
     f = densities.gaussian_mixture_batch(tr.points, range(1000, 1000 + n_rhs))
-    got = _run(tr, tables, f if n_rhs > 1 else f[0])
+    got = _run(tr, tables, f if n_rhs > 1 else f[0], method)
     meta = orbit_ref.case_metadata(3, 6)
     ref = ocpu.near_field(tr, tables[0].compact_values(), tables[0].mode_normalizers, meta, f,
                           mode="canonical", table_root_extent=2.0)
@@ -70,32 +77,32 @@ def test_laplace3d_q6_level2_batched(cuda, n_rhs):
 
 
 @pytest.mark.parametrize("dim,q", [(2, 3), (2, 6), (3, 2), (3, 5)])
-def test_adaptive_per_level_tables(cuda, dim, q):
-    print(f"case: adaptive {dim}D q={q} per-level tables", flush=True)
+def test_adaptive_per_level_tables(cuda, dim, q, method):
+    print(f"case: adaptive {dim}D q={q} per-level tables, {method}", flush=True)
     tr = adaptive(dim, q)
     tables = synthetic_table_levels(dim, q, n_tables=tr.nlevels, seed=5)
     f = densities.gaussian_mixture_batch(tr.points, [11, 12])
-    got = _run(tr, tables, f)
+    got = _run(tr, tables, f, method)
     ref = _oracle(tr, tables, f, "per_level")
     l2, linf = rel_errors(got, ref)
     assert l2 <= TOL and linf <= TOL, (l2, linf)
 
 
 @pytest.mark.parametrize("dim,q", [(2, 4), (3, 3)])
-def test_adaptive_canonical_scaling(cuda, dim, q):
-    print(f"case: adaptive {dim}D q={q} canonical", flush=True)
+def test_adaptive_canonical_scaling(cuda, dim, q, method):
+    print(f"case: adaptive {dim}D q={q} canonical, {method}", flush=True)
     tr = adaptive(dim, q, max_level=5)
     tables = synthetic_table_levels(dim, q, seed=6)
     f = densities.gaussian_mixture_batch(tr.points, [21, 22, 23])
-    got = _run(tr, tables, f)
+    got = _run(tr, tables, f, method)
     ref = _oracle(tr, tables, f, "canonical", 2, dim == 2)
     l2, linf = rel_errors(got, ref)
     assert l2 <= TOL and linf <= TOL, (l2, linf)
 
 
-def test_permuted_box_local_ids(cuda):
+def test_permuted_box_local_ids(cuda, method):
     """Particles inside a box in any order (source_mode_ids/target_point_ids)."""
-    print("case: permuted local ids", flush=True)
+    print(f"case: permuted local ids, {method}", flush=True)
     tr = vtree.uniform_tree(2, 3, 3)
     rng = np.random.default_rng(9)
     n = tr.n_q_points
@@ -103,21 +110,21 @@ def test_permuted_box_local_ids(cuda):
     tr.local_ids = perm
     tables = synthetic_table_levels(2, 3, seed=2)
     f = densities.gaussian_mixture_batch(tr.points, [31, 32])
-    got = _run(tr, tables, f)
+    got = _run(tr, tables, f, method)
     ref = _oracle(tr, tables, f, "canonical", 2, True)
     l2, linf = rel_errors(got, ref)
     assert l2 <= TOL and linf <= TOL, (l2, linf)
 
 
-def test_only_listed_targets_overwritten(cuda):
+def test_only_listed_targets_overwritten(cuda, method):
     """Targets of unlisted boxes keep the caller's zeros."""
     from volumential_amd.list1 import NearFieldFromCSR  # noqa: F401
 
-    print("case: partial target list", flush=True)
+    print(f"case: partial target list, {method}", flush=True)
     tr = vtree.uniform_tree(2, 3, 3)
     tables = synthetic_table_levels(2, 3, seed=3)
     f = densities.gaussian_mixture(tr.points, 41)
-    w = NearFieldWrangler(tr, tables[0])
+    w = NearFieldWrangler(tr, tables[0], direct_method=method)
     sel = np.arange(0, tr.nboxes, 3)
     starts = np.zeros(sel.size + 1, np.int32)
     lists = []
@@ -140,18 +147,19 @@ def test_only_listed_targets_overwritten(cuda):
     assert l2 <= TOL and linf <= TOL
 
 
-def test_error_paths(cuda):
-    print("case: error paths", flush=True)
+def test_error_paths(cuda, method):
+    print(f"case: error paths, {method}", flush=True)
     tr = vtree.uniform_tree(2, 3, 2)
     tables = synthetic_table_levels(2, 3)
     f = torch.ones(tr.n_points, dtype=torch.float64, device="cuda")
     # a source box far away in the negative direction -> invalid case code
-    w = NearFieldWrangler(tr, tables[0])
+    w = NearFieldWrangler(tr, tables[0], direct_method=method)
     bad_lists = np.array([tr.nboxes - 1], np.int32)
     with pytest.raises(RuntimeError, match="invalid case id"):
         w.eval_direct(np.array([0], np.int32), np.array([0, 1], np.int32), bad_lists, f)
     # fixed single table (no inferred scaling) at a level that does not match
-    w2 = NearFieldWrangler(tr, tables[0], list1_extra_kwargs={"infer_kernel_scaling": False})
+    w2 = NearFieldWrangler(tr, tables[0], list1_extra_kwargs={"infer_kernel_scaling": False},
+                           direct_method=method)
     with pytest.raises(RuntimeError, match="level mismatch"):
         w2.eval_direct(tr.target_boxes, tr.neighbor_source_boxes_starts,
                        tr.neighbor_source_boxes_lists, f)
@@ -161,3 +169,43 @@ def test_error_paths(cuda):
     with pytest.raises(ValueError, match="quadrature points per active"):
         w.eval_direct(tr.target_boxes, tr.neighbor_source_boxes_starts,
                       tr.neighbor_source_boxes_lists, f)
+
+
+def test_plan_route_rhs_chunks_and_repeat_calls(cuda):
+    """The plan route splits R = 70 densities into tile-kernel chunks (64 + 6)
+    and reuses its cached plan on a second call; both agree with the plain
+    kernel to 1e-13 and the second call with the oracle to the FP64 bar."""
+    print("case: plan route, R=70 then R=3", flush=True)
+    tr = vtree.uniform_tree(2, 4, 4)
+    tables = synthetic_table_levels(2, 4, seed=8)
+    wp = NearFieldWrangler(tr, tables[0])
+    wq = NearFieldWrangler(tr, tables[0], direct_method="plain")
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    for seeds in (range(70), range(500, 503)):
+        f = torch.as_tensor(densities.gaussian_mixture_batch(tr.points, seeds), device="cuda")
+        a = wp.eval_direct_batched(*args, f)[0][0].cpu().numpy()
+        b = wq.eval_direct_batched(*args, f)[0][0].cpu().numpy()
+        l2, linf = rel_errors(a, b)
+        assert l2 <= 1e-13 and linf <= 1e-13, (len(seeds), l2, linf)
+    assert len(wp._plans) == 1
+    ref = _oracle(tr, tables, f.cpu().numpy(), "canonical", 2, True)
+    l2, linf = rel_errors(a, ref)
+    assert l2 <= TOL and linf <= TOL, (l2, linf)
+
+
+def test_plan_route_box_size_beyond_tiles(cuda):
+    """3D q=7 (343 nodes per box) is beyond the tile kernels: the plan route
+    hands the kernel to the plain kernel and still meets the FP64 bar."""
+    print("case: plan route, 3D q=7 -> plain", flush=True)
+    tr = vtree.uniform_tree(3, 7, 1)
+    tables = synthetic_table_levels(3, 7, seed=4)
+    f = densities.gaussian_mixture_batch(tr.points, [61])
+    w = NearFieldWrangler(tr, tables[0])
+    got = w.eval_direct(tr.target_boxes, tr.neighbor_source_boxes_starts,
+                        tr.neighbor_source_boxes_lists, torch.as_tensor(f[0], device="cuda"))[0]
+    assert w._plain_only == {repr(tables[0].integral_knl)}
+    meta = orbit_ref.case_metadata(3, 7)
+    ref = ocpu.near_field(tr, tables[0].compact_values(), tables[0].mode_normalizers, meta, f,
+                          mode="canonical", table_root_extent=2.0)
+    l2, linf = rel_errors(got[0].cpu().numpy(), ref[0])
+    assert l2 <= TOL and linf <= TOL, (l2, linf)

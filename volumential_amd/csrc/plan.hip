@@ -1072,7 +1072,7 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
       const int cid = (vid >= 0 && vid < (int64_t)pay->case_indices.size()) ? (int)pay->case_indices[vid] : -1;
       if (cid < 0) {
         delete pl;
-        return fail(VL1_E_ARG, "list-1 case encoding produced an invalid case id (target box " +
+        return fail(VL1_E_COVERAGE, "list-1 case encoding produced an invalid case id (target box " +
                                    std::to_string(tb) + ", source box " + std::to_string(sb) + ")");
       }
       int lev = 0;

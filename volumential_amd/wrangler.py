@@ -71,12 +71,21 @@ class NearFieldWrangler:
         dtype=np.float64,
         list1_extra_kwargs=None,
         helmholtz_split=False,
+        direct_method="plan",
     ):
+        """``direct_method``: how ``eval_direct`` evaluates list 1 --
+        ``"plan"`` (default) through a plan cached per list-1 CSR, FP64
+        MFMA form (the same sums as the reference kernel to ~1e-15, at a
+        fraction of the per-call cost); ``"plain"`` through the plain FP64
+        kernel with the reference's per-call semantics (``NearFieldFromCSR``)."""
         if not getattr(tree, "sources_are_targets", False):
             raise ValueError(
                 "table-based near-field evaluation requires sources and targets "
                 "to coincide (tree.sources_are_targets=True)"
             )
+        if direct_method not in ("plan", "plain"):
+            raise ValueError("direct_method must be 'plan' or 'plain'")
+        self.direct_method = direct_method
         self.tree = tree
         self.dtype = np.dtype(dtype)
         if helmholtz_split:
@@ -116,6 +125,7 @@ class NearFieldWrangler:
                 )
         self._evaluators = {}
         self._plans = {}
+        self._plain_only = set()  # kernels the plan route cannot take
 
     @staticmethod
     def _as_list(t):
@@ -150,6 +160,20 @@ class NearFieldWrangler:
                     "Insufficient list of tables: the coarsest level mesh cells at level "
                     f"{min_lev} are not covered."
                 )
+        if self.direct_method == "plan" and kname not in self._plain_only:
+            key = _csr_key(out_kernel, target_boxes, starts, lists)
+            if key not in self._plans:  # host layout checks, once per CSR
+                _validate_layout(self.tree, np.asarray(_host(target_boxes), dtype=np.int64),
+                                 np.asarray(_host(lists), dtype=np.int64), t0.n_q_points)
+                try:
+                    self.get_plan(out_kernel, target_boxes, starts, lists)
+                except NotImplementedError:
+                    # box sizes / the complex 2D log term the tile kernels do not
+                    # take: the plain kernel evaluates this kernel from now on
+                    self._plain_only.add(kname)
+            if kname not in self._plain_only:
+                return self._eval_plan(out_pot, out_kernel, target_boxes, starts, lists,
+                                       mode_coefs)
         data, nmlz, _, _, _, info = prepare_table_data_and_entry_map(tables)
         shapes = {
             "n_tables": len(tables),
@@ -201,6 +225,24 @@ class NearFieldWrangler:
         )
         return res
 
+    def _eval_plan(self, out_pot, out_kernel, target_boxes, starts, lists, mode_coefs):
+        """FP64 plan evaluation into ``out_pot`` (targets of listed boxes
+        overwritten), in RHS chunks the tile kernels take (<= 64, or whole
+        multiples of 64)."""
+        mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
+            np.asarray(mode_coefs))
+        mc = mc.to(device="cuda", dtype=self._dtype(out_kernel))
+        one = mc.dim() == 1
+        mc = (mc.unsqueeze(0) if one else mc).contiguous()
+        out = out_pot.unsqueeze(0) if one else out_pot
+        plan = self.get_plan(out_kernel, target_boxes, starts, lists)
+        r = int(mc.shape[0])
+        bulk = r - r % 64 if r > 64 else r
+        plan.eval(mc[:bulk], out=out[:bulk], form="f64")
+        if bulk < r:
+            plan.eval(mc[bulk:], out=out[bulk:], form="f64")
+        return out_pot
+
     def eval_direct(self, target_boxes, neighbor_source_boxes_starts, neighbor_source_boxes_lists,
                     mode_coefs):
         pot = self.output_zeros()
@@ -235,10 +277,8 @@ class NearFieldWrangler:
         from .payload import DevicePayload
         from .plan import List1Plan
 
-        tb = np.ascontiguousarray(_host(target_boxes), dtype=np.int32)
-        st = np.ascontiguousarray(_host(neighbor_source_boxes_starts), dtype=np.int32)
-        ls = np.ascontiguousarray(_host(neighbor_source_boxes_lists), dtype=np.int32)
-        key = (repr(out_kernel), tb.tobytes(), st.tobytes(), ls.tobytes())
+        key = _csr_key(out_kernel, target_boxes, neighbor_source_boxes_starts,
+                       neighbor_source_boxes_lists)
         if key in self._plans:
             return self._plans[key]
         tables = self.near_field_table[repr(out_kernel)]
@@ -262,7 +302,9 @@ class NearFieldWrangler:
             table_root_extent=t0.source_box_extent,
             table_starting_level=self.table_starting_level(tables),
             case_encoding_bias=nf.extra_kwargs["case_encoding_bias"],
-            target_boxes=tb, starts=st, lists=ls,
+            target_boxes=np.frombuffer(key[1], dtype=np.int32),
+            starts=np.frombuffer(key[2], dtype=np.int32),
+            lists=np.frombuffer(key[3], dtype=np.int32),
         )
         self._plans[key] = plan
         return plan
@@ -288,3 +330,10 @@ def _host(a):
     if isinstance(a, torch.Tensor):
         return a.detach().cpu().numpy()
     return np.asarray(a)
+
+
+def _csr_key(out_kernel, target_boxes, starts, lists):
+    """Cache key of a plan: the kernel and the int32 bytes of the CSR."""
+    return (repr(out_kernel),) + tuple(
+        np.ascontiguousarray(_host(a), dtype=np.int32).tobytes()
+        for a in (target_boxes, starts, lists))

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 1
+#define VL1_ABI_VERSION 2
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -50,7 +50,12 @@ typedef enum vl1_status {
 typedef enum vl1_kernel_kind {
   VL1_KERNEL_CONSTANT = 0,  /* G = 1 */
   VL1_KERNEL_LAPLACE = 1,   /* 2D: -log|d|/(2 pi); 3D: 1/(4 pi |d|) */
-  VL1_KERNEL_HELMHOLTZ = 2  /* 2D: (i/4) H0^(1)(k|d|); 3D: e^{ik|d|}/(4 pi |d|) */
+  VL1_KERNEL_HELMHOLTZ = 2, /* 2D: (i/4) H0^(1)(k|d|); 3D: e^{ik|d|}/(4 pi |d|) */
+  /* one axis derivative of Laplace (sumpy AxisTargetDerivative /
+   * AxisSourceDerivative(axis, LaplaceKernel(dim)), table build only):
+   * target dG/dd_a = 2D -d_a/(2 pi |d|^2), 3D -d_a/(4 pi |d|^3); source = minus */
+  VL1_KERNEL_LAPLACE_TARGET_DERIV = 3,
+  VL1_KERNEL_LAPLACE_SOURCE_DERIV = 4
 } vl1_kernel_kind;
 
 /* list1.py:361-437 (get_kernel_scaling_policy) */
@@ -86,6 +91,11 @@ typedef struct vl1_payload_desc {
   const int8_t* case_axis_sign;        /* [n_cases][dim] */
   const uint8_t* case_axis_group;      /* [n_cases][dim] */
   const int32_t* case_value_offsets;   /* [n_case_orbits + 1] */
+  /* [dim] or NULL (all zero): arithmetic_axis_sign_power of the
+   * signed-arithmetic-orbit kind (orbit_arithmetic.py:176-369,
+   * list1.py:799-832).  An entry whose canonicalisation flips an axis with
+   * sign power 1 reads its compact value negated. */
+  const uint8_t* axis_sign_power;
 } vl1_payload_desc;
 
 int vl1_payload_create(vl1_payload** out, const vl1_payload_desc* desc, void* stream);
@@ -224,6 +234,7 @@ typedef struct vl1_duffy_args {
   const double* bary_w;          /* [quad_order] barycentric weights (validated; the kernel
                                     evaluates the basis in product form) */
   double* result;                /* [n_entries] (x2 interleaved if Helmholtz) */
+  int deriv_axis;                /* derivative axis of the *_DERIV kinds */
 } vl1_duffy_args;
 
 int vl1_duffy_build(const vl1_duffy_args* a, void* stream);
@@ -253,6 +264,12 @@ int vl1_csr_builder_destroy(vl1_csr_builder* b);
 int vl1_orbit_rows_host(int dim, int q, const uint8_t* perm, const int8_t* sign,
                         const uint8_t* group, const int32_t* s, const int32_t* t,
                         int64_t count, int64_t* out);
+/* Host evaluation of the entry sign the device kernels apply for signed
+ * (axis-derivative) kernels (orbit_arithmetic.py:372-432 entry_sign):
+ * out[i] = +1 / -1 for (s[i], t[i]) of one case, axis_sign_power [dim]. */
+int vl1_orbit_signs_host(int dim, int q, const uint8_t* perm, const int8_t* sign,
+                         const uint8_t* axis_sign_power, const int32_t* s, const int32_t* t,
+                         int64_t count, int8_t* out);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,8 @@ def near_field(tree, table_data, mode_nmlz, meta, coefs, *, mode, scaling_power=
         offs=np.ascontiguousarray(meta["case_value_offsets"], np.int32),
     )
     mode_code = {"canonical": 0, "fixed": 1, "per_level": 2}[mode]
+    power = np.asarray(meta.get("axis_sign_power", np.zeros(tree.dim)), dtype=np.int64)
+    sp_mask = int(sum(int(v & 1) << a for a, v in enumerate(power)))
     st = L.oracle_near_field_f64(
         C.c_int(tree.dim), C.c_int(tree.quad_order), C.c_int(len(tree.target_boxes)),
         C.c_int(tree.nboxes), _p(arrs["centers"]), _p(arrs["levels"]), _p(arrs["starts"]),
@@ -81,7 +83,7 @@ def near_field(tree, table_data, mode_nmlz, meta, coefs, *, mode, scaling_power=
         _p(nmlz), C.c_int(mode_code), C.c_int(scaling_power), C.c_int(int(log_displacement)),
         C.c_double(tree.root_extent), C.c_double(table_root_extent),
         C.c_int(table_starting_level), C.c_double(bias), C.c_int(R), _p(coefs),
-        C.c_int64(npts), _p(out), C.c_int64(npts), C.c_int(nthreads),
+        C.c_int64(npts), _p(out), C.c_int64(npts), C.c_int(nthreads), C.c_int(sp_mask),
     )
     if st != 0:
         raise RuntimeError("oracle: invalid list-1 case id")

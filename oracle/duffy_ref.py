@@ -14,8 +14,10 @@ Restates:
 * ``lagrange.py:45-92`` -- barycentric weights and basis evaluation.
 
 Closed forms used to pin the quadrature (independent of the reference):
-``box_potential_laplace3d`` (Newtonian potential of a uniform box) and
-``box_potential_laplace2d`` (logarithmic potential of a uniform rectangle).
+``box_potential_laplace3d`` (Newtonian potential of a uniform box),
+``box_potential_laplace2d`` (logarithmic potential of a uniform rectangle)
+and their gradients (``box_potential_gradient``, for the axis-derivative
+kernels).
 """
 
 from __future__ import annotations
@@ -85,6 +87,23 @@ def laplace(dim):
     if dim == 2:
         return lambda *d: -0.25 / np.pi * np.log(sum(c * c for c in d))
     return lambda *d: 0.25 / np.pi / np.sqrt(sum(c * c for c in d))
+
+
+def laplace_target_derivative(dim, axis):
+    """sumpy ``AxisTargetDerivative(axis, LaplaceKernel(dim))`` with its
+    global scaling: ``dG/dd_axis`` (d = target - source) of ``laplace(dim)``
+    -- 2D ``-d_a / (2 pi r^2)``, 3D ``-d_a / (4 pi r^3)``.  (sumpy is not
+    importable here; the formula is the derivative of ``laplace``, pinned by
+    finite differences of the closed-form box potentials in the tests.)"""
+    if dim == 2:
+        return lambda *d: -0.5 / np.pi * d[axis] / sum(c * c for c in d)
+    return lambda *d: -0.25 / np.pi * d[axis] / np.sqrt(sum(c * c for c in d)) ** 3
+
+
+def laplace_source_derivative(dim, axis):
+    """``AxisSourceDerivative(axis, LaplaceKernel(dim))``: ``-dG/dd_axis``."""
+    g = laplace_target_derivative(dim, axis)
+    return lambda *d: -g(*d)
 
 
 def constant(dim):
@@ -243,6 +262,25 @@ class TableGeometry:
 
 
 # ----------------------------------------------------------- closed forms
+def box_potential_gradient(x, lo, hi, axis, h=1e-3):
+    """``d/dx_axis`` of the closed-form uniform-box Laplace potential
+    (``box_potential_laplace2d/3d``), by the 4th-order central difference
+    with step ``h * max(hi - lo)``; the potential is smooth away from the
+    box surface, so for a point at distance >> h from it the error is
+    O(h^4) + rounding O(eps / h), about 1e-12 relative."""
+    dim = len(x)
+    f = box_potential_laplace2d if dim == 2 else box_potential_laplace3d
+    step = h * float(np.max(np.asarray(hi) - np.asarray(lo)))
+
+    def at(k):
+        y = np.array(x, dtype=np.float64)
+        y[axis] += k * step
+        return f(y, lo, hi)
+
+    return (8.0 * (at(1) - at(-1)) - (at(2) - at(-2))) / (12.0 * step)
+
+
+
 def box_potential_laplace3d(x, lo, hi):
     """int_box 1/(4 pi |x - y|) dy for the box [lo, hi] (closed form).
 

@@ -8,7 +8,8 @@
  * encoding (list1.py:439-459, 1083-1091), the scaling policy
  * (list1.py:361-437), then for every (target node, source node) the
  * arithmetic orbit reconstruction of the compact row
- * (list1.py:604-902 / orbit_arithmetic.py:372-634) and a table gather.
+ * (list1.py:604-902 / orbit_arithmetic.py:372-634, with the entry sign of
+ * axis-derivative kernels) and a table gather.
  * Like the reference, each density (RHS) is a separate evaluation.
  */
 #define _GNU_SOURCE
@@ -19,8 +20,11 @@
 #include <omp.h>
 #endif
 
+/* Compact row of (s, t); *neg = 1 when the entry is read negated: an odd
+ * number of flipped axes whose input axis has its bit set in sp_mask (the
+ * axis-derivative sign of orbit_arithmetic.py:372-432). */
 static inline int64_t compact_row(int dim, int q, const uint8_t* perm, const int8_t* sign,
-                                  const uint8_t* group, int s, int t) {
+                                  const uint8_t* group, int sp_mask, int s, int t, int* neg) {
   int sr[3] = {0, 0, 0}, tr[3] = {0, 0, 0};
   for (int ax = dim - 1; ax >= 0; --ax) {
     sr[ax] = s % q;
@@ -29,6 +33,7 @@ static inline int64_t compact_row(int dim, int q, const uint8_t* perm, const int
     t /= q;
   }
   int pair[3];
+  *neg = 0;
   for (int o = 0; o < dim; ++o) {
     int in = perm[o];
     int si = sr[in], ti = tr[in];
@@ -37,6 +42,7 @@ static inline int64_t compact_row(int dim, int q, const uint8_t* perm, const int
     if (sg < 0 || (sg == 0 && (sf < si || (sf == si && tf < ti)))) {
       si = sf;
       ti = tf;
+      if ((sp_mask >> in) & 1) *neg ^= 1;
     }
     pair[o] = si * q + ti;
   }
@@ -87,7 +93,7 @@ int oracle_near_field_f64(int dim, int q, int n_tgt_boxes, int nboxes, const dou
                           const double* nmlz, int mode, int power, int log_disp,
                           double root_extent, double table_root_extent, int tsl, double bias,
                           int n_rhs, const double* coefs, int64_t ldf, double* result,
-                          int64_t ldy, int nthreads) {
+                          int64_t ldy, int nthreads, int sp_mask) {
   int bad = 0;
   const int n = (int)lround(pow((double)q, (double)dim));
 #ifdef _OPENMP
@@ -143,7 +149,9 @@ int oracle_near_field_f64(int dim, int q, int n_tgt_boxes, int nboxes, const dou
           const int sbeg = box_starts[sb], ns = box_counts[sb];
           for (int sid = 0; sid < ns; ++sid) {
             const int s = src_ids[sbeg + sid];
-            const double integ = T[compact_row(dim, q, pp, ss, gg, s, t)] * scal +
+            int neg;
+            const double tv = T[compact_row(dim, q, pp, ss, gg, sp_mask, s, t, &neg)];
+            const double integ = (neg ? -tv : tv) * scal +
                                  (log_disp ? dsc * nmlz[(int64_t)lev * n + sid] : 0.0);
             acc += f[sbeg + sid] * integ;
           }

@@ -2,8 +2,9 @@
 
 Restates volumential/list1.py:1057-1148 (kernel body), 439-459 and
 1083-1091 (float case-id encoding with bias and +0.5 truncation), 278-437
-(scaling policies) and 604-902 (arithmetic orbit reconstruction, via
-``orbit_ref.rows_for_case``).  Loops over (target box, source box) pairs in
+(scaling policies) and 604-902 (arithmetic orbit reconstruction and the
+entry sign of axis-derivative kernels, via ``orbit_ref.rows_for_case`` /
+``signs_for_case``).  Loops over (target box, source box) pairs in
 Python and contracts each pair's q^d x q^d block with numpy, so it is meant
 for small trees.  ``mode_coefs`` may hold several densities ``[R, n]``.
 """
@@ -94,9 +95,11 @@ def near_field(
             sbeg, ns = int(box_starts[sb]), int(box_counts[sb])
             smodes = np.asarray(source_mode_ids[sbeg : sbeg + ns])
             if cid not in rows_cache:
-                rows_cache[cid] = orbit_ref.rows_for_case(meta, cid, q, dim)
-            rows = rows_cache[cid][np.ix_(smodes, tmodes)]  # [ns, nt]
-            integ = table[lev][rows] * scal
+                rows_cache[cid] = (orbit_ref.rows_for_case(meta, cid, q, dim),
+                                   orbit_ref.signs_for_case(meta, cid, q, dim))
+            rmap, smap = rows_cache[cid]
+            rows = rmap[np.ix_(smodes, tmodes)]  # [ns, nt]
+            integ = table[lev][rows] * smap[np.ix_(smodes, tmodes)] * scal
             if mode == "canonical" and log_displacement:
                 disp = -0.5 / np.pi * scal * np.log(sext / table_root_extent) * nmlz[lev, :ns]
                 integ = integ + disp[:, None]

@@ -102,28 +102,38 @@ def encode_axes(axes, q):
     return r
 
 
-def case_axis_descriptors(case_vec):
+def case_axis_descriptors(case_vec, fixed_axes=()):
+    """``orbit_arithmetic.py:45-163`` with ``axis_groups`` = one group per
+    fixed (derivative) axis, then the free axes (``245-251``); the
+    directional-derivative branch is not restated."""
     vec = [int(v) for v in case_vec]
     dim = len(vec)
-    out_axes = tuple(range(dim))
-    free_order = sorted(out_axes, key=lambda i: (-abs(vec[i]), i))
+    fixed = sorted(set(fixed_axes))
+    specs = [(a,) for a in fixed]
+    free = tuple(a for a in range(dim) if a not in fixed)
+    if free:
+        specs.append(free)
     perm = [0] * dim
     sign = [0] * dim
     group = [0] * dim
-    last = None
-    cur = -1
-    for out_axis, in_axis in zip(out_axes, free_order):
-        v = vec[in_axis]
-        if last != abs(v):
-            cur += 1
-            last = abs(v)
-        perm[out_axis] = in_axis
-        sign[out_axis] = -1 if v > 0 else (1 if v < 0 else 0)
-        group[out_axis] = cur
+    nxt = 0
+    for out_axes in specs:
+        free_order = sorted(out_axes, key=lambda i: (-abs(vec[i]), i))
+        last = None
+        cur = nxt - 1
+        for out_axis, in_axis in zip(out_axes, free_order):
+            v = vec[in_axis]
+            if last != abs(v):
+                cur += 1
+                last = abs(v)
+            perm[out_axis] = in_axis
+            sign[out_axis] = -1 if v > 0 else (1 if v < 0 else 0)
+            group[out_axis] = cur
+        nxt = cur + 1
     return perm, sign, group
 
 
-def case_metadata(dim, q):
+def case_metadata(dim, q, fixed_axes=(), sign_power=None):
     vecs, case_indices = gallery(dim)
     n = len(vecs)
     perm = np.empty((n, dim), np.uint8)
@@ -131,7 +141,7 @@ def case_metadata(dim, q):
     group = np.empty((n, dim), np.uint8)
     canon = np.empty(n, np.int32)
     for cid, v in enumerate(vecs):
-        p, s, g = case_axis_descriptors(v)
+        p, s, g = case_axis_descriptors(v, fixed_axes)
         perm[cid], sign[cid], group[cid] = p, s, g
         cv = [0 if sg == 0 else v[pi] * sg for pi, sg in zip(p, s)]
         canon[cid] = case_indices[encode(cv)]
@@ -150,7 +160,8 @@ def case_metadata(dim, q):
         "case_axis_sign": sign,
         "case_axis_group": group,
         "case_value_offsets": offsets.astype(np.int32),
-        "axis_sign_power": np.zeros(dim, np.uint8),
+        "axis_sign_power": np.asarray(sign_power if sign_power is not None else [0] * dim,
+                                      np.uint8),
         "n_compact_entries": int(offsets[-1]),
     }
 
@@ -171,24 +182,38 @@ def value_count(sign, group, q):
 
 # --------------------------------------------------------- entry mapping
 def canonicalize(case_id, s, t, meta, q, dim):
+    sa, ta, groups, signs, _ = canonicalize_signed(case_id, s, t, meta, q, dim)
+    return sa, ta, groups, signs
+
+
+def canonicalize_signed(case_id, s, t, meta, q, dim):
+    """``orbit_arithmetic.py:372-432``, with the entry sign of axis
+    derivatives (directional derivatives not restated)."""
     s_raw = decode_axes(s, q, dim)
     t_raw = decode_axes(t, q, dim)
+    power = meta.get("axis_sign_power", [0] * dim)
     sa, ta, groups, signs = [], [], [], []
+    esign = 1
     for out in range(dim):
         ia = int(meta["case_axis_perm"][case_id, out])
         sg = int(meta["case_axis_sign"][case_id, out])
         sv, tv = s_raw[ia], t_raw[ia]
+        applied = 1
         if sg < 0:
             sv, tv = q - 1 - sv, q - 1 - tv
+            applied = -1
         elif sg == 0:
             fs, ft = q - 1 - sv, q - 1 - tv
             if (fs, ft) < (sv, tv):
                 sv, tv = fs, ft
+                applied = -1
+        if int(power[ia]):
+            esign *= applied
         sa.append(sv)
         ta.append(tv)
         groups.append(int(meta["case_axis_group"][case_id, out]))
         signs.append(sg)
-    return sa, ta, groups, signs
+    return sa, ta, groups, signs, esign
 
 
 def arithmetic_entry(case_id, s, t, meta, q, dim):
@@ -228,7 +253,8 @@ def rank_multiset(values, alpha):
 
 
 def compact_entry(case_id, s, t, meta, q, dim):
-    sa, ta, groups, signs = canonicalize(case_id, s, t, meta, q, dim)
+    """``(row, entry sign)``, ``orbit_arithmetic.py:566-634``."""
+    sa, ta, groups, signs, esign = canonicalize_signed(case_id, s, t, meta, q, dim)
     codes = [a * q + b for a, b in zip(sa, ta)]
     pc = q * q
     fpc = (pc + 1) // 2
@@ -241,7 +267,7 @@ def compact_entry(case_id, s, t, meta, q, dim):
         vals = sorted(codes[a] for a in axes)
         comp = comp * math.comb(alpha + len(axes) - 1, len(axes)) + rank_multiset(vals, alpha)
     rank = int(meta["case_orbit_ranks"][case_id])
-    return int(meta["case_value_offsets"][rank]) + comp, 1
+    return int(meta["case_value_offsets"][rank]) + comp, esign
 
 
 def representatives(meta, q, dim):
@@ -331,6 +357,29 @@ def rows_for_case(meta, case_id, q, dim):
     return int(meta["case_value_offsets"][rank]) + comp
 
 
+def signs_for_case(meta, case_id, q, dim):
+    """Dense ``signs[s, t]`` of one case in the device-code form of
+    ``list1.py:799-832`` (``entry_sign_factor``: the transform sign of every
+    axis whose permuted input axis has sign power 1)."""
+    n = q**dim
+    s = np.arange(n)[:, None] * np.ones((1, n), np.int64)
+    t = np.ones((n, 1), np.int64) * np.arange(n)[None, :]
+    sr = [(s // q ** (dim - 1 - a)) % q for a in range(dim)]
+    tr = [(t // q ** (dim - 1 - a)) % q for a in range(dim)]
+    perm = meta["case_axis_perm"][case_id].astype(int)
+    sign = meta["case_axis_sign"][case_id].astype(int)
+    power = meta.get("axis_sign_power", [0] * dim)
+    out = np.ones((n, n), np.int64)
+    for ax in range(dim):
+        sp, tp = sr[perm[ax]], tr[perm[ax]]
+        sfl, tfl = q - 1 - sp, q - 1 - tp
+        zf = (sign[ax] == 0) & ((sfl < sp) | ((sfl == sp) & (tfl < tp)))
+        tsign = np.where((sign[ax] < 0) | zf, -1, 1)
+        if int(power[perm[ax]]):
+            out = out * tsign
+    return out
+
+
 # ----------------------------------------------------- brute-force orbits
 def orbit_min_ids(dim, q):
     """Smallest full entry id of the orbit of every entry (dense, int64)."""
@@ -360,3 +409,42 @@ def orbit_min_ids(dim, q):
             )
             best = ids if best is None else np.minimum(best, ids)
     return best.reshape(nc * n * n)
+
+
+def group_transforms(dim, fixed_axes=(), sign_power=None):
+    """The symmetry group of a kernel as (axis signs, axis permutation,
+    value sign): ``nearfield_potential_table.py:55-140``
+    (``_kernel_symmetry_meta``) for axis derivatives -- a permutation must fix
+    every derivative axis, and a flip of a derivative axis negates values."""
+    power = list(sign_power) if sign_power is not None else [0] * dim
+    out = []
+    for sgn in itertools.product((-1, 1), repeat=dim):
+        for perm in itertools.permutations(range(dim)):
+            if any(perm[a] != a for a in fixed_axes):
+                continue
+            vs = 1
+            for a in range(dim):
+                if power[a]:
+                    vs *= sgn[a]
+            out.append((sgn, perm, vs))
+    return out
+
+
+def transform_entries(dim, q, sgn, perm):
+    """Full-id map of one group element: ``new_id[old_id]`` (int64, dense)."""
+    vecs, case_indices = gallery(dim)
+    n = q**dim
+    axes = np.array([decode_axes(m, q, dim) for m in range(n)], np.int64)
+    sgn = np.asarray(sgn)
+    perm = list(perm)
+    m_ax = axes.copy()
+    m_ax[:, sgn < 0] = q - 1 - m_ax[:, sgn < 0]
+    m_ax = m_ax[:, perm]
+    mode_map = np.zeros(n, np.int64)
+    for ax in range(dim):
+        mode_map = mode_map * q + m_ax[:, ax]
+    cv = (np.asarray(vecs, np.int64) * sgn)[:, perm]
+    case_map = np.array([case_indices[encode(v)] for v in cv], np.int64)
+    assert np.all(case_map >= 0)
+    ids = case_map[:, None, None] * n * n + mode_map[None, :, None] * n + mode_map[None, None, :]
+    return ids.reshape(-1)

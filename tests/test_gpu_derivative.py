@@ -1,0 +1,160 @@
+"""Axis-derivative kernels (signed-arithmetic-orbit tables) on the GPU.
+
+Kernels: ``AxisTargetDerivative(a, LaplaceKernel(d))`` and
+``AxisSourceDerivative(a, LaplaceKernel(d))`` -- the gradient tables the
+reference builds and evaluates through its signed arithmetic orbit
+(``orbit_arithmetic.py:176-369``, ``list1.py:604-902``,
+``nearfield_potential_table.py:55-140``).
+
+* table build: sampled canonical entries vs the oracle's per-entry Duffy
+  quadrature, |diff| / max(1, |T|) <= 1e-12; full-entry mode sums (through
+  the signed reconstruction) vs the gradient of the closed-form box
+  potential;
+* evaluation: plan FP64 / plain kernel (rel <= 1e-12) and the split plan
+  (<= 1e-10) vs the C oracle with the entry signs, uniform and adaptive
+  trees, canonical scaling (power 1);
+* physics: density 1 on a uniform tree -- every target's list-1 value equals
+  the summed gradients of its neighbours' closed-form box potentials.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu as ocpu
+from oracle import duffy_ref as D
+from oracle import orbit_ref
+from tests.gpu_helpers import adaptive, rel_errors
+from volumential_amd import densities, tree as vtree
+from volumential_amd.kernels import AxisSourceDerivative, AxisTargetDerivative, LaplaceKernel
+from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+from volumential_amd.wrangler import NearFieldWrangler
+
+pytestmark = pytest.mark.gpu
+
+_TABLES = {}
+
+
+def _kernel(kind, axis, dim):
+    cls = AxisTargetDerivative if kind == "target" else AxisSourceDerivative
+    return cls(axis, LaplaceKernel(dim))
+
+
+def _table(kind, axis, dim, q, extent=2.0, orders=(20, 61)):
+    key = (kind, axis, dim, q, extent, orders)
+    if key not in _TABLES:
+        t = NearFieldInteractionTable(q, dim, _kernel(kind, axis, dim), source_box_extent=extent)
+        t.build_table(DuffyBuildConfig(regular_quad_order=orders[0], radial_quad_order=orders[1]))
+        _TABLES[key] = t
+    return _TABLES[key]
+
+
+def _oracle_kernel(kind, axis, dim):
+    return (D.laplace_target_derivative if kind == "target" else D.laplace_source_derivative)(
+        dim, axis)
+
+
+@pytest.mark.parametrize("kind,axis,dim,q", [("target", 0, 2, 4), ("source", 1, 2, 3),
+                                             ("target", 2, 3, 3), ("source", 0, 3, 2)])
+def test_derivative_entries_match_oracle(cuda, kind, axis, dim, q):
+    print(f"case: {kind} derivative axis {axis}, {dim}D q={q} entries", flush=True)
+    reg, rad = 12, 41
+    t = _table(kind, axis, dim, q, 1.0, (reg, rad))
+    ids, vals = t.get_reduced_table_data()
+    assert t.arithmetic_metadata().axis_sign_power[axis] == 1
+    g = D.TableGeometry(dim, q, H=1.0)
+    rng = np.random.default_rng(dim * 11 + q + axis)
+    pick = np.unique(np.concatenate([[0, ids.size - 1], rng.integers(0, ids.size, 20)]))
+    knl = _oracle_kernel(kind, axis, dim)
+    for i in pick:
+        ref = g.entry(int(ids[i]), knl, reg, rad)
+        assert abs(vals[i] - ref) / max(1.0, abs(ref)) <= 1e-12, (ids[i], vals[i], ref)
+
+
+@pytest.mark.parametrize("dim,q,axis", [(2, 4, 1), (3, 3, 0)])
+def test_derivative_mode_sums_are_box_gradients(cuda, dim, q, axis):
+    """Full entries read back through rows and signs, summed over source
+    modes: the gradient of the closed-form uniform-box potential."""
+    print(f"case: {dim}D q={q} axis {axis} mode sums", flush=True)
+    t = _table("target", axis, dim, q, 1.0, (30, 91))
+    g = D.TableGeometry(dim, q, H=1.0)
+    n = t.n_q_points
+    cases = (0, 5, 16, 27) if dim == 2 else (0, 69, 100, 138)
+    for case in cases:
+        for tt in (0, n // 2, n - 1):
+            tot = sum(t.get_entry_data(case * n * n + s * n + tt) for s in range(n))
+            cf = D.box_potential_gradient(g.target_point(tt, case), [0.0] * dim, [1.0] * dim,
+                                          axis)
+            assert abs(tot - cf) <= 1e-10 * max(1.0, abs(cf)), (case, tt, tot, cf)
+
+
+def _eval(tr, table, f, route, form="f64"):
+    w = NearFieldWrangler(tr, table, direct_method="plain" if route == "plain" else "plan")
+    c = torch.as_tensor(f, device="cuda")
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists, c)
+    if route == "split":
+        pot, _ = w.eval_direct_plan(*args, form="split", n_slices=5)
+    else:
+        pot, _ = w.eval_direct_batched(*args)
+    return pot[0].cpu().numpy()
+
+
+def _ref(tr, table, kind, axis, f):
+    dim, q = tr.dim, tr.quad_order
+    power = [0] * dim
+    power[axis] = 1
+    meta = orbit_ref.case_metadata(dim, q, (axis,), power)
+    return ocpu.near_field(tr, table.compact_values(), table.mode_normalizers, meta, f,
+                           mode="canonical", scaling_power=1,
+                           table_root_extent=table.source_box_extent)
+
+
+@pytest.mark.parametrize("route,tol", [("plan", 1e-12), ("plain", 1e-12), ("split", 1e-10)])
+@pytest.mark.parametrize("kind,axis,dim,q,level", [("target", 0, 2, 4, 3),
+                                                   ("source", 1, 2, 5, 3),
+                                                   ("target", 2, 3, 3, 2)])
+def test_derivative_eval_uniform(cuda, route, tol, kind, axis, dim, q, level):
+    print(f"case: eval {kind} axis {axis} {dim}D q={q} L{level} via {route}", flush=True)
+    tr = vtree.uniform_tree(dim, q, level)
+    table = _table(kind, axis, dim, q)
+    f = densities.gaussian_mixture_batch(tr.points, [71, 72, 73])
+    got = _eval(tr, table, f, route)
+    ref = _ref(tr, table, kind, axis, f)
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= tol and linf <= tol, (l2, linf)
+
+
+@pytest.mark.parametrize("route,tol", [("plan", 1e-12), ("plain", 1e-12), ("split", 1e-10)])
+def test_derivative_eval_adaptive(cuda, route, tol):
+    """Mixed source levels: the canonical scaling (h/H)^1 of the gradient."""
+    print(f"case: eval target axis 1 adaptive 2D q=4 via {route}", flush=True)
+    tr = adaptive(2, 4, max_level=5)
+    table = _table("target", 1, 2, 4)
+    f = densities.gaussian_mixture_batch(tr.points, [81, 82])
+    got = _eval(tr, table, f, route)
+    ref = _ref(tr, table, "target", 1, f)
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= tol and linf <= tol, (l2, linf)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_unit_density_gives_box_potential_gradients(cuda, axis):
+    """Density 1 on a uniform 2D tree: the list-1 value at every target node
+    is the sum over its list-1 boxes of d/dx_axis of the closed-form box
+    potential -- table build, signed reconstruction and scaling together."""
+    print(f"case: unit density, gradient axis {axis}", flush=True)
+    tr = vtree.uniform_tree(2, 4, 2)
+    table = _table("target", axis, 2, 4, 2.0, (30, 91))
+    f = np.ones(tr.n_points)
+    got = _eval(tr, table, f[None], "plan")[0]
+    h = tr.root_extent / 2**2
+    exp = np.zeros(tr.n_points)
+    for i, tb in enumerate(tr.target_boxes):
+        beg, cnt = tr.box_target_starts[tb], tr.box_target_counts_nonchild[tb]
+        for sb in tr.neighbor_source_boxes_lists[tr.neighbor_source_boxes_starts[i]:
+                                                 tr.neighbor_source_boxes_starts[i + 1]]:
+            c = tr.box_centers[:, sb]
+            for k in range(beg, beg + cnt):
+                exp[k] += D.box_potential_gradient(tr.points[:, k], c - h / 2, c + h / 2, axis)
+    err = np.max(np.abs(got - exp)) / np.max(np.abs(exp))
+    assert err <= 1e-9, err

@@ -9,7 +9,9 @@
   of a uniform box at the target point;
 * symmetry: a non-canonical entry integrated directly equals the canonical
   value at its compact row (pins target-point geometry + orbit arithmetic);
-* 2D: the triangle form and the n-D form agree.
+* 2D: the triangle form and the n-D form agree;
+* axis derivatives of Laplace: mode sums equal the gradient of the
+  closed-form box potential, and direct entries follow the signed orbits.
 """
 
 import numpy as np
@@ -103,3 +105,49 @@ def test_symmetric_entries_equal_canonical(dim, q):
         direct = g.entry(int(e), knl, 20, 61)
         canon = g.entry(int(reps[row]), knl, 20, 61)
         assert abs(direct - canon) <= 1e-12 * max(1.0, abs(canon))
+
+
+# ---------------------------------------------------- axis derivatives
+@pytest.mark.parametrize("dim,q,case,axis", [(2, 3, 0, 0), (2, 3, 16, 1), (2, 3, 27, 0),
+                                             (3, 2, 0, 2), (3, 2, 138, 0), (3, 2, 100, 1)])
+def test_derivative_mode_sum_is_box_potential_gradient(dim, q, case, axis):
+    """Sum over source modes of target-derivative entries = d/dx_axis of the
+    closed-form uniform-box potential at the target; source derivative =
+    minus that.  Pins the derivative kernels' formula, sign and axis."""
+    H = 1.0
+    g = D.TableGeometry(dim, q, H=H)
+    n = g.n
+    tk = D.laplace_target_derivative(dim, axis)
+    sk = D.laplace_source_derivative(dim, axis)
+    reg, rad = (30, 91) if dim == 2 else (20, 61)
+    for t in (0, n - 1):
+        x = g.target_point(t, case)
+        tot = sum(g.entry(case * n * n + s * n + t, tk, reg, rad) for s in range(n))
+        cf = D.box_potential_gradient(x, [0.0] * dim, [H] * dim, axis)
+        assert abs(tot - cf) <= 1e-10 * max(1.0, abs(cf)), (t, tot, cf)
+        src = g.entry(case * n * n + t, sk, reg, rad)
+        assert abs(src + g.entry(case * n * n + t, tk, reg, rad)) <= 1e-15 * max(1.0, abs(src))
+
+
+@pytest.mark.parametrize("dim,q,axis", [(2, 3, 0), (2, 3, 1), (3, 2, 2)])
+def test_derivative_entries_follow_signed_orbits(dim, q, axis):
+    """A derivative entry integrated directly equals sign * the value at its
+    compact row's representative (signed-arithmetic-orbit reconstruction)."""
+    g = D.TableGeometry(dim, q, H=1.0)
+    power = [0] * dim
+    power[axis] = 1
+    m = orbit.build_case_metadata(dim, q, (axis,), power)
+    reps = orbit.enumerate_representatives(m)
+    n = q**dim
+    knl = D.laplace_target_derivative(dim, axis)
+    rng = np.random.default_rng(3)
+    seen_neg = False
+    for e in rng.integers(0, m.n_cases * n * n, 12):
+        c, s, t = e // (n * n), (e % (n * n)) // n, e % n
+        row = int(orbit.entry_rows(m, c, s, t))
+        sg = int(orbit.entry_signs(m, c, s, t))
+        seen_neg |= sg < 0
+        direct = g.entry(int(e), knl, 20, 61)
+        canon = g.entry(int(reps[row]), knl, 20, 61)
+        assert abs(direct - sg * canon) <= 1e-12 * max(1.0, abs(canon)), (e, direct, sg, canon)
+    assert seen_neg

@@ -107,3 +107,102 @@ def test_scalar_compact_entry_matches_vectorised():
         rows = orbit.entry_rows(m, c, s, t)
         ref = [orbit_ref.compact_entry(int(a), int(b), int(d), mo, q, dim)[0] for a, b, d in zip(c, s, t)]
         assert np.array_equal(rows, ref)
+
+
+# ----------------------------------------------- signed (axis-derivative)
+SIGNED_KERNELS = {
+    # name: (fixed axes, axis sign power) -- AxisTargetDerivative(a, Lap),
+    # AxisSourceDerivative(a, Lap), AxisTargetDerivative(0, AxisSource(1, Lap))
+    "target0": ((0,), (1, 0, 0)),
+    "source1": ((1,), (0, 1, 0)),
+    "target2": ((2,), (0, 0, 1)),
+    "mixed01": ((0, 1), (1, 1, 0)),
+}
+
+
+@pytest.mark.parametrize("dim,q", [(2, 2), (2, 3), (3, 2), (3, 3)])
+@pytest.mark.parametrize("kname", list(SIGNED_KERNELS))
+def test_signed_rows_and_signs_match_bruteforce_group(dim, q, kname):
+    """Under every element g of the kernel's symmetry group (permutations
+    fixing the derivative axes; value sign = product of the derivative axes'
+    flips), entry e and g(e) share a compact row and
+    sign(g(e)) = value_sign(g) * sign(e), unless e's orbit has an odd
+    stabilizer (then the entry is zero and any sign reconstructs it).
+    Distinct orbits get distinct rows, and every row is used
+    (test_nearfield_potential_table.py:2401-2597 checks the same invariants
+    against the reference's dense orbit tables)."""
+    fixed, power = SIGNED_KERNELS[kname]
+    if max(fixed) >= dim:
+        pytest.skip("derivative axis beyond dim")
+    power = power[:dim]
+    m = orbit.build_case_metadata(dim, q, fixed, power)
+    assert np.any(m.axis_sign_power)
+    n = q**dim
+    ids = np.arange(m.n_cases * n * n)
+    case, s, t = ids // (n * n), (ids % (n * n)) // n, ids % n
+    rows = orbit.entry_rows(m, case, s, t)
+    sg = orbit.entry_signs(m, case, s, t).astype(np.int64)
+    odd_row = np.zeros(m.n_compact_entries, bool)
+    orbit_min = ids.copy()
+    mism = np.zeros(ids.size, bool)
+    for sgn, perm, vs in orbit_ref.group_transforms(dim, fixed, power):
+        new = orbit_ref.transform_entries(dim, q, sgn, perm)
+        assert np.array_equal(rows[new], rows), (sgn, perm)
+        mism |= sg[new] != vs * sg
+        if vs < 0:
+            odd_row[rows[(new == ids)]] = True
+        orbit_min = np.minimum(orbit_min, new)
+    assert np.all(odd_row[rows[mism]]), "sign relation broken outside odd-stabilizer orbits"
+    assert np.any(sg < 0)
+    # orbits <-> rows one-to-one, all rows used
+    pairs = np.unique(np.stack([orbit_min, rows]), axis=1)
+    assert pairs.shape[1] == np.unique(orbit_min).size == np.unique(rows).size
+    assert np.array_equal(np.unique(rows), np.arange(m.n_compact_entries))
+    # representatives are canonical: they map to their own row with sign +1
+    reps = orbit.enumerate_representatives(m)
+    rc, rs, rt = reps // (n * n), (reps % (n * n)) // n, reps % n
+    assert np.array_equal(orbit.entry_rows(m, rc, rs, rt), np.arange(reps.size))
+    assert np.all(orbit.entry_signs(m, rc, rs, rt) == 1)
+
+
+@pytest.mark.parametrize("dim,q,kname", [(2, 3, "target0"), (3, 2, "mixed01"),
+                                         (3, 3, "target2")])
+def test_signed_metadata_matches_oracle_restatement(dim, q, kname):
+    """Product metadata, rows and signs equal the oracle's restatement
+    (per-entry ``compact_entry`` and the dense closed forms)."""
+    fixed, power = SIGNED_KERNELS[kname]
+    power = power[:dim]
+    m = orbit.build_case_metadata(dim, q, fixed, power)
+    r = orbit_ref.case_metadata(dim, q, fixed, power)
+    for k in ("case_orbit_ranks", "case_axis_perm", "case_axis_sign", "case_axis_group",
+              "case_value_offsets", "axis_sign_power"):
+        assert np.array_equal(getattr(m, k), r[k]), k
+    assert np.array_equal(m.canonical_case_ids, r["canonical_case_ids"])
+    assert np.array_equal(orbit.enumerate_representatives(m),
+                          orbit_ref.representatives(r, q, dim))
+    n = q**dim
+    for cid in range(0, m.n_cases, 7):
+        s, t = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        assert np.array_equal(orbit.entry_rows(m, cid, s, t), orbit_ref.rows_for_case(r, cid, q, dim))
+        assert np.array_equal(orbit.entry_signs(m, cid, s, t),
+                              orbit_ref.signs_for_case(r, cid, q, dim))
+        for ss, tt in [(0, n - 1), (n // 2, 1), (n - 1, 0)]:
+            row, sign = orbit_ref.compact_entry(cid, ss, tt, r, q, dim)
+            assert row == int(orbit.entry_rows(m, cid, ss, tt))
+            assert sign == int(orbit.entry_signs(m, cid, ss, tt))
+
+
+def test_scalar_metadata_unchanged_by_generalisation():
+    """No fixed axes and zero sign power give the scalar-kernel metadata."""
+    a = orbit.build_case_metadata(3, 3)
+    b = orbit.build_case_metadata(3, 3, (), (0, 0, 0))
+    assert a is b
+    kat = KAT["q3_3d_arithmetic_payload"]
+    assert a.n_compact_entries == kat["canonical_rows"]
+
+
+def test_signed_q3_3d_metadata_bytes_small():
+    """test_nearfield_potential_table.py:2514: signed payload metadata stays
+    under 2000 bytes for q=3 in 3D."""
+    m = orbit.build_case_metadata(3, 3, (0,), (1, 0, 0))
+    assert m.metadata_bytes < 2000

@@ -27,6 +27,8 @@ VL1_E_UNSUPPORTED = -8
 KERNEL_CONSTANT = 0
 KERNEL_LAPLACE = 1
 KERNEL_HELMHOLTZ = 2
+KERNEL_LAPLACE_TARGET_DERIV = 3
+KERNEL_LAPLACE_SOURCE_DERIV = 4
 
 SCALING_CANONICAL = 0
 SCALING_FIXED = 1
@@ -54,6 +56,7 @@ EXPORTED_SYMBOLS = (
     "vl1_csr_builder_destroy",
     "vl1_duffy_build",
     "vl1_orbit_rows_host",
+    "vl1_orbit_signs_host",
 )
 
 vp = C.c_void_p
@@ -79,6 +82,7 @@ class PayloadDesc(C.Structure):
         ("case_axis_sign", vp),
         ("case_axis_group", vp),
         ("case_value_offsets", vp),
+        ("axis_sign_power", vp),
     ]
 
 
@@ -163,6 +167,7 @@ class DuffyArgs(C.Structure):
         ("interp_nodes", vp),
         ("bary_w", vp),
         ("result", vp),
+        ("deriv_axis", C.c_int),
     ]
 
 
@@ -199,7 +204,8 @@ def lib():
     L.vl1_csr_builder_fill.argtypes = [vp, vp, vp, vp]
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 1:
+    L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
+    if L.vl1_abi_version() != 2:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

@@ -126,4 +126,30 @@ __host__ __device__ inline int64_t compact_rel_row(int dim, int q, const uint8_t
   return compact;
 }
 
+// Sign of entry (s, t) of one case for signed (axis-derivative) kernels:
+// -1 when the canonicalisation above flips an odd number of axes whose input
+// axis has its bit set in sp_mask (orbit_arithmetic.py:372-432, entry_sign;
+// list1.py:799-832, entry_sign_factor).  +1 for scalar kernels (sp_mask 0).
+__host__ __device__ inline int entry_sign(int dim, int q, const uint8_t* perm, const int8_t* sign,
+                                          int sp_mask, int s, int t) {
+  if (!sp_mask) return 1;
+  int sr[3] = {0, 0, 0}, tr[3] = {0, 0, 0};
+  for (int ax = dim - 1; ax >= 0; --ax) {
+    sr[ax] = s % q;
+    s /= q;
+    tr[ax] = t % q;
+    t /= q;
+  }
+  int neg = 0;
+  for (int o = 0; o < dim; ++o) {
+    const int in = perm[o];
+    const int si = sr[in], ti = tr[in];
+    const int sf = q - 1 - si, tf = q - 1 - ti;
+    const int sg = sign[o];
+    const bool flip = (sg < 0) || (sg == 0 && (sf < si || (sf == si && tf < ti)));
+    if (flip && ((sp_mask >> in) & 1)) neg ^= 1;
+  }
+  return neg ? -1 : 1;
+}
+
 }  // namespace vl1

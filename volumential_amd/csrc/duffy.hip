@@ -22,7 +22,7 @@
 namespace vl1 {
 
 struct DuffyParams {
-  int dim, q, kind;
+  int dim, q, kind, deriv_axis;
   double k;
   int64_t n_entries, n_nodes, nodes_per_sign;
   double H;
@@ -62,10 +62,21 @@ __device__ inline double lagrange_basis(double y, int m, int q, double cm, const
 
 // Kernel values, sumpy conventions (expression times global scaling):
 // Laplace 2D -log(r)/(2 pi), 3D 1/(4 pi r); Helmholtz 2D (i/4) H0^(1)(k r)
-// = (-Y0(kr) + i J0(kr)) / 4, 3D e^{ikr}/(4 pi r).
+// = (-Y0(kr) + i J0(kr)) / 4, 3D e^{ikr}/(4 pi r).  Axis derivatives of
+// Laplace (da = component of d = target - source on the derivative axis):
+// target dG/dd_a = 2D -da/(2 pi r^2), 3D -da/(4 pi r^3); source = minus.
 template <int DIM, int KIND>
-__device__ inline void kernel_value(double r2, double k, double& re, double& im) {
-  if (KIND == VL1_KERNEL_CONSTANT) {
+__device__ inline void kernel_value(double r2, double da, double k, double& re, double& im) {
+  if (KIND == VL1_KERNEL_LAPLACE_TARGET_DERIV || KIND == VL1_KERNEL_LAPLACE_SOURCE_DERIV) {
+    constexpr double sgn = KIND == VL1_KERNEL_LAPLACE_TARGET_DERIV ? -1.0 : 1.0;
+    if (DIM == 2) {
+      re = sgn * 0.5 / M_PI * da / r2;
+    } else {
+      const double ir = rsqrt(r2);
+      re = sgn * 0.25 / M_PI * da * ir * ir * ir;
+    }
+    im = 0.0;
+  } else if (KIND == VL1_KERNEL_CONSTANT) {
     re = 1.0;
     im = 0.0;
   } else if (KIND == VL1_KERNEL_LAPLACE) {
@@ -139,20 +150,21 @@ __global__ void __launch_bounds__(256) duffy_kernel(const DuffyParams p) {
     }
     for (int64_t i = 0; i < p.nodes_per_sign; ++i) {
       const int64_t node = base + i;
-      double r2 = 0.0;
+      double r2 = 0.0, da = 0.0;
       double yk[DIM];
 #pragma unroll
       for (int a = 0; a < DIM; ++a) {
         yk[a] = v[a] + sl[a] * nu[a * p.n_nodes + node];
         const double d = x[a] - yk[a];
         r2 += d * d;
+        if (a == p.deriv_axis) da = d;
       }
       if (!(r2 > 0.0)) continue;
       double basis = njac[node] * lenprod;
 #pragma unroll
       for (int a = 0; a < DIM; ++a) basis *= lagrange_basis<Q>(yk[a], m[a], q, cm[a], xi, tol);
       double kr, ki;
-      kernel_value<DIM, KIND>(r2, p.k, kr, ki);
+      kernel_value<DIM, KIND>(r2, da, p.k, kr, ki);
       acc_re = fma(kr, basis, acc_re);
       if (KIND == VL1_KERNEL_HELMHOLTZ) acc_im = fma(ki, basis, acc_im);
     }
@@ -204,7 +216,12 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
   if (!a->target_points || !a->decomposition || !a->mode_axes || !a->node_u || !a->node_sign ||
       !a->node_jac || !a->interp_nodes || !a->bary_w || !a->result)
     return fail(VL1_E_ARG, "vl1_duffy_build: null device array");
-  DuffyParams prm{a->dim, a->quad_order, a->kernel_kind, a->helmholtz_k, a->n_entries,
+  const bool deriv = a->kernel_kind == VL1_KERNEL_LAPLACE_TARGET_DERIV ||
+                     a->kernel_kind == VL1_KERNEL_LAPLACE_SOURCE_DERIV;
+  if (deriv && (a->deriv_axis < 0 || a->deriv_axis >= a->dim))
+    return fail(VL1_E_ARG, "vl1_duffy_build: derivative axis out of range");
+  DuffyParams prm{a->dim, a->quad_order, a->kernel_kind, deriv ? a->deriv_axis : -1,
+                  a->helmholtz_k, a->n_entries,
                   a->n_nodes, a->nodes_per_sign, a->source_box_extent, a->target_points,
                   a->decomposition, a->mode_axes, a->node_u, a->node_sign, a->node_jac,
                   a->interp_nodes, a->bary_w, a->result};
@@ -215,11 +232,19 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
     if (kind == VL1_KERNEL_CONSTANT) e = launch_duffy<2, VL1_KERNEL_CONSTANT>(prm, s);
     else if (kind == VL1_KERNEL_LAPLACE) e = launch_duffy<2, VL1_KERNEL_LAPLACE>(prm, s);
     else if (kind == VL1_KERNEL_HELMHOLTZ) e = launch_duffy<2, VL1_KERNEL_HELMHOLTZ>(prm, s);
+    else if (kind == VL1_KERNEL_LAPLACE_TARGET_DERIV)
+      e = launch_duffy<2, VL1_KERNEL_LAPLACE_TARGET_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_LAPLACE_SOURCE_DERIV)
+      e = launch_duffy<2, VL1_KERNEL_LAPLACE_SOURCE_DERIV>(prm, s);
     else return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: kernel kind");
   } else {
     if (kind == VL1_KERNEL_CONSTANT) e = launch_duffy<3, VL1_KERNEL_CONSTANT>(prm, s);
     else if (kind == VL1_KERNEL_LAPLACE) e = launch_duffy<3, VL1_KERNEL_LAPLACE>(prm, s);
     else if (kind == VL1_KERNEL_HELMHOLTZ) e = launch_duffy<3, VL1_KERNEL_HELMHOLTZ>(prm, s);
+    else if (kind == VL1_KERNEL_LAPLACE_TARGET_DERIV)
+      e = launch_duffy<3, VL1_KERNEL_LAPLACE_TARGET_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_LAPLACE_SOURCE_DERIV)
+      e = launch_duffy<3, VL1_KERNEL_LAPLACE_SOURCE_DERIV>(prm, s);
     else return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: kernel kind");
   }
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("duffy launch: ") + hipGetErrorString(e));

@@ -9,6 +9,7 @@ struct vl1_payload {
   int dim = 0, q = 0, n = 0, n_cases = 0, n_orbits = 0, n_tables = 0, is_complex = 0;
   int64_t n_entries = 0;   // compact rows per table level
   int rowmap_u16 = 1;      // rowmap element type: 1 = uint16, 0 = int32
+  int sign_mask = 0;       // bit a set: axis a has sign power 1 (signed kernels)
   int max_orbit_count = 0; // largest canonical orbit (rows)
   // host copies of the small descriptors
   std::vector<int64_t> case_indices;
@@ -22,7 +23,8 @@ struct vl1_payload {
   vl1::DevBuf case_idx;    // int64 [13^dim]
   vl1::DevBuf ranks;       // uint16 [n_cases]
   vl1::DevBuf offs;        // int32 [n_orbits+1]
-  vl1::DevBuf rowmap;      // [n_cases][n (s)][n (t)] relative compact rows
+  vl1::DevBuf rowmap;      // [n_cases][n (s)][n (t)] relative compact rows; for signed
+                           // kernels (int32) ~row marks an entry read negated
   int64_t device_bytes() const {
     return (int64_t)(table.bytes + nmlz.bytes + case_idx.bytes + ranks.bytes + offs.bytes +
                      rowmap.bytes);
@@ -33,4 +35,9 @@ namespace vl1 {
 // Launchers implemented in the .hip translation units.
 hipError_t launch_rowmap(const vl1_payload& p, const uint8_t* d_perm, const int8_t* d_sign,
                          const uint8_t* d_group, hipStream_t s);
+// Decode a signed row-map element: row >= 0 as is, ~row read negated.
+__host__ __device__ inline int decode_signed_row(int v, double& sgn) {
+  sgn = v < 0 ? -1.0 : 1.0;
+  return v < 0 ? ~v : v;
+}
 }  // namespace vl1

@@ -21,6 +21,8 @@
 #include <cmath>
 #include <string>
 
+#include <type_traits>
+
 #include "internal.h"
 
 namespace vl1 {
@@ -157,16 +159,22 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
           for (int sid = part; sid < ns; sid += P) {
             const int s = sMode[sid];
             if ((unsigned)s >= (unsigned)n) continue;  // malformed mode id: no contribution
-            const int rel = (int)rm[(int64_t)s * n];
+            int rel = (int)rm[(int64_t)s * n];
+            double esc = scal;  // scaling times the entry sign of signed kernels
+            if constexpr (std::is_signed<IdxT>::value) {
+              double sg;
+              rel = decode_signed_row(rel, sg);
+              esc *= sg;
+            }
             const double* fs = sF + sid * RC * VW;
             if constexpr (!CPLX) {
               const double tv = staged ? sT[rel] : gT[rel];
-              const double v = tv * scal + sD[sid];
+              const double v = tv * esc + sD[sid];
 #pragma unroll
               for (int r = 0; r < RC; ++r) acc[r] = fma(fs[r], v, acc[r]);
             } else {
-              const double vr = (staged ? sT[2 * rel] : gT[2 * rel]) * scal;
-              const double vi = (staged ? sT[2 * rel + 1] : gT[2 * rel + 1]) * scal;
+              const double vr = (staged ? sT[2 * rel] : gT[2 * rel]) * esc;
+              const double vi = (staged ? sT[2 * rel + 1] : gT[2 * rel + 1]) * esc;
 #pragma unroll
               for (int r = 0; r < RC; ++r) {
                 const double fr = fs[2 * r], fi = fs[2 * r + 1];

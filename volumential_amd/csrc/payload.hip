@@ -27,7 +27,7 @@ int fail(int code, const std::string& msg) {
 
 template <typename IdxT>
 __global__ void __launch_bounds__(256)
-rowmap_kernel(int dim, int q, int n, int n_cases, const uint8_t* __restrict__ perm,
+rowmap_kernel(int dim, int q, int n, int n_cases, int sp_mask, const uint8_t* __restrict__ perm,
               const int8_t* __restrict__ sign, const uint8_t* __restrict__ group,
               IdxT* __restrict__ out) {
   const int64_t total = (int64_t)n_cases * n * n;
@@ -37,7 +37,10 @@ rowmap_kernel(int dim, int q, int n, int n_cases, const uint8_t* __restrict__ pe
     const int64_t cs = i / n;
     const int s = (int)(cs % n);
     const int c = (int)(cs / n);
-    out[i] = (IdxT)compact_rel_row(dim, q, perm + c * dim, sign + c * dim, group + c * dim, s, t);
+    const int64_t rel = compact_rel_row(dim, q, perm + c * dim, sign + c * dim, group + c * dim, s, t);
+    // sp_mask != 0 only with int32 maps: a negated entry is stored as ~rel
+    const bool neg = entry_sign(dim, q, perm + c * dim, sign + c * dim, sp_mask, s, t) < 0;
+    out[i] = (IdxT)(neg ? ~rel : rel);
   }
 }
 
@@ -48,10 +51,10 @@ hipError_t launch_rowmap(const vl1_payload& p, const uint8_t* d_perm, const int8
   const int grid = (int)std::min<int64_t>((total + block - 1) / block, 65536);
   if (p.rowmap_u16) {
     hipLaunchKernelGGL(rowmap_kernel<uint16_t>, dim3(grid), dim3(block), 0, s, p.dim, p.q, p.n,
-                       p.n_cases, d_perm, d_sign, d_group, p.rowmap.as<uint16_t>());
+                       p.n_cases, 0, d_perm, d_sign, d_group, p.rowmap.as<uint16_t>());
   } else {
     hipLaunchKernelGGL(rowmap_kernel<int32_t>, dim3(grid), dim3(block), 0, s, p.dim, p.q, p.n,
-                       p.n_cases, d_perm, d_sign, d_group, p.rowmap.as<int32_t>());
+                       p.n_cases, p.sign_mask, d_perm, d_sign, d_group, p.rowmap.as<int32_t>());
   }
   return hipGetLastError();
 }
@@ -130,7 +133,16 @@ int vl1_payload_create(vl1_payload** out, const vl1_payload_desc* d, void* strea
       return fail(VL1_E_TABLE, "near-field table contains non-finite entries");
     }
   }
-  p->rowmap_u16 = p->max_orbit_count <= 65535 ? 1 : 0;
+  if (d->axis_sign_power)
+    for (int a = 0; a < d->dim; ++a) {
+      if (d->axis_sign_power[a] > 1) {
+        delete p;
+        return fail(VL1_E_TABLE, "axis_sign_power entries must be 0 or 1");
+      }
+      p->sign_mask |= (int)d->axis_sign_power[a] << a;
+    }
+  // signed kernels keep the entry sign in the row map (int32, ~row negated)
+  p->rowmap_u16 = (p->max_orbit_count <= 65535 && p->sign_mask == 0) ? 1 : 0;
 
   hipError_t e = hipSuccess;
   const size_t vbytes = sizeof(double) * (p->is_complex ? 2 : 1);
@@ -175,6 +187,18 @@ int vl1_orbit_rows_host(int dim, int q, const uint8_t* perm, const int8_t* sign,
   if ((dim != 2 && dim != 3) || q < 1 || !perm || !sign || !group || !s || !t || !out)
     return fail(VL1_E_ARG, "vl1_orbit_rows_host: bad argument");
   for (int64_t i = 0; i < count; ++i) out[i] = compact_rel_row(dim, q, perm, sign, group, s[i], t[i]);
+  return VL1_OK;
+}
+
+int vl1_orbit_signs_host(int dim, int q, const uint8_t* perm, const int8_t* sign,
+                         const uint8_t* axis_sign_power, const int32_t* s, const int32_t* t,
+                         int64_t count, int8_t* out) {
+  clear_error();
+  if ((dim != 2 && dim != 3) || q < 1 || !perm || !sign || !axis_sign_power || !s || !t || !out)
+    return fail(VL1_E_ARG, "vl1_orbit_signs_host: bad argument");
+  int mask = 0;
+  for (int a = 0; a < dim; ++a) mask |= (axis_sign_power[a] & 1) << a;
+  for (int64_t i = 0; i < count; ++i) out[i] = (int8_t)entry_sign(dim, q, perm, sign, mask, s[i], t[i]);
   return VL1_OK;
 }
 

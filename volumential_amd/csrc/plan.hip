@@ -50,7 +50,8 @@ static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 // [[Re T, -Im T], [Im T, Re T]] acting on [Re f; Im f] and giving
 // [Re y; Im y], so both contraction kernels run unchanged on complex data
 // (8 real flops per complex multiply-add, as the complex product itself).
-__global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int MP, int KP, int n_blocks,
+__global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int sp_mask, int MP, int KP,
+                                     int n_blocks,
                                      const int32_t* __restrict__ blk_case,
                                      const int32_t* __restrict__ blk_lev,
                                      const uint8_t* __restrict__ perm,
@@ -80,6 +81,8 @@ __global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int MP, in
         const double re = table[2 * at], im = table[2 * at + 1];
         v = (tr == sr) ? re : (tr ? -im : im);
       }
+      // signed (axis-derivative) kernels: the entry sign of (s, t)
+      if (entry_sign(dim, q, perm + c * dim, sign + c * dim, sp_mask, ss, tt) < 0) v = -v;
     }
     M[i] = v;
   }
@@ -1132,8 +1135,8 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
     if (e == hipSuccess) {
       const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
       const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
-      hipLaunchKernelGGL(expand_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n, pl->cplx, pl->MP,
-                         pl->KP, pl->n_blocks, dcase.as<int32_t>(), dlev.as<int32_t>(),
+      hipLaunchKernelGGL(expand_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n, pl->cplx,
+                         pay->sign_mask, pl->MP, pl->KP, pl->n_blocks, dcase.as<int32_t>(), dlev.as<int32_t>(),
                          dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
                          pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(), pay->table.as<double>(),
                          pay->n_entries, pl->M.as<double>());

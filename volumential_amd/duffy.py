@@ -57,6 +57,7 @@ def duffy_values(table, entry_ids, regular_order, radial_order, stream=None) -> 
         interp_nodes=d_xi.data_ptr(),
         bary_w=d_bw.data_ptr(),
         result=out.data_ptr(),
+        deriv_axis=int(getattr(table.integral_knl, "axis", -1)),
     )
     st = stream if stream is not None else torch.cuda.current_stream()
     ev0 = torch.cuda.Event(enable_timing=True)

@@ -12,6 +12,8 @@ Kernel conventions (sumpy expression times global scaling constant):
 * Laplace 2D ``-log|d| / (2 pi)``, 3D ``1 / (4 pi |d|)``
 * Helmholtz 2D ``(i/4) H0^(1)(k|d|)``, 3D ``e^{ik|d|} / (4 pi |d|)``
 * Constant ``1``
+* Axis derivatives of Laplace: target ``dG/dd_a`` = 2D ``-d_a / (2 pi |d|^2)``,
+  3D ``-d_a / (4 pi |d|^3)`` (``d = target - source``); source = minus that
 """
 
 from __future__ import annotations
@@ -61,8 +63,87 @@ class HelmholtzKernel:
         return self
 
 
+@dataclass(frozen=True)
+class AxisTargetDerivative:
+    """``d/dx_axis`` of ``inner_kernel`` at the target (sumpy's wrapper of the
+    same name): ``G_a(d) = dG/dd_a`` with ``d = target - source``."""
+
+    axis: int
+    inner_kernel: object
+    kind = _capi.KERNEL_LAPLACE_TARGET_DERIV
+
+    @property
+    def dim(self):
+        return self.inner_kernel.dim
+
+    @property
+    def is_complex_valued(self):
+        return self.inner_kernel.is_complex_valued
+
+    def __repr__(self):
+        return f"AxisTargetDerivative({self.axis}, {self.inner_kernel!r})"
+
+    def get_base_kernel(self):
+        return self.inner_kernel.get_base_kernel()
+
+
+@dataclass(frozen=True)
+class AxisSourceDerivative:
+    """``d/dy_axis`` of ``inner_kernel`` at the source: ``-dG/dd_a``."""
+
+    axis: int
+    inner_kernel: object
+    kind = _capi.KERNEL_LAPLACE_SOURCE_DERIV
+
+    @property
+    def dim(self):
+        return self.inner_kernel.dim
+
+    @property
+    def is_complex_valued(self):
+        return self.inner_kernel.is_complex_valued
+
+    def __repr__(self):
+        return f"AxisSourceDerivative({self.axis}, {self.inner_kernel!r})"
+
+    def get_base_kernel(self):
+        return self.inner_kernel.get_base_kernel()
+
+
+_DERIVATIVES = (AxisTargetDerivative, AxisSourceDerivative)
+
+
+def derivative_axes(kernel):
+    """``(fixed_axes, axis_sign_power)`` of a kernel's axis-derivative chain.
+
+    ``orbit_arithmetic.py:176-197``: every axis derivative pins its axis (no
+    axis permutation may move it) and toggles that axis's sign power (a flip
+    of the axis negates the entry).  Scalar kernels give ``((), zeros)``.
+    """
+    dim = kernel.dim
+    fixed = set()
+    power = [0] * dim
+    k = kernel
+    while isinstance(k, _DERIVATIVES):
+        if not 0 <= int(k.axis) < dim:
+            raise ValueError(f"derivative axis {k.axis} out of range for dim {dim}")
+        fixed.add(int(k.axis))
+        power[int(k.axis)] ^= 1
+        k = k.inner_kernel
+    return tuple(sorted(fixed)), tuple(power)
+
+
+def _single_laplace_derivative(kernel, dim=None):
+    return (isinstance(kernel, _DERIVATIVES)
+            and isinstance(kernel.inner_kernel, LaplaceKernel)
+            and (dim is None or kernel.dim == dim))
+
+
 def supports_inferred_scaling(kernel) -> bool:
-    """``list1.py:263-276``: Laplace and constant kernels in 2D/3D."""
+    """``list1.py:263-276``: Laplace and constant kernels in 2D/3D, and one
+    axis target/source derivative of Laplace."""
+    if _single_laplace_derivative(kernel):
+        return kernel.dim in (2, 3)
     return isinstance(kernel, (LaplaceKernel, ConstantKernel)) and kernel.dim in (2, 3)
 
 
@@ -71,9 +152,12 @@ def inferred_scaling(kernel):
 
     ``list1.py:278-355``: ``G(h r) = (h/H)^p G(H r)`` with ``p = 2`` for the
     Laplace volume potential in 2D and 3D (the 2D one adds the
-    ``-(1/2pi) (h/H)^2 log(h/H) * mode_normalizer`` term) and ``p = dim`` for
-    the constant kernel.
+    ``-(1/2pi) (h/H)^2 log(h/H) * mode_normalizer`` term), ``p = 1`` and no
+    displacement for an axis derivative of Laplace (``grad G`` is
+    homogeneous of degree ``1 - d``), and ``p = dim`` for the constant kernel.
     """
+    if _single_laplace_derivative(kernel) and kernel.dim in (2, 3):
+        return 1, False
     if isinstance(kernel, LaplaceKernel) and kernel.dim in (2, 3):
         return 2, kernel.dim == 2
     if isinstance(kernel, ConstantKernel) and kernel.dim in (2, 3):

@@ -78,10 +78,12 @@ class NearFieldFromCSR:
         self.reconstruction_kind = table_data_shapes.get(
             "reconstruction_kind", "scalar-arithmetic-orbit"
         )
-        if self.reconstruction_kind != "scalar-arithmetic-orbit":
+        if self.reconstruction_kind not in ("scalar-arithmetic-orbit",
+                                            "signed-arithmetic-orbit"):
             raise NotImplementedError(
                 f"reconstruction kind {self.reconstruction_kind!r} is not supported; "
-                "scalar kernels use 'scalar-arithmetic-orbit'"
+                "scalar kernels use 'scalar-arithmetic-orbit', axis derivatives "
+                "'signed-arithmetic-orbit'"
             )
         self.dim = integral_kernel.dim
         self.quad_order = int(
@@ -145,11 +147,17 @@ class NearFieldFromCSR:
     # ------------------------------------------------------------ payload
     def _payload(self, kw):
         table_data = _to_host(kw["table_data_combined"])
+        sp = kw.get("arithmetic_axis_sign_power")
+        sign_power = (np.zeros(self.dim, np.uint8) if sp is None
+                      else np.ascontiguousarray(_to_host(sp), dtype=np.uint8).reshape(self.dim))
+        if self.reconstruction_kind == "scalar-arithmetic-orbit" and np.any(sign_power):
+            raise ValueError("scalar-arithmetic-orbit payloads have no axis sign powers")
         key = (
             repr(self.integral_kernel),
             self.quad_order,
             fingerprint(table_data),
             fingerprint(_to_host(kw["mode_nmlz_combined"])),
+            tuple(int(v) for v in sign_power),
         )
 
         def make():
@@ -167,10 +175,12 @@ class NearFieldFromCSR:
                 case_value_offsets=np.asarray(
                     _to_host(kw["arithmetic_case_value_offsets"]), np.int32
                 ),
-                axis_sign_power=np.zeros(self.dim, np.uint8),
+                axis_sign_power=sign_power,
             )
-            if np.any(np.asarray(_to_host(kw.get("arithmetic_axis_sign_power", 0))) != 0):
-                raise NotImplementedError("signed (derivative) orbit payloads are not supported")
+            if int(kw.get("arithmetic_direction_sign_axis", -1)) >= 0:
+                raise NotImplementedError(
+                    "directional source derivatives (arithmetic_direction_sign_axis) are "
+                    "not part of this build; axis derivatives are")
             return DevicePayload(
                 self.dim,
                 self.quad_order,

@@ -1,4 +1,4 @@
-"""Arithmetic orbit reduction of near-field tables for scalar kernels.
+"""Arithmetic orbit reduction of near-field tables (scalar and signed kernels).
 
 A table entry ``(case, s, t)`` (source mode ``s``, target node ``t``) is
 invariant under the hyperoctahedral group acting jointly on the case vector
@@ -16,6 +16,11 @@ entry is mapped to its compact row by a few integer operations:
 4. each group's multiset of pair codes ``s_k*q + t_k`` is ranked among all
    multisets of that size, and the group ranks are combined mixed-radix;
 5. the result is offset by the canonical case's first compact row.
+
+For an axis-derivative kernel (``AxisTargetDerivative`` /
+``AxisSourceDerivative``) the derivative axis is never permuted (a group of
+its own) and a flip of it negates the entry: the entry is read as
+``sign * compact[row]`` (the reference's signed-arithmetic-orbit kind).
 
 Follows ``volumential/orbit_arithmetic.py``: ``_case_arithmetic_axis_descriptors``
 (45-163, scalar-kernel branch), ``_canonicalize_arithmetic_entry_axes``
@@ -76,30 +81,46 @@ class ArithmeticCaseMetadata:
         )
 
 
-def case_axis_descriptors(case_vec):
+def axis_group_specs(dim, fixed_axes=()):
+    """Axis sets the symmetry group may permute among themselves.
+
+    ``orbit_arithmetic.py:245-251``: every derivative (fixed) axis is a group
+    of its own, in increasing order, followed by the remaining free axes.
+    """
+    fixed = sorted(set(int(a) for a in fixed_axes))
+    free = tuple(a for a in range(dim) if a not in fixed)
+    return tuple((a,) for a in fixed) + ((free,) if free else ())
+
+
+def case_axis_descriptors(case_vec, fixed_axes=()):
     """Signed permutation mapping ``case_vec`` to its canonical vector.
 
-    Output axis ``k`` reads input axis ``perm[k]``; axes are ordered by
-    ``(-|v|, axis)``.  ``sign[k]`` is -1 for a positive component, +1 for a
-    negative one and 0 for a zero component, so canonical components are
-    ``<= 0``.  Runs of equal ``|v|`` share a group id (0, 1, ...).
+    Within each axis group (``axis_group_specs``) output axis ``k`` reads
+    input axis ``perm[k]``, the group's axes ordered by ``(-|v|, axis)``.
+    ``sign[k]`` is -1 for a positive component, +1 for a negative one and 0
+    for a zero component, so canonical components are ``<= 0``.  Runs of
+    equal ``|v|`` inside a group share a runtime group id; ids count up over
+    the groups in order (``orbit_arithmetic.py:140-163``).
     """
     vec = [int(v) for v in case_vec]
     dim = len(vec)
-    order = sorted(range(dim), key=lambda a: (-abs(vec[a]), a))
     perm = np.empty(dim, dtype=np.uint8)
     sign = np.empty(dim, dtype=np.int8)
     group = np.empty(dim, dtype=np.uint8)
-    gid = -1
-    last = None
-    for out_axis, in_axis in enumerate(order):
-        v = vec[in_axis]
-        if abs(v) != last:
-            gid += 1
-            last = abs(v)
-        perm[out_axis] = in_axis
-        sign[out_axis] = -1 if v > 0 else (1 if v < 0 else 0)
-        group[out_axis] = gid
+    next_gid = 0
+    for out_axes in axis_group_specs(dim, fixed_axes):
+        order = sorted(out_axes, key=lambda a: (-abs(vec[a]), a))
+        gid = next_gid - 1
+        last = None
+        for out_axis, in_axis in zip(out_axes, order):
+            v = vec[in_axis]
+            if abs(v) != last:
+                gid += 1
+                last = abs(v)
+            perm[out_axis] = in_axis
+            sign[out_axis] = -1 if v > 0 else (1 if v < 0 else 0)
+            group[out_axis] = gid
+        next_gid = gid + 1
     return perm, sign, group
 
 
@@ -126,7 +147,8 @@ def case_value_count(sign_row, group_row, q: int) -> int:
 
 
 @lru_cache(maxsize=None)
-def _metadata_cached(dim: int, q: int) -> ArithmeticCaseMetadata:
+def _metadata_cached(dim: int, q: int, fixed_axes: tuple, sign_power: tuple
+                     ) -> ArithmeticCaseMetadata:
     if dim not in (2, 3):
         raise NotImplementedError("arithmetic orbit reduction supports dim 2 or 3")
     vecs, _, case_indices = gallery.generate_list1_gallery(dim)
@@ -136,7 +158,7 @@ def _metadata_cached(dim: int, q: int) -> ArithmeticCaseMetadata:
     group = np.empty((n_cases, dim), dtype=np.uint8)
     canon_ids = np.empty(n_cases, dtype=np.int32)
     for cid, vec in enumerate(vecs):
-        p, s, g = case_axis_descriptors(vec)
+        p, s, g = case_axis_descriptors(vec, fixed_axes)
         perm[cid], sign[cid], group[cid] = p, s, g
         canon_ids[cid] = case_indices[gallery.case_encode(canonical_case_vector(vec, p, s))]
     uniq = np.unique(canon_ids)
@@ -158,13 +180,35 @@ def _metadata_cached(dim: int, q: int) -> ArithmeticCaseMetadata:
         case_axis_sign=sign,
         case_axis_group=group,
         case_value_offsets=offsets.astype(np.int32),
-        axis_sign_power=np.zeros(dim, dtype=np.uint8),
+        axis_sign_power=np.asarray(sign_power, dtype=np.uint8),
     )
 
 
-def build_case_metadata(dim: int, quad_order: int) -> ArithmeticCaseMetadata:
-    """Per-case descriptors (``orbit_arithmetic.py:665-745``)."""
-    return _metadata_cached(int(dim), int(quad_order))
+def build_case_metadata(dim: int, quad_order: int, fixed_axes=(), axis_sign_power=None
+                        ) -> ArithmeticCaseMetadata:
+    """Per-case descriptors (``orbit_arithmetic.py:665-745``).
+
+    ``fixed_axes`` / ``axis_sign_power`` describe an axis-derivative kernel
+    (``kernels.derivative_axes``, ``orbit_arithmetic.py:176-369``); the
+    defaults give the scalar-kernel metadata.
+    """
+    dim = int(dim)
+    power = tuple(int(v) for v in (axis_sign_power if axis_sign_power is not None
+                                   else (0,) * dim))
+    if len(power) != dim or any(v not in (0, 1) for v in power):
+        raise ValueError("axis_sign_power must hold dim values in {0, 1}")
+    fixed = tuple(sorted(set(int(a) for a in fixed_axes)))
+    if any(not 0 <= a < dim for a in fixed):
+        raise ValueError("fixed axis out of range")
+    return _metadata_cached(dim, int(quad_order), fixed, power)
+
+
+def kernel_case_metadata(kernel, quad_order: int) -> ArithmeticCaseMetadata:
+    """Metadata for a kernel: its derivative axes fix and sign the group."""
+    from .kernels import derivative_axes
+
+    fixed, power = derivative_axes(kernel)
+    return build_case_metadata(kernel.dim, quad_order, fixed, power)
 
 
 def _combos_with_replacement(alpha: int, k: int) -> np.ndarray:
@@ -240,8 +284,8 @@ def _rank_sorted_multiset(values: np.ndarray, alpha: int) -> np.ndarray:
 def entry_rows(meta: ArithmeticCaseMetadata, case_ids, s_ids, t_ids) -> np.ndarray:
     """Compact row of each ``(case, s, t)`` (vectorised).
 
-    ``orbit_arithmetic.py:372-432`` followed by ``566-634``.  For scalar
-    kernels the entry sign is always +1.
+    ``orbit_arithmetic.py:372-432`` followed by ``566-634``.  The entry sign
+    of signed kernels is ``entry_signs``.
     """
     dim, q = meta.dim, meta.quad_order
     case_ids = np.asarray(case_ids, dtype=np.int64)
@@ -318,6 +362,43 @@ def entry_rows(meta: ArithmeticCaseMetadata, case_ids, s_ids, t_ids) -> np.ndarr
     offsets = meta.case_value_offsets.astype(np.int64)
     row = offsets[meta.case_orbit_ranks[case_ids].astype(np.int64)] + compact
     return row.reshape(shape)
+
+
+def entry_signs(meta: ArithmeticCaseMetadata, case_ids, s_ids, t_ids) -> np.ndarray:
+    """Entry sign (+1/-1, int8) of each ``(case, s, t)``: the reconstructed
+    value is ``sign * compact[row]``.  ``orbit_arithmetic.py:372-432``: every
+    axis the canonicalisation flips (negative case component, or the zero-
+    component fold) negates the entry when its input axis has sign power 1.
+    All +1 for scalar kernels."""
+    dim, q = meta.dim, meta.quad_order
+    case_ids, s_ids, t_ids = np.broadcast_arrays(
+        np.asarray(case_ids, dtype=np.int64), np.asarray(s_ids, dtype=np.int64),
+        np.asarray(t_ids, dtype=np.int64))
+    shape = case_ids.shape
+    power = np.asarray(meta.axis_sign_power, dtype=np.int64)
+    if not np.any(power):
+        return np.ones(shape, dtype=np.int8)
+    case_ids = case_ids.ravel()
+    rs, rt = s_ids.ravel().copy(), t_ids.ravel().copy()
+    s_raw = np.empty((case_ids.size, dim), dtype=np.int64)
+    t_raw = np.empty((case_ids.size, dim), dtype=np.int64)
+    for ax in range(dim - 1, -1, -1):
+        s_raw[:, ax] = rs % q
+        t_raw[:, ax] = rt % q
+        rs //= q
+        rt //= q
+    perm = meta.case_axis_perm[case_ids].astype(np.int64)
+    sign = meta.case_axis_sign[case_ids].astype(np.int64)
+    rows = np.arange(case_ids.size)
+    neg = np.zeros(case_ids.size, dtype=bool)
+    for out in range(dim):
+        s_in = s_raw[rows, perm[:, out]]
+        t_in = t_raw[rows, perm[:, out]]
+        s_f, t_f = q - 1 - s_in, q - 1 - t_in
+        fold = (sign[:, out] == 0) & ((s_f < s_in) | ((s_f == s_in) & (t_f < t_in)))
+        flip = (sign[:, out] < 0) | fold
+        neg ^= flip & (power[perm[:, out]] == 1)
+    return np.where(neg, -1, 1).astype(np.int8).reshape(shape)
 
 
 def case_row_map(meta: ArithmeticCaseMetadata, case_id: int) -> np.ndarray:

@@ -2,8 +2,8 @@
 
 Follows ``_prepare_table_data_and_entry_map`` and
 ``_build_fast_scalar_arithmetic_orbit_reconstruction``
-(volumential/expansion_wrangler_fpnd.py:1667-1726, 1805-2066) for the
-scalar-arithmetic-orbit kind, and the device upload/caching of
+(volumential/expansion_wrangler_fpnd.py:1451-1660, 1667-1726, 1805-2066)
+for the scalar- and signed-arithmetic-orbit kinds, and the device upload/caching of
 ``_get_cached_nearfield_payload`` (2939-3158).
 """
 
@@ -43,15 +43,19 @@ def prepare_table_data_and_entry_map(table_levels):
     for t in levels[1:]:
         if (t.dim, t.quad_order) != (t0.dim, t0.quad_order):
             raise RuntimeError("near-field levels disagree on dimension or order")
-    meta = orbit.build_case_metadata(t0.dim, t0.quad_order)
+    meta = t0.arithmetic_metadata()
+    for t in levels[1:]:
+        if not np.array_equal(t.arithmetic_metadata().axis_sign_power, meta.axis_sign_power):
+            raise RuntimeError("near-field levels disagree on the kernel's derivative axes")
     layout_ids = orbit.enumerate_representatives(meta)
+    signed = bool(np.any(meta.axis_sign_power))
     dtype = np.dtype(t0.dtype)
     data = np.zeros((len(levels), layout_ids.size), dtype=dtype)
     nmlz = np.zeros((len(levels), t0.n_q_points), dtype=t0.mode_normalizers.dtype)
     ext = np.zeros((len(levels), t0.n_q_points), dtype=t0.kernel_exterior_normalizers.dtype)
     for lev, t in enumerate(levels):
         try:
-            vals = t.get_entry_data_for_full_indices(layout_ids)
+            vals = t.compact_values()
         except RuntimeError as exc:
             raise RuntimeError(
                 "near-field reconstruction layout references missing data"
@@ -62,7 +66,7 @@ def prepare_table_data_and_entry_map(table_levels):
         nmlz[lev] = t.mode_normalizers
         ext[lev] = t.kernel_exterior_normalizers
     info = {
-        "kind": "scalar-arithmetic-orbit",
+        "kind": "signed-arithmetic-orbit" if signed else "scalar-arithmetic-orbit",
         "case_orbit_ranks": meta.case_orbit_ranks,
         "case_axis_perm": meta.case_axis_perm,
         "case_axis_sign": meta.case_axis_sign,
@@ -118,6 +122,7 @@ class DevicePayload:
             np.ascontiguousarray(meta.case_axis_sign, np.int8),
             np.ascontiguousarray(meta.case_axis_group, np.uint8),
             np.ascontiguousarray(meta.case_value_offsets, np.int32),
+            np.ascontiguousarray(meta.axis_sign_power, np.uint8),
         ]
         desc = _capi.PayloadDesc(
             dim=int(dim),
@@ -135,6 +140,7 @@ class DevicePayload:
             case_axis_sign=self._keep[5].ctypes.data,
             case_axis_group=self._keep[6].ctypes.data,
             case_value_offsets=self._keep[7].ctypes.data,
+            axis_sign_power=self._keep[8].ctypes.data,
         )
         handle = _capi.vp()
         _capi.check(

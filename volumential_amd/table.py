@@ -72,6 +72,7 @@ class NearFieldInteractionTable:
         self.kernel_exterior_normalizers = np.zeros(self.n_q_points, dtype=self.dtype)
         self.reduced_entry_ids = None
         self._data = None
+        self._compact = None
         self.table_data_is_symmetry_reduced = False
         self.is_built = False
         self.last_build_seconds = None
@@ -94,7 +95,8 @@ class NearFieldInteractionTable:
         return out
 
     def arithmetic_metadata(self) -> orbit.ArithmeticCaseMetadata:
-        return orbit.build_case_metadata(self.dim, self.quad_order)
+        """Orbit metadata of this table's kernel (signed for axis derivatives)."""
+        return orbit.kernel_case_metadata(self.integral_knl, self.quad_order)
 
     def representative_entry_ids(self) -> np.ndarray:
         """Canonical representatives in compact-row order."""
@@ -129,6 +131,7 @@ class NearFieldInteractionTable:
                 raise ValueError("entry_ids must be unique")
         self.reduced_entry_ids = np.ascontiguousarray(entry_ids)
         self._data = np.ascontiguousarray(values)
+        self._compact = None
         self.table_data_is_symmetry_reduced = True
 
     def get_reduced_table_data(self):
@@ -147,19 +150,41 @@ class NearFieldInteractionTable:
         return self._data[pos]
 
     def compact_values(self) -> np.ndarray:
-        """Values in compact-row order (the device layout)."""
-        return self.get_entry_data_for_full_indices(self.representative_entry_ids())
+        """Values in compact-row order (the device layout).
+
+        Every stored entry ``e`` lands on its row as ``sign(e) * value(e)``
+        (``sign`` is +1 for scalar kernels), so any choice of stored orbit
+        members works -- this build's representatives or the reference's.
+        """
+        if self._data is None:
+            raise RuntimeError("table data has not been built")
+        if getattr(self, "_compact", None) is not None:
+            return self._compact
+        meta = self.arithmetic_metadata()
+        ids = self.reduced_entry_ids
+        case = ids // self.n_pairs
+        pair = ids % self.n_pairs
+        s, t = pair // self.n_q_points, pair % self.n_q_points
+        rows = orbit.entry_rows(meta, case, s, t)
+        sg = orbit.entry_signs(meta, case, s, t)
+        out = np.zeros(meta.n_compact_entries, dtype=self._data.dtype)
+        hit = np.zeros(meta.n_compact_entries, dtype=bool)
+        out[rows] = sg * self._data
+        hit[rows] = True
+        if not hit.all():
+            raise RuntimeError(
+                "missing canonical table entry data for compact orbit-reduced table"
+            )
+        self._compact = out
+        return out
 
     def get_entry_data(self, entry_id):
         """Value of any full entry, reconstructed through its orbit."""
         info = self.decode_index(int(entry_id))
-        row = orbit.entry_rows(
-            self.arithmetic_metadata(),
-            info["case_index"],
-            info["source_mode_index"],
-            info["target_point_index"],
-        )
-        return self.compact_values()[int(row)]
+        args = (self.arithmetic_metadata(), info["case_index"], info["source_mode_index"],
+                info["target_point_index"])
+        row = orbit.entry_rows(*args)
+        return int(orbit.entry_signs(*args)) * self.compact_values()[int(row)]
 
     # --------------------------------------------------------------- build
     def build_normalizer_table(self):

@@ -21,6 +21,8 @@
  *                        batched over RHS, FP64 or split-precision form
  *   vl1_duffy_build      nearfield_potential_table.py:2034-2213,2328-2463
  *                        (duffy_invariant_fused_table kernel and its driver)
+ *   vl1_split_correction expansion_wrangler_fpnd.py:3491-3836, 212-312
+ *                        (Helmholtz split correction, order 1)
  *   vl1_orbit_rows_host  orbit_arithmetic.py:566-634 (host twin of the
  *                        device compact-row function, for CPU-side checks)
  */
@@ -238,6 +240,40 @@ typedef struct vl1_duffy_args {
 } vl1_duffy_args;
 
 int vl1_duffy_build(const vl1_duffy_args* a, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Helmholtz split correction of list 1 (split order 1).               */
+/* Replaces eval_direct_helmholtz_split_correction,                    */
+/* expansion_wrangler_fpnd.py:3491-3836, with the series remainder     */
+/* kernel of 212-312 and the self limit of 5293-5317.                  */
+/* ------------------------------------------------------------------ */
+typedef struct vl1_split_args {
+  int dim;                 /* 2 or 3 */
+  int split_order;         /* 1 (the only order implemented) */
+  int series_nmax;         /* series terms, 1..96 (expansion_wrangler_fpnd.py:4337-4389) */
+  int n_tgt_boxes;
+  int n_rhs;
+  int max_box_points;      /* largest box_counts entry (sizes the lane layout) */
+  double k_re, k_im;       /* wave number (Yukawa: k = i lambda) */
+  int64_t n_points;
+  /* device pointers */
+  const int32_t* target_boxes;   /* [n_tgt_boxes] */
+  const int32_t* nbr_starts;     /* [n_tgt_boxes + 1] list-1 CSR */
+  const int32_t* nbr_lists;
+  const int32_t* box_starts;     /* first point of each box (sources = targets) */
+  const int32_t* box_counts;     /* points of each box */
+  const double* points;          /* [dim][n_points] */
+  const double* strengths;       /* [n_rhs][ld_strengths] complex (re, im) = density x weight */
+  int64_t ld_strengths;          /* doubles */
+  double* result;                /* [n_rhs][ld_result] complex; targets of the listed boxes
+                                    overwritten */
+  int64_t ld_result;             /* doubles */
+} vl1_split_args;
+
+/* corr[t] = sum over the list-1 source points s of R(|x_t - y_s|) strength[s],
+ * R = G_k - G_0 as the reference's series (self pairs: its r -> 0 limit).
+ * Synchronous (returns after the launch completes). */
+int vl1_split_correction(const vl1_split_args* a, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* List-1 CSR of a 2:1-balanced tree of leaves, built on the GPU.      */

@@ -1,0 +1,143 @@
+"""Oracle: the Helmholtz split correction of list 1 (order 1), numpy.
+
+TEST INFRASTRUCTURE ONLY.  Restates, for ``helmholtz_split_order = 1`` on
+the base quadrature (the reference's default path):
+
+* ``expansion_wrangler_fpnd.py:212-312`` -- ``_HelmholtzSplitSeriesRemainderKernel``:
+  the smooth remainder ``G_k - G_0`` as a truncated series,
+  2D ``c0 + sum_n s_n r^(2n) [-(1/2pi) log r + (H_n - log(k/2) - gamma)/(2pi) + i/4]``
+  with ``s_n = (-1)^n (k^2/4)^n / (n!)^2``, ``c0 = i/4 - (log(k/2) + gamma)/(2pi)``;
+  3D ``sum_{n>=1} (ik)^n r^(n-1) / (4 pi n!)``;
+* ``4325-4389`` -- the series length rule (``_helmholtz_split_series_nmax``)
+  from ``|k| * 2 sqrt(d) * (extent of the coarsest active box)``;
+* ``5293-5317`` -- the self-pair limit (``c0`` in 2D, ``ik/(4 pi)`` in 3D),
+  which the series also gives at r = 0;
+* ``3491-3836`` -- the correction: a P2P sum over list 1 of
+  ``remainder(x_t - y_s) * strength_s`` (strength = density x weight),
+  self pairs excluded and replaced by ``limit * strength_t``;
+* Yukawa: ``k = i lambda`` (``4175-4206``).
+
+Pins (tests): the series equals the closed form ``G_k - G_0`` from scipy's
+Hankel function (2D) / ``exp`` (3D) where the series is well conditioned,
+and its r -> 0 value equals the limit.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+EULER_GAMMA = 0.5772156649015329
+
+
+def series_nmax(dim, k, r_max, split_order=1):
+    """``_helmholtz_split_series_nmax`` (4337-4389)."""
+    if split_order < 1:
+        raise ValueError("split_order must be >= 1")
+    abs_k = float(abs(complex(k)))
+    z_max = abs_k * float(r_max)
+    tol, max_n = 1.0e-16, 96
+    start_n = 1
+    if z_max == 0.0:
+        return start_n
+    if dim == 3:
+        mag = 1.0
+        for i in range(1, start_n + 1):
+            mag = mag * z_max / i
+        n = start_n
+        while n < max_n and mag > tol:
+            n += 1
+            mag = mag * z_max / n
+        return min(max_n, n + 2)
+    z2 = 0.25 * z_max * z_max
+    mag = 1.0
+    for i in range(1, start_n + 1):
+        mag = mag * z2 / (i * i)
+    log_r = abs(math.log(max(r_max, 1e-300)))
+    log_k = abs(math.log(max(0.5 * abs_k, 1e-300)))
+    scale = max(1.0, log_r, log_k + abs(EULER_GAMMA))
+    n = start_n
+    while n < max_n and mag * scale > tol:
+        n += 1
+        mag = mag * z2 / (n * n)
+    return min(max_n, n + 2)
+
+
+def max_neighbor_distance(tree):
+    """``_helmholtz_split_max_neighbor_distance`` (4325-4335)."""
+    lv = np.asarray(tree.box_levels)[np.asarray(tree.box_target_counts_nonchild) > 0]
+    if lv.size == 0:
+        return 0.0
+    return float(2.0 * np.sqrt(tree.dim) * tree.root_extent * 0.5 ** int(lv.min()))
+
+
+def remainder(dim, k, nmax, r):
+    """The series remainder kernel at distances ``r`` (any shape), complex."""
+    k = complex(k)
+    r = np.asarray(r, dtype=np.float64)
+    out = np.zeros(r.shape, dtype=np.complex128)
+    if dim == 2:
+        if k == 0:
+            return out
+        lkh = np.log(0.5 * k)
+        out += 0.25j - (1.0 / (2.0 * np.pi)) * (lkh + EULER_GAMMA)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            logr = np.where(r <= 1e-300, 0.0, np.log(np.where(r > 0, r, 1.0)))
+        for n in range(1, nmax + 1):
+            s_n = ((-1) ** n) * (k * k / 4.0) ** n / (math.factorial(n) ** 2)
+            h_n = float(np.sum(1.0 / np.arange(1, n + 1)))
+            c_log = -s_n / (2.0 * np.pi)
+            c_pow = s_n * ((h_n - (lkh + EULER_GAMMA)) / (2.0 * np.pi) + 0.25j)
+            rp = r ** (2 * n)
+            out += c_log * np.where(r <= 1e-300, 0.0, rp * logr) + c_pow * rp
+        return out
+    for n in range(1, nmax + 1):
+        out += (1j * k) ** n / (4.0 * np.pi * math.factorial(n)) * r ** (n - 1)
+    return out
+
+
+def self_limit(dim, k):
+    """``_helmholtz_split_self_diagonal_limit`` (5293-5317)."""
+    k = complex(k)
+    if dim == 3:
+        return 1j * k / (4.0 * np.pi)
+    if k == 0:
+        return 0.0j
+    return 0.25j - (1.0 / (2.0 * np.pi)) * (np.log(0.5 * k) + EULER_GAMMA)
+
+
+def closed_form_difference(dim, k, r):
+    """``G_k(r) - G_0(r)`` from library functions (the pin of ``remainder``)."""
+    r = np.asarray(r, dtype=np.float64)
+    if dim == 3:
+        return (np.exp(1j * k * r) - 1.0) / (4.0 * np.pi * r)
+    from scipy.special import hankel1
+
+    return 0.25j * hankel1(0, k * r) + np.log(r) / (2.0 * np.pi)
+
+
+def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=None):
+    """P2P of the remainder over list 1: ``out[R, n_points]`` (complex);
+    targets of unlisted boxes stay zero."""
+    st = np.atleast_2d(np.asarray(strengths, dtype=np.complex128))
+    tb_list = tree.target_boxes if target_boxes is None else target_boxes
+    starts = tree.neighbor_source_boxes_starts if starts is None else starts
+    lists = tree.neighbor_source_boxes_lists if lists is None else lists
+    out = np.zeros(st.shape, dtype=np.complex128)
+    pts = np.asarray(tree.points)
+    bs, bc = tree.box_target_starts, tree.box_target_counts_nonchild
+    for i, tb in enumerate(tb_list):
+        t0, nt = int(bs[tb]), int(bc[tb])
+        xt = pts[:, t0:t0 + nt]
+        acc = np.zeros((st.shape[0], nt), dtype=np.complex128)
+        for sb in lists[starts[i]:starts[i + 1]]:
+            s0, ns = int(bs[sb]), int(bc[sb])
+            ys = pts[:, s0:s0 + ns]
+            r = np.sqrt(np.sum((xt[:, :, None] - ys[:, None, :]) ** 2, axis=0))  # [nt, ns]
+            rem = remainder(tree.dim, k, nmax, r)
+            same = (xt[:, :, None] == ys[:, None, :]).all(axis=0)
+            rem = np.where(same, self_limit(tree.dim, k), rem)
+            acc += st[:, s0:s0 + ns] @ rem.T
+        out[:, t0:t0 + nt] = acc
+    return out

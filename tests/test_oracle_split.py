@@ -1,0 +1,95 @@
+"""Pins of the Helmholtz split-correction oracle (CPU).
+
+* the series remainder kernel equals the closed form G_k - G_0 (scipy's
+  Hankel function in 2D, exp in 3D) wherever the series is well conditioned
+  (|k| r_max <= 4), for real and Yukawa (k = i lambda) wave numbers;
+* its r -> 0 value equals the self-pair limit;
+* the series length rule stops when the term magnitude falls below 1e-16
+  (closed-form check of the last term) and grows with |k| r_max;
+* the list-1 P2P correction is linear in the strengths and, for one source
+  box, equals the explicit double sum.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import split_ref as S
+from volumential_amd import tree as vtree
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("k", [1.0, 5.0, 2.0j, 3.0 + 0.5j])
+def test_series_matches_closed_form(dim, k):
+    if dim == 2 and complex(k).imag != 0 and complex(k).real != 0:
+        pytest.skip("scipy closed form checked for real and imaginary k")
+    r_max = 4.0 / abs(k)
+    nmax = S.series_nmax(dim, k, r_max)
+    r = np.linspace(1e-3, r_max, 97)
+    ser = S.remainder(dim, k, nmax, r)
+    if dim == 2 and complex(k).real == 0:
+        # Yukawa in 2D: G = K0(lam r)/(2 pi) = (i/4) H0(i lam r)
+        from scipy.special import k0
+
+        lam = complex(k).imag
+        cf = k0(lam * r) / (2 * np.pi) + np.log(r) / (2 * np.pi)
+    else:
+        cf = S.closed_form_difference(dim, k, r)
+    err = np.max(np.abs(ser - cf)) / max(1.0, np.max(np.abs(cf)))
+    assert err <= 1e-13, err
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_series_at_zero_is_self_limit(dim):
+    for k in (0.7, 12.0, 1.5j):
+        nmax = S.series_nmax(dim, k, 1.0)
+        assert abs(S.remainder(dim, k, nmax, np.array([0.0]))[0] - S.self_limit(dim, k)) < 1e-15
+        # and the closed form approaches it
+        if complex(k).imag == 0:
+            r = 1e-7
+            assert abs(S.closed_form_difference(dim, k, r) - S.self_limit(dim, k)) < 1e-6
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_series_nmax_rule(dim):
+    n1 = S.series_nmax(dim, 1.0, 0.1)
+    n2 = S.series_nmax(dim, 40.0, 0.35)
+    assert 1 < n1 < n2 <= 96
+    assert S.series_nmax(dim, 0.0, 1.0) == 1
+    # the last accepted term magnitude is below tol (the rule adds 2 terms)
+    z = 40.0 * 0.35
+    n = n2 - 2
+    if dim == 3:
+        assert z**n / math.factorial(n) <= 1e-16
+    else:
+        assert (z * z / 4) ** n / math.factorial(n) ** 2 * max(1.0, abs(math.log(20.0)) + 0.5773) <= 1e-16
+
+
+def test_correction_linear_and_explicit():
+    tr = vtree.uniform_tree(2, 3, 2)
+    k = 7.0
+    nmax = S.series_nmax(2, k, S.max_neighbor_distance(tr))
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(tr.n_points) + 1j * rng.standard_normal(tr.n_points)
+    b = rng.standard_normal(tr.n_points)
+    ca = S.correction(tr, a, k, nmax)[0]
+    cb = S.correction(tr, b, k, nmax)[0]
+    cab = S.correction(tr, np.stack([a, 2 * a - 3 * b]), k, nmax)
+    assert np.allclose(cab[0], ca, rtol=0, atol=1e-14)
+    assert np.allclose(cab[1], 2 * ca - 3 * cb, rtol=0, atol=1e-13)
+    # one target box, explicit double sum
+    i = 5
+    tb = tr.target_boxes[i]
+    n = tr.n_q_points
+    exp = np.zeros(n, complex)
+    for sb in tr.neighbor_source_boxes_lists[tr.neighbor_source_boxes_starts[i]:
+                                             tr.neighbor_source_boxes_starts[i + 1]]:
+        for t in range(n):
+            for s in range(n):
+                xt = tr.points[:, tb * n + t]
+                ys = tr.points[:, sb * n + s]
+                r = float(np.linalg.norm(xt - ys))
+                val = S.self_limit(2, k) if r == 0 else S.remainder(2, k, nmax, np.array([r]))[0]
+                exp[t] += val * a[sb * n + s]
+    assert np.allclose(ca[tb * n:(tb + 1) * n], exp, rtol=1e-13, atol=1e-13)

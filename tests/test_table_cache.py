@@ -84,3 +84,36 @@ def test_payload_arrays_and_dense_form():
     np.savez(fh, **arrs)
     u = table_from_payload(fh.getvalue(), 4, 2, LaplaceKernel(2), 2.0)
     assert np.array_equal(u.compact_values(), t.compact_values())
+
+
+def test_signed_table_any_representative_convention(tmp_path):
+    """A gradient table stored at other orbit members (as another builder
+    may pick them) reads back to the same compact values: each stored entry
+    e lands on its row as sign(e) * value(e).  Round trip through the cache."""
+    from oracle import orbit_ref
+    from volumential_amd.kernels import AxisTargetDerivative
+
+    kernel = AxisTargetDerivative(0, LaplaceKernel(2))
+    t = _table(kernel, 3, 2, 2.0, seed=5)
+    meta = t.arithmetic_metadata()
+    n = t.n_q_points
+    compact = t.compact_values()
+    # largest member of every orbit under the kernel's group, with its value
+    ids = np.arange(t.n_cases * n * n)
+    best = ids.copy()
+    for sgn, perm, _ in orbit_ref.group_transforms(2, (0,), (1, 0)):
+        best = np.maximum(best, orbit_ref.transform_entries(2, 3, sgn, perm))
+    members = np.unique(best)
+    c, s, tt = members // (n * n), (members % (n * n)) // n, members % n
+    vals = orbit.entry_signs(meta, c, s, tt) * compact[orbit.entry_rows(meta, c, s, tt)]
+    assert not np.array_equal(np.sort(members), np.sort(t.get_reduced_table_data()[0]))
+    u = NearFieldInteractionTable(3, 2, kernel, source_box_extent=2.0)
+    u.build_normalizer_table()
+    u.set_reduced_table_data(members, vals)
+    assert np.array_equal(u.compact_values(), compact)
+    cache = TableCache(tmp_path / "g.sqlite", root_extent=2.0)
+    cache.save(u, "LaplaceGradX", 0)
+    w = cache.load(2, "LaplaceGradX", 3, 0, kernel)
+    assert np.array_equal(w.compact_values(), compact)
+    for e in members[:50]:
+        assert w.get_entry_data(int(e)) == u.get_entry_data(int(e))

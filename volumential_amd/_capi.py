@@ -55,6 +55,7 @@ EXPORTED_SYMBOLS = (
     "vl1_csr_builder_fill",
     "vl1_csr_builder_destroy",
     "vl1_duffy_build",
+    "vl1_split_correction",
     "vl1_orbit_rows_host",
     "vl1_orbit_signs_host",
 )
@@ -171,6 +172,30 @@ class DuffyArgs(C.Structure):
     ]
 
 
+class SplitArgs(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("split_order", C.c_int),
+        ("series_nmax", C.c_int),
+        ("n_tgt_boxes", C.c_int),
+        ("n_rhs", C.c_int),
+        ("max_box_points", C.c_int),
+        ("k_re", f64),
+        ("k_im", f64),
+        ("n_points", i64),
+        ("target_boxes", vp),
+        ("nbr_starts", vp),
+        ("nbr_lists", vp),
+        ("box_starts", vp),
+        ("box_counts", vp),
+        ("points", vp),
+        ("strengths", vp),
+        ("ld_strengths", i64),
+        ("result", vp),
+        ("ld_result", i64),
+    ]
+
+
 _lib = None
 
 
@@ -200,6 +225,7 @@ def lib():
     L.vl1_plan_tune.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, i64, C.POINTER(SplitOpts),
                                 C.c_int, C.POINTER(i32), C.POINTER(f64), vp]
     L.vl1_duffy_build.argtypes = [C.POINTER(DuffyArgs), vp]
+    L.vl1_split_correction.argtypes = [C.POINTER(SplitArgs), vp]
     L.vl1_csr_builder_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, vp, vp, C.POINTER(i64), vp]
     L.vl1_csr_builder_fill.argtypes = [vp, vp, vp, vp]
     L.vl1_csr_builder_destroy.argtypes = [vp]

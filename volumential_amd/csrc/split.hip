@@ -1,0 +1,300 @@
+// Helmholtz split correction of list 1 (split order 1, base quadrature).
+//
+// With the split, list 1 of a Helmholtz (or Yukawa, k = i lambda) potential
+// is the Laplace-table evaluation plus an online point-to-point sum of the
+// smooth remainder R = G_k - G_0 over every list-1 (target box, source box)
+// pair (volumential/expansion_wrangler_fpnd.py:3491-3836):
+//   corr[t] = sum_{s in list-1 boxes of t} R(|x_t - y_s|) * strength[s],
+// strength = density x quadrature weight.  R is the reference's series
+// (_HelmholtzSplitSeriesRemainderKernel, 212-312), evaluated by Horner:
+//   2D  R = c0 + P_b(r^2) - log(r) / (2 pi) * P_a(r^2),
+//       P_a(u) = sum_{n=1..N} s_n u^n,  s_n = (-1)^n (k^2/4)^n / (n!)^2,
+//       P_b(u) = sum_n s_n ((H_n - log(k/2) - gamma) / (2 pi) + i/4) u^n,
+//       c0 = i/4 - (log(k/2) + gamma) / (2 pi)  (the log term is 0 at r = 0);
+//   3D  R = sum_{n=1..N} (i k)^n / (4 pi n!) r^(n-1).
+// At r = 0 both give the self-pair limit of 5293-5317, so self pairs need no
+// special case.  N is the caller's series length (4337-4389).
+//
+// GPU mapping: one workgroup per target box.  Each listed source box's
+// point coordinates and complex strengths (a chunk of RC densities) are
+// staged in LDS; a lane owns one target point and a share of the source
+// points; R is evaluated once per (target, source) pair and applied to all
+// RC densities; partial sums are reduced through LDS.  The series
+// coefficients sit in LDS for the whole launch.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace vl1 {
+
+constexpr int kSplitBlock = 256;
+constexpr int kSplitMaxTerms = 96;
+constexpr double kEulerGamma = 0.5772156649015329;
+
+struct SplitParams {
+  int dim, n_rhs, nt, n_terms;
+  const double* coef;  // 2D: c0, then (a_n, b_n) n = 1..N; 3D: c_n n = 1..N; complex
+  const int32_t* target_boxes;
+  const int32_t* nbr_starts;
+  const int32_t* nbr_lists;
+  const int32_t* box_starts;
+  const int32_t* box_counts;
+  const double* points;  // [dim][n_points]
+  int64_t n_points;
+  const double* strengths;  // [n_rhs][lds] interleaved complex
+  int64_t lds;
+  double* result;  // [n_rhs][ldr] interleaved complex
+  int64_t ldr;
+};
+
+template <int DIM>
+__device__ inline void remainder_value(double r2, const double* __restrict__ cf, int N, double& vr,
+                                       double& vi) {
+  if (DIM == 2) {
+    // Horner in u = r^2 for P_a and P_b (coefficients n = N .. 1)
+    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+    for (int n = N; n >= 1; --n) {
+      const double* c = cf + 2 + 4 * (n - 1);
+      const double nar = fma(ar, r2, c[0]), nai = fma(ai, r2, c[1]);
+      const double nbr = fma(br, r2, c[2]), nbi = fma(bi, r2, c[3]);
+      ar = nar;
+      ai = nai;
+      br = nbr;
+      bi = nbi;
+    }
+    ar *= r2;
+    ai *= r2;
+    br *= r2;
+    bi *= r2;
+    const double lg = r2 > 0.0 ? -0.25 / M_PI * log(r2) : 0.0;  // -log(r) / (2 pi)
+    vr = cf[0] + br + lg * ar;
+    vi = cf[1] + bi + lg * ai;
+  } else {
+    const double r = sqrt(r2);
+    double pr = 0.0, pi = 0.0;
+    for (int n = N; n >= 1; --n) {
+      const double* c = cf + 2 * (n - 1);
+      const double npr = fma(pr, r, c[0]), npi = fma(pi, r, c[1]);
+      pr = npr;
+      pi = npi;
+    }
+    vr = pr;
+    vi = pi;
+  }
+}
+
+template <int DIM, int RC>
+__global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const SplitParams p) {
+  extern __shared__ double smem[];
+  const int N = p.n_terms;
+  const int ncoef = DIM == 2 ? 2 + 4 * N : 2 * N;
+  const int P = kSplitBlock / p.nt;
+  double* sC = smem;                 // series coefficients
+  double* sX = sC + ncoef;           // source coordinates [DIM][nt * P] (chunked)
+  const int chunk = p.nt * P;        // source points staged per pass
+  double* sF = sX + DIM * chunk;     // strengths [chunk][RC][2]
+  double* red = sF + (size_t)chunk * RC * 2;
+  const int tid = threadIdx.x;
+  const int lane_t = tid % p.nt;
+  const int part = tid / p.nt;
+  for (int i = tid; i < ncoef; i += kSplitBlock) sC[i] = p.coef[i];
+
+  const int tbox = blockIdx.x;
+  const int tb = p.target_boxes[tbox];
+  const int tbeg = p.box_starts[tb];
+  const int ntb = p.box_counts[tb];
+  const int l0 = p.nbr_starts[tbox], l1 = p.nbr_starts[tbox + 1];
+
+  for (int r0 = 0; r0 < p.n_rhs; r0 += RC) {
+    const int rc = min(RC, p.n_rhs - r0);
+    for (int t0 = 0; t0 < ntb; t0 += p.nt) {
+      const int ti = t0 + lane_t;
+      const bool tact = ti < ntb;
+      double xt[DIM];
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) xt[a] = tact ? p.points[(int64_t)a * p.n_points + tbeg + ti] : 0.0;
+      double acc[RC], aci[RC];
+#pragma unroll
+      for (int r = 0; r < RC; ++r) acc[r] = aci[r] = 0.0;
+      for (int li = l0; li < l1; ++li) {
+        const int sb = p.nbr_lists[li];
+        const int sbeg = p.box_starts[sb];
+        const int ns = p.box_counts[sb];
+        for (int c0 = 0; c0 < ns; c0 += chunk) {
+          const int nc = min(chunk, ns - c0);
+          __syncthreads();  // previous chunk done with LDS (and coefficients staged)
+          for (int i = tid; i < DIM * nc; i += kSplitBlock) {
+            const int a = i / nc, j = i % nc;
+            sX[a * chunk + j] = p.points[(int64_t)a * p.n_points + sbeg + c0 + j];
+          }
+          for (int i = tid; i < nc * RC * 2; i += kSplitBlock) {
+            const int c = i & 1, rj = i >> 1;
+            const int j = rj / RC, r = rj % RC;
+            sF[i] = r < rc ? p.strengths[(int64_t)(r0 + r) * p.lds + 2 * (int64_t)(sbeg + c0 + j) + c]
+                           : 0.0;
+          }
+          __syncthreads();
+          if (tact) {
+            for (int j = part; j < nc; j += P) {
+              double r2 = 0.0;
+#pragma unroll
+              for (int a = 0; a < DIM; ++a) {
+                const double d = xt[a] - sX[a * chunk + j];
+                r2 = fma(d, d, r2);
+              }
+              double vr, vi;
+              remainder_value<DIM>(r2, sC, N, vr, vi);
+              const double* f = sF + (size_t)j * RC * 2;
+#pragma unroll
+              for (int r = 0; r < RC; ++r) {
+                const double fr = f[2 * r], fi = f[2 * r + 1];
+                acc[r] = fma(vr, fr, fma(-vi, fi, acc[r]));
+                aci[r] = fma(vr, fi, fma(vi, fr, aci[r]));
+              }
+            }
+          }
+        }
+      }
+      if (P > 1) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RC; ++r) {
+          red[((part * p.nt + lane_t) * RC + r) * 2] = acc[r];
+          red[((part * p.nt + lane_t) * RC + r) * 2 + 1] = aci[r];
+        }
+        __syncthreads();
+        if (part == 0)
+          for (int q = 1; q < P; ++q)
+#pragma unroll
+            for (int r = 0; r < RC; ++r) {
+              acc[r] += red[((q * p.nt + lane_t) * RC + r) * 2];
+              aci[r] += red[((q * p.nt + lane_t) * RC + r) * 2 + 1];
+            }
+      }
+      if (part == 0 && tact)
+        for (int r = 0; r < rc; ++r) {
+          p.result[(int64_t)(r0 + r) * p.ldr + 2 * (int64_t)(tbeg + ti)] = acc[r];
+          p.result[(int64_t)(r0 + r) * p.ldr + 2 * (int64_t)(tbeg + ti) + 1] = aci[r];
+        }
+      __syncthreads();  // reduction buffer reused by the next target chunk
+    }
+  }
+}
+
+// Series coefficients (interleaved complex), the reference's formulas.
+static std::vector<double> split_coefficients(int dim, std::complex<double> k, int N) {
+  using cd = std::complex<double>;
+  std::vector<double> out;
+  if (dim == 2) {
+    const bool zero = std::abs(k) == 0.0;
+    const cd L = zero ? cd(0.0) : std::log(0.5 * k) + kEulerGamma;
+    const cd c0 = zero ? cd(0.0) : cd(0.0, 0.25) - L / (2.0 * M_PI);
+    out.push_back(c0.real());
+    out.push_back(c0.imag());
+    double fact = 1.0, harm = 0.0;
+    cd kp = 1.0;
+    for (int n = 1; n <= N; ++n) {
+      fact *= n;
+      harm += 1.0 / n;
+      kp *= -k * k / 4.0;  // (-1)^n (k^2/4)^n
+      const cd s = zero ? cd(0.0) : kp / (fact * fact);
+      const cd b = s * ((harm - L) / (2.0 * M_PI) + cd(0.0, 0.25));
+      out.push_back(s.real());
+      out.push_back(s.imag());
+      out.push_back(b.real());
+      out.push_back(b.imag());
+    }
+  } else {
+    double fact = 1.0;
+    cd ik = cd(0.0, 1.0) * k, p = 1.0;
+    for (int n = 1; n <= N; ++n) {
+      fact *= n;
+      p *= ik;
+      const cd c = p / (4.0 * M_PI * fact);
+      out.push_back(c.real());
+      out.push_back(c.imag());
+    }
+  }
+  return out;
+}
+
+template <int DIM, int RC>
+static hipError_t launch_split(const SplitParams& prm, int grid, size_t smem, hipStream_t s) {
+  auto fn = split_correction_kernel<DIM, RC>;
+  hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)smem);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(kSplitBlock), smem, s, prm);
+  return hipGetLastError();
+}
+
+}  // namespace vl1
+
+using namespace vl1;
+
+extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
+  clear_error();
+  if (!a) return fail(VL1_E_ARG, "vl1_split_correction: null argument");
+  if (a->dim != 2 && a->dim != 3) return fail(VL1_E_ARG, "vl1_split_correction: dim must be 2 or 3");
+  if (a->split_order != 1)
+    return fail(VL1_E_UNSUPPORTED, "vl1_split_correction: only helmholtz_split_order = 1 is implemented");
+  if (a->series_nmax < 1 || a->series_nmax > kSplitMaxTerms)
+    return fail(VL1_E_ARG, "vl1_split_correction: series_nmax must be 1..96");
+  if (a->n_tgt_boxes < 0 || a->n_rhs < 1 || a->n_points < 1 || a->max_box_points < 1)
+    return fail(VL1_E_ARG, "vl1_split_correction: bad sizes");
+  if (a->n_tgt_boxes == 0) return VL1_OK;
+  if (!a->target_boxes || !a->nbr_starts || !a->nbr_lists || !a->box_starts || !a->box_counts ||
+      !a->points || !a->strengths || !a->result)
+    return fail(VL1_E_ARG, "vl1_split_correction: null device array");
+  if (a->n_rhs > 1 && (a->ld_strengths < 2 * a->n_points || a->ld_result < 2 * a->n_points))
+    return fail(VL1_E_ARG, "vl1_split_correction: bad leading dimension");
+  hipStream_t s = (hipStream_t)stream;
+  const std::complex<double> k(a->k_re, a->k_im);
+  std::vector<double> coef = split_coefficients(a->dim, k, a->series_nmax);
+  DevBuf dcoef;
+  hipError_t e = upload(dcoef, coef.data(), coef.size() * sizeof(double), s);
+  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_split_correction: ") + hipGetErrorString(e));
+
+  SplitParams prm{};
+  prm.dim = a->dim;
+  prm.n_rhs = a->n_rhs;
+  prm.n_terms = a->series_nmax;
+  prm.coef = dcoef.as<double>();
+  prm.target_boxes = a->target_boxes;
+  prm.nbr_starts = a->nbr_starts;
+  prm.nbr_lists = a->nbr_lists;
+  prm.box_starts = a->box_starts;
+  prm.box_counts = a->box_counts;
+  prm.points = a->points;
+  prm.n_points = a->n_points;
+  prm.strengths = a->strengths;
+  prm.lds = a->n_rhs > 1 ? a->ld_strengths : 2 * a->n_points;
+  prm.result = a->result;
+  prm.ldr = a->n_rhs > 1 ? a->ld_result : 2 * a->n_points;
+  int nt = 1;
+  while (nt < a->max_box_points && nt < kSplitBlock) nt *= 2;
+  prm.nt = nt;
+  const int P = kSplitBlock / nt;
+  const int RC = a->n_rhs == 1 ? 1 : (a->n_rhs <= 4 ? 4 : 8);
+  const int chunk = nt * P;
+  const size_t smem = sizeof(double) * ((size_t)coef.size() + (size_t)a->dim * chunk +
+                                        (size_t)chunk * RC * 2 +
+                                        (P > 1 ? (size_t)kSplitBlock * RC * 2 : 0));
+  if (smem > 160 * 1024) return fail(VL1_E_UNSUPPORTED, "vl1_split_correction: LDS budget exceeded");
+  const int grid = a->n_tgt_boxes;
+  if (a->dim == 2) {
+    e = RC == 1 ? launch_split<2, 1>(prm, grid, smem, s)
+                : (RC == 4 ? launch_split<2, 4>(prm, grid, smem, s) : launch_split<2, 8>(prm, grid, smem, s));
+  } else {
+    e = RC == 1 ? launch_split<3, 1>(prm, grid, smem, s)
+                : (RC == 4 ? launch_split<3, 4>(prm, grid, smem, s) : launch_split<3, 8>(prm, grid, smem, s));
+  }
+  // the coefficient buffer is freed when dcoef leaves scope: wait for the launch
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_split_correction: ") + hipGetErrorString(e));
+  return VL1_OK;
+}

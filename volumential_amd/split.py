@@ -1,0 +1,151 @@
+"""Helmholtz split correction of list 1 (``vl1_split_correction``).
+
+With ``helmholtz_split`` the reference evaluates list 1 of a Helmholtz
+potential as the Laplace-table evaluation plus an online point-to-point sum
+of the smooth remainder ``G_k - G_0`` over list 1
+(``expansion_wrangler_fpnd.py:3491-3836``; ``volume_fmm.py:1000-1026`` adds
+the two).  This module holds the host side of the correction for split
+order 1 on the base quadrature: the series length rule
+(``_helmholtz_split_series_nmax``, 4325-4389), the device tree arrays, and
+the launch.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _capi
+
+EULER_GAMMA = 0.5772156649015329
+
+
+def series_nmax(dim, k, r_max, split_order=1):
+    """Series terms so that the first dropped term is below 1e-16 (+2):
+    ``expansion_wrangler_fpnd.py:4337-4389``."""
+    if split_order < 1:
+        raise ValueError("split_order must be >= 1")
+    abs_k = float(abs(complex(k)))
+    z = abs_k * float(r_max)
+    tol, max_n, n = 1.0e-16, 96, 1
+    if z == 0.0:
+        return 1
+    if dim == 3:
+        mag = z
+        while n < max_n and mag > tol:
+            n += 1
+            mag = mag * z / n
+        return min(max_n, n + 2)
+    if dim != 2:
+        raise NotImplementedError("split remainder series supports only 2D/3D")
+    z2 = 0.25 * z * z
+    mag = z2
+    scale = max(1.0, abs(math.log(max(r_max, 1e-300))),
+                abs(math.log(max(0.5 * abs_k, 1e-300))) + EULER_GAMMA)
+    while n < max_n and mag * scale > tol:
+        n += 1
+        mag = mag * z2 / (n * n)
+    return min(max_n, n + 2)
+
+
+def max_neighbor_distance(tree):
+    """``2 sqrt(d)`` times the extent of the coarsest box holding points
+    (``_helmholtz_split_max_neighbor_distance``, 4325-4335)."""
+    lv = np.asarray(tree.box_levels)[np.asarray(tree.box_target_counts_nonchild) > 0]
+    if lv.size == 0:
+        return 0.0
+    return float(2.0 * np.sqrt(tree.dim) * tree.root_extent * 0.5 ** int(lv.min()))
+
+
+def self_limit(dim, k):
+    """The r -> 0 value of the remainder (5293-5317)."""
+    k = complex(k)
+    if dim == 3:
+        return 1j * k / (4.0 * np.pi)
+    if k == 0:
+        return 0j
+    return 0.25j - (np.log(0.5 * k) + EULER_GAMMA) / (2.0 * np.pi)
+
+
+class HelmholtzSplitCorrection:
+    """``corr = sum_{list 1} (G_k - G_0)(x_t - y_s) * strength_s`` on the GPU.
+
+    ``k`` is the Helmholtz wave number (complex allowed; Yukawa ``lambda``
+    is ``k = 1j * lambda``).  Points, box starts and counts come from the
+    tree (sources = targets, tree order)."""
+
+    def __init__(self, tree, k, split_order=1, series_terms=None):
+        if int(split_order) != 1:
+            raise NotImplementedError("only helmholtz_split_order=1 is implemented")
+        self.dim = int(tree.dim)
+        self.k = complex(k)
+        self.split_order = 1
+        self.nmax = int(series_terms) if series_terms is not None else series_nmax(
+            self.dim, self.k, max_neighbor_distance(tree))
+        self.n_points = int(np.asarray(tree.points).shape[1])
+        dev = "cuda"
+        self._points = torch.as_tensor(np.ascontiguousarray(tree.points, np.float64), device=dev)
+        counts = np.ascontiguousarray(tree.box_target_counts_nonchild, np.int32)
+        bstarts = np.ascontiguousarray(tree.box_target_starts, np.int32)
+        if counts.size and (counts.min() < 0 or bstarts.min() < 0
+                            or int((bstarts.astype(np.int64) + counts).max()) > self.n_points):
+            raise ValueError("box starts/counts do not fit the tree's points")
+        self.n_boxes = int(counts.size)
+        self._starts = torch.as_tensor(bstarts, device=dev)
+        self._counts = torch.as_tensor(counts, device=dev)
+        self.max_box_points = int(max(1, counts.max(initial=1)))
+
+    def __call__(self, strengths, target_boxes, starts, lists, out=None, stream=None):
+        """``strengths[R, n_points]`` (or ``[n_points]``), complex; returns the
+        correction with the same shape (targets of unlisted boxes zero, or
+        left as they are in ``out``)."""
+        st = strengths if isinstance(strengths, torch.Tensor) else torch.as_tensor(
+            np.asarray(strengths))
+        one = st.dim() == 1
+        st = (st.unsqueeze(0) if one else st).to(device="cuda", dtype=torch.complex128)
+        st = st.contiguous()
+        if st.shape[1] != self.n_points:
+            raise ValueError("strengths have the wrong number of points")
+        if out is None:
+            out = torch.zeros_like(st)
+        else:
+            out = out.unsqueeze(0) if out.dim() == 1 else out
+            if out.shape != st.shape or out.dtype != torch.complex128 or not out.is_contiguous():
+                raise ValueError("out must be a contiguous complex128 tensor shaped like strengths")
+        tb_h = np.ascontiguousarray(_host(target_boxes), np.int32)
+        ns_h = np.ascontiguousarray(_host(starts), np.int32)
+        nl_h = np.ascontiguousarray(_host(lists), np.int32)
+        if ns_h.size != tb_h.size + 1:
+            raise ValueError("neighbor_source_boxes_starts must have n_target_boxes + 1 entries")
+        if ns_h[0] != 0 or np.any(np.diff(ns_h) < 0) or ns_h[-1] != nl_h.size:
+            raise ValueError("neighbor_source_boxes_starts is not a CSR offset array")
+        for arr in (tb_h, nl_h):
+            if arr.size and (arr.min() < 0 or arr.max() >= self.n_boxes):
+                raise ValueError("box index out of range")
+        tb = torch.as_tensor(tb_h, device="cuda")
+        ns = torch.as_tensor(ns_h, device="cuda")
+        nl = torch.as_tensor(nl_h, device="cuda")
+        sr = torch.view_as_real(st)
+        yr = torch.view_as_real(out)
+        args = _capi.SplitArgs(
+            dim=self.dim, split_order=self.split_order, series_nmax=self.nmax,
+            n_tgt_boxes=int(tb.numel()), n_rhs=int(st.shape[0]),
+            max_box_points=self.max_box_points, k_re=self.k.real, k_im=self.k.imag,
+            n_points=self.n_points,
+            target_boxes=tb.data_ptr(), nbr_starts=ns.data_ptr(),
+            nbr_lists=nl.data_ptr() if nl.numel() else ns.data_ptr(),
+            box_starts=self._starts.data_ptr(), box_counts=self._counts.data_ptr(),
+            points=self._points.data_ptr(), strengths=sr.data_ptr(),
+            ld_strengths=2 * self.n_points, result=yr.data_ptr(), ld_result=2 * self.n_points,
+        )
+        _capi.check(_capi.lib().vl1_split_correction(args, _capi.stream_handle(stream)),
+                    "vl1_split_correction")
+        return out[0] if one else out
+
+
+def _host(a):
+    if isinstance(a, torch.Tensor):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)

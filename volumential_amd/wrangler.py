@@ -14,6 +14,12 @@ kernel repr, ``table_starting_level``, coverage and extent checks) follows
 ``eval_direct_batched`` evaluates many densities ``mode_coefs[R, n_points]``
 in one call (the per-RHS loop of ``volume_fmm.py:983-1031`` collapsed), and
 ``eval_direct_plan`` runs the batched plan forms (dense FP64 or Ozaki split).
+
+Helmholtz split (``helmholtz_split``, 2253-2600 and 3491-3836): a Helmholtz
+output kernel backed by Laplace tables evaluates its table part with the
+Laplace scaling (real and imaginary parts of the density as two real
+densities), and ``eval_direct_helmholtz_split_correction`` adds the online
+remainder ``G_k - G_0`` over list 1 (``split.HelmholtzSplitCorrection``).
 """
 
 from __future__ import annotations
@@ -22,6 +28,7 @@ import numpy as np
 import torch
 
 from . import _capi, gallery
+from .kernels import HelmholtzKernel, LaplaceKernel
 from .list1 import NearFieldFromCSR
 from .payload import prepare_table_data_and_entry_map
 
@@ -70,14 +77,19 @@ class NearFieldWrangler:
         target_kernels=None,
         dtype=np.float64,
         list1_extra_kwargs=None,
-        helmholtz_split=False,
+        helmholtz_split=None,
         direct_method="plan",
+        helmholtz_split_order=1,
     ):
         """``direct_method``: how ``eval_direct`` evaluates list 1 --
         ``"plan"`` (default) through a plan cached per list-1 CSR, FP64
         MFMA form (the same sums as the reference kernel to ~1e-15, at a
         fraction of the per-call cost); ``"plain"`` through the plain FP64
-        kernel with the reference's per-call semantics (``NearFieldFromCSR``)."""
+        kernel with the reference's per-call semantics (``NearFieldFromCSR``).
+
+        ``helmholtz_split``: ``None`` enables the split for Helmholtz output
+        kernels whose tables are Laplace tables (``expansion_wrangler_fpnd.py:
+        2482-2520``), ``True`` requires it, ``False`` disables it."""
         if not getattr(tree, "sources_are_targets", False):
             raise ValueError(
                 "table-based near-field evaluation requires sources and targets "
@@ -88,11 +100,6 @@ class NearFieldWrangler:
         self.direct_method = direct_method
         self.tree = tree
         self.dtype = np.dtype(dtype)
-        if helmholtz_split:
-            raise NotImplementedError(
-                "the Helmholtz split correction is not part of this build; "
-                "pass helmholtz_split=False with per-level Helmholtz tables"
-            )
         if isinstance(near_field_table, dict):
             tables = {k: self._as_list(v) for k, v in near_field_table.items()}
         else:
@@ -125,7 +132,41 @@ class NearFieldWrangler:
                 )
         self._evaluators = {}
         self._plans = {}
+        self._corrections = {}
+        self.helmholtz_split_order = helmholtz_split_order
+        self.helmholtz_split = False
+        if helmholtz_split is None or helmholtz_split:
+            ok, why = self._split_support_status()
+            if ok:
+                if int(helmholtz_split_order) != 1:
+                    raise NotImplementedError("only helmholtz_split_order=1 is implemented")
+                self.helmholtz_split = True
+            elif helmholtz_split:
+                raise RuntimeError(f"helmholtz_split is not supported here: {why}")
         self._plain_only = set()  # kernels the plan route cannot take
+
+    def _split_support_status(self):
+        """``_split_target_kernel_support_status`` /
+        ``_split_base_table_support_status`` (4009-4073, 3964-4007) for the
+        kernels of this build: Helmholtz output kernels (no derivative
+        wrappers) whose tables all hold ``LaplaceKernel(dim)``."""
+        for knl in self.target_kernels:
+            if not isinstance(knl, HelmholtzKernel):
+                return False, f"{knl!r} is not a Helmholtz kernel"
+            want = repr(LaplaceKernel(knl.dim))
+            for lev, t in enumerate(self.near_field_table.get(repr(knl), [])):
+                if repr(t.integral_knl) != want:
+                    return False, (f"{knl!r} level {lev} kernel mismatch: expected {want}, "
+                                   f"got {t.integral_knl!r}")
+        return True, ""
+
+    def _policy_kernel(self, out_kernel):
+        """The kernel whose scaling rule the table part follows: the tables'
+        Laplace kernel under the split (4126-4153, 4801-4833), else the
+        output kernel."""
+        if self.helmholtz_split:
+            return self.near_field_table[repr(out_kernel)][0].integral_knl
+        return out_kernel
 
     @staticmethod
     def _as_list(t):
@@ -148,6 +189,23 @@ class NearFieldWrangler:
 
     # -------------------------------------------------------------- eval
     def _eval_single(self, out_pot, out_kernel, target_boxes, starts, lists, mode_coefs):
+        if self.helmholtz_split:
+            # real Laplace tables: real and imaginary parts as two real densities
+            mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
+                np.asarray(mode_coefs))
+            mc = mc.to(device="cuda", dtype=torch.complex128)
+            one = mc.dim() == 1
+            m2 = mc.unsqueeze(0) if one else mc
+            r = int(m2.shape[0])
+            stacked = torch.cat([m2.real, m2.imag], 0).contiguous()
+            tmp = torch.zeros_like(stacked)
+            self._eval_real(tmp, out_kernel, target_boxes, starts, lists, stacked)
+            res = torch.complex(tmp[:r], tmp[r:])
+            out_pot.copy_(res[0] if one else res)
+            return out_pot
+        return self._eval_real(out_pot, out_kernel, target_boxes, starts, lists, mode_coefs)
+
+    def _eval_real(self, out_pot, out_kernel, target_boxes, starts, lists, mode_coefs):
         kname = repr(out_kernel)
         tables = self.near_field_table[kname]
         t0 = tables[0]
@@ -186,7 +244,8 @@ class NearFieldWrangler:
         }
         key = (kname, len(tables))
         if key not in self._evaluators:
-            self._evaluators[key] = NearFieldFromCSR(out_kernel, shapes, **self.list1_extra_kwargs)
+            self._evaluators[key] = NearFieldFromCSR(self._policy_kernel(out_kernel), shapes,
+                                                     **self.list1_extra_kwargs)
         nf = self._evaluators[key]
         tb = np.asarray(_host(target_boxes), dtype=np.int64)
         ls = np.asarray(_host(lists), dtype=np.int64)
@@ -231,7 +290,7 @@ class NearFieldWrangler:
         multiples of 64)."""
         mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
             np.asarray(mode_coefs))
-        mc = mc.to(device="cuda", dtype=self._dtype(out_kernel))
+        mc = mc.to(device="cuda", dtype=self._dtype(self._policy_kernel(out_kernel)))
         one = mc.dim() == 1
         mc = (mc.unsqueeze(0) if one else mc).contiguous()
         out = out_pot.unsqueeze(0) if one else out_pot
@@ -287,7 +346,7 @@ class NearFieldWrangler:
         payload = DevicePayload(t0.dim, t0.quad_order, data, nmlz, t0.case_indices,
                                 t0.arithmetic_metadata())
         nf = NearFieldFromCSR(
-            out_kernel,
+            self._policy_kernel(out_kernel),
             {"n_tables": len(tables), "n_q_points": t0.n_q_points,
              "n_table_entries": data.shape[1], "n_cases": t0.n_cases,
              "quad_order": t0.quad_order},
@@ -319,10 +378,73 @@ class NearFieldWrangler:
             mc0 = mc0.unsqueeze(0)
         pot = self.output_zeros(int(mc0.shape[0]))
         for i, knl in enumerate(self.target_kernels):
-            mc = mc0.to(device="cuda", dtype=self._dtype(knl)).contiguous()
             plan = self.get_plan(knl, target_boxes, neighbor_source_boxes_starts,
                                  neighbor_source_boxes_lists)
+            if self.helmholtz_split:  # real Laplace plan on [Re f; Im f]
+                mc = mc0.to(device="cuda", dtype=torch.complex128)
+                r = int(mc.shape[0])
+                stacked = torch.cat([mc.real, mc.imag], 0).contiguous()
+                tmp = torch.zeros_like(stacked)
+                plan.eval(stacked, out=tmp, form=form, n_slices=n_slices)
+                pot[i].copy_(torch.complex(tmp[:r], tmp[r:]))
+                continue
+            mc = mc0.to(device="cuda", dtype=self._dtype(knl)).contiguous()
             plan.eval(mc, out=pot[i], form=form, n_slices=n_slices)
+        return pot, TimingFuture()
+
+    # --------------------------------------------------------- split
+    def _correction(self, knl):
+        key = repr(knl)
+        if key not in self._corrections:
+            from .split import HelmholtzSplitCorrection
+
+            self._corrections[key] = HelmholtzSplitCorrection(
+                self.tree, knl.k, split_order=self.helmholtz_split_order)
+        return self._corrections[key]
+
+    def eval_direct_helmholtz_split_correction(self, target_boxes, neighbor_source_boxes_starts,
+                                               neighbor_source_boxes_lists, src_weights,
+                                               src_func=None):
+        """The online list-1 remainder ``sum (G_k - G_0)(x_t - y_s) w_s`` per
+        output kernel (``expansion_wrangler_fpnd.py:3491-3836``), with
+        ``src_weights`` = density x quadrature weight (``[n_points]`` or
+        ``[R, n_points]``).  Zeros when the split is off.  ``src_func`` (the
+        smooth-quadrature path) is accepted and not used: this build
+        implements the base-quadrature path."""
+        if not self.helmholtz_split:
+            return self.output_zeros(), TimingFuture()
+        w = src_weights if isinstance(src_weights, torch.Tensor) else torch.as_tensor(
+            np.asarray(src_weights))
+        pot = np.empty(len(self.target_kernels), dtype=object)
+        for i, knl in enumerate(self.target_kernels):
+            pot[i] = self._correction(knl)(w, target_boxes, neighbor_source_boxes_starts,
+                                           neighbor_source_boxes_lists)
+        return pot, TimingFuture()
+
+    def eval_direct_split(self, target_boxes, neighbor_source_boxes_starts,
+                          neighbor_source_boxes_lists, mode_coefs, form="f64", n_slices=5):
+        """List 1 as ``drive_volume_fmm`` sums it under the split
+        (``volume_fmm.py:983-1026``): the Laplace-table part of
+        ``mode_coefs[R, n_points]`` (plan ``form``) plus the correction with
+        strengths ``mode_coefs * tree.weights``."""
+        if not self.helmholtz_split:
+            raise RuntimeError("eval_direct_split needs helmholtz_split enabled")
+        mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
+            np.asarray(mode_coefs))
+        mc = mc.to(device="cuda", dtype=torch.complex128)
+        one = mc.dim() == 1
+        mc2 = mc.unsqueeze(0) if one else mc
+        pot, _ = self.eval_direct_plan(target_boxes, neighbor_source_boxes_starts,
+                                       neighbor_source_boxes_lists, mc2, form=form,
+                                       n_slices=n_slices)
+        wts = torch.as_tensor(np.asarray(self.tree.weights, np.float64), device="cuda")
+        corr, _ = self.eval_direct_helmholtz_split_correction(
+            target_boxes, neighbor_source_boxes_starts, neighbor_source_boxes_lists,
+            mc2 * wts[None, :])
+        for i in range(len(pot)):
+            pot[i] = pot[i] + corr[i]
+            if one:
+                pot[i] = pot[i][0]
         return pot, TimingFuture()
 
 

@@ -19,6 +19,9 @@
  *   vl1_plan_tune        expansion_wrangler_fpnd.py:3222-3489 (eval_direct
  *                        called once per RHS by volume_fmm.py:983-989);
  *                        batched over RHS, FP64 or split-precision form
+ *   vl1_plan_create_helmholtz_split
+ *                        expansion_wrangler_fpnd.py:3457-3489 + 3491-3836
+ *                        (table part and split correction in one plan)
  *   vl1_duffy_build      nearfield_potential_table.py:2034-2213,2328-2463
  *                        (duffy_invariant_fused_table kernel and its driver)
  *   vl1_split_correction expansion_wrangler_fpnd.py:3491-3836, 212-312
@@ -183,6 +186,27 @@ typedef struct vl1_split_opts {
 } vl1_split_opts;
 
 int vl1_plan_create(vl1_plan** out, const vl1_payload* p, const vl1_plan_desc* d, void* stream);
+
+/* Helmholtz split folded into the plan (split order 1): complex blocks per
+ * (case, source level) holding the Laplace table part (canonical scaling and
+ * the 2D log term folded in) plus the Gauss quadrature of the remainder
+ * G_k - G_0 on the source nodes -- the table part of eval_direct
+ * (expansion_wrangler_fpnd.py:3457-3489) and the correction of
+ * eval_direct_helmholtz_split_correction (3491-3836) in one contraction.
+ * `laplace` is a real scalar Laplace payload, d->scaling_mode canonical; the
+ * plan then evaluates complex densities like a complex-table plan.  In 2D
+ * the box-local node ids must be in mode order (VL1_E_UNSUPPORTED). */
+typedef struct vl1_helmholtz_split_blocks {
+  double k_re, k_im;          /* wave number (Yukawa: k = i lambda) */
+  int series_nmax;            /* series terms, 1..96 (4337-4389) */
+  const double* unit_nodes;   /* [q] Gauss-Legendre nodes on [0, 1] (host) */
+  const double* unit_weights; /* [q] their weights (host) */
+  const int32_t* case_vecs;   /* [n_cases][dim] list-1 gallery vectors (host) */
+} vl1_helmholtz_split_blocks;
+
+int vl1_plan_create_helmholtz_split(vl1_plan** out, const vl1_payload* laplace,
+                                    const vl1_plan_desc* d, const vl1_helmholtz_split_blocks* sp,
+                                    void* stream);
 int vl1_plan_destroy(vl1_plan* plan);
 /* f: [n_rhs][ldf] densities (tree order), y: [n_rhs][ldy] outputs;
  * every target of the plan's target boxes is overwritten. */

@@ -157,3 +157,50 @@ def test_split_total_matches_helmholtz_tables(cuda, dim, level, k, qs, tol):
         assert rem >= 10 * gap, (rem, gap)
         gaps.append(gap)
     assert gaps[-1] <= tol and gaps[-1] < 0.5 * gaps[0], gaps
+
+
+@pytest.mark.parametrize("form,tol", [("f64", 1e-12), ("split", 1e-10)])
+@pytest.mark.parametrize("dim,q,kind,k,n_rhs", [(2, 4, "uniform", 10.0, 3),
+                                                (2, 5, "adaptive", 6.0 + 0.5j, 2),
+                                                (3, 3, "uniform", 5.0, 5),
+                                                (3, 5, "uniform", 2.0 + 0.5j, 1)])
+def test_fused_split_plan_matches_two_launches(cuda, form, tol, dim, q, kind, k, n_rhs):
+    """The split folded into the plan's (case, level) blocks equals the table
+    part plus the point-to-point correction."""
+    print(f"case: fused split plan {dim}D q={q} {kind} k={k} R={n_rhs} {form}", flush=True)
+    tr = vtree.uniform_tree(dim, q, 3 if dim == 2 else 2) if kind == "uniform" else adaptive(
+        dim, q, max_level=5)
+    knl = HelmholtzKernel(dim, k)
+    w = NearFieldWrangler(tr, {repr(knl): [_laplace_table(dim, q)]}, target_kernels=[knl])
+    f = densities.gaussian_mixture_batch(tr.points, range(40, 40 + n_rhs))
+    fc = torch.as_tensor(f * np.exp(0.3j * np.arange(n_rhs))[:, None], device="cuda")
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    fused, _ = w.eval_direct_split(*args, fc, form=form)
+    assert any(key[-1] for key in w._plans), "the fused plan was not used"
+    two, _ = w.eval_direct_split(*args, fc, form="f64", fused=False)
+    l2, linf = rel_errors(fused[0].cpu().numpy(), two[0].cpu().numpy())
+    assert l2 <= tol and linf <= tol, (l2, linf)
+
+
+def test_fused_split_falls_back_beyond_its_limits(cuda):
+    """3D q=6 (432 block rows) and 2D permuted node ids take the two-launch
+    route; the result is the same sum."""
+    print("case: fused split fallbacks", flush=True)
+    for tr in (vtree.uniform_tree(3, 6, 1), vtree.uniform_tree(2, 3, 3)):
+        if tr.dim == 2:
+            rng = np.random.default_rng(4)
+            n = tr.n_q_points
+            tr.local_ids = np.concatenate([rng.permutation(n) for _ in range(tr.nboxes)]).astype(
+                np.int32)
+        knl = HelmholtzKernel(tr.dim, 3.0)
+        w = NearFieldWrangler(tr, {repr(knl): [_laplace_table(tr.dim, tr.quad_order)]},
+                              target_kernels=[knl])
+        f = torch.as_tensor(densities.gaussian_mixture_batch(tr.points, [9]).astype(np.complex128),
+                            device="cuda")
+        args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+        a, _ = w.eval_direct_split(*args, f)
+        assert not any(key[-1] for key in w._plans)
+        b, _ = w.eval_direct_split(*args, f, fused=False)
+        # the tile kernels' XCD phase hint may reorder the step sums: not bitwise
+        l2, linf = rel_errors(a[0].cpu().numpy(), b[0].cpu().numpy())
+        assert l2 <= 1e-14 and linf <= 1e-14, (l2, linf)

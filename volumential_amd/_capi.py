@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "vl1_payload_device_bytes",
     "vl1_near_field_f64",
     "vl1_plan_create",
+    "vl1_plan_create_helmholtz_split",
     "vl1_plan_destroy",
     "vl1_plan_eval",
     "vl1_plan_stage",
@@ -145,6 +146,17 @@ class PlanDesc(C.Structure):
     ]
 
 
+class HelmholtzSplitBlocks(C.Structure):
+    _fields_ = [
+        ("k_re", f64),
+        ("k_im", f64),
+        ("series_nmax", C.c_int),
+        ("unit_nodes", vp),
+        ("unit_weights", vp),
+        ("case_vecs", vp),
+    ]
+
+
 class SplitOpts(C.Structure):
     _fields_ = [("n_slices", C.c_int), ("n_products", C.c_int)]
 
@@ -217,6 +229,8 @@ def lib():
     L.vl1_payload_device_bytes.restype = i64
     L.vl1_near_field_f64.argtypes = [vp, C.POINTER(NearFieldArgs), vp]
     L.vl1_plan_create.argtypes = [C.POINTER(vp), vp, C.POINTER(PlanDesc), vp]
+    L.vl1_plan_create_helmholtz_split.argtypes = [C.POINTER(vp), vp, C.POINTER(PlanDesc),
+                                                  C.POINTER(HelmholtzSplitBlocks), vp]
     L.vl1_plan_destroy.argtypes = [vp]
     L.vl1_plan_eval.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, i64, C.POINTER(SplitOpts), vp]
     L.vl1_plan_stage.argtypes = [vp, C.c_int, vp, i64, C.c_int, C.POINTER(SplitOpts), vp]

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "split_series.h"
 
 namespace vl1 {
 
@@ -83,6 +84,76 @@ __global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int sp_mas
       }
       // signed (axis-derivative) kernels: the entry sign of (s, t)
       if (entry_sign(dim, q, perm + c * dim, sign + c * dim, sp_mask, ss, tt) < 0) v = -v;
+    }
+    M[i] = v;
+  }
+}
+
+// Helmholtz split blocks (complex, real 2n x 2n embedding) per (case, source
+// level): the Laplace table part with the canonical scaling folded in, plus
+// the quadrature of the split remainder on the q^d source nodes,
+//   V[t][s] = alpha T[row(case, s, t)] + disp nmlz[s]
+//             + w_s h^d R(|x_t - y_s|),
+// x_t - y_s = vec h / 4 + size h (xi_t - 1/2) - h (xi_s - 1/2) (source box
+// extent h, target box extent size * h, nearfield_potential_table.py:
+// 917-946), R the series of split_series.h.  This is, block by block, the
+// sum the point-to-point correction forms for every list-1 pair of that
+// case and level (expansion_wrangler_fpnd.py:3491-3836) added to the table
+// part (3457-3489), so one contraction evaluates both.
+__global__ void expand_split_blocks_kernel(int dim, int q, int n, int MP, int KP, int n_blocks,
+                                           const int32_t* __restrict__ blk_case,
+                                           const double* __restrict__ blk_alpha,
+                                           const double* __restrict__ blk_disp,
+                                           const double* __restrict__ blk_h,
+                                           const int32_t* __restrict__ case_vecs,
+                                           const uint8_t* __restrict__ perm,
+                                           const int8_t* __restrict__ sign,
+                                           const uint8_t* __restrict__ group,
+                                           const uint16_t* __restrict__ ranks,
+                                           const int32_t* __restrict__ offs,
+                                           const double* __restrict__ table,
+                                           const double* __restrict__ nmlz,
+                                           const double* __restrict__ unodes,
+                                           const double* __restrict__ uweights,
+                                           const double* __restrict__ coef, int N,
+                                           double* __restrict__ M) {
+  const int64_t total = (int64_t)n_blocks * MP * KP;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(i % KP);
+    const int t = (int)((i / KP) % MP);
+    const int b = (int)(i / ((int64_t)KP * MP));
+    double v = 0.0;
+    if (s < 2 * n && t < 2 * n) {
+      const int c = blk_case[b];
+      const int ss = s % n, tt = t % n;
+      const int64_t rel =
+          compact_rel_row(dim, q, perm + c * dim, sign + c * dim, group + c * dim, ss, tt);
+      double re = blk_alpha[b] * table[offs[ranks[c]] + rel] + blk_disp[b] * nmlz[ss];
+      const double h = blk_h[b];
+      const int32_t* vec = case_vecs + c * dim;
+      int m = 0;
+      for (int a = 0; a < dim; ++a) m = max(m, abs((int)vec[a]));
+      const double size = m == 0 ? 1.0 : 0.5 * (m - 2);
+      int sr = ss, tr = tt;
+      double r2 = 0.0, w = 1.0;
+      for (int a = dim - 1; a >= 0; --a) {
+        const int sa = sr % q, ta = tr % q;
+        sr /= q;
+        tr /= q;
+        const double d = 0.25 * vec[a] * h + size * h * (unodes[ta] - 0.5) - h * (unodes[sa] - 0.5);
+        r2 = fma(d, d, r2);
+        w *= uweights[sa] * h;
+      }
+      double rr, ri;
+      if (dim == 2)
+        remainder_value<2>(r2, coef, N, rr, ri);
+      else
+        remainder_value<3>(r2, coef, N, rr, ri);
+      re = fma(w, rr, re);
+      const double im = w * ri;
+      const bool tre = t < n, sre = s < n;
+      v = (tre == sre) ? re : (tre ? -im : im);
     }
     M[i] = v;
   }
@@ -832,6 +903,7 @@ struct vl1_plan {
   std::vector<double> pair_alpha, pair_disp;
   int n_blocks = 0;
   std::vector<int32_t> blk_case, blk_lev;
+  std::vector<double> blk_alpha, blk_disp, blk_h;  // Helmholtz split blocks only
   // device
   DevBuf mode_point, box_starts, M, MF, rowscale, A;
   int A_slices = 0;
@@ -956,9 +1028,8 @@ static int ensure_A(vl1_plan* pl, int S, hipStream_t s) {
   return VL1_OK;
 }
 
-extern "C" {
-
-int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc* d, void* stream) {
+static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc* d,
+                       const vl1_helmholtz_split_blocks* sp, void* stream) {
   clear_error();
   if (!out || !pay || !d) return fail(VL1_E_ARG, "vl1_plan_create: null argument");
   *out = nullptr;
@@ -976,18 +1047,33 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
   pl->payload = pay;
   pl->dim = dim;
   pl->n = n;
-  pl->cplx = pay->is_complex ? 1 : 0;
+  // Helmholtz split: complex blocks over a real Laplace payload, the
+  // canonical scaling and the 2D log term folded into per-(case, level) blocks
+  const bool split = sp != nullptr;
+  if (split) {
+    if (pay->is_complex || pay->sign_mask)
+      return fail(VL1_E_ARG, "Helmholtz split plans take a real scalar (Laplace) table payload");
+    if (d->scaling_mode != VL1_SCALING_CANONICAL)
+      return fail(VL1_E_ARG, "Helmholtz split plans use the canonical single-table scaling");
+    if (sp->series_nmax < 1 || sp->series_nmax > kSplitMaxTerms)
+      return fail(VL1_E_ARG, "Helmholtz split plans: series_nmax must be 1..96");
+    if (!sp->unit_nodes || !sp->unit_weights || !sp->case_vecs)
+      return fail(VL1_E_ARG, "Helmholtz split plans: null host array");
+  }
+  pl->cplx = (pay->is_complex || split) ? 1 : 0;
   pl->rows = pl->cplx ? 2 * n : n;
   pl->MP = round_up(pl->rows, 32);
   pl->KP = round_up(pl->rows, 32);
   pl->nboxes = d->n_boxes;
   pl->n_tgt = d->n_tgt_boxes;
   pl->n_points = d->n_points;
-  pl->log_disp = (d->scaling_mode == VL1_SCALING_CANONICAL && d->log_displacement) ? 1 : 0;
+  const bool want_log = d->scaling_mode == VL1_SCALING_CANONICAL && d->log_displacement;
+  pl->log_disp = (want_log && !split) ? 1 : 0;  // split blocks carry the term themselves
   if (pl->MP / 32 > 8) {
+    const bool c = pl->cplx;
     delete pl;
-    return fail(VL1_E_UNSUPPORTED, pl->cplx ? "vl1_plan_create: more than 128 nodes per box (complex)"
-                                            : "vl1_plan_create: more than 256 nodes per box");
+    return fail(VL1_E_UNSUPPORTED, c ? "vl1_plan_create: more than 128 nodes per box (complex)"
+                                     : "vl1_plan_create: more than 256 nodes per box");
   }
   if (pl->cplx && pl->log_disp) {
     delete pl;
@@ -1024,6 +1110,13 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
       if (m < 0 || m >= n || mode_point[(size_t)b * n + m] >= 0) {
         delete pl;
         return fail(VL1_E_LAYOUT, "box-local node ids must be a permutation of 0..q^d-1");
+      }
+      if (split && want_log && m != k) {
+        // the log term pairs nmlz[sid] with the sid-th particle: folded into
+        // a block it needs particle k to carry mode k
+        delete pl;
+        return fail(VL1_E_UNSUPPORTED,
+                    "2D Helmholtz split plans need box-local node ids in mode order");
       }
       mode_point[(size_t)b * n + m] = (int32_t)pt;
     }
@@ -1097,7 +1190,8 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
                                      std::to_string(lvl) + " but table_starting_level is " +
                                      std::to_string(d->table_starting_level));
       }
-      auto key = std::make_pair(cid, lev);
+      // split blocks depend on the source level itself (R is not scale free)
+      auto key = std::make_pair(cid, split ? lvl : lev);
       auto f = blk_of.find(key);
       int blk;
       if (f == blk_of.end()) {
@@ -1105,14 +1199,19 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
         blk_of[key] = blk;
         pl->blk_case.push_back(cid);
         pl->blk_lev.push_back(lev);
+        if (split) {
+          pl->blk_alpha.push_back(alpha);
+          pl->blk_disp.push_back(want_log ? -0.5 / M_PI * alpha * log(sext / H) : 0.0);
+          pl->blk_h.push_back(sext);
+        }
       } else {
         blk = f->second;
       }
       pl->pair_tgt.push_back(i);
       pl->pair_src.push_back(sb);
       pl->pair_block.push_back(blk);
-      pl->pair_alpha.push_back(alpha);
-      pl->pair_disp.push_back(disp);
+      pl->pair_alpha.push_back(split ? 1.0 : alpha);
+      pl->pair_disp.push_back(split ? 0.0 : disp);
     }
   }
   pl->n_blocks = (int)pl->blk_case.size();
@@ -1132,7 +1231,33 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
     if (e == hipSuccess) e = upload(dsign, pay->sign.data(), pay->sign.size(), s);
     if (e == hipSuccess) e = upload(dgroup, pay->group.data(), pay->group.size(), s);
     if (e == hipSuccess) e = pl->M.alloc((size_t)pl->n_blocks * pl->MP * pl->KP * sizeof(double));
-    if (e == hipSuccess) {
+    if (e == hipSuccess && split) {
+      std::vector<int32_t> vecs((size_t)pay->n_cases * dim);
+      for (size_t k = 0; k < vecs.size(); ++k) vecs[k] = sp->case_vecs[k];
+      const std::vector<double> coef =
+          split_coefficients(dim, std::complex<double>(sp->k_re, sp->k_im), sp->series_nmax);
+      DevBuf dal, ddi, dh, dvec, dun, duw, dco;
+      e = upload(dal, pl->blk_alpha.data(), pl->blk_alpha.size() * 8, s);
+      if (e == hipSuccess) e = upload(ddi, pl->blk_disp.data(), pl->blk_disp.size() * 8, s);
+      if (e == hipSuccess) e = upload(dh, pl->blk_h.data(), pl->blk_h.size() * 8, s);
+      if (e == hipSuccess) e = upload(dvec, vecs.data(), vecs.size() * 4, s);
+      if (e == hipSuccess) e = upload(dun, sp->unit_nodes, (size_t)pay->q * 8, s);
+      if (e == hipSuccess) e = upload(duw, sp->unit_weights, (size_t)pay->q * 8, s);
+      if (e == hipSuccess) e = upload(dco, coef.data(), coef.size() * 8, s);
+      if (e == hipSuccess) {
+        const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
+        const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(expand_split_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n,
+                           pl->MP, pl->KP, pl->n_blocks, dcase.as<int32_t>(), dal.as<double>(),
+                           ddi.as<double>(), dh.as<double>(), dvec.as<int32_t>(),
+                           dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
+                           pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(),
+                           pay->table.as<double>(), pay->nmlz.as<double>(), dun.as<double>(),
+                           duw.as<double>(), dco.as<double>(), sp->series_nmax, pl->M.as<double>());
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(s);  // before the uploads are freed
+    } else if (e == hipSuccess) {
       const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
       const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
       hipLaunchKernelGGL(expand_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n, pl->cplx,
@@ -1158,6 +1283,22 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc*
   }
   *out = pl;
   return VL1_OK;
+}
+
+extern "C" {
+
+int vl1_plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_desc* d, void* stream) {
+  return plan_create(out, pay, d, nullptr, stream);
+}
+
+int vl1_plan_create_helmholtz_split(vl1_plan** out, const vl1_payload* laplace,
+                                    const vl1_plan_desc* d, const vl1_helmholtz_split_blocks* sp,
+                                    void* stream) {
+  if (!sp) {
+    clear_error();
+    return fail(VL1_E_ARG, "vl1_plan_create_helmholtz_split: null split description");
+  }
+  return plan_create(out, laplace, d, sp, stream);
 }
 
 int vl1_plan_destroy(vl1_plan* plan) {

@@ -6,13 +6,8 @@
 // pair (volumential/expansion_wrangler_fpnd.py:3491-3836):
 //   corr[t] = sum_{s in list-1 boxes of t} R(|x_t - y_s|) * strength[s],
 // strength = density x quadrature weight.  R is the reference's series
-// (_HelmholtzSplitSeriesRemainderKernel, 212-312), evaluated by Horner:
-//   2D  R = c0 + P_b(r^2) - log(r) / (2 pi) * P_a(r^2),
-//       P_a(u) = sum_{n=1..N} s_n u^n,  s_n = (-1)^n (k^2/4)^n / (n!)^2,
-//       P_b(u) = sum_n s_n ((H_n - log(k/2) - gamma) / (2 pi) + i/4) u^n,
-//       c0 = i/4 - (log(k/2) + gamma) / (2 pi)  (the log term is 0 at r = 0);
-//   3D  R = sum_{n=1..N} (i k)^n / (4 pi n!) r^(n-1).
-// At r = 0 both give the self-pair limit of 5293-5317, so self pairs need no
+// (_HelmholtzSplitSeriesRemainderKernel, 212-312; split_series.h), by Horner.
+// At r = 0 it gives the self-pair limit of 5293-5317, so self pairs need no
 // special case.  N is the caller's series length (4337-4389).
 //
 // GPU mapping: one workgroup per target box.  Each listed source box's
@@ -28,12 +23,11 @@
 #include <vector>
 
 #include "internal.h"
+#include "split_series.h"
 
 namespace vl1 {
 
 constexpr int kSplitBlock = 256;
-constexpr int kSplitMaxTerms = 96;
-constexpr double kEulerGamma = 0.5772156649015329;
 
 struct SplitParams {
   int dim, n_rhs, nt, n_terms;
@@ -50,42 +44,6 @@ struct SplitParams {
   double* result;  // [n_rhs][ldr] interleaved complex
   int64_t ldr;
 };
-
-template <int DIM>
-__device__ inline void remainder_value(double r2, const double* __restrict__ cf, int N, double& vr,
-                                       double& vi) {
-  if (DIM == 2) {
-    // Horner in u = r^2 for P_a and P_b (coefficients n = N .. 1)
-    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
-    for (int n = N; n >= 1; --n) {
-      const double* c = cf + 2 + 4 * (n - 1);
-      const double nar = fma(ar, r2, c[0]), nai = fma(ai, r2, c[1]);
-      const double nbr = fma(br, r2, c[2]), nbi = fma(bi, r2, c[3]);
-      ar = nar;
-      ai = nai;
-      br = nbr;
-      bi = nbi;
-    }
-    ar *= r2;
-    ai *= r2;
-    br *= r2;
-    bi *= r2;
-    const double lg = r2 > 0.0 ? -0.25 / M_PI * log(r2) : 0.0;  // -log(r) / (2 pi)
-    vr = cf[0] + br + lg * ar;
-    vi = cf[1] + bi + lg * ai;
-  } else {
-    const double r = sqrt(r2);
-    double pr = 0.0, pi = 0.0;
-    for (int n = N; n >= 1; --n) {
-      const double* c = cf + 2 * (n - 1);
-      const double npr = fma(pr, r, c[0]), npi = fma(pi, r, c[1]);
-      pr = npr;
-      pi = npi;
-    }
-    vr = pr;
-    vi = pi;
-  }
-}
 
 template <int DIM, int RC>
 __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const SplitParams p) {
@@ -183,43 +141,6 @@ __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const Spl
       __syncthreads();  // reduction buffer reused by the next target chunk
     }
   }
-}
-
-// Series coefficients (interleaved complex), the reference's formulas.
-static std::vector<double> split_coefficients(int dim, std::complex<double> k, int N) {
-  using cd = std::complex<double>;
-  std::vector<double> out;
-  if (dim == 2) {
-    const bool zero = std::abs(k) == 0.0;
-    const cd L = zero ? cd(0.0) : std::log(0.5 * k) + kEulerGamma;
-    const cd c0 = zero ? cd(0.0) : cd(0.0, 0.25) - L / (2.0 * M_PI);
-    out.push_back(c0.real());
-    out.push_back(c0.imag());
-    double fact = 1.0, harm = 0.0;
-    cd kp = 1.0;
-    for (int n = 1; n <= N; ++n) {
-      fact *= n;
-      harm += 1.0 / n;
-      kp *= -k * k / 4.0;  // (-1)^n (k^2/4)^n
-      const cd s = zero ? cd(0.0) : kp / (fact * fact);
-      const cd b = s * ((harm - L) / (2.0 * M_PI) + cd(0.0, 0.25));
-      out.push_back(s.real());
-      out.push_back(s.imag());
-      out.push_back(b.real());
-      out.push_back(b.imag());
-    }
-  } else {
-    double fact = 1.0;
-    cd ik = cd(0.0, 1.0) * k, p = 1.0;
-    for (int n = 1; n <= N; ++n) {
-      fact *= n;
-      p *= ik;
-      const cd c = p / (4.0 * M_PI * fact);
-      out.push_back(c.real());
-      out.push_back(c.imag());
-    }
-  }
-  return out;
 }
 
 template <int DIM, int RC>

@@ -26,7 +26,12 @@ FORMS = {"f64": _capi.FORM_F64, "split": _capi.FORM_SPLIT_BF16}
 class List1Plan:
     def __init__(self, tree, payload, *, scaling_mode, scaling_power=2, log_displacement=False,
                  table_root_extent, table_starting_level=0, case_encoding_bias=1e-10,
-                 target_boxes=None, starts=None, lists=None, stream=None):
+                 target_boxes=None, starts=None, lists=None, stream=None,
+                 helmholtz_split=None):
+        """``helmholtz_split``: ``{"k": k, "series_nmax": N}`` folds the
+        Helmholtz split correction into the blocks of a real Laplace payload
+        (``vl1_plan_create_helmholtz_split``); the plan then evaluates complex
+        densities to the split's full list-1 value."""
         L = _capi.lib()
         tb = np.ascontiguousarray(tree.target_boxes if target_boxes is None else target_boxes,
                                   dtype=np.int32)
@@ -68,14 +73,36 @@ class List1Plan:
             boxes_per_tile=0,
         )
         handle = _capi.vp()
-        _capi.check(
-            L.vl1_plan_create(_capi.C.byref(handle), payload.handle, desc, _capi.stream_handle(stream)),
-            "vl1_plan_create",
-        )
+        if helmholtz_split is None:
+            _capi.check(
+                L.vl1_plan_create(_capi.C.byref(handle), payload.handle, desc,
+                                  _capi.stream_handle(stream)),
+                "vl1_plan_create",
+            )
+        else:
+            from . import gallery, quadrature
+
+            xi, wi = quadrature.gauss_legendre_01(payload.quad_order)
+            vecs, _, _ = gallery.generate_list1_gallery(int(tree.dim))
+            sk = complex(helmholtz_split["k"])
+            k["split_nodes"] = np.ascontiguousarray(xi, np.float64)
+            k["split_weights"] = np.ascontiguousarray(wi, np.float64)
+            k["case_vecs"] = np.ascontiguousarray(np.asarray(vecs, np.int32))
+            sp = _capi.HelmholtzSplitBlocks(
+                k_re=sk.real, k_im=sk.imag, series_nmax=int(helmholtz_split["series_nmax"]),
+                unit_nodes=k["split_nodes"].ctypes.data,
+                unit_weights=k["split_weights"].ctypes.data,
+                case_vecs=k["case_vecs"].ctypes.data,
+            )
+            _capi.check(
+                L.vl1_plan_create_helmholtz_split(_capi.C.byref(handle), payload.handle, desc, sp,
+                                                  _capi.stream_handle(stream)),
+                "vl1_plan_create_helmholtz_split",
+            )
         self.handle = handle
         self.payload = payload  # keep alive
         self.n_points = int(k["ids"].size)
-        self.is_complex = bool(payload.is_complex)
+        self.is_complex = bool(payload.is_complex) or helmholtz_split is not None
 
     def eval(self, mode_coefs: torch.Tensor, out: torch.Tensor | None = None, form="split",
              n_slices=5, stream=None) -> torch.Tensor:

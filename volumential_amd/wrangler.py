@@ -219,7 +219,7 @@ class NearFieldWrangler:
                     f"{min_lev} are not covered."
                 )
         if self.direct_method == "plan" and kname not in self._plain_only:
-            key = _csr_key(out_kernel, target_boxes, starts, lists)
+            key = _csr_key(out_kernel, target_boxes, starts, lists) + (False,)
             if key not in self._plans:  # host layout checks, once per CSR
                 _validate_layout(self.tree, np.asarray(_host(target_boxes), dtype=np.int64),
                                  np.asarray(_host(lists), dtype=np.int64), t0.n_q_points)
@@ -331,13 +331,15 @@ class NearFieldWrangler:
 
     # ------------------------------------------------------------ plans
     def get_plan(self, out_kernel, target_boxes, neighbor_source_boxes_starts,
-                 neighbor_source_boxes_lists):
-        """Plan for one output kernel and one list-1 CSR (cached)."""
+                 neighbor_source_boxes_lists, fused_split=False):
+        """Plan for one output kernel and one list-1 CSR (cached).
+        ``fused_split``: the Helmholtz split plan (table part + correction in
+        one contraction, ``vl1_plan_create_helmholtz_split``)."""
         from .payload import DevicePayload
         from .plan import List1Plan
 
         key = _csr_key(out_kernel, target_boxes, neighbor_source_boxes_starts,
-                       neighbor_source_boxes_lists)
+                       neighbor_source_boxes_lists) + (bool(fused_split),)
         if key in self._plans:
             return self._plans[key]
         tables = self.near_field_table[repr(out_kernel)]
@@ -353,6 +355,10 @@ class NearFieldWrangler:
             **self.list1_extra_kwargs,
         )
         pol = nf.get_kernel_scaling_policy()
+        split_desc = None
+        if fused_split:
+            corr = self._correction(out_kernel)
+            split_desc = {"k": corr.k, "series_nmax": corr.nmax}
         plan = List1Plan(
             self.tree, payload,
             scaling_mode=nf._mode_code(pol),
@@ -364,6 +370,7 @@ class NearFieldWrangler:
             target_boxes=np.frombuffer(key[1], dtype=np.int32),
             starts=np.frombuffer(key[2], dtype=np.int32),
             lists=np.frombuffer(key[3], dtype=np.int32),
+            helmholtz_split=split_desc,
         )
         self._plans[key] = plan
         return plan
@@ -422,18 +429,44 @@ class NearFieldWrangler:
         return pot, TimingFuture()
 
     def eval_direct_split(self, target_boxes, neighbor_source_boxes_starts,
-                          neighbor_source_boxes_lists, mode_coefs, form="f64", n_slices=5):
+                          neighbor_source_boxes_lists, mode_coefs, form="f64", n_slices=5,
+                          fused=True):
         """List 1 as ``drive_volume_fmm`` sums it under the split
         (``volume_fmm.py:983-1026``): the Laplace-table part of
-        ``mode_coefs[R, n_points]`` (plan ``form``) plus the correction with
-        strengths ``mode_coefs * tree.weights``."""
+        ``mode_coefs[R, n_points]`` plus the correction with strengths
+        ``mode_coefs * tree.weights``.  ``fused``: both in one plan
+        contraction (``form`` "f64" or "split") where the plan takes the box
+        size (2 q^d <= 256; 2D: node ids in mode order); otherwise, or with
+        ``fused=False``, the table part (plan ``form``) plus the
+        point-to-point correction."""
         if not self.helmholtz_split:
             raise RuntimeError("eval_direct_split needs helmholtz_split enabled")
         mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
             np.asarray(mode_coefs))
         mc = mc.to(device="cuda", dtype=torch.complex128)
         one = mc.dim() == 1
-        mc2 = mc.unsqueeze(0) if one else mc
+        mc2 = (mc.unsqueeze(0) if one else mc).contiguous()
+        if fused and all(len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels):
+            # one contraction: complex blocks = table part + remainder quadrature
+            # the split form's LDS budget can exclude 256-row blocks: the
+            # fused FP64 form (more accurate) then stands in before two launches
+            for fform in ((form,) if form == "f64" else (form, "f64")):
+                pot = self.output_zeros(int(mc2.shape[0]))
+                try:
+                    for i, knl in enumerate(self.target_kernels):
+                        plan = self.get_plan(knl, target_boxes, neighbor_source_boxes_starts,
+                                             neighbor_source_boxes_lists, fused_split=True)
+                        r = int(mc2.shape[0])
+                        bulk = r - r % 64 if r > 64 else r
+                        plan.eval(mc2[:bulk], out=pot[i][:bulk], form=fform, n_slices=n_slices)
+                        if bulk < r:
+                            plan.eval(mc2[bulk:], out=pot[i][bulk:], form=fform,
+                                      n_slices=n_slices)
+                        if one:
+                            pot[i] = pot[i][0]
+                    return pot, TimingFuture()
+                except NotImplementedError:
+                    pass  # box size / node order / LDS beyond the fused plan
         pot, _ = self.eval_direct_plan(target_boxes, neighbor_source_boxes_starts,
                                        neighbor_source_boxes_lists, mc2, form=form,
                                        n_slices=n_slices)

@@ -1,0 +1,70 @@
+"""Host-side logic of NearFieldFromCSR (CPU, no GPU calls).
+
+* the case-encoding drift warning (list1.py:468-530): silent on an exact
+  dyadic tree, and on a perturbed box centre it reports the largest drift
+  with the pair and dimension that produced it;
+* the kernel scaling policy (list1.py:361-437): canonical for Laplace,
+  constant and axis derivatives of Laplace (powers 2, d and 1; the 2D log
+  term only for Laplace), fixed single table / per-level tables otherwise,
+  and Helmholtz with inferred scaling rejected (list1.py:183-195).
+"""
+
+import logging
+
+import numpy as np
+import pytest
+
+from volumential_amd import tree as vtree
+from volumential_amd.kernels import (AxisSourceDerivative, AxisTargetDerivative,
+                                     ConstantKernel, HelmholtzKernel, LaplaceKernel)
+from volumential_amd.list1 import NearFieldFromCSR, warn_case_encoding_drift
+
+
+def _drift(tr, tol):
+    return warn_case_encoding_drift(tr.dim, tr.box_centers, tr.box_levels, tr.root_extent,
+                                    tr.target_boxes, tr.neighbor_source_boxes_starts,
+                                    tr.neighbor_source_boxes_lists, 6.0, tol)
+
+
+def test_drift_silent_on_exact_tree(caplog):
+    tr = vtree.uniform_tree(3, 2, 3)
+    with caplog.at_level(logging.WARNING):
+        d = _drift(tr, 1e-12)
+    assert d <= 1e-12
+    assert not caplog.records
+    assert _drift(tr, None) is None
+
+
+def test_drift_reports_worst_pair(caplog):
+    tr = vtree.uniform_tree(2, 2, 3)
+    h = tr.root_extent / 8
+    tr.box_centers = tr.box_centers.copy()
+    tr.box_centers[1, 10] += 3e-7 * h  # box 10, dim 1: drift 4 * 3e-7
+    with caplog.at_level(logging.WARNING):
+        d = _drift(tr, 1e-8)
+    assert abs(d - 1.2e-6) < 1e-9
+    msg = caplog.records[-1].getMessage()
+    assert "dim 1" in msg and ("target box 10" in msg or "source box 10" in msg)
+
+
+@pytest.mark.parametrize("kernel,power,logd", [
+    (LaplaceKernel(2), 2, True), (LaplaceKernel(3), 2, False), (ConstantKernel(3), 3, False),
+    (AxisTargetDerivative(0, LaplaceKernel(2)), 1, False),
+    (AxisSourceDerivative(2, LaplaceKernel(3)), 1, False)])
+def test_inferred_scaling_policy(kernel, power, logd):
+    nf = NearFieldFromCSR(kernel, {"n_tables": 1, "n_q_points": 9 if kernel.dim == 2 else 27,
+                                   "n_table_entries": 10})
+    pol = nf.get_kernel_scaling_policy()
+    assert pol.mode == "canonical_single_table"
+    assert (pol.scaling_power, pol.log_displacement) == (power, logd)
+
+
+def test_policy_fallbacks_and_helmholtz_rejection():
+    shapes = {"n_tables": 1, "n_q_points": 36, "n_table_entries": 10}
+    assert NearFieldFromCSR(HelmholtzKernel(2, 5.0), shapes).get_kernel_scaling_policy().mode == \
+        "fixed_single_table"
+    shapes3 = dict(shapes, n_tables=3)
+    assert NearFieldFromCSR(HelmholtzKernel(2, 5.0), shapes3).get_kernel_scaling_policy().mode == \
+        "per_level_tables"
+    with pytest.raises(RuntimeError, match="infer_kernel_scaling is unsupported"):
+        NearFieldFromCSR(HelmholtzKernel(2, 5.0), shapes, infer_kernel_scaling=True)

@@ -13,6 +13,7 @@ fixed-single-table and per-level-table policies are.
 
 from __future__ import annotations
 
+import logging
 from dataclasses import dataclass
 
 import numpy as np
@@ -23,6 +24,38 @@ from .kernels import inferred_scaling, supports_inferred_scaling
 from .payload import DevicePayload, PayloadCache, fingerprint
 
 CASE_ENCODING_BIAS_DEFAULT = 1.0e-10
+
+logger = logging.getLogger(__name__)
+
+
+def warn_case_encoding_drift(dim, box_centers, box_levels, root_extent, target_boxes, starts,
+                             lists, encoding_shift, warn_tol):
+    """Log a warning when a list-1 pair's unrounded case-vector component
+    ``(c_t - c_s) / h_s * 4 + shift`` is farther than ``warn_tol`` from an
+    integer (``list1.py:468-530``, enabled by ``case_encoding_warn_tol``).
+    Vectorised over all pairs; returns the largest drift."""
+    if warn_tol is None:
+        return None
+    tb = np.asarray(_to_host(target_boxes), dtype=np.int64)
+    st = np.asarray(_to_host(starts), dtype=np.int64)
+    ls = np.asarray(_to_host(lists), dtype=np.int64)
+    if tb.size == 0 or ls.size == 0:
+        return 0.0
+    centers = np.asarray(_to_host(box_centers), dtype=np.float64)
+    levels = np.asarray(_to_host(box_levels), dtype=np.int64)
+    tgt = np.repeat(tb, np.diff(st))
+    ext = float(root_extent) * 0.5 ** levels[ls]
+    raw = (centers[:dim, tgt] - centers[:dim, ls]) / ext[None, :] * 4.0 + float(encoding_shift)
+    drift = np.abs(raw - np.rint(raw))
+    worst = np.unravel_index(int(np.argmax(drift)), drift.shape)
+    max_drift = float(drift[worst])
+    if max_drift > float(warn_tol):
+        d, k = int(worst[0]), int(worst[1])
+        logger.warning(
+            "List1 case encoding drift %.3e exceeds %.3e for target box %d, source box %d, "
+            "dim %d (raw %.17g)", max_drift, float(warn_tol), int(tgt[k]), int(ls[k]), d,
+            float(raw[d, k]))
+    return max_drift
 
 
 @dataclass(frozen=True)
@@ -111,6 +144,9 @@ class NearFieldFromCSR:
                 "size, which a single fixed-k table cannot represent. Use multi-level "
                 "tables."
             )
+        if self.extra_kwargs.get("case_encoding_warn_tol") is not None:
+            self.extra_kwargs["case_encoding_warn_tol"] = float(
+                self.extra_kwargs["case_encoding_warn_tol"])
         bs = int(self.extra_kwargs.get("list1_target_batch_size", 1))
         if bs < 1:
             raise ValueError("list1_target_batch_size must be positive")
@@ -226,6 +262,11 @@ class NearFieldFromCSR:
                         )
                 self._level_check_key = key
 
+        warn_case_encoding_drift(
+            self.dim, kw["box_centers"], box_levels, kw["root_extent"], target_boxes,
+            kw["neighbor_source_boxes_starts"], lists,
+            kw.get("encoding_shift", gallery.ENCODING_SHIFT),
+            self.extra_kwargs.get("case_encoding_warn_tol"))
         coefs_2d = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
             np.asarray(mode_coefs)
         )

@@ -355,6 +355,14 @@ class NearFieldWrangler:
             **self.list1_extra_kwargs,
         )
         pol = nf.get_kernel_scaling_policy()
+        # the reference checks the drift on every call; a plan checks its CSR once
+        from .list1 import warn_case_encoding_drift
+
+        warn_case_encoding_drift(
+            self.tree.dim, self.tree.box_centers, self.tree.box_levels, self.tree.root_extent,
+            np.frombuffer(key[1], dtype=np.int32), np.frombuffer(key[2], dtype=np.int32),
+            np.frombuffer(key[3], dtype=np.int32), gallery.ENCODING_SHIFT,
+            nf.extra_kwargs.get("case_encoding_warn_tol"))
         split_desc = None
         if fused_split:
             corr = self._correction(out_kernel)

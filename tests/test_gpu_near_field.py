@@ -209,3 +209,18 @@ def test_plan_route_box_size_beyond_tiles(cuda):
                           mode="canonical", table_root_extent=2.0)
     l2, linf = rel_errors(got[0].cpu().numpy(), ref[0])
     assert l2 <= TOL and linf <= TOL, (l2, linf)
+
+
+def test_single_precision_wrangler_rounds_fp64_result(cuda):
+    """A float32 wrangler returns float32 (complex64 for complex kernels)
+    potentials: the FP64 evaluation rounded (expansion_wrangler_fpnd.py:
+    3283-3297 picks f32 / c64 for such wranglers)."""
+    print("case: float32 wrangler", flush=True)
+    tr = vtree.uniform_tree(2, 3, 3)
+    tables = synthetic_table_levels(2, 3, seed=12)
+    f = torch.as_tensor(densities.gaussian_mixture(tr.points, 3), device="cuda")
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    p64, _ = NearFieldWrangler(tr, tables[0]).eval_direct(*args, f)
+    p32, _ = NearFieldWrangler(tr, tables[0], dtype=np.float32).eval_direct(*args, f)
+    assert p32[0].dtype == torch.float32
+    assert torch.equal(p32[0], p64[0].to(torch.float32))

@@ -180,6 +180,18 @@ class NearFieldWrangler:
         """complex128 for complex kernels (``expansion_wrangler_fpnd.py:3283-3297``)."""
         return torch.complex128 if getattr(kernel, "is_complex_valued", False) else torch.float64
 
+    def _finish(self, pot):
+        """Round results to the wrangler's single precision when it has one:
+        ``expansion_wrangler_fpnd.py:3283-3297`` evaluates in f32 / c64 for a
+        float32 wrangler; this build evaluates in FP64 and rounds the result
+        (it agrees with an f32 evaluation to f32 rounding, and is closer to
+        the exact sum)."""
+        if self.dtype not in (np.dtype(np.float32), np.dtype(np.complex64)):
+            return pot
+        for i in range(len(pot)):
+            pot[i] = pot[i].to(torch.complex64 if pot[i].is_complex() else torch.float32)
+        return pot
+
     def output_zeros(self, n_rhs=None):
         shape = (self.tree.n_points,) if n_rhs is None else (n_rhs, self.tree.n_points)
         pot = np.empty(len(self.target_kernels), dtype=object)
@@ -310,7 +322,7 @@ class NearFieldWrangler:
                 pot[i], knl, target_boxes, neighbor_source_boxes_starts,
                 neighbor_source_boxes_lists, mode_coefs,
             )
-        return pot, TimingFuture()
+        return self._finish(pot), TimingFuture()
 
     def eval_direct_batched(self, target_boxes, neighbor_source_boxes_starts,
                             neighbor_source_boxes_lists, mode_coefs):
@@ -326,7 +338,7 @@ class NearFieldWrangler:
                 pot[i], knl, target_boxes, neighbor_source_boxes_starts,
                 neighbor_source_boxes_lists, mc,
             )
-        return pot, TimingFuture()
+        return self._finish(pot), TimingFuture()
 
 
     # ------------------------------------------------------------ plans
@@ -405,7 +417,7 @@ class NearFieldWrangler:
                 continue
             mc = mc0.to(device="cuda", dtype=self._dtype(knl)).contiguous()
             plan.eval(mc, out=pot[i], form=form, n_slices=n_slices)
-        return pot, TimingFuture()
+        return self._finish(pot), TimingFuture()
 
     # --------------------------------------------------------- split
     def _correction(self, knl):
@@ -427,14 +439,14 @@ class NearFieldWrangler:
         smooth-quadrature path) is accepted and not used: this build
         implements the base-quadrature path."""
         if not self.helmholtz_split:
-            return self.output_zeros(), TimingFuture()
+            return self._finish(self.output_zeros()), TimingFuture()
         w = src_weights if isinstance(src_weights, torch.Tensor) else torch.as_tensor(
             np.asarray(src_weights))
         pot = np.empty(len(self.target_kernels), dtype=object)
         for i, knl in enumerate(self.target_kernels):
             pot[i] = self._correction(knl)(w, target_boxes, neighbor_source_boxes_starts,
                                            neighbor_source_boxes_lists)
-        return pot, TimingFuture()
+        return self._finish(pot), TimingFuture()
 
     def eval_direct_split(self, target_boxes, neighbor_source_boxes_starts,
                           neighbor_source_boxes_lists, mode_coefs, form="f64", n_slices=5,
@@ -472,7 +484,7 @@ class NearFieldWrangler:
                                       n_slices=n_slices)
                         if one:
                             pot[i] = pot[i][0]
-                    return pot, TimingFuture()
+                    return self._finish(pot), TimingFuture()
                 except NotImplementedError:
                     pass  # box size / node order / LDS beyond the fused plan
         pot, _ = self.eval_direct_plan(target_boxes, neighbor_source_boxes_starts,
@@ -486,7 +498,7 @@ class NearFieldWrangler:
             pot[i] = pot[i] + corr[i]
             if one:
                 pot[i] = pot[i][0]
-        return pot, TimingFuture()
+        return self._finish(pot), TimingFuture()
 
 
 def _host(a):

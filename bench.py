@@ -53,12 +53,13 @@ def parse():
                     help="do not time the plain FP64 kernel (profiling runs)")
     ap.add_argument("--no-table-build", action="store_true",
                     help="skip the near-field table build record (3D Laplace q=6, all cases)")
-    ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c2", "c1"],
+    ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c2", "c1", "hsplit"],
                     help="c3 (default): 3D Laplace uniform level 5 q=6, 64 RHS; "
                          "c4: 2D Helmholtz k=40, 8 adaptive trees depth 4..10, q=6, 16 complex "
                          "RHS in one plan; c2: 3D Laplace adaptive (interpolation error 1e-8, "
                          "depth 3..7), q=5, 1 RHS; c1: 2D Laplace uniform level 5, q=4, "
-                         "exp(-|x|^2/0.01), 1 RHS")
+                         "exp(-|x|^2/0.01), 1 RHS; hsplit: 2D Helmholtz k=40 with the "
+                         "split (Laplace table + remainder), uniform level 7, q=6, 16 RHS")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan autotuner (vl1_plan_tune) before timing")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -555,6 +556,135 @@ def run_c1(args):
         forms, extra)
 
 
+def run_hsplit(args):
+    """2D Helmholtz k=40 on [-1,1]^2 with the reference's default split:
+    uniform level 7 (16,384 leaves), q=6 (589,824 points), 16 complex
+    random-phase Gaussian-beam densities, one canonical Laplace table.  A
+    step is the full list 1 (Laplace-table part + remainder G_k - G_0) of
+    all 16 densities.  Forms: the fused split plan (one contraction, FP64 or
+    Ozaki split) and the two-launch route (Laplace plan on Re/Im + the
+    point-to-point correction kernel)."""
+    import torch
+
+    from oracle import cpu as ocpu
+    from oracle import orbit_ref
+    from oracle import split_ref as S
+    from volumential_amd import densities, tree as vtree
+    from volumential_amd.kernels import HelmholtzKernel, LaplaceKernel
+    from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+    from volumential_amd.wrangler import NearFieldWrangler
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(2)
+    dim, q, level, k, R = 2, 6, 7, 40.0, 16
+    tr = vtree.uniform_tree(dim, q, level)
+    knl = HelmholtzKernel(dim, k)
+    lap = NearFieldInteractionTable(q, dim, LaplaceKernel(dim), source_box_extent=2.0)
+    lap.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=61))
+    w = NearFieldWrangler(tr, {repr(knl): [lap]}, target_kernels=[knl])
+    f = np.stack([densities.helmholtz_beams(tr.points, 500 + r, k=k) for r in range(R)])
+    fd = torch.as_tensor(f, device="cuda").contiguous()
+    args_csr = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    n, pairs, n_pts = tr.n_q_points, tr.n_list1_pairs, tr.n_points
+    log(f"tree: 2D q={q} level {level}, {tr.nboxes} boxes, {n_pts} points, {pairs} pairs")
+    ref = w.eval_direct_split(*args_csr, fd, fused=False)[0][0]
+    plan = w.get_plan(knl, *args_csr, fused_split=True)
+    flops = 8.0 * R * pairs * n * n  # complex MACs, 8 real flops each
+    forms = _plan_forms(plan, fd, R, n_pts, flops, args.steps, args.warmup, args.slices, True)
+    # the two-launch route: table part (Laplace plan, Re/Im as 2R real rows) + P2P
+    stream = torch.cuda.current_stream()
+    corr = w._correction(knl)
+    wts = torch.as_tensor(np.asarray(tr.weights), device="cuda")
+    st = (fd * wts[None, :]).contiguous()
+    times = {"two_launch": [], "p2p": []}
+    for it in range(args.warmup + args.steps):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(stream)
+        w.eval_direct_plan(*args_csr, fd, form="f64")
+        e1.record(stream)
+        corr(st, *args_csr)
+        e2.record(stream)
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            times["two_launch"].append(e0.elapsed_time(e2))
+            times["p2p"].append(e1.elapsed_time(e2))
+    two_ms = float(np.mean(times["two_launch"]))
+    p2p_ms = float(np.mean(times["p2p"]))
+    fused = {}
+    for form in ("f64", "split"):
+        out = w.eval_direct_split(*args_csr, fd, form=form)[0][0]
+        d = (out - ref).abs().pow(2).sum(dim=1).sqrt() / ref.abs().pow(2).sum(dim=1).sqrt()
+        fused[form] = float(d.max())
+    log(f"fused f64 {forms['f64']['ms']:.3f} ms, fused split {forms['split']['ms']:.3f} ms, "
+        f"two launches {two_ms:.3f} ms (P2P kernel {p2p_ms:.3f} ms); "
+        f"rel L2 vs two-launch: f64 {fused['f64']:.2e}, split {fused['split']:.2e}")
+    # P2P roofline: FP64 VALU, declared 8 N + 24 flop per (target, source)
+    # point pair for the Horner series and log, plus 8 per complex MAC
+    pt_pairs = pairs * n * n
+    p2p_flops = pt_pairs * (8.0 * corr.nmax + 24.0 + 8.0 * R)
+    best = "f64"
+    extra = {"two_launch_ms": two_ms, "p2p_kernel_ms": p2p_ms,
+             "p2p_declared_fp64_tflops": p2p_flops / (p2p_ms * 1e-3) / 1e12,
+             "p2p_fp64_frac": p2p_flops / (p2p_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+             "series_terms": corr.nmax,
+             "rel_l2_vs_two_launch": fused,
+             "speedup_fused_vs_two_launch": two_ms / forms[best]["ms"]}
+    if not args.no_cpu_baseline:
+        # oracle on a sample of target boxes: C/OpenMP table part (Re, Im) +
+        # numpy remainder sum; scaled to points x RHS per second
+        import copy
+
+        meta = orbit_ref.case_metadata(2, q)
+        cores = min(16, os.cpu_count() or 1)
+        perm = np.random.default_rng(0).permutation(tr.nboxes)
+        f1 = f[:1]
+
+        def run(nb):
+            sel = np.sort(perm[:nb]).astype(np.int32)
+            starts = np.zeros(nb + 1, np.int32)
+            lists = []
+            for i, b in enumerate(sel):
+                seg = tr.neighbor_source_boxes_lists[tr.neighbor_source_boxes_starts[b]:
+                                                     tr.neighbor_source_boxes_starts[b + 1]]
+                lists.append(seg)
+                starts[i + 1] = starts[i] + seg.size
+            lists = np.concatenate(lists).astype(np.int32)
+            sub = copy.copy(tr)
+            sub.target_boxes, sub.neighbor_source_boxes_starts = sel, starts
+            sub.neighbor_source_boxes_lists = lists
+            t0 = time.perf_counter()
+            ocpu.near_field(sub, lap.compact_values()[None], lap.mode_normalizers[None], meta,
+                            np.concatenate([f1.real, f1.imag]), mode="canonical",
+                            scaling_power=2, log_displacement=True, table_root_extent=2.0,
+                            nthreads=cores)
+            S.correction(tr, f1 * np.asarray(tr.weights)[None], k, corr.nmax,
+                         target_boxes=sel, starts=starts, lists=lists)
+            return time.perf_counter() - t0
+
+        dt0 = run(128)
+        nb = int(min(tr.nboxes, max(128, 128 * args.cpu_seconds / max(dt0, 1e-9))))
+        dt = run(nb)
+        extra["cpu_baseline"] = {
+            "value": nb * n / dt / 1e9, "unit": "Gpts*RHS/s", "cores": cores, "kind": "port",
+            "sample": f"{nb} target boxes x 1 RHS: C/OpenMP Laplace-table part (Re, Im) + "
+                      f"numpy remainder sum ({dt:.1f} s)"}
+    return _line(
+        forms[best]["value"], forms[best]["ms"], args.steps, args.warmup, "f64 (complex, fused)",
+        "synthetic (16 seeded random-phase Gaussian beams, k=40; GPU-built Laplace 2D table)",
+        {"workload": "2D Helmholtz k=40 split list 1, uniform level 7, q=6, 16 complex RHS",
+         "boxes": tr.nboxes, "target_points": n_pts, "rhs": R, "list1_pairs": pairs,
+         "form": "fused split plan f64", "parallelism": "single GPU"},
+        {"bound": "mfma", "achieved": flops / (forms[best]["kernel_ms"] * 1e-3) / 1e12,
+         "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+         "frac": flops / (forms[best]["kernel_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+         "traffic": None, "kernel": "dense_f64_kernel", "kernel_ms": forms[best]["kernel_ms"],
+         "flops_per_launch": flops,
+         "note": "a complex block runs as the real 2n x 2n block: 8 n^2 flops per RHS, "
+                 "the complex-MAC count"},
+        forms, extra)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -568,6 +698,9 @@ def main():
         return
     if args.workload == "c1":
         print(json.dumps(run_c1(args)), flush=True)
+        return
+    if args.workload == "hsplit":
+        print(json.dumps(run_hsplit(args)), flush=True)
         return
     import torch
 

@@ -30,7 +30,7 @@ namespace vl1 {
 constexpr int kSplitBlock = 256;
 
 struct SplitParams {
-  int dim, n_rhs, nt, n_terms;
+  int dim, n_rhs, nt, n_terms, chunk;
   const double* coef;  // 2D: c0, then (a_n, b_n) n = 1..N; 3D: c_n n = 1..N; complex
   const int32_t* target_boxes;
   const int32_t* nbr_starts;
@@ -52,10 +52,10 @@ __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const Spl
   const int ncoef = DIM == 2 ? 2 + 4 * N : 2 * N;
   const int P = kSplitBlock / p.nt;
   double* sC = smem;                 // series coefficients
-  double* sX = sC + ncoef;           // source coordinates [DIM][nt * P] (chunked)
-  const int chunk = p.nt * P;        // source points staged per pass
+  double* sX = sC + ncoef;           // source coordinates [DIM][chunk]
+  const int chunk = p.chunk;         // source points staged per pass
   double* sF = sX + DIM * chunk;     // strengths [chunk][RC][2]
-  double* red = sF + (size_t)chunk * RC * 2;
+  double* red = sX;                  // part sums [256][RC][2], after the staging is done
   const int tid = threadIdx.x;
   const int lane_t = tid % p.nt;
   const int part = tid / p.nt;
@@ -200,19 +200,25 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   while (nt < a->max_box_points && nt < kSplitBlock) nt *= 2;
   prm.nt = nt;
   const int P = kSplitBlock / nt;
+  // the remainder is evaluated once per (target, source) pair and RC-chunk;
+  // RC <= 8 keeps the LDS small enough for several workgroups per CU
   const int RC = a->n_rhs == 1 ? 1 : (a->n_rhs <= 4 ? 4 : 8);
-  const int chunk = nt * P;
-  const size_t smem = sizeof(double) * ((size_t)coef.size() + (size_t)a->dim * chunk +
-                                        (size_t)chunk * RC * 2 +
-                                        (P > 1 ? (size_t)kSplitBlock * RC * 2 : 0));
+  // source points staged per pass: the box size, rounded up to the parts
+  const int chunk = std::min(nt * P, (a->max_box_points + P - 1) / P * P);
+  prm.chunk = chunk;
+  const size_t stage = (size_t)a->dim * chunk + (size_t)chunk * RC * 2;
+  const size_t reduce = P > 1 ? (size_t)kSplitBlock * RC * 2 : 0;  // aliases the staging
+  const size_t smem = sizeof(double) * ((size_t)coef.size() + std::max(stage, reduce));
   if (smem > 160 * 1024) return fail(VL1_E_UNSUPPORTED, "vl1_split_correction: LDS budget exceeded");
   const int grid = a->n_tgt_boxes;
   if (a->dim == 2) {
-    e = RC == 1 ? launch_split<2, 1>(prm, grid, smem, s)
-                : (RC == 4 ? launch_split<2, 4>(prm, grid, smem, s) : launch_split<2, 8>(prm, grid, smem, s));
+    e = RC == 1   ? launch_split<2, 1>(prm, grid, smem, s)
+        : RC == 4 ? launch_split<2, 4>(prm, grid, smem, s)
+                  : launch_split<2, 8>(prm, grid, smem, s);
   } else {
-    e = RC == 1 ? launch_split<3, 1>(prm, grid, smem, s)
-                : (RC == 4 ? launch_split<3, 4>(prm, grid, smem, s) : launch_split<3, 8>(prm, grid, smem, s));
+    e = RC == 1   ? launch_split<3, 1>(prm, grid, smem, s)
+        : RC == 4 ? launch_split<3, 4>(prm, grid, smem, s)
+                  : launch_split<3, 8>(prm, grid, smem, s);
   }
   // the coefficient buffer is freed when dcoef leaves scope: wait for the launch
   if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 2
+#define VL1_ABI_VERSION 3
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -261,6 +261,12 @@ typedef struct vl1_duffy_args {
                                     evaluates the basis in product form) */
   double* result;                /* [n_entries] (x2 interleaved if Helmholtz) */
   int deriv_axis;                /* derivative axis of the *_DERIV kinds */
+  /* Optional (NULL = one lane per entry): entries sorted so that each chunk
+   * [chunk_start[c], chunk_start[c + 1]) holds at most 64 entries of one
+   * target point (same target_points and decomposition); every node's
+   * geometry and kernel value is then computed once per chunk. */
+  const int64_t* chunk_start;    /* device [n_chunks + 1] */
+  int64_t n_chunks;
 } vl1_duffy_args;
 
 int vl1_duffy_build(const vl1_duffy_args* a, void* stream);

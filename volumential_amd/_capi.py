@@ -181,6 +181,8 @@ class DuffyArgs(C.Structure):
         ("bary_w", vp),
         ("result", vp),
         ("deriv_axis", C.c_int),
+        ("chunk_start", vp),
+        ("n_chunks", i64),
     ]
 
 
@@ -245,7 +247,7 @@ def lib():
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 2:
+    if L.vl1_abi_version() != 3:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

@@ -10,10 +10,12 @@
 // `duffy_invariant_fused_table` (volumential/nearfield_potential_table.py:
 // 2098-2201), launched over every canonical entry of every case at once.
 //
-// GPU mapping: one lane per entry; the node loop is sequential and all lanes
-// of a wavefront read the same node (uniform, scalar-cached loads).  A sign
-// block whose sub-box is empty for the entry (some len_k == 0) is skipped as
-// a whole: every node in it is inactive.
+// GPU mapping (per-entry form): one lane per entry; the node loop is
+// sequential and all lanes of a wavefront read the same node (uniform,
+// scalar-cached loads).  A sign block whose sub-box is empty for the entry
+// (some len_k == 0) is skipped as a whole: every node in it is inactive.
+// The segmented form (duffy_seg_kernel, below) shares each node's geometry
+// and kernel value among the entries of one target point.
 #include <cmath>
 #include <string>
 
@@ -35,6 +37,8 @@ struct DuffyParams {
   const double* xi;
   const double* bw;
   double* out;
+  const int64_t* chunk_start;  // segmented form: [n_chunks + 1], entries sorted by target point
+  int64_t n_chunks;
 };
 
 // Lagrange basis l_m(y) on the q interpolation nodes, product form:
@@ -177,8 +181,170 @@ __global__ void __launch_bounds__(256) duffy_kernel(const DuffyParams p) {
   }
 }
 
+// Segmented form.  The entries of one (case, target node) pair share the
+// target point x, the decomposition vertex v and therefore every node y and
+// kernel value G(x - y); they differ only in the source mode m, i.e. in the
+// Lagrange factor prod_k l_{m_k}(y_k).  One wavefront takes a chunk of at
+// most 64 such entries.  Per round each lane produces one node record in
+// LDS -- K = G(x - y) jac prod_k len_k and the q basis values l_0..l_{q-1}
+// of each axis -- and then every lane accumulates its entry over the
+// wave's 64 records: K * prod_k l_{m_k}(y_k), a few LDS reads and
+// multiplies per (entry, node) instead of the whole geometry and kernel
+// evaluation.  With fewer than 64 entries, R = 64 / ne' lanes share an
+// entry (ne' = ne rounded up to a power of two) and split the records,
+// then combine by shuffles.  Nodes, signs and the skip rule are those of
+// duffy_kernel; only the summation order differs.
+constexpr int kSegNodes = 64;
+
+template <int DIM, int KIND, int Q>
+__global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
+  constexpr int kMaxQ = 16;
+  constexpr int VW = KIND == VL1_KERNEL_HELMHOLTZ ? 2 : 1;
+  const int q = Q > 0 ? Q : p.q;
+  extern __shared__ double rec[];  // [kSegNodes][VW + DIM * q]
+  const int stride = VW + DIM * q;
+  const int lane = threadIdx.x;
+  const int64_t e0 = p.chunk_start[blockIdx.x];
+  const int ne = (int)(p.chunk_start[blockIdx.x + 1] - e0);
+  int nep = 1;
+  while (nep < ne) nep *= 2;                 // ne' (<= 64)
+  const int R = kSegNodes / nep;             // lanes per entry
+  const int el = lane % nep, part = lane / nep;
+  const bool active = el < ne;
+  double x[DIM], v[DIM];
+  int m[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    x[a] = p.xt[a * p.n_entries + e0];
+    v[a] = p.vt[a * p.n_entries + e0];
+    m[a] = active ? p.mode[a * p.n_entries + e0 + el] : 0;
+  }
+  double xi[kMaxQ], cm[kMaxQ];
+#pragma unroll
+  for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j)
+    if (Q > 0 || j < q) xi[j] = p.xi[j];
+#pragma unroll
+  for (int i = 0; i < (Q > 0 ? Q : kMaxQ); ++i) {
+    if (Q == 0 && i >= q) break;
+    double den = 1.0;
+#pragma unroll
+    for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j)
+      if (Q > 0 || j < q) den *= (j == i) ? 1.0 : (xi[i] - xi[j]);
+    cm[i] = 1.0 / den;
+  }
+  const double tol = 8.0 * 2.220446049250313e-16 * p.H;
+  double acc_re = 0.0, acc_im = 0.0;
+  const int64_t nsign = p.n_nodes / p.nodes_per_sign;
+  double* my = rec + (size_t)lane * stride;
+  for (int64_t b = 0; b < nsign; ++b) {
+    const int64_t base = b * p.nodes_per_sign;
+    double len[DIM], sl[DIM], lenprod = 1.0;
+    bool pos = true;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      const double sg = p.sg[a * p.n_nodes + base];
+      len[a] = sg < 0.0 ? v[a] : p.H - v[a];
+      pos = pos && (len[a] > 0.0);
+      sl[a] = sg * len[a];
+      lenprod *= len[a];
+    }
+    if (!pos) continue;  // empty sub-box (uniform over the wave)
+    for (int64_t r0 = 0; r0 < p.nodes_per_sign; r0 += kSegNodes) {
+      // produce this lane's node record
+      const int64_t i = r0 + lane;
+      double kr = 0.0, ki = 0.0;
+      double bas[DIM][kMaxQ];
+      bool ok = false;
+      if (i < p.nodes_per_sign) {
+        const int64_t node = base + i;
+        double r2 = 0.0, da = 0.0, yk[DIM];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+          yk[a] = v[a] + sl[a] * p.u[a * p.n_nodes + node];
+          const double d = x[a] - yk[a];
+          r2 += d * d;
+          if (a == p.deriv_axis) da = d;
+        }
+        if (r2 > 0.0) {
+          ok = true;
+          kernel_value<DIM, KIND>(r2, da, p.k, kr, ki);
+          const double w = p.jac[node] * lenprod;
+          kr *= w;
+          ki *= w;
+#pragma unroll
+          for (int a = 0; a < DIM; ++a) {
+            // l_i(y) = c_i prod_{j != i} (y - xi_j) by prefix / suffix
+            // products, with the node-hit rule of lagrange_basis
+            double d[kMaxQ], pre[kMaxQ];
+            int hit = -1;
+            double acc = 1.0;
+#pragma unroll
+            for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j) {
+              if (Q == 0 && j >= q) break;
+              d[j] = yk[a] - xi[j];
+              if (hit < 0 && fabs(d[j]) < tol) hit = j;
+              pre[j] = acc;
+              acc *= d[j];
+            }
+            double suf = 1.0;
+#pragma unroll
+            for (int j = (Q > 0 ? Q : kMaxQ) - 1; j >= 0; --j) {
+              if (Q == 0 && j >= q) continue;
+              bas[a][j] = hit >= 0 ? (j == hit ? 1.0 : 0.0) : cm[j] * pre[j] * suf;
+              suf *= d[j];
+            }
+          }
+        }
+      }
+      __syncthreads();  // the previous round's records have been read
+      my[0] = kr;
+      if (VW == 2) my[1] = ki;
+#pragma unroll
+      for (int a = 0; a < DIM; ++a)
+#pragma unroll
+        for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j) {
+          if (Q == 0 && j >= q) break;
+          my[VW + a * q + j] = ok ? bas[a][j] : 0.0;
+        }
+      __syncthreads();
+      // consume: this lane's share of the wave's records for its entry
+      const int nrec = (int)min<int64_t>(kSegNodes, p.nodes_per_sign - r0);
+      if (active) {
+        for (int t = part; t < nrec; t += R) {
+          const double* rr = rec + (size_t)t * stride;
+          double bp = rr[VW + m[0]];
+#pragma unroll
+          for (int a = 1; a < DIM; ++a) bp *= rr[VW + a * q + m[a]];
+          acc_re = fma(rr[0], bp, acc_re);
+          if (VW == 2) acc_im = fma(rr[1], bp, acc_im);
+        }
+      }
+    }
+  }
+  // combine the R lanes of each entry (lanes el + nep * part)
+  for (int off = nep; off < kSegNodes; off *= 2) {
+    acc_re += __shfl_xor(acc_re, off);
+    if (VW == 2) acc_im += __shfl_xor(acc_im, off);
+  }
+  if (active && part == 0) {
+    if (VW == 2) {
+      p.out[2 * (e0 + el)] = acc_re;
+      p.out[2 * (e0 + el) + 1] = acc_im;
+    } else {
+      p.out[e0 + el] = acc_re;
+    }
+  }
+}
+
 template <int DIM, int KIND, int Q>
 static hipError_t launch_duffy_q(const DuffyParams& prm, hipStream_t s) {
+  if (prm.chunk_start) {
+    const int vw = KIND == VL1_KERNEL_HELMHOLTZ ? 2 : 1;
+    const size_t smem = sizeof(double) * kSegNodes * (vw + (size_t)DIM * prm.q);
+    hipLaunchKernelGGL((duffy_seg_kernel<DIM, KIND, Q>), dim3((unsigned)prm.n_chunks),
+                       dim3(kSegNodes), smem, s, prm);
+    return hipGetLastError();
+  }
   const int block = 256;
   const int64_t grid = (prm.n_entries + block - 1) / block;
   hipLaunchKernelGGL((duffy_kernel<DIM, KIND, Q>), dim3((unsigned)grid), dim3(block), 0, s, prm);
@@ -220,11 +386,14 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
                      a->kernel_kind == VL1_KERNEL_LAPLACE_SOURCE_DERIV;
   if (deriv && (a->deriv_axis < 0 || a->deriv_axis >= a->dim))
     return fail(VL1_E_ARG, "vl1_duffy_build: derivative axis out of range");
+  if (a->chunk_start && (a->n_chunks < 1 || a->n_chunks > INT32_MAX))
+    return fail(VL1_E_ARG, "vl1_duffy_build: bad chunk count");
   DuffyParams prm{a->dim, a->quad_order, a->kernel_kind, deriv ? a->deriv_axis : -1,
                   a->helmholtz_k, a->n_entries,
                   a->n_nodes, a->nodes_per_sign, a->source_box_extent, a->target_points,
                   a->decomposition, a->mode_axes, a->node_u, a->node_sign, a->node_jac,
-                  a->interp_nodes, a->bary_w, a->result};
+                  a->interp_nodes, a->bary_w, a->result, a->chunk_start,
+                  a->chunk_start ? a->n_chunks : 0};
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   const int kind = a->kernel_kind;

@@ -20,11 +20,40 @@ def _dev(a: np.ndarray, dtype=None) -> torch.Tensor:
     return t.to("cuda", non_blocking=False)
 
 
-def duffy_values(table, entry_ids, regular_order, radial_order, stream=None) -> np.ndarray:
+def target_point_chunks(table, entry_ids, max_chunk=64):
+    """Sort entries by target point (case, target node) and cut each run
+    into near-equal chunks of at most ``max_chunk``: ``(order, starts)``
+    with ``entry_ids[order]`` the sorted entries and ``starts`` the chunk
+    offsets (int64, ``n_chunks + 1``)."""
+    entry_ids = np.asarray(entry_ids, dtype=np.int64)
+    key = (entry_ids // table.n_pairs) * table.n_q_points + entry_ids % table.n_q_points
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    run = np.flatnonzero(np.r_[True, ks[1:] != ks[:-1], True])
+    starts = [0]
+    for a, b in zip(run[:-1], run[1:]):
+        n = int(b - a)
+        k = -(-n // max_chunk)
+        cuts = a + (np.arange(1, k + 1) * n) // k
+        starts.extend(int(c) for c in cuts)
+    return order, np.asarray(starts, dtype=np.int64)
+
+
+def duffy_values(table, entry_ids, regular_order, radial_order, stream=None,
+                 segmented=True) -> np.ndarray:
+    """Duffy values of ``entry_ids`` (one launch).  ``segmented``: share each
+    node's geometry and kernel value among the entries of one target point
+    (the fast form); ``False`` runs one lane per entry."""
     if not torch.cuda.is_available():
         raise RuntimeError("the Duffy table build runs on the GPU; no GPU is visible")
     L = _capi.lib()
     nodes = quadrature.duffy_node_set(table.dim, int(regular_order), int(radial_order))
+    entry_ids = np.asarray(entry_ids, dtype=np.int64)
+    order = None
+    if segmented and entry_ids.size:
+        order, starts = target_point_chunks(table, entry_ids)
+        entry_ids = entry_ids[order]
+        d_starts = _dev(starts)
     tp, dv, axes = table.duffy_inputs(entry_ids)
     xi = table.interp_nodes
     bw = quadrature.barycentric_weights(xi)
@@ -58,6 +87,8 @@ def duffy_values(table, entry_ids, regular_order, radial_order, stream=None) -> 
         bary_w=d_bw.data_ptr(),
         result=out.data_ptr(),
         deriv_axis=int(getattr(table.integral_knl, "axis", -1)),
+        chunk_start=d_starts.data_ptr() if order is not None else None,
+        n_chunks=int(starts.size - 1) if order is not None else 0,
     )
     st = stream if stream is not None else torch.cuda.current_stream()
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -68,7 +99,13 @@ def duffy_values(table, entry_ids, regular_order, radial_order, stream=None) -> 
     torch.cuda.synchronize()
     table.last_build_kernel_ms = ev0.elapsed_time(ev1)
     table.last_build_node_evals = int(n_entries) * int(nodes.n_nodes)
+    table.last_build_chunks = int(starts.size - 1) if order is not None else int(n_entries)
+    table.last_build_n_nodes = int(nodes.n_nodes)
     res = out.cpu().numpy()
     if cplx:
         res = res[0::2] + 1j * res[1::2]
+    if order is not None:
+        unsorted = np.empty_like(res)
+        unsorted[order] = res
+        res = unsorted
     return res

@@ -224,3 +224,48 @@ def test_single_precision_wrangler_rounds_fp64_result(cuda):
     p32, _ = NearFieldWrangler(tr, tables[0], dtype=np.float32).eval_direct(*args, f)
     assert p32[0].dtype == torch.float32
     assert torch.equal(p32[0], p64[0].to(torch.float32))
+
+
+@pytest.mark.parametrize("gather", [False, True])
+def test_plain_table_paths(cuda, gather, monkeypatch):
+    """The plain kernel's two table paths -- the expanded per-(level, case)
+    blocks (default) and the staged orbit block with row-map gathers
+    (VL1_PLAIN_TABLE=gather, used beyond 8 GiB) -- on real, complex and
+    signed tables, against the oracle."""
+    from volumential_amd.kernels import AxisTargetDerivative, HelmholtzKernel, LaplaceKernel
+    from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
+
+    print(f"case: plain table path gather={gather}", flush=True)
+    if gather:
+        monkeypatch.setenv("VL1_PLAIN_TABLE", "gather")
+    tr = vtree.uniform_tree(2, 4, 3)
+    f = densities.gaussian_mixture_batch(tr.points, [90, 91, 92])
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    meta = orbit_ref.case_metadata(2, 4)
+    # real canonical (2D log term)
+    tables = synthetic_table_levels(2, 4, seed=21)
+    got = NearFieldWrangler(tr, tables[0], direct_method="plain").eval_direct_batched(
+        *args, torch.as_tensor(f, device="cuda"))[0][0].cpu().numpy()
+    ref = _oracle(tr, tables, f, "canonical", 2, True)
+    assert max(rel_errors(got, ref)) <= TOL
+    # signed (gradient) table
+    g = NearFieldInteractionTable(4, 2, AxisTargetDerivative(1, LaplaceKernel(2)),
+                                  source_box_extent=2.0)
+    g.build_table(DuffyBuildConfig(regular_quad_order=8, radial_quad_order=21))
+    got = NearFieldWrangler(tr, g, direct_method="plain").eval_direct_batched(
+        *args, torch.as_tensor(f, device="cuda"))[0][0].cpu().numpy()
+    ref = ocpu.near_field(tr, g.compact_values(), g.mode_normalizers,
+                          orbit_ref.case_metadata(2, 4, (1,), (0, 1)), f, mode="canonical",
+                          scaling_power=1, table_root_extent=2.0)
+    assert max(rel_errors(got, ref)) <= TOL
+    # complex per-level table at the tree's level (fixed single table)
+    h = NearFieldInteractionTable(4, 2, HelmholtzKernel(2, 9.0), source_box_extent=0.25,
+                                  source_box_level=3)
+    h.build_table(DuffyBuildConfig(regular_quad_order=8, radial_quad_order=21))
+    fc = f[0] + 1j * f[1]
+    got = NearFieldWrangler(tr, h, direct_method="plain").eval_direct(
+        *args, torch.as_tensor(fc, device="cuda"))[0][0].cpu().numpy()
+    ref = list1_ref.near_field_from_tree(
+        tr, h.compact_values()[None], h.mode_normalizers[None], meta, fc, mode="fixed",
+        table_root_extent=0.25, table_starting_level=3)[0]
+    assert max(rel_errors(got, ref)) <= TOL

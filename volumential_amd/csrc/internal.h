@@ -25,9 +25,12 @@ struct vl1_payload {
   vl1::DevBuf offs;        // int32 [n_orbits+1]
   vl1::DevBuf rowmap;      // [n_cases][n (s)][n (t)] relative compact rows; for signed
                            // kernels (int32) ~row marks an entry read negated
+  // [n_tables][n_cases][n (s)][n (t)] table values (sign applied; x2 complex),
+  // expanded on the first plain evaluation (mutable: a cache of the payload)
+  mutable vl1::DevBuf dense;
   int64_t device_bytes() const {
     return (int64_t)(table.bytes + nmlz.bytes + case_idx.bytes + ranks.bytes + offs.bytes +
-                     rowmap.bytes);
+                     rowmap.bytes + dense.bytes);
   }
 };
 

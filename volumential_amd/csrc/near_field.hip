@@ -10,15 +10,19 @@
 // and the scaling/level policy of list1.py:361-437.
 //
 // GPU mapping: one workgroup per target box, taken in CSR order.  For every
-// source box the canonical orbit block of the table (all compact rows of the
-// case's orbit) is staged in LDS when it fits, together with the source
-// densities of up to RC right-hand sides; each lane owns one target node and
-// walks the source modes, gathering T through the precomputed per-case row
-// map (coalesced over target nodes).  For boxes with fewer than 256 nodes the
-// lanes are split into P parts over the source modes and reduced through LDS
-// at the end ("wavefront reduction over list-1 sources").
+// source box the densities of up to RC right-hand sides are staged in LDS;
+// each lane owns one target node and walks the source modes.  The table
+// values come from the payload's expanded per-(level, case) blocks
+// D[s][t] (built once, on the first plain evaluation, while they stay
+// under 8 GiB): one coalesced line per source mode.  Beyond that size the
+// canonical orbit block is staged in LDS when it fits and gathered through
+// the per-case row map.  For boxes with fewer than 256 nodes the lanes are
+// split into P parts over the source modes and reduced through LDS at the
+// end ("wavefront reduction over list-1 sources").
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <mutex>
 #include <string>
 
 #include <type_traits>
@@ -49,6 +53,8 @@ struct NFParams {
   int n_tables;
   const double* nmlz;
   const void* rowmap;
+  const double* dense;  // expanded table (DENSE kernels), [n_tables][n_cases][n][n] (x VW)
+  int n_cases;
   int dim, n;
   int scaling_mode, scaling_power, log_disp;
   double root_extent, table_root_extent;
@@ -68,7 +74,11 @@ constexpr int kNFBlock = 256;
 
 // CPLX: complex table, densities and result (interleaved re, im doubles;
 // leading dimensions in doubles); products are complex, the scaling real.
-template <typename IdxT, int RC, bool CPLX>
+// DENSE: table values read from the expanded per-(level, case) blocks,
+// [s][t] with t fastest, so the lanes of a wave (consecutive target nodes)
+// read one coalesced line per source mode instead of a row-map load plus an
+// LDS gather; no orbit block is staged.
+template <typename IdxT, int RC, bool CPLX, bool DENSE>
 __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams p) {
   extern __shared__ double smem[];
   constexpr int VW = CPLX ? 2 : 1;  // doubles per value
@@ -138,7 +148,9 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
         const int off = p.offs[rank];
         const int cnt = p.offs[rank + 1] - off;
         const double* gT = p.table + ((int64_t)tlev * p.n_entries + off) * VW;
-        const bool staged = cnt * VW <= p.tbuf;
+        const bool staged = !DENSE && cnt * VW <= p.tbuf;
+        const double* gD =
+            DENSE ? p.dense + (((int64_t)tlev * p.n_cases + cid) * n * n + tmode) * VW : nullptr;
         const int sbeg = p.src_starts[sb];
         const int ns = min(p.src_counts[sb], n);
         if (staged)
@@ -159,22 +171,29 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
           for (int sid = part; sid < ns; sid += P) {
             const int s = sMode[sid];
             if ((unsigned)s >= (unsigned)n) continue;  // malformed mode id: no contribution
-            int rel = (int)rm[(int64_t)s * n];
+            double tr, ti;  // table value (imaginary part for complex data)
             double esc = scal;  // scaling times the entry sign of signed kernels
-            if constexpr (std::is_signed<IdxT>::value) {
-              double sg;
-              rel = decode_signed_row(rel, sg);
-              esc *= sg;
+            if constexpr (DENSE) {
+              tr = gD[(int64_t)s * n * VW];
+              ti = CPLX ? gD[(int64_t)s * n * VW + 1] : 0.0;
+            } else {
+              int rel = (int)rm[(int64_t)s * n];
+              if constexpr (std::is_signed<IdxT>::value) {
+                double sg;
+                rel = decode_signed_row(rel, sg);
+                esc *= sg;
+              }
+              tr = staged ? sT[rel * VW] : gT[rel * VW];
+              ti = CPLX ? (staged ? sT[rel * VW + 1] : gT[rel * VW + 1]) : 0.0;
             }
             const double* fs = sF + sid * RC * VW;
             if constexpr (!CPLX) {
-              const double tv = staged ? sT[rel] : gT[rel];
-              const double v = tv * esc + sD[sid];
+              const double v = tr * esc + sD[sid];
 #pragma unroll
               for (int r = 0; r < RC; ++r) acc[r] = fma(fs[r], v, acc[r]);
             } else {
-              const double vr = (staged ? sT[2 * rel] : gT[2 * rel]) * esc;
-              const double vi = (staged ? sT[2 * rel + 1] : gT[2 * rel + 1]) * esc;
+              const double vr = tr * esc;
+              const double vi = ti * esc;
 #pragma unroll
               for (int r = 0; r < RC; ++r) {
                 const double fr = fs[2 * r], fi = fs[2 * r + 1];
@@ -214,9 +233,58 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
   }
 }
 
-template <typename IdxT, int RC, bool CPLX>
+// Expanded table for the DENSE kernels: D[lev][case][s][t] = sign * T[lev,
+// row(case, s, t)], read through the payload's row map.
+template <typename IdxT>
+__global__ void __launch_bounds__(256)
+dense_table_kernel(int n, int n_cases, int n_tables, int vw, const IdxT* __restrict__ rowmap,
+                   const uint16_t* __restrict__ ranks, const int32_t* __restrict__ offs,
+                   const double* __restrict__ table, int64_t n_entries, double* __restrict__ D) {
+  const int64_t nn = (int64_t)n * n;
+  const int64_t total = (int64_t)n_tables * n_cases * nn;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cst = i % ((int64_t)n_cases * nn);
+    const int lev = (int)(i / ((int64_t)n_cases * nn));
+    const int c = (int)(cst / nn);
+    int rel = (int)rowmap[cst];
+    double sg = 1.0;
+    if constexpr (std::is_signed<IdxT>::value) rel = decode_signed_row(rel, sg);
+    const double* T = table + ((int64_t)lev * n_entries + offs[ranks[c]] + rel) * vw;
+    D[i * vw] = sg * T[0];
+    if (vw == 2) D[i * vw + 1] = sg * T[1];
+  }
+}
+
+// Build the payload's expanded table once (first plain evaluation).
+static hipError_t ensure_dense(const vl1_payload* pl, hipStream_t s) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (pl->dense.bytes) return hipSuccess;
+  const int vw = pl->is_complex ? 2 : 1;
+  const int64_t total = (int64_t)pl->n_tables * pl->n_cases * pl->n * pl->n;
+  hipError_t e = pl->dense.alloc((size_t)total * vw * sizeof(double));
+  if (e != hipSuccess) return e;
+  const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+  if (pl->rowmap_u16)
+    hipLaunchKernelGGL(dense_table_kernel<uint16_t>, dim3(grid), dim3(256), 0, s, pl->n,
+                       pl->n_cases, pl->n_tables, vw, pl->rowmap.as<uint16_t>(),
+                       pl->ranks.as<uint16_t>(), pl->offs.as<int32_t>(), pl->table.as<double>(),
+                       pl->n_entries, pl->dense.as<double>());
+  else
+    hipLaunchKernelGGL(dense_table_kernel<int32_t>, dim3(grid), dim3(256), 0, s, pl->n,
+                       pl->n_cases, pl->n_tables, vw, pl->rowmap.as<int32_t>(),
+                       pl->ranks.as<uint16_t>(), pl->offs.as<int32_t>(), pl->table.as<double>(),
+                       pl->n_entries, pl->dense.as<double>());
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) pl->dense.release();
+  return e;
+}
+
+template <typename IdxT, int RC, bool CPLX, bool DENSE>
 static hipError_t launch_nf(const NFParams& prm, int grid, size_t smem, hipStream_t s) {
-  auto fn = near_field_f64_kernel<IdxT, RC, CPLX>;
+  auto fn = near_field_f64_kernel<IdxT, RC, CPLX, DENSE>;
   hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)smem);
   if (e != hipSuccess) return e;
@@ -295,37 +363,47 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
   while (nt < pl->n && nt < kNFBlock) nt *= 2;
   prm.nt = nt;
   const int P = kNFBlock / nt;
+  // densities per pass (32 measured 4.6x slower than 16 on 3D q=6: register
+  // pressure)
   const int RC = a->n_rhs == 1 ? 1 : (a->n_rhs <= 4 ? 4 : 16);
   const size_t lds_cap = 150 * 1024;
   const size_t vw = pl->is_complex ? 2 : 1;
+  hipStream_t s = (hipStream_t)stream;
+  // expanded table: coalesced reads per source mode, no row-map gathers;
+  // used while it stays under 8 GiB (a cache of the payload)
+  // (VL1_PLAIN_TABLE=gather in the environment selects the row-map path)
+  const char* mode_env = std::getenv("VL1_PLAIN_TABLE");
+  const bool force_gather = mode_env && std::string(mode_env) == "gather";
+  const bool dense = !force_gather &&
+      (double)pl->n_tables * pl->n_cases * pl->n * pl->n * vw * sizeof(double) <= 8.0 * (1ull << 30);
+  if (dense) {
+    hipError_t de = ensure_dense(pl, s);
+    if (de != hipSuccess)
+      return fail(VL1_E_HIP, std::string("vl1_near_field_f64: dense table: ") + hipGetErrorString(de));
+    prm.dense = pl->dense.as<double>();
+  }
+  prm.n_cases = pl->n_cases;
   const size_t fixed =
       sizeof(double) * ((size_t)pl->n * RC * vw + pl->n + (P > 1 ? kNFBlock * RC * vw : 0)) +
       sizeof(int) * pl->n;
   if (fixed > lds_cap) return fail(VL1_E_UNSUPPORTED, "vl1_near_field_f64: box too large for LDS");
-  size_t tbuf = std::min<size_t>((size_t)pl->max_orbit_count * vw, (lds_cap - fixed) / sizeof(double));
+  size_t tbuf = dense ? 0
+                      : std::min<size_t>((size_t)pl->max_orbit_count * vw,
+                                         (lds_cap - fixed) / sizeof(double));
   prm.tbuf = (int)tbuf;
   const size_t smem = fixed + sizeof(double) * tbuf;
-  hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   const bool u16 = pl->rowmap_u16 != 0;
   const int grid = a->n_tgt_boxes;
-  if (pl->is_complex) {
-    if (RC == 1)
-      e = u16 ? launch_nf<uint16_t, 1, true>(prm, grid, smem, s) : launch_nf<int32_t, 1, true>(prm, grid, smem, s);
-    else if (RC == 4)
-      e = u16 ? launch_nf<uint16_t, 4, true>(prm, grid, smem, s) : launch_nf<int32_t, 4, true>(prm, grid, smem, s);
-    else
-      e = u16 ? launch_nf<uint16_t, 16, true>(prm, grid, smem, s)
-              : launch_nf<int32_t, 16, true>(prm, grid, smem, s);
-  } else {
-    if (RC == 1)
-      e = u16 ? launch_nf<uint16_t, 1, false>(prm, grid, smem, s) : launch_nf<int32_t, 1, false>(prm, grid, smem, s);
-    else if (RC == 4)
-      e = u16 ? launch_nf<uint16_t, 4, false>(prm, grid, smem, s) : launch_nf<int32_t, 4, false>(prm, grid, smem, s);
-    else
-      e = u16 ? launch_nf<uint16_t, 16, false>(prm, grid, smem, s)
-              : launch_nf<int32_t, 16, false>(prm, grid, smem, s);
-  }
+#define VL1_NF_LAUNCH(RCV, CPX)                                                                  \
+  (dense ? launch_nf<uint16_t, RCV, CPX, true>(prm, grid, smem, s)                               \
+         : (u16 ? launch_nf<uint16_t, RCV, CPX, false>(prm, grid, smem, s)                       \
+                : launch_nf<int32_t, RCV, CPX, false>(prm, grid, smem, s)))
+  if (pl->is_complex)
+    e = RC == 1 ? VL1_NF_LAUNCH(1, true) : (RC == 4 ? VL1_NF_LAUNCH(4, true) : VL1_NF_LAUNCH(16, true));
+  else
+    e = RC == 1 ? VL1_NF_LAUNCH(1, false) : (RC == 4 ? VL1_NF_LAUNCH(4, false) : VL1_NF_LAUNCH(16, false));
+#undef VL1_NF_LAUNCH
   if (e != hipSuccess)
     return fail(VL1_E_HIP, std::string("near_field_f64 launch: ") + hipGetErrorString(e));
   return VL1_OK;

@@ -193,6 +193,29 @@ def test_plan_route_rhs_chunks_and_repeat_calls(cuda):
     assert l2 <= TOL and linf <= TOL, (l2, linf)
 
 
+def test_plan_cache_evicts_least_recent(cuda):
+    """With room for one plan, a second CSR evicts (and frees) the first;
+    evaluating the first again rebuilds it with the same result."""
+    print("case: plan cache eviction, 2D q=4 L4", flush=True)
+    tr = vtree.uniform_tree(2, 4, 4)
+    tables = synthetic_table_levels(2, 4, seed=9)
+    w = NearFieldWrangler(tr, tables[0])
+    w.max_cached_plans = 1
+    f = torch.as_tensor(densities.gaussian_mixture_batch(tr.points, [3])[0], device="cuda")
+    full = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    n_half = len(tr.target_boxes) // 2
+    starts = np.asarray(tr.neighbor_source_boxes_starts)
+    half = (np.asarray(tr.target_boxes)[:n_half], starts[:n_half + 1],
+            np.asarray(tr.neighbor_source_boxes_lists)[:starts[n_half]])
+    a = w.eval_direct(*full, f)[0].clone()
+    first = next(iter(w._plans.values()))
+    w.eval_direct(*half, f)
+    assert len(w._plans) == 1 and next(iter(w._plans.values())) is not first
+    b = w.eval_direct(*full, f)[0]
+    assert len(w._plans) == 1
+    assert torch.equal(a, b)
+
+
 def test_plan_route_box_size_beyond_tiles(cuda):
     """3D q=7 (343 nodes per box) is beyond the tile kernels: the plan route
     hands the kernel to the plain kernel and still meets the FP64 bar."""

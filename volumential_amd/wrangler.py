@@ -24,6 +24,8 @@ remainder ``G_k - G_0`` over list 1 (``split.HelmholtzSplitCorrection``).
 
 from __future__ import annotations
 
+from collections import OrderedDict
+
 import numpy as np
 import torch
 
@@ -131,7 +133,10 @@ class NearFieldWrangler:
                     f"only {len(lst)} tables starting at level {lvl0} were given."
                 )
         self._evaluators = {}
-        self._plans = {}
+        # plans by (kernel, CSR, fused): least recently used first; each holds
+        # its expanded blocks on the device, so the cache is bounded
+        self._plans = OrderedDict()
+        self.max_cached_plans = 8
         self._corrections = {}
         self.helmholtz_split_order = helmholtz_split_order
         self.helmholtz_split = False
@@ -353,6 +358,7 @@ class NearFieldWrangler:
         key = _csr_key(out_kernel, target_boxes, neighbor_source_boxes_starts,
                        neighbor_source_boxes_lists) + (bool(fused_split),)
         if key in self._plans:
+            self._plans.move_to_end(key)
             return self._plans[key]
         tables = self.near_field_table[repr(out_kernel)]
         t0 = tables[0]
@@ -393,6 +399,9 @@ class NearFieldWrangler:
             helmholtz_split=split_desc,
         )
         self._plans[key] = plan
+        while len(self._plans) > self.max_cached_plans:
+            _, old = self._plans.popitem(last=False)
+            old.close()
         return plan
 
     def eval_direct_plan(self, target_boxes, neighbor_source_boxes_starts,

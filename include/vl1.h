@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 3
+#define VL1_ABI_VERSION 4
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -195,13 +195,27 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* p, const vl1_plan_desc* d
  * eval_direct_helmholtz_split_correction (3491-3836) in one contraction.
  * `laplace` is a real scalar Laplace payload, d->scaling_mode canonical; the
  * plan then evaluates complex densities like a complex-table plan.  In 2D
- * the box-local node ids must be in mode order (VL1_E_UNSUPPORTED). */
+ * the box-local node ids must be in mode order (VL1_E_UNSUPPORTED).
+ * An axis target / source derivative wrapper (3884-3957, order 1) takes
+ * the wrapped Laplace table's signed payload (canonical power 1, no log
+ * term) and the remainder's axis derivative +-(x_a - y_a) R'(r) / r. */
+/* Derivative wrapper of a split output kernel (expansion_wrangler_fpnd.py:
+ * 3884-3957): none, AxisTargetDerivative (d/dx_a), AxisSourceDerivative
+ * (d/dy_a = -d/dx_a). */
+typedef enum vl1_split_deriv {
+  VL1_SPLIT_DERIV_NONE = 0,
+  VL1_SPLIT_DERIV_TARGET = 1,
+  VL1_SPLIT_DERIV_SOURCE = 2
+} vl1_split_deriv;
+
 typedef struct vl1_helmholtz_split_blocks {
   double k_re, k_im;          /* wave number (Yukawa: k = i lambda) */
   int series_nmax;            /* series terms, 1..96 (4337-4389) */
   const double* unit_nodes;   /* [q] Gauss-Legendre nodes on [0, 1] (host) */
   const double* unit_weights; /* [q] their weights (host) */
   const int32_t* case_vecs;   /* [n_cases][dim] list-1 gallery vectors (host) */
+  int deriv_kind;             /* vl1_split_deriv: derivative wrapper of the output kernel */
+  int deriv_axis;             /* its axis (deriv_kind != VL1_SPLIT_DERIV_NONE) */
 } vl1_helmholtz_split_blocks;
 
 int vl1_plan_create_helmholtz_split(vl1_plan** out, const vl1_payload* laplace,
@@ -298,10 +312,14 @@ typedef struct vl1_split_args {
   double* result;                /* [n_rhs][ld_result] complex; targets of the listed boxes
                                     overwritten */
   int64_t ld_result;             /* doubles */
+  int deriv_kind;                /* vl1_split_deriv of the output kernel */
+  int deriv_axis;                /* its axis */
 } vl1_split_args;
 
 /* corr[t] = sum over the list-1 source points s of R(|x_t - y_s|) strength[s],
- * R = G_k - G_0 as the reference's series (self pairs: its r -> 0 limit).
+ * R = G_k - G_0 as the reference's series (self pairs: its r -> 0 limit);
+ * with a derivative wrapper +-(x_a - y_a) R'(r)/r (self pairs: 0, the
+ * reference's limit for a wrapped chain, 5293-5300).
  * Synchronous (returns after the launch completes). */
 int vl1_split_correction(const vl1_split_args* a, void* stream);
 

@@ -15,11 +15,19 @@ the base quadrature (the reference's default path):
 * ``3491-3836`` -- the correction: a P2P sum over list 1 of
   ``remainder(x_t - y_s) * strength_s`` (strength = density x weight),
   self pairs excluded and replaced by ``limit * strength_t``;
-* Yukawa: ``k = i lambda`` (``4175-4206``).
+* Yukawa: ``k = i lambda`` (``4175-4206``);
+* axis target / source derivative wrappers around the Helmholtz kernel
+  (``3884-3957``: the remainder P2P runs on the wrapped series kernel, its
+  self limit is 0 for a wrapped chain, ``5293-5300``): the term-wise
+  derivative ``d/dx_a R = (x_a - y_a) R'(r) / r`` (sign flipped for a
+  source derivative).
 
 Pins (tests): the series equals the closed form ``G_k - G_0`` from scipy's
 Hankel function (2D) / ``exp`` (3D) where the series is well conditioned,
-and its r -> 0 value equals the limit.
+and its r -> 0 value equals the limit; the derivative series equals the
+closed form ``R'(r)/r`` (``-(ik/4) H_1(kr) + 1/(2 pi r)`` over r in 2D,
+``(e^{ikr}(ikr - 1) + 1)/(4 pi r^3)`` in 3D) and a central difference of
+``remainder``.
 """
 
 from __future__ import annotations
@@ -97,6 +105,43 @@ def remainder(dim, k, nmax, r):
     return out
 
 
+def remainder_radial_derivative(dim, k, nmax, r):
+    """``R'(r) / r`` of the truncated series, term by term (0 at r = 0,
+    where every derivative-wrapped pair value ``d_a R'(r)/r`` vanishes)."""
+    k = complex(k)
+    r = np.asarray(r, dtype=np.float64)
+    out = np.zeros(r.shape, dtype=np.complex128)
+    pos = r > 1e-300
+    rs = np.where(pos, r, 1.0)
+    if dim == 2:
+        if k == 0:
+            return out
+        lkh = np.log(0.5 * k)
+        logr = np.log(rs)
+        for n in range(1, nmax + 1):
+            s_n = ((-1) ** n) * (k * k / 4.0) ** n / (math.factorial(n) ** 2)
+            h_n = float(np.sum(1.0 / np.arange(1, n + 1)))
+            c_log = -s_n / (2.0 * np.pi)
+            c_pow = s_n * ((h_n - (lkh + EULER_GAMMA)) / (2.0 * np.pi) + 0.25j)
+            # d/dr [c_log r^2n log r + c_pow r^2n] / r
+            out += rs ** (2 * n - 2) * (c_log * (2 * n * logr + 1.0) + 2 * n * c_pow)
+    else:
+        for n in range(2, nmax + 1):
+            out += (n - 1) * (1j * k) ** n / (4.0 * np.pi * math.factorial(n)) * rs ** (n - 3)
+    return np.where(pos, out, 0.0)
+
+
+def closed_form_radial_derivative(dim, k, r):
+    """``(G_k - G_0)'(r) / r`` from library functions (pins the above)."""
+    r = np.asarray(r, dtype=np.float64)
+    k = complex(k)
+    if dim == 3:
+        return (np.exp(1j * k * r) * (1j * k * r - 1.0) + 1.0) / (4.0 * np.pi * r ** 3)
+    from scipy.special import hankel1
+
+    return (-0.25j * k * hankel1(1, k * r) + 1.0 / (2.0 * np.pi * r)) / r
+
+
 def self_limit(dim, k):
     """``_helmholtz_split_self_diagonal_limit`` (5293-5317)."""
     k = complex(k)
@@ -117,9 +162,11 @@ def closed_form_difference(dim, k, r):
     return 0.25j * hankel1(0, k * r) + np.log(r) / (2.0 * np.pi)
 
 
-def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=None):
+def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=None,
+               deriv_axis=None, deriv_sign=1):
     """P2P of the remainder over list 1: ``out[R, n_points]`` (complex);
-    targets of unlisted boxes stay zero."""
+    targets of unlisted boxes stay zero.  ``deriv_axis``: the axis of a
+    target (``deriv_sign = +1``) or source (``-1``) derivative wrapper."""
     st = np.atleast_2d(np.asarray(strengths, dtype=np.complex128))
     tb_list = tree.target_boxes if target_boxes is None else target_boxes
     starts = tree.neighbor_source_boxes_starts if starts is None else starts
@@ -134,10 +181,15 @@ def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=N
         for sb in lists[starts[i]:starts[i + 1]]:
             s0, ns = int(bs[sb]), int(bc[sb])
             ys = pts[:, s0:s0 + ns]
-            r = np.sqrt(np.sum((xt[:, :, None] - ys[:, None, :]) ** 2, axis=0))  # [nt, ns]
-            rem = remainder(tree.dim, k, nmax, r)
+            dv = xt[:, :, None] - ys[:, None, :]  # [dim, nt, ns]
+            r = np.sqrt(np.sum(dv ** 2, axis=0))
             same = (xt[:, :, None] == ys[:, None, :]).all(axis=0)
-            rem = np.where(same, self_limit(tree.dim, k), rem)
+            if deriv_axis is None:
+                rem = remainder(tree.dim, k, nmax, r)
+                rem = np.where(same, self_limit(tree.dim, k), rem)
+            else:
+                rem = deriv_sign * dv[deriv_axis] * remainder_radial_derivative(tree.dim, k, nmax, r)
+                rem = np.where(same, 0.0, rem)
             acc += st[:, s0:s0 + ns] @ rem.T
         out[:, t0:t0 + nt] = acc
     return out

@@ -207,11 +207,11 @@ def test_plan_cache_evicts_least_recent(cuda):
     starts = np.asarray(tr.neighbor_source_boxes_starts)
     half = (np.asarray(tr.target_boxes)[:n_half], starts[:n_half + 1],
             np.asarray(tr.neighbor_source_boxes_lists)[:starts[n_half]])
-    a = w.eval_direct(*full, f)[0].clone()
+    a = w.eval_direct(*full, f)[0][0].clone()
     first = next(iter(w._plans.values()))
     w.eval_direct(*half, f)
     assert len(w._plans) == 1 and next(iter(w._plans.values())) is not first
-    b = w.eval_direct(*full, f)[0]
+    b = w.eval_direct(*full, f)[0][0]
     assert len(w._plans) == 1
     assert torch.equal(a, b)
 

@@ -11,7 +11,12 @@
   ends below 1e-3 (2D) / 5e-3 (3D) of the total and stays under a tenth of
   the remainder itself (a wrong sign or term fails);
 * the split's table part: Laplace-table list 1 of Re f and Im f (FP64 plan
-  and plain kernel) vs the oracle.
+  and plain kernel) vs the oracle;
+* axis target / source derivative wrappers of the output kernel: the
+  correction with the remainder's axis derivative vs the oracle, and the
+  whole split list 1 (wrapped Laplace table, power-1 scaling, signed
+  entries, plus that correction) through the two-launch and fused routes vs
+  the oracle's sum.
 """
 
 import numpy as np
@@ -23,7 +28,9 @@ from oracle import orbit_ref
 from oracle import split_ref as S
 from tests.gpu_helpers import adaptive, rel_errors
 from volumential_amd import densities, tree as vtree
-from volumential_amd.kernels import HelmholtzKernel, LaplaceKernel
+from volumential_amd import _capi
+from volumential_amd.kernels import (AxisSourceDerivative, AxisTargetDerivative, HelmholtzKernel,
+                                     LaplaceKernel)
 from volumential_amd.split import HelmholtzSplitCorrection, max_neighbor_distance, series_nmax
 from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
 from volumential_amd.wrangler import NearFieldWrangler
@@ -204,3 +211,80 @@ def test_fused_split_falls_back_beyond_its_limits(cuda):
         # the tile kernels' XCD phase hint may reorder the step sums: not bitwise
         l2, linf = rel_errors(a[0].cpu().numpy(), b[0].cpu().numpy())
         assert l2 <= 1e-14 and linf <= 1e-14, (l2, linf)
+
+
+_DERIV_TABLES = {}
+
+
+def _deriv_table(kind, axis, dim, q):
+    key = (kind, axis, dim, q)
+    if key not in _DERIV_TABLES:
+        cls = AxisTargetDerivative if kind == "target" else AxisSourceDerivative
+        t = NearFieldInteractionTable(q, dim, cls(axis, LaplaceKernel(dim)), source_box_extent=2.0)
+        t.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=61))
+        _DERIV_TABLES[key] = t
+    return _DERIV_TABLES[key]
+
+
+def _deriv_args(kind):
+    if kind == "target":
+        return _capi.SPLIT_DERIV_TARGET, 1
+    return _capi.SPLIT_DERIV_SOURCE, -1
+
+
+@pytest.mark.parametrize("dim,q,level,k,kind,axis,n_rhs", [
+    (2, 4, 3, 10.0, "target", 0, 3), (2, 3, 2, 3.0j, "source", 1, 1),
+    (3, 3, 2, 5.0, "target", 2, 2), (3, 2, 2, 2.0 + 0.5j, "source", 0, 9)])
+def test_derivative_correction_matches_oracle(cuda, dim, q, level, k, kind, axis, n_rhs):
+    print(f"case: derivative correction {kind} axis {axis} {dim}D q={q} L{level} k={k} "
+          f"R={n_rhs}", flush=True)
+    tr = vtree.uniform_tree(dim, q, level)
+    st = _strengths(tr, n_rhs, 7 * dim + q)
+    dk, sgn = _deriv_args(kind)
+    corr = HelmholtzSplitCorrection(tr, k, deriv_kind=dk, deriv_axis=axis)
+    got = corr(torch.as_tensor(st, device="cuda"), tr.target_boxes,
+               tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists).cpu().numpy()
+    ref = S.correction(tr, st, k, corr.nmax, deriv_axis=axis, deriv_sign=sgn)
+    l2, linf = rel_errors(got, ref)
+    assert l2 <= 1e-12 and linf <= 1e-12, (l2, linf)
+
+
+@pytest.mark.parametrize("dim,q,tree_kind,kind,axis,k", [
+    (2, 4, "uniform", "target", 1, 8.0), (2, 5, "adaptive", "source", 0, 6.0 + 0.5j),
+    (3, 3, "uniform", "source", 0, 4.0 + 0.5j), (3, 4, "uniform", "target", 2, 3.0j)])
+def test_derivative_split_matches_oracle(cuda, dim, q, tree_kind, kind, axis, k):
+    """Split list 1 of a derivative-wrapped Helmholtz kernel: the wrapped
+    Laplace table (canonical power 1, signed entries) on Re f and Im f plus
+    the derivative remainder with strengths f x weights -- two launches,
+    the fused FP64 plan and the fused split form against the oracle."""
+    print(f"case: derivative split {kind} axis {axis} {dim}D q={q} {tree_kind} k={k}",
+          flush=True)
+    tr = vtree.uniform_tree(dim, q, 3 if dim == 2 else 2) if tree_kind == "uniform" else adaptive(
+        dim, q, max_level=5)
+    cls = AxisTargetDerivative if kind == "target" else AxisSourceDerivative
+    hk = cls(axis, HelmholtzKernel(dim, k))
+    table = _deriv_table(kind, axis, dim, q)
+    w = NearFieldWrangler(tr, {repr(hk): [table]}, target_kernels=[hk])
+    assert w.helmholtz_split
+    f = densities.gaussian_mixture_batch(tr.points, [21, 22])
+    fc = f * np.exp(0.4j * np.arange(2))[:, None]
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    fd = torch.as_tensor(fc, device="cuda")
+    power = [0] * dim
+    power[axis] = 1
+    meta = orbit_ref.case_metadata(dim, q, (axis,), power)
+    part = ocpu.near_field(tr, table.compact_values(), table.mode_normalizers, meta,
+                           np.concatenate([fc.real, fc.imag]), mode="canonical", scaling_power=1,
+                           table_root_extent=table.source_box_extent)
+    _, sgn = _deriv_args(kind)
+    nmax = S.series_nmax(dim, k, S.max_neighbor_distance(tr))
+    ref = part[:2] + 1j * part[2:] + S.correction(
+        tr, fc * np.asarray(tr.weights)[None, :], k, nmax, deriv_axis=axis, deriv_sign=sgn)
+    two, _ = w.eval_direct_split(*args, fd, form="f64", fused=False)
+    l2, linf = rel_errors(two[0].cpu().numpy(), ref)
+    assert l2 <= 1e-12 and linf <= 1e-12, ("two launches", l2, linf)
+    for form, tol in (("f64", 1e-12), ("split", 1e-10)):
+        fused, _ = w.eval_direct_split(*args, fd, form=form)
+        assert any(key[-1] for key in w._plans), "the fused plan was not used"
+        l2, linf = rel_errors(fused[0].cpu().numpy(), ref)
+        assert l2 <= tol and linf <= tol, (form, l2, linf)

@@ -68,3 +68,39 @@ def test_policy_fallbacks_and_helmholtz_rejection():
         "per_level_tables"
     with pytest.raises(RuntimeError, match="infer_kernel_scaling is unsupported"):
         NearFieldFromCSR(HelmholtzKernel(2, 5.0), shapes, infer_kernel_scaling=True)
+
+
+def test_split_support_with_derivative_wrappers():
+    """``_split_target_kernel_support_status`` / ``_split_base_table_support_status``
+    (expansion_wrangler_fpnd.py:3964-4073) with axis derivative wrappers:
+    the tables must hold the same wrapper around the Laplace kernel; deeper
+    chains are rejected here (no nested derivative tables)."""
+    from volumential_amd import _capi, tree as vtree
+    from volumential_amd.kernels import (AxisSourceDerivative, AxisTargetDerivative,
+                                         HelmholtzKernel, LaplaceKernel)
+    from volumential_amd.table import NearFieldInteractionTable
+    from volumential_amd.wrangler import NearFieldWrangler, split_wrapper_chain
+
+    tr = vtree.uniform_tree(2, 3, 2)
+    helm = HelmholtzKernel(2, 3.0)
+    cases = [
+        (AxisTargetDerivative(1, helm), AxisTargetDerivative(1, LaplaceKernel(2)), True),
+        (AxisSourceDerivative(0, helm), AxisSourceDerivative(0, LaplaceKernel(2)), True),
+        (AxisSourceDerivative(0, helm), AxisTargetDerivative(0, LaplaceKernel(2)), False),
+        (AxisTargetDerivative(0, helm), LaplaceKernel(2), False),
+        (AxisTargetDerivative(0, AxisTargetDerivative(1, helm)), LaplaceKernel(2), False),
+        (AxisTargetDerivative(0, LaplaceKernel(2)), AxisTargetDerivative(0, LaplaceKernel(2)), False),
+    ]
+    for knl, tk, ok in cases:
+        t = NearFieldInteractionTable(3, 2, tk, source_box_extent=2.0)
+        w = NearFieldWrangler(tr, {repr(knl): [t]}, target_kernels=[knl])
+        assert w.helmholtz_split == ok, (knl, tk, w._split_support_status())
+        if ok:
+            assert repr(w._policy_kernel(knl)) == repr(tk)
+            with pytest.raises(RuntimeError, match="not supported"):
+                NearFieldWrangler(tr, {repr(knl): [NearFieldInteractionTable(
+                    3, 2, LaplaceKernel(2), source_box_extent=2.0)]}, target_kernels=[knl],
+                    helmholtz_split=True)
+    assert split_wrapper_chain(helm) == (helm, _capi.SPLIT_DERIV_NONE, 0)
+    assert split_wrapper_chain(AxisTargetDerivative(1, helm)) == (helm, _capi.SPLIT_DERIV_TARGET, 1)
+    assert split_wrapper_chain(AxisSourceDerivative(0, helm)) == (helm, _capi.SPLIT_DERIV_SOURCE, 0)

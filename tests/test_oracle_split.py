@@ -93,3 +93,60 @@ def test_correction_linear_and_explicit():
                 val = S.self_limit(2, k) if r == 0 else S.remainder(2, k, nmax, np.array([r]))[0]
                 exp[t] += val * a[sb * n + s]
     assert np.allclose(ca[tb * n:(tb + 1) * n], exp, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("k", [1.0, 5.0, 2.0j])
+def test_derivative_series_matches_closed_form(dim, k):
+    """R'(r)/r of the series against the library closed form and against a
+    central difference of the (already pinned) series itself."""
+    r_max = 4.0 / abs(k)
+    nmax = S.series_nmax(dim, k, r_max)
+    # from 0.05 r_max on: below it the 3D closed form cancels (O(r^2) over r^3)
+    r = np.linspace(0.05 * r_max, r_max, 61)
+    ser = S.remainder_radial_derivative(dim, k, nmax, r)
+    if dim == 2 and complex(k).real == 0:
+        from scipy.special import k1
+
+        lam = complex(k).imag
+        # G = K0(lam r)/(2 pi): G' = -lam K1(lam r)/(2 pi); G_0' = -1/(2 pi r)
+        cf = (-lam * k1(lam * r) / (2 * np.pi) + 1.0 / (2 * np.pi * r)) / r
+    else:
+        cf = S.closed_form_radial_derivative(dim, k, r)
+    err = np.max(np.abs(ser - cf)) / max(1.0, np.max(np.abs(cf)))
+    assert err <= 1e-12, err
+    h = 1e-5
+    fd = (S.remainder(dim, k, nmax, r + h) - S.remainder(dim, k, nmax, r - h)) / (2 * h) / r
+    err = np.max(np.abs(ser - fd)) / max(1.0, np.max(np.abs(ser)))
+    assert err <= 1e-7, err
+    assert S.remainder_radial_derivative(dim, k, nmax, np.array([0.0]))[0] == 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_derivative_correction_explicit(dim):
+    """The derivative-wrapped correction is the explicit double sum of
+    (x_a - y_a) R'(r)/r strength (self pairs 0); a source derivative is its
+    negative."""
+    tr = vtree.uniform_tree(dim, 2, 1)
+    k = 3.0
+    nmax = S.series_nmax(dim, k, S.max_neighbor_distance(tr))
+    rng = np.random.default_rng(4)
+    st = rng.standard_normal(tr.n_points) + 1j * rng.standard_normal(tr.n_points)
+    axis = dim - 1
+    got = S.correction(tr, st, k, nmax, deriv_axis=axis)[0]
+    src = S.correction(tr, st, k, nmax, deriv_axis=axis, deriv_sign=-1)[0]
+    assert np.array_equal(src, -got)
+    pts = np.asarray(tr.points)
+    bs, bc = tr.box_target_starts, tr.box_target_counts_nonchild
+    want = np.zeros(tr.n_points, dtype=complex)
+    for i, tb in enumerate(tr.target_boxes):
+        for sb in tr.neighbor_source_boxes_lists[tr.neighbor_source_boxes_starts[i]:
+                                                 tr.neighbor_source_boxes_starts[i + 1]]:
+            for t in range(bs[tb], bs[tb] + bc[tb]):
+                for s in range(bs[sb], bs[sb] + bc[sb]):
+                    if s == t:
+                        continue
+                    d = pts[:, t] - pts[:, s]
+                    r = np.linalg.norm(d)
+                    want[t] += d[axis] * S.closed_form_radial_derivative(dim, k, r) * st[s]
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))

@@ -30,6 +30,10 @@ KERNEL_HELMHOLTZ = 2
 KERNEL_LAPLACE_TARGET_DERIV = 3
 KERNEL_LAPLACE_SOURCE_DERIV = 4
 
+SPLIT_DERIV_NONE = 0
+SPLIT_DERIV_TARGET = 1
+SPLIT_DERIV_SOURCE = 2
+
 SCALING_CANONICAL = 0
 SCALING_FIXED = 1
 SCALING_PER_LEVEL = 2
@@ -154,6 +158,8 @@ class HelmholtzSplitBlocks(C.Structure):
         ("unit_nodes", vp),
         ("unit_weights", vp),
         ("case_vecs", vp),
+        ("deriv_kind", C.c_int),
+        ("deriv_axis", C.c_int),
     ]
 
 
@@ -207,6 +213,8 @@ class SplitArgs(C.Structure):
         ("ld_strengths", i64),
         ("result", vp),
         ("ld_result", i64),
+        ("deriv_kind", C.c_int),
+        ("deriv_axis", C.c_int),
     ]
 
 
@@ -247,7 +255,7 @@ def lib():
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 3:
+    if L.vl1_abi_version() != 4:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

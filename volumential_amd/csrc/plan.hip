@@ -99,8 +99,11 @@ __global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int sp_mas
 // 917-946), R the series of split_series.h.  This is, block by block, the
 // sum the point-to-point correction forms for every list-1 pair of that
 // case and level (expansion_wrangler_fpnd.py:3491-3836) added to the table
-// part (3457-3489), so one contraction evaluates both.
-__global__ void expand_split_blocks_kernel(int dim, int q, int n, int MP, int KP, int n_blocks,
+// part (3457-3489), so one contraction evaluates both.  With an axis
+// derivative wrapper (deriv_axis >= 0, 3884-3957) T is the wrapped Laplace
+// table (signed entries, sp_mask) and R(|d|) becomes deriv_sign d_a R'(r)/r.
+__global__ void expand_split_blocks_kernel(int dim, int q, int n, int sp_mask, int deriv_axis,
+                                           double deriv_sign, int MP, int KP, int n_blocks,
                                            const int32_t* __restrict__ blk_case,
                                            const double* __restrict__ blk_alpha,
                                            const double* __restrict__ blk_disp,
@@ -129,14 +132,16 @@ __global__ void expand_split_blocks_kernel(int dim, int q, int n, int MP, int KP
       const int ss = s % n, tt = t % n;
       const int64_t rel =
           compact_rel_row(dim, q, perm + c * dim, sign + c * dim, group + c * dim, ss, tt);
-      double re = blk_alpha[b] * table[offs[ranks[c]] + rel] + blk_disp[b] * nmlz[ss];
+      double tv = table[offs[ranks[c]] + rel];
+      if (entry_sign(dim, q, perm + c * dim, sign + c * dim, sp_mask, ss, tt) < 0) tv = -tv;
+      double re = blk_alpha[b] * tv + blk_disp[b] * nmlz[ss];
       const double h = blk_h[b];
       const int32_t* vec = case_vecs + c * dim;
       int m = 0;
       for (int a = 0; a < dim; ++a) m = max(m, abs((int)vec[a]));
       const double size = m == 0 ? 1.0 : 0.5 * (m - 2);
       int sr = ss, tr = tt;
-      double r2 = 0.0, w = 1.0;
+      double r2 = 0.0, w = 1.0, da = 0.0;
       for (int a = dim - 1; a >= 0; --a) {
         const int sa = sr % q, ta = tr % q;
         sr /= q;
@@ -144,12 +149,22 @@ __global__ void expand_split_blocks_kernel(int dim, int q, int n, int MP, int KP
         const double d = 0.25 * vec[a] * h + size * h * (unodes[ta] - 0.5) - h * (unodes[sa] - 0.5);
         r2 = fma(d, d, r2);
         w *= uweights[sa] * h;
+        if (a == deriv_axis) da = d;
       }
-      double rr, ri;
-      if (dim == 2)
-        remainder_value<2>(r2, coef, N, rr, ri);
-      else
-        remainder_value<3>(r2, coef, N, rr, ri);
+      double rr = 0.0, ri = 0.0;
+      if (deriv_axis < 0) {
+        if (dim == 2)
+          remainder_value<2>(r2, coef, N, rr, ri);
+        else
+          remainder_value<3>(r2, coef, N, rr, ri);
+      } else if (r2 > 0.0) {  // coincident nodes (r = 0) give 0
+        if (dim == 2)
+          remainder_radial_derivative<2>(r2, coef, N, rr, ri);
+        else
+          remainder_radial_derivative<3>(r2, coef, N, rr, ri);
+        rr *= deriv_sign * da;
+        ri *= deriv_sign * da;
+      }
       re = fma(w, rr, re);
       const double im = w * ri;
       const bool tre = t < n, sre = s < n;
@@ -1043,15 +1058,15 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
     return fail(VL1_E_ARG, "vl1_plan_create: bad scaling mode");
   hipStream_t s = (hipStream_t)stream;
   const int n = pay->n, dim = pay->dim;
-  auto pl = new vl1_plan();
-  pl->payload = pay;
-  pl->dim = dim;
-  pl->n = n;
   // Helmholtz split: complex blocks over a real Laplace payload, the
   // canonical scaling and the 2D log term folded into per-(case, level) blocks
   const bool split = sp != nullptr;
   if (split) {
-    if (pay->is_complex || pay->sign_mask)
+    if (sp->deriv_kind < VL1_SPLIT_DERIV_NONE || sp->deriv_kind > VL1_SPLIT_DERIV_SOURCE ||
+        (sp->deriv_kind != VL1_SPLIT_DERIV_NONE && (sp->deriv_axis < 0 || sp->deriv_axis >= dim)))
+      return fail(VL1_E_ARG, "Helmholtz split plans: bad derivative wrapper");
+    // a derivative wrapper takes the wrapped (signed) Laplace table
+    if (pay->is_complex || (pay->sign_mask && sp->deriv_kind == VL1_SPLIT_DERIV_NONE))
       return fail(VL1_E_ARG, "Helmholtz split plans take a real scalar (Laplace) table payload");
     if (d->scaling_mode != VL1_SCALING_CANONICAL)
       return fail(VL1_E_ARG, "Helmholtz split plans use the canonical single-table scaling");
@@ -1060,6 +1075,10 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
     if (!sp->unit_nodes || !sp->unit_weights || !sp->case_vecs)
       return fail(VL1_E_ARG, "Helmholtz split plans: null host array");
   }
+  auto pl = new vl1_plan();
+  pl->payload = pay;
+  pl->dim = dim;
+  pl->n = n;
   pl->cplx = (pay->is_complex || split) ? 1 : 0;
   pl->rows = pl->cplx ? 2 * n : n;
   pl->MP = round_up(pl->rows, 32);
@@ -1234,8 +1253,11 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
     if (e == hipSuccess && split) {
       std::vector<int32_t> vecs((size_t)pay->n_cases * dim);
       for (size_t k = 0; k < vecs.size(); ++k) vecs[k] = sp->case_vecs[k];
-      const std::vector<double> coef =
-          split_coefficients(dim, std::complex<double>(sp->k_re, sp->k_im), sp->series_nmax);
+      const bool deriv = sp->deriv_kind != VL1_SPLIT_DERIV_NONE;
+      const std::complex<double> kk(sp->k_re, sp->k_im);
+      std::vector<double> coef = deriv ? split_derivative_coefficients(dim, kk, sp->series_nmax)
+                                       : split_coefficients(dim, kk, sp->series_nmax);
+      if (coef.empty()) coef.assign(2, 0.0);  // 3D derivative with one term: R' = 0
       DevBuf dal, ddi, dh, dvec, dun, duw, dco;
       e = upload(dal, pl->blk_alpha.data(), pl->blk_alpha.size() * 8, s);
       if (e == hipSuccess) e = upload(ddi, pl->blk_disp.data(), pl->blk_disp.size() * 8, s);
@@ -1248,7 +1270,8 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
         const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
         const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
         hipLaunchKernelGGL(expand_split_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n,
-                           pl->MP, pl->KP, pl->n_blocks, dcase.as<int32_t>(), dal.as<double>(),
+                           pay->sign_mask, deriv ? sp->deriv_axis : -1,
+                           sp->deriv_kind == VL1_SPLIT_DERIV_SOURCE ? -1.0 : 1.0, pl->MP, pl->KP, pl->n_blocks, dcase.as<int32_t>(), dal.as<double>(),
                            ddi.as<double>(), dh.as<double>(), dvec.as<int32_t>(),
                            dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
                            pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(),

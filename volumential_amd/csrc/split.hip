@@ -8,7 +8,10 @@
 // strength = density x quadrature weight.  R is the reference's series
 // (_HelmholtzSplitSeriesRemainderKernel, 212-312; split_series.h), by Horner.
 // At r = 0 it gives the self-pair limit of 5293-5317, so self pairs need no
-// special case.  N is the caller's series length (4337-4389).
+// special case.  N is the caller's series length (4337-4389).  For an axis
+// target (source) derivative wrapper of the output kernel (3884-3957) the
+// pair value is +(-)(x_a - y_a) R'(r) / r, and self pairs give 0 (the
+// reference's limit for a wrapped chain, 5293-5300).
 //
 // GPU mapping: one workgroup per target box.  Each listed source box's
 // point coordinates and complex strengths (a chunk of RC densities) are
@@ -30,8 +33,10 @@ namespace vl1 {
 constexpr int kSplitBlock = 256;
 
 struct SplitParams {
-  int dim, n_rhs, nt, n_terms, chunk;
-  const double* coef;  // 2D: c0, then (a_n, b_n) n = 1..N; 3D: c_n n = 1..N; complex
+  int dim, n_rhs, nt, n_terms, chunk, ncoef;
+  int deriv_axis;      // derivative wrapper (DERIV kernels): axis and sign
+  double deriv_sign;   // +1 target derivative, -1 source derivative
+  const double* coef;  // split_coefficients / split_derivative_coefficients
   const int32_t* target_boxes;
   const int32_t* nbr_starts;
   const int32_t* nbr_lists;
@@ -45,11 +50,11 @@ struct SplitParams {
   int64_t ldr;
 };
 
-template <int DIM, int RC>
+template <int DIM, int RC, bool DERIV>
 __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const SplitParams p) {
   extern __shared__ double smem[];
   const int N = p.n_terms;
-  const int ncoef = DIM == 2 ? 2 + 4 * N : 2 * N;
+  const int ncoef = p.ncoef;
   const int P = kSplitBlock / p.nt;
   double* sC = smem;                 // series coefficients
   double* sX = sC + ncoef;           // source coordinates [DIM][chunk]
@@ -98,14 +103,26 @@ __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const Spl
           __syncthreads();
           if (tact) {
             for (int j = part; j < nc; j += P) {
-              double r2 = 0.0;
+              double r2 = 0.0, da = 0.0;
 #pragma unroll
               for (int a = 0; a < DIM; ++a) {
                 const double d = xt[a] - sX[a * chunk + j];
                 r2 = fma(d, d, r2);
+                if (DERIV && a == p.deriv_axis) da = d;
               }
               double vr, vi;
-              remainder_value<DIM>(r2, sC, N, vr, vi);
+              if constexpr (DERIV) {
+                // +-(x_a - y_a) R'(r) / r; the self pair (r = 0) gives 0
+                vr = vi = 0.0;
+                if (r2 > 0.0) {
+                  remainder_radial_derivative<DIM>(r2, sC, N, vr, vi);
+                  const double f = p.deriv_sign * da;
+                  vr *= f;
+                  vi *= f;
+                }
+              } else {
+                remainder_value<DIM>(r2, sC, N, vr, vi);
+              }
               const double* f = sF + (size_t)j * RC * 2;
 #pragma unroll
               for (int r = 0; r < RC; ++r) {
@@ -143,9 +160,9 @@ __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const Spl
   }
 }
 
-template <int DIM, int RC>
+template <int DIM, int RC, bool DERIV>
 static hipError_t launch_split(const SplitParams& prm, int grid, size_t smem, hipStream_t s) {
-  auto fn = split_correction_kernel<DIM, RC>;
+  auto fn = split_correction_kernel<DIM, RC, DERIV>;
   hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)smem);
   if (e != hipSuccess) return e;
@@ -173,9 +190,16 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
     return fail(VL1_E_ARG, "vl1_split_correction: null device array");
   if (a->n_rhs > 1 && (a->ld_strengths < 2 * a->n_points || a->ld_result < 2 * a->n_points))
     return fail(VL1_E_ARG, "vl1_split_correction: bad leading dimension");
+  if (a->deriv_kind < VL1_SPLIT_DERIV_NONE || a->deriv_kind > VL1_SPLIT_DERIV_SOURCE)
+    return fail(VL1_E_ARG, "vl1_split_correction: bad deriv_kind");
+  const bool deriv = a->deriv_kind != VL1_SPLIT_DERIV_NONE;
+  if (deriv && (a->deriv_axis < 0 || a->deriv_axis >= a->dim))
+    return fail(VL1_E_ARG, "vl1_split_correction: deriv_axis out of range");
   hipStream_t s = (hipStream_t)stream;
   const std::complex<double> k(a->k_re, a->k_im);
-  std::vector<double> coef = split_coefficients(a->dim, k, a->series_nmax);
+  std::vector<double> coef = deriv ? split_derivative_coefficients(a->dim, k, a->series_nmax)
+                                   : split_coefficients(a->dim, k, a->series_nmax);
+  if (coef.empty()) coef.assign(2, 0.0);  // 3D derivative with one term: R' = 0
   DevBuf dcoef;
   hipError_t e = upload(dcoef, coef.data(), coef.size() * sizeof(double), s);
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_split_correction: ") + hipGetErrorString(e));
@@ -184,6 +208,9 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   prm.dim = a->dim;
   prm.n_rhs = a->n_rhs;
   prm.n_terms = a->series_nmax;
+  prm.ncoef = (int)coef.size();
+  prm.deriv_axis = deriv ? a->deriv_axis : -1;
+  prm.deriv_sign = a->deriv_kind == VL1_SPLIT_DERIV_SOURCE ? -1.0 : 1.0;
   prm.coef = dcoef.as<double>();
   prm.target_boxes = a->target_boxes;
   prm.nbr_starts = a->nbr_starts;
@@ -212,13 +239,19 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   if (smem > 160 * 1024) return fail(VL1_E_UNSUPPORTED, "vl1_split_correction: LDS budget exceeded");
   const int grid = a->n_tgt_boxes;
   if (a->dim == 2) {
-    e = RC == 1   ? launch_split<2, 1>(prm, grid, smem, s)
-        : RC == 4 ? launch_split<2, 4>(prm, grid, smem, s)
-                  : launch_split<2, 8>(prm, grid, smem, s);
+    e = deriv ? (RC == 1   ? launch_split<2, 1, true>(prm, grid, smem, s)
+                 : RC == 4 ? launch_split<2, 4, true>(prm, grid, smem, s)
+                           : launch_split<2, 8, true>(prm, grid, smem, s))
+              : (RC == 1   ? launch_split<2, 1, false>(prm, grid, smem, s)
+                 : RC == 4 ? launch_split<2, 4, false>(prm, grid, smem, s)
+                           : launch_split<2, 8, false>(prm, grid, smem, s));
   } else {
-    e = RC == 1   ? launch_split<3, 1>(prm, grid, smem, s)
-        : RC == 4 ? launch_split<3, 4>(prm, grid, smem, s)
-                  : launch_split<3, 8>(prm, grid, smem, s);
+    e = deriv ? (RC == 1   ? launch_split<3, 1, true>(prm, grid, smem, s)
+                 : RC == 4 ? launch_split<3, 4, true>(prm, grid, smem, s)
+                           : launch_split<3, 8, true>(prm, grid, smem, s))
+              : (RC == 1   ? launch_split<3, 1, false>(prm, grid, smem, s)
+                 : RC == 4 ? launch_split<3, 4, false>(prm, grid, smem, s)
+                           : launch_split<3, 8, false>(prm, grid, smem, s));
   }
   // the coefficient buffer is freed when dcoef leaves scope: wait for the launch
   if (e == hipSuccess) e = hipStreamSynchronize(s);

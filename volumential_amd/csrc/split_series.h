@@ -57,6 +57,47 @@ __device__ inline void remainder_value(double r2, const double* __restrict__ cf,
   }
 }
 
+// R'(r) / r of the same series, for axis target / source derivative
+// wrappers (the pair value is +-(x_a - y_a) R'(r) / r; the reference runs
+// its P2P on the wrapped series kernel, 3884-3957):
+//   2D  sum_{n=1..N} u^(n-1) (B'_n + lg A'_n),  lg = -log(r) / (2 pi),
+//       A'_n = 2n s_n,  B'_n = 2n b_n - s_n / (2 pi)   (u = r^2);
+//   3D  (1/r) sum_{m=0..N-2} (m + 1) c_{m+2} r^m.
+// Coefficient layout (interleaved complex): 2D (A'_n, B'_n) n = 1..N; 3D
+// the N - 1 values (m + 1) c_{m+2}.  Callers treat r = 0 (self pairs,
+// value 0) themselves.
+template <int DIM>
+__device__ inline void remainder_radial_derivative(double r2, const double* __restrict__ cf, int N,
+                                                   double& gr, double& gi) {
+  if (DIM == 2) {
+    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+    for (int n = N; n >= 1; --n) {
+      const double* c = cf + 4 * (n - 1);
+      const double nar = fma(ar, r2, c[0]), nai = fma(ai, r2, c[1]);
+      const double nbr = fma(br, r2, c[2]), nbi = fma(bi, r2, c[3]);
+      ar = nar;
+      ai = nai;
+      br = nbr;
+      bi = nbi;
+    }
+    const double lg = -0.25 / M_PI * log(r2);
+    gr = fma(lg, ar, br);
+    gi = fma(lg, ai, bi);
+  } else {
+    const double r = sqrt(r2);
+    double pr = 0.0, pi = 0.0;
+    for (int m = N - 2; m >= 0; --m) {
+      const double* c = cf + 2 * m;
+      const double npr = fma(pr, r, c[0]), npi = fma(pi, r, c[1]);
+      pr = npr;
+      pi = npi;
+    }
+    const double ir = 1.0 / r;
+    gr = pr * ir;
+    gi = pi * ir;
+  }
+}
+
 // Series coefficients (interleaved complex), the reference's formulas.
 inline std::vector<double> split_coefficients(int dim, std::complex<double> k, int N) {
   using cd = std::complex<double>;
@@ -89,6 +130,27 @@ inline std::vector<double> split_coefficients(int dim, std::complex<double> k, i
       const cd c = p / (4.0 * M_PI * fact);
       out.push_back(c.real());
       out.push_back(c.imag());
+    }
+  }
+  return out;
+}
+
+// Coefficients of remainder_radial_derivative, from those above.
+inline std::vector<double> split_derivative_coefficients(int dim, std::complex<double> k, int N) {
+  const std::vector<double> c = split_coefficients(dim, k, N);
+  std::vector<double> out;
+  if (dim == 2) {
+    for (int n = 1; n <= N; ++n) {
+      const double* sb = c.data() + 2 + 4 * (n - 1);  // s_n, b_n
+      out.push_back(2.0 * n * sb[0]);
+      out.push_back(2.0 * n * sb[1]);
+      out.push_back(2.0 * n * sb[2] - sb[0] / (2.0 * M_PI));
+      out.push_back(2.0 * n * sb[3] - sb[1] / (2.0 * M_PI));
+    }
+  } else {
+    for (int m = 0; m + 2 <= N; ++m) {
+      out.push_back((m + 1) * c[2 * (m + 1)]);
+      out.push_back((m + 1) * c[2 * (m + 1) + 1]);
     }
   }
   return out;

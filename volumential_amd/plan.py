@@ -31,7 +31,10 @@ class List1Plan:
         """``helmholtz_split``: ``{"k": k, "series_nmax": N}`` folds the
         Helmholtz split correction into the blocks of a real Laplace payload
         (``vl1_plan_create_helmholtz_split``); the plan then evaluates complex
-        densities to the split's full list-1 value."""
+        densities to the split's full list-1 value.  Optional keys
+        ``deriv_kind`` (``_capi.SPLIT_DERIV_*``) and ``deriv_axis`` describe
+        an axis derivative wrapper of the output kernel (the payload then
+        holds the wrapped Laplace table)."""
         L = _capi.lib()
         tb = np.ascontiguousarray(tree.target_boxes if target_boxes is None else target_boxes,
                                   dtype=np.int32)
@@ -93,6 +96,8 @@ class List1Plan:
                 unit_nodes=k["split_nodes"].ctypes.data,
                 unit_weights=k["split_weights"].ctypes.data,
                 case_vecs=k["case_vecs"].ctypes.data,
+                deriv_kind=int(helmholtz_split.get("deriv_kind", _capi.SPLIT_DERIV_NONE)),
+                deriv_axis=int(helmholtz_split.get("deriv_axis", 0)),
             )
             _capi.check(
                 L.vl1_plan_create_helmholtz_split(_capi.C.byref(handle), payload.handle, desc, sp,

@@ -74,14 +74,26 @@ class HelmholtzSplitCorrection:
 
     ``k`` is the Helmholtz wave number (complex allowed; Yukawa ``lambda``
     is ``k = 1j * lambda``).  Points, box starts and counts come from the
-    tree (sources = targets, tree order)."""
+    tree (sources = targets, tree order).  ``deriv_kind`` /
+    ``deriv_axis``: an axis target (``_capi.SPLIT_DERIV_TARGET``) or source
+    (``SPLIT_DERIV_SOURCE``) derivative wrapper of the output kernel
+    (``expansion_wrangler_fpnd.py:3884-3957``): the pair value becomes
+    ``+-(x_a - y_a) R'(r)/r`` and self pairs give 0 (5293-5300)."""
 
-    def __init__(self, tree, k, split_order=1, series_terms=None):
+    def __init__(self, tree, k, split_order=1, series_terms=None,
+                 deriv_kind=_capi.SPLIT_DERIV_NONE, deriv_axis=0):
         if int(split_order) != 1:
             raise NotImplementedError("only helmholtz_split_order=1 is implemented")
         self.dim = int(tree.dim)
         self.k = complex(k)
         self.split_order = 1
+        self.deriv_kind = int(deriv_kind)
+        self.deriv_axis = int(deriv_axis)
+        if self.deriv_kind not in (_capi.SPLIT_DERIV_NONE, _capi.SPLIT_DERIV_TARGET,
+                                   _capi.SPLIT_DERIV_SOURCE):
+            raise ValueError("bad deriv_kind")
+        if self.deriv_kind and not 0 <= self.deriv_axis < self.dim:
+            raise ValueError("deriv_axis out of range")
         self.nmax = int(series_terms) if series_terms is not None else series_nmax(
             self.dim, self.k, max_neighbor_distance(tree))
         self.n_points = int(np.asarray(tree.points).shape[1])
@@ -139,6 +151,7 @@ class HelmholtzSplitCorrection:
             box_starts=self._starts.data_ptr(), box_counts=self._counts.data_ptr(),
             points=self._points.data_ptr(), strengths=sr.data_ptr(),
             ld_strengths=2 * self.n_points, result=yr.data_ptr(), ld_result=2 * self.n_points,
+            deriv_kind=self.deriv_kind, deriv_axis=self.deriv_axis,
         )
         _capi.check(_capi.lib().vl1_split_correction(args, _capi.stream_handle(stream)),
                     "vl1_split_correction")

@@ -153,12 +153,17 @@ class NearFieldWrangler:
     def _split_support_status(self):
         """``_split_target_kernel_support_status`` /
         ``_split_base_table_support_status`` (4009-4073, 3964-4007) for the
-        kernels of this build: Helmholtz output kernels (no derivative
-        wrappers) whose tables all hold ``LaplaceKernel(dim)``."""
+        kernels of this build: Helmholtz output kernels, bare or under one
+        axis target / source derivative wrapper, whose tables all hold the
+        same wrapper around ``LaplaceKernel(dim)``."""
         for knl in self.target_kernels:
-            if not isinstance(knl, HelmholtzKernel):
+            try:
+                base, _, _ = split_wrapper_chain(knl)
+            except NotImplementedError as exc:
+                return False, str(exc)
+            if not isinstance(base, HelmholtzKernel):
                 return False, f"{knl!r} is not a Helmholtz kernel"
-            want = repr(LaplaceKernel(knl.dim))
+            want = repr(_rewrap(knl, LaplaceKernel(knl.dim)))
             for lev, t in enumerate(self.near_field_table.get(repr(knl), [])):
                 if repr(t.integral_knl) != want:
                     return False, (f"{knl!r} level {lev} kernel mismatch: expected {want}, "
@@ -384,7 +389,8 @@ class NearFieldWrangler:
         split_desc = None
         if fused_split:
             corr = self._correction(out_kernel)
-            split_desc = {"k": corr.k, "series_nmax": corr.nmax}
+            split_desc = {"k": corr.k, "series_nmax": corr.nmax,
+                          "deriv_kind": corr.deriv_kind, "deriv_axis": corr.deriv_axis}
         plan = List1Plan(
             self.tree, payload,
             scaling_mode=nf._mode_code(pol),
@@ -434,8 +440,10 @@ class NearFieldWrangler:
         if key not in self._corrections:
             from .split import HelmholtzSplitCorrection
 
+            base, kind, axis = split_wrapper_chain(knl)
             self._corrections[key] = HelmholtzSplitCorrection(
-                self.tree, knl.k, split_order=self.helmholtz_split_order)
+                self.tree, base.k, split_order=self.helmholtz_split_order,
+                deriv_kind=kind, deriv_axis=axis)
         return self._corrections[key]
 
     def eval_direct_helmholtz_split_correction(self, target_boxes, neighbor_source_boxes_starts,
@@ -508,6 +516,34 @@ class NearFieldWrangler:
             if one:
                 pot[i] = pot[i][0]
         return self._finish(pot), TimingFuture()
+
+
+def split_wrapper_chain(knl):
+    """``(base kernel, deriv_kind, deriv_axis)`` of a split output kernel
+    (``_extract_helmholtz_split_kernel_wrapper_chain``, 3884-3910): the bare
+    kernel, or one axis target / source derivative around it.  Deeper
+    chains raise ``NotImplementedError`` (their wrapped Laplace tables are
+    nested derivatives, which this build does not build)."""
+    from .kernels import AxisSourceDerivative, AxisTargetDerivative
+
+    if isinstance(knl, (AxisTargetDerivative, AxisSourceDerivative)):
+        inner = knl.inner_kernel
+        if isinstance(inner, (AxisTargetDerivative, AxisSourceDerivative)):
+            raise NotImplementedError(
+                "helmholtz_split here supports at most one axis derivative wrapper")
+        kind = (_capi.SPLIT_DERIV_TARGET if isinstance(knl, AxisTargetDerivative)
+                else _capi.SPLIT_DERIV_SOURCE)
+        return inner, kind, int(knl.axis)
+    return knl, _capi.SPLIT_DERIV_NONE, 0
+
+
+def _rewrap(knl, base):
+    """``base`` under the derivative wrapper of ``knl`` (3932-3950)."""
+    from .kernels import AxisSourceDerivative, AxisTargetDerivative
+
+    if isinstance(knl, (AxisTargetDerivative, AxisSourceDerivative)):
+        return type(knl)(knl.axis, base)
+    return base
 
 
 def _host(a):

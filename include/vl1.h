@@ -60,7 +60,12 @@ typedef enum vl1_kernel_kind {
    * AxisSourceDerivative(axis, LaplaceKernel(dim)), table build only):
    * target dG/dd_a = 2D -d_a/(2 pi |d|^2), 3D -d_a/(4 pi |d|^3); source = minus */
   VL1_KERNEL_LAPLACE_TARGET_DERIV = 3,
-  VL1_KERNEL_LAPLACE_SOURCE_DERIV = 4
+  VL1_KERNEL_LAPLACE_SOURCE_DERIV = 4,
+  /* one axis derivative of Helmholtz (complex, table build only): target
+   * dG/dd_a = G'(|d|) d_a/|d|, G' = 2D -(ik/4) H1^(1)(k|d|), 3D
+   * e^{ik|d|}(ik|d| - 1)/(4 pi |d|^2); source = minus */
+  VL1_KERNEL_HELMHOLTZ_TARGET_DERIV = 5,
+  VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV = 6
 } vl1_kernel_kind;
 
 /* list1.py:361-437 (get_kernel_scaling_policy) */

@@ -130,6 +130,31 @@ def helmholtz2d(k):
     return g
 
 
+def helmholtz_target_derivative(dim, axis, k):
+    """``AxisTargetDerivative(axis, HelmholtzKernel(dim))``: ``G'(r) d_a / r``
+    with ``G'`` = 2D ``-(ik/4) H1^(1)(kr)`` (scipy), 3D
+    ``e^{ikr}(ikr - 1) / (4 pi r^2)`` -- the derivative of ``helmholtz2d`` /
+    ``helmholtz3d``, pinned by central differences of those in the tests."""
+    if dim == 2:
+        from scipy.special import hankel1
+
+        def g(*d):
+            r = np.sqrt(sum(c * c for c in d))
+            return -0.25j * k * hankel1(1, k * r) * d[axis] / r
+    else:
+        def g(*d):
+            r = np.sqrt(sum(c * c for c in d))
+            return np.exp(1j * k * r) * (1j * k * r - 1.0) / (4 * np.pi * r ** 3) * d[axis]
+
+    return g
+
+
+def helmholtz_source_derivative(dim, axis, k):
+    """``AxisSourceDerivative(axis, HelmholtzKernel(dim))``: ``-dG/dd_axis``."""
+    g = helmholtz_target_derivative(dim, axis, k)
+    return lambda *d: -g(*d)
+
+
 # ---------------------------------------------------------- quadratures
 def box_quad_duffy_nd(func, bounds, singular_point, deg_regular, radial_order):
     """``singular_integral_2d.py:856-949`` (tanh-sinh-fast), vectorised."""

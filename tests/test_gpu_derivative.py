@@ -14,7 +14,13 @@ reference builds and evaluates through its signed arithmetic orbit
   (<= 1e-10) vs the C oracle with the entry signs, uniform and adaptive
   trees, canonical scaling (power 1);
 * physics: density 1 on a uniform tree -- every target's list-1 value equals
-  the summed gradients of its neighbours' closed-form box potentials.
+  the summed gradients of its neighbours' closed-form box potentials;
+* Helmholtz gradients (``AxisTargetDerivative(a, HelmholtzKernel(d))``, complex
+  signed tables): sampled entries vs the oracle's quadrature of the
+  analytic derivative kernel (1e-12); per-level evaluation through the
+  plan (FP64 / split) and the plain kernel vs the oracle; and the
+  Helmholtz split of the same gradient (wrapped Laplace table + derivative
+  remainder) converging to the Helmholtz-gradient tables as q grows.
 """
 
 import numpy as np
@@ -23,10 +29,11 @@ import torch
 
 from oracle import cpu as ocpu
 from oracle import duffy_ref as D
-from oracle import orbit_ref
+from oracle import list1_ref, orbit_ref
 from tests.gpu_helpers import adaptive, rel_errors
 from volumential_amd import densities, tree as vtree
-from volumential_amd.kernels import AxisSourceDerivative, AxisTargetDerivative, LaplaceKernel
+from volumential_amd.kernels import (AxisSourceDerivative, AxisTargetDerivative, HelmholtzKernel,
+                                     LaplaceKernel)
 from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
 from volumential_amd.wrangler import NearFieldWrangler
 
@@ -158,3 +165,105 @@ def test_unit_density_gives_box_potential_gradients(cuda, axis):
                 exp[k] += D.box_potential_gradient(tr.points[:, k], c - h / 2, c + h / 2, axis)
     err = np.max(np.abs(got - exp)) / np.max(np.abs(exp))
     assert err <= 1e-9, err
+
+
+def _helm_table(kind, axis, dim, q, k, extent, level=None, orders=(20, 61)):
+    key = ("helm", kind, axis, dim, q, k, extent, level, orders)
+    if key not in _TABLES:
+        cls = AxisTargetDerivative if kind == "target" else AxisSourceDerivative
+        t = NearFieldInteractionTable(q, dim, cls(axis, HelmholtzKernel(dim, k)),
+                                      source_box_extent=extent, source_box_level=level)
+        t.build_table(DuffyBuildConfig(regular_quad_order=orders[0], radial_quad_order=orders[1]))
+        _TABLES[key] = t
+    return _TABLES[key]
+
+
+@pytest.mark.parametrize("kind,axis,dim,q,k", [("target", 0, 2, 4, 5.0), ("source", 1, 2, 3, 2.0),
+                                               ("target", 2, 3, 3, 4.0), ("source", 0, 3, 2, 3.0)])
+def test_helmholtz_derivative_entries_match_oracle(cuda, kind, axis, dim, q, k):
+    print(f"case: Helmholtz {kind} derivative axis {axis}, {dim}D q={q} k={k} entries",
+          flush=True)
+    reg, rad = 12, 41
+    t = _helm_table(kind, axis, dim, q, k, 1.0, orders=(reg, rad))
+    ids, vals = t.get_reduced_table_data()
+    assert np.iscomplexobj(vals)
+    assert t.arithmetic_metadata().axis_sign_power[axis] == 1
+    g = D.TableGeometry(dim, q, H=1.0)
+    rng = np.random.default_rng(dim * 13 + q + axis)
+    pick = np.unique(np.concatenate([[0, ids.size - 1], rng.integers(0, ids.size, 16)]))
+    knl = (D.helmholtz_target_derivative if kind == "target" else D.helmholtz_source_derivative)(
+        dim, axis, k)
+    for i in pick:
+        ref = g.entry(int(ids[i]), knl, reg, rad)
+        assert abs(vals[i] - ref) / max(1.0, abs(ref)) <= 1e-12, (ids[i], vals[i], ref)
+
+
+@pytest.mark.parametrize("route,tol", [("plan", 1e-12), ("plain", 1e-12), ("split", 1e-10)])
+def test_helmholtz_derivative_eval_per_level(cuda, route, tol):
+    """Per-level complex signed tables on an adaptive 2D tree (levels 1..5)."""
+    print(f"case: Helmholtz gradient per-level tables, 2D q=3 adaptive, {route}", flush=True)
+    tr = adaptive(2, 3, max_level=5)
+    k = 6.0
+    tables = [_helm_table("target", 1, 2, 3, k, tr.root_extent / 2**lev, lev, (10, 31))
+              for lev in range(1, tr.nlevels)]
+    hk = AxisTargetDerivative(1, HelmholtzKernel(2, k))
+    f = densities.gaussian_mixture_batch(tr.points, [51, 52])
+    fc = f * np.exp(0.7j * np.arange(2))[:, None]
+    w = NearFieldWrangler(tr, {repr(hk): tables}, target_kernels=[hk],
+                          direct_method="plain" if route == "plain" else "plan")
+    assert not w.helmholtz_split
+    c = torch.as_tensor(fc, device="cuda")
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists, c)
+    if route == "split":
+        pot, _ = w.eval_direct_plan(*args, form="split", n_slices=5)
+    else:
+        pot, _ = w.eval_direct_batched(*args)
+    meta = orbit_ref.case_metadata(2, 3, (1,), [0, 1])
+    data = np.stack([t.compact_values() for t in tables])
+    nm = np.stack([t.mode_normalizers for t in tables]).astype(np.complex128)
+    ref = list1_ref.near_field_from_tree(
+        tr, data, nm, meta, fc, mode="per_level", scaling_power=0, log_displacement=False,
+        table_root_extent=tables[0].source_box_extent, table_starting_level=1)
+    l2, linf = rel_errors(pot[0].cpu().numpy(), ref)
+    assert l2 <= tol and linf <= tol, (l2, linf)
+
+
+@pytest.mark.parametrize("dim,level,k,qs,tol", [(2, 3, 4.0, (4, 6), 2e-3),
+                                                (3, 2, 2.0, (3, 5), 5e-3)])
+def test_split_gradient_matches_helmholtz_gradient_tables(cuda, dim, level, k, qs, tol):
+    """Two independent routes to the list 1 of a Helmholtz gradient: the
+    split (wrapped Laplace table + derivative remainder, point to point) and
+    Duffy-built Helmholtz-gradient tables.  The gap is the Gauss-rule error
+    of the remainder's derivative: it shrinks as q grows and stays well below
+    the remainder's share (a wrong sign or a dropped term would make it as
+    large as the remainder).  In 3D the derivative remainder keeps a bounded
+    but discontinuous (x_a - y_a)/r term at the target, so the rule converges
+    slowly there: margin 4 instead of 10."""
+    margin = 10 if dim == 2 else 4
+    gaps, rems = [], []
+    for q in qs:
+        print(f"case: gradient split vs Helmholtz-gradient tables {dim}D q={q} L{level}",
+              flush=True)
+        tr = vtree.uniform_tree(dim, q, level)
+        hk = AxisTargetDerivative(0, HelmholtzKernel(dim, k))
+        h = tr.root_extent / 2**level
+        lap = _table("target", 0, dim, q)
+        helm = _helm_table("target", 0, dim, q, k, h, level)
+        f = densities.gaussian_mixture_batch(tr.points, [3])[0].astype(np.complex128)
+        fd = torch.as_tensor(f, device="cuda")
+        args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+        ws = NearFieldWrangler(tr, {repr(hk): [lap]}, target_kernels=[hk])
+        assert ws.helmholtz_split
+        split, _ = ws.eval_direct_split(*args, fd)
+        lap_only, _ = ws.eval_direct(*args, fd)
+        wh = NearFieldWrangler(tr, {repr(hk): [helm]}, target_kernels=[hk])
+        assert not wh.helmholtz_split
+        ht, _ = wh.eval_direct(*args, fd)
+        a, b, c = split[0].cpu().numpy(), ht[0].cpu().numpy(), lap_only[0].cpu().numpy()
+        scale = np.max(np.abs(b))
+        gap, rem = np.max(np.abs(a - b)) / scale, np.max(np.abs(a - c)) / scale
+        print(f"  gap {gap:.3e}, remainder {rem:.3e}", flush=True)
+        gaps.append(gap)
+        rems.append(rem)
+    assert all(r >= margin * g for r, g in zip(rems, gaps)), (rems, gaps)
+    assert gaps[-1] <= tol and gaps[-1] < 0.5 * gaps[0], gaps

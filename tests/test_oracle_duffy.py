@@ -151,3 +151,24 @@ def test_derivative_entries_follow_signed_orbits(dim, q, axis):
         canon = g.entry(int(reps[row]), knl, 20, 61)
         assert abs(direct - sg * canon) <= 1e-12 * max(1.0, abs(canon)), (e, direct, sg, canon)
     assert seen_neg
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_helmholtz_derivative_kernels_are_derivatives(dim, axis):
+    """The Helmholtz axis-derivative kernels equal central differences of
+    the (library-function) Helmholtz kernels along the axis; source = minus."""
+    rng = np.random.default_rng(dim + 2 * axis)
+    k = 3.7
+    base = D.helmholtz2d(k) if dim == 2 else D.helmholtz3d(k)
+    gt = D.helmholtz_target_derivative(dim, axis, k)
+    gs = D.helmholtz_source_derivative(dim, axis, k)
+    h = 1e-6
+    for _ in range(20):
+        d = rng.uniform(-1, 1, dim)
+        dp, dm = d.copy(), d.copy()
+        dp[axis] += h
+        dm[axis] -= h
+        fd = (base(*dp) - base(*dm)) / (2 * h)
+        assert abs(gt(*d) - fd) <= 1e-7 * max(1.0, abs(fd)), (d, gt(*d), fd)
+        assert gs(*d) == -gt(*d)

@@ -29,6 +29,8 @@ KERNEL_LAPLACE = 1
 KERNEL_HELMHOLTZ = 2
 KERNEL_LAPLACE_TARGET_DERIV = 3
 KERNEL_LAPLACE_SOURCE_DERIV = 4
+KERNEL_HELMHOLTZ_TARGET_DERIV = 5
+KERNEL_HELMHOLTZ_SOURCE_DERIV = 6
 
 SPLIT_DERIV_NONE = 0
 SPLIT_DERIV_TARGET = 1

@@ -69,6 +69,16 @@ __device__ inline double lagrange_basis(double y, int m, int q, double cm, const
 // = (-Y0(kr) + i J0(kr)) / 4, 3D e^{ikr}/(4 pi r).  Axis derivatives of
 // Laplace (da = component of d = target - source on the derivative axis):
 // target dG/dd_a = 2D -da/(2 pi r^2), 3D -da/(4 pi r^3); source = minus.
+// Axis derivatives of Helmholtz: G'(r) da / r with G' = 2D -(ik/4) H1^(1)(kr)
+// = (k/4)(Y1(kr) - i J1(kr)), 3D e^{ikr}(ikr - 1)/(4 pi r^2).
+__host__ __device__ constexpr bool kind_is_complex(int kind) {
+  return kind == VL1_KERNEL_HELMHOLTZ || kind == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV ||
+         kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV;
+}
+__host__ __device__ constexpr bool kind_is_deriv(int kind) {
+  return kind == VL1_KERNEL_LAPLACE_TARGET_DERIV || kind == VL1_KERNEL_LAPLACE_SOURCE_DERIV ||
+         kind == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV || kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV;
+}
 template <int DIM, int KIND>
 __device__ inline void kernel_value(double r2, double da, double k, double& re, double& im) {
   if (KIND == VL1_KERNEL_LAPLACE_TARGET_DERIV || KIND == VL1_KERNEL_LAPLACE_SOURCE_DERIV) {
@@ -80,6 +90,22 @@ __device__ inline void kernel_value(double r2, double da, double k, double& re, 
       re = sgn * 0.25 / M_PI * da * ir * ir * ir;
     }
     im = 0.0;
+  } else if (KIND == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV || KIND == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV) {
+    constexpr double sgn = KIND == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV ? 1.0 : -1.0;
+    const double ir = rsqrt(r2);
+    const double r = r2 * ir;
+    const double kr = k * r;
+    if (DIM == 2) {
+      const double f = sgn * 0.25 * k * da * ir;
+      re = f * y1(kr);
+      im = -f * j1(kr);
+    } else {
+      double sn, cs;
+      sincos(kr, &sn, &cs);
+      const double f = sgn * 0.25 / M_PI * da * ir * ir * ir;
+      re = f * (-cs - kr * sn);
+      im = f * (kr * cs - sn);
+    }
   } else if (KIND == VL1_KERNEL_CONSTANT) {
     re = 1.0;
     im = 0.0;
@@ -170,10 +196,10 @@ __global__ void __launch_bounds__(256) duffy_kernel(const DuffyParams p) {
       double kr, ki;
       kernel_value<DIM, KIND>(r2, da, p.k, kr, ki);
       acc_re = fma(kr, basis, acc_re);
-      if (KIND == VL1_KERNEL_HELMHOLTZ) acc_im = fma(ki, basis, acc_im);
+      if (kind_is_complex(KIND)) acc_im = fma(ki, basis, acc_im);
     }
   }
-  if (KIND == VL1_KERNEL_HELMHOLTZ) {
+  if (kind_is_complex(KIND)) {
     p.out[2 * e] = acc_re;
     p.out[2 * e + 1] = acc_im;
   } else {
@@ -199,7 +225,7 @@ constexpr int kSegNodes = 64;
 template <int DIM, int KIND, int Q>
 __global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
   constexpr int kMaxQ = 16;
-  constexpr int VW = KIND == VL1_KERNEL_HELMHOLTZ ? 2 : 1;
+  constexpr int VW = kind_is_complex(KIND) ? 2 : 1;
   const int q = Q > 0 ? Q : p.q;
   extern __shared__ double rec[];  // [kSegNodes][VW + DIM * q]
   const int stride = VW + DIM * q;
@@ -339,7 +365,7 @@ __global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
 template <int DIM, int KIND, int Q>
 static hipError_t launch_duffy_q(const DuffyParams& prm, hipStream_t s) {
   if (prm.chunk_start) {
-    const int vw = KIND == VL1_KERNEL_HELMHOLTZ ? 2 : 1;
+    const int vw = kind_is_complex(KIND) ? 2 : 1;
     const size_t smem = sizeof(double) * kSegNodes * (vw + (size_t)DIM * prm.q);
     hipLaunchKernelGGL((duffy_seg_kernel<DIM, KIND, Q>), dim3((unsigned)prm.n_chunks),
                        dim3(kSegNodes), smem, s, prm);
@@ -382,8 +408,7 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
   if (!a->target_points || !a->decomposition || !a->mode_axes || !a->node_u || !a->node_sign ||
       !a->node_jac || !a->interp_nodes || !a->bary_w || !a->result)
     return fail(VL1_E_ARG, "vl1_duffy_build: null device array");
-  const bool deriv = a->kernel_kind == VL1_KERNEL_LAPLACE_TARGET_DERIV ||
-                     a->kernel_kind == VL1_KERNEL_LAPLACE_SOURCE_DERIV;
+  const bool deriv = kind_is_deriv(a->kernel_kind);
   if (deriv && (a->deriv_axis < 0 || a->deriv_axis >= a->dim))
     return fail(VL1_E_ARG, "vl1_duffy_build: derivative axis out of range");
   if (a->chunk_start && (a->n_chunks < 1 || a->n_chunks > INT32_MAX))
@@ -405,6 +430,10 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
       e = launch_duffy<2, VL1_KERNEL_LAPLACE_TARGET_DERIV>(prm, s);
     else if (kind == VL1_KERNEL_LAPLACE_SOURCE_DERIV)
       e = launch_duffy<2, VL1_KERNEL_LAPLACE_SOURCE_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV)
+      e = launch_duffy<2, VL1_KERNEL_HELMHOLTZ_TARGET_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV)
+      e = launch_duffy<2, VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV>(prm, s);
     else return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: kernel kind");
   } else {
     if (kind == VL1_KERNEL_CONSTANT) e = launch_duffy<3, VL1_KERNEL_CONSTANT>(prm, s);
@@ -414,6 +443,10 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
       e = launch_duffy<3, VL1_KERNEL_LAPLACE_TARGET_DERIV>(prm, s);
     else if (kind == VL1_KERNEL_LAPLACE_SOURCE_DERIV)
       e = launch_duffy<3, VL1_KERNEL_LAPLACE_SOURCE_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV)
+      e = launch_duffy<3, VL1_KERNEL_HELMHOLTZ_TARGET_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV)
+      e = launch_duffy<3, VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV>(prm, s);
     else return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: kernel kind");
   }
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("duffy launch: ") + hipGetErrorString(e));

@@ -72,7 +72,7 @@ def duffy_values(table, entry_ids, regular_order, radial_order, stream=None,
         dim=table.dim,
         quad_order=table.quad_order,
         kernel_kind=int(table.integral_knl.kind),
-        helmholtz_k=float(getattr(table.integral_knl, "k", 0.0)),
+        helmholtz_k=float(getattr(table.integral_knl.get_base_kernel(), "k", 0.0)),
         n_entries=n_entries,
         n_nodes=nodes.n_nodes,
         nodes_per_sign=nodes_per_sign,

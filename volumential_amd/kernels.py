@@ -14,6 +14,9 @@ Kernel conventions (sumpy expression times global scaling constant):
 * Constant ``1``
 * Axis derivatives of Laplace: target ``dG/dd_a`` = 2D ``-d_a / (2 pi |d|^2)``,
   3D ``-d_a / (4 pi |d|^3)`` (``d = target - source``); source = minus that
+* Axis derivatives of Helmholtz: target ``G'(|d|) d_a / |d|`` with
+  ``G'`` = 2D ``-(ik/4) H1^(1)(k|d|)``, 3D ``e^{ik|d|}(ik|d| - 1) / (4 pi |d|^2)``;
+  source = minus that
 """
 
 from __future__ import annotations
@@ -70,7 +73,12 @@ class AxisTargetDerivative:
 
     axis: int
     inner_kernel: object
-    kind = _capi.KERNEL_LAPLACE_TARGET_DERIV
+
+    @property
+    def kind(self):
+        """Table-builder kernel id (one derivative of Laplace or Helmholtz)."""
+        return _derivative_kind(self, _capi.KERNEL_LAPLACE_TARGET_DERIV,
+                                _capi.KERNEL_HELMHOLTZ_TARGET_DERIV)
 
     @property
     def dim(self):
@@ -93,7 +101,11 @@ class AxisSourceDerivative:
 
     axis: int
     inner_kernel: object
-    kind = _capi.KERNEL_LAPLACE_SOURCE_DERIV
+
+    @property
+    def kind(self):
+        return _derivative_kind(self, _capi.KERNEL_LAPLACE_SOURCE_DERIV,
+                                _capi.KERNEL_HELMHOLTZ_SOURCE_DERIV)
 
     @property
     def dim(self):
@@ -111,6 +123,15 @@ class AxisSourceDerivative:
 
 
 _DERIVATIVES = (AxisTargetDerivative, AxisSourceDerivative)
+
+
+def _derivative_kind(kernel, laplace_kind, helmholtz_kind):
+    inner = kernel.inner_kernel
+    if isinstance(inner, LaplaceKernel):
+        return laplace_kind
+    if isinstance(inner, HelmholtzKernel):
+        return helmholtz_kind
+    raise NotImplementedError(f"no table builder for {kernel!r} (nested derivatives)")
 
 
 def derivative_axes(kernel):

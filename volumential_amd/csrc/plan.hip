@@ -538,6 +538,12 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
   __syncthreads();
 
   const int total = nst * nkc;
+  // With two or more chunks per step the next step's column descriptors
+  // are written one chunk early (chunk nkc - 2), so its first chunk is
+  // fetched before this step's last MFMAs like any other chunk; with one
+  // chunk per step they are written in that chunk and the fetch waits for
+  // the barrier.
+  const bool early = nkc >= 2;
   for (int it = 0; it < total; ++it) {
     const int k = it / nkc, kc = it % nkc;
     const int ring = it & 1, cslot = k & 1;
@@ -548,12 +554,12 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
     const bool next_step = kc + 1 == nkc;
     const int nkc_ = next_step ? 0 : kc + 1;
     int nblk = blk;
-    if (has_next && next_step) {
-      nblk = p.step_block[step_of(k + 1)];
-      load_cols(step_of(k + 1), cslot ^ 1);  // read after the barrier below
-    }
+    if (has_next && next_step) nblk = p.step_block[step_of(k + 1)];
+    // slot cslot ^ 1 held step k - 1, whose last use (its step end) is done
+    if (k + 1 < nst && (early ? kc + 2 == nkc : next_step))
+      load_cols(step_of(k + 1), cslot ^ 1);  // read after the barriers below
     uint4 xnext[NV];
-    if (has_next && !next_step) fetch_chunk(cslot, nkc_, xnext);
+    if (has_next && (!next_step || early)) fetch_chunk(next_step ? cslot ^ 1 : cslot, nkc_, xnext);
 
     // MFMAs of this chunk: density slice j from LDS, table slices from regs
     const __bf16* rX = sX + ring * RING;
@@ -621,7 +627,7 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
     }
     __syncthreads();  // ring[ring ^ 1] free, next descriptors visible
     if (has_next) {
-      if (next_step) fetch_chunk(cslot ^ 1, 0, xnext);
+      if (next_step && !early) fetch_chunk(cslot ^ 1, 0, xnext);
       store_chunk(ring ^ 1, xnext);
     }
     __syncthreads();  // next chunk visible
@@ -1545,8 +1551,8 @@ int vl1_plan_tune(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rh
     }
   }
   pl->forced_variant = -1;
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (rc != VL1_OK) return rc;
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_tune: ") + hipGetErrorString(e));
   pl->tuned[tune_key(form, S, Rc)] = best;

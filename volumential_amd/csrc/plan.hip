@@ -625,12 +625,15 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
           for (int b = 0; b < 4; ++b) acc[g][a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
       if (has_next && threadIdx.x == 0) __builtin_nontemporal_store(nblk, p.phase + xcd);
     }
-    __syncthreads();  // ring[ring ^ 1] free, next descriptors visible
+    // ring[ring ^ 1] was last read in the previous iteration, which ended
+    // with a barrier; only the late descriptor path (one chunk per step)
+    // needs one before the fetch reads them
+    if (!early) __syncthreads();
     if (has_next) {
       if (next_step && !early) fetch_chunk(cslot ^ 1, 0, xnext);
       store_chunk(ring ^ 1, xnext);
     }
-    __syncthreads();  // next chunk visible
+    __syncthreads();  // next chunk (and next descriptors) visible
   }
   // epilogue: write targets of the tile's boxes
 #pragma unroll

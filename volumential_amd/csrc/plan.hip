@@ -768,18 +768,10 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
     store_chunk(0, buf, alC);
     const int blk = p.step_block[step_of(0)];
     if (threadIdx.x == 0) __builtin_nontemporal_store(blk, p.phase + xcd);
-    // through the next-chunk registers, as in the loop: the copy waits for
-    // these loads here, so no current-fragment register enters the loop
-    // with a load pending (otherwise every iteration's MFMAs are made to
-    // wait on the vector-memory counter, i.e. on the next chunk's loads)
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-      for (int u = 0; u < 8; ++u) an[rt][u] = *frag(blk, rt, u);
-#pragma unroll
-    for (int rt = 0; rt < NRT; ++rt)
-#pragma unroll
-      for (int u = 0; u < 8; ++u) ac[rt][u] = an[rt][u];
+      for (int u = 0; u < 8; ++u) ac[rt][u] = *frag(blk, rt, u);
   }
   __syncthreads();
 

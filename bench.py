@@ -219,6 +219,20 @@ def measured_traffic(kernel, level, q, rhs):
         return None, None
 
 
+def host_cpu():
+    """The host's CPU model (/proc/cpuinfo) and logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(tree, table, f_host_rows, seconds):
     """C/OpenMP oracle (reference semantics, one evaluation per RHS) on a
     bounded sample of interior target boxes; returns points x RHS / s."""
@@ -270,6 +284,7 @@ def cpu_baseline(tree, table, f_host_rows, seconds):
         "kind": "port",
         "sample": f"{nb} target boxes x {rhs} RHS of the same tree/table "
                   f"({nb * tree.n_q_points * rhs} target-point evaluations, {t:.1f} s)",
+        **host_cpu(),
     }
 
 
@@ -681,7 +696,7 @@ def run_hsplit(args):
         extra["cpu_baseline"] = {
             "value": nb * n / dt / 1e9, "unit": "Gpts*RHS/s", "cores": cores, "kind": "port",
             "sample": f"{nb} target boxes x 1 RHS: C/OpenMP Laplace-table part (Re, Im) + "
-                      f"numpy remainder sum ({dt:.1f} s)"}
+                      f"numpy remainder sum ({dt:.1f} s)", **host_cpu()}
     return _line(
         forms[best]["value"], forms[best]["ms"], args.steps, args.warmup, "f64 (complex, fused)",
         "synthetic (16 seeded random-phase Gaussian beams, k=40; GPU-built Laplace 2D table)",
@@ -736,7 +751,7 @@ def run_c5(args):
             "value": done * n_nodes / dt / 1e9, "unit": "Gnode-evals/s", "cores": 1,
             "kind": "port",
             "sample": f"{done} seeded canonical entries of the same table, rule (20, 20), "
-                      f"numpy per-entry Duffy quadrature ({dt:.1f} s)"}
+                      f"numpy per-entry Duffy quadrature ({dt:.1f} s)", **host_cpu()}
     ms = r20["kernel_ms"]
     return _line(
         r20["node_evals_per_s"] / 1e9, ms, len(recs), 0, "f64",

@@ -227,7 +227,10 @@ __global__ void frag_f64_kernel(int MP, int KP, int n_blocks, const double* __re
 
 // Densities -> per (box, rhs) column: BF16 slices X[box][j][r][k] (mode
 // order, zero padded to KP), column scale 2^(e - 7) and, for the 2D log term,
-// nm[box][r] = sum_sid nmlz[sid] f[sbeg + sid].  One wavefront per column.
+// nm[box][r] = sum_sid nmlz[sid] f[sbeg + sid].  Half a wavefront per column
+// (KP <= 256): each lane owns 8 consecutive k, so every slice goes out as one
+// 16-byte store per lane.
+constexpr int kColK = 8;
 __global__ void split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, int S,
                                   const double* __restrict__ f, int64_t ldf,
                                   const int32_t* __restrict__ mode_point,
@@ -235,38 +238,51 @@ __global__ void split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, in
                                   const double* __restrict__ nmlz, int log_disp,
                                   __bf16* __restrict__ X, double* __restrict__ colscale,
                                   double* __restrict__ nmdot) {
-  const int col = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-  const int lane = threadIdx.x % 64;
-  if (col >= nboxes * R) return;
-  const int box = col / R, r = col % R;
+  const int col = (blockIdx.x * blockDim.x + threadIdx.x) / 32;
+  const int hl = threadIdx.x % 32;  // lane within the half wavefront
+  const bool live = col < nboxes * R;
+  const int box = live ? col / R : 0, r = live ? col % R : 0;
   const double* fr = f + (int64_t)r * ldf;
   const int rows = cplx ? 2 * n : n;
-  auto val = [&](int s) {  // component s of the column: [Re f; Im f] if complex
-    const int pt = mode_point[(int64_t)box * n + s % n];
-    return cplx ? fr[2 * pt + s / n] : fr[pt];
-  };
+  const int s0 = kColK * hl;
+  double v[kColK];
   double mx = 0.0, dot = 0.0;
-  for (int s = lane; s < rows; s += 64) {
-    mx = fmax(mx, fabs(val(s)));
-    if (log_disp) dot += nmlz[s] * fr[box_starts[box] + s];
+#pragma unroll
+  for (int u = 0; u < kColK; ++u) {
+    const int s = s0 + u;
+    v[u] = 0.0;
+    if (live && s < rows) {  // component s of the column: [Re f; Im f] if complex
+      const int pt = mode_point[(int64_t)box * n + s % n];
+      v[u] = cplx ? fr[2 * pt + s / n] : fr[pt];
+      mx = fmax(mx, fabs(v[u]));
+      if (log_disp && s < n) dot += nmlz[s] * fr[box_starts[box] + s];
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    mx = fmax(mx, __shfl_xor(mx, o));
-    dot += __shfl_xor(dot, o);
+  for (int o = 16; o > 0; o >>= 1) {  // within the half wavefront
+    mx = fmax(mx, __shfl_xor(mx, o, 32));
+    dot += __shfl_xor(dot, o, 32);
   }
+  if (!live) return;
   int e = 0;
   if (mx > 0.0) frexp(mx, &e);
-  if (lane == 0) {
+  if (hl == 0) {
     colscale[col] = mx > 0.0 ? ldexp(1.0, e - 7) : 0.0;
     nmdot[col] = dot;
   }
-  for (int s = lane; s < KP; s += 64) {
-    double v = (s < rows && mx > 0.0) ? ldexp(val(s), -e) : 0.0;
-    for (int j = 0; j < S; ++j) {
-      const double d = rint(ldexp(v, 7 + 8 * j));
-      v -= ldexp(d, -7 - 8 * j);
-      X[(((int64_t)box * S + j) * R + r) * KP + s] = (__bf16)(float)ldexp(d, -8 * j);
+  if (s0 >= KP) return;
+#pragma unroll
+  for (int u = 0; u < kColK; ++u) v[u] = mx > 0.0 ? ldexp(v[u], -e) : 0.0;
+  for (int j = 0; j < S; ++j) {
+    uint32_t w[kColK / 2];
+#pragma unroll
+    for (int u = 0; u < kColK; ++u) {
+      const double d = rint(ldexp(v[u], 7 + 8 * j));
+      v[u] -= ldexp(d, -7 - 8 * j);
+      const uint32_t bits = __builtin_bit_cast(uint16_t, (__bf16)(float)ldexp(d, -8 * j));
+      w[u / 2] = (u % 2) ? (w[u / 2] | (bits << 16)) : bits;
     }
+    *reinterpret_cast<uint4*>(X + (((int64_t)box * S + j) * R + r) * KP + s0) =
+        make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
@@ -1403,7 +1419,8 @@ int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_r
     if (e == hipSuccess && pl->colscale.bytes < (size_t)ncolumns * 8)
       e = pl->colscale.alloc((size_t)ncolumns * 8);
     if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
-    hipLaunchKernelGGL(split_cols_kernel, dim3((ncolumns + wpb - 1) / wpb), dim3(64 * wpb), 0, s, pl->n,
+    // two columns per wavefront
+    hipLaunchKernelGGL(split_cols_kernel, dim3((ncolumns + 2 * wpb - 1) / (2 * wpb)), dim3(64 * wpb), 0, s, pl->n,
                        pl->cplx, pl->KP, pl->nboxes, R, S, f, ldf, pl->mode_point.as<int32_t>(),
                        pl->box_starts.as<int32_t>(), nmlz, pl->log_disp, pl->X.as<__bf16>(),
                        pl->colscale.as<double>(), pl->nmdot.as<double>());

@@ -82,3 +82,33 @@ def test_host_rows_and_signs_signed_kernels(dim, q, fixed, power):
         off = m.case_value_offsets[m.case_orbit_ranks[c]]
         assert np.array_equal(rows + off, orbit.entry_rows(m, cc, s, t))
         assert np.array_equal(sg, orbit.entry_signs(m, cc, s, t))
+
+
+def test_split_correction_rejects_bad_derivative_wrapper():
+    """vl1_split_correction validates the derivative wrapper before any
+    device work (the dummy device pointers are never touched)."""
+    L = _capi.lib()
+    dummy = 256
+    base = dict(dim=2, split_order=1, series_nmax=5, n_tgt_boxes=1, n_rhs=1, max_box_points=4,
+                k_re=3.0, k_im=0.0, n_points=16, target_boxes=dummy, nbr_starts=dummy,
+                nbr_lists=dummy, box_starts=dummy, box_counts=dummy, points=dummy,
+                strengths=dummy, ld_strengths=32, result=dummy, ld_result=32)
+    for kind, axis, what in ((3, 0, "deriv_kind"), (-1, 0, "deriv_kind"),
+                             (_capi.SPLIT_DERIV_TARGET, 2, "deriv_axis"),
+                             (_capi.SPLIT_DERIV_SOURCE, -1, "deriv_axis")):
+        args = _capi.SplitArgs(**base, deriv_kind=kind, deriv_axis=axis)
+        with pytest.raises(ValueError):
+            _capi.check(L.vl1_split_correction(C.byref(args), None), "vl1_split_correction")
+        assert what in _capi.last_error()
+
+
+def test_split_correction_host_validates_derivative_options():
+    """HelmholtzSplitCorrection rejects a bad wrapper before any device use."""
+    from volumential_amd import tree as vtree
+    from volumential_amd.split import HelmholtzSplitCorrection
+
+    tr = vtree.uniform_tree(2, 3, 2)
+    with pytest.raises(ValueError, match="deriv_kind"):
+        HelmholtzSplitCorrection(tr, 3.0, deriv_kind=7)
+    with pytest.raises(ValueError, match="deriv_axis"):
+        HelmholtzSplitCorrection(tr, 3.0, deriv_kind=_capi.SPLIT_DERIV_TARGET, deriv_axis=2)

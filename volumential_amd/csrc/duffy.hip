@@ -212,9 +212,10 @@ __global__ void __launch_bounds__(256) duffy_kernel(const DuffyParams p) {
 // kernel value G(x - y); they differ only in the source mode m, i.e. in the
 // Lagrange factor prod_k l_{m_k}(y_k).  One wavefront takes a chunk of at
 // most 64 such entries.  Per round each lane produces one node record in
-// LDS -- K = G(x - y) jac prod_k len_k and the q basis values l_0..l_{q-1}
-// of each axis -- and then every lane accumulates its entry over the
-// wave's 64 records: K * prod_k l_{m_k}(y_k), a few LDS reads and
+// LDS -- K l_0..l_{q-1} of axis 0 (K = G(x - y) jac prod_k len_k folded in,
+// complex for Helmholtz kinds) and the q basis values of each other axis --
+// and then every lane accumulates its entry over the wave's 64 records:
+// (K l_{m_0}(y_0)) * prod_{k>0} l_{m_k}(y_k), a few LDS reads and
 // multiplies per (entry, node) instead of the whole geometry and kernel
 // evaluation.  With fewer than 64 entries, R = 64 / ne' lanes share an
 // entry (ne' = ne rounded up to a power of two) and split the records,
@@ -227,8 +228,8 @@ __global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
   constexpr int kMaxQ = 16;
   constexpr int VW = kind_is_complex(KIND) ? 2 : 1;
   const int q = Q > 0 ? Q : p.q;
-  extern __shared__ double rec[];  // [kSegNodes][VW + DIM * q]
-  const int stride = VW + DIM * q;
+  extern __shared__ double rec[];  // [kSegNodes][VW * q + (DIM - 1) * q]
+  const int stride = (VW + DIM - 1) * q;
   const int lane = threadIdx.x;
   const int64_t e0 = p.chunk_start[blockIdx.x];
   const int ne = (int)(p.chunk_start[blockIdx.x + 1] - e0);
@@ -323,14 +324,19 @@ __global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
         }
       }
       __syncthreads();  // the previous round's records have been read
-      my[0] = kr;
-      if (VW == 2) my[1] = ki;
+      // [K l(y_0) re][K l(y_0) im (complex)][l(y_1)]...[l(y_{DIM-1})]
 #pragma unroll
-      for (int a = 0; a < DIM; ++a)
+      for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j) {
+        if (Q == 0 && j >= q) break;
+        my[j] = ok ? kr * bas[0][j] : 0.0;
+        if (VW == 2) my[q + j] = ok ? ki * bas[0][j] : 0.0;
+      }
+#pragma unroll
+      for (int a = 1; a < DIM; ++a)
 #pragma unroll
         for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j) {
           if (Q == 0 && j >= q) break;
-          my[VW + a * q + j] = ok ? bas[a][j] : 0.0;
+          my[(VW + a - 1) * q + j] = ok ? bas[a][j] : 0.0;
         }
       __syncthreads();
       // consume: this lane's share of the wave's records for its entry
@@ -338,11 +344,11 @@ __global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
       if (active) {
         for (int t = part; t < nrec; t += R) {
           const double* rr = rec + (size_t)t * stride;
-          double bp = rr[VW + m[0]];
+          double bp = rr[VW * q + m[1]];
 #pragma unroll
-          for (int a = 1; a < DIM; ++a) bp *= rr[VW + a * q + m[a]];
-          acc_re = fma(rr[0], bp, acc_re);
-          if (VW == 2) acc_im = fma(rr[1], bp, acc_im);
+          for (int a = 2; a < DIM; ++a) bp *= rr[(VW + a - 1) * q + m[a]];
+          acc_re = fma(rr[m[0]], bp, acc_re);
+          if (VW == 2) acc_im = fma(rr[q + m[0]], bp, acc_im);
         }
       }
     }
@@ -366,7 +372,7 @@ template <int DIM, int KIND, int Q>
 static hipError_t launch_duffy_q(const DuffyParams& prm, hipStream_t s) {
   if (prm.chunk_start) {
     const int vw = kind_is_complex(KIND) ? 2 : 1;
-    const size_t smem = sizeof(double) * kSegNodes * (vw + (size_t)DIM * prm.q);
+    const size_t smem = sizeof(double) * kSegNodes * ((size_t)(vw + DIM - 1) * prm.q);
     hipLaunchKernelGGL((duffy_seg_kernel<DIM, KIND, Q>), dim3((unsigned)prm.n_chunks),
                        dim3(kSegNodes), smem, s, prm);
     return hipGetLastError();

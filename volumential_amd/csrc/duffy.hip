@@ -221,15 +221,20 @@ __global__ void __launch_bounds__(256) duffy_kernel(const DuffyParams p) {
 // entry (ne' = ne rounded up to a power of two) and split the records,
 // then combine by shuffles.  Nodes, signs and the skip rule are those of
 // duffy_kernel; only the summation order differs.
+//
+// LDS holds the records field-major, rec[field][record] with a row pitch of
+// kSegPitch doubles (rows start 16 bytes apart modulo the 128-byte bank
+// window), so a lane reads one field of two consecutive records with one
+// 16-byte load.  Records past the end of a sign block are zero.
 constexpr int kSegNodes = 64;
+constexpr int kSegPitch = kSegNodes + 2;
 
 template <int DIM, int KIND, int Q>
 __global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
   constexpr int kMaxQ = 16;
   constexpr int VW = kind_is_complex(KIND) ? 2 : 1;
   const int q = Q > 0 ? Q : p.q;
-  extern __shared__ double rec[];  // [kSegNodes][VW * q + (DIM - 1) * q]
-  const int stride = (VW + DIM - 1) * q;
+  extern __shared__ double rec[];  // [VW * q + (DIM - 1) * q][kSegPitch]
   const int lane = threadIdx.x;
   const int64_t e0 = p.chunk_start[blockIdx.x];
   const int ne = (int)(p.chunk_start[blockIdx.x + 1] - e0);
@@ -262,7 +267,7 @@ __global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
   const double tol = 8.0 * 2.220446049250313e-16 * p.H;
   double acc_re = 0.0, acc_im = 0.0;
   const int64_t nsign = p.n_nodes / p.nodes_per_sign;
-  double* my = rec + (size_t)lane * stride;
+  double* my = rec + lane;  // this lane's record: field f at my[f * kSegPitch]
   for (int64_t b = 0; b < nsign; ++b) {
     const int64_t base = b * p.nodes_per_sign;
     double len[DIM], sl[DIM], lenprod = 1.0;
@@ -328,27 +333,39 @@ __global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
 #pragma unroll
       for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j) {
         if (Q == 0 && j >= q) break;
-        my[j] = ok ? kr * bas[0][j] : 0.0;
-        if (VW == 2) my[q + j] = ok ? ki * bas[0][j] : 0.0;
+        my[j * kSegPitch] = ok ? kr * bas[0][j] : 0.0;
+        if (VW == 2) my[(q + j) * kSegPitch] = ok ? ki * bas[0][j] : 0.0;
       }
 #pragma unroll
       for (int a = 1; a < DIM; ++a)
 #pragma unroll
         for (int j = 0; j < (Q > 0 ? Q : kMaxQ); ++j) {
           if (Q == 0 && j >= q) break;
-          my[(VW + a - 1) * q + j] = ok ? bas[a][j] : 0.0;
+          my[((VW + a - 1) * q + j) * kSegPitch] = ok ? bas[a][j] : 0.0;
         }
       __syncthreads();
       // consume: this lane's share of the wave's records for its entry
       const int nrec = (int)min<int64_t>(kSegNodes, p.nodes_per_sign - r0);
       if (active) {
-        for (int t = part; t < nrec; t += R) {
-          const double* rr = rec + (size_t)t * stride;
-          double bp = rr[VW * q + m[1]];
-#pragma unroll
-          for (int a = 2; a < DIM; ++a) bp *= rr[(VW + a - 1) * q + m[a]];
-          acc_re = fma(rr[m[0]], bp, acc_re);
-          if (VW == 2) acc_im = fma(rr[q + m[0]], bp, acc_im);
+        // record pairs (t, t + 1); the zero records past nrec add nothing
+        const double* f0 = rec + (size_t)m[0] * kSegPitch;
+        const double* f0i = rec + (size_t)(q + m[0]) * kSegPitch;
+        const double* f1 = rec + (size_t)(VW * q + m[1]) * kSegPitch;
+        const double* f2 = rec + (size_t)((VW + 1) * q + m[DIM - 1]) * kSegPitch;
+        for (int t = 2 * part; t < nrec; t += 2 * R) {
+          const double2 b1 = *reinterpret_cast<const double2*>(f1 + t);
+          double2 bp = b1;
+          if (DIM == 3) {
+            const double2 b2 = *reinterpret_cast<const double2*>(f2 + t);
+            bp.x *= b2.x;
+            bp.y *= b2.y;
+          }
+          const double2 k0 = *reinterpret_cast<const double2*>(f0 + t);
+          acc_re = fma(k0.x, bp.x, fma(k0.y, bp.y, acc_re));
+          if (VW == 2) {
+            const double2 k0i = *reinterpret_cast<const double2*>(f0i + t);
+            acc_im = fma(k0i.x, bp.x, fma(k0i.y, bp.y, acc_im));
+          }
         }
       }
     }
@@ -372,7 +389,7 @@ template <int DIM, int KIND, int Q>
 static hipError_t launch_duffy_q(const DuffyParams& prm, hipStream_t s) {
   if (prm.chunk_start) {
     const int vw = kind_is_complex(KIND) ? 2 : 1;
-    const size_t smem = sizeof(double) * kSegNodes * ((size_t)(vw + DIM - 1) * prm.q);
+    const size_t smem = sizeof(double) * kSegPitch * ((size_t)(vw + DIM - 1) * prm.q);
     hipLaunchKernelGGL((duffy_seg_kernel<DIM, KIND, Q>), dim3((unsigned)prm.n_chunks),
                        dim3(kSegNodes), smem, s, prm);
     return hipGetLastError();

@@ -770,7 +770,7 @@ def run_c5(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
     if args.workload == "c4":
         print(json.dumps(run_c4(args)), flush=True)

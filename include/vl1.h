@@ -49,7 +49,7 @@ typedef enum vl1_status {
   VL1_E_TABLE = -5,       /* missing / non-finite table data (RuntimeError) */
   VL1_E_NONFINITE = -6,   /* non-finite output (RuntimeError) */
   VL1_E_HIP = -7,         /* HIP runtime failure */
-  VL1_E_UNSUPPORTED = -8  /* configuration this build does not implement */
+  VL1_E_UNSUPPORTED = -8  /* configuration this library does not implement */
 } vl1_status;
 
 typedef enum vl1_kernel_kind {

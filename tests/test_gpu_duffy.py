@@ -4,7 +4,7 @@
   test_nearfield_potential_table.py:107-132), and in general the GL weights;
 * Laplace 2D/3D: sampled canonical entries vs the oracle's per-entry Duffy
   quadrature with the same node rule, |diff| / max(1, |T|) <= 1e-12 (the
-  north star's entrywise bar); mode sums vs closed-form box potentials;
+  entrywise table bar); mode sums vs closed-form box potentials;
 * end to end: a GPU-built table evaluated on the GPU vs the oracle;
 * the full 3D Laplace q=6 build (all 139 cases, 140,496 entries, one
   launch) at GL tail / tanh-sinh orders (20, 20) and (40, 40): sampled

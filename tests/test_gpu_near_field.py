@@ -2,7 +2,7 @@
 through both list-1 routes: the cached FP64 plan (default, vl1_plan_* with
 FORM_F64) and the plain FP64 kernel (vl1_near_field_f64).
 
-Tolerance (north star, FP64 form): rel-L2 and rel-Linf <= 1e-12 per RHS.
+Tolerance (FP64 form): rel-L2 and rel-Linf <= 1e-12 per RHS.
 """
 
 import numpy as np

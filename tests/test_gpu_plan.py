@@ -3,7 +3,7 @@
 * FP64 dense MFMA form vs the CPU oracles: rel-L2 and rel-Linf <= 1e-12 per
   RHS (uniform, adaptive canonical / per-level, 2D log term, permuted ids);
 * Ozaki-split BF16 form vs the CPU oracle: rel-L2 and rel-Linf <= 1e-10 per
-  RHS with 5 slices (north star bar), error decreasing with the slice count;
+  RHS with 5 slices (the split-form bar), error decreasing with the slice count;
 * the split form at a larger size vs the FP64 form (itself pinned above).
 """
 

@@ -7,7 +7,7 @@ Same constructor and call convention as volumential's
 ...)`` returning ``(result, event)``.  Arrays may be torch tensors (device or
 host) or numpy arrays; the evaluation always runs on the GPU through
 ``vl1_near_field_f64``.  User scaling/displacement *code strings* are not
-supported (this build has no runtime code generation); the inferred,
+supported (this library has no runtime code generation); the inferred,
 fixed-single-table and per-level-table policies are.
 """
 
@@ -100,7 +100,7 @@ class NearFieldFromCSR:
     def __init__(self, integral_kernel, table_data_shapes, potential_kind=1, **kwargs):
         if potential_kind != 1:
             raise NotImplementedError(
-                "potential_kind=2 (exterior normalisers) is not part of this build"
+                "potential_kind=2 (exterior normalisers) is not part of this library"
             )
         self.integral_kernel = integral_kernel
         self.potential_kind = potential_kind
@@ -216,7 +216,7 @@ class NearFieldFromCSR:
             if int(kw.get("arithmetic_direction_sign_axis", -1)) >= 0:
                 raise NotImplementedError(
                     "directional source derivatives (arithmetic_direction_sign_axis) are "
-                    "not part of this build; axis derivatives are")
+                    "not part of this library; axis derivatives are")
             return DevicePayload(
                 self.dim,
                 self.quad_order,

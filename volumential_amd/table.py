@@ -154,7 +154,7 @@ class NearFieldInteractionTable:
 
         Every stored entry ``e`` lands on its row as ``sign(e) * value(e)``
         (``sign`` is +1 for scalar kernels), so any choice of stored orbit
-        members works -- this build's representatives or the reference's.
+        members works -- this package's representatives or the reference's.
         """
         if self._data is None:
             raise RuntimeError("table data has not been built")

@@ -52,7 +52,7 @@ def table_payload(table: NearFieldInteractionTable) -> bytes:
 def table_from_payload(blob: bytes, quad_order, dim, kernel, source_box_extent,
                        source_box_level=None) -> NearFieldInteractionTable:
     """A table from an npz payload; reduced or dense storage.  The gallery
-    stored with the payload must be this build's (same case order)."""
+    stored with the payload must be this package's (same case order)."""
     with BytesIO(blob) as fh:
         with np.load(fh, allow_pickle=False) as z:
             arrs = {k: z[k] for k in z.files}
@@ -63,7 +63,7 @@ def table_from_payload(blob: bytes, quad_order, dim, kernel, source_box_extent,
     if not np.array_equal(np.asarray(arrs["interaction_case_vecs"]),
                           np.asarray(t.interaction_case_vecs)) or \
             not np.array_equal(arrs["case_indices"], t.case_indices):
-        raise ValueError("payload gallery (case vectors / indices) differs from this build's")
+        raise ValueError("payload gallery (case vectors / indices) differs from this package's")
     if "reduced_entry_ids" in arrs and "reduced_data" in arrs:
         t.set_reduced_table_data(arrs["reduced_entry_ids"], arrs["reduced_data"])
     elif "data" in arrs:

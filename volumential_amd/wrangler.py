@@ -153,7 +153,7 @@ class NearFieldWrangler:
     def _split_support_status(self):
         """``_split_target_kernel_support_status`` /
         ``_split_base_table_support_status`` (4009-4073, 3964-4007) for the
-        kernels of this build: Helmholtz output kernels, bare or under one
+        kernels of this library: Helmholtz output kernels, bare or under one
         axis target / source derivative wrapper, whose tables all hold the
         same wrapper around ``LaplaceKernel(dim)``."""
         for knl in self.target_kernels:
@@ -193,7 +193,7 @@ class NearFieldWrangler:
     def _finish(self, pot):
         """Round results to the wrangler's single precision when it has one:
         ``expansion_wrangler_fpnd.py:3283-3297`` evaluates in f32 / c64 for a
-        float32 wrangler; this build evaluates in FP64 and rounds the result
+        float32 wrangler; this library evaluates in FP64 and rounds the result
         (it agrees with an f32 evaluation to f32 rounding, and is closer to
         the exact sum)."""
         if self.dtype not in (np.dtype(np.float32), np.dtype(np.complex64)):
@@ -453,7 +453,7 @@ class NearFieldWrangler:
         output kernel (``expansion_wrangler_fpnd.py:3491-3836``), with
         ``src_weights`` = density x quadrature weight (``[n_points]`` or
         ``[R, n_points]``).  Zeros when the split is off.  ``src_func`` (the
-        smooth-quadrature path) is accepted and not used: this build
+        smooth-quadrature path) is accepted and not used: this library
         implements the base-quadrature path."""
         if not self.helmholtz_split:
             return self._finish(self.output_zeros()), TimingFuture()
@@ -523,7 +523,7 @@ def split_wrapper_chain(knl):
     (``_extract_helmholtz_split_kernel_wrapper_chain``, 3884-3910): the bare
     kernel, or one axis target / source derivative around it.  Deeper
     chains raise ``NotImplementedError`` (their wrapped Laplace tables are
-    nested derivatives, which this build does not build)."""
+    nested derivatives, which this library does not build)."""
     from .kernels import AxisSourceDerivative, AxisTargetDerivative
 
     if isinstance(knl, (AxisTargetDerivative, AxisSourceDerivative)):

@@ -90,3 +90,28 @@ def test_helmholtz3d_uniform_fixed_table(cuda, form, tol):
     l2, linf = rel_errors(pot[0].cpu().numpy(), ref)
     print(f"{form}: rel L2 {l2:.2e} Linf {linf:.2e}")
     assert l2 <= tol and linf <= tol, (l2, linf)
+
+
+@pytest.mark.parametrize("form,tol", [("f64", TOL64), ("split", TOLSPLIT)])
+def test_helmholtz3d_q6_row_groups(cuda, form, tol):
+    """3D Helmholtz q = 6: 432 rows per complex block, beyond one tile, so the
+    plan cuts each block into two row groups of 216 rows (one tile each,
+    the full 432 columns); both forms vs the oracle.  Smooth densities (the
+    split form's error bar is relative to the largest |T| |f| products, so
+    white-noise densities with heavy cancellation are not its workload)."""
+    print(f"case: Helmholtz 3D q=6 uniform level 2 (row groups) form={form}", flush=True)
+    tr = vtree.uniform_tree(3, 6, 2)
+    tables = helmholtz_tables(3, 6, 4.0, 2, 1)
+    g = densities.gaussian_mixture_batch(tr.points, [71, 72, 73, 74])
+    f = g[0::2] + 1j * g[1::2]
+    w = NearFieldWrangler(tr, tables[0])
+    pot, _ = w.eval_direct_plan(tr.target_boxes, tr.neighbor_source_boxes_starts,
+                                tr.neighbor_source_boxes_lists, torch.as_tensor(f, device="cuda"),
+                                form=form, n_slices=5)
+    plan = next(iter(w._plans.values()))
+    boxes_per_tile = 64 // f.shape[0]  # two row-group tiles per group of target boxes
+    assert plan.stats()["tiles"] == 2 * -(-len(tr.target_boxes) // boxes_per_tile)
+    ref = _oracle(tr, tables, f, "fixed")
+    l2, linf = rel_errors(pot[0].cpu().numpy(), ref)
+    print(f"{form}: rel L2 {l2:.2e} Linf {linf:.2e}")
+    assert l2 <= tol and linf <= tol, (l2, linf)

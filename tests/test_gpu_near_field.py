@@ -217,21 +217,27 @@ def test_plan_cache_evicts_least_recent(cuda):
 
 
 def test_plan_route_box_size_beyond_tiles(cuda):
-    """3D q=7 (343 nodes per box) is beyond the tile kernels: the plan route
-    hands the kernel to the plain kernel and still meets the FP64 bar."""
-    print("case: plan route, 3D q=7 -> plain", flush=True)
+    """3D q=7 (343 nodes per box) is beyond one tile: the plan cuts each block
+    into two row groups (172 rows each, all 343 columns), one tile each; the
+    FP64 form meets the FP64 bar and the split form the split bar."""
+    print("case: plan route, 3D q=7 row groups", flush=True)
     tr = vtree.uniform_tree(3, 7, 1)
     tables = synthetic_table_levels(3, 7, seed=4)
-    f = densities.gaussian_mixture_batch(tr.points, [61])
+    f = densities.gaussian_mixture_batch(tr.points, [61, 62])
     w = NearFieldWrangler(tr, tables[0])
-    got = w.eval_direct(tr.target_boxes, tr.neighbor_source_boxes_starts,
-                        tr.neighbor_source_boxes_lists, torch.as_tensor(f[0], device="cuda"))[0]
-    assert w._plain_only == {repr(tables[0].integral_knl)}
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = w.eval_direct(*args, torch.as_tensor(f[0], device="cuda"))[0]
+    assert not w._plain_only
+    plan = next(iter(w._plans.values()))
+    assert plan.stats()["tiles"] == 2 * -(-len(tr.target_boxes) // 64)  # 1 RHS: 64 boxes a tile
     meta = orbit_ref.case_metadata(3, 7)
     ref = ocpu.near_field(tr, tables[0].compact_values(), tables[0].mode_normalizers, meta, f,
                           mode="canonical", table_root_extent=2.0)
     l2, linf = rel_errors(got[0].cpu().numpy(), ref[0])
     assert l2 <= TOL and linf <= TOL, (l2, linf)
+    sp, _ = w.eval_direct_plan(*args, torch.as_tensor(f, device="cuda"), form="split")
+    l2, linf = rel_errors(sp[0].cpu().numpy(), ref)
+    assert l2 <= 1e-10 and linf <= 1e-10, (l2, linf)
 
 
 def test_single_precision_wrangler_rounds_fp64_result(cuda):

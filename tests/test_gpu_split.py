@@ -170,7 +170,8 @@ def test_split_total_matches_helmholtz_tables(cuda, dim, level, k, qs, tol):
 @pytest.mark.parametrize("dim,q,kind,k,n_rhs", [(2, 4, "uniform", 10.0, 3),
                                                 (2, 5, "adaptive", 6.0 + 0.5j, 2),
                                                 (3, 3, "uniform", 5.0, 5),
-                                                (3, 5, "uniform", 2.0 + 0.5j, 1)])
+                                                (3, 5, "uniform", 2.0 + 0.5j, 1),
+                                                (3, 6, "uniform", 3.0, 2)])
 def test_fused_split_plan_matches_two_launches(cuda, form, tol, dim, q, kind, k, n_rhs):
     """The split folded into the plan's (case, level) blocks equals the table
     part plus the point-to-point correction."""
@@ -190,10 +191,10 @@ def test_fused_split_plan_matches_two_launches(cuda, form, tol, dim, q, kind, k,
 
 
 def test_fused_split_falls_back_beyond_its_limits(cuda):
-    """3D q=6 (432 block rows) and 2D permuted node ids take the two-launch
-    route; the result is the same sum."""
+    """2D permuted node ids take the two-launch route (the log term pairs
+    nmlz[sid] with the sid-th particle); the result is the same sum."""
     print("case: fused split fallbacks", flush=True)
-    for tr in (vtree.uniform_tree(3, 6, 1), vtree.uniform_tree(2, 3, 3)):
+    for tr in (vtree.uniform_tree(2, 3, 3),):
         if tr.dim == 2:
             rng = np.random.default_rng(4)
             n = tr.n_q_points

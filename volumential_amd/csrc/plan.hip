@@ -42,6 +42,7 @@ typedef __attribute__((ext_vector_type(4))) double doublex4;
 
 constexpr int kCols = 64;      // output columns per tile (boxes x RHS)
 constexpr int kMaxSlices = 6;
+constexpr int kMaxTileRows = 224;  // block rows per tile (row groups beyond)
 
 static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -51,8 +52,11 @@ static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 // [[Re T, -Im T], [Im T, Re T]] acting on [Re f; Im f] and giving
 // [Re y; Im y], so both contraction kernels run unchanged on complex data
 // (8 real flops per complex multiply-add, as the complex product itself).
+// Blocks with more rows than a tile takes are cut into n_rg row groups of
+// rg_size rows (all K columns each): plan block b is row group b % n_rg of
+// (case, level) block b / n_rg.
 __global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int sp_mask, int MP, int KP,
-                                     int n_blocks,
+                                     int n_blocks, int n_rg, int rg_size,
                                      const int32_t* __restrict__ blk_case,
                                      const int32_t* __restrict__ blk_lev,
                                      const uint8_t* __restrict__ perm,
@@ -66,11 +70,12 @@ __global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int sp_mas
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int s = (int)(i % KP);
-    const int t = (int)((i / KP) % MP);
-    const int b = (int)(i / ((int64_t)KP * MP));
+    const int tl = (int)((i / KP) % MP);
+    const int bg = (int)(i / ((int64_t)KP * MP));
+    const int b = bg / n_rg, t = (bg % n_rg) * rg_size + tl;  // (case, level) block, its row
     double v = 0.0;
     const int rows = cplx ? 2 * n : n;
-    if (s < rows && t < rows) {
+    if (s < rows && tl < rg_size && t < rows) {
       const int c = blk_case[b];
       const int ss = s % n, tt = t % n;
       const int64_t rel = compact_rel_row(dim, q, perm + c * dim, sign + c * dim, group + c * dim, ss, tt);
@@ -104,6 +109,7 @@ __global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int sp_mas
 // table (signed entries, sp_mask) and R(|d|) becomes deriv_sign d_a R'(r)/r.
 __global__ void expand_split_blocks_kernel(int dim, int q, int n, int sp_mask, int deriv_axis,
                                            double deriv_sign, int MP, int KP, int n_blocks,
+                                           int n_rg, int rg_size,
                                            const int32_t* __restrict__ blk_case,
                                            const double* __restrict__ blk_alpha,
                                            const double* __restrict__ blk_disp,
@@ -124,10 +130,11 @@ __global__ void expand_split_blocks_kernel(int dim, int q, int n, int sp_mask, i
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int s = (int)(i % KP);
-    const int t = (int)((i / KP) % MP);
-    const int b = (int)(i / ((int64_t)KP * MP));
+    const int tl = (int)((i / KP) % MP);
+    const int bg = (int)(i / ((int64_t)KP * MP));
+    const int b = bg / n_rg, t = (bg % n_rg) * rg_size + tl;  // (case, level) block, its row
     double v = 0.0;
-    if (s < 2 * n && t < 2 * n) {
+    if (s < 2 * n && tl < rg_size && t < 2 * n) {
       const int c = blk_case[b];
       const int ss = s % n, tt = t % n;
       const int64_t rel =
@@ -227,9 +234,9 @@ __global__ void frag_f64_kernel(int MP, int KP, int n_blocks, const double* __re
 
 // Densities -> per (box, rhs) column: BF16 slices X[box][j][r][k] (mode
 // order, zero padded to KP), column scale 2^(e - 7) and, for the 2D log term,
-// nm[box][r] = sum_sid nmlz[sid] f[sbeg + sid].  Half a wavefront per column
-// (KP <= 256): each lane owns 8 consecutive k, so every slice goes out as one
-// 16-byte store per lane.
+// nm[box][r] = sum_sid nmlz[sid] f[sbeg + sid].  Half a wavefront per column:
+// each lane owns 8 consecutive k of every 256-wide window, so every slice
+// goes out as one 16-byte store per lane.
 constexpr int kColK = 8;
 __global__ void split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, int S,
                                   const double* __restrict__ f, int64_t ldf,
@@ -244,18 +251,29 @@ __global__ void split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, in
   const int box = live ? col / R : 0, r = live ? col % R : 0;
   const double* fr = f + (int64_t)r * ldf;
   const int rows = cplx ? 2 * n : n;
+  constexpr int kWin = 32 * kColK;  // k per window
   const int s0 = kColK * hl;
+  // component s of the column: [Re f; Im f] if complex, zero past the rows
+  auto load = [&](int w, double (&v)[kColK]) {
+#pragma unroll
+    for (int u = 0; u < kColK; ++u) {
+      const int s = w + s0 + u;
+      v[u] = 0.0;
+      if (live && s < rows) {
+        const int pt = mode_point[(int64_t)box * n + s % n];
+        v[u] = cplx ? fr[2 * pt + s / n] : fr[pt];
+      }
+    }
+  };
   double v[kColK];
   double mx = 0.0, dot = 0.0;
+  for (int w = 0; w < KP; w += kWin) {
+    load(w, v);
 #pragma unroll
-  for (int u = 0; u < kColK; ++u) {
-    const int s = s0 + u;
-    v[u] = 0.0;
-    if (live && s < rows) {  // component s of the column: [Re f; Im f] if complex
-      const int pt = mode_point[(int64_t)box * n + s % n];
-      v[u] = cplx ? fr[2 * pt + s / n] : fr[pt];
+    for (int u = 0; u < kColK; ++u) {
+      const int s = w + s0 + u;
       mx = fmax(mx, fabs(v[u]));
-      if (log_disp && s < n) dot += nmlz[s] * fr[box_starts[box] + s];
+      if (live && log_disp && s < n) dot += nmlz[s] * fr[box_starts[box] + s];
     }
   }
   for (int o = 16; o > 0; o >>= 1) {  // within the half wavefront
@@ -269,20 +287,23 @@ __global__ void split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, in
     colscale[col] = mx > 0.0 ? ldexp(1.0, e - 7) : 0.0;
     nmdot[col] = dot;
   }
-  if (s0 >= KP) return;
+  for (int w0 = 0; w0 < KP; w0 += kWin) {
+    if (w0 + s0 >= KP) break;
+    if (KP > kWin) load(w0, v);  // one window: v still holds it
 #pragma unroll
-  for (int u = 0; u < kColK; ++u) v[u] = mx > 0.0 ? ldexp(v[u], -e) : 0.0;
-  for (int j = 0; j < S; ++j) {
-    uint32_t w[kColK / 2];
+    for (int u = 0; u < kColK; ++u) v[u] = mx > 0.0 ? ldexp(v[u], -e) : 0.0;
+    for (int j = 0; j < S; ++j) {
+      uint32_t w[kColK / 2];
 #pragma unroll
-    for (int u = 0; u < kColK; ++u) {
-      const double d = rint(ldexp(v[u], 7 + 8 * j));
-      v[u] -= ldexp(d, -7 - 8 * j);
-      const uint32_t bits = __builtin_bit_cast(uint16_t, (__bf16)(float)ldexp(d, -8 * j));
-      w[u / 2] = (u % 2) ? (w[u / 2] | (bits << 16)) : bits;
+      for (int u = 0; u < kColK; ++u) {
+        const double d = rint(ldexp(v[u], 7 + 8 * j));
+        v[u] -= ldexp(d, -7 - 8 * j);
+        const uint32_t bits = __builtin_bit_cast(uint16_t, (__bf16)(float)ldexp(d, -8 * j));
+        w[u / 2] = (u % 2) ? (w[u / 2] | (bits << 16)) : bits;
+      }
+      *reinterpret_cast<uint4*>(X + (((int64_t)box * S + j) * R + r) * KP + w0 + s0) =
+          make_uint4(w[0], w[1], w[2], w[3]);
     }
-    *reinterpret_cast<uint4*>(X + (((int64_t)box * S + j) * R + r) * KP + s0) =
-        make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
@@ -315,6 +336,7 @@ __global__ void gather_cols_f64_kernel(int n, int cplx, int KP, int nboxes, int 
 
 struct TileParams {
   int n, MP, KP, S;      // n: block rows in use (2 x modes for complex data)
+  int n_rg, rg_size;     // row groups per block and their rows: tile t holds group t % n_rg
   int nm, cplx;          // modes per box; complex data (interleaved re, im)
   int B, Rc, R;          // boxes per tile, RHS per tile, total RHS in the staged arrays
   int r0;                // first RHS of this launch
@@ -659,7 +681,8 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = 16 * rt16 + 4 * lg + e;
-      if (row >= p.n) continue;
+      const int grow = (tile % p.n_rg) * p.rg_size + row;  // row of the whole block
+      if (row >= p.rg_size || grow >= p.n) continue;
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
         const int c = 16 * ct + l16;
@@ -667,7 +690,7 @@ __global__ void __launch_bounds__(512) split_bf16_kernel(const TileParams p) {
         const int b = c / p.Rc, r = c % p.Rc;
         const int tb = p.tile_boxes[tile * p.B + b];
         if (tb < 0) continue;
-        p.y[(int64_t)(p.r0 + r) * p.ldy + out_index(p, tb, row)] = *yslot(rt16, ct, e);
+        p.y[(int64_t)(p.r0 + r) * p.ldy + out_index(p, tb, grow)] = *yslot(rt16, ct, e);
       }
     }
   }
@@ -864,7 +887,8 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = 16 * rt16 + lg + 4 * e;
-      if (row >= p.n) continue;
+      const int grow = (tile % p.n_rg) * p.rg_size + row;  // row of the whole block
+      if (row >= p.rg_size || grow >= p.n) continue;
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
         const int c = 16 * ct + l16;
@@ -872,7 +896,7 @@ __global__ void __launch_bounds__(512) dense_f64_kernel(const TileParams p) {
         const int b = c / p.Rc, r = c % p.Rc;
         const int tb = p.tile_boxes[tile * p.B + b];
         if (tb < 0) continue;
-        p.y[(int64_t)(p.r0 + r) * p.ldy + out_index(p, tb, row)] = yacc[rt][ct][e];
+        p.y[(int64_t)(p.r0 + r) * p.ldy + out_index(p, tb, grow)] = yacc[rt][ct][e];
       }
     }
   }
@@ -941,7 +965,9 @@ struct vl1_plan {
   std::vector<int32_t> tgt_boxes, pair_tgt, pair_src, pair_block;
   std::vector<uint64_t> tgt_morton;
   std::vector<double> pair_alpha, pair_disp;
-  int n_blocks = 0;
+  int n_blocks = 0;  // plan blocks: n_keys (case, level) blocks x n_rg row groups
+  int n_keys = 0, n_rg = 1, rg_size = 0;
+  bool fused_split = false;  // Helmholtz split blocks (table part + remainder)
   std::vector<int32_t> blk_case, blk_lev;
   std::vector<double> blk_alpha, blk_disp, blk_h;  // Helmholtz split blocks only
   // device
@@ -998,25 +1024,31 @@ static int build_schedule(vl1_plan* pl, int Rc, bool morton_only, TileSchedule**
   auto sch = new TileSchedule();
   sch->B = B;
   sch->Rc = Rc;
-  sch->n_tiles = (pl->n_tgt + B - 1) / B;
+  // one tile per (group of B target boxes, row group); the row groups of a
+  // box group are consecutive tiles (same XCD: they share density images)
+  const int n_box_tiles = (pl->n_tgt + B - 1) / B;
+  const int G = pl->n_rg;
+  sch->n_tiles = n_box_tiles * G;
   std::vector<int32_t> tss(sch->n_tiles + 1, 0), tboxes((size_t)sch->n_tiles * B, -1);
   std::vector<int32_t> sblk, ssrc;
   std::vector<double> salpha, sdisp;
-  for (int t = 0; t < sch->n_tiles; ++t) {
-    std::vector<int> blocks;
+  for (int bt = 0; bt < n_box_tiles; ++bt) {
+    std::vector<int> blocks;  // (case, level) keys
+    std::vector<int32_t> boxes(B, -1);
     for (int b = 0; b < B; ++b) {
-      const int idx = t * B + b;
+      const int idx = bt * B + b;
       if (idx >= pl->n_tgt) break;
       const int tg = order[idx];
-      tboxes[(size_t)t * B + b] = pl->tgt_boxes[tg];
+      boxes[b] = pl->tgt_boxes[tg];
       for (auto& pr : per_tgt[tg]) blocks.push_back(pr.first);
     }
     std::sort(blocks.begin(), blocks.end());
     blocks.erase(std::unique(blocks.begin(), blocks.end()), blocks.end());
+    std::vector<int32_t> src_l;
+    std::vector<double> al_l, ds_l;
     for (int blk : blocks) {
-      sblk.push_back(blk);
       for (int b = 0; b < B; ++b) {
-        const int idx = t * B + b;
+        const int idx = bt * B + b;
         int src = -1;
         double al = 0.0, ds = 0.0;
         if (idx < pl->n_tgt) {
@@ -1028,12 +1060,20 @@ static int build_schedule(vl1_plan* pl, int Rc, bool morton_only, TileSchedule**
             ds = pl->pair_disp[f->second];
           }
         }
-        ssrc.push_back(src);
-        salpha.push_back(al);
-        sdisp.push_back(ds);
+        src_l.push_back(src);
+        al_l.push_back(al);
+        ds_l.push_back(ds);
       }
     }
-    tss[t + 1] = (int32_t)sblk.size();
+    for (int g = 0; g < G; ++g) {
+      const int t = bt * G + g;
+      for (int b = 0; b < B; ++b) tboxes[(size_t)t * B + b] = boxes[b];
+      for (int blk : blocks) sblk.push_back(blk * G + g);  // plan block of this row group
+      ssrc.insert(ssrc.end(), src_l.begin(), src_l.end());
+      salpha.insert(salpha.end(), al_l.begin(), al_l.end());
+      sdisp.insert(sdisp.end(), ds_l.begin(), ds_l.end());
+      tss[t + 1] = (int32_t)sblk.size();
+    }
   }
   sch->n_steps = (int64_t)sblk.size();
   hipError_t e = upload(sch->tile_step_start, tss.data(), tss.size() * 4, s);
@@ -1104,21 +1144,20 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
   pl->payload = pay;
   pl->dim = dim;
   pl->n = n;
+  pl->fused_split = split;
   pl->cplx = (pay->is_complex || split) ? 1 : 0;
   pl->rows = pl->cplx ? 2 * n : n;
-  pl->MP = round_up(pl->rows, 32);
+  // tiles take at most kMaxTileRows block rows (the split form's LDS budget
+  // at 5 slices): larger blocks are cut into row groups, one tile per group
+  pl->n_rg = (pl->rows + kMaxTileRows - 1) / kMaxTileRows;
+  pl->rg_size = (pl->rows + pl->n_rg - 1) / pl->n_rg;
+  pl->MP = round_up(pl->rg_size, 32);
   pl->KP = round_up(pl->rows, 32);
   pl->nboxes = d->n_boxes;
   pl->n_tgt = d->n_tgt_boxes;
   pl->n_points = d->n_points;
   const bool want_log = d->scaling_mode == VL1_SCALING_CANONICAL && d->log_displacement;
   pl->log_disp = (want_log && !split) ? 1 : 0;  // split blocks carry the term themselves
-  if (pl->MP / 32 > 8) {
-    const bool c = pl->cplx;
-    delete pl;
-    return fail(VL1_E_UNSUPPORTED, c ? "vl1_plan_create: more than 128 nodes per box (complex)"
-                                     : "vl1_plan_create: more than 256 nodes per box");
-  }
   if (pl->cplx && pl->log_disp) {
     delete pl;
     return fail(VL1_E_UNSUPPORTED, "vl1_plan_create: the 2D log displacement applies to real tables only");
@@ -1258,7 +1297,8 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
       pl->pair_disp.push_back(split ? 0.0 : disp);
     }
   }
-  pl->n_blocks = (int)pl->blk_case.size();
+  pl->n_keys = (int)pl->blk_case.size();
+  pl->n_blocks = pl->n_keys * pl->n_rg;
 
   hipError_t e = upload(pl->mode_point, mode_point.data(), mode_point.size() * 4, s);
   if (e == hipSuccess) {
@@ -1296,7 +1336,8 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
         const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
         hipLaunchKernelGGL(expand_split_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n,
                            pay->sign_mask, deriv ? sp->deriv_axis : -1,
-                           sp->deriv_kind == VL1_SPLIT_DERIV_SOURCE ? -1.0 : 1.0, pl->MP, pl->KP, pl->n_blocks, dcase.as<int32_t>(), dal.as<double>(),
+                           sp->deriv_kind == VL1_SPLIT_DERIV_SOURCE ? -1.0 : 1.0, pl->MP, pl->KP,
+                           pl->n_blocks, pl->n_rg, pl->rg_size, dcase.as<int32_t>(), dal.as<double>(),
                            ddi.as<double>(), dh.as<double>(), dvec.as<int32_t>(),
                            dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
                            pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(),
@@ -1309,7 +1350,8 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
       const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
       const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
       hipLaunchKernelGGL(expand_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n, pl->cplx,
-                         pay->sign_mask, pl->MP, pl->KP, pl->n_blocks, dcase.as<int32_t>(), dlev.as<int32_t>(),
+                         pay->sign_mask, pl->MP, pl->KP, pl->n_blocks, pl->n_rg, pl->rg_size,
+                         dcase.as<int32_t>(), dlev.as<int32_t>(),
                          dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
                          pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(), pay->table.as<double>(),
                          pay->n_entries, pl->M.as<double>());
@@ -1384,6 +1426,13 @@ static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts*
   if (form == VL1_FORM_SPLIT_BF16) {
     S = opts ? opts->n_slices : 5;
     if (S < 1 || S > kMaxSlices) return fail(VL1_E_ARG, "vl1_plan: n_slices must be 1..6");
+    // fused split blocks hold the near-singular table part and the smooth
+    // remainder in one row; beyond 256 columns (3D q >= 6, complex) the
+    // split form's error, relative to each row's largest entry, exceeds
+    // 1e-10 of the result at the targets: the FP64 form evaluates these
+    if (pl->fused_split && pl->KP > 256)
+      return fail(VL1_E_UNSUPPORTED,
+                  "vl1_plan: fused Helmholtz split blocks beyond 256 columns take the FP64 form");
     if (split_lds_bytes(S, pl->MP) > 160 * 1024)
       return fail(VL1_E_UNSUPPORTED, "vl1_plan: too many slices for the box size (LDS)");
     int rc = ensure_A(pl, S, s);
@@ -1460,6 +1509,8 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
   hipError_t e;
   TileParams prm{};
   prm.n = pl->rows;
+  prm.n_rg = pl->n_rg;
+  prm.rg_size = pl->rg_size;
   prm.nm = pl->n;
   prm.cplx = pl->cplx;
   prm.MP = pl->MP;
@@ -1592,7 +1643,7 @@ int vl1_plan_stats(const vl1_plan* pl, int64_t* out5) {
   }
   out5[1] = tiles;
   out5[2] = steps;
-  out5[3] = pl->n_blocks;
+  out5[3] = pl->n_keys;
   out5[4] = (int64_t)(pl->M.bytes + pl->MF.bytes + pl->A.bytes + pl->rowscale.bytes + pl->X.bytes + pl->XF.bytes +
                       pl->colscale.bytes + pl->nmdot.bytes + pl->mode_point.bytes);
   return VL1_OK;

@@ -472,10 +472,9 @@ class NearFieldWrangler:
         (``volume_fmm.py:983-1026``): the Laplace-table part of
         ``mode_coefs[R, n_points]`` plus the correction with strengths
         ``mode_coefs * tree.weights``.  ``fused``: both in one plan
-        contraction (``form`` "f64" or "split") where the plan takes the box
-        size (2 q^d <= 256; 2D: node ids in mode order); otherwise, or with
-        ``fused=False``, the table part (plan ``form``) plus the
-        point-to-point correction."""
+        contraction (``form`` "f64" or "split") where the plan takes it (2D:
+        node ids in mode order); otherwise, or with ``fused=False``, the
+        table part (plan ``form``) plus the point-to-point correction."""
         if not self.helmholtz_split:
             raise RuntimeError("eval_direct_split needs helmholtz_split enabled")
         mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
@@ -485,8 +484,8 @@ class NearFieldWrangler:
         mc2 = (mc.unsqueeze(0) if one else mc).contiguous()
         if fused and all(len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels):
             # one contraction: complex blocks = table part + remainder quadrature
-            # the split form's LDS budget can exclude 256-row blocks: the
-            # fused FP64 form (more accurate) then stands in before two launches
+            # a split-form limit (slices vs the LDS budget) lets the fused
+            # FP64 form (more accurate) stand in before two launches
             for fform in ((form,) if form == "f64" else (form, "f64")):
                 pot = self.output_zeros(int(mc2.shape[0]))
                 try:

@@ -234,45 +234,47 @@ __global__ void frag_f64_kernel(int MP, int KP, int n_blocks, const double* __re
 
 // Densities -> per (box, rhs) column: BF16 slices X[box][j][r][k] (mode
 // order, zero padded to KP), column scale 2^(e - 7) and, for the 2D log term,
-// nm[box][r] = sum_sid nmlz[sid] f[sbeg + sid].  Half a wavefront per column:
-// each lane owns 8 consecutive k of every 256-wide window, so every slice
-// goes out as one 16-byte store per lane.
+// nm[box][r] = sum_sid nmlz[sid] f[sbeg + sid].  Half a wavefront per column,
+// 256-wide windows of k: the lanes load the window coalesced (lane l holds
+// k = l + 32 u), pass it through LDS, and then each lane owns 8 consecutive
+// k, so every slice goes out as one 16-byte store per lane.
 constexpr int kColK = 8;
-__global__ void split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, int S,
-                                  const double* __restrict__ f, int64_t ldf,
-                                  const int32_t* __restrict__ mode_point,
-                                  const int32_t* __restrict__ box_starts,
-                                  const double* __restrict__ nmlz, int log_disp,
-                                  __bf16* __restrict__ X, double* __restrict__ colscale,
-                                  double* __restrict__ nmdot) {
+constexpr int kColBlock = 256;  // threads per block: 8 columns
+__global__ void __launch_bounds__(kColBlock) split_cols_kernel(
+    int n, int cplx, int KP, int nboxes, int R, int S, const double* __restrict__ f, int64_t ldf,
+    const int32_t* __restrict__ mode_point, const int32_t* __restrict__ box_starts,
+    const double* __restrict__ nmlz, int log_disp, __bf16* __restrict__ X,
+    double* __restrict__ colscale, double* __restrict__ nmdot) {
+  constexpr int kWin = 32 * kColK;  // k per window
+  __shared__ __align__(16) double win[kColBlock / 32][kWin];
   const int col = (blockIdx.x * blockDim.x + threadIdx.x) / 32;
   const int hl = threadIdx.x % 32;  // lane within the half wavefront
+  double* wrow = win[threadIdx.x / 32];
   const bool live = col < nboxes * R;
   const int box = live ? col / R : 0, r = live ? col % R : 0;
   const double* fr = f + (int64_t)r * ldf;
   const int rows = cplx ? 2 * n : n;
-  constexpr int kWin = 32 * kColK;  // k per window
-  const int s0 = kColK * hl;
-  // component s of the column: [Re f; Im f] if complex, zero past the rows
-  auto load = [&](int w, double (&v)[kColK]) {
+  // window w, coalesced: c[u] = component w + hl + 32 u of the column
+  // ([Re f; Im f] if complex, zero past the rows)
+  auto load = [&](int w, double (&c)[kColK]) {
 #pragma unroll
     for (int u = 0; u < kColK; ++u) {
-      const int s = w + s0 + u;
-      v[u] = 0.0;
+      const int s = w + hl + 32 * u;
+      c[u] = 0.0;
       if (live && s < rows) {
         const int pt = mode_point[(int64_t)box * n + s % n];
-        v[u] = cplx ? fr[2 * pt + s / n] : fr[pt];
+        c[u] = cplx ? fr[2 * pt + s / n] : fr[pt];
       }
     }
   };
-  double v[kColK];
+  double c[kColK];
   double mx = 0.0, dot = 0.0;
   for (int w = 0; w < KP; w += kWin) {
-    load(w, v);
+    load(w, c);
 #pragma unroll
     for (int u = 0; u < kColK; ++u) {
-      const int s = w + s0 + u;
-      mx = fmax(mx, fabs(v[u]));
+      const int s = w + hl + 32 * u;
+      mx = fmax(mx, fabs(c[u]));
       if (live && log_disp && s < n) dot += nmlz[s] * fr[box_starts[box] + s];
     }
   }
@@ -280,29 +282,40 @@ __global__ void split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, in
     mx = fmax(mx, __shfl_xor(mx, o, 32));
     dot += __shfl_xor(dot, o, 32);
   }
-  if (!live) return;
   int e = 0;
   if (mx > 0.0) frexp(mx, &e);
-  if (hl == 0) {
+  if (live && hl == 0) {
     colscale[col] = mx > 0.0 ? ldexp(1.0, e - 7) : 0.0;
     nmdot[col] = dot;
   }
+  const int s0 = kColK * hl;
   for (int w0 = 0; w0 < KP; w0 += kWin) {
-    if (w0 + s0 >= KP) break;
-    if (KP > kWin) load(w0, v);  // one window: v still holds it
+    if (KP > kWin) load(w0, c);  // one window: c still holds it
+    __syncthreads();               // the previous window's reads are done
+#pragma unroll
+    for (int u = 0; u < kColK; ++u) wrow[hl + 32 * u] = c[u];
+    __syncthreads();
+    double v[kColK];
+#pragma unroll
+    for (int u = 0; u < kColK; u += 2) {
+      const double2 d2 = *reinterpret_cast<const double2*>(wrow + s0 + u);
+      v[u] = d2.x;
+      v[u + 1] = d2.y;
+    }
+    if (!live || w0 + s0 >= KP) continue;
 #pragma unroll
     for (int u = 0; u < kColK; ++u) v[u] = mx > 0.0 ? ldexp(v[u], -e) : 0.0;
     for (int j = 0; j < S; ++j) {
-      uint32_t w[kColK / 2];
+      uint32_t wd[kColK / 2];
 #pragma unroll
       for (int u = 0; u < kColK; ++u) {
         const double d = rint(ldexp(v[u], 7 + 8 * j));
         v[u] -= ldexp(d, -7 - 8 * j);
         const uint32_t bits = __builtin_bit_cast(uint16_t, (__bf16)(float)ldexp(d, -8 * j));
-        w[u / 2] = (u % 2) ? (w[u / 2] | (bits << 16)) : bits;
+        wd[u / 2] = (u % 2) ? (wd[u / 2] | (bits << 16)) : bits;
       }
       *reinterpret_cast<uint4*>(X + (((int64_t)box * S + j) * R + r) * KP + w0 + s0) =
-          make_uint4(w[0], w[1], w[2], w[3]);
+          make_uint4(wd[0], wd[1], wd[2], wd[3]);
     }
   }
 }
@@ -1468,8 +1481,9 @@ int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_r
     if (e == hipSuccess && pl->colscale.bytes < (size_t)ncolumns * 8)
       e = pl->colscale.alloc((size_t)ncolumns * 8);
     if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
-    // two columns per wavefront
-    hipLaunchKernelGGL(split_cols_kernel, dim3((ncolumns + 2 * wpb - 1) / (2 * wpb)), dim3(64 * wpb), 0, s, pl->n,
+    // two columns per wavefront, kColBlock / 32 columns per block
+    hipLaunchKernelGGL(split_cols_kernel, dim3((ncolumns + kColBlock / 32 - 1) / (kColBlock / 32)),
+                       dim3(kColBlock), 0, s, pl->n,
                        pl->cplx, pl->KP, pl->nboxes, R, S, f, ldf, pl->mode_point.as<int32_t>(),
                        pl->box_starts.as<int32_t>(), nmlz, pl->log_disp, pl->X.as<__bf16>(),
                        pl->colscale.as<double>(), pl->nmdot.as<double>());

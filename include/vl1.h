@@ -252,6 +252,13 @@ int vl1_plan_contract(vl1_plan* plan, int form, int n_rhs, double* y, int64_t ld
 int vl1_plan_tune(vl1_plan* plan, int form, const double* f, int64_t ldf, int n_rhs, double* y,
                   int64_t ldy, const vl1_split_opts* opts, int reps, int32_t* chosen,
                   double* times_ms, void* stream);
+/* Pins the tile-kernel variant (the bits of vl1_plan_tune) for this (form,
+ * slices, RHS per tile) without timing, e.g. to reproduce a tuned setting
+ * (list1_target_batch_size, doc/source/nearfield_symmetry.rst:99-106, is the
+ * reference's hand-set counterpart); variant -1 returns to the default.
+ * VL1_E_UNSUPPORTED for one row tile per wave on blocks of more than 128
+ * rows.  n_slices is ignored for VL1_FORM_F64. */
+int vl1_plan_set_variant(vl1_plan* plan, int form, int n_slices, int n_rhs, int variant);
 /* Plan statistics: [0] list-1 pairs, [1] tiles, [2] (tile, case) steps,
  * [3] distinct (case, level) blocks, [4] device bytes. */
 int vl1_plan_stats(const vl1_plan* plan, int64_t* out5);

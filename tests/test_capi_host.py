@@ -30,6 +30,9 @@ def test_status_mapping():
     with pytest.raises(ValueError):
         _capi.check(L.vl1_duffy_build(None, None))
     assert "null" in _capi.last_error()
+    with pytest.raises(ValueError):
+        _capi.check(L.vl1_plan_set_variant(None, 1, 5, 4, 0))
+    assert "null plan" in _capi.last_error()
 
 
 @pytest.mark.parametrize("dim,q", [(2, 4), (2, 6), (3, 3), (3, 6)])

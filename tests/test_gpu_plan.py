@@ -181,6 +181,38 @@ def test_autotune_keeps_results(cuda, form):
     assert l2 <= tol and linf <= tol, (l2, linf)
 
 
+def test_every_variant_matches_oracle(cuda):
+    """Each tile-kernel variant, pinned with vl1_plan_set_variant, meets the
+    oracle's bar in both forms (the tuner itself only checks its pick); the
+    one-row-tile split kernel (its own product order) agrees with the
+    two-row one at every slice count."""
+    print("case: all variants, adaptive 2D q=6, 4 RHS", flush=True)
+    tr = adaptive(2, 6)
+    tables = synthetic_table_levels(2, 6, n_tables=tr.nlevels, seed=8)
+    f = densities.gaussian_mixture_batch(tr.points, [31, 32, 33, 34])
+    w = NearFieldWrangler(tr, tables)
+    plan = w.get_plan(w.target_kernels[0], tr.target_boxes, tr.neighbor_source_boxes_starts,
+                      tr.neighbor_source_boxes_lists)
+    fd = torch.as_tensor(f, device="cuda").contiguous()
+    ref = _oracle(tr, tables, f, "per_level")
+    for form, tol in (("f64", TOL64), ("split", TOLSPLIT)):
+        for v in range(8):
+            plan.set_variant(v, form=form, n_slices=5, n_rhs=4)
+            got = plan.eval(fd, form=form, n_slices=5).cpu().numpy()
+            l2, linf = rel_errors(got, ref)
+            print(f"form={form} variant={v}: {l2:.2e} {linf:.2e}", flush=True)
+            assert l2 <= tol and linf <= tol, (form, v, l2, linf)
+    for S in range(1, 7):
+        outs = []
+        for v in (0, 1):
+            plan.set_variant(v, form="split", n_slices=S, n_rhs=4)
+            outs.append(plan.eval(fd, form="split", n_slices=S).cpu().numpy())
+        l2, linf = rel_errors(outs[1], outs[0])
+        print(f"slices={S}: one-row vs two-row {l2:.2e} {linf:.2e}", flush=True)
+        assert l2 <= 1e-10 and linf <= 1e-10, (S, l2, linf)
+    plan.set_variant(None, form="split", n_slices=5, n_rhs=4)
+
+
 @pytest.mark.parametrize("form,tol", [("f64", TOL64), ("split", TOLSPLIT)])
 def test_c2_style_adaptive_canonical(cuda, form, tol):
     """A small instance of the adaptive 3D workload: q=5, refined while the

@@ -58,6 +58,7 @@ EXPORTED_SYMBOLS = (
     "vl1_plan_contract",
     "vl1_plan_stats",
     "vl1_plan_tune",
+    "vl1_plan_set_variant",
     "vl1_csr_builder_create",
     "vl1_csr_builder_fill",
     "vl1_csr_builder_destroy",
@@ -250,6 +251,7 @@ def lib():
     L.vl1_plan_stats.argtypes = [vp, C.POINTER(i64)]
     L.vl1_plan_tune.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, i64, C.POINTER(SplitOpts),
                                 C.c_int, C.POINTER(i32), C.POINTER(f64), vp]
+    L.vl1_plan_set_variant.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.vl1_duffy_build.argtypes = [C.POINTER(DuffyArgs), vp]
     L.vl1_split_correction.argtypes = [C.POINTER(SplitArgs), vp]
     L.vl1_csr_builder_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, vp, vp, C.POINTER(i64), vp]

@@ -1690,6 +1690,27 @@ int vl1_plan_tune(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rh
   return VL1_OK;
 }
 
+int vl1_plan_set_variant(vl1_plan* pl, int form, int n_slices, int n_rhs, int variant) {
+  clear_error();
+  if (!pl) return fail(VL1_E_ARG, "vl1_plan_set_variant: null plan");
+  if (form != VL1_FORM_F64 && form != VL1_FORM_SPLIT_BF16)
+    return fail(VL1_E_ARG, "vl1_plan_set_variant: unknown form");
+  if (n_rhs < 1) return fail(VL1_E_ARG, "vl1_plan_set_variant: n_rhs must be >= 1");
+  const int S = form == VL1_FORM_SPLIT_BF16 ? n_slices : 0;
+  if (form == VL1_FORM_SPLIT_BF16 && (S < 1 || S > kMaxSlices))
+    return fail(VL1_E_ARG, "vl1_plan_set_variant: n_slices must be 1..6");
+  if (variant < -1 || variant > 7) return fail(VL1_E_ARG, "vl1_plan_set_variant: variant must be -1..7");
+  const int key = tune_key(form, S, std::min(n_rhs, kCols));
+  if (variant < 0) {
+    pl->tuned.erase(key);
+    return VL1_OK;
+  }
+  if ((variant & kVarOneRow) && pl->MP > 128)
+    return fail(VL1_E_UNSUPPORTED, "vl1_plan_set_variant: one row tile per wave needs blocks of at most 128 rows");
+  pl->tuned[key] = variant;
+  return VL1_OK;
+}
+
 int vl1_plan_stats(const vl1_plan* pl, int64_t* out5) {
   clear_error();
   if (!pl || !out5) return fail(VL1_E_ARG, "vl1_plan_stats: null argument");

@@ -168,6 +168,15 @@ class List1Plan:
         return {"variant": v, "flags": [n for b, n in self.VARIANT_BITS.items() if v & b],
                 "times_ms": {i: float(t) for i, t in enumerate(times) if t >= 0}}
 
+    def set_variant(self, variant, form="split", n_slices=5, n_rhs=64):
+        """Pin the tile-kernel variant (bits as in ``VARIANT_BITS``) used by
+        later evaluations of this form / slice count / RHS count, without
+        timing; ``variant=None`` returns to the default."""
+        v = -1 if variant is None else int(variant)
+        _capi.check(
+            _capi.lib().vl1_plan_set_variant(self.handle, FORMS[form], int(n_slices), int(n_rhs), v),
+            "vl1_plan_set_variant")
+
     def _prep(self, mode_coefs):
         f = mode_coefs if mode_coefs.dim() == 2 else mode_coefs.unsqueeze(0)
         want = torch.complex128 if self.is_complex else torch.float64

@@ -7,13 +7,18 @@ canonical FP64 table (140,496 orbit-reduced values) built on the GPU.
 A "step" is one evaluation of all 64 densities on all targets.
 
 Forms:
-  f64    plain FP64 kernel (vl1_near_field_f64): CSR target boxes, canonical
-         orbit block staged in LDS, lanes over target nodes.
-  split  (when built) Ozaki-split BF16 MFMA batched plan.
+  plain_f64       vl1_near_field_f64: CSR target boxes, one workgroup per
+                  target box, lanes over target nodes, FP64 VALU; table
+                  values from the expanded per-(level, case) blocks.
+  dense_f64_mfma  batched plan, FP64 MFMA (vl1_plan_*, VL1_FORM_F64).
+  split           batched plan, Ozaki split slices on MFMA with FP64 sums
+                  (the headline form).
 
 Prints one JSON line: metric/value (target points x RHS per second, 1e9),
-timing, dtype, roofline of the dominant kernel (HIP events on the stream the
-kernel runs on) and the CPU baseline (C/OpenMP oracle on a bounded sample).
+timing, dtype, the roofline of the dominant kernel (algorithmic flops over
+its time from HIP events on the stream the kernel runs on), the error of
+every form per RHS against the FP64 oracle on sampled target boxes, and the
+CPU baseline (C/OpenMP oracle on bounded samples, R = 1 and 4).
 """
 
 from __future__ import annotations
@@ -219,6 +224,21 @@ def measured_traffic(kernel, level, q, rhs):
         return None, None
 
 
+def measured_counters(kernel, level, q, rhs):
+    """MFMA busy fraction and related counters of `kernel` on the default
+    workload from the committed counter passes (profiles/c3_traffic.json,
+    key "counters"), or None."""
+    if (level, q, rhs) != (5, 6, 64):
+        return None
+    path = os.path.join(ROOT, "profiles", "c3_traffic.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return d["kernels"][kernel].get("counters")
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def host_cpu():
     """The host's CPU model (/proc/cpuinfo) and logical CPU count."""
     model = "unknown"
@@ -233,59 +253,145 @@ def host_cpu():
     return {"cpu_model": model, "nproc": os.cpu_count()}
 
 
-def cpu_baseline(tree, table, f_host_rows, seconds):
-    """C/OpenMP oracle (reference semantics, one evaluation per RHS) on a
-    bounded sample of interior target boxes; returns points x RHS / s."""
+def host_threads():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when set (the CPU share
+    of a GPU host), else every CPU this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def _sub_tree(tree, pos):
+    """Shallow copy of `tree` whose target boxes are the entries `pos` of its
+    target-box list (with their list-1 sources)."""
     import copy
 
+    sub = copy.copy(tree)
+    starts = np.zeros(pos.size + 1, np.int32)
+    lists = []
+    for i, b in enumerate(pos):
+        seg = tree.neighbor_source_boxes_lists[
+            tree.neighbor_source_boxes_starts[b] : tree.neighbor_source_boxes_starts[b + 1]
+        ]
+        lists.append(seg)
+        starts[i + 1] = starts[i] + seg.size
+    sub.target_boxes = np.asarray(tree.target_boxes, np.int32)[pos]
+    sub.neighbor_source_boxes_starts = starts
+    sub.neighbor_source_boxes_lists = np.concatenate(lists).astype(np.int32)
+    return sub
+
+
+def _oracle_eval(tree, table, f_rows, sel, cores, own_csr=True):
+    """C/OpenMP oracle on the target-box list entries `sel` of `tree` for the densities
+    `f_rows` [R, n_pts]; returns (output [R, n_pts], seconds).  With
+    `own_csr` the oracle derives the list-1 sources of `sel` from the box
+    geometry (not timed apart); the timed baseline samples take the tree's
+    lists."""
     from oracle import cpu as ocpu
     from oracle import orbit_ref
 
     meta = orbit_ref.case_metadata(tree.dim, tree.quad_order)
-    data = table.compact_values()[None]
-    nm = table.mode_normalizers[None]
+    sub = _sub_tree(tree, sel)
+    t0 = time.perf_counter()
+    out = ocpu.near_field(sub, table.compact_values()[None], table.mode_normalizers[None], meta,
+                          f_rows, mode="canonical", scaling_power=2,
+                          log_displacement=(tree.dim == 2),
+                          table_root_extent=table.source_box_extent, nthreads=cores,
+                          own_csr=own_csr)
+    return out, time.perf_counter() - t0
+
+
+def cpu_baseline(tree, table, f_host_rows, seconds, rhs_counts=(1,)):
+    """C/OpenMP oracle (reference semantics, one evaluation per RHS) on
+    bounded samples of interior target boxes, one sample per RHS count in
+    `rhs_counts` (each about seconds / len(rhs_counts) of CPU work); returns
+    points x RHS / s at the largest RHS count, the rate of every count, and
+    (key "_samples", not part of the JSON line) the sampled boxes with the
+    oracle's outputs."""
+    from oracle import cpu as ocpu
+
     counts = np.diff(tree.neighbor_source_boxes_starts)
     full = np.flatnonzero(counts == counts.max())
     if full.size < 1024:  # adaptive trees: a seeded sample of all target boxes
         full = np.random.default_rng(0).permutation(counts.size)
-    cores = min(16, os.cpu_count() or 1)
-
-    def run(nb):
-        sub = copy.copy(tree)
-        sel = full[:: max(1, full.size // nb)][:nb]
-        starts = np.zeros(sel.size + 1, np.int32)
-        lists = []
-        for i, b in enumerate(sel):
-            seg = tree.neighbor_source_boxes_lists[
-                tree.neighbor_source_boxes_starts[b] : tree.neighbor_source_boxes_starts[b + 1]
-            ]
-            lists.append(seg)
-            starts[i + 1] = starts[i] + seg.size
-        sub.target_boxes = sel.astype(np.int32)
-        sub.neighbor_source_boxes_starts = starts
-        sub.neighbor_source_boxes_lists = np.concatenate(lists).astype(np.int32)
-        t0 = time.perf_counter()
-        ocpu.near_field(sub, data, nm, meta, f_host_rows, mode="canonical", scaling_power=2,
-                        log_displacement=(tree.dim == 2), table_root_extent=table.source_box_extent,
-                        nthreads=cores)
-        return time.perf_counter() - t0, sel.size
-
+    cores = host_threads()
     ocpu.lib()
-    t, nb = run(max(cores, 8))
-    per_box = t / nb
-    nb2 = int(min(full.size, max(cores, seconds / max(per_box, 1e-9))))
-    t, nb = run(nb2)
-    rhs = f_host_rows.shape[0]
-    value = nb * tree.n_q_points * rhs / t / 1e9
+    rates, samples, notes = {}, [], []
+    budget = seconds / len(rhs_counts)
+    for R_ in rhs_counts:
+        rows = f_host_rows[:R_]
+
+        def pick(nb):
+            return np.asarray(full[:: max(1, full.size // nb)][:nb], np.int32)
+
+        sel = pick(max(cores, 8))
+        _, t = _oracle_eval(tree, table, rows, sel, cores, own_csr=False)
+        per_box = t / sel.size
+        sel = pick(int(min(full.size, max(cores, budget / max(per_box, 1e-9)))))
+        out, t = _oracle_eval(tree, table, rows, sel, cores, own_csr=False)
+        rates[R_] = sel.size * tree.n_q_points * R_ / t / 1e9
+        samples.append((np.asarray(tree.target_boxes, np.int32)[sel], out))
+        notes.append(f"{sel.size} target boxes x {R_} RHS ({t:.1f} s)")
+    top = max(rhs_counts)
     return {
-        "value": value,
+        "value": rates[top],
         "unit": "Gpts*RHS/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{nb} target boxes x {rhs} RHS of the same tree/table "
-                  f"({nb * tree.n_q_points * rhs} target-point evaluations, {t:.1f} s)",
+        "sample": "; ".join(notes) + " of the same tree/table/densities; the oracle "
+                  "evaluates each RHS separately, so the rate is linear in the RHS count",
+        "value_by_rhs": {str(k): v for k, v in rates.items()},
         **host_cpu(),
+        "_samples": samples,
     }
+
+
+def oracle_check_sample(tree, table, f_host, n_boxes=48, seed=7):
+    """The oracle (own list-1 CSR) on a seeded sample of `n_boxes` target
+    boxes for every density of `f_host`; returns [(boxes, output)] for
+    ``error_vs_oracle``."""
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    nt = tree.target_boxes.size
+    pos = np.sort(rng.choice(nt, size=min(n_boxes, nt), replace=False)).astype(np.int32)
+    out, _ = _oracle_eval(tree, table, f_host, pos, host_threads())
+    log(f"oracle check sample: {pos.size} boxes x {f_host.shape[0]} RHS in "
+        f"{time.perf_counter() - t0:.1f} s")
+    return [(np.asarray(tree.target_boxes, np.int32)[pos], out)]
+
+
+def sample_points(tree, sel):
+    """Tree-order point indices of the target boxes `sel`."""
+    return np.concatenate([np.arange(tree.box_target_starts[b],
+                                     tree.box_target_starts[b] + tree.box_target_counts_nonchild[b])
+                           for b in sel])
+
+
+def error_vs_oracle(tree, samples, forms_out):
+    """Per-RHS relative L2 and Linf error of each GPU form on the oracle's
+    sampled target points: {form: {rel_l2_per_rhs_max, rel_linf_per_rhs_max,
+    rhs, points}}.  `samples`: [(boxes, oracle output [R, n_pts])];
+    `forms_out`: {form: device tensor [R_all, n_pts]}."""
+    res = {}
+    for name, y in forms_out.items():
+        l2, li, npt, nr = [], [], 0, 0
+        for sel, ref in samples:
+            idx = sample_points(tree, sel)
+            R_ = ref.shape[0]
+            got = y[:R_, idx].double().cpu().numpy()
+            want = ref[:, idx]
+            d = got - want
+            l2.extend(np.linalg.norm(d, axis=1) / np.linalg.norm(want, axis=1))
+            li.extend(np.abs(d).max(axis=1) / np.abs(want).max(axis=1))
+            npt += idx.size
+            nr = max(nr, R_)
+        res[name] = {"rel_l2_per_rhs_max": float(np.max(l2)), "rel_linf_per_rhs_max": float(np.max(li)),
+                     "rhs": nr, "points": int(npt)}
+    return res
 
 
 def _build_beam_tree(seed):
@@ -353,7 +459,7 @@ def _plan_forms(plan, f, R, n_pts, flops, steps, warmup, slices, complex_data):
     res["split"].update({"slices": slices, "rel_l2_max": float(l2.max()),
                          "rel_linf_max": float(li.max())})
     log(f"split vs f64: rel L2 max {l2.max():.2e}, Linf max {li.max():.2e}")
-    return res
+    return res, ys
 
 
 def _line(metric_value, ms, steps, warmup, dtype, data, config, roofline, forms, extra=None,
@@ -413,10 +519,10 @@ def run_c4(args):
                         device="cuda").contiguous()
     n = 36
     flops = 8.0 * R * fo.n_list1_pairs * n * n
-    forms = _plan_forms(plan, f, R, fo.n_points, flops, args.steps, args.warmup, args.slices, True)
+    forms, _ = _plan_forms(plan, f, R, fo.n_points, flops, args.steps, args.warmup, args.slices, True)
     sp = forms["split"]
     nprod = args.slices * (args.slices + 1) // 2
-    achieved = nprod * flops / (sp["kernel_ms"] * 1e-3) / 1e12
+    achieved = flops / (sp["kernel_ms"] * 1e-3) / 1e12
     return _line(
         sp["value"], sp["ms"], args.steps, args.warmup, "bf16 slices (MFMA) + f64 sums, complex128 I/O",
         "synthetic (seeded Gaussian beams, GPU-built per-level Helmholtz tables)",
@@ -427,7 +533,9 @@ def run_c4(args):
          "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
          "kernel": f"split_bf16_kernel<{args.slices}, {split_rows_per_wave(sp)}>",
          "kernel_ms": sp["kernel_ms"],
-         "flops_per_launch": nprod * flops},
+         "flops_per_launch": flops,
+         "mfma_work": {"slice_products": nprod, "ops_per_launch": nprod * flops,
+                       "frac": nprod * flops / (sp["kernel_ms"] * 1e-3) / 1e12 / PEAK_BF16_TFLOPS}},
         forms,
         {"error_vs_fp64": {"rel_l2_per_rhs_max": sp["rel_l2_max"],
                            "rel_linf_per_rhs_max": sp["rel_linf_max"]},
@@ -495,19 +603,25 @@ def run_c2(args):
     payload = DevicePayload(3, 5, data, nmlz, table.case_indices, table.arithmetic_metadata())
     plan = List1Plan(tr, payload, scaling_mode=_capi.SCALING_CANONICAL, scaling_power=2,
                      table_root_extent=2.0)
-    forms = _plan_forms(plan, f, R, tr.n_points, flops, args.steps, args.warmup, args.slices, False)
+    forms, ys = _plan_forms(plan, f, R, tr.n_points, flops, args.steps, args.warmup, args.slices,
+                            False)
     forms["plain_f64"] = {"ms": ms_plain, "value": tr.n_points * R / (ms_plain * 1e-3) / 1e9,
                           "tflops": flops / (ms_plain * 1e-3) / 1e12}
     sp = forms["split"]
     nprod = args.slices * (args.slices + 1) // 2
-    achieved = nprod * flops / (sp["kernel_ms"] * 1e-3) / 1e12
+    achieved = flops / (sp["kernel_ms"] * 1e-3) / 1e12
     extra = {"error_vs_fp64": {"rel_l2_per_rhs_max": sp["rel_l2_max"],
                                "rel_linf_per_rhs_max": sp["rel_linf_max"]},
              "speedup_vs_plain_f64": ms_plain / sp["ms"],
              "speedup_vs_dense_f64_mfma": forms["f64"]["ms"] / sp["ms"],
              "list1_csr": {"host_s": csr_host_s, "gpu_s": csr_gpu_s, "identical": csr_same}}
     if not args.no_cpu_baseline:
-        extra["cpu_baseline"] = cpu_baseline(tr, table, f.cpu().numpy(), args.cpu_seconds)
+        cb = cpu_baseline(tr, table, f.cpu().numpy(), args.cpu_seconds)
+        cb.pop("_samples")
+        extra["cpu_baseline"] = cb
+    extra["error_vs_oracle"] = error_vs_oracle(
+        tr, oracle_check_sample(tr, table, f.cpu().numpy()),
+        {"plain_f64": y, "dense_f64_mfma": ys["f64"], "split_bf16": ys["split"]})
     return _line(
         sp["value"], sp["ms"], args.steps, args.warmup, "bf16 slices (MFMA) + f64 sums",
         "synthetic (seed-0 8-Gaussian density, GPU-built Laplace table)",
@@ -519,7 +633,9 @@ def run_c2(args):
          "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
          "kernel": f"split_bf16_kernel<{args.slices}, {split_rows_per_wave(sp)}>",
          "kernel_ms": sp["kernel_ms"],
-         "flops_per_launch": nprod * flops},
+         "flops_per_launch": flops,
+         "mfma_work": {"slice_products": nprod, "ops_per_launch": nprod * flops,
+                       "frac": nprod * flops / (sp["kernel_ms"] * 1e-3) / 1e12 / PEAK_BF16_TFLOPS}},
         forms, extra)
 
 
@@ -565,12 +681,18 @@ def run_c1(args):
     payload = DevicePayload(2, 4, data, nmlz, table.case_indices, table.arithmetic_metadata())
     plan = List1Plan(tr, payload, scaling_mode=_capi.SCALING_CANONICAL, scaling_power=2,
                      log_displacement=True, table_root_extent=2.0)
-    forms = _plan_forms(plan, f, 1, tr.n_points, flops, reps, max(args.warmup, 1), args.slices, False)
+    forms, ys = _plan_forms(plan, f, 1, tr.n_points, flops, reps, max(args.warmup, 1), args.slices,
+                            False)
     forms["plain_f64"] = {"ms": ms_plain, "value": tr.n_points / (ms_plain * 1e-3) / 1e9,
                           "gflops": flops / (ms_plain * 1e-3) / 1e9}
     extra = {"speedup_plain_vs_split": forms["split"]["ms"] / ms_plain}
     if not args.no_cpu_baseline:
-        extra["cpu_baseline"] = cpu_baseline(tr, table, f.cpu().numpy(), min(args.cpu_seconds, 5.0))
+        cb = cpu_baseline(tr, table, f.cpu().numpy(), min(args.cpu_seconds, 5.0))
+        cb.pop("_samples")
+        extra["cpu_baseline"] = cb
+    extra["error_vs_oracle"] = error_vs_oracle(
+        tr, oracle_check_sample(tr, table, f.cpu().numpy()),
+        {"plain_f64": y, "dense_f64_mfma": ys["f64"], "split_bf16": ys["split"]})
     return _line(
         forms["plain_f64"]["value"], ms_plain, reps, args.warmup, "f64",
         "synthetic (exp(-|x|^2/0.01) on the nodes, GPU-built Laplace 2D table)",
@@ -619,7 +741,7 @@ def run_hsplit(args):
     ref = w.eval_direct_split(*args_csr, fd, fused=False)[0][0]
     plan = w.get_plan(knl, *args_csr, fused_split=True)
     flops = 8.0 * R * pairs * n * n  # complex MACs, 8 real flops each
-    forms = _plan_forms(plan, fd, R, n_pts, flops, args.steps, args.warmup, args.slices, True)
+    forms, _ = _plan_forms(plan, fd, R, n_pts, flops, args.steps, args.warmup, args.slices, True)
     # the two-launch route: table part (Laplace plan, Re/Im as 2R real rows) + P2P
     stream = torch.cuda.current_stream()
     corr = w._correction(knl)
@@ -664,7 +786,7 @@ def run_hsplit(args):
         import copy
 
         meta = orbit_ref.case_metadata(2, q)
-        cores = min(16, os.cpu_count() or 1)
+        cores = host_threads()
         perm = np.random.default_rng(0).permutation(tr.nboxes)
         f1 = f[:1]
 
@@ -685,7 +807,7 @@ def run_hsplit(args):
             ocpu.near_field(sub, lap.compact_values()[None], lap.mode_normalizers[None], meta,
                             np.concatenate([f1.real, f1.imag]), mode="canonical",
                             scaling_power=2, log_displacement=True, table_root_extent=2.0,
-                            nthreads=cores)
+                            nthreads=cores, own_csr=False)
             S.correction(tr, f1 * np.asarray(tr.weights)[None], k, corr.nmax,
                          target_boxes=sel, starts=starts, lists=lists)
             return time.perf_counter() - t0
@@ -855,15 +977,16 @@ def main():
     if not args.skip_plain:
         runner = F64Runner(tr, table, f, y)
         ms_plain, _ = timed(lambda: runner(sh), args.steps, args.warmup)
-        y_plain = y.clone()
+        y_plain = y
         forms["plain_f64"] = {"ms": ms_plain, "value": n_pts * R / (ms_plain * 1e-3) / 1e9,
                               "tflops": flops / (ms_plain * 1e-3) / 1e12,
+                              "fp64_frac": flops / (ms_plain * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
                               "hbm_gbs_alg": bytes_alg / (ms_plain * 1e-3) / 1e9}
         log(f"plain f64: {ms_plain:.2f} ms/step, {forms['plain_f64']['value']:.3f} Gpts*RHS/s, "
             f"{forms['plain_f64']['tflops']:.2f} TFLOP/s")
         del runner
 
-    # batched plan: dense FP64 MFMA and Ozaki split BF16
+    # batched plan: dense FP64 MFMA and Ozaki split
     from volumential_amd.payload import DevicePayload, prepare_table_data_and_entry_map
     from volumential_amd.plan import FORMS, List1Plan
 
@@ -898,14 +1021,14 @@ def main():
     ms64, k64 = timed(lambda: plan_step("f64", 0, y64), 1 if args.only_split else args.steps,
                       0 if args.only_split else args.warmup)
     forms["dense_f64_mfma"] = {"ms": ms64, "kernel_ms": k64, "value": n_pts * R / (ms64 * 1e-3) / 1e9,
-                               "tflops": flops / (k64 * 1e-3) / 1e12}
+                               "tflops": flops / (k64 * 1e-3) / 1e12,
+                               "fp64_frac": flops / (k64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
     log(f"dense f64: {ms64:.2f} ms/step (contract {k64:.2f} ms, "
         f"{forms['dense_f64_mfma']['tflops']:.2f} TFLOP/s)")
     if y_plain is not None:
         d_plain = (y64 - y_plain).norm(dim=1) / y64.norm(dim=1)
         forms["plain_f64"]["rel_l2_vs_dense_f64"] = float(d_plain.max())
         log(f"plain vs dense f64 max rel L2 {float(d_plain.max()):.2e}")
-        del y_plain
 
     S = args.slices
     ysp = torch.zeros_like(f)
@@ -913,19 +1036,44 @@ def main():
     err_l2 = ((ysp - y64).norm(dim=1) / y64.norm(dim=1)).cpu().numpy()
     err_inf = ((ysp - y64).abs().amax(dim=1) / y64.abs().amax(dim=1)).cpu().numpy()
     nprod = S * (S + 1) // 2
-    mfma_flops = nprod * flops
     forms["split_bf16"] = {"ms": mss, "kernel_ms": ks, "value": n_pts * R / (mss * 1e-3) / 1e9,
                            "slices": S, "slice_products": nprod,
                            "tflops_fp64_equiv": flops / (ks * 1e-3) / 1e12,
-                           "rel_l2_max": float(err_l2.max()), "rel_linf_max": float(err_inf.max())}
-    log(f"split S={S}: {mss:.2f} ms/step (contract {ks:.2f} ms), rel L2 max {err_l2.max():.2e}, "
-        f"Linf max {err_inf.max():.2e}")
+                           "rel_l2_vs_dense_f64_max": float(err_l2.max()),
+                           "rel_linf_vs_dense_f64_max": float(err_inf.max())}
+    log(f"split S={S}: {mss:.2f} ms/step (contract {ks:.2f} ms), vs dense f64: rel L2 max "
+        f"{err_l2.max():.2e}, Linf max {err_inf.max():.2e}")
     value = forms["split_bf16"]["value"]
     ms = mss
-    achieved = mfma_flops / (ks * 1e-3) / 1e12
     forms["split_bf16"]["autotune"] = tuned.get("split")
     kname = f"split_bf16_kernel<{S}, {split_rows_per_wave(forms['split_bf16'])}>"
     traffic, traffic_note = measured_traffic(kname, level, q, R)
+    counters = measured_counters(kname, level, q, R)
+    # roofline of the dominant kernel from the algorithmic work (flops_alg =
+    # 2 R sum_pairs n^2) over its measured time, against the peak of the
+    # unit it runs on (BF16 MFMA); the slice products the split scheme
+    # executes on that unit are reported beside it as mfma_work
+    achieved = flops / (ks * 1e-3) / 1e12
+    mfma_ops = nprod * flops
+
+    # error per RHS against the FP64 oracle (C/OpenMP restatement of the
+    # reference kernel) on seeded samples of target boxes: all R densities on
+    # a small sample, then the CPU-baseline samples (R = 1 and 4)
+    f_host = f.cpu().numpy()
+    samples = oracle_check_sample(tr, table, f_host)
+    cb = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cb = cpu_baseline(tr, table, f_host[:4], args.cpu_seconds, rhs_counts=(1, 4))
+        cb.pop("_samples")  # timing samples (tree's own lists): not used as the check
+        log(f"cpu: {cb}")
+    outs = {"dense_f64_mfma": y64, "split_bf16": ysp}
+    if y_plain is not None:
+        outs["plain_f64"] = y_plain
+    errs = error_vs_oracle(tr, samples, outs)
+    for k_, v_ in errs.items():
+        log(f"{k_} vs oracle: rel L2 {v_['rel_l2_per_rhs_max']:.2e}, rel Linf "
+            f"{v_['rel_linf_per_rhs_max']:.2e} ({v_['rhs']} RHS, {v_['points']} points)")
 
     res = {
         "metric": "list-1 eval Gpts*RHS/s at <=1e-10 rel err; split-MFMA vs FP64 GPU kernel",
@@ -959,11 +1107,18 @@ def main():
             "traffic_source": traffic_note,
             "kernel": kname,
             "kernel_ms": ks,
-            "flops_per_launch": mfma_flops,
+            "flops_per_launch": flops,
+            "flops_note": "algorithmic: 2 R sum_pairs n^2 (FP64-equivalent useful work)",
             "hbm_gbs_alg": bytes_alg / (ks * 1e-3) / 1e9,
+            "mfma_work": {"slice_products": nprod, "ops_per_launch": mfma_ops,
+                          "achieved": mfma_ops / (ks * 1e-3) / 1e12,
+                          "frac": mfma_ops / (ks * 1e-3) / 1e12 / PEAK_BF16_TFLOPS},
+            **({"counters": counters} if counters else {}),
         },
+        "error_vs_oracle": errs,
         "error_vs_fp64": {"rel_l2_per_rhs_max": float(err_l2.max()),
-                          "rel_linf_per_rhs_max": float(err_inf.max())},
+                          "rel_linf_per_rhs_max": float(err_inf.max()), "rhs": R,
+                          "points": int(n_pts), "against": "dense FP64 MFMA form, all points"},
         "speedup_vs_plain_f64": (ms_plain / mss) if ms_plain else None,
         "speedup_vs_dense_f64_mfma": ms64 / mss,
         "autotune": tuned,
@@ -971,12 +1126,8 @@ def main():
     }
     if not args.no_table_build:
         res["table_build"] = table_build_record([(20, 20), (40, 40)])
-    if not args.no_cpu_baseline:
-        rows = min(R, 1)
-        f_host = f[:rows].cpu().numpy()
-        log("cpu baseline ...")
-        res["cpu_baseline"] = cpu_baseline(tr, table, f_host, args.cpu_seconds)
-        log(f"cpu: {res['cpu_baseline']}")
+    if cb is not None:
+        res["cpu_baseline"] = cb
     print(json.dumps(res), flush=True)
 
 

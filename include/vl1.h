@@ -288,9 +288,12 @@ typedef struct vl1_duffy_args {
   double* result;                /* [n_entries] (x2 interleaved if Helmholtz) */
   int deriv_axis;                /* derivative axis of the *_DERIV kinds */
   /* Optional (NULL = one lane per entry): entries sorted so that each chunk
-   * [chunk_start[c], chunk_start[c + 1]) holds at most 64 entries of one
+   * [chunk_start[c], chunk_start[c + 1]) holds 1..64 entries of one
    * target point (same target_points and decomposition); every node's
-   * geometry and kernel value is then computed once per chunk. */
+   * geometry and kernel value is then computed once per chunk.  The
+   * contract is checked on the device: a chunk outside it (more than 64 or
+   * no entries, or reaching past n_entries) writes NaN to the result
+   * entries it names and adds nothing else. */
   const int64_t* chunk_start;    /* device [n_chunks + 1] */
   int64_t n_chunks;
 } vl1_duffy_args;

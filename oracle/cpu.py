@@ -46,9 +46,20 @@ def _p(a):
 
 def near_field(tree, table_data, mode_nmlz, meta, coefs, *, mode, scaling_power=2,
                log_displacement=False, table_root_extent, table_starting_level=0,
-               nthreads=0, bias=1e-10):
-    """Evaluate all densities ``coefs[R, n]``; returns ``[R, n]``."""
+               nthreads=0, bias=1e-10, own_csr=True):
+    """Evaluate all densities ``coefs[R, n]``; returns ``[R, n]``.  With
+    `own_csr` (default) the list-1 CSR of the tree's target boxes is
+    recomputed by geometry (``list1_ref.list1_csr_bruteforce``) rather than
+    taken from the tree builder."""
+    from .list1_ref import list1_csr_bruteforce, tree_groups
+
     L = lib()
+    if own_csr:
+        nstarts, nlists = list1_csr_bruteforce(tree.box_centers, tree.box_levels,
+                                               tree.root_extent, tree.target_boxes,
+                                               box_group=tree_groups(tree))
+    else:
+        nstarts, nlists = tree.neighbor_source_boxes_starts, tree.neighbor_source_boxes_lists
     coefs = np.ascontiguousarray(np.atleast_2d(coefs), dtype=np.float64)
     R, npts = coefs.shape
     table = np.ascontiguousarray(np.atleast_2d(table_data), dtype=np.float64)
@@ -60,8 +71,8 @@ def near_field(tree, table_data, mode_nmlz, meta, coefs, *, mode, scaling_power=
         starts=np.ascontiguousarray(tree.box_target_starts, np.int32),
         counts=np.ascontiguousarray(tree.box_target_counts_nonchild, np.int32),
         tboxes=np.ascontiguousarray(tree.target_boxes, np.int32),
-        nstarts=np.ascontiguousarray(tree.neighbor_source_boxes_starts, np.int32),
-        nlists=np.ascontiguousarray(tree.neighbor_source_boxes_lists, np.int32),
+        nstarts=np.ascontiguousarray(nstarts, np.int32),
+        nlists=np.ascontiguousarray(nlists, np.int32),
         ids=np.ascontiguousarray(tree.local_ids, np.int32),
         cidx=np.ascontiguousarray(meta["case_indices"], np.int64),
         ranks=np.ascontiguousarray(meta["case_orbit_ranks"], np.uint16),

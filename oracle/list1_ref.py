@@ -108,8 +108,61 @@ def near_field(
     return out
 
 
+def list1_csr_bruteforce(box_centers, box_levels, root_extent, target_boxes, chunk=256,
+                         box_group=None):
+    """List 1 of each target leaf by geometry alone: every leaf box whose
+    closed cell touches the target's (a shared face, edge or corner, or the
+    box itself), sorted by box id.  For the 2:1-balanced leaf trees used
+    here this is the list boxtree's traversal produces (meshgen.py:391-444,
+    tree_interactive_build.py:202-470); it is computed from the box centres
+    and levels only, independent of the tree builder's own lists.
+    ``box_group`` (optional, one label per box) keeps boxes of different
+    trees of a forest apart.  Returns (starts int32 [n_targets + 1], lists
+    int32)."""
+    centers = np.asarray(box_centers, dtype=np.float64)
+    h = root_extent / 2.0 ** np.asarray(box_levels, dtype=np.float64)
+    lo = centers.T - 0.5 * h[:, None]
+    hi = centers.T + 0.5 * h[:, None]
+    eps = 1e-9 * root_extent
+    tb = np.asarray(target_boxes, dtype=np.int64)
+    lists = []
+    for c0 in range(0, tb.size, chunk):
+        t = tb[c0:c0 + chunk]
+        touch = np.all((lo[None, :, :] <= hi[t][:, None, :] + eps)
+                       & (hi[None, :, :] >= lo[t][:, None, :] - eps), axis=2)
+        if box_group is not None:
+            g = np.asarray(box_group)
+            touch &= g[None, :] == g[t][:, None]
+        for row in touch:
+            lists.append(np.flatnonzero(row))
+    starts = np.zeros(tb.size + 1, dtype=np.int64)
+    starts[1:] = np.cumsum([len(v) for v in lists])
+    flat = np.concatenate(lists) if lists else np.zeros(0, np.int64)
+    return starts.astype(np.int32), flat.astype(np.int32)
+
+
+def tree_groups(tree):
+    """Tree label of every box of a forest (trees merged side by side, with
+    ``tree_box_offsets``), or None for a single tree."""
+    offs = getattr(tree, "tree_box_offsets", None)
+    if offs is None:
+        return None
+    return np.searchsorted(np.asarray(offs), np.arange(len(tree.box_levels)), side="right") - 1
+
+
 def near_field_from_tree(tree, table_data, mode_nmlz, meta, mode_coefs, *, mode, scaling_power=2,
-                         log_displacement=False, table_root_extent, table_starting_level=0):
+                         log_displacement=False, table_root_extent, table_starting_level=0,
+                         own_csr=True):
+    """The kernel semantics on `tree`.  With `own_csr` (default) the list-1
+    CSR of the tree's target boxes is recomputed by ``list1_csr_bruteforce``
+    instead of taken from the tree builder."""
+    if own_csr:
+        nbr_starts, nbr_lists = list1_csr_bruteforce(tree.box_centers, tree.box_levels,
+                                                     tree.root_extent, tree.target_boxes,
+                                                     box_group=tree_groups(tree))
+    else:
+        nbr_starts = tree.neighbor_source_boxes_starts
+        nbr_lists = tree.neighbor_source_boxes_lists
     return near_field(
         dim=tree.dim,
         q=tree.quad_order,
@@ -118,8 +171,8 @@ def near_field_from_tree(tree, table_data, mode_nmlz, meta, mode_coefs, *, mode,
         box_starts=tree.box_target_starts,
         box_counts=tree.box_target_counts_nonchild,
         target_boxes=tree.target_boxes,
-        nbr_starts=tree.neighbor_source_boxes_starts,
-        nbr_lists=tree.neighbor_source_boxes_lists,
+        nbr_starts=nbr_starts,
+        nbr_lists=nbr_lists,
         source_mode_ids=tree.local_ids,
         target_point_ids=tree.local_ids,
         table_data=table_data,

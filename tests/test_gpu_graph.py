@@ -65,3 +65,45 @@ def test_graph_replay_complex(cuda, form):
                                          for r in range(4)]), device="cuda").contiguous()
 
     _replays(plan, make_f, torch.complex128, form)
+
+
+def test_graph_outlives_regrowth_and_eviction(cuda):
+    """A graph stays valid after the plan's buffers are outgrown (more
+    densities, another slice count, the other form) and after the wrangler
+    drops the plan from its cache: the graph holds the plan, and outgrown
+    buffers are retired rather than freed while the plan lives."""
+    print("case: graph lifetime 3D q=4 level 2 (R=4 S=5 captured; R=128, S=4, f64 eager)",
+          flush=True)
+    tr = vtree.uniform_tree(3, 4, 2)
+    tables = synthetic_table_levels(3, 4, seed=5)
+    w = NearFieldWrangler(tr, tables[0])
+    w.max_cached_plans = 1
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    plan = w.get_plan(w.target_kernels[0], *args)
+
+    def dens(seeds):
+        return torch.as_tensor(densities.gaussian_mixture_batch(tr.points, seeds),
+                               device="cuda").contiguous()
+
+    f = dens(range(4))
+    out = torch.zeros_like(f)
+    graph = plan.capture(f, out, form="split", n_slices=5)
+    plan.eval(dens(range(100, 228)), form="split", n_slices=5)  # grows the staging buffers
+    plan.eval(f.clone(), form="split", n_slices=4)             # a second slice set
+    plan.eval(dens(range(300, 428)), form="f64")                # FP64 staging
+    half = len(tr.target_boxes) // 2
+    st = np.asarray(tr.neighbor_source_boxes_starts)
+    w.get_plan(w.target_kernels[0], np.asarray(tr.target_boxes)[:half], st[:half + 1],
+               np.asarray(tr.neighbor_source_boxes_lists)[:st[half]])  # evicts `plan`
+    assert len(w._plans) == 1
+    del plan
+    for it in range(2):
+        f.copy_(dens(range(10 + 4 * it, 14 + 4 * it)))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        ref = graph.plan.eval(f.clone(), form="split", n_slices=5)
+        torch.cuda.synchronize()
+        err = float((out - ref).abs().max() / ref.abs().max())
+        print(f"replay {it}: {err:.2e}")
+        assert err <= 1e-13

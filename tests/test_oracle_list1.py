@@ -152,3 +152,84 @@ def test_list1_csr_adjacency_bruteforce():
             tr.neighbor_source_boxes_starts[i] : tr.neighbor_source_boxes_starts[i + 1]
         ]
         assert np.array_equal(np.sort(got), expect)
+
+
+def _integer_touch_lists(levels, index):
+    """List 1 from integer cell indices alone: leaves a (level la, index ia)
+    and b touch iff on every axis the closed intervals
+    [ia 2^(L-la), (ia+1) 2^(L-la)] and [ib 2^(L-lb), (ib+1) 2^(L-lb)] on the
+    finest grid L overlap (a second formulation of the geometric test of
+    ``list1_ref.list1_csr_bruteforce``, exact in integers)."""
+    L = int(levels.max())
+    scale = (1 << (L - levels.astype(np.int64)))[:, None]
+    lo = index.astype(np.int64) * scale
+    hi = lo + scale
+    out = []
+    for a in range(levels.size):
+        touch = np.all((lo <= hi[a]) & (hi >= lo[a]), axis=1)
+        out.append(np.flatnonzero(touch))
+    return out
+
+
+def _c2_style_tree(max_level=4):
+    fden = (lambda P: densities.gaussian_mixture(P, 0))
+    return vtree.adaptive_tree_batched(3, 2, vtree.interpolation_refiner(fden, 3, 2, 1e-3),
+                                       min_level=2, max_level=max_level)
+
+
+@pytest.mark.parametrize("kind", ["uniform3", "adaptive3", "c2style", "adaptive2", "forest"])
+def test_list1_csr_matches_oracle_bruteforce(kind):
+    """The tree builder's list-1 CSR (the one every parity test hands to the
+    GPU) equals the oracle's own geometric list 1 and the integer-index
+    formulation, in 3D (uniform level 3, a point-refined adaptive octree and
+    a C2-style density-refined 2:1 octree), 2D, and on a forest of two
+    beam-refined quadtrees (list 1 never crosses trees)."""
+    if kind == "uniform3":
+        tr = vtree.uniform_tree(3, 2, 3)
+    elif kind == "adaptive3":
+        tr = _adaptive(3, 2)
+    elif kind == "c2style":
+        tr = _c2_style_tree()
+        assert tr.nlevels >= 4
+    elif kind == "adaptive2":
+        tr = _adaptive(2, 2)
+    else:
+        trees = [vtree.adaptive_tree_batched(
+            2, 4, vtree.interpolation_refiner(
+                (lambda P, s=s: densities.helmholtz_beams(P, s, k=40.0)), 2, 4, 1e-5,
+                relative_to=4.0), min_level=3, max_level=6) for s in (0, 1)]
+        tr = vtree.Forest(trees)
+    starts, lists = list1_ref.list1_csr_bruteforce(tr.box_centers, tr.box_levels, tr.root_extent,
+                                                   tr.target_boxes,
+                                                   box_group=list1_ref.tree_groups(tr))
+    if kind == "forest":
+        offs = tr.tree_box_offsets
+        ints = [v + offs[i] for i, t in enumerate(tr.trees)
+                for v in _integer_touch_lists(t.leaf_levels, t.leaf_index)]
+    else:
+        ints = _integer_touch_lists(tr.leaf_levels, tr.leaf_index)
+    for i, tb in enumerate(tr.target_boxes):
+        got = np.sort(tr.neighbor_source_boxes_lists[
+            tr.neighbor_source_boxes_starts[i]:tr.neighbor_source_boxes_starts[i + 1]])
+        want = lists[starts[i]:starts[i + 1]]
+        assert np.array_equal(got, want), (kind, i)
+        assert np.array_equal(want, ints[tb]), (kind, i)
+
+
+@pytest.mark.parametrize("dim,level", [(3, 3), (3, 4), (2, 5)])
+def test_oracle_csr_uniform_counts(dim, level):
+    """Closed form on a uniform tree: a box at integer index i has
+    prod_axes (3 - [i_a = 0] - [i_a = 2^L - 1]) list-1 boxes, P = (3 2^L -
+    2)^d pairs in all (94^3 = 830,584 at level 5), and the relation is
+    symmetric."""
+    tr = vtree.uniform_tree(dim, 1, level)
+    starts, lists = list1_ref.list1_csr_bruteforce(tr.box_centers, tr.box_levels, tr.root_extent,
+                                                   tr.target_boxes)
+    m = 2**level
+    edge = (tr.leaf_index == 0) | (tr.leaf_index == m - 1)
+    want = np.prod(3 - edge.astype(int) - (m == 1), axis=1)
+    assert np.array_equal(np.diff(starts), want)
+    assert starts[-1] == (3 * m - 2) ** dim
+    pairs = {(int(t), int(s)) for i, t in enumerate(tr.target_boxes)
+             for s in lists[starts[i]:starts[i + 1]]}
+    assert all((s, t) in pairs for t, s in pairs)

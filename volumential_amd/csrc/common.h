@@ -40,11 +40,22 @@ struct DevBuf {
     ptr = nullptr;
     bytes = 0;
   }
+  // On failure the buffer stays empty (null, 0 bytes) and the runtime's
+  // recorded error is cleared, so a later call neither sees a stale
+  // capacity nor inherits the failed allocation's error from
+  // hipGetLastError().
   hipError_t alloc(size_t n) {
     release();
-    bytes = n;
     if (n == 0) return hipSuccess;
-    return hipMalloc(&ptr, n);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return e;
+    }
+    ptr = p;
+    bytes = n;
+    return hipSuccess;
   }
   template <typename T>
   T* as() const { return static_cast<T*>(ptr); }

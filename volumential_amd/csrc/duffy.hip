@@ -237,7 +237,20 @@ __global__ void __launch_bounds__(64) duffy_seg_kernel(const DuffyParams p) {
   extern __shared__ double rec[];  // [VW * q + (DIM - 1) * q][kSegPitch]
   const int lane = threadIdx.x;
   const int64_t e0 = p.chunk_start[blockIdx.x];
-  const int ne = (int)(p.chunk_start[blockIdx.x + 1] - e0);
+  const int64_t ne64 = p.chunk_start[blockIdx.x + 1] - e0;
+  // A chunk holds 1..64 entries (one lane per entry at least); any other
+  // count breaks the lane layout below.  Such a chunk writes NaN to the
+  // entries it names inside the result (the host's finiteness check then
+  // rejects the table) and returns.
+  if (ne64 < 1 || ne64 > kSegNodes || e0 < 0 || e0 + ne64 > p.n_entries) {
+    if (e0 >= 0 && ne64 > 0)
+      for (int64_t i = lane; i < ne64 && e0 + i < p.n_entries; i += kSegNodes) {
+        p.out[VW * (e0 + i)] = __builtin_nan("");
+        if (VW == 2) p.out[VW * (e0 + i) + 1] = __builtin_nan("");
+      }
+    return;
+  }
+  const int ne = (int)ne64;
   int nep = 1;
   while (nep < ne) nep *= 2;                 // ne' (<= 64)
   const int R = kSegNodes / nep;             // lanes per entry

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <type_traits>
@@ -1029,17 +1030,42 @@ struct vl1_plan {
   std::vector<int32_t> blk_case, blk_lev;
   std::vector<double> blk_alpha, blk_disp, blk_h;  // Helmholtz split blocks only
   // device
-  DevBuf mode_point, box_starts, M, MF, rowscale, A;
-  int A_slices = 0;
+  DevBuf mode_point, box_starts, M, MF, rowscale;
+  // split slices of the blocks, one set per slice count used (a graph
+  // captured at one count keeps reading its own set)
+  std::map<int, DevBuf*> A_by_slices;
   std::map<int, TileSchedule*> schedules;  // by 2 Rc + (Morton-only order)
   std::map<int, int> tuned;                 // variant by (form, slices, Rc), from vl1_plan_tune
   int forced_variant = -1;                  // candidate under test in vl1_plan_tune
-  // staging buffers (grown on demand)
+  // staging buffers (grown on demand).  A buffer that is outgrown is
+  // retired, not freed, until the plan is destroyed: a HIP graph captured
+  // from an earlier evaluation holds its address.
   DevBuf X, colscale, nmdot, XF;
+  std::vector<DevBuf*> retired;
   DevBuf phase;  // int32 [8]
   int staged_form = -1, staged_rhs = 0, staged_slices = -1;
   ~vl1_plan() {
     for (auto& kv : schedules) delete kv.second;
+    for (auto& kv : A_by_slices) delete kv.second;
+    for (DevBuf* b : retired) delete b;
+  }
+  // Ensure `b` holds at least `bytes`; a smaller buffer in use moves to
+  // `retired` first.
+  hipError_t grow(DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes) return hipSuccess;
+    if (b.ptr) {
+      auto old = new DevBuf();
+      old->ptr = b.ptr;
+      old->bytes = b.bytes;
+      b.ptr = nullptr;
+      b.bytes = 0;
+      retired.push_back(old);
+    }
+    return b.alloc(bytes);
+  }
+  const DevBuf* A(int S) const {
+    auto it = A_by_slices.find(S);
+    return it == A_by_slices.end() ? nullptr : it->second;
   }
 };
 
@@ -1150,19 +1176,28 @@ static int build_schedule(vl1_plan* pl, int Rc, bool morton_only, TileSchedule**
 }
 
 static int ensure_A(vl1_plan* pl, int S, hipStream_t s) {
-  if (pl->A_slices == S) return VL1_OK;
-  hipError_t e = pl->A.alloc((size_t)pl->n_blocks * S * pl->MP * pl->KP * sizeof(__bf16));
+  if (pl->A(S)) return VL1_OK;
+  auto A = new DevBuf();
+  hipError_t e = A->alloc((size_t)pl->n_blocks * S * pl->MP * pl->KP * sizeof(__bf16));
   if (e == hipSuccess && pl->rowscale.bytes == 0)
     e = pl->rowscale.alloc((size_t)pl->n_blocks * pl->MP * sizeof(double));
-  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("plan A alloc: ") + hipGetErrorString(e));
+  if (e != hipSuccess) {
+    delete A;
+    return fail(VL1_E_HIP, std::string("plan A alloc: ") + hipGetErrorString(e));
+  }
   const int rows = pl->n_blocks * pl->MP;
   const int wpb = 4;
+  // the row scales do not depend on S: rewriting them leaves earlier slice
+  // sets (and graphs reading them) valid
   hipLaunchKernelGGL(split_rows_kernel, dim3((rows + wpb - 1) / wpb), dim3(64 * wpb), 0, s, pl->MP,
-                     pl->KP, pl->n_blocks, S, pl->M.as<double>(), pl->A.as<__bf16>(),
+                     pl->KP, pl->n_blocks, S, pl->M.as<double>(), A->as<__bf16>(),
                      pl->rowscale.as<double>());
   e = hipGetLastError();
-  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("split_rows: ") + hipGetErrorString(e));
-  pl->A_slices = S;
+  if (e != hipSuccess) {
+    delete A;
+    return fail(VL1_E_HIP, std::string("split_rows: ") + hipGetErrorString(e));
+  }
+  pl->A_by_slices[S] = A;
   return VL1_OK;
 }
 
@@ -1221,6 +1256,25 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
     return fail(VL1_E_UNSUPPORTED, "vl1_plan_create: the 2D log displacement applies to real tables only");
   }
 
+  // CSR and box ids are host arrays from the caller: range-check them
+  // before anything is indexed with them
+  if (d->nbr_starts[0] != 0) {
+    delete pl;
+    return fail(VL1_E_ARG, "vl1_plan_create: neighbor source starts must begin at 0");
+  }
+  for (int i = 0; i < d->n_tgt_boxes; ++i) {
+    const int tb = d->target_boxes[i];
+    if (tb < 0 || tb >= d->n_boxes) {
+      delete pl;
+      return fail(VL1_E_ARG, "vl1_plan_create: target box id " + std::to_string(tb) +
+                                 " out of range [0, " + std::to_string(d->n_boxes) + ")");
+    }
+    if (d->nbr_starts[i + 1] < d->nbr_starts[i]) {
+      delete pl;
+      return fail(VL1_E_ARG, "vl1_plan_create: neighbor source starts must be non-decreasing");
+    }
+  }
+
   // layout: every listed box holds exactly n particles whose local ids are a
   // permutation of 0..n-1 (expansion_wrangler_fpnd.py:427-463)
   std::vector<int32_t> mode_point((size_t)d->n_boxes * n, -1);
@@ -1236,6 +1290,10 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
   }
   for (int b = 0; b < d->n_boxes; ++b) {
     if (!used[b]) continue;
+    if (d->box_levels[b] > 30) {
+      delete pl;
+      return fail(VL1_E_ARG, "vl1_plan_create: box level above 30");
+    }
     if (d->box_counts[b] != n) {
       delete pl;
       return fail(VL1_E_LAYOUT, "table-based near-field evaluation requires exactly " +
@@ -1473,13 +1531,18 @@ static int plan_variant(const vl1_plan* pl, int form, int S, int Rc) {
   return it != pl->tuned.end() ? it->second : default_variant(pl, form);
 }
 
-static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts* opts,
-                        hipStream_t s, int* S_out, TileSchedule** sch_out) {
+// Argument checks shared by every evaluation entry point; *S_out receives
+// the slice count (0 for the FP64 form).
+static int plan_check(const vl1_plan* pl, int form, int n_rhs, const vl1_split_opts* opts,
+                      int* S_out) {
   if (form != VL1_FORM_F64 && form != VL1_FORM_SPLIT_BF16)
     return fail(VL1_E_ARG, "vl1_plan: unknown form");
   if (n_rhs < 1) return fail(VL1_E_ARG, "vl1_plan: n_rhs must be >= 1");
   if (n_rhs > kCols && n_rhs % kCols != 0)
     return fail(VL1_E_UNSUPPORTED, "vl1_plan: RHS count must be <= 64 or a multiple of 64");
+  // one staging column per (box, RHS); the kernels index columns with int
+  if ((int64_t)pl->nboxes * n_rhs > (int64_t)INT32_MAX)
+    return fail(VL1_E_UNSUPPORTED, "vl1_plan: boxes x RHS exceeds 2^31 - 1 columns");
   int S = 0;
   if (form == VL1_FORM_SPLIT_BF16) {
     S = opts ? opts->n_slices : 5;
@@ -1493,7 +1556,20 @@ static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts*
                   "vl1_plan: fused Helmholtz split blocks beyond 256 columns take the FP64 form");
     if (split_lds_bytes(S, pl->MP) > 160 * 1024)
       return fail(VL1_E_UNSUPPORTED, "vl1_plan: too many slices for the box size (LDS)");
-    int rc = ensure_A(pl, S, s);
+  }
+  *S_out = S;
+  return VL1_OK;
+}
+
+// Checks, then builds (allocating) the slice set and tile schedule this
+// evaluation needs if the plan does not hold them yet.
+static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts* opts,
+                        hipStream_t s, int* S_out, TileSchedule** sch_out) {
+  int S = 0;
+  int rc = plan_check(pl, form, n_rhs, opts, &S);
+  if (rc != VL1_OK) return rc;
+  if (form == VL1_FORM_SPLIT_BF16) {
+    rc = ensure_A(pl, S, s);
     if (rc != VL1_OK) return rc;
   }
   *S_out = S;
@@ -1501,69 +1577,62 @@ static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts*
   return build_schedule(pl, Rc, (plan_variant(pl, form, S, Rc) & kVarMorton) != 0, sch_out, s);
 }
 
-int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs,
-                   const vl1_split_opts* opts, void* stream) {
-  clear_error();
-  if (!pl || !f) return fail(VL1_E_ARG, "vl1_plan_stage: null argument");
-  if (ldf < pl->n_points * (pl->cplx ? 2 : 1))
-    return fail(VL1_E_ARG, "vl1_plan_stage: bad leading dimension");
-  if (pl->n_tgt == 0 || pl->n_blocks == 0) return VL1_OK;
-  hipStream_t s = (hipStream_t)stream;
-  int S = 0;
-  TileSchedule* sch = nullptr;
-  int rc = plan_prepare(pl, form, n_rhs, opts, s, &S, &sch);
-  if (rc != VL1_OK) return rc;
-  const vl1_payload* pay = pl->payload;
-  const int R = n_rhs;
-  hipError_t e = hipSuccess;
-  const int ncolumns = pl->nboxes * R;
-  if (pl->nmdot.bytes < (size_t)ncolumns * 8) e = pl->nmdot.alloc((size_t)ncolumns * 8);
-  const int wpb = 4;
-  const double* nmlz = pay->nmlz.as<double>();  // level-0 normalisers
+// Per-evaluation staging: density slices, column scales and the 2D log-term
+// dots (split form), or the gathered FP64 densities and dots (FP64 form).
+struct Staging {
+  __bf16* X = nullptr;
+  double* colscale = nullptr;
+  double* nmdot = nullptr;
+  double* XF = nullptr;
+};
+
+// Bytes of staging for (form, S, R) laid out in one region, each part
+// 256-byte aligned: split X | colscale | nmdot, FP64 XF | nmdot.
+static size_t staging_layout(const vl1_plan* pl, int form, int S, int R, char* base, Staging* st) {
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t cols = (size_t)pl->nboxes * R;
+  size_t off = 0;
+  Staging t;
   if (form == VL1_FORM_SPLIT_BF16) {
-    const size_t xb = (size_t)pl->nboxes * S * R * pl->KP * 2;
-    if (e == hipSuccess && pl->X.bytes < xb) e = pl->X.alloc(xb);
-    if (e == hipSuccess && pl->colscale.bytes < (size_t)ncolumns * 8)
-      e = pl->colscale.alloc((size_t)ncolumns * 8);
-    if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
+    t.X = reinterpret_cast<__bf16*>(base + off);
+    off += al((size_t)pl->nboxes * S * R * pl->KP * 2);
+    t.colscale = reinterpret_cast<double*>(base + off);
+    off += al(cols * 8);
+  } else {
+    t.XF = reinterpret_cast<double*>(base + off);
+    off += al((size_t)pl->nboxes * R * pl->KP * 8);
+  }
+  t.nmdot = reinterpret_cast<double*>(base + off);
+  off += al(cols * 8);
+  if (st) *st = t;
+  return off;
+}
+
+// Launch the staging kernel of `form` into `st`.
+static int stage_launch(vl1_plan* pl, int form, int S, const double* f, int64_t ldf, int R,
+                        const Staging& st, hipStream_t s) {
+  const int ncolumns = pl->nboxes * R;
+  const int wpb = 4;
+  const double* nmlz = pl->payload->nmlz.as<double>();  // level-0 normalisers
+  if (form == VL1_FORM_SPLIT_BF16) {
     // two columns per wavefront, kColBlock / 32 columns per block
     hipLaunchKernelGGL(split_cols_kernel, dim3((ncolumns + kColBlock / 32 - 1) / (kColBlock / 32)),
-                       dim3(kColBlock), 0, s, pl->n,
-                       pl->cplx, pl->KP, pl->nboxes, R, S, f, ldf, pl->mode_point.as<int32_t>(),
-                       pl->box_starts.as<int32_t>(), nmlz, pl->log_disp, pl->X.as<__bf16>(),
-                       pl->colscale.as<double>(), pl->nmdot.as<double>());
+                       dim3(kColBlock), 0, s, pl->n, pl->cplx, pl->KP, pl->nboxes, R, S, f, ldf,
+                       pl->mode_point.as<int32_t>(), pl->box_starts.as<int32_t>(), nmlz,
+                       pl->log_disp, st.X, st.colscale, st.nmdot);
   } else {
-    const size_t xb = (size_t)pl->nboxes * R * pl->KP * 8;
-    if (e == hipSuccess && pl->XF.bytes < xb) e = pl->XF.alloc(xb);
-    if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
     hipLaunchKernelGGL(gather_cols_f64_kernel, dim3((ncolumns + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
                        pl->n, pl->cplx, pl->KP, pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),
-                       pl->box_starts.as<int32_t>(), nmlz, pl->log_disp, pl->XF.as<double>(),
-                       pl->nmdot.as<double>());
+                       pl->box_starts.as<int32_t>(), nmlz, pl->log_disp, st.XF, st.nmdot);
   }
-  e = hipGetLastError();
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("staging launch: ") + hipGetErrorString(e));
-  pl->staged_form = form;
-  pl->staged_rhs = R;
-  pl->staged_slices = S;
   return VL1_OK;
 }
 
-int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
-                      const vl1_split_opts* opts, void* stream) {
-  clear_error();
-  if (!pl || !y) return fail(VL1_E_ARG, "vl1_plan_contract: null argument");
-  if (ldy < pl->n_points * (pl->cplx ? 2 : 1))
-    return fail(VL1_E_ARG, "vl1_plan_contract: bad leading dimension");
-  if (pl->n_tgt == 0 || pl->n_blocks == 0) return VL1_OK;
-  hipStream_t s = (hipStream_t)stream;
-  int S = 0;
-  TileSchedule* sch = nullptr;
-  int rc = plan_prepare(pl, form, n_rhs, opts, s, &S, &sch);
-  if (rc != VL1_OK) return rc;
-  if (pl->staged_form != form || pl->staged_rhs != n_rhs || pl->staged_slices != S)
-    return fail(VL1_E_ARG, "vl1_plan_contract: densities were not staged for this form/RHS/slices");
-  const int R = n_rhs;
+// Launch the tile kernels of `form` reading the staging `st`.
+static int contract_launch(vl1_plan* pl, int form, int S, TileSchedule* sch, int R, double* y,
+                           int64_t ldy, const Staging& st, hipStream_t s) {
   const int Rc = std::min(R, kCols);
   hipError_t e;
   TileParams prm{};
@@ -1586,14 +1655,14 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
   prm.step_alpha = sch->step_alpha.as<double>();
   prm.step_disp = sch->step_disp.as<double>();
   prm.mode_point = pl->mode_point.as<int32_t>();
-  prm.A = pl->A.as<__bf16>();
+  prm.A = form == VL1_FORM_SPLIT_BF16 ? pl->A(S)->as<__bf16>() : nullptr;
   prm.rowscale = pl->rowscale.as<double>();
-  prm.X = pl->X.as<__bf16>();
-  prm.colscale = pl->colscale.as<double>();
+  prm.X = st.X;
+  prm.colscale = st.colscale;
   prm.M = pl->M.as<double>();
   prm.MF = pl->MF.as<double>();
-  prm.XF = pl->XF.as<double>();
-  prm.nmdot = pl->nmdot.as<double>();
+  prm.XF = st.XF;
+  prm.nmdot = st.nmdot;
   prm.y = y;
   prm.ldy = ldy;
   prm.phase = pl->phase.as<int32_t>();
@@ -1625,6 +1694,67 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
     if (e != hipSuccess) return fail(VL1_E_HIP, std::string("tile kernel launch: ") + hipGetErrorString(e));
   }
   return VL1_OK;
+}
+
+int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs,
+                   const vl1_split_opts* opts, void* stream) {
+  clear_error();
+  if (!pl || !f) return fail(VL1_E_ARG, "vl1_plan_stage: null argument");
+  if (ldf < pl->n_points * (pl->cplx ? 2 : 1))
+    return fail(VL1_E_ARG, "vl1_plan_stage: bad leading dimension");
+  if (pl->n_tgt == 0 || pl->n_blocks == 0) return VL1_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int S = 0;
+  TileSchedule* sch = nullptr;
+  int rc = plan_prepare(pl, form, n_rhs, opts, s, &S, &sch);
+  if (rc != VL1_OK) return rc;
+  const int R = n_rhs;
+  const size_t ncolumns = (size_t)pl->nboxes * R;
+  hipError_t e = hipSuccess;
+  // the largest buffer first: a failed allocation then leaves the others
+  // untouched
+  if (form == VL1_FORM_SPLIT_BF16) {
+    e = pl->grow(pl->X, (size_t)pl->nboxes * S * R * pl->KP * 2);
+    if (e == hipSuccess) e = pl->grow(pl->nmdot, ncolumns * 8);
+    if (e == hipSuccess) e = pl->grow(pl->colscale, ncolumns * 8);
+  } else {
+    e = pl->grow(pl->XF, (size_t)pl->nboxes * R * pl->KP * 8);
+    if (e == hipSuccess) e = pl->grow(pl->nmdot, ncolumns * 8);
+  }
+  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
+  Staging st;
+  st.X = pl->X.as<__bf16>();
+  st.colscale = pl->colscale.as<double>();
+  st.nmdot = pl->nmdot.as<double>();
+  st.XF = pl->XF.as<double>();
+  rc = stage_launch(pl, form, S, f, ldf, R, st, s);
+  if (rc != VL1_OK) return rc;
+  pl->staged_form = form;
+  pl->staged_rhs = R;
+  pl->staged_slices = S;
+  return VL1_OK;
+}
+
+int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
+                      const vl1_split_opts* opts, void* stream) {
+  clear_error();
+  if (!pl || !y) return fail(VL1_E_ARG, "vl1_plan_contract: null argument");
+  if (ldy < pl->n_points * (pl->cplx ? 2 : 1))
+    return fail(VL1_E_ARG, "vl1_plan_contract: bad leading dimension");
+  if (pl->n_tgt == 0 || pl->n_blocks == 0) return VL1_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int S = 0;
+  TileSchedule* sch = nullptr;
+  int rc = plan_prepare(pl, form, n_rhs, opts, s, &S, &sch);
+  if (rc != VL1_OK) return rc;
+  if (pl->staged_form != form || pl->staged_rhs != n_rhs || pl->staged_slices != S)
+    return fail(VL1_E_ARG, "vl1_plan_contract: densities were not staged for this form/RHS/slices");
+  Staging st;
+  st.X = pl->X.as<__bf16>();
+  st.colscale = pl->colscale.as<double>();
+  st.nmdot = pl->nmdot.as<double>();
+  st.XF = pl->XF.as<double>();
+  return contract_launch(pl, form, S, sch, n_rhs, y, ldy, st, s);
 }
 
 int vl1_plan_eval(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs, double* y,
@@ -1724,8 +1854,12 @@ int vl1_plan_stats(const vl1_plan* pl, int64_t* out5) {
   out5[1] = tiles;
   out5[2] = steps;
   out5[3] = pl->n_keys;
-  out5[4] = (int64_t)(pl->M.bytes + pl->MF.bytes + pl->A.bytes + pl->rowscale.bytes + pl->X.bytes + pl->XF.bytes +
-                      pl->colscale.bytes + pl->nmdot.bytes + pl->mode_point.bytes);
+  int64_t bytes = (int64_t)(pl->M.bytes + pl->MF.bytes + pl->rowscale.bytes + pl->X.bytes +
+                            pl->XF.bytes + pl->colscale.bytes + pl->nmdot.bytes +
+                            pl->mode_point.bytes);
+  for (auto& kv : pl->A_by_slices) bytes += (int64_t)kv.second->bytes;
+  for (const DevBuf* b : pl->retired) bytes += (int64_t)b->bytes;
+  out5[4] = bytes;
   return VL1_OK;
 }
 

@@ -23,6 +23,19 @@ from . import _capi
 FORMS = {"f64": _capi.FORM_F64, "split": _capi.FORM_SPLIT_BF16}
 
 
+class PlanGraph:
+    """A captured plan evaluation: the HIP graph plus references to the plan
+    and the tensors it reads and writes, which must outlive it."""
+
+    def __init__(self, graph, plan, tensors):
+        self.graph = graph
+        self.plan = plan
+        self.tensors = tensors
+
+    def replay(self):
+        self.graph.replay()
+
+
 class List1Plan:
     def __init__(self, tree, payload, *, scaling_mode, scaling_power=2, log_displacement=False,
                  table_root_extent, table_starting_level=0, case_encoding_bias=1e-10,
@@ -190,16 +203,23 @@ class List1Plan:
 
     def capture(self, mode_coefs: torch.Tensor, out: torch.Tensor, form="split", n_slices=5):
         """Record one evaluation (density staging + tile contraction) of
-        ``mode_coefs`` into ``out`` as a HIP graph and return it; each
-        ``graph.replay()`` re-evaluates whatever ``mode_coefs`` then holds.
-        One eager evaluation runs first: it allocates the staging buffers and
-        uploads the tile schedule, so the captured launches allocate nothing."""
+        ``mode_coefs`` into ``out`` as a HIP graph and return it as a
+        ``PlanGraph``; each ``replay()`` re-evaluates whatever ``mode_coefs``
+        then holds.  One eager evaluation runs first: it allocates the staging
+        buffers and uploads the tile schedule, so the captured launches
+        allocate nothing.
+
+        Lifetime: the graph holds this plan (and the tensors it was captured
+        with), so the plan outlives it.  Later evaluations of the plan with
+        more densities or another slice count allocate new staging / slice
+        buffers; the ones the graph reads are retired, not freed, until the
+        plan is destroyed, so replays stay valid."""
         self.eval(mode_coefs, out=out, form=form, n_slices=n_slices)
         torch.cuda.synchronize()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             self.eval(mode_coefs, out=out, form=form, n_slices=n_slices)
-        return graph
+        return PlanGraph(graph, self, (mode_coefs, out))
 
     def stats(self):
         arr = (_capi.i64 * 5)()
@@ -207,6 +227,7 @@ class List1Plan:
         return dict(zip(["pairs", "tiles", "steps", "blocks", "device_bytes"], list(arr)))
 
     def close(self):
+        """Destroy the plan now (graphs captured from it become invalid)."""
         if getattr(self, "handle", None) is not None and self.handle.value:
             _capi.lib().vl1_plan_destroy(self.handle)
             self.handle = _capi.vp()

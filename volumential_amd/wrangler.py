@@ -406,8 +406,9 @@ class NearFieldWrangler:
         )
         self._plans[key] = plan
         while len(self._plans) > self.max_cached_plans:
-            _, old = self._plans.popitem(last=False)
-            old.close()
+            # dropped from the cache, not destroyed: a captured graph may still
+            # hold the plan (its device memory is freed with the last reference)
+            self._plans.popitem(last=False)
         return plan
 
     def eval_direct_plan(self, target_boxes, neighbor_source_boxes_starts,
@@ -551,8 +552,33 @@ def _host(a):
     return np.asarray(a)
 
 
+# Device-resident CSR arrays: identity key -> int32 bytes, so that a tensor
+# is copied to the host (a stream sync) only the first time it is seen.  An
+# entry holds its tensor, so the address cannot be reused by another tensor
+# while the entry exists; the version counter changes on every in-place
+# write through torch.
+_DEVICE_CSR_BYTES: "OrderedDict[tuple, tuple]" = OrderedDict()
+_DEVICE_CSR_MEMO_SIZE = 32
+
+
+def _csr_bytes(a):
+    if isinstance(a, torch.Tensor) and a.device.type != "cpu":
+        ident = (a.data_ptr(), tuple(a.shape), tuple(a.stride()), str(a.dtype), str(a.device),
+                 a._version)
+        hit = _DEVICE_CSR_BYTES.get(ident)
+        if hit is not None:
+            _DEVICE_CSR_BYTES.move_to_end(ident)
+            return hit[1]
+        b = np.ascontiguousarray(_host(a), dtype=np.int32).tobytes()
+        _DEVICE_CSR_BYTES[ident] = (a, b)
+        while len(_DEVICE_CSR_BYTES) > _DEVICE_CSR_MEMO_SIZE:
+            _DEVICE_CSR_BYTES.popitem(last=False)
+        return b
+    # host arrays: their int32 bytes (no device traffic)
+    return np.ascontiguousarray(_host(a), dtype=np.int32).tobytes()
+
+
 def _csr_key(out_kernel, target_boxes, starts, lists):
-    """Cache key of a plan: the kernel and the int32 bytes of the CSR."""
-    return (repr(out_kernel),) + tuple(
-        np.ascontiguousarray(_host(a), dtype=np.int32).tobytes()
-        for a in (target_boxes, starts, lists))
+    """Cache key of a plan: the kernel and the int32 bytes of the CSR (for
+    device tensors looked up by identity and version after the first call)."""
+    return (repr(out_kernel),) + tuple(_csr_bytes(a) for a in (target_boxes, starts, lists))

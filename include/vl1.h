@@ -262,6 +262,22 @@ int vl1_plan_set_variant(vl1_plan* plan, int form, int n_slices, int n_rhs, int 
 /* Plan statistics: [0] list-1 pairs, [1] tiles, [2] (tile, case) steps,
  * [3] distinct (case, level) blocks, [4] device bytes. */
 int vl1_plan_stats(const vl1_plan* plan, int64_t* out5);
+/* Allocation-free vl1_plan_eval.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory during evaluation (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as vl1_plan_eval, but
+ * the per-evaluation staging (density slices, column scales, log-term
+ * dots) lives in a fixed 64 MiB device array that the library holds on
+ * every GPU it runs on until the process ends; the call makes no hipMalloc
+ * / hipFree.  Requirements: the plan was evaluated once with vl1_plan_eval
+ * (or tuned) for this form, slice count and RHS count per tile, which
+ * builds its slice set and tile schedule (VL1_E_UNSUPPORTED otherwise); the
+ * staging fits the array (VL1_E_UNSUPPORTED beyond 64 MiB, e.g. 3D q = 6 at
+ * 5 slices takes about 2.2 KiB per (box, RHS)).  Every call on one GPU
+ * uses that GPU's array: calls that overlap in time on one GPU (other
+ * streams, other host threads) race and must be serialised by the caller. */
+int vl1_plan_eval_noalloc(vl1_plan* plan, int form, const double* f, int64_t ldf, int n_rhs,
+                          double* y, int64_t ldy, const vl1_split_opts* opts, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Duffy table build (one launch over all canonical entries).          */

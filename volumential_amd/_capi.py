@@ -57,6 +57,7 @@ EXPORTED_SYMBOLS = (
     "vl1_plan_stage",
     "vl1_plan_contract",
     "vl1_plan_stats",
+    "vl1_plan_eval_noalloc",
     "vl1_plan_tune",
     "vl1_plan_set_variant",
     "vl1_csr_builder_create",
@@ -249,6 +250,8 @@ def lib():
     L.vl1_plan_stage.argtypes = [vp, C.c_int, vp, i64, C.c_int, C.POINTER(SplitOpts), vp]
     L.vl1_plan_contract.argtypes = [vp, C.c_int, C.c_int, vp, i64, C.POINTER(SplitOpts), vp]
     L.vl1_plan_stats.argtypes = [vp, C.POINTER(i64)]
+    L.vl1_plan_eval_noalloc.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, i64,
+                                        C.POINTER(SplitOpts), vp]
     L.vl1_plan_tune.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, i64, C.POINTER(SplitOpts),
                                 C.c_int, C.POINTER(i32), C.POINTER(f64), vp]
     L.vl1_plan_set_variant.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]

@@ -1757,6 +1757,75 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
   return contract_launch(pl, form, S, sch, n_rhs, y, ldy, st, s);
 }
 
+// ------------------------------------------------ allocation-free evaluation
+// Staging region of vl1_plan_eval_noalloc: a fixed device array, one copy
+// per GPU, resident until the process ends.  Host code never passes
+// `scratch` itself to a kernel (on the host it only names the symbol): the
+// device address comes from hipGetSymbolAddress, looked up once per device
+// with that device current.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static void* scratch_ptr[kMaxDevices] = {};
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(VL1_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(VL1_E_UNSUPPORTED, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(VL1_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return VL1_OK;
+}
+
+int vl1_plan_eval_noalloc(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs,
+                          double* y, int64_t ldy, const vl1_split_opts* opts, void* stream) {
+  clear_error();
+  if (!pl || !f || !y) return fail(VL1_E_ARG, "vl1_plan_eval_noalloc: null argument");
+  const int64_t need_ld = pl->n_points * (pl->cplx ? 2 : 1);
+  if (ldf < need_ld || ldy < need_ld)
+    return fail(VL1_E_ARG, "vl1_plan_eval_noalloc: bad leading dimension");
+  if (pl->n_tgt == 0 || pl->n_blocks == 0) return VL1_OK;
+  int S = 0;
+  int rc = plan_check(pl, form, n_rhs, opts, &S);
+  if (rc != VL1_OK) return rc;
+  // the slice set and tile schedule must exist already: building them
+  // allocates (one vl1_plan_eval with this form, slice count and RHS count
+  // per tile builds both)
+  const int Rc = std::min(n_rhs, kCols);
+  const int key = Rc * 2 + ((plan_variant(pl, form, S, Rc) & kVarMorton) ? 1 : 0);
+  auto it = pl->schedules.find(key);
+  if ((form == VL1_FORM_SPLIT_BF16 && !pl->A(S)) || it == pl->schedules.end())
+    return fail(VL1_E_UNSUPPORTED,
+                "vl1_plan_eval_noalloc: the plan is not prepared for this form / slice count / "
+                "RHS count (evaluate it once with vl1_plan_eval first)");
+  const size_t need = staging_layout(pl, form, S, n_rhs, nullptr, nullptr);
+  if (need > kScratchBytes)
+    return fail(VL1_E_UNSUPPORTED, "vl1_plan_eval_noalloc: staging needs " + std::to_string(need) +
+                                       " bytes, above the fixed " + std::to_string(kScratchBytes) +
+                                       "-byte scratch (use vl1_plan_eval, or fewer RHS per call)");
+  void* base = nullptr;
+  rc = scratch_address(&base);
+  if (rc != VL1_OK) return rc;
+  Staging st;
+  staging_layout(pl, form, S, n_rhs, static_cast<char*>(base), &st);
+  hipStream_t s = (hipStream_t)stream;
+  // every word the contraction reads is written by the staging kernel
+  // (padding rows included), so nothing needs clearing between calls
+  rc = stage_launch(pl, form, S, f, ldf, n_rhs, st, s);
+  if (rc != VL1_OK) return rc;
+  return contract_launch(pl, form, S, it->second, n_rhs, y, ldy, st, s);
+}
+
 int vl1_plan_eval(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs, double* y,
                   int64_t ldy, const vl1_split_opts* opts, void* stream) {
   int rc = vl1_plan_stage(pl, form, f, ldf, n_rhs, opts, stream);

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 4
+#define VL1_ABI_VERSION 5
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -158,7 +158,9 @@ typedef struct vl1_plan vl1_plan;
 
 typedef enum vl1_eval_form {
   VL1_FORM_F64 = 0,         /* dense per-case blocks, FP64 MFMA */
-  VL1_FORM_SPLIT_BF16 = 1   /* Ozaki split: BF16 MFMA, FP32 accumulate, FP64 sums */
+  VL1_FORM_SPLIT_BF16 = 1,  /* Ozaki split: BF16 MFMA, FP32 accumulate, FP64 sums */
+  VL1_FORM_SPLIT_I8 = 2     /* Ozaki split: int8 MFMA, exact int32 sums over a tile's boxes,
+                               one FP64 recombination per output; error estimate per RHS */
 } vl1_eval_form;
 
 typedef struct vl1_plan_desc {
@@ -186,8 +188,14 @@ typedef struct vl1_plan_desc {
 } vl1_plan_desc;
 
 typedef struct vl1_split_opts {
-  int n_slices;      /* BF16 slices per operand (1..6) */
+  int n_slices;      /* slices per operand (1..6; default 5 BF16, 6 int8) */
   int n_products;    /* 0 = all slice products with i + j < n_slices */
+  /* VL1_FORM_SPLIT_I8 only, optional (NULL = none): device [n_rhs][2] that
+   * receives per RHS the estimated relative error of the result over the
+   * plan's targets, [0] max |error| / max |y| (Linf) and [1]
+   * ||error||_2 / ||y||_2 (L2).  The estimate is 4/254 of the last slice
+   * diagonal kept, the size of the first diagonal dropped, per output. */
+  double* error_est;
 } vl1_split_opts;
 
 int vl1_plan_create(vl1_plan** out, const vl1_payload* p, const vl1_plan_desc* d, void* stream);

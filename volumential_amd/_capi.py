@@ -42,6 +42,7 @@ SCALING_PER_LEVEL = 2
 
 FORM_F64 = 0
 FORM_SPLIT_BF16 = 1
+FORM_SPLIT_I8 = 2
 
 EXPORTED_SYMBOLS = (
     "vl1_last_error",
@@ -168,7 +169,7 @@ class HelmholtzSplitBlocks(C.Structure):
 
 
 class SplitOpts(C.Structure):
-    _fields_ = [("n_slices", C.c_int), ("n_products", C.c_int)]
+    _fields_ = [("n_slices", C.c_int), ("n_products", C.c_int), ("error_est", C.c_void_p)]
 
 
 class DuffyArgs(C.Structure):
@@ -262,7 +263,7 @@ def lib():
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 4:
+    if L.vl1_abi_version() != 5:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

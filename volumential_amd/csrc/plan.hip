@@ -374,6 +374,15 @@ struct TileParams {
   int64_t ldy;
   int32_t* phase;                  // [8] block each XCD is working on (hint)
   int use_phase;                   // start tiles at their XCD's current block
+  // int8 split form
+  const int8_t* A8;                // table digit slices, fragment order
+  const int8_t* X8;                // [nboxes][KP/32][S][R][32] density digits
+  const double* rho;               // [n_rg][MP] row scales
+  const double* sigma;             // [R] column scales
+  const double* box_disp;          // [nboxes] 2D log-term factor of a source box
+  double* est_part;                // [n_tiles][64][4] error-estimate partials (optional)
+  int nslot;                       // density ring slots
+  int log_disp;                    // 2D log term in use
 };
 
 // Output element of block row `row` of target box `tb`: the row's mode is
@@ -441,6 +450,8 @@ __device__ __forceinline__ void static_for(F&& f) {
     static_for<I + 1, N>(f);
   }
 }
+
+#include "split_i8.h"
 
 // Split form, software-pipelined.  Workgroup = 8 waves, two per SIMD.  Wave
 // w owns row tiles w and w + 8 below MP/16 and all 64 columns, so every
@@ -967,7 +978,7 @@ using namespace vl1;
 
 // ------------------------------------------------------------------- plan
 struct TileSchedule {
-  int B = 0, Rc = 0, n_tiles = 0;
+  int B = 0, Rc = 0, n_tiles = 0, max_steps = 0;
   DevBuf tile_step_start, tile_boxes, step_block, step_src, step_alpha, step_disp;
   int64_t n_steps = 0;
 };
@@ -1013,6 +1024,27 @@ static hipError_t launch_split(int S, bool one, dim3 grid, size_t lds, hipStream
   }
 }
 
+template <int S>
+static hipError_t launch_i8_t(dim3 grid, dim3 block, size_t lds, hipStream_t s, const TileParams& prm) {
+  static std::atomic<unsigned> attr_done{0};
+  hipError_t e = allow_full_lds((const void*)split_i8_kernel<S>, attr_done);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((split_i8_kernel<S>), grid, block, lds, s, prm);
+  return hipGetLastError();
+}
+
+static hipError_t launch_i8(int S, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                            const TileParams& prm) {
+  switch (S) {
+    case 1: return launch_i8_t<1>(grid, block, lds, s, prm);
+    case 2: return launch_i8_t<2>(grid, block, lds, s, prm);
+    case 3: return launch_i8_t<3>(grid, block, lds, s, prm);
+    case 4: return launch_i8_t<4>(grid, block, lds, s, prm);
+    case 5: return launch_i8_t<5>(grid, block, lds, s, prm);
+    default: return launch_i8_t<6>(grid, block, lds, s, prm);
+  }
+}
+
 struct vl1_plan {
   const vl1_payload* payload = nullptr;
   int dim = 0, n = 0, MP = 0, KP = 0;
@@ -1034,19 +1066,24 @@ struct vl1_plan {
   // split slices of the blocks, one set per slice count used (a graph
   // captured at one count keeps reading its own set)
   std::map<int, DevBuf*> A_by_slices;
+  // int8 split form: digit slices per slice count, row scales [n_rg][MP],
+  // and per source box the pair scaling alpha and the 2D log-term factor
+  std::map<int, DevBuf*> A8_by_slices;
+  DevBuf rho, box_alpha, box_disp;
   std::map<int, TileSchedule*> schedules;  // by 2 Rc + (Morton-only order)
   std::map<int, int> tuned;                 // variant by (form, slices, Rc), from vl1_plan_tune
   int forced_variant = -1;                  // candidate under test in vl1_plan_tune
   // staging buffers (grown on demand).  A buffer that is outgrown is
   // retired, not freed, until the plan is destroyed: a HIP graph captured
   // from an earlier evaluation holds its address.
-  DevBuf X, colscale, nmdot, XF;
+  DevBuf X, colscale, nmdot, XF, X8, sigma, est_part;
   std::vector<DevBuf*> retired;
   DevBuf phase;  // int32 [8]
   int staged_form = -1, staged_rhs = 0, staged_slices = -1;
   ~vl1_plan() {
     for (auto& kv : schedules) delete kv.second;
     for (auto& kv : A_by_slices) delete kv.second;
+    for (auto& kv : A8_by_slices) delete kv.second;
     for (DevBuf* b : retired) delete b;
   }
   // Ensure `b` holds at least `bytes`; a smaller buffer in use moves to
@@ -1066,6 +1103,14 @@ struct vl1_plan {
   const DevBuf* A(int S) const {
     auto it = A_by_slices.find(S);
     return it == A_by_slices.end() ? nullptr : it->second;
+  }
+  const DevBuf* A8(int S) const {
+    auto it = A8_by_slices.find(S);
+    return it == A8_by_slices.end() ? nullptr : it->second;
+  }
+  // slices the form needs (null: not built)
+  const DevBuf* slices(int form, int S) const {
+    return form == VL1_FORM_SPLIT_I8 ? A8(S) : (form == VL1_FORM_SPLIT_BF16 ? A(S) : nullptr);
   }
 };
 
@@ -1157,6 +1202,7 @@ static int build_schedule(vl1_plan* pl, int Rc, bool morton_only, TileSchedule**
       salpha.insert(salpha.end(), al_l.begin(), al_l.end());
       sdisp.insert(sdisp.end(), ds_l.begin(), ds_l.end());
       tss[t + 1] = (int32_t)sblk.size();
+      sch->max_steps = std::max(sch->max_steps, tss[t + 1] - tss[t]);
     }
   }
   sch->n_steps = (int64_t)sblk.size();
@@ -1198,6 +1244,49 @@ static int ensure_A(vl1_plan* pl, int S, hipStream_t s) {
     return fail(VL1_E_HIP, std::string("split_rows: ") + hipGetErrorString(e));
   }
   pl->A_by_slices[S] = A;
+  return VL1_OK;
+}
+
+// int8 digit slices for S (and, once, the row scales).
+static int ensure_A8(vl1_plan* pl, int S, hipStream_t s) {
+  if (pl->A8(S)) return VL1_OK;
+  hipError_t e = hipSuccess;
+  const int G = pl->n_rg;
+  if (pl->rho.bytes == 0) {
+    e = pl->rho.alloc((size_t)G * pl->MP * sizeof(double));
+    if (e == hipSuccess) {
+      const int rows = G * pl->MP, wpb = 4;
+      hipLaunchKernelGGL(i8_rowmax_kernel, dim3((rows + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
+                         pl->MP, pl->KP, pl->n_keys, G, pl->M.as<double>(), pl->rho.as<double>());
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+      pl->rho.release();
+      return fail(VL1_E_HIP, std::string("plan row scales: ") + hipGetErrorString(e));
+    }
+  }
+  auto A = new DevBuf();
+  e = A->alloc((size_t)pl->n_blocks * S * pl->MP * pl->KP);
+  if (e == hipSuccess) {
+    const int64_t total = (int64_t)pl->n_blocks * pl->MP * (pl->KP / 16);
+    const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    switch (S) {
+#define VL1_I8_ROWS(SV)                                                                          \
+  case SV:                                                                                       \
+    hipLaunchKernelGGL(i8_split_rows_kernel<SV>, dim3(grid), dim3(256), 0, s, pl->MP, pl->KP,   \
+                       pl->n_blocks, G, pl->M.as<double>(), pl->rho.as<double>(),                 \
+                       A->as<int8_t>());                                                          \
+    break;
+      VL1_I8_ROWS(1) VL1_I8_ROWS(2) VL1_I8_ROWS(3) VL1_I8_ROWS(4) VL1_I8_ROWS(5) VL1_I8_ROWS(6)
+#undef VL1_I8_ROWS
+    }
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) {
+    delete A;
+    return fail(VL1_E_HIP, std::string("plan int8 slices: ") + hipGetErrorString(e));
+  }
+  pl->A8_by_slices[S] = A;
   return VL1_OK;
 }
 
@@ -1415,6 +1504,14 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
   }
   pl->n_keys = (int)pl->blk_case.size();
   pl->n_blocks = pl->n_keys * pl->n_rg;
+  // per source box: the pair scaling and the 2D log-term factor depend on
+  // the source box alone (its level); the int8 form folds alpha into the
+  // box's density digits
+  std::vector<double> balpha(d->n_boxes, 0.0), bdisp(d->n_boxes, 0.0);
+  for (int64_t k = 0; k < n_pairs; ++k) {
+    balpha[pl->pair_src[k]] = pl->pair_alpha[k];
+    bdisp[pl->pair_src[k]] = pl->pair_disp[k];
+  }
 
   hipError_t e = upload(pl->mode_point, mode_point.data(), mode_point.size() * 4, s);
   if (e == hipSuccess) {
@@ -1423,6 +1520,8 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
   }
   std::vector<int32_t> bstarts(d->box_starts, d->box_starts + d->n_boxes);
   if (e == hipSuccess) e = upload(pl->box_starts, bstarts.data(), bstarts.size() * 4, s);
+  if (e == hipSuccess) e = upload(pl->box_alpha, balpha.data(), balpha.size() * 8, s);
+  if (e == hipSuccess) e = upload(pl->box_disp, bdisp.data(), bdisp.size() * 8, s);
   DevBuf dcase, dlev, dperm, dsign, dgroup;
   if (e == hipSuccess && pl->n_blocks > 0) {
     e = upload(dcase, pl->blk_case.data(), pl->blk_case.size() * 4, s);
@@ -1535,7 +1634,7 @@ static int plan_variant(const vl1_plan* pl, int form, int S, int Rc) {
 // the slice count (0 for the FP64 form).
 static int plan_check(const vl1_plan* pl, int form, int n_rhs, const vl1_split_opts* opts,
                       int* S_out) {
-  if (form != VL1_FORM_F64 && form != VL1_FORM_SPLIT_BF16)
+  if (form != VL1_FORM_F64 && form != VL1_FORM_SPLIT_BF16 && form != VL1_FORM_SPLIT_I8)
     return fail(VL1_E_ARG, "vl1_plan: unknown form");
   if (n_rhs < 1) return fail(VL1_E_ARG, "vl1_plan: n_rhs must be >= 1");
   if (n_rhs > kCols && n_rhs % kCols != 0)
@@ -1556,8 +1655,34 @@ static int plan_check(const vl1_plan* pl, int form, int n_rhs, const vl1_split_o
                   "vl1_plan: fused Helmholtz split blocks beyond 256 columns take the FP64 form");
     if (split_lds_bytes(S, pl->MP) > 160 * 1024)
       return fail(VL1_E_UNSUPPORTED, "vl1_plan: too many slices for the box size (LDS)");
+  } else if (form == VL1_FORM_SPLIT_I8) {
+    S = opts ? opts->n_slices : 6;
+    if (S < 1 || S > kI8MaxSlices) return fail(VL1_E_ARG, "vl1_plan: n_slices must be 1..6");
+    if (pl->fused_split && pl->KP > 256)
+      return fail(VL1_E_UNSUPPORTED,
+                  "vl1_plan: fused Helmholtz split blocks beyond 256 columns take the FP64 form");
   }
   *S_out = S;
+  return VL1_OK;
+}
+
+// Ring slots of the int8 tile kernel (at least 3: the consumers read the
+// next chunk's block id one barrier after the producer wrote it).
+static int i8_ring_slots(int S) {
+  int ns = kI8Ring;
+  while (ns > 3 && i8_lds_bytes(S, ns) > 160 * 1024) --ns;
+  return ns;
+}
+
+// The int8 form sums a tile's digit products exactly in int32: the largest
+// diagonal (S products per K column, |d| <= 127) over all K columns of all
+// the tile's steps must stay below 2^31.
+static int i8_check_range(const vl1_plan* pl, const TileSchedule* sch, int S) {
+  const double worst = (double)sch->max_steps * pl->KP * S * 127.0 * 127.0;
+  if (worst >= 2147483647.0)
+    return fail(VL1_E_UNSUPPORTED, "vl1_plan: int8 split form: " + std::to_string(sch->max_steps) +
+                                       " steps per tile could overflow the exact int32 sums; "
+                                       "use the BF16 split or FP64 form");
   return VL1_OK;
 }
 
@@ -1571,10 +1696,15 @@ static int plan_prepare(vl1_plan* pl, int form, int n_rhs, const vl1_split_opts*
   if (form == VL1_FORM_SPLIT_BF16) {
     rc = ensure_A(pl, S, s);
     if (rc != VL1_OK) return rc;
+  } else if (form == VL1_FORM_SPLIT_I8) {
+    rc = ensure_A8(pl, S, s);
+    if (rc != VL1_OK) return rc;
   }
   *S_out = S;
   const int Rc = std::min(n_rhs, kCols);
-  return build_schedule(pl, Rc, (plan_variant(pl, form, S, Rc) & kVarMorton) != 0, sch_out, s);
+  rc = build_schedule(pl, Rc, (plan_variant(pl, form, S, Rc) & kVarMorton) != 0, sch_out, s);
+  if (rc == VL1_OK && form == VL1_FORM_SPLIT_I8) rc = i8_check_range(pl, *sch_out, S);
+  return rc;
 }
 
 // Per-evaluation staging: density slices, column scales and the 2D log-term
@@ -1584,16 +1714,27 @@ struct Staging {
   double* colscale = nullptr;
   double* nmdot = nullptr;
   double* XF = nullptr;
+  int8_t* X8 = nullptr;      // int8 form: density digits
+  double* sigma = nullptr;   // int8 form: column scale per RHS
+  double* est_part = nullptr;  // int8 form: error-estimate partials per (tile, column)
 };
 
 // Bytes of staging for (form, S, R) laid out in one region, each part
 // 256-byte aligned: split X | colscale | nmdot, FP64 XF | nmdot.
-static size_t staging_layout(const vl1_plan* pl, int form, int S, int R, char* base, Staging* st) {
+static size_t staging_layout(const vl1_plan* pl, int form, int S, int R, int n_tiles, char* base,
+                             Staging* st) {
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t cols = (size_t)pl->nboxes * R;
   size_t off = 0;
   Staging t;
-  if (form == VL1_FORM_SPLIT_BF16) {
+  if (form == VL1_FORM_SPLIT_I8) {
+    t.X8 = reinterpret_cast<int8_t*>(base + off);
+    off += al((size_t)pl->nboxes * S * R * pl->KP);
+    t.sigma = reinterpret_cast<double*>(base + off);
+    off += al((size_t)R * 8);
+    t.est_part = reinterpret_cast<double*>(base + off);
+    off += al((size_t)n_tiles * 64 * 4 * 8);
+  } else if (form == VL1_FORM_SPLIT_BF16) {
     t.X = reinterpret_cast<__bf16*>(base + off);
     off += al((size_t)pl->nboxes * S * R * pl->KP * 2);
     t.colscale = reinterpret_cast<double*>(base + off);
@@ -1614,7 +1755,26 @@ static int stage_launch(vl1_plan* pl, int form, int S, const double* f, int64_t 
   const int ncolumns = pl->nboxes * R;
   const int wpb = 4;
   const double* nmlz = pl->payload->nmlz.as<double>();  // level-0 normalisers
-  if (form == VL1_FORM_SPLIT_BF16) {
+  if (form == VL1_FORM_SPLIT_I8) {
+    // column scale per RHS: the largest |f alpha| over all boxes (a fill
+    // kernel clears it, so the launch sequence can be captured in a graph)
+    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, s, st.sigma, (int64_t)R, 0.0);
+    hipLaunchKernelGGL(i8_colmax_kernel, dim3((ncolumns + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
+                       pl->n, pl->cplx, pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),
+                       pl->box_starts.as<int32_t>(), pl->box_alpha.as<double>(), nmlz,
+                       pl->log_disp, st.sigma, st.nmdot);
+    const dim3 g((ncolumns + wpb - 1) / wpb), b(64 * wpb);
+    switch (S) {
+#define VL1_I8_COLS(SV)                                                                          \
+  case SV:                                                                                       \
+    hipLaunchKernelGGL(i8_split_cols_kernel<SV>, g, b, 0, s, pl->n, pl->cplx, pl->KP,           \
+                       pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),                       \
+                       pl->box_alpha.as<double>(), st.sigma, st.X8);                              \
+    break;
+      VL1_I8_COLS(1) VL1_I8_COLS(2) VL1_I8_COLS(3) VL1_I8_COLS(4) VL1_I8_COLS(5) VL1_I8_COLS(6)
+#undef VL1_I8_COLS
+    }
+  } else if (form == VL1_FORM_SPLIT_BF16) {
     // two columns per wavefront, kColBlock / 32 columns per block
     hipLaunchKernelGGL(split_cols_kernel, dim3((ncolumns + kColBlock / 32 - 1) / (kColBlock / 32)),
                        dim3(kColBlock), 0, s, pl->n, pl->cplx, pl->KP, pl->nboxes, R, S, f, ldf,
@@ -1631,8 +1791,10 @@ static int stage_launch(vl1_plan* pl, int form, int S, const double* f, int64_t 
 }
 
 // Launch the tile kernels of `form` reading the staging `st`.
+// error_est: optional device [R][2] (rel Linf, rel L2 estimates per RHS).
 static int contract_launch(vl1_plan* pl, int form, int S, TileSchedule* sch, int R, double* y,
-                           int64_t ldy, const Staging& st, hipStream_t s) {
+                           int64_t ldy, const Staging& st, hipStream_t s,
+                           double* error_est = nullptr) {
   const int Rc = std::min(R, kCols);
   hipError_t e;
   TileParams prm{};
@@ -1656,6 +1818,14 @@ static int contract_launch(vl1_plan* pl, int form, int S, TileSchedule* sch, int
   prm.step_disp = sch->step_disp.as<double>();
   prm.mode_point = pl->mode_point.as<int32_t>();
   prm.A = form == VL1_FORM_SPLIT_BF16 ? pl->A(S)->as<__bf16>() : nullptr;
+  prm.A8 = form == VL1_FORM_SPLIT_I8 ? pl->A8(S)->as<int8_t>() : nullptr;
+  prm.X8 = st.X8;
+  prm.rho = pl->rho.as<double>();
+  prm.sigma = st.sigma;
+  prm.box_disp = pl->box_disp.as<double>();
+  prm.est_part = error_est ? st.est_part : nullptr;
+  prm.nslot = i8_ring_slots(S);
+  prm.log_disp = pl->log_disp;
   prm.rowscale = pl->rowscale.as<double>();
   prm.X = st.X;
   prm.colscale = st.colscale;
@@ -1671,6 +1841,18 @@ static int contract_launch(vl1_plan* pl, int form, int S, TileSchedule* sch, int
   const int grid = ((sch->n_tiles + 7) / 8) * 8;
   for (int r0 = 0; r0 < R; r0 += Rc) {
     prm.r0 = r0;
+    if (form == VL1_FORM_SPLIT_I8) {
+      const int waves = pl->MP / 32 + 1;  // a consumer per 32-row tile + the producer
+      e = launch_i8(S, dim3(grid), dim3(64 * waves), i8_lds_bytes(S, prm.nslot), s, prm);
+      if (e != hipSuccess) return fail(VL1_E_HIP, std::string("int8 split launch: ") + hipGetErrorString(e));
+      if (error_est) {
+        hipLaunchKernelGGL(i8_rhs_error_kernel, dim3(Rc), dim3(256), 0, s, sch->n_tiles, sch->B, Rc,
+                           r0, st.est_part, error_est);
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail(VL1_E_HIP, std::string("error estimate launch: ") + hipGetErrorString(e));
+      }
+      continue;
+    }
     if (form == VL1_FORM_SPLIT_BF16) {
       const size_t lds = split_lds_bytes(S, pl->MP);
       // one row tile per wave (blocks of at most 8 row tiles only)
@@ -1696,6 +1878,19 @@ static int contract_launch(vl1_plan* pl, int form, int S, TileSchedule* sch, int
   return VL1_OK;
 }
 
+// The plan's own staging buffers.
+static Staging plan_staging(vl1_plan* pl) {
+  Staging st;
+  st.X = pl->X.as<__bf16>();
+  st.colscale = pl->colscale.as<double>();
+  st.nmdot = pl->nmdot.as<double>();
+  st.XF = pl->XF.as<double>();
+  st.X8 = pl->X8.as<int8_t>();
+  st.sigma = pl->sigma.as<double>();
+  st.est_part = pl->est_part.as<double>();
+  return st;
+}
+
 int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs,
                    const vl1_split_opts* opts, void* stream) {
   clear_error();
@@ -1713,7 +1908,12 @@ int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_r
   hipError_t e = hipSuccess;
   // the largest buffer first: a failed allocation then leaves the others
   // untouched
-  if (form == VL1_FORM_SPLIT_BF16) {
+  if (form == VL1_FORM_SPLIT_I8) {
+    e = pl->grow(pl->X8, (size_t)pl->nboxes * S * R * pl->KP);
+    if (e == hipSuccess) e = pl->grow(pl->nmdot, ncolumns * 8);
+    if (e == hipSuccess) e = pl->grow(pl->sigma, (size_t)R * 8);
+    if (e == hipSuccess) e = pl->grow(pl->est_part, (size_t)sch->n_tiles * 64 * 4 * 8);
+  } else if (form == VL1_FORM_SPLIT_BF16) {
     e = pl->grow(pl->X, (size_t)pl->nboxes * S * R * pl->KP * 2);
     if (e == hipSuccess) e = pl->grow(pl->nmdot, ncolumns * 8);
     if (e == hipSuccess) e = pl->grow(pl->colscale, ncolumns * 8);
@@ -1722,11 +1922,7 @@ int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_r
     if (e == hipSuccess) e = pl->grow(pl->nmdot, ncolumns * 8);
   }
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_plan_stage alloc: ") + hipGetErrorString(e));
-  Staging st;
-  st.X = pl->X.as<__bf16>();
-  st.colscale = pl->colscale.as<double>();
-  st.nmdot = pl->nmdot.as<double>();
-  st.XF = pl->XF.as<double>();
+  const Staging st = plan_staging(pl);
   rc = stage_launch(pl, form, S, f, ldf, R, st, s);
   if (rc != VL1_OK) return rc;
   pl->staged_form = form;
@@ -1749,12 +1945,8 @@ int vl1_plan_contract(vl1_plan* pl, int form, int n_rhs, double* y, int64_t ldy,
   if (rc != VL1_OK) return rc;
   if (pl->staged_form != form || pl->staged_rhs != n_rhs || pl->staged_slices != S)
     return fail(VL1_E_ARG, "vl1_plan_contract: densities were not staged for this form/RHS/slices");
-  Staging st;
-  st.X = pl->X.as<__bf16>();
-  st.colscale = pl->colscale.as<double>();
-  st.nmdot = pl->nmdot.as<double>();
-  st.XF = pl->XF.as<double>();
-  return contract_launch(pl, form, S, sch, n_rhs, y, ldy, st, s);
+  return contract_launch(pl, form, S, sch, n_rhs, y, ldy, plan_staging(pl), s,
+                         form == VL1_FORM_SPLIT_I8 && opts ? opts->error_est : nullptr);
 }
 
 // ------------------------------------------------ allocation-free evaluation
@@ -1804,11 +1996,15 @@ int vl1_plan_eval_noalloc(vl1_plan* pl, int form, const double* f, int64_t ldf, 
   const int Rc = std::min(n_rhs, kCols);
   const int key = Rc * 2 + ((plan_variant(pl, form, S, Rc) & kVarMorton) ? 1 : 0);
   auto it = pl->schedules.find(key);
-  if ((form == VL1_FORM_SPLIT_BF16 && !pl->A(S)) || it == pl->schedules.end())
+  if ((form != VL1_FORM_F64 && !pl->slices(form, S)) || it == pl->schedules.end())
     return fail(VL1_E_UNSUPPORTED,
                 "vl1_plan_eval_noalloc: the plan is not prepared for this form / slice count / "
                 "RHS count (evaluate it once with vl1_plan_eval first)");
-  const size_t need = staging_layout(pl, form, S, n_rhs, nullptr, nullptr);
+  if (form == VL1_FORM_SPLIT_I8) {
+    rc = i8_check_range(pl, it->second, S);
+    if (rc != VL1_OK) return rc;
+  }
+  const size_t need = staging_layout(pl, form, S, n_rhs, it->second->n_tiles, nullptr, nullptr);
   if (need > kScratchBytes)
     return fail(VL1_E_UNSUPPORTED, "vl1_plan_eval_noalloc: staging needs " + std::to_string(need) +
                                        " bytes, above the fixed " + std::to_string(kScratchBytes) +
@@ -1817,13 +2013,14 @@ int vl1_plan_eval_noalloc(vl1_plan* pl, int form, const double* f, int64_t ldf, 
   rc = scratch_address(&base);
   if (rc != VL1_OK) return rc;
   Staging st;
-  staging_layout(pl, form, S, n_rhs, static_cast<char*>(base), &st);
+  staging_layout(pl, form, S, n_rhs, it->second->n_tiles, static_cast<char*>(base), &st);
   hipStream_t s = (hipStream_t)stream;
   // every word the contraction reads is written by the staging kernel
   // (padding rows included), so nothing needs clearing between calls
   rc = stage_launch(pl, form, S, f, ldf, n_rhs, st, s);
   if (rc != VL1_OK) return rc;
-  return contract_launch(pl, form, S, it->second, n_rhs, y, ldy, st, s);
+  return contract_launch(pl, form, S, it->second, n_rhs, y, ldy, st, s,
+                         form == VL1_FORM_SPLIT_I8 && opts ? opts->error_est : nullptr);
 }
 
 int vl1_plan_eval(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rhs, double* y,
@@ -1860,6 +2057,9 @@ int vl1_plan_tune(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rh
   float best_ms = 0.f;
   for (int v = 0; v < 8 && e == hipSuccess; ++v) {
     if ((v & kVarOneRow) && pl->MP > 128) continue;
+    // the int8 kernel has one row tile per consumer wave and no phase hint:
+    // only the tile order is a choice
+    if (form == VL1_FORM_SPLIT_I8 && (v & (kVarOneRow | kVarNoPhase))) continue;
     if ((v & kVarMorton) && !multi_box) continue;
     pl->forced_variant = v;
     rc = vl1_plan_contract(pl, form, n_rhs, y, ldy, opts, stream);  // warm (and schedule)
@@ -1892,11 +2092,11 @@ int vl1_plan_tune(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_rh
 int vl1_plan_set_variant(vl1_plan* pl, int form, int n_slices, int n_rhs, int variant) {
   clear_error();
   if (!pl) return fail(VL1_E_ARG, "vl1_plan_set_variant: null plan");
-  if (form != VL1_FORM_F64 && form != VL1_FORM_SPLIT_BF16)
+  if (form != VL1_FORM_F64 && form != VL1_FORM_SPLIT_BF16 && form != VL1_FORM_SPLIT_I8)
     return fail(VL1_E_ARG, "vl1_plan_set_variant: unknown form");
   if (n_rhs < 1) return fail(VL1_E_ARG, "vl1_plan_set_variant: n_rhs must be >= 1");
-  const int S = form == VL1_FORM_SPLIT_BF16 ? n_slices : 0;
-  if (form == VL1_FORM_SPLIT_BF16 && (S < 1 || S > kMaxSlices))
+  const int S = form != VL1_FORM_F64 ? n_slices : 0;
+  if (form != VL1_FORM_F64 && (S < 1 || S > kMaxSlices))
     return fail(VL1_E_ARG, "vl1_plan_set_variant: n_slices must be 1..6");
   if (variant < -1 || variant > 7) return fail(VL1_E_ARG, "vl1_plan_set_variant: variant must be -1..7");
   const int key = tune_key(form, S, std::min(n_rhs, kCols));

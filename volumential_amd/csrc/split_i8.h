@@ -1,0 +1,496 @@
+// Ozaki split on int8 MFMA with exact cross-box integer sums.
+//
+// Rows of the (case, level) blocks and the densities are cut into S int8
+// digits: a value v = scale * sum_i d_i / (127 * 254^i), d_i in [-127, 127]
+// (d_0 = rint(127 x), then d_i = rint(254 r_{i-1}) of the remainders, so
+// every digit fits int8 and the expansion carries 7 + 8 (S - 1) bits).
+// The row scale of block row t is the largest |entry| of row t over all the
+// plan's blocks (rho[t]); the column scale of RHS r is the largest |f alpha|
+// over all boxes (sigma[r]), with the canonical pair scaling alpha of the
+// source box folded into its digits.  With scales that do not change from
+// one (case, level) step to the next, the digit products of a whole tile --
+// every K chunk of every list-1 source box -- accumulate exactly in int32,
+// one accumulator set per diagonal d = i + j < S, and are turned into FP64
+// once per tile:
+//     y[t][c] = rho[t] sigma[r] sum_d acc_d[t][c] / (127^2 254^d) + sum disp.
+// v_mfma_i32_32x32x32_i8 runs at twice the BF16 MFMA rate.
+//
+// Error estimate: the last diagonal kept, D_{S-1} = acc_{S-1} W_{S-1},
+// bounds the size of the first dropped one (about D_{S-1} / 254, S + 1
+// terms against S); est[t][c] = 4 |D_{S-1}| / 254 rho sigma per output.
+// Included by plan.hip inside namespace vl1, after TileParams, xcd_tile,
+// out_index and static_for.
+#pragma once
+
+typedef __attribute__((ext_vector_type(4))) int i32x4;
+typedef __attribute__((ext_vector_type(16))) int i32x16;
+
+constexpr int kI8MaxSlices = 6;
+constexpr int kI8Ring = 8;  // density ring slots (chunks in flight)
+
+// 1 / (127^2 254^d)
+__host__ __device__ constexpr double i8_weight(int d) {
+  double w = 1.0 / (127.0 * 127.0);
+  for (int i = 0; i < d; ++i) w /= 254.0;
+  return w;
+}
+
+// Digit products (i, j), i, j < S, on diagonal i + j = d.
+__host__ __device__ constexpr int i8_diag_terms(int S, int d) { return d < S ? d + 1 : 2 * S - 1 - d; }
+
+// rho[g][t] = max over the plan's (case, level) blocks of max_s |M[k G + g][t][s]|
+// (row group g, row t); one wavefront per (g, t).
+__global__ void i8_rowmax_kernel(int MP, int KP, int n_keys, int G, const double* __restrict__ M,
+                                 double* __restrict__ rho) {
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (row >= G * MP) return;
+  const int g = row / MP, t = row % MP;
+  double mx = 0.0;
+  for (int k = 0; k < n_keys; ++k) {
+    const double* m = M + ((int64_t)(k * G + g) * MP + t) * KP;
+    for (int s = lane; s < KP; s += 64) mx = fmax(mx, fabs(m[s]));
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  if (lane == 0) rho[row] = mx;
+}
+
+// Digits of one value in [-1, 1]: d_0 = rint(127 x), d_i = rint(254 r).
+template <int S>
+__device__ inline void i8_digits(double x, int8_t (&d)[S]) {
+  double t = 127.0 * x;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const double r = rint(t);
+    d[i] = (int8_t)(int)r;
+    t = 254.0 * (t - r);
+  }
+}
+
+// Block rows -> int8 digit slices in the fragment order of
+// v_mfma_i32_32x32x32_i8: for row tile rt (32 rows), K chunk kc (32
+// columns) and slice i, lane l holds row 32 rt + l % 32, columns
+// 32 kc + 16 (l / 32) + 0..15 as 16 bytes; one 1 KiB block per
+// (block, rt, kc, i).  One thread per (block, row, 16-column group).
+template <int S>
+__global__ void i8_split_rows_kernel(int MP, int KP, int n_blocks, int G,
+                                     const double* __restrict__ M, const double* __restrict__ rho,
+                                     int8_t* __restrict__ A8) {
+  const int64_t total = (int64_t)n_blocks * MP * (KP / 16);
+  const int nrt = MP / 32, nkc = KP / 32;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int sg = (int)(i % (KP / 16));
+    const int t = (int)((i / (KP / 16)) % MP);
+    const int b = (int)(i / ((int64_t)(KP / 16) * MP));
+    const double r = rho[(b % G) * MP + t];
+    const double inv = r > 0.0 ? 1.0 / r : 0.0;
+    const double* m = M + ((int64_t)b * MP + t) * KP + sg * 16;
+    int8_t dg[16][S];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      // |M| <= rho: x in [-1, 1] (the division may round above 1 by an ulp;
+      // clamp so d_0 stays in [-127, 127])
+      const double x = fmin(1.0, fmax(-1.0, m[u] * inv));
+      i8_digits<S>(x, dg[u]);
+    }
+    const int rt = t / 32, kc = sg / 2, h = sg % 2;
+    const int lane = t % 32 + 32 * h;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      uint32_t w[4];
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4)
+        w[q4] = (uint32_t)(uint8_t)dg[4 * q4][j] | ((uint32_t)(uint8_t)dg[4 * q4 + 1][j] << 8) |
+                ((uint32_t)(uint8_t)dg[4 * q4 + 2][j] << 16) | ((uint32_t)(uint8_t)dg[4 * q4 + 3][j] << 24);
+      *reinterpret_cast<uint4*>(A8 + (((((int64_t)b * nrt + rt) * nkc + kc) * S + j) * 64 + lane) * 16) =
+          make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  }
+}
+
+// Non-negative doubles order like their bit patterns.
+__device__ inline void atomic_max_nonneg(double* addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Density staging, pass 1: sigma[r] = max over boxes of |f| alpha(box) (the
+// column scale of RHS r) and, for the 2D log term, nmdot[box][r] =
+// sum_sid nmlz[sid] f[sbeg + sid].  One wavefront per (box, RHS) column.
+__global__ void i8_colmax_kernel(int n, int cplx, int nboxes, int R, const double* __restrict__ f,
+                                 int64_t ldf, const int32_t* __restrict__ mode_point,
+                                 const int32_t* __restrict__ box_starts,
+                                 const double* __restrict__ box_alpha,
+                                 const double* __restrict__ nmlz, int log_disp,
+                                 double* __restrict__ sigma, double* __restrict__ nmdot) {
+  const int col = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (col >= nboxes * R) return;
+  const int box = col / R, r = col % R;
+  const double* fr = f + (int64_t)r * ldf;
+  const int rows = cplx ? 2 * n : n;
+  double mx = 0.0, dot = 0.0;
+  for (int s = lane; s < rows; s += 64) {
+    const int pt = mode_point[(int64_t)box * n + s % n];
+    mx = fmax(mx, fabs(cplx ? fr[2 * pt + s / n] : fr[pt]));
+    if (log_disp && s < n) dot += nmlz[s] * fr[box_starts[box] + s];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o));
+    dot += __shfl_xor(dot, o);
+  }
+  if (lane == 0) {
+    atomic_max_nonneg(sigma + r, mx * fabs(box_alpha[box]));
+    nmdot[col] = dot;
+  }
+}
+
+// Density staging, pass 2: digits of f alpha / sigma into
+// X8[box][kc][j][r][32 bytes] (K chunk kc, slice j), zero past the rows.
+// One wavefront per (box, RHS) column; a lane owns 4 consecutive K per
+// window of 256.
+template <int S>
+__global__ void i8_split_cols_kernel(int n, int cplx, int KP, int nboxes, int R,
+                                     const double* __restrict__ f, int64_t ldf,
+                                     const int32_t* __restrict__ mode_point,
+                                     const double* __restrict__ box_alpha,
+                                     const double* __restrict__ sigma, int8_t* __restrict__ X8) {
+  const int col = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (col >= nboxes * R) return;
+  const int box = col / R, r = col % R;
+  const double* fr = f + (int64_t)r * ldf;
+  const int rows = cplx ? 2 * n : n;
+  const int nkc = KP / 32;
+  const double sg = sigma[r];
+  const double scale = sg > 0.0 ? box_alpha[box] / sg : 0.0;
+  for (int w = 0; w < KP; w += 256) {
+    const int k0 = w + 4 * lane;
+    if (k0 >= KP) continue;
+    int8_t dg[4][S];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int s = k0 + u;
+      double v = 0.0;
+      if (s < rows) {
+        const int pt = mode_point[(int64_t)box * n + s % n];
+        v = cplx ? fr[2 * pt + s / n] : fr[pt];
+      }
+      i8_digits<S>(fmin(1.0, fmax(-1.0, v * scale)), dg[u]);
+    }
+    const int kc = k0 / 32;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const uint32_t wd = (uint32_t)(uint8_t)dg[0][j] | ((uint32_t)(uint8_t)dg[1][j] << 8) |
+                          ((uint32_t)(uint8_t)dg[2][j] << 16) | ((uint32_t)(uint8_t)dg[3][j] << 24);
+      *reinterpret_cast<uint32_t*>(X8 + ((((int64_t)box * nkc + kc) * S + j) * R + r) * 32 + k0 % 32) = wd;
+    }
+  }
+}
+
+// Per-RHS relative error estimates from the per-(tile, column) partials:
+// out[r][0] = max est / max |y| (Linf), out[r][1] = sqrt(sum est^2 / sum y^2)
+// (L2), over the plan's targets.  One block per RHS.
+__global__ void i8_rhs_error_kernel(int n_tiles, int B, int Rc, int r0,
+                                    const double* __restrict__ part, double* __restrict__ out) {
+  const int r = blockIdx.x;  // RHS within this launch group
+  __shared__ double red[4][256];
+  double em = 0.0, e2 = 0.0, ym = 0.0, y2 = 0.0;
+  for (int64_t i = threadIdx.x; i < (int64_t)n_tiles * B; i += blockDim.x) {
+    const int64_t tile = i / B;
+    const int c = (int)(i % B) * Rc + r;
+    const double* q = part + (tile * 64 + c) * 4;
+    em = fmax(em, q[0]);
+    e2 += q[1];
+    ym = fmax(ym, q[2]);
+    y2 += q[3];
+  }
+  red[0][threadIdx.x] = em;
+  red[1][threadIdx.x] = e2;
+  red[2][threadIdx.x] = ym;
+  red[3][threadIdx.x] = y2;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+      red[1][threadIdx.x] += red[1][threadIdx.x + o];
+      red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + o]);
+      red[3][threadIdx.x] += red[3][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * (r0 + r)] = red[2][0] > 0.0 ? red[0][0] / red[2][0] : 0.0;
+    out[2 * (r0 + r) + 1] = red[3][0] > 0.0 ? sqrt(red[1][0] / red[3][0]) : 0.0;
+  }
+}
+
+// Tile kernel.  Workgroup = MP/32 consumer waves (wave w owns block rows
+// 32w .. 32w + 31 and all 64 columns of the tile) + 1 producer wave.
+//
+// The producer streams each (step, K chunk) of the tile -- the 32-column
+// chunk of the S digit slices of the step's source box densities -- into a
+// ring of NS LDS slots, NS - 1 chunks ahead of the consumers, together with
+// the step's block id; one workgroup barrier per chunk separates writes of
+// a slot from its reads.  Consumers take the table's digit fragments for
+// their rows straight from global memory (fragment order, 1 KiB per wave
+// load), refilled for the next chunk right after their last use, and the
+// density fragments from the ring; products (i, j), i + j < S, accumulate
+// into set i + j for the whole tile.  Ring slot layout:
+// [S][2 K halves][64 columns][16 bytes], so a wave's fragment read of 32
+// columns touches 512 consecutive bytes.
+inline size_t i8_lds_bytes(int S, int NS) {
+  return (size_t)NS * S * 2048 + 64 * 8 + (size_t)NS * 4;
+}
+
+template <int S>
+__global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int SLOT = S * 2048;
+  const int NS = p.nslot;
+  unsigned char* sR = smem_raw;                                            // [NS][SLOT]
+  double* sDsp = reinterpret_cast<double*>(smem_raw + (size_t)NS * SLOT);  // [64]
+  int* sBlk = reinterpret_cast<int*>(sDsp + 64);                           // [NS]
+
+  const int tile = xcd_tile(blockIdx.x, p.n_tiles);
+  if (tile >= p.n_tiles) return;  // uniform over the workgroup
+  const int nrt = p.MP / 32;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int lane = threadIdx.x % 64;
+  const int ncols = p.B * p.Rc;
+  const int nkc = p.KP / 32;
+  const int st0 = p.tile_step_start[tile], st1 = p.tile_step_start[tile + 1];
+  const int total = (st1 - st0) * nkc;
+  const int D = NS - 1;  // chunks the producer runs ahead
+  const int64_t jstride = (int64_t)p.R * 32;  // bytes between the slices of a chunk
+
+  if (wave == nrt) {
+    // ------------------------------------------------------------ producer
+    // Chunk c's global loads are issued one chunk before its LDS stores, so
+    // they have a whole consumer iteration to arrive; the step descriptors
+    // (source boxes of this lane's two 16-byte pieces, block id) are read
+    // one step ahead.
+    double dsum = 0.0;  // lane c: sum over the tile's steps of the column's 2D log term
+    const int nst = st1 - st0;
+    int col[2], half[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      col[m] = (lane + 64 * m) / 2;  // 16-byte piece lane + 64 m: column, K half
+      half[m] = (lane + 64 * m) % 2;
+    }
+    auto load_desc = [&](int k, int (&sb)[2], int& blk) {
+      blk = 0;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) sb[m] = -1;
+      if (k >= nst) return;
+      const int st = st0 + k;
+      blk = p.step_block[st];
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+        if (col[m] < ncols) sb[m] = p.step_src[st * p.B + col[m] / p.Rc];
+    };
+    int sbC[2], sbN[2], blkC = 0, blkN = 0;
+    load_desc(0, sbN, blkN);
+    i32x4 buf[2][S];
+    int buf_blk = 0;
+    auto issue = [&](int c) {
+      const int k = c / nkc, kc = c % nkc;
+      if (kc == 0) {  // a new step: its descriptors, then prefetch the next step's
+#pragma unroll
+        for (int m = 0; m < 2; ++m) sbC[m] = sbN[m];
+        blkC = blkN;
+        load_desc(k + 1, sbN, blkN);
+        if (p.log_disp && lane < ncols) {
+          const int r = lane % p.Rc;
+          const int sb = p.step_src[(st0 + k) * p.B + lane / p.Rc];
+          if (sb >= 0) dsum += p.box_disp[sb] * p.nmdot[(int64_t)sb * p.R + p.r0 + r];
+        }
+      }
+      buf_blk = blkC;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const int64_t base =
+            sbC[m] >= 0
+                ? ((((int64_t)sbC[m] * nkc + kc) * S) * p.R + p.r0 + col[m] % p.Rc) * 32 + 16 * half[m]
+                : -1;
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+          buf[m][j] = base >= 0 ? *reinterpret_cast<const i32x4*>(p.X8 + base + j * jstride)
+                                : i32x4{0, 0, 0, 0};
+      }
+    };
+    auto commit = [&](int c) {
+      const int slot = c % NS;
+      if (lane == 0) sBlk[slot] = buf_blk;
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+          *reinterpret_cast<i32x4*>(sR + (size_t)slot * SLOT + j * 2048 + half[m] * 1024 +
+                                    col[m] * 16) = buf[m][j];
+    };
+    const int pre = total < D ? total : D;
+    for (int c = 0; c < pre; ++c) {
+      issue(c);
+      commit(c);
+    }
+    if (D < total) issue(D);  // in flight across the first barrier
+    __syncthreads();
+    for (int it = 0; it < total; ++it) {
+      const int c = it + D;  // slot c % NS was last read in iteration it - 1
+      if (c < total) {
+        commit(c);
+        if (c + 1 < total) issue(c + 1);
+      }
+      __syncthreads();
+    }
+    sDsp[lane] = dsum;
+    __syncthreads();  // end of the step loop
+  } else {
+    // ------------------------------------------------------------ consumers
+    i32x16 acc[S][2];
+#pragma unroll
+    for (int d = 0; d < S; ++d)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[d][ct][v] = 0;
+    const int8_t* arow = p.A8 + (int64_t)wave * nkc * S * 1024 + lane * 16;
+    const int64_t blk_stride = (int64_t)nrt * nkc * S * 1024;
+    auto afrag = [&](int blk, int kc, int i) {
+      return *reinterpret_cast<const i32x4*>(arow + blk * blk_stride + ((int64_t)kc * S + i) * 1024);
+    };
+    __syncthreads();  // the first D chunks and their block ids are in the ring
+    // Table fragments: slices i < NB of the next chunk are loaded at the top
+    // of the current one (a second register set); the others are refilled
+    // right after their last use (slice i meets density slices 0..S-1-i).
+    // Density slice 0 of a chunk takes the table slices in the order they
+    // were loaded: the double-buffered ones, then S-1 down to NB.
+    constexpr int NB = S <= 5 ? S : 3;
+    i32x4 a[S], an[NB];
+    if (total > 0) {
+      const int blk = sBlk[0];
+#pragma unroll
+      for (int i = 0; i < S; ++i) a[i] = afrag(blk, 0, i);
+    }
+    const unsigned char* rsl = sR + (lane / 32) * 1024 + (lane % 32) * 16;
+    for (int it = 0; it < total; ++it) {
+      const int kc = it % nkc;
+      const bool has_next = it + 1 < total;
+      // the next chunk's block id was written at least one barrier ago
+      const int nblk = has_next ? sBlk[(it + 1) % NS] : 0;
+      const int nkc_ = kc + 1 == nkc ? 0 : kc + 1;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) an[i] = has_next ? afrag(nblk, nkc_, i) : i32x4{0, 0, 0, 0};
+      const unsigned char* rs = rsl + (size_t)(it % NS) * SLOT;
+      static_for<0, S>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const i32x4 b0 = *reinterpret_cast<const i32x4*>(rs + j * 2048);
+        const i32x4 b1 = *reinterpret_cast<const i32x4*>(rs + j * 2048 + 512);
+        static_for<0, S - j>([&](auto ic) {
+          constexpr int u = decltype(ic)::value;
+          constexpr int i = (j == 0 && u >= NB) ? S - 1 - (u - NB) : u;
+          acc[i + j][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b0, acc[i + j][0], 0, 0, 0);
+          acc[i + j][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b1, acc[i + j][1], 0, 0, 0);
+        });
+        // table slice S-1-j had its last partner: refill it for the next chunk
+        if constexpr (S - 1 - j >= NB) {
+          if (has_next) a[S - 1 - j] = afrag(nblk, nkc_, S - 1 - j);
+        }
+      });
+#pragma unroll
+      for (int i = 0; i < NB; ++i) a[i] = an[i];
+      __syncthreads();  // ring slot it % NS may be refilled
+    }
+    __syncthreads();  // end of the step loop (the producer's log-term sums)
+
+    // epilogue: y = rho sigma sum_d acc_d W_d + sum disp, per output; the
+    // error estimate 4 |acc_{S-1}| W_{S-1} / 254 rho sigma; per-column
+    // partials (max est, sum est^2, max |y|, sum y^2) for the RHS estimates
+    const int g = tile % p.n_rg;
+    const double* rho = p.rho + (int64_t)g * p.MP;
+    double sg[2], dsp[2];
+    int tbx[2], rr[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      const int c = 32 * ct + lane % 32;
+      const bool vc = c < ncols;
+      const int b = vc ? c / p.Rc : 0, r = vc ? c % p.Rc : 0;
+      tbx[ct] = vc ? p.tile_boxes[tile * p.B + b] : -1;
+      rr[ct] = r;
+      sg[ct] = vc ? p.sigma[p.r0 + r] : 0.0;
+      dsp[ct] = sDsp[c];
+    }
+    double em[2] = {0.0, 0.0}, e2[2] = {0.0, 0.0}, ym[2] = {0.0, 0.0}, y2[2] = {0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int row = 32 * wave + 8 * (v / 4) + 4 * (lane / 32) + v % 4;
+      const int grow = g * p.rg_size + row;
+      const bool vr = row < p.rg_size && grow < p.n;
+      const double rh = vr ? rho[row] : 0.0;
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        // sum_d acc_d 254^-d in FP64 (Horner from the smallest diagonal),
+        // then the common factor 1/127^2
+        double val = (double)acc[S - 1][ct][v];
+#pragma unroll
+        for (int d = S - 2; d >= 0; --d) val = fma(val, 1.0 / 254.0, (double)acc[d][ct][v]);
+        val *= i8_weight(0);
+        const double sc = rh * sg[ct];
+        const double y = fma(val, sc, dsp[ct]);
+        const double est =
+            4.0 / 254.0 * fabs((double)acc[S - 1][ct][v]) * i8_weight(S - 1) * fabs(sc);
+        if (vr && tbx[ct] >= 0) {
+          p.y[(int64_t)(p.r0 + rr[ct]) * p.ldy + out_index(p, tbx[ct], grow)] = y;
+          em[ct] = fmax(em[ct], est);
+          e2[ct] = fma(est, est, e2[ct]);
+          ym[ct] = fmax(ym[ct], fabs(y));
+          y2[ct] = fma(y, y, y2[ct]);
+        }
+      }
+    }
+    if (p.est_part) {
+      double* scr = reinterpret_cast<double*>(smem_raw);  // the ring is free now: [nrt][64][4]
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        em[ct] = fmax(em[ct], __shfl_xor(em[ct], 32));
+        e2[ct] += __shfl_xor(e2[ct], 32);
+        ym[ct] = fmax(ym[ct], __shfl_xor(ym[ct], 32));
+        y2[ct] += __shfl_xor(y2[ct], 32);
+        if (lane < 32) {
+          double* q = scr + ((int64_t)wave * 64 + 32 * ct + lane) * 4;
+          q[0] = em[ct];
+          q[1] = e2[ct];
+          q[2] = ym[ct];
+          q[3] = y2[ct];
+        }
+      }
+    }
+  }
+  if (p.est_part) {
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const double* scr = reinterpret_cast<const double*>(smem_raw);
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      for (int w = 0; w < nrt; ++w) {
+        const double* q = scr + ((int64_t)w * 64 + threadIdx.x) * 4;
+        a0 = fmax(a0, q[0]);
+        a1 += q[1];
+        a2 = fmax(a2, q[2]);
+        a3 += q[3];
+      }
+      double* o = p.est_part + ((int64_t)tile * 64 + threadIdx.x) * 4;
+      o[0] = a0;
+      o[1] = a1;
+      o[2] = a2;
+      o[3] = a3;
+    }
+  }
+}
+

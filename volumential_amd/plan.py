@@ -7,7 +7,10 @@ the per-case dense blocks, and then evaluates many densities per call
 
 * ``form="f64"``   dense FP64 blocks on FP64 MFMA (rel. error ~1e-15);
 * ``form="split"`` Ozaki-split BF16 MFMA with FP32 accumulation and FP64
-  cross-box sums (``n_slices`` BF16 pieces per operand).
+  cross-box sums (``n_slices`` BF16 pieces per operand);
+* ``form="split_i8"`` Ozaki split on int8 MFMA: ``n_slices`` int8 digits per
+  operand (default 6), exact int32 sums over all of a tile's list-1 boxes,
+  one FP64 recombination per output, and an error estimate per density.
 
 This is the batched counterpart of the per-RHS ``eval_direct`` loop of
 ``volume_fmm.py:983-1031``.
@@ -20,7 +23,8 @@ import torch
 
 from . import _capi
 
-FORMS = {"f64": _capi.FORM_F64, "split": _capi.FORM_SPLIT_BF16}
+FORMS = {"f64": _capi.FORM_F64, "split": _capi.FORM_SPLIT_BF16, "split_i8": _capi.FORM_SPLIT_I8}
+DEFAULT_SLICES = {"f64": 0, "split": 5, "split_i8": 6}
 
 
 class PlanGraph:
@@ -123,11 +127,16 @@ class List1Plan:
         self.is_complex = bool(payload.is_complex) or helmholtz_split is not None
 
     def eval(self, mode_coefs: torch.Tensor, out: torch.Tensor | None = None, form="split",
-             n_slices=5, stream=None) -> torch.Tensor:
+             n_slices=None, stream=None, error_est: torch.Tensor | None = None) -> torch.Tensor:
         """Evaluate ``mode_coefs[R, n_points]`` into ``out`` (targets of the
         plan's boxes overwritten; allocated if None).  float64 CUDA data for
         real tables, complex128 for complex (Helmholtz) tables; the split
-        form then slices the real and imaginary parts alike."""
+        forms then slice the real and imaginary parts alike.  ``error_est``
+        (``form="split_i8"``): a float64 CUDA tensor ``[R, 2]`` that receives
+        the estimated relative error per density (rel Linf, rel L2 over the
+        plan's targets)."""
+        if n_slices is None:
+            n_slices = DEFAULT_SLICES[form]
         f = mode_coefs
         if f.dim() == 1:
             f = f.unsqueeze(0)
@@ -148,6 +157,13 @@ class List1Plan:
         yr = torch.view_as_real(out) if self.is_complex else out
         scale = 2 if self.is_complex else 1
         opts = _capi.SplitOpts(n_slices=int(n_slices), n_products=0)
+        if error_est is not None:
+            if form != "split_i8":
+                raise ValueError("error estimates come with form='split_i8'")
+            if (error_est.dtype != torch.float64 or error_est.device.type != "cuda"
+                    or tuple(error_est.shape) != (int(f.shape[0]), 2) or not error_est.is_contiguous()):
+                raise ValueError("error_est must be a contiguous float64 CUDA tensor [R, 2]")
+            opts.error_est = error_est.data_ptr()
         _capi.check(
             _capi.lib().vl1_plan_eval(
                 self.handle, FORMS[form], fr.data_ptr(), int(ldf_el) * scale, int(f.shape[0]),
@@ -159,10 +175,12 @@ class List1Plan:
 
     VARIANT_BITS = {1: "one_row_tile_per_wave", 2: "morton_order", 4: "no_phase_alignment"}
 
-    def tune(self, mode_coefs: torch.Tensor, form="split", n_slices=5, reps=3, stream=None):
+    def tune(self, mode_coefs: torch.Tensor, form="split", n_slices=None, reps=3, stream=None):
         """Time every valid tile-kernel variant on ``mode_coefs`` and keep the
         fastest for later evaluations of this form / slice count / RHS count.
         Returns ``{"variant": v, "flags": [...], "times_ms": {v: ms}}``."""
+        if n_slices is None:
+            n_slices = DEFAULT_SLICES[form]
         f, fr, ldf, scale = self._prep(mode_coefs)
         out = torch.zeros(f.shape, dtype=f.dtype, device=f.device)  # compact rows
         yr = torch.view_as_real(out) if self.is_complex else out
@@ -181,11 +199,13 @@ class List1Plan:
         return {"variant": v, "flags": [n for b, n in self.VARIANT_BITS.items() if v & b],
                 "times_ms": {i: float(t) for i, t in enumerate(times) if t >= 0}}
 
-    def set_variant(self, variant, form="split", n_slices=5, n_rhs=64):
+    def set_variant(self, variant, form="split", n_slices=None, n_rhs=64):
         """Pin the tile-kernel variant (bits as in ``VARIANT_BITS``) used by
         later evaluations of this form / slice count / RHS count, without
         timing; ``variant=None`` returns to the default."""
         v = -1 if variant is None else int(variant)
+        if n_slices is None:
+            n_slices = DEFAULT_SLICES[form]
         _capi.check(
             _capi.lib().vl1_plan_set_variant(self.handle, FORMS[form], int(n_slices), int(n_rhs), v),
             "vl1_plan_set_variant")
@@ -201,7 +221,7 @@ class List1Plan:
         ldf = f.stride(0) if f.shape[0] > 1 else f.shape[1]
         return f, fr, int(ldf), 2 if self.is_complex else 1
 
-    def capture(self, mode_coefs: torch.Tensor, out: torch.Tensor, form="split", n_slices=5):
+    def capture(self, mode_coefs: torch.Tensor, out: torch.Tensor, form="split", n_slices=None):
         """Record one evaluation (density staging + tile contraction) of
         ``mode_coefs`` into ``out`` as a HIP graph and return it as a
         ``PlanGraph``; each ``replay()`` re-evaluates whatever ``mode_coefs``

@@ -37,6 +37,8 @@ if ROOT not in sys.path:
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_INT8_TOPS = 5000.0
+SPLIT_TOL = 1e-10
 PEAK_HBM_GBS = 8000.0
 
 
@@ -51,7 +53,11 @@ def parse():
     ap.add_argument("--form", default="auto", choices=["auto", "f64", "split"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--slices", type=int, default=5)
+    ap.add_argument("--slices", type=int, default=5,
+                    help="BF16 split slices (the int8 form picks its own from its error estimate)")
+    ap.add_argument("--i8-slices", type=int, default=0,
+                    help="int8 split slices (0 = the fewest, from 5, whose per-RHS estimate "
+                         "is <= 1e-10)")
     ap.add_argument("--only-split", action="store_true",
                     help="time only the split form (profiling runs)")
     ap.add_argument("--skip-plain", action="store_true",
@@ -998,8 +1004,10 @@ def main():
     log(f"plan built in {time.perf_counter() - t0:.2f} s")
     L = _capi.lib()
 
-    def plan_step(form, slices, y_out):
+    def plan_step(form, slices, y_out, est=None):
         opts = _capi.SplitOpts(n_slices=slices, n_products=0)
+        if est is not None:
+            opts.error_est = est.data_ptr()
         fm = FORMS[form]
         _capi.check(L.vl1_plan_stage(plan.handle, fm, f.data_ptr(), f.stride(0), R, opts, sh))
         a = torch.cuda.Event(enable_timing=True)
@@ -1018,8 +1026,10 @@ def main():
     y64 = torch.zeros_like(f)
     if args.only_split:
         args.steps, args.warmup = max(1, args.steps), 0
-    ms64, k64 = timed(lambda: plan_step("f64", 0, y64), 1 if args.only_split else args.steps,
-                      0 if args.only_split else args.warmup)
+    if not args.only_split:
+        ms64, k64 = timed(lambda: plan_step("f64", 0, y64), args.steps, args.warmup)
+    else:
+        ms64, k64 = timed(lambda: plan_step("f64", 0, y64), 1, 0)
     forms["dense_f64_mfma"] = {"ms": ms64, "kernel_ms": k64, "value": n_pts * R / (ms64 * 1e-3) / 1e9,
                                "tflops": flops / (k64 * 1e-3) / 1e12,
                                "fp64_frac": flops / (k64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
@@ -1030,35 +1040,72 @@ def main():
         forms["plain_f64"]["rel_l2_vs_dense_f64"] = float(d_plain.max())
         log(f"plain vs dense f64 max rel L2 {float(d_plain.max()):.2e}")
 
+    def vs_dense(y):
+        l2 = ((y - y64).norm(dim=1) / y64.norm(dim=1)).cpu().numpy()
+        li = ((y - y64).abs().amax(dim=1) / y64.abs().amax(dim=1)).cpu().numpy()
+        return l2, li
+
+    # BF16 split (the previous form), for comparison
     S = args.slices
     ysp = torch.zeros_like(f)
-    mss, ks = timed(lambda: plan_step("split", S, ysp), args.steps, args.warmup)
-    err_l2 = ((ysp - y64).norm(dim=1) / y64.norm(dim=1)).cpu().numpy()
-    err_inf = ((ysp - y64).abs().amax(dim=1) / y64.abs().amax(dim=1)).cpu().numpy()
-    nprod = S * (S + 1) // 2
-    forms["split_bf16"] = {"ms": mss, "kernel_ms": ks, "value": n_pts * R / (mss * 1e-3) / 1e9,
-                           "slices": S, "slice_products": nprod,
-                           "tflops_fp64_equiv": flops / (ks * 1e-3) / 1e12,
-                           "rel_l2_vs_dense_f64_max": float(err_l2.max()),
-                           "rel_linf_vs_dense_f64_max": float(err_inf.max())}
-    log(f"split S={S}: {mss:.2f} ms/step (contract {ks:.2f} ms), vs dense f64: rel L2 max "
-        f"{err_l2.max():.2e}, Linf max {err_inf.max():.2e}")
-    value = forms["split_bf16"]["value"]
-    ms = mss
-    forms["split_bf16"]["autotune"] = tuned.get("split")
-    kname = f"split_bf16_kernel<{S}, {split_rows_per_wave(forms['split_bf16'])}>"
+    if not args.only_split:
+        mss, ks = timed(lambda: plan_step("split", S, ysp), args.steps, args.warmup)
+        err_l2, err_inf = vs_dense(ysp)
+        forms["split_bf16"] = {"ms": mss, "kernel_ms": ks, "value": n_pts * R / (mss * 1e-3) / 1e9,
+                               "slices": S, "slice_products": S * (S + 1) // 2,
+                               "tflops_fp64_equiv": flops / (ks * 1e-3) / 1e12,
+                               "rel_l2_vs_dense_f64_max": float(err_l2.max()),
+                               "rel_linf_vs_dense_f64_max": float(err_inf.max()),
+                               "autotune": tuned.get("split")}
+        log(f"bf16 split S={S}: {mss:.2f} ms/step (contract {ks:.2f} ms), vs dense f64: rel L2 "
+            f"max {err_l2.max():.2e}, Linf max {err_inf.max():.2e}")
+
+    # int8 split, the headline form: the slice count is the fewest (from 5)
+    # whose per-RHS error estimate is within 1e-10 (the estimate comes with
+    # every evaluation and is part of the timed step)
+    yi8 = torch.zeros_like(f)
+    est = torch.zeros((R, 2), dtype=torch.float64, device="cuda")
+    if args.i8_slices:
+        S8 = args.i8_slices
+    else:
+        S8 = 5
+        while True:
+            plan_step("split_i8", S8, yi8, est)
+            torch.cuda.synchronize()
+            if float(est.max()) <= SPLIT_TOL or S8 == 6:
+                break
+            S8 += 1
+    msi, ki = timed(lambda: plan_step("split_i8", S8, yi8, est), args.steps, args.warmup)
+    e8 = est.cpu().numpy()
+    i8_l2, i8_li = vs_dense(yi8)
+    nprod = S8 * (S8 + 1) // 2
+    forms["split_i8"] = {"ms": msi, "kernel_ms": ki, "value": n_pts * R / (msi * 1e-3) / 1e9,
+                         "slices": S8, "slice_products": nprod,
+                         "tflops_fp64_equiv": flops / (ki * 1e-3) / 1e12,
+                         "rel_l2_vs_dense_f64_max": float(i8_l2.max()),
+                         "rel_linf_vs_dense_f64_max": float(i8_li.max()),
+                         "estimate_rel_linf_max": float(e8[:, 0].max()),
+                         "estimate_rel_l2_max": float(e8[:, 1].max()),
+                         "estimate_covers_error": bool(np.all(e8[:, 0] >= i8_li) and
+                                                       np.all(e8[:, 1] >= i8_l2))}
+    log(f"int8 split S={S8}: {msi:.2f} ms/step (contract {ki:.2f} ms), vs dense f64: rel L2 max "
+        f"{i8_l2.max():.2e}, Linf max {i8_li.max():.2e}; estimate max Linf {e8[:, 0].max():.2e} "
+        f"L2 {e8[:, 1].max():.2e}")
+    value = forms["split_i8"]["value"]
+    ms = msi
+    kname = f"split_i8_kernel<{S8}>"
     traffic, traffic_note = measured_traffic(kname, level, q, R)
     counters = measured_counters(kname, level, q, R)
     # roofline of the dominant kernel from the algorithmic work (flops_alg =
     # 2 R sum_pairs n^2) over its measured time, against the peak of the
-    # unit it runs on (BF16 MFMA); the slice products the split scheme
+    # unit it runs on (int8 MFMA); the digit products the split scheme
     # executes on that unit are reported beside it as mfma_work
-    achieved = flops / (ks * 1e-3) / 1e12
+    achieved = flops / (ki * 1e-3) / 1e12
     mfma_ops = nprod * flops
 
     # error per RHS against the FP64 oracle (C/OpenMP restatement of the
-    # reference kernel) on seeded samples of target boxes: all R densities on
-    # a small sample, then the CPU-baseline samples (R = 1 and 4)
+    # reference kernel, its own list-1 CSR) on a seeded sample of target
+    # boxes, all R densities
     f_host = f.cpu().numpy()
     samples = oracle_check_sample(tr, table, f_host)
     cb = None
@@ -1067,7 +1114,9 @@ def main():
         cb = cpu_baseline(tr, table, f_host[:4], args.cpu_seconds, rhs_counts=(1, 4))
         cb.pop("_samples")  # timing samples (tree's own lists): not used as the check
         log(f"cpu: {cb}")
-    outs = {"dense_f64_mfma": y64, "split_bf16": ysp}
+    outs = {"dense_f64_mfma": y64, "split_i8": yi8}
+    if not args.only_split:
+        outs["split_bf16"] = ysp
     if y_plain is not None:
         outs["plain_f64"] = y_plain
     errs = error_vs_oracle(tr, samples, outs)
@@ -1086,7 +1135,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16 slices (MFMA) + f64 sums",
+        "dtype": "int8 digits (MFMA, exact int32 sums) + f64 recombination",
         "data": "synthetic (seeded 8-Gaussian densities, GPU-built Laplace table)",
         "config": {
             "workload": f"3D Laplace uniform level {level}, q={q}, {R} RHS",
@@ -1094,33 +1143,38 @@ def main():
             "target_points": int(n_pts),
             "rhs": R,
             "list1_pairs": int(pairs),
-            "form": f"split_bf16 S={S}",
+            "form": f"split_i8 S={S8} (slices chosen by the per-RHS error estimate <= 1e-10)",
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "mfma",
             "achieved": achieved,
-            "peak": PEAK_BF16_TFLOPS,
+            "peak": PEAK_INT8_TOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_BF16_TFLOPS,
+            "frac": achieved / PEAK_INT8_TOPS,
             "traffic": traffic,
             "traffic_source": traffic_note,
             "kernel": kname,
-            "kernel_ms": ks,
+            "kernel_ms": ki,
             "flops_per_launch": flops,
             "flops_note": "algorithmic: 2 R sum_pairs n^2 (FP64-equivalent useful work)",
-            "hbm_gbs_alg": bytes_alg / (ks * 1e-3) / 1e9,
+            "hbm_gbs_alg": bytes_alg / (ki * 1e-3) / 1e9,
             "mfma_work": {"slice_products": nprod, "ops_per_launch": mfma_ops,
-                          "achieved": mfma_ops / (ks * 1e-3) / 1e12,
-                          "frac": mfma_ops / (ks * 1e-3) / 1e12 / PEAK_BF16_TFLOPS},
+                          "achieved": mfma_ops / (ki * 1e-3) / 1e12,
+                          "frac": mfma_ops / (ki * 1e-3) / 1e12 / PEAK_INT8_TOPS,
+                          "unit": "int8 TOP/s"},
             **({"counters": counters} if counters else {}),
         },
         "error_vs_oracle": errs,
-        "error_vs_fp64": {"rel_l2_per_rhs_max": float(err_l2.max()),
-                          "rel_linf_per_rhs_max": float(err_inf.max()), "rhs": R,
+        "error_vs_fp64": {"rel_l2_per_rhs_max": float(i8_l2.max()),
+                          "rel_linf_per_rhs_max": float(i8_li.max()), "rhs": R,
                           "points": int(n_pts), "against": "dense FP64 MFMA form, all points"},
-        "speedup_vs_plain_f64": (ms_plain / mss) if ms_plain else None,
-        "speedup_vs_dense_f64_mfma": ms64 / mss,
+        "error_estimate": {"rel_linf_per_rhs_max": float(e8[:, 0].max()),
+                           "rel_l2_per_rhs_max": float(e8[:, 1].max()),
+                           "covers_error_vs_fp64": forms["split_i8"]["estimate_covers_error"]},
+        "speedup_vs_plain_f64": (ms_plain / msi) if ms_plain else None,
+        "speedup_vs_dense_f64_mfma": ms64 / msi,
+        "speedup_vs_split_bf16": (forms["split_bf16"]["ms"] / msi) if "split_bf16" in forms else None,
         "autotune": tuned,
         "forms": forms,
     }

@@ -1730,6 +1730,8 @@ static size_t staging_layout(const vl1_plan* pl, int form, int S, int R, int n_t
   if (form == VL1_FORM_SPLIT_I8) {
     t.X8 = reinterpret_cast<int8_t*>(base + off);
     off += al((size_t)pl->nboxes * S * R * pl->KP);
+    t.colscale = reinterpret_cast<double*>(base + off);  // per-column maxima
+    off += al(cols * 8);
     t.sigma = reinterpret_cast<double*>(base + off);
     off += al((size_t)R * 8);
     t.est_part = reinterpret_cast<double*>(base + off);
@@ -1756,13 +1758,14 @@ static int stage_launch(vl1_plan* pl, int form, int S, const double* f, int64_t 
   const int wpb = 4;
   const double* nmlz = pl->payload->nmlz.as<double>();  // level-0 normalisers
   if (form == VL1_FORM_SPLIT_I8) {
-    // column scale per RHS: the largest |f alpha| over all boxes (a fill
-    // kernel clears it, so the launch sequence can be captured in a graph)
-    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, s, st.sigma, (int64_t)R, 0.0);
+    // column scale per RHS: the largest |f alpha| over all boxes (per-column
+    // maxima, then one reduction per RHS)
     hipLaunchKernelGGL(i8_colmax_kernel, dim3((ncolumns + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
                        pl->n, pl->cplx, pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),
                        pl->box_starts.as<int32_t>(), pl->box_alpha.as<double>(), nmlz,
-                       pl->log_disp, st.sigma, st.nmdot);
+                       pl->log_disp, st.colscale, st.nmdot);
+    hipLaunchKernelGGL(i8_sigma_kernel, dim3(R), dim3(256), 0, s, pl->nboxes, R, st.colscale,
+                       st.sigma);
     const dim3 g((ncolumns + wpb - 1) / wpb), b(64 * wpb);
     switch (S) {
 #define VL1_I8_COLS(SV)                                                                          \
@@ -1911,6 +1914,7 @@ int vl1_plan_stage(vl1_plan* pl, int form, const double* f, int64_t ldf, int n_r
   if (form == VL1_FORM_SPLIT_I8) {
     e = pl->grow(pl->X8, (size_t)pl->nboxes * S * R * pl->KP);
     if (e == hipSuccess) e = pl->grow(pl->nmdot, ncolumns * 8);
+    if (e == hipSuccess) e = pl->grow(pl->colscale, ncolumns * 8);
     if (e == hipSuccess) e = pl->grow(pl->sigma, (size_t)R * 8);
     if (e == hipSuccess) e = pl->grow(pl->est_part, (size_t)sch->n_tiles * 64 * 4 * 8);
   } else if (form == VL1_FORM_SPLIT_BF16) {

@@ -109,26 +109,15 @@ __global__ void i8_split_rows_kernel(int MP, int KP, int n_blocks, int G,
   }
 }
 
-// Non-negative doubles order like their bit patterns.
-__device__ inline void atomic_max_nonneg(double* addr, double v) {
-  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
-}
-
-__global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = v;
-}
-
-// Density staging, pass 1: sigma[r] = max over boxes of |f| alpha(box) (the
-// column scale of RHS r) and, for the 2D log term, nmdot[box][r] =
-// sum_sid nmlz[sid] f[sbeg + sid].  One wavefront per (box, RHS) column.
+// Density staging, pass 1: colmax[box][r] = max |f| alpha(box) over the
+// column and, for the 2D log term, nmdot[box][r] = sum_sid nmlz[sid]
+// f[sbeg + sid].  One wavefront per (box, RHS) column.
 __global__ void i8_colmax_kernel(int n, int cplx, int nboxes, int R, const double* __restrict__ f,
                                  int64_t ldf, const int32_t* __restrict__ mode_point,
                                  const int32_t* __restrict__ box_starts,
                                  const double* __restrict__ box_alpha,
                                  const double* __restrict__ nmlz, int log_disp,
-                                 double* __restrict__ sigma, double* __restrict__ nmdot) {
+                                 double* __restrict__ colmax, double* __restrict__ nmdot) {
   const int col = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   if (col >= nboxes * R) return;
@@ -146,9 +135,26 @@ __global__ void i8_colmax_kernel(int n, int cplx, int nboxes, int R, const doubl
     dot += __shfl_xor(dot, o);
   }
   if (lane == 0) {
-    atomic_max_nonneg(sigma + r, mx * fabs(box_alpha[box]));
+    colmax[col] = mx * fabs(box_alpha[box]);
     nmdot[col] = dot;
   }
+}
+
+// sigma[r] = max over boxes of colmax[box][r]; one block per RHS.
+__global__ void __launch_bounds__(256) i8_sigma_kernel(int nboxes, int R,
+                                                       const double* __restrict__ colmax,
+                                                       double* __restrict__ sigma) {
+  __shared__ double red[256];
+  const int r = blockIdx.x;
+  double mx = 0.0;
+  for (int b = threadIdx.x; b < nboxes; b += blockDim.x) mx = fmax(mx, colmax[(int64_t)b * R + r]);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sigma[r] = red[0];
 }
 
 // Density staging, pass 2: digits of f alpha / sigma into
@@ -295,6 +301,9 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
       for (int m = 0; m < 2; ++m)
         if (col[m] < ncols) sb[m] = p.step_src[st * p.B + col[m] / p.Rc];
     };
+    // every slot holds a valid block id from the start (consumers prefetch
+    // table fragments for the chunk after the last one from a stale slot)
+    if (lane < NS) sBlk[lane] = 0;
     int sbC[2], sbN[2], blkC = 0, blkN = 0;
     load_desc(0, sbN, blkN);
     i32x4 buf[2][S];
@@ -368,26 +377,38 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
     };
     __syncthreads();  // the first D chunks and their block ids are in the ring
     // Table fragments: slices i < NB of the next chunk are loaded at the top
-    // of the current one (a second register set); the others are refilled
-    // right after their last use (slice i meets density slices 0..S-1-i).
-    // Density slice 0 of a chunk takes the table slices in the order they
-    // were loaded: the double-buffered ones, then S-1 down to NB.
+    // of the current one into the other of two register sets (ping-pong:
+    // the loop runs two chunks per trip, so no register copies sit between
+    // a load and its use); the others (S = 6 only, for registers) are
+    // refilled in place right after their last use (slice i meets density
+    // slices 0..S-1-i).  Density slice 0 of a chunk takes the table slices
+    // in the order they were loaded: the ping-pong ones, then S-1 down to
+    // NB.  The loads are unconditional (the last chunk reloads block 0,
+    // chunk 0), so no branch separates a load from its wait.
     constexpr int NB = S <= 5 ? S : 3;
-    i32x4 a[S], an[NB];
+    constexpr int NSH = S - NB > 0 ? S - NB : 1;
+    i32x4 app[2][NB], ash[NSH];
     if (total > 0) {
       const int blk = sBlk[0];
 #pragma unroll
-      for (int i = 0; i < S; ++i) a[i] = afrag(blk, 0, i);
+      for (int i = 0; i < S; ++i) {
+        if (i < NB) app[0][i] = afrag(blk, 0, i);
+        else ash[i - NB] = afrag(blk, 0, i);
+      }
     }
     const unsigned char* rsl = sR + (lane / 32) * 1024 + (lane % 32) * 16;
-    for (int it = 0; it < total; ++it) {
+    auto chunk = [&](int it, auto pc) {
+      constexpr int P = decltype(pc)::value;  // register set of this chunk
       const int kc = it % nkc;
-      const bool has_next = it + 1 < total;
-      // the next chunk's block id was written at least one barrier ago
-      const int nblk = has_next ? sBlk[(it + 1) % NS] : 0;
+      // the next chunk's block id was written at least one barrier ago (after
+      // the last chunk the slot holds an older, still valid, block id)
+      const int nblk = sBlk[(it + 1) % NS];
       const int nkc_ = kc + 1 == nkc ? 0 : kc + 1;
 #pragma unroll
-      for (int i = 0; i < NB; ++i) an[i] = has_next ? afrag(nblk, nkc_, i) : i32x4{0, 0, 0, 0};
+      for (int i = 0; i < NB; ++i) app[1 - P][i] = afrag(nblk, nkc_, i);
+      // keep these loads at the top: the scheduler would otherwise sink them
+      // below the MFMAs, leaving their latency to the next chunk's start
+      __builtin_amdgcn_sched_barrier(0);
       const unsigned char* rs = rsl + (size_t)(it % NS) * SLOT;
       static_for<0, S>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
@@ -396,18 +417,21 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
         static_for<0, S - j>([&](auto ic) {
           constexpr int u = decltype(ic)::value;
           constexpr int i = (j == 0 && u >= NB) ? S - 1 - (u - NB) : u;
-          acc[i + j][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b0, acc[i + j][0], 0, 0, 0);
-          acc[i + j][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b1, acc[i + j][1], 0, 0, 0);
+          const i32x4& ai = i < NB ? app[P][i < NB ? i : 0] : ash[i < NB ? 0 : i - NB];
+          acc[i + j][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai, b0, acc[i + j][0], 0, 0, 0);
+          acc[i + j][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai, b1, acc[i + j][1], 0, 0, 0);
         });
         // table slice S-1-j had its last partner: refill it for the next chunk
-        if constexpr (S - 1 - j >= NB) {
-          if (has_next) a[S - 1 - j] = afrag(nblk, nkc_, S - 1 - j);
-        }
+        if constexpr (S - 1 - j >= NB) ash[S - 1 - j - NB] = afrag(nblk, nkc_, S - 1 - j);
       });
-#pragma unroll
-      for (int i = 0; i < NB; ++i) a[i] = an[i];
       __syncthreads();  // ring slot it % NS may be refilled
+    };
+    int it = 0;
+    for (; it + 1 < total; it += 2) {
+      chunk(it, std::integral_constant<int, 0>{});
+      chunk(it + 1, std::integral_constant<int, 1>{});
     }
+    if (it < total) chunk(it, std::integral_constant<int, 0>{});
     __syncthreads();  // end of the step loop (the producer's log-term sums)
 
     // epilogue: y = rho sigma sum_d acc_d W_d + sum disp, per output; the

@@ -202,3 +202,32 @@ def test_estimate_covers_error(cuda, kind):
         if prev is not None:
             assert li.max() < prev / 50, (S, li.max(), prev)
         prev = li.max()
+
+
+def test_wrangler_guard(cuda):
+    """eval_direct_plan(form="split_i8") raises the slice count of a density
+    whose estimate exceeds tol and falls back to the FP64 form past 6
+    slices: at tol = 1e-10 every RHS meets 1e-10 against the oracle (3D q=6,
+    uniform level 2, 64 RHS); at tol = 1e-16 every RHS takes the FP64 form
+    (<= 1e-12)."""
+    print("case: int8 guarded wrangler evaluation, 3D q=6 level 2, 64 RHS", flush=True)
+    tr = vtree.uniform_tree(3, 6, 2)
+    table = _laplace_table(3, 6)
+    f = densities.gaussian_mixture_batch(tr.points, range(1000, 1064))
+    w = NearFieldWrangler(tr, table)
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    fd = torch.as_tensor(f, device="cuda").contiguous()
+    ref = _oracle(tr, [table], f, "canonical")
+    got = w.eval_direct_plan(*args, fd, form="split_i8", tol=1e-10)[0][0].cpu().numpy()
+    rep = w.last_split_report[0]
+    l2, linf = _per_rhs(got, ref)
+    print(f"slices {np.bincount(rep['slices'])}, fallback {int(rep['fallback_f64'].sum())}, "
+          f"error max Linf {linf.max():.2e} L2 {l2.max():.2e}, estimate max {rep['estimate'].max():.2e}")
+    assert linf.max() <= TOL and l2.max() <= TOL
+    ok = ~rep["fallback_f64"]
+    assert np.all(rep["estimate"][ok].max(axis=1) <= TOL)
+    got2 = w.eval_direct_plan(*args, fd, form="split_i8", tol=1e-16)[0][0].cpu().numpy()
+    rep2 = w.last_split_report[0]
+    assert rep2["fallback_f64"].all() and np.all(rep2["slices"] == 6)
+    l2, linf = _per_rhs(got2, ref)
+    assert linf.max() <= 1e-12 and l2.max() <= 1e-12, (linf.max(), l2.max())

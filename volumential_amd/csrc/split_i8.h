@@ -306,9 +306,15 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
     if (lane < NS) sBlk[lane] = 0;
     int sbC[2], sbN[2], blkC = 0, blkN = 0;
     load_desc(0, sbN, blkN);
-    i32x4 buf[2][S];
-    int buf_blk = 0;
-    auto issue = [&](int c) {
+    // two chunks' loads in flight: chunk c lives in buffer (c - D) % 2 and
+    // is loaded two iterations before its LDS stores; the loads are
+    // unconditional (a column without a source reads offset 0 and stores
+    // zeros), so no branch separates a load from its wait
+    i32x4 buf[2][2][S];
+    int buf_blk[2] = {0, 0};
+    bool live[2][2] = {{false, false}, {false, false}};
+    auto issue = [&](int c, auto pc) {
+      constexpr int P = decltype(pc)::value;
       const int k = c / nkc, kc = c % nkc;
       if (kc == 0) {  // a new step: its descriptors, then prefetch the next step's
 #pragma unroll
@@ -321,44 +327,56 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
           if (sb >= 0) dsum += p.box_disp[sb] * p.nmdot[(int64_t)sb * p.R + p.r0 + r];
         }
       }
-      buf_blk = blkC;
+      buf_blk[P] = blkC;
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
+        live[P][m] = sbC[m] >= 0;
         const int64_t base =
-            sbC[m] >= 0
+            live[P][m]
                 ? ((((int64_t)sbC[m] * nkc + kc) * S) * p.R + p.r0 + col[m] % p.Rc) * 32 + 16 * half[m]
-                : -1;
+                : 0;
 #pragma unroll
         for (int j = 0; j < S; ++j)
-          buf[m][j] = base >= 0 ? *reinterpret_cast<const i32x4*>(p.X8 + base + j * jstride)
-                                : i32x4{0, 0, 0, 0};
+          buf[P][m][j] = *reinterpret_cast<const i32x4*>(p.X8 + base + (live[P][m] ? j * jstride : 0));
       }
     };
-    auto commit = [&](int c) {
+    auto commit = [&](int c, auto pc) {
+      constexpr int P = decltype(pc)::value;
       const int slot = c % NS;
-      if (lane == 0) sBlk[slot] = buf_blk;
+      if (lane == 0) sBlk[slot] = buf_blk[P];
 #pragma unroll
       for (int m = 0; m < 2; ++m)
 #pragma unroll
         for (int j = 0; j < S; ++j)
           *reinterpret_cast<i32x4*>(sR + (size_t)slot * SLOT + j * 2048 + half[m] * 1024 +
-                                    col[m] * 16) = buf[m][j];
+                                    col[m] * 16) = live[P][m] ? buf[P][m][j] : i32x4{0, 0, 0, 0};
     };
+    using Z = std::integral_constant<int, 0>;
+    using O = std::integral_constant<int, 1>;
     const int pre = total < D ? total : D;
     for (int c = 0; c < pre; ++c) {
-      issue(c);
-      commit(c);
+      issue(c, Z{});
+      commit(c, Z{});
     }
-    if (D < total) issue(D);  // in flight across the first barrier
+    if (D < total) issue(D, Z{});  // in flight across the first barrier
+    if (D + 1 < total) issue(D + 1, O{});
     __syncthreads();
-    for (int it = 0; it < total; ++it) {
-      const int c = it + D;  // slot c % NS was last read in iteration it - 1
+    // iteration it stores chunk it + D (slot (it + D) % NS was last read in
+    // iteration it - 1) and issues the loads of chunk it + D + 2
+    auto step = [&](int it, auto pc) {
+      const int c = it + D;
       if (c < total) {
-        commit(c);
-        if (c + 1 < total) issue(c + 1);
+        commit(c, pc);
+        if (c + 2 < total) issue(c + 2, pc);
       }
       __syncthreads();
+    };
+    int it = 0;
+    for (; it + 1 < total; it += 2) {
+      step(it, Z{});
+      step(it + 1, O{});
     }
+    if (it < total) step(it, Z{});
     sDsp[lane] = dsum;
     __syncthreads();  // end of the step loop
   } else {
@@ -397,6 +415,16 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
       }
     }
     const unsigned char* rsl = sR + (lane / 32) * 1024 + (lane % 32) * 16;
+    // S <= 5: density slice 0 of the next chunk is read from the ring while
+    // this chunk's MFMAs run (the producer stored that chunk before the
+    // previous barrier: it runs NS - 1 >= 2 chunks ahead), so a chunk's first
+    // MFMAs do not wait on LDS
+    constexpr bool PRE_B = S <= 5;
+    i32x4 bp0, bp1;
+    if (PRE_B && total > 0) {
+      bp0 = *reinterpret_cast<const i32x4*>(rsl);
+      bp1 = *reinterpret_cast<const i32x4*>(rsl + 512);
+    }
     auto chunk = [&](int it, auto pc) {
       constexpr int P = decltype(pc)::value;  // register set of this chunk
       const int kc = it % nkc;
@@ -412,8 +440,9 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
       const unsigned char* rs = rsl + (size_t)(it % NS) * SLOT;
       static_for<0, S>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        const i32x4 b0 = *reinterpret_cast<const i32x4*>(rs + j * 2048);
-        const i32x4 b1 = *reinterpret_cast<const i32x4*>(rs + j * 2048 + 512);
+        const i32x4 b0 = (PRE_B && j == 0) ? bp0 : *reinterpret_cast<const i32x4*>(rs + j * 2048);
+        const i32x4 b1 =
+            (PRE_B && j == 0) ? bp1 : *reinterpret_cast<const i32x4*>(rs + j * 2048 + 512);
         static_for<0, S - j>([&](auto ic) {
           constexpr int u = decltype(ic)::value;
           constexpr int i = (j == 0 && u >= NB) ? S - 1 - (u - NB) : u;
@@ -423,6 +452,13 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
         });
         // table slice S-1-j had its last partner: refill it for the next chunk
         if constexpr (S - 1 - j >= NB) ash[S - 1 - j - NB] = afrag(nblk, nkc_, S - 1 - j);
+        if constexpr (PRE_B && j == 0) {
+          // the next chunk's density slice 0 (a stale slot after the last
+          // chunk); the producer stores into slot (it + NS - 1) % NS meanwhile
+          const unsigned char* rn = rsl + (size_t)((it + 1) % NS) * SLOT;
+          bp0 = *reinterpret_cast<const i32x4*>(rn);
+          bp1 = *reinterpret_cast<const i32x4*>(rn + 512);
+        }
       });
       __syncthreads();  // ring slot it % NS may be refilled
     };

@@ -412,14 +412,25 @@ class NearFieldWrangler:
         return plan
 
     def eval_direct_plan(self, target_boxes, neighbor_source_boxes_starts,
-                         neighbor_source_boxes_lists, mode_coefs, form="split", n_slices=5):
+                         neighbor_source_boxes_lists, mode_coefs, form="split", n_slices=None,
+                         tol=1e-10):
         """Batched evaluation of ``mode_coefs[R, n_points]`` with a plan form
-        ("f64" or "split"); returns ``(pot, timing_future)`` like eval_direct."""
+        ("f64", "split" (BF16) or "split_i8"); returns ``(pot,
+        timing_future)`` like eval_direct.
+
+        ``form="split_i8"`` is accuracy-guarded: every evaluation comes with
+        its per-RHS error estimate (relative Linf and L2 over the targets);
+        with ``n_slices=None`` a density whose estimate exceeds ``tol`` is
+        evaluated again with one more slice (5, then 6), and one still above
+        ``tol`` at 6 slices with the FP64 form.  The slices used, the
+        estimates and the FP64 fallbacks of the last call are in
+        ``self.last_split_report``."""
         mc0 = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
             np.asarray(mode_coefs))
         if mc0.dim() == 1:
             mc0 = mc0.unsqueeze(0)
         pot = self.output_zeros(int(mc0.shape[0]))
+        self.last_split_report = []
         for i, knl in enumerate(self.target_kernels):
             plan = self.get_plan(knl, target_boxes, neighbor_source_boxes_starts,
                                  neighbor_source_boxes_lists)
@@ -428,12 +439,44 @@ class NearFieldWrangler:
                 r = int(mc.shape[0])
                 stacked = torch.cat([mc.real, mc.imag], 0).contiguous()
                 tmp = torch.zeros_like(stacked)
-                plan.eval(stacked, out=tmp, form=form, n_slices=n_slices)
+                self._plan_eval(plan, stacked, tmp, form, n_slices, tol)
                 pot[i].copy_(torch.complex(tmp[:r], tmp[r:]))
                 continue
             mc = mc0.to(device="cuda", dtype=self._dtype(knl)).contiguous()
-            plan.eval(mc, out=pot[i], form=form, n_slices=n_slices)
+            self._plan_eval(plan, mc, pot[i], form, n_slices, tol)
         return self._finish(pot), TimingFuture()
+
+    def _plan_eval(self, plan, mc, out, form, n_slices, tol):
+        if form != "split_i8":
+            plan.eval(mc, out=out, form=form, n_slices=n_slices)
+            return
+        R = int(mc.shape[0])
+        est = torch.zeros((R, 2), dtype=torch.float64, device="cuda")
+        S = 5 if n_slices is None else int(n_slices)
+        plan.eval(mc, out=out, form="split_i8", n_slices=S, error_est=est)
+        e = est.cpu().numpy()
+        slices = np.full(R, S, dtype=np.int32)
+        bad = np.flatnonzero(e.max(axis=1) > tol)
+        while bad.size and n_slices is None and S < 6:
+            S += 1
+            for c0 in range(0, bad.size, 64):  # plans take <= 64 RHS or multiples of 64
+                idx = bad[c0:c0 + 64]
+                sub, sub_out = mc[idx].contiguous(), out[idx].contiguous()
+                sub_est = torch.zeros((idx.size, 2), dtype=torch.float64, device="cuda")
+                plan.eval(sub, out=sub_out, form="split_i8", n_slices=S, error_est=sub_est)
+                out[idx] = sub_out
+                e[idx] = sub_est.cpu().numpy()
+            slices[bad] = S
+            bad = bad[e[bad].max(axis=1) > tol]
+        fallback = np.zeros(R, dtype=bool)
+        if bad.size and n_slices is None:
+            for c0 in range(0, bad.size, 64):
+                idx = bad[c0:c0 + 64]
+                sub_out = out[idx].contiguous()
+                plan.eval(mc[idx].contiguous(), out=sub_out, form="f64")
+                out[idx] = sub_out
+            fallback[bad] = True
+        self.last_split_report.append({"slices": slices, "estimate": e, "fallback_f64": fallback})
 
     # --------------------------------------------------------- split
     def _correction(self, knl):

@@ -1509,6 +1509,13 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
   // box's density digits
   std::vector<double> balpha(d->n_boxes, 0.0), bdisp(d->n_boxes, 0.0);
   for (int64_t k = 0; k < n_pairs; ++k) {
+    // a zero scaling marks a box that is no source in the int8 staging
+    if (!(std::isfinite(pl->pair_alpha[k]) && pl->pair_alpha[k] != 0.0)) {
+      const double a = pl->pair_alpha[k];
+      delete pl;
+      return fail(VL1_E_ARG, "vl1_plan_create: source box scaling " + std::to_string(a) +
+                                 " must be finite and nonzero (root extents)");
+    }
     balpha[pl->pair_src[k]] = pl->pair_alpha[k];
     bdisp[pl->pair_src[k]] = pl->pair_disp[k];
   }
@@ -1658,6 +1665,11 @@ static int plan_check(const vl1_plan* pl, int form, int n_rhs, const vl1_split_o
   } else if (form == VL1_FORM_SPLIT_I8) {
     S = opts ? opts->n_slices : 6;
     if (S < 1 || S > kI8MaxSlices) return fail(VL1_E_ARG, "vl1_plan: n_slices must be 1..6");
+    const int rows = pl->cplx ? 2 * pl->n : pl->n;
+    if (rows / 2 + 2 + rows > kI8ColsLds)  // the staging's offsets and one column in LDS
+      return fail(VL1_E_UNSUPPORTED, "vl1_plan: int8 split form: more than " +
+                                         std::to_string(2 * (kI8ColsLds - 2) / 3) +
+                                         " real rows per box");
     if (pl->fused_split && pl->KP > 256)
       return fail(VL1_E_UNSUPPORTED,
                   "vl1_plan: fused Helmholtz split blocks beyond 256 columns take the FP64 form");
@@ -1760,19 +1772,20 @@ static int stage_launch(vl1_plan* pl, int form, int S, const double* f, int64_t 
   if (form == VL1_FORM_SPLIT_I8) {
     // column scale per RHS: the largest |f alpha| over all boxes (per-column
     // maxima, then one reduction per RHS)
-    hipLaunchKernelGGL(i8_colmax_kernel, dim3((ncolumns + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
-                       pl->n, pl->cplx, pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),
-                       pl->box_starts.as<int32_t>(), pl->box_alpha.as<double>(), nmlz,
-                       pl->log_disp, st.colscale, st.nmdot);
+    const int64_t cm_waves = (int64_t)((pl->nboxes + kI8ColmaxBoxes - 1) / kI8ColmaxBoxes) * R;
+    hipLaunchKernelGGL(i8_colmax_kernel, dim3((unsigned)((cm_waves + wpb - 1) / wpb)), dim3(64 * wpb), 0, s,
+                       pl->n, pl->cplx, pl->nboxes, R, f, ldf, pl->box_starts.as<int32_t>(),
+                       pl->box_alpha.as<double>(), nmlz, pl->log_disp, st.colscale, st.nmdot);
     hipLaunchKernelGGL(i8_sigma_kernel, dim3(R), dim3(256), 0, s, pl->nboxes, R, st.colscale,
                        st.sigma);
-    const dim3 g((ncolumns + wpb - 1) / wpb), b(64 * wpb);
+    const dim3 g(pl->nboxes), b(256);
     switch (S) {
 #define VL1_I8_COLS(SV)                                                                          \
   case SV:                                                                                       \
     hipLaunchKernelGGL(i8_split_cols_kernel<SV>, g, b, 0, s, pl->n, pl->cplx, pl->KP,           \
                        pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),                       \
-                       pl->box_alpha.as<double>(), st.sigma, st.X8);                              \
+                       pl->box_starts.as<int32_t>(), pl->box_alpha.as<double>(), st.sigma,        \
+                       st.X8);                                                                    \
     break;
       VL1_I8_COLS(1) VL1_I8_COLS(2) VL1_I8_COLS(3) VL1_I8_COLS(4) VL1_I8_COLS(5) VL1_I8_COLS(6)
 #undef VL1_I8_COLS

@@ -111,32 +111,67 @@ __global__ void i8_split_rows_kernel(int MP, int KP, int n_blocks, int G,
 
 // Density staging, pass 1: colmax[box][r] = max |f| alpha(box) over the
 // column and, for the 2D log term, nmdot[box][r] = sum_sid nmlz[sid]
-// f[sbeg + sid].  One wavefront per (box, RHS) column.
+// f[sbeg + sid].  A source box's particles are contiguous from
+// box_starts[box], so both read the column in place.  One wavefront per RHS
+// row and kI8ColmaxBoxes consecutive boxes (columns of up to 256 values: all
+// loads issued before the first reduction).  A box that is no list-1 source
+// has alpha 0 (the plan rejects a zero pair scaling) and gets 0 without
+// reading.
+constexpr int kI8ColmaxBoxes = 4;
 __global__ void i8_colmax_kernel(int n, int cplx, int nboxes, int R, const double* __restrict__ f,
-                                 int64_t ldf, const int32_t* __restrict__ mode_point,
-                                 const int32_t* __restrict__ box_starts,
+                                 int64_t ldf, const int32_t* __restrict__ box_starts,
                                  const double* __restrict__ box_alpha,
                                  const double* __restrict__ nmlz, int log_disp,
                                  double* __restrict__ colmax, double* __restrict__ nmdot) {
-  const int col = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  constexpr int NB = kI8ColmaxBoxes;
+  const int ngrp = (nboxes + NB - 1) / NB;
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
-  if (col >= nboxes * R) return;
-  const int box = col / R, r = col % R;
+  if (w >= (int64_t)ngrp * R) return;
+  const int r = (int)(w / ngrp), b0 = (int)(w % ngrp) * NB;
   const double* fr = f + (int64_t)r * ldf;
   const int rows = cplx ? 2 * n : n;
-  double mx = 0.0, dot = 0.0;
-  for (int s = lane; s < rows; s += 64) {
-    const int pt = mode_point[(int64_t)box * n + s % n];
-    mx = fmax(mx, fabs(cplx ? fr[2 * pt + s / n] : fr[pt]));
-    if (log_disp && s < n) dot += nmlz[s] * fr[box_starts[box] + s];
+  double alpha[NB], mx[NB];
+  const double* fb[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int box = b0 + b;
+    alpha[b] = box < nboxes ? box_alpha[box] : 0.0;
+    const int64_t bs = alpha[b] != 0.0 ? box_starts[box] : 0;
+    fb[b] = fr + (cplx ? 2 * bs : bs);
+    mx[b] = 0.0;
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    mx = fmax(mx, __shfl_xor(mx, o));
-    dot += __shfl_xor(dot, o);
+  if (rows <= 256) {
+    double v[NB][4];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        v[b][k] = (alpha[b] != 0.0 && lane + 64 * k < rows) ? fb[b][lane + 64 * k] : 0.0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) mx[b] = fmax(mx[b], fabs(v[b][k]));
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (alpha[b] != 0.0)
+        for (int q = lane; q < rows; q += 64) mx[b] = fmax(mx[b], fabs(fb[b][q]));
   }
-  if (lane == 0) {
-    colmax[col] = mx * fabs(box_alpha[box]);
-    nmdot[col] = dot;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    double m = mx[b], dot = 0.0;
+    if (log_disp && alpha[b] != 0.0)
+      for (int q = lane; q < n; q += 64) dot += nmlz[q] * fb[b][q];
+    for (int o = 32; o > 0; o >>= 1) {
+      m = fmax(m, __shfl_xor(m, o));
+      dot += __shfl_xor(dot, o);
+    }
+    const int box = b0 + b;
+    if (lane == 0 && box < nboxes) {
+      colmax[(int64_t)box * R + r] = m * fabs(alpha[b]);
+      nmdot[(int64_t)box * R + r] = dot;
+    }
   }
 }
 
@@ -159,44 +194,75 @@ __global__ void __launch_bounds__(256) i8_sigma_kernel(int nboxes, int R,
 
 // Density staging, pass 2: digits of f alpha / sigma into
 // X8[box][kc][j][r][32 bytes] (K chunk kc, slice j), zero past the rows.
-// One wavefront per (box, RHS) column; a lane owns 4 consecutive K per
-// window of 256.
+// One block per box.  The box's K -> particle offsets (mode order) go to
+// LDS once; rounds of RHS columns are then read into LDS in place (a box's
+// particles are contiguous; row stride rows + 1 doubles spreads the rows
+// over the banks), and a thread turns 16 consecutive K of one column into S
+// 16-byte digit words, consecutive threads writing consecutive 16-byte
+// pieces.  Boxes that are no source (alpha 0) are skipped: no step reads
+// their digits.
+constexpr int kI8ColsLds = 4096;  // doubles: offsets + one round of columns
 template <int S>
-__global__ void i8_split_cols_kernel(int n, int cplx, int KP, int nboxes, int R,
-                                     const double* __restrict__ f, int64_t ldf,
-                                     const int32_t* __restrict__ mode_point,
-                                     const double* __restrict__ box_alpha,
-                                     const double* __restrict__ sigma, int8_t* __restrict__ X8) {
-  const int col = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-  const int lane = threadIdx.x % 64;
-  if (col >= nboxes * R) return;
-  const int box = col / R, r = col % R;
-  const double* fr = f + (int64_t)r * ldf;
+__global__ void __launch_bounds__(256)
+    i8_split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, const double* __restrict__ f,
+                         int64_t ldf, const int32_t* __restrict__ mode_point,
+                         const int32_t* __restrict__ box_starts,
+                         const double* __restrict__ box_alpha, const double* __restrict__ sigma,
+                         int8_t* __restrict__ X8) {
+  __shared__ double sf[kI8ColsLds];
+  const int box = blockIdx.x;
+  const double alpha = box_alpha[box];
+  if (alpha == 0.0) return;
   const int rows = cplx ? 2 * n : n;
   const int nkc = KP / 32;
-  const double sg = sigma[r];
-  const double scale = sg > 0.0 ? box_alpha[box] / sg : 0.0;
-  for (int w = 0; w < KP; w += 256) {
-    const int k0 = w + 4 * lane;
-    if (k0 >= KP) continue;
-    int8_t dg[4][S];
+  const int LS = rows + 1;
+  int* soff = reinterpret_cast<int*>(sf);  // rows ints
+  double* sv = sf + (rows + 1) / 2;
+  const int cap = (kI8ColsLds - (rows + 1) / 2) / LS;  // >= 1: 1.5 rows + 1 <= kI8ColsLds
+  const int RR = cap < R ? cap : R;
+  const int64_t bs = box_starts[box];
+  const int64_t fo = cplx ? 2 * bs : bs;
+  const int32_t* mp = mode_point + (int64_t)box * n;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  for (int k = threadIdx.x; k < rows; k += blockDim.x) {
+    const int pl = (int)(mp[k % n] - bs);
+    soff[k] = cplx ? 2 * pl + k / n : pl;
+  }
+  for (int r0 = 0; r0 < R; r0 += RR) {
+    const int nr = R - r0 < RR ? R - r0 : RR;
+    for (int rr = wave; rr < nr; rr += blockDim.x / 64) {
+      const double* src = f + (int64_t)(r0 + rr) * ldf + fo;
+      double* dst = sv + rr * LS;
+#pragma unroll 4
+      for (int q = lane; q < rows; q += 64) dst[q] = src[q];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nr * nkc * 2; t += blockDim.x) {
+      const int h = t % 2, rr = (t / 2) % nr, kc = t / (2 * nr);
+      const int r = r0 + rr;
+      const double sg = sigma[r];
+      const double scale = sg > 0.0 ? alpha / sg : 0.0;
+      const double* sr = sv + rr * LS;
+      int8_t dg[16][S];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int s = k0 + u;
-      double v = 0.0;
-      if (s < rows) {
-        const int pt = mode_point[(int64_t)box * n + s % n];
-        v = cplx ? fr[2 * pt + s / n] : fr[pt];
+      for (int u = 0; u < 16; ++u) {
+        const int k = 32 * kc + 16 * h + u;
+        const double v = k < rows ? sr[soff[k]] : 0.0;
+        i8_digits<S>(fmin(1.0, fmax(-1.0, v * scale)), dg[u]);
       }
-      i8_digits<S>(fmin(1.0, fmax(-1.0, v * scale)), dg[u]);
-    }
-    const int kc = k0 / 32;
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-      const uint32_t wd = (uint32_t)(uint8_t)dg[0][j] | ((uint32_t)(uint8_t)dg[1][j] << 8) |
-                          ((uint32_t)(uint8_t)dg[2][j] << 16) | ((uint32_t)(uint8_t)dg[3][j] << 24);
-      *reinterpret_cast<uint32_t*>(X8 + ((((int64_t)box * nkc + kc) * S + j) * R + r) * 32 + k0 % 32) = wd;
+      for (int j = 0; j < S; ++j) {
+        uint32_t w[4];
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4)
+          w[q4] = (uint32_t)(uint8_t)dg[4 * q4][j] | ((uint32_t)(uint8_t)dg[4 * q4 + 1][j] << 8) |
+                  ((uint32_t)(uint8_t)dg[4 * q4 + 2][j] << 16) |
+                  ((uint32_t)(uint8_t)dg[4 * q4 + 3][j] << 24);
+        *reinterpret_cast<uint4*>(X8 + ((((int64_t)box * nkc + kc) * S + j) * R + r) * 32 + 16 * h) =
+            make_uint4(w[0], w[1], w[2], w[3]);
+      }
     }
+    __syncthreads();
   }
 }
 

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 5
+#define VL1_ABI_VERSION 6
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -109,6 +109,39 @@ typedef struct vl1_payload_desc {
 } vl1_payload_desc;
 
 int vl1_payload_create(vl1_payload** out, const vl1_payload_desc* desc, void* stream);
+
+/* The "dense" reconstruction kind, the default of NearFieldFromCSR
+ * (list1.py:120; entry reconstruction list1.py:1018-1045, has_entry gating
+ * list1.py:1111-1116; arrays assembled by expansion_wrangler_fpnd.py:
+ * 1887-2066 and 2980-2995).  The value of (case c, source mode s, target
+ * node t) at table level lev is
+ *   T[lev][ids[f]] * entry_scales[f] * mode_case_scale[s][c],
+ *   f = (mode_case_map[s][c] * n + mode_qpoint_map[s][s]) * n + mode_qpoint_map[s][t],
+ * and an entry with ids[f] < 0 contributes nothing, its displacement
+ * included.  The payload expands every (level, case) block once on the
+ * device ([n_tables][n_cases][n][n], at most 8 GiB); the arrays are
+ * copied, the host pointers may be released on return. */
+typedef struct vl1_dense_payload_desc {
+  int dim;                /* 2 or 3 */
+  int quad_order;         /* q; n = q^dim */
+  int n_cases;
+  int n_tables;
+  int is_complex;         /* values, scales and normalisers complex128 (re, im) */
+  int64_t n_table_entries;            /* layout entries per level */
+  const double* table_data;           /* [n_tables][n_table_entries] (x2 if complex) */
+  const double* mode_nmlz;            /* [n_tables][n] (x2 if complex) */
+  const int64_t* case_indices;        /* [13^dim] */
+  const int32_t* table_entry_ids;     /* [n_cases][n][n]; < 0: no entry */
+  const double* table_entry_scales;   /* [n_cases][n][n] (x2 if complex) */
+  const int32_t* mode_qpoint_map;     /* [n][n] */
+  const int32_t* mode_case_map;       /* [n][n_cases] */
+  const double* mode_case_scale;      /* [n][n_cases] (x2 if complex) */
+} vl1_dense_payload_desc;
+
+int vl1_payload_create_dense(vl1_payload** out, const vl1_dense_payload_desc* desc, void* stream);
+/* 1 if a dense payload has entries without data (ids < 0) reachable from a
+ * (case, s, t), else 0 (always 0 for arithmetic-orbit payloads). */
+int vl1_payload_has_missing(const vl1_payload* p);
 int vl1_payload_destroy(vl1_payload* p);
 /* Bytes of device memory the payload holds. */
 int64_t vl1_payload_device_bytes(const vl1_payload* p);

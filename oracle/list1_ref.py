@@ -2,9 +2,10 @@
 
 Restates volumential/list1.py:1057-1148 (kernel body), 439-459 and
 1083-1091 (float case-id encoding with bias and +0.5 truncation), 278-437
-(scaling policies) and 604-902 (arithmetic orbit reconstruction and the
+(scaling policies), 604-902 (arithmetic orbit reconstruction and the
 entry sign of axis-derivative kernels, via ``orbit_ref.rows_for_case`` /
-``signs_for_case``).  Loops over (target box, source box) pairs in
+``signs_for_case``) and 1018-1045 (the ``dense`` kind, ``dense_entries``;
+``meta["kind"] == "dense"``).  Loops over (target box, source box) pairs in
 Python and contracts each pair's q^d x q^d block with numpy, so it is meant
 for small trees.  ``mode_coefs`` may hold several densities ``[R, n]``.
 """
@@ -94,6 +95,14 @@ def near_field(
                 lev = 0 if lvl < table_starting_level else (rel if rel < n_tables else n_tables - 1)
             sbeg, ns = int(box_starts[sb]), int(box_counts[sb])
             smodes = np.asarray(source_mode_ids[sbeg : sbeg + ns])
+            if meta.get("kind") == "dense":
+                val, has = dense_entries(meta, table[lev], cid, smodes, tmodes, q**dim)
+                integ = val * scal
+                if mode == "canonical" and log_displacement:
+                    disp = -0.5 / np.pi * scal * np.log(sext / table_root_extent) * nmlz[lev, :ns]
+                    integ = integ + disp[:, None]
+                acc += coefs[:, sbeg : sbeg + ns] @ np.where(has, integ, 0.0)
+                continue
             if cid not in rows_cache:
                 rows_cache[cid] = (orbit_ref.rows_for_case(meta, cid, q, dim),
                                    orbit_ref.signs_for_case(meta, cid, q, dim))
@@ -106,6 +115,27 @@ def near_field(
             acc += coefs[:, sbeg : sbeg + ns] @ integ
         out[:, tbeg : tbeg + nt] = acc
     return out
+
+
+def dense_entries(meta, table_row, cid, smodes, tmodes, n):
+    """The ``dense`` reconstruction kind (list1.py:1018-1045) for one case:
+    value[s, t] = T[ids[f]] * table_entry_scales[f] * mode_case_scale[s, c]
+    with f = (mode_case_map[s, c] n + mode_qpoint_map[s, s]) n
+    + mode_qpoint_map[s, t], and has[s, t] = ids[f] >= 0 (list1.py:1111-1116:
+    no entry, no contribution).  Returns (value, has), [len(smodes),
+    len(tmodes)]."""
+    mqm = np.asarray(meta["mode_qpoint_map"]).reshape(n, n)
+    mcm = np.asarray(meta["mode_case_map"]).reshape(n, -1)
+    mcs = np.asarray(meta["mode_case_scale"]).reshape(n, -1)
+    ids = np.asarray(meta["table_entry_ids"]).reshape(-1)
+    tes = np.asarray(meta["table_entry_scales"]).reshape(-1)
+    s = np.asarray(smodes)[:, None]
+    t = np.asarray(tmodes)[None, :]
+    f = (mcm[s, cid].astype(np.int64) * n + mqm[s, s]) * n + mqm[s, t]
+    eid = ids[f]
+    has = eid >= 0
+    val = np.where(has, np.asarray(table_row)[np.where(has, eid, 0)], 0.0) * tes[f] * mcs[s, cid]
+    return val, has
 
 
 def list1_csr_bruteforce(box_centers, box_levels, root_extent, target_boxes, chunk=256,

@@ -48,6 +48,8 @@ EXPORTED_SYMBOLS = (
     "vl1_last_error",
     "vl1_abi_version",
     "vl1_payload_create",
+    "vl1_payload_create_dense",
+    "vl1_payload_has_missing",
     "vl1_payload_destroy",
     "vl1_payload_device_bytes",
     "vl1_near_field_f64",
@@ -94,6 +96,25 @@ class PayloadDesc(C.Structure):
         ("case_axis_group", vp),
         ("case_value_offsets", vp),
         ("axis_sign_power", vp),
+    ]
+
+
+class DensePayloadDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("quad_order", C.c_int),
+        ("n_cases", C.c_int),
+        ("n_tables", C.c_int),
+        ("is_complex", C.c_int),
+        ("n_table_entries", i64),
+        ("table_data", vp),
+        ("mode_nmlz", vp),
+        ("case_indices", vp),
+        ("table_entry_ids", vp),
+        ("table_entry_scales", vp),
+        ("mode_qpoint_map", vp),
+        ("mode_case_map", vp),
+        ("mode_case_scale", vp),
     ]
 
 
@@ -239,6 +260,8 @@ def lib():
     L.vl1_last_error.restype = C.c_char_p
     L.vl1_abi_version.restype = C.c_int
     L.vl1_payload_create.argtypes = [C.POINTER(vp), C.POINTER(PayloadDesc), vp]
+    L.vl1_payload_create_dense.argtypes = [C.POINTER(vp), C.POINTER(DensePayloadDesc), vp]
+    L.vl1_payload_has_missing.argtypes = [vp]
     L.vl1_payload_destroy.argtypes = [vp]
     L.vl1_payload_device_bytes.argtypes = [vp]
     L.vl1_payload_device_bytes.restype = i64
@@ -263,7 +286,7 @@ def lib():
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 5:
+    if L.vl1_abi_version() != 6:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

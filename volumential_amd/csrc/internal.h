@@ -11,6 +11,8 @@ struct vl1_payload {
   int rowmap_u16 = 1;      // rowmap element type: 1 = uint16, 0 = int32
   int sign_mask = 0;       // bit a set: axis a has sign power 1 (signed kernels)
   int max_orbit_count = 0; // largest canonical orbit (rows)
+  int dense_kind = 0;      // 1: the reference's "dense" kind (expanded blocks only)
+  int has_missing = 0;     // dense kind: some (case, s, t) has no entry (NaN in `dense`)
   // host copies of the small descriptors
   std::vector<int64_t> case_indices;
   std::vector<uint16_t> orbit_ranks;
@@ -26,7 +28,9 @@ struct vl1_payload {
   vl1::DevBuf rowmap;      // [n_cases][n (s)][n (t)] relative compact rows; for signed
                            // kernels (int32) ~row marks an entry read negated
   // [n_tables][n_cases][n (s)][n (t)] table values (sign applied; x2 complex),
-  // expanded on the first plain evaluation (mutable: a cache of the payload)
+  // expanded on the first plain evaluation (mutable: a cache of the payload);
+  // built at creation for the dense kind, where a quiet NaN marks an entry
+  // without data
   mutable vl1::DevBuf dense;
   int64_t device_bytes() const {
     return (int64_t)(table.bytes + nmlz.bytes + case_idx.bytes + ranks.bytes + offs.bytes +

@@ -144,9 +144,12 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
           const int rel = lvl - p.table_starting_level;
           tlev = lvl < p.table_starting_level ? 0 : (rel < p.n_tables ? rel : p.n_tables - 1);
         }
-        const int rank = p.ranks[cid];
-        const int off = p.offs[rank];
-        const int cnt = p.offs[rank + 1] - off;
+        int off = 0, cnt = 0;  // the case's canonical orbit (row-map kernels only)
+        if constexpr (!DENSE) {
+          const int rank = p.ranks[cid];
+          off = p.offs[rank];
+          cnt = p.offs[rank + 1] - off;
+        }
         const double* gT = p.table + ((int64_t)tlev * p.n_entries + off) * VW;
         const bool staged = !DENSE && cnt * VW <= p.tbuf;
         const double* gD =
@@ -176,6 +179,9 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
             if constexpr (DENSE) {
               tr = gD[(int64_t)s * n * VW];
               ti = CPLX ? gD[(int64_t)s * n * VW + 1] : 0.0;
+              // dense kind: no entry (NaN) -> no contribution, displacement
+              // included (list1.py:1111-1116)
+              if (tr != tr) continue;
             } else {
               int rel = (int)rm[(int64_t)s * n];
               if constexpr (std::is_signed<IdxT>::value) {
@@ -374,8 +380,9 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
   // (VL1_PLAIN_TABLE=gather in the environment selects the row-map path)
   const char* mode_env = std::getenv("VL1_PLAIN_TABLE");
   const bool force_gather = mode_env && std::string(mode_env) == "gather";
-  const bool dense = !force_gather &&
-      (double)pl->n_tables * pl->n_cases * pl->n * pl->n * vw * sizeof(double) <= 8.0 * (1ull << 30);
+  const bool dense = pl->dense_kind ||  // dense-kind payloads hold only the expanded blocks
+                     (!force_gather && (double)pl->n_tables * pl->n_cases * pl->n * pl->n * vw *
+                                               sizeof(double) <= 8.0 * (1ull << 30));
   if (dense) {
     hipError_t de = ensure_dense(pl, s);
     if (de != hipSuccess)

@@ -59,6 +59,45 @@ hipError_t launch_rowmap(const vl1_payload& p, const uint8_t* d_perm, const int8
   return hipGetLastError();
 }
 
+// Dense kind: D[lev][c][s][t] = T[lev][id] * entry_scales[f] * mode_case_scale[s][c]
+// with f = (mcm[s][c] n + mqm[s][s]) n + mqm[s][t], id = ids[f]; a quiet NaN
+// (both parts) where id < 0.  Complex values multiply as complex numbers.
+__global__ void __launch_bounds__(256)
+dense_kind_kernel(int n, int n_cases, int n_tables, int cplx, int64_t n_entries,
+                  const double* __restrict__ table, const int32_t* __restrict__ ids,
+                  const double* __restrict__ scales, const int32_t* __restrict__ mqm,
+                  const int32_t* __restrict__ mcm, const double* __restrict__ mcs,
+                  double* __restrict__ D) {
+  const int64_t nn = (int64_t)n * n;
+  const int64_t total = (int64_t)n_tables * n_cases * nn;
+  const int vw = cplx ? 2 : 1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(i % n);
+    const int s = (int)((i / n) % n);
+    const int c = (int)((i / nn) % n_cases);
+    const int lev = (int)(i / (nn * n_cases));
+    const int64_t f = ((int64_t)mcm[(int64_t)s * n_cases + c] * n + mqm[(int64_t)s * n + s]) * n +
+                      mqm[(int64_t)s * n + t];
+    const int id = ids[f];
+    if (id < 0) {
+      D[i * vw] = __builtin_nan("");
+      if (cplx) D[i * vw + 1] = __builtin_nan("");
+      continue;
+    }
+    const double* v = table + ((int64_t)lev * n_entries + id) * vw;
+    const double* a = scales + f * vw;
+    const double* b = mcs + ((int64_t)s * n_cases + c) * vw;
+    if (!cplx) {
+      D[i] = v[0] * a[0] * b[0];
+    } else {
+      const double re = v[0] * a[0] - v[1] * a[1], im = v[0] * a[1] + v[1] * a[0];
+      D[i * 2] = re * b[0] - im * b[1];
+      D[i * 2 + 1] = re * b[1] + im * b[0];
+    }
+  }
+}
+
 }  // namespace vl1
 
 using namespace vl1;
@@ -170,6 +209,114 @@ int vl1_payload_create(vl1_payload** out, const vl1_payload_desc* d, void* strea
   *out = p;
   return VL1_OK;
 }
+
+int vl1_payload_create_dense(vl1_payload** out, const vl1_dense_payload_desc* d, void* stream) {
+  clear_error();
+  if (!out || !d) return fail(VL1_E_ARG, "vl1_payload_create_dense: null argument");
+  *out = nullptr;
+  if (d->dim != 2 && d->dim != 3)
+    return fail(VL1_E_UNSUPPORTED, "dense payloads support dim 2 or 3");
+  if (d->quad_order < 1 || d->quad_order > 16) return fail(VL1_E_ARG, "quad_order out of range");
+  if (d->n_tables < 1 || d->n_table_entries < 1 || d->n_cases < 1)
+    return fail(VL1_E_ARG, "empty table payload");
+  if (!d->table_data || !d->mode_nmlz || !d->case_indices || !d->table_entry_ids ||
+      !d->table_entry_scales || !d->mode_qpoint_map || !d->mode_case_map || !d->mode_case_scale)
+    return fail(VL1_E_ARG, "vl1_payload_create_dense: null array");
+  if (d->n_table_entries > INT32_MAX)
+    return fail(VL1_E_UNSUPPORTED, "dense payload: entry ids are int32");
+  hipStream_t s = (hipStream_t)stream;
+  int n = 1;
+  for (int i = 0; i < d->dim; ++i) n *= d->quad_order;
+  const int vw = d->is_complex ? 2 : 1;
+  const int64_t nn = (int64_t)n * n, nfull = (int64_t)d->n_cases * nn;
+  const double dense_bytes = (double)d->n_tables * nfull * vw * sizeof(double);
+  if (dense_bytes > 8.0 * (1ull << 30))
+    return fail(VL1_E_UNSUPPORTED, "dense payload: expanded blocks exceed 8 GiB");
+
+  // host checks: every index the reconstruction forms is in range
+  for (int64_t i = 0; i < nn; ++i)
+    if (d->mode_qpoint_map[i] < 0 || d->mode_qpoint_map[i] >= n)
+      return fail(VL1_E_TABLE, "mode_qpoint_map entries must lie in [0, n_q_points)");
+  for (int64_t i = 0; i < (int64_t)n * d->n_cases; ++i)
+    if (d->mode_case_map[i] < 0 || d->mode_case_map[i] >= d->n_cases)
+      return fail(VL1_E_TABLE, "mode_case_map entries must lie in [0, n_cases)");
+  for (int64_t i = 0; i < nfull; ++i)
+    if (d->table_entry_ids[i] >= d->n_table_entries)
+      return fail(VL1_E_TABLE, "table_entry_ids beyond the table data");
+  for (int64_t i = 0; i < (int64_t)d->n_tables * d->n_table_entries * vw; ++i)
+    if (!std::isfinite(d->table_data[i]))
+      return fail(VL1_E_TABLE, "near-field table contains non-finite entries");
+  for (int64_t i = 0; i < nfull * vw; ++i)
+    if (!std::isfinite(d->table_entry_scales[i]))
+      return fail(VL1_E_TABLE, "table_entry_scales contains non-finite entries");
+  for (int64_t i = 0; i < (int64_t)n * d->n_cases * vw; ++i)
+    if (!std::isfinite(d->mode_case_scale[i]))
+      return fail(VL1_E_TABLE, "mode_case_scale contains non-finite entries");
+  // entries without data that an evaluation can reach
+  int missing = 0;
+  for (int sm = 0; sm < n && !missing; ++sm) {
+    const int32_t* mq = d->mode_qpoint_map + (int64_t)sm * n;
+    for (int c = 0; c < d->n_cases && !missing; ++c) {
+      const int64_t base = ((int64_t)d->mode_case_map[(int64_t)sm * d->n_cases + c] * n + mq[sm]) * n;
+      for (int t = 0; t < n; ++t)
+        if (d->table_entry_ids[base + mq[t]] < 0) {
+          missing = 1;
+          break;
+        }
+    }
+  }
+
+  auto p = new vl1_payload();
+  p->dim = d->dim;
+  p->q = d->quad_order;
+  p->n = n;
+  p->n_cases = d->n_cases;
+  p->n_orbits = 0;
+  p->n_tables = d->n_tables;
+  p->is_complex = d->is_complex ? 1 : 0;
+  p->n_entries = d->n_table_entries;
+  p->dense_kind = 1;
+  p->has_missing = missing;
+  int64_t n_codes = 1;
+  for (int i = 0; i < d->dim; ++i) n_codes *= 13;
+  p->case_indices.assign(d->case_indices, d->case_indices + n_codes);
+  for (int64_t v : p->case_indices)
+    if (v >= d->n_cases) {
+      delete p;
+      return fail(VL1_E_TABLE, "case_indices beyond n_cases");
+    }
+
+  const size_t vbytes = sizeof(double) * vw;
+  hipError_t e = upload(p->table, d->table_data, vbytes * d->n_tables * d->n_table_entries, s);
+  if (e == hipSuccess) e = upload(p->nmlz, d->mode_nmlz, vbytes * d->n_tables * n, s);
+  if (e == hipSuccess)
+    e = upload(p->case_idx, p->case_indices.data(), sizeof(int64_t) * n_codes, s);
+  DevBuf dids, dsc, dmq, dmc, dms;
+  if (e == hipSuccess) e = upload(dids, d->table_entry_ids, sizeof(int32_t) * nfull, s);
+  if (e == hipSuccess) e = upload(dsc, d->table_entry_scales, vbytes * nfull, s);
+  if (e == hipSuccess) e = upload(dmq, d->mode_qpoint_map, sizeof(int32_t) * nn, s);
+  if (e == hipSuccess) e = upload(dmc, d->mode_case_map, sizeof(int32_t) * n * d->n_cases, s);
+  if (e == hipSuccess) e = upload(dms, d->mode_case_scale, vbytes * n * d->n_cases, s);
+  if (e == hipSuccess) e = p->dense.alloc((size_t)d->n_tables * nfull * vbytes);
+  if (e == hipSuccess) {
+    const int64_t total = (int64_t)d->n_tables * nfull;
+    const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(dense_kind_kernel, dim3(grid), dim3(256), 0, s, n, d->n_cases, d->n_tables,
+                       p->is_complex, d->n_table_entries, p->table.as<double>(),
+                       dids.as<int32_t>(), dsc.as<double>(), dmq.as<int32_t>(), dmc.as<int32_t>(),
+                       dms.as<double>(), p->dense.as<double>());
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // before the uploads are freed
+  if (e != hipSuccess) {
+    delete p;
+    return fail(VL1_E_HIP, std::string("vl1_payload_create_dense: ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return VL1_OK;
+}
+
+int vl1_payload_has_missing(const vl1_payload* p) { return p ? p->has_missing : 0; }
 
 int vl1_payload_destroy(vl1_payload* p) {
   delete p;

@@ -95,6 +95,38 @@ __global__ void expand_blocks_kernel(int dim, int q, int n, int cplx, int sp_mas
   }
 }
 
+// Dense-kind payloads: M[b][t][s] from the payload's expanded blocks
+// D[lev][case][s][t] (a NaN marks an entry without data: 0), with the
+// complex embedding of expand_blocks_kernel.
+__global__ void expand_dense_blocks_kernel(int n, int n_cases, int cplx, int MP, int KP,
+                                           int n_blocks, int n_rg, int rg_size,
+                                           const int32_t* __restrict__ blk_case,
+                                           const int32_t* __restrict__ blk_lev,
+                                           const double* __restrict__ D, double* __restrict__ M) {
+  const int64_t total = (int64_t)n_blocks * MP * KP;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(i % KP);
+    const int tl = (int)((i / KP) % MP);
+    const int bg = (int)(i / ((int64_t)KP * MP));
+    const int b = bg / n_rg, t = (bg % n_rg) * rg_size + tl;
+    double v = 0.0;
+    const int rows = cplx ? 2 * n : n;
+    if (s < rows && tl < rg_size && t < rows) {
+      const int64_t at = (((int64_t)blk_lev[b] * n_cases + blk_case[b]) * n + s % n) * n + t % n;
+      if (!cplx) {
+        v = D[at];
+      } else {
+        const bool tr = t < n, sr = s < n;
+        const double re = D[2 * at], im = D[2 * at + 1];
+        v = (tr == sr) ? re : (tr ? -im : im);
+      }
+      if (v != v) v = 0.0;
+    }
+    M[i] = v;
+  }
+}
+
 // Helmholtz split blocks (complex, real 2n x 2n embedding) per (case, source
 // level): the Laplace table part with the canonical scaling folded in, plus
 // the quadrature of the split remainder on the q^d source nodes,
@@ -1315,6 +1347,9 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
     // a derivative wrapper takes the wrapped (signed) Laplace table
     if (pay->is_complex || (pay->sign_mask && sp->deriv_kind == VL1_SPLIT_DERIV_NONE))
       return fail(VL1_E_ARG, "Helmholtz split plans take a real scalar (Laplace) table payload");
+    if (pay->dense_kind)
+      return fail(VL1_E_UNSUPPORTED,
+                  "Helmholtz split plans take an arithmetic-orbit payload, not the dense kind");
     if (d->scaling_mode != VL1_SCALING_CANONICAL)
       return fail(VL1_E_ARG, "Helmholtz split plans use the canonical single-table scaling");
     if (sp->series_nmax < 1 || sp->series_nmax > kSplitMaxTerms)
@@ -1340,6 +1375,14 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
   pl->n_points = d->n_points;
   const bool want_log = d->scaling_mode == VL1_SCALING_CANONICAL && d->log_displacement;
   pl->log_disp = (want_log && !split) ? 1 : 0;  // split blocks carry the term themselves
+  if (pl->log_disp && pay->has_missing) {
+    // the blocks add the log term for every (s, t); the reference skips it
+    // where an entry is missing
+    delete pl;
+    return fail(VL1_E_UNSUPPORTED,
+                "vl1_plan_create: a dense payload with missing entries and the 2D log "
+                "displacement evaluates through vl1_near_field_f64");
+  }
   if (pl->cplx && pl->log_disp) {
     delete pl;
     return fail(VL1_E_UNSUPPORTED, "vl1_plan_create: the 2D log displacement applies to real tables only");
@@ -1568,6 +1611,14 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
         e = hipGetLastError();
       }
       if (e == hipSuccess) e = hipStreamSynchronize(s);  // before the uploads are freed
+    } else if (e == hipSuccess && pay->dense_kind) {
+      const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
+      const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+      hipLaunchKernelGGL(expand_dense_blocks_kernel, dim3(grid), dim3(256), 0, s, n, pay->n_cases,
+                         pl->cplx, pl->MP, pl->KP, pl->n_blocks, pl->n_rg, pl->rg_size,
+                         dcase.as<int32_t>(), dlev.as<int32_t>(), pay->dense.as<double>(),
+                         pl->M.as<double>());
+      e = hipGetLastError();
     } else if (e == hipSuccess) {
       const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
       const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);

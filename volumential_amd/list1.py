@@ -21,7 +21,7 @@ import torch
 
 from . import _capi, gallery, orbit
 from .kernels import inferred_scaling, supports_inferred_scaling
-from .payload import DevicePayload, PayloadCache, fingerprint
+from .payload import DenseDevicePayload, DevicePayload, PayloadCache, fingerprint
 
 CASE_ENCODING_BIAS_DEFAULT = 1.0e-10
 
@@ -108,16 +108,18 @@ class NearFieldFromCSR:
         self.n_q_points = int(table_data_shapes["n_q_points"])
         self.n_table_entries = int(table_data_shapes["n_table_entries"])
         self.n_cases = int(table_data_shapes.get("n_cases", 0))
-        self.reconstruction_kind = table_data_shapes.get(
-            "reconstruction_kind", "scalar-arithmetic-orbit"
-        )
-        if self.reconstruction_kind not in ("scalar-arithmetic-orbit",
-                                            "signed-arithmetic-orbit"):
+        # the reference's default kind (list1.py:120)
+        self.reconstruction_kind = table_data_shapes.get("reconstruction_kind", "dense")
+        if self.reconstruction_kind == "generated-orbit":
             raise NotImplementedError(
-                f"reconstruction kind {self.reconstruction_kind!r} is not supported; "
-                "scalar kernels use 'scalar-arithmetic-orbit', axis derivatives "
-                "'signed-arithmetic-orbit'"
+                "reconstruction kind 'generated-orbit' (transform scans and lookup tables, "
+                "list1.py:903-1017) is not part of this library: pass the same table as "
+                "'dense' (table_entry_ids / table_entry_scales with the mode maps) or as an "
+                "arithmetic orbit kind"
             )
+        if self.reconstruction_kind not in ("dense", "scalar-arithmetic-orbit",
+                                            "signed-arithmetic-orbit"):
+            raise ValueError(f"unknown reconstruction kind {self.reconstruction_kind!r}")
         self.dim = integral_kernel.dim
         self.quad_order = int(
             table_data_shapes.get("quad_order", round(self.n_q_points ** (1.0 / self.dim)))
@@ -181,7 +183,31 @@ class NearFieldFromCSR:
         }[policy.mode]
 
     # ------------------------------------------------------------ payload
+    def _dense_payload(self, kw):
+        """Device payload of the ``dense`` kind (list1.py:1018-1045)."""
+        names = ("table_data_combined", "mode_nmlz_combined", "case_indices", "table_entry_ids",
+                 "table_entry_scales", "mode_qpoint_map", "mode_case_map", "mode_case_scale")
+        missing = [k for k in names if kw.get(k) is None]
+        if missing:
+            raise ValueError("the 'dense' reconstruction kind needs " + ", ".join(missing))
+        host = {k: _to_host(kw[k]) for k in names}
+        key = ("dense", repr(self.integral_kernel), self.quad_order) + tuple(
+            fingerprint(np.asarray(host[k])) for k in names)
+
+        def make():
+            return DenseDevicePayload(
+                self.dim, self.quad_order,
+                np.asarray(host["table_data_combined"]).reshape(self.n_tables, -1),
+                np.asarray(host["mode_nmlz_combined"]).reshape(self.n_tables, -1),
+                host["case_indices"], host["table_entry_ids"], host["table_entry_scales"],
+                host["mode_qpoint_map"], host["mode_case_map"], host["mode_case_scale"],
+                n_cases=self.n_cases or np.asarray(host["mode_case_map"]).size // self.n_q_points)
+
+        return _PAYLOADS.get(key, make)
+
     def _payload(self, kw):
+        if self.reconstruction_kind == "dense":
+            return self._dense_payload(kw)
         table_data = _to_host(kw["table_data_combined"])
         sp = kw.get("arithmetic_axis_sign_power")
         sign_power = (np.zeros(self.dim, np.uint8) if sp is None

@@ -173,6 +173,77 @@ class DevicePayload:
             pass
 
 
+class DenseDevicePayload:
+    """Owns a ``vl1_payload`` of the reference's ``dense`` reconstruction
+    kind (``list1.py:1018-1045``): the arrays of
+    ``expansion_wrangler_fpnd.py:1887-2066, 2980-2995`` are copied to the
+    device and every (level, case) block is expanded once there."""
+
+    def __init__(self, dim, quad_order, table_data, mode_nmlz, case_indices, table_entry_ids,
+                 table_entry_scales, mode_qpoint_map, mode_case_map, mode_case_scale,
+                 n_cases, stream=None):
+        L = _capi.lib()
+        table_data = np.asarray(table_data)
+        n = int(quad_order) ** int(dim)
+        self.is_complex = bool(np.iscomplexobj(table_data) or np.iscomplexobj(table_entry_scales)
+                               or np.iscomplexobj(mode_case_scale))
+        vdt = np.complex128 if self.is_complex else np.float64
+
+        def vals(a, shape):
+            a = np.asarray(a)
+            if not self.is_complex:
+                a = a.real
+            return np.ascontiguousarray(a.astype(vdt).reshape(shape))
+
+        data = np.atleast_2d(table_data)
+        data = vals(data.reshape(data.shape[0], -1), (data.shape[0], -1))
+        n_tables, n_entries = data.shape
+        nmlz = vals(np.asarray(mode_nmlz).reshape(n_tables, -1)[:, :n], (n_tables, n))
+        ids = np.ascontiguousarray(np.asarray(table_entry_ids).reshape(-1), dtype=np.int32)
+        if ids.size != int(n_cases) * n * n:
+            raise ValueError(f"table_entry_ids must hold n_cases * n_q_points^2 = "
+                             f"{int(n_cases) * n * n} entries, got {ids.size}")
+        self._keep = [
+            data.view(np.float64) if self.is_complex else data,
+            nmlz.view(np.float64) if self.is_complex else nmlz,
+            np.ascontiguousarray(case_indices, dtype=np.int64),
+            ids,
+            vals(table_entry_scales, (-1,)),
+            np.ascontiguousarray(np.asarray(mode_qpoint_map).reshape(n, n), dtype=np.int32),
+            np.ascontiguousarray(np.asarray(mode_case_map).reshape(n, int(n_cases)),
+                                 dtype=np.int32),
+            vals(mode_case_scale, (n, int(n_cases))),
+        ]
+        for k in (4, 7):
+            if self.is_complex:
+                self._keep[k] = self._keep[k].view(np.float64)
+        if self._keep[4].size != ids.size * (2 if self.is_complex else 1):
+            raise ValueError("table_entry_scales must match table_entry_ids")
+        desc = _capi.DensePayloadDesc(
+            dim=int(dim), quad_order=int(quad_order), n_cases=int(n_cases),
+            n_tables=int(n_tables), is_complex=int(self.is_complex),
+            n_table_entries=int(n_entries),
+            table_data=self._keep[0].ctypes.data, mode_nmlz=self._keep[1].ctypes.data,
+            case_indices=self._keep[2].ctypes.data, table_entry_ids=self._keep[3].ctypes.data,
+            table_entry_scales=self._keep[4].ctypes.data,
+            mode_qpoint_map=self._keep[5].ctypes.data, mode_case_map=self._keep[6].ctypes.data,
+            mode_case_scale=self._keep[7].ctypes.data,
+        )
+        handle = _capi.vp()
+        _capi.check(L.vl1_payload_create_dense(_capi.C.byref(handle), desc,
+                                               _capi.stream_handle(stream)),
+                    "vl1_payload_create_dense")
+        self._keep = None  # copied by the library
+        self.handle = handle
+        self.dim, self.quad_order = int(dim), int(quad_order)
+        self.n_tables, self.n_entries = int(n_tables), int(n_entries)
+        self.has_missing = bool(L.vl1_payload_has_missing(handle))
+
+    device_bytes = DevicePayload.device_bytes
+    close = DevicePayload.close
+    __del__ = DevicePayload.__del__
+
+
 def fingerprint(arr, sample_bytes=4096):
     """Content key of a table array (``expansion_wrangler_fpnd.py:99-112``)."""
     narr = np.ascontiguousarray(arr)

@@ -367,14 +367,14 @@ class NearFieldWrangler:
             return self._plans[key]
         tables = self.near_field_table[repr(out_kernel)]
         t0 = tables[0]
-        data, nmlz, _, _, _, _ = prepare_table_data_and_entry_map(tables)
+        data, nmlz, _, _, _, info = prepare_table_data_and_entry_map(tables)
         payload = DevicePayload(t0.dim, t0.quad_order, data, nmlz, t0.case_indices,
                                 t0.arithmetic_metadata())
         nf = NearFieldFromCSR(
             self._policy_kernel(out_kernel),
             {"n_tables": len(tables), "n_q_points": t0.n_q_points,
              "n_table_entries": data.shape[1], "n_cases": t0.n_cases,
-             "quad_order": t0.quad_order},
+             "quad_order": t0.quad_order, "reconstruction_kind": info["kind"]},
             **self.list1_extra_kwargs,
         )
         pol = nf.get_kernel_scaling_policy()

@@ -239,6 +239,247 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
   }
 }
 
+// ------------------------------------------------------------------------
+// FP64-MFMA form of the same evaluation (real tables, boxes of at most 256
+// nodes): per target box and group of 64 densities, every list-1 pair is a
+// product Y[tid][r] += sum_sid A[tid][sid] X[sid][r] with
+//   A[tid][sid] = D[lev][case][mode(sid)][mode(tid)]  (expanded blocks),
+//   X[sid][r]   = scaling * f[r][sbeg + sid],
+// rows indexed by the target particles and K by the source particles of
+// the pair, so ragged boxes, permuted, repeated or invalid mode ids keep the
+// semantics of the scalar kernel above: an invalid id (or a missing entry,
+// NaN in the expanded blocks) gives A = 0.  v_mfma_f64_16x16x4f64 tiles:
+// wave w owns row tiles w, w + 8 (16 rows each) and all four 16-column
+// tiles; K advances in chunks of 32 source particles.  A chunk's densities
+// go through a two-slot LDS ring and its table fragments through
+// registers, both loaded one chunk ahead; a pair's source modes (and its
+// descriptor) are prepared one pair ahead.  The 2D log displacement is the
+// same for every target of a pair: sum_sid nmlz[sid] f[sid], accumulated in
+// LDS while the densities are staged and added when the pair ends (only
+// used where no entry is missing: the scalar kernel drops the displacement
+// of a missing entry, list1.py:1111-1116).
+constexpr int kNFPitch = 34;  // doubles per column of a staged chunk (32 + 2)
+constexpr int kNFMaxK = 256;  // source particles per box in the MFMA form
+
+typedef __attribute__((ext_vector_type(4))) double nf_doublex4;
+
+struct NFPair {
+  int sbeg, ns;
+  int64_t dbase;  // first value of the pair's expanded block
+  double scal, dscale;
+  int tlev;
+  bool ok;
+};
+
+__device__ inline NFPair nf_pair(const NFParams& p, int li, const double (&ct)[3]) {
+  NFPair d;
+  const int sb = p.nbr_lists[li];
+  const int lvl = p.levels[sb];
+  const double sext = p.root_extent * (1.0 / (double)(1 << lvl));
+  double code = 0.0;
+  for (int k = 0; k < p.dim; ++k) {
+    const double cs = p.centers[(int64_t)k * p.nboxes + sb];
+    code = code * (double)p.enc_base + ((ct[k] - cs) / sext * 4.0 + p.enc_shift + p.enc_bias);
+  }
+  const int64_t vid = (int64_t)(code + 0.5);
+  const int cid = (vid >= 0 && vid < p.n_codes) ? (int)p.case_indices[vid] : -1;
+  d.ok = cid >= 0;
+  d.tlev = 0;
+  d.scal = 1.0;
+  d.dscale = 0.0;
+  if (p.scaling_mode == VL1_SCALING_CANONICAL) {
+    double num = 1.0, den = 1.0;
+    for (int k = 0; k < p.scaling_power; ++k) {
+      num *= sext;
+      den *= p.table_root_extent;
+    }
+    d.scal = num / den;
+    if (p.log_disp) d.dscale = -0.5 / M_PI * d.scal * log(sext / p.table_root_extent);
+  } else if (p.scaling_mode == VL1_SCALING_PER_LEVEL) {
+    const int rel = lvl - p.table_starting_level;
+    d.tlev = lvl < p.table_starting_level ? 0 : (rel < p.n_tables ? rel : p.n_tables - 1);
+  }
+  d.dbase = d.ok ? ((int64_t)d.tlev * p.n_cases + cid) * p.n * p.n : 0;
+  d.sbeg = p.src_starts[sb];
+  d.ns = min(p.src_counts[sb], p.n);
+  if (!d.ok) d.ns = 0;
+  return d;
+}
+
+template <int W, int NRT>
+__global__ void __launch_bounds__(64 * W) near_field_mfma_kernel(const NFParams p, int KP) {
+  constexpr int T = 64 * W;
+  constexpr int VPT = 2048 / T;  // densities of a chunk per thread
+  __shared__ __align__(16) double sX[2][64 * kNFPitch];
+  __shared__ int sMode[2][kNFMaxK];
+  __shared__ double sDisp[2][64];
+
+  const int n = p.n;
+  const int tid = threadIdx.x;
+  const int wave = tid / 64, lane = tid % 64;
+  const int l16 = lane % 16, lg = lane / 16;
+  const int tbox = blockIdx.x;
+  const int r0 = blockIdx.y * 64;
+  const int rc = min(64, p.n_rhs - r0);
+  const int tb = p.target_boxes[tbox];
+  const int tbeg = p.tgt_starts[tb];
+  const int ntb = min(p.tgt_counts[tb], n);
+  const int l0 = p.nbr_starts[tbox];
+  const int npairs = p.nbr_starts[tbox + 1] - l0;
+  const int nkc = KP / 32;
+  double ct[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < p.dim; ++k) ct[k] = p.centers[(int64_t)k * p.nboxes + tb];
+
+  // this lane's A rows (target particles) and their modes (-1: no row)
+  int tmA[NRT];
+#pragma unroll
+  for (int rt = 0; rt < NRT; ++rt) {
+    const int row = 16 * (wave + W * rt) + l16;
+    const int m = row < ntb ? p.tgt_ids[tbeg + row] : -1;
+    tmA[rt] = (unsigned)m < (unsigned)n ? m : -1;
+  }
+
+  // pair k's source modes into sMode[k & 1] (-1 past the box or invalid);
+  // its displacement sums start at zero
+  auto prep_pair = [&](const NFPair& d, int slot) {
+    for (int i = tid; i < KP; i += T) {
+      const int m = i < d.ns ? p.src_ids[d.sbeg + i] : -1;
+      sMode[slot][i] = (unsigned)m < (unsigned)n ? m : -1;
+    }
+    if (tid < 64) sDisp[slot][tid] = 0.0;
+  };
+  // densities of chunk kc of a pair: thread value v is column (tid + T v) / 32,
+  // source particle 32 kc + (tid + T v) % 32 (consecutive threads read
+  // consecutive particles of one density)
+  auto load_x = [&](const NFPair& d, int kc, double (&x)[VPT]) {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      const int u = tid + T * v;
+      const int col = u / 32, sid = 32 * kc + u % 32;
+      x[v] = (col < rc && sid < d.ns) ? p.f[(int64_t)(r0 + col) * p.ldf + d.sbeg + sid] : 0.0;
+    }
+  };
+  auto store_x = [&](const NFPair& d, int kc, int slot, int ring, const double (&x)[VPT]) {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      const int u = tid + T * v;
+      const int col = u / 32, k = u % 32, sid = 32 * kc + k;
+      const bool live = sMode[slot][sid] >= 0;
+      sX[ring][col * kNFPitch + k] = live ? d.scal * x[v] : 0.0;
+      if (p.log_disp && live && x[v] != 0.0)
+        atomicAdd(&sDisp[slot][col], p.nmlz[(int64_t)d.tlev * n + sid] * x[v]);
+    }
+  };
+  // table fragments of chunk kc: lane (l16, lg) of k group u holds
+  // A[row 16 rt16 + l16][sid 32 kc + 4 u + lg]
+  auto load_a = [&](const NFPair& d, int kc, int slot, double (&a)[NRT][8]) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int m = sMode[slot][32 * kc + 4 * u + lg];
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt) {
+        double v = 0.0;
+        if (m >= 0 && tmA[rt] >= 0) v = p.dense[d.dbase + (int64_t)m * n + tmA[rt]];
+        a[rt][u] = v == v ? v : 0.0;  // NaN: no entry
+      }
+    }
+  };
+
+  nf_doublex4 yacc[NRT][4];
+#pragma unroll
+  for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) yacc[rt][c] = nf_doublex4{0.0, 0.0, 0.0, 0.0};
+
+  NFPair pdC{}, pdN{};
+  double ac[NRT][8], an[NRT][8];
+  const int total = npairs * nkc;
+  if (total > 0) {
+    pdC = nf_pair(p, l0, ct);
+    if (tid == 0 && !pdC.ok && p.err) atomicOr(p.err, 1);
+    prep_pair(pdC, 0);
+    if (npairs > 1) {
+      pdN = nf_pair(p, l0 + 1, ct);
+      prep_pair(pdN, 1);
+    }
+    __syncthreads();
+    double x[VPT];
+    load_x(pdC, 0, x);
+    store_x(pdC, 0, 0, 0, x);
+    load_a(pdC, 0, 0, ac);
+  }
+  __syncthreads();
+
+  for (int it = 0; it < total; ++it) {
+    const int k = it / nkc, kc = it % nkc;
+    if (kc == 0 && k > 0) {  // pair k begins: the next pair's descriptor and modes
+      pdC = pdN;
+      if (tid == 0 && !pdC.ok && p.err) atomicOr(p.err, 1);
+      if (k + 1 < npairs) {
+        pdN = nf_pair(p, l0 + k + 1, ct);
+        prep_pair(pdN, (k + 1) & 1);
+      }
+      if (nkc == 1) __syncthreads();  // read within this same iteration
+    }
+    const bool has_next = it + 1 < total;
+    const bool pair_end = kc + 1 == nkc;
+    const int nk = pair_end ? k + 1 : k, nkc_ = pair_end ? 0 : kc + 1;
+    const NFPair& pdX = pair_end ? pdN : pdC;
+    double x[VPT];
+    if (has_next) {
+      load_x(pdX, nkc_, x);
+      load_a(pdX, nkc_, nk & 1, an);
+    }
+    const double* rX = sX[it & 1];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      double bx[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bx[c] = rX[(16 * c + l16) * kNFPitch + 4 * u + lg];
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          yacc[rt][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[rt][u], bx[c], yacc[rt][c], 0, 0, 0);
+    }
+    if (pair_end && p.log_disp) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double ds = pdC.dscale * sDisp[k & 1][16 * c + l16];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) yacc[rt][c][e] += ds;
+      }
+    }
+    if (has_next) {
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ac[rt][u] = an[rt][u];
+      store_x(pdX, nkc_, nk & 1, (it + 1) & 1, x);
+    }
+    __syncthreads();
+  }
+
+  // outputs: lane holds rows 16 rt16 + lg + 4 e, column 16 c + l16
+#pragma unroll
+  for (int rt = 0; rt < NRT; ++rt) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 16 * (wave + W * rt) + lg + 4 * e;
+      if (row >= ntb) continue;
+      const int m = p.tgt_ids[tbeg + row];
+      if ((unsigned)m >= (unsigned)n) continue;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int col = 16 * c + l16;
+        if (col < rc) p.y[(int64_t)(r0 + col) * p.ldy + tbeg + row] = yacc[rt][c][e];
+      }
+    }
+  }
+}
+
 // Expanded table for the DENSE kernels: D[lev][case][s][t] = sign * T[lev,
 // row(case, s, t)], read through the payload's row map.
 template <typename IdxT>
@@ -390,6 +631,33 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
     prm.dense = pl->dense.as<double>();
   }
   prm.n_cases = pl->n_cases;
+  // FP64-MFMA form: real tables, boxes of at most 256 nodes, 16 or more
+  // densities (a pass always computes 64 columns), and no missing entry
+  // under the log displacement (VL1_PLAIN_KERNEL=scalar selects the scalar
+  // kernel below)
+  const char* kern_env = std::getenv("VL1_PLAIN_KERNEL");
+  const bool force_scalar = kern_env && std::string(kern_env) == "scalar";
+  if (dense && !force_scalar && !pl->is_complex && pl->n <= kNFMaxK && a->n_rhs >= 16 &&
+      !(pl->has_missing && prm.log_disp)) {
+    const int m16 = (pl->n + 15) / 16;
+    const int KP = (pl->n + 31) / 32 * 32;
+    const dim3 grid(a->n_tgt_boxes, (a->n_rhs + 63) / 64);
+    hipStream_t st = (hipStream_t)stream;
+    if (m16 <= 1)
+      hipLaunchKernelGGL((near_field_mfma_kernel<1, 1>), grid, dim3(64), 0, st, prm, KP);
+    else if (m16 <= 2)
+      hipLaunchKernelGGL((near_field_mfma_kernel<2, 1>), grid, dim3(128), 0, st, prm, KP);
+    else if (m16 <= 4)
+      hipLaunchKernelGGL((near_field_mfma_kernel<4, 1>), grid, dim3(256), 0, st, prm, KP);
+    else if (m16 <= 8)
+      hipLaunchKernelGGL((near_field_mfma_kernel<8, 1>), grid, dim3(512), 0, st, prm, KP);
+    else
+      hipLaunchKernelGGL((near_field_mfma_kernel<8, 2>), grid, dim3(512), 0, st, prm, KP);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return fail(VL1_E_HIP, std::string("near_field_mfma launch: ") + hipGetErrorString(e));
+    return VL1_OK;
+  }
   const size_t fixed =
       sizeof(double) * ((size_t)pl->n * RC * vw + pl->n + (P > 1 ? kNFBlock * RC * vw : 0)) +
       sizeof(int) * pl->n;

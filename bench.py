@@ -370,6 +370,34 @@ def oracle_check_sample(tree, table, f_host, n_boxes=48, seed=7):
     return [(np.asarray(tree.target_boxes, np.int32)[pos], out)]
 
 
+def oracle_check_sample_numpy(tree, tables, f_host, mode, n_boxes=16, seed=7, **kw):
+    """The numpy oracle (complex data, per-level tables; own list-1 CSR) on
+    a seeded sample of `n_boxes` target boxes; returns [(boxes, output)]
+    like ``oracle_check_sample``."""
+    from oracle import list1_ref, orbit_ref
+
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    tb_all = np.asarray(tree.target_boxes, np.int32)
+    sel = np.sort(rng.choice(tb_all.size, size=min(n_boxes, tb_all.size), replace=False))
+    boxes = tb_all[sel]
+    st, ls = list1_ref.list1_csr_bruteforce(tree.box_centers, tree.box_levels, tree.root_extent,
+                                            boxes, box_group=list1_ref.tree_groups(tree))
+    out = list1_ref.near_field(
+        dim=tree.dim, q=tree.quad_order, box_centers=tree.box_centers,
+        box_levels=tree.box_levels, box_starts=tree.box_target_starts,
+        box_counts=tree.box_target_counts_nonchild, target_boxes=boxes, nbr_starts=st,
+        nbr_lists=ls, source_mode_ids=tree.local_ids, target_point_ids=tree.local_ids,
+        table_data=np.stack([t.compact_values() for t in tables]),
+        mode_nmlz=np.stack([t.mode_normalizers for t in tables]),
+        meta=orbit_ref.case_metadata(tree.dim, tree.quad_order), mode=mode,
+        root_extent=tree.root_extent, table_root_extent=tables[0].source_box_extent,
+        mode_coefs=f_host, **kw)
+    log(f"oracle check sample (numpy): {boxes.size} boxes x {f_host.shape[0]} RHS in "
+        f"{time.perf_counter() - t0:.1f} s")
+    return [(boxes, out)]
+
+
 def sample_points(tree, sel):
     """Tree-order point indices of the target boxes `sel`."""
     return np.concatenate([np.arange(tree.box_target_starts[b],
@@ -396,7 +424,8 @@ def error_vs_oracle(tree, samples, forms_out, full_ref=None):
         for sel, ref in samples:
             idx = sample_points(tree, sel)
             R_ = ref.shape[0]
-            got = y[:R_, idx].double().cpu().numpy()
+            got = y[:R_, idx].cpu().numpy()
+            got = got.astype(np.complex128 if np.iscomplexobj(got) else np.float64)
             want = ref[:, idx]
             d = got - want
             sl2.extend(np.linalg.norm(d, axis=1) / np.linalg.norm(want, axis=1))
@@ -406,7 +435,7 @@ def error_vs_oracle(tree, samples, forms_out, full_ref=None):
                 ymax = fr.abs().amax(dim=1).cpu().numpy()
                 ynorm = fr.norm(dim=1).cpu().numpy()
                 li.extend(np.abs(d).max(axis=1) / ymax)
-                l2.extend(np.sqrt((d * d).sum(axis=1) * n_all / idx.size) / ynorm)
+                l2.extend(np.sqrt((np.abs(d) ** 2).sum(axis=1) * n_all / idx.size) / ynorm)
             npt += idx.size
             nr = max(nr, R_)
         rec = {"rhs": nr, "points": int(npt)}
@@ -634,7 +663,16 @@ def run_c4(args):
                         device="cuda").contiguous()
     n = 36
     flops = 8.0 * R * fo.n_list1_pairs * n * n
-    forms, _ = _plan_forms(plan, f, R, fo.n_points, flops, args.steps, args.warmup, args.slices, True)
+    forms, ys = _plan_forms(plan, f, R, fo.n_points, flops, args.steps, args.warmup, args.slices,
+                            True)
+    outs = {"dense_f64_mfma": ys["f64"], "split_bf16": ys["split"]}
+    if "split_i8" in ys:
+        outs["split_i8"] = ys["split_i8"]
+    errs = error_vs_oracle(fo, oracle_check_sample_numpy(
+        fo, tables, f.cpu().numpy(), "per_level", table_starting_level=4), outs, full_ref=ys["f64"])
+    for k_, v_ in errs.items():
+        log(f"{k_} vs oracle: rel L2 {v_['rel_l2_per_rhs_max']:.2e}, rel Linf "
+            f"{v_['rel_linf_per_rhs_max']:.2e} ({v_['rhs']} RHS, {v_['points']} points)")
     key = _headline_split(forms)
     sp = forms[key]
     dtype, label, roof = _split_line_parts(forms, key, flops, args.slices)
@@ -648,6 +686,7 @@ def run_c4(args):
         forms,
         {"error_vs_fp64": {"rel_l2_per_rhs_max": sp["rel_l2_max"],
                            "rel_linf_per_rhs_max": sp["rel_linf_max"]},
+         "error_vs_oracle": errs,
          "speedup_vs_dense_f64_mfma": forms["f64"]["ms"] / sp["ms"]})
 
 

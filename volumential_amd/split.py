@@ -12,6 +12,7 @@ the launch.
 
 from __future__ import annotations
 
+import logging
 import math
 
 import numpy as np
@@ -20,6 +21,8 @@ import torch
 from . import _capi
 
 EULER_GAMMA = 0.5772156649015329
+
+logger = logging.getLogger(__name__)
 
 
 def series_nmax(dim, k, r_max, split_order=1):
@@ -48,6 +51,34 @@ def series_nmax(dim, k, r_max, split_order=1):
         n += 1
         mag = mag * z2 / (n * n)
     return min(max_n, n + 2)
+
+
+def series_roundoff(dim, k, r_max):
+    """Relative rounding level of the remainder series at the largest list-1
+    distance: unit roundoff times the largest term magnitude (the magnitude
+    recursion of ``series_nmax``; the remainder itself is of order one in
+    the same units).  At large |k| r the terms grow to about e^{|k| r} before
+    they fall, and the alternating sum loses that many digits; the
+    reference's series (expansion_wrangler_fpnd.py:212-312) behaves the
+    same.  An estimate, not a bound."""
+    z = float(abs(complex(k))) * float(r_max)
+    if z == 0.0:
+        return 0.0
+    big, n = 1.0, 1
+    if dim == 3:
+        mag = z
+        while n < 96:
+            big = max(big, mag)
+            n += 1
+            mag = mag * z / n
+    else:
+        z2 = 0.25 * z * z
+        mag = z2
+        while n < 96:
+            big = max(big, mag)
+            n += 1
+            mag = mag * z2 / (n * n)
+    return 1.1e-16 * big
 
 
 def max_neighbor_distance(tree):
@@ -94,8 +125,18 @@ class HelmholtzSplitCorrection:
             raise ValueError("bad deriv_kind")
         if self.deriv_kind and not 0 <= self.deriv_axis < self.dim:
             raise ValueError("deriv_axis out of range")
+        r_max = max_neighbor_distance(tree)
         self.nmax = int(series_terms) if series_terms is not None else series_nmax(
-            self.dim, self.k, max_neighbor_distance(tree))
+            self.dim, self.k, r_max)
+        # the series cancels badly at large |k| r: say so instead of
+        # returning a silently inaccurate remainder
+        self.roundoff = series_roundoff(self.dim, self.k, r_max)
+        if self.roundoff > 1e-10:
+            logger.warning(
+                "Helmholtz split remainder: |k| r_max = %.3g; the series loses digits to "
+                "cancellation (estimated relative error %.1e of the remainder). Refine the "
+                "tree or use per-level Helmholtz tables without the split.",
+                abs(self.k) * r_max, self.roundoff)
         self.n_points = int(np.asarray(tree.points).shape[1])
         dev = "cuda"
         self._points = torch.as_tensor(np.ascontiguousarray(tree.points, np.float64), device=dev)

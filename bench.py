@@ -1124,6 +1124,14 @@ def main():
                               "tflops": flops / (ms_plain * 1e-3) / 1e12,
                               "fp64_frac": flops / (ms_plain * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
                               "hbm_gbs_alg": bytes_alg / (ms_plain * 1e-3) / 1e9}
+        if R >= 16:  # the MFMA kernel of the plain route: its counter traffic
+            kp = "near_field_mfma_kernel<8, 2>" if n > 128 else None
+            tb, src = measured_traffic(kp, level, q, R) if kp else (None, None)
+            if tb is not None:
+                forms["plain_f64"].update({"kernel": kp, "traffic_bytes": tb,
+                                           "hbm_gbs_measured": tb / (ms_plain * 1e-3) / 1e9,
+                                           "hbm_frac": tb / (ms_plain * 1e-3) / 1e12 / 8.0,
+                                           "traffic_source": src})
         log(f"plain f64: {ms_plain:.2f} ms/step, {forms['plain_f64']['value']:.3f} Gpts*RHS/s, "
             f"{forms['plain_f64']['tflops']:.2f} TFLOP/s")
         del runner

@@ -5,7 +5,7 @@ memset) is replayed several times with new densities written between the
 replays by ordinary eager work; every replay must match an eager evaluation
 of the same densities to 1e-13 relative (same kernels and inputs; the
 order of a tile's steps follows a racy L2 hint, so FP64 sums may differ in
-the last bits), for both forms and for real and complex data.
+the last bits), for the FP64, BF16 and int8 forms and for real and complex data.
 """
 
 import numpy as np
@@ -35,7 +35,7 @@ def _replays(plan, make_f, dtype, form):
         assert out.abs().max() > 0
 
 
-@pytest.mark.parametrize("form", ["f64", "split"])
+@pytest.mark.parametrize("form", ["f64", "split", "split_i8"])
 def test_graph_replay_real(cuda, form):
     print(f"case: graph replay real 3D q=6 level 2 form={form}", flush=True)
     tr = vtree.uniform_tree(3, 6, 2)
@@ -51,7 +51,7 @@ def test_graph_replay_real(cuda, form):
     _replays(plan, make_f, torch.float64, form)
 
 
-@pytest.mark.parametrize("form", ["f64", "split"])
+@pytest.mark.parametrize("form", ["f64", "split", "split_i8"])
 def test_graph_replay_complex(cuda, form):
     print(f"case: graph replay complex 2D forest form={form}", flush=True)
     fo = beam_forest([3], q=6, tol=1e-4, min_level=4, max_level=5)

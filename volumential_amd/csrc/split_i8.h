@@ -460,18 +460,22 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
       return *reinterpret_cast<const i32x4*>(arow + blk * blk_stride + ((int64_t)kc * S + i) * 1024);
     };
     __syncthreads();  // the first D chunks and their block ids are in the ring
-    // Table fragments: slices i < NB of the next chunk are loaded at the top
-    // of the current one into the other of two register sets (ping-pong:
-    // the loop runs two chunks per trip, so no register copies sit between
-    // a load and its use); the others (S = 6 only, for registers) are
-    // refilled in place right after their last use (slice i meets density
-    // slices 0..S-1-i).  Density slice 0 of a chunk takes the table slices
-    // in the order they were loaded: the ping-pong ones, then S-1 down to
-    // NB.  The loads are unconditional (the last chunk reloads block 0,
-    // chunk 0), so no branch separates a load from its wait.
+    // Table fragments: slices i < NB of chunk it + NA - 1 are loaded at the
+    // top of chunk it into one of NA register sets used in rotation (the
+    // loop runs NA chunks per trip, so no register copies sit between a load
+    // and its use; S <= 5 loads two chunks ahead, which covers a miss in L2);
+    // the others (S = 6 only, for registers) are refilled in place right
+    // after their last use (slice i meets density slices 0..S-1-i).  Density
+    // slice 0 of a chunk takes the table slices in the order they were
+    // loaded: the rotating ones, then S-1 down to NB.  The loads are
+    // unconditional (past the last chunk they reload an older, valid block),
+    // so no branch separates a load from its wait.
     constexpr int NB = S <= 5 ? S : 3;
     constexpr int NSH = S - NB > 0 ? S - NB : 1;
-    i32x4 app[2][NB], ash[NSH];
+    // NA register sets of the NB ping-pong slices: the loads for chunk it + NA - 1
+    // are issued at the top of chunk it (two chunks ahead where registers allow)
+    constexpr int NA = S <= 5 ? 3 : 2;
+    i32x4 app[NA][NB], ash[NSH];
     if (total > 0) {
       const int blk = sBlk[0];
 #pragma unroll
@@ -479,6 +483,11 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
         if (i < NB) app[0][i] = afrag(blk, 0, i);
         else ash[i - NB] = afrag(blk, 0, i);
       }
+    }
+    if (NA == 3 && total > 1) {
+      const int blk = sBlk[1 % NS];
+#pragma unroll
+      for (int i = 0; i < NB; ++i) app[NA - 2][i] = afrag(blk, 1 % nkc, i);
     }
     const unsigned char* rsl = sR + (lane / 32) * 1024 + (lane % 32) * 16;
     // S <= 5: density slice 0 of the next chunk is read from the ring while
@@ -492,14 +501,16 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
       bp1 = *reinterpret_cast<const i32x4*>(rsl + 512);
     }
     auto chunk = [&](int it, auto pc) {
-      constexpr int P = decltype(pc)::value;  // register set of this chunk
-      const int kc = it % nkc;
-      // the next chunk's block id was written at least one barrier ago (after
-      // the last chunk the slot holds an older, still valid, block id)
+      constexpr int P = decltype(pc)::value;  // register set of this chunk (it % NA)
+      // the block ids of the next NA - 1 chunks were written at least one
+      // barrier ago (after the last chunk a slot holds an older, still valid,
+      // block id)
       const int nblk = sBlk[(it + 1) % NS];
-      const int nkc_ = kc + 1 == nkc ? 0 : kc + 1;
+      const int nkc_ = (it + 1) % nkc;
+      const int ablk = NA == 2 ? nblk : sBlk[(it + NA - 1) % NS];
+      const int akc = NA == 2 ? nkc_ : (it + NA - 1) % nkc;
 #pragma unroll
-      for (int i = 0; i < NB; ++i) app[1 - P][i] = afrag(nblk, nkc_, i);
+      for (int i = 0; i < NB; ++i) app[(P + NA - 1) % NA][i] = afrag(ablk, akc, i);
       // keep these loads at the top: the scheduler would otherwise sink them
       // below the MFMAs, leaving their latency to the next chunk's start
       __builtin_amdgcn_sched_barrier(0);
@@ -528,12 +539,25 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
       });
       __syncthreads();  // ring slot it % NS may be refilled
     };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, NA == 3 ? 2 : 0>;
     int it = 0;
-    for (; it + 1 < total; it += 2) {
-      chunk(it, std::integral_constant<int, 0>{});
-      chunk(it + 1, std::integral_constant<int, 1>{});
+    if constexpr (NA == 3) {
+      for (; it + 2 < total; it += 3) {
+        chunk(it, I0{});
+        chunk(it + 1, I1{});
+        chunk(it + 2, I2{});
+      }
+      if (it < total) chunk(it, I0{});
+      if (it + 1 < total) chunk(it + 1, I1{});
+    } else {
+      for (; it + 1 < total; it += 2) {
+        chunk(it, I0{});
+        chunk(it + 1, I1{});
+      }
+      if (it < total) chunk(it, I0{});
     }
-    if (it < total) chunk(it, std::integral_constant<int, 0>{});
     __syncthreads();  // end of the step loop (the producer's log-term sums)
 
     // epilogue: y = rho sigma sum_d acc_d W_d + sum disp, per output; the

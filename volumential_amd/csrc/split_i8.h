@@ -490,15 +490,17 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
       for (int i = 0; i < NB; ++i) app[NA - 2][i] = afrag(blk, 1 % nkc, i);
     }
     const unsigned char* rsl = sR + (lane / 32) * 1024 + (lane % 32) * 16;
-    // S <= 5: density slice 0 of the next chunk is read from the ring while
-    // this chunk's MFMAs run (the producer stored that chunk before the
-    // previous barrier: it runs NS - 1 >= 2 chunks ahead), so a chunk's first
-    // MFMAs do not wait on LDS
-    constexpr bool PRE_B = S <= 5;
-    i32x4 bp0, bp1;
-    if (PRE_B && total > 0) {
-      bp0 = *reinterpret_cast<const i32x4*>(rsl);
-      bp1 = *reinterpret_cast<const i32x4*>(rsl + 512);
+    // Density fragments (S <= 5; S = 6 has no registers for it): slice j + 1
+    // (after the last slice, slice 0 of the next chunk) is read from the
+    // ring before slice j's MFMAs are issued, so its LDS latency runs under
+    // them.  The next chunk was stored before the previous barrier (the
+    // producer runs NS - 1 >= 2 chunks ahead); after the last chunk the read
+    // hits a stale slot and is not used.
+    constexpr bool AHEAD_B = S <= 5;
+    i32x4 bc0, bc1;
+    if (AHEAD_B && total > 0) {
+      bc0 = *reinterpret_cast<const i32x4*>(rsl);
+      bc1 = *reinterpret_cast<const i32x4*>(rsl + 512);
     }
     auto chunk = [&](int it, auto pc) {
       constexpr int P = decltype(pc)::value;  // register set of this chunk (it % NA)
@@ -515,11 +517,21 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
       // below the MFMAs, leaving their latency to the next chunk's start
       __builtin_amdgcn_sched_barrier(0);
       const unsigned char* rs = rsl + (size_t)(it % NS) * SLOT;
+      const unsigned char* rn = rsl + (size_t)((it + 1) % NS) * SLOT;
       static_for<0, S>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        const i32x4 b0 = (PRE_B && j == 0) ? bp0 : *reinterpret_cast<const i32x4*>(rs + j * 2048);
-        const i32x4 b1 =
-            (PRE_B && j == 0) ? bp1 : *reinterpret_cast<const i32x4*>(rs + j * 2048 + 512);
+        i32x4 b0, b1;
+        if constexpr (AHEAD_B) {
+          b0 = bc0;
+          b1 = bc1;
+          const unsigned char* src = j + 1 < S ? rs + (j + 1) * 2048 : rn;
+          bc0 = *reinterpret_cast<const i32x4*>(src);
+          bc1 = *reinterpret_cast<const i32x4*>(src + 512);
+          __builtin_amdgcn_sched_barrier(0);  // the reads go out before the MFMAs
+        } else {
+          b0 = *reinterpret_cast<const i32x4*>(rs + j * 2048);
+          b1 = *reinterpret_cast<const i32x4*>(rs + j * 2048 + 512);
+        }
         static_for<0, S - j>([&](auto ic) {
           constexpr int u = decltype(ic)::value;
           constexpr int i = (j == 0 && u >= NB) ? S - 1 - (u - NB) : u;
@@ -529,13 +541,6 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
         });
         // table slice S-1-j had its last partner: refill it for the next chunk
         if constexpr (S - 1 - j >= NB) ash[S - 1 - j - NB] = afrag(nblk, nkc_, S - 1 - j);
-        if constexpr (PRE_B && j == 0) {
-          // the next chunk's density slice 0 (a stale slot after the last
-          // chunk); the producer stores into slot (it + NS - 1) % NS meanwhile
-          const unsigned char* rn = rsl + (size_t)((it + 1) % NS) * SLOT;
-          bp0 = *reinterpret_cast<const i32x4*>(rn);
-          bp1 = *reinterpret_cast<const i32x4*>(rn + 512);
-        }
       });
       __syncthreads();  // ring slot it % NS may be refilled
     };

@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(64 * W) near_field_mfma_kernel(const NFParams 
 
   const int n = p.n;
   const int tid = threadIdx.x;
-  const int wave = tid / 64, lane = tid % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64;
   const int l16 = lane % 16, lg = lane / 16;
   const int tbox = blockIdx.x;
   const int r0 = blockIdx.y * 64;
@@ -330,6 +330,8 @@ __global__ void __launch_bounds__(64 * W) near_field_mfma_kernel(const NFParams 
   double ct[3] = {0.0, 0.0, 0.0};
   for (int k = 0; k < p.dim; ++k) ct[k] = p.centers[(int64_t)k * p.nboxes + tb];
 
+  // whether this wave's second row tile holds any of the box's targets
+  const bool rows_rt1 = NRT > 1 && 16 * (wave + W) < ntb;
   // this lane's A rows (target particles) and their modes (-1: no row)
   int tmA[NRT];
 #pragma unroll
@@ -438,9 +440,10 @@ __global__ void __launch_bounds__(64 * W) near_field_mfma_kernel(const NFParams 
       for (int c = 0; c < 4; ++c) bx[c] = rX[(16 * c + l16) * kNFPitch + 4 * u + lg];
 #pragma unroll
       for (int rt = 0; rt < NRT; ++rt)
+        if (rt == 0 || rows_rt1)  // a second row tile past the box's targets is skipped
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-          yacc[rt][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[rt][u], bx[c], yacc[rt][c], 0, 0, 0);
+          for (int c = 0; c < 4; ++c)
+            yacc[rt][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[rt][u], bx[c], yacc[rt][c], 0, 0, 0);
     }
     if (pair_end && p.log_disp) {
 #pragma unroll

@@ -219,7 +219,9 @@ __global__ void __launch_bounds__(256)
   int* soff = reinterpret_cast<int*>(sf);  // rows ints
   double* sv = sf + (rows + 1) / 2;
   const int cap = (kI8ColsLds - (rows + 1) / 2) / LS;  // >= 1: 1.5 rows + 1 <= kI8ColsLds
-  const int RR = cap < R ? cap : R;
+  // columns per round; a multiple of 4 where possible, so a round's digit
+  // words fill whole 128-byte lines (4 columns x 32 bytes per K chunk)
+  const int RR = cap < R ? (cap >= 4 ? cap / 4 * 4 : cap) : R;
   const int64_t bs = box_starts[box];
   const int64_t fo = cplx ? 2 * bs : bs;
   const int32_t* mp = mode_point + (int64_t)box * n;

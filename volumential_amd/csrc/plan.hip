@@ -1730,10 +1730,13 @@ static int plan_check(const vl1_plan* pl, int form, int n_rhs, const vl1_split_o
 }
 
 // Ring slots of the int8 tile kernel (at least 3: the consumers read the
-// next chunk's block id one barrier after the producer wrote it).
-static int i8_ring_slots(int S) {
+// next chunk's block id one barrier after the producer wrote it).  A
+// workgroup of at most 4 waves (blocks of at most 96 rows) gets a ring small
+// enough for two workgroups per CU, so each SIMD runs two waves.
+static int i8_ring_slots(int S, int waves) {
+  const size_t budget = waves <= 4 ? 80 * 1024 : 160 * 1024;
   int ns = kI8Ring;
-  while (ns > 3 && i8_lds_bytes(S, ns) > 160 * 1024) --ns;
+  while (ns > 3 && i8_lds_bytes(S, ns) > budget) --ns;
   return ns;
 }
 
@@ -1891,7 +1894,7 @@ static int contract_launch(vl1_plan* pl, int form, int S, TileSchedule* sch, int
   prm.sigma = st.sigma;
   prm.box_disp = pl->box_disp.as<double>();
   prm.est_part = error_est ? st.est_part : nullptr;
-  prm.nslot = i8_ring_slots(S);
+  prm.nslot = i8_ring_slots(S, pl->MP / 32 + 1);
   prm.log_disp = pl->log_disp;
   prm.rowscale = pl->rowscale.as<double>();
   prm.X = st.X;

@@ -1826,18 +1826,39 @@ static int stage_launch(vl1_plan* pl, int form, int S, const double* f, int64_t 
   if (form == VL1_FORM_SPLIT_I8) {
     // column scale per RHS: the largest |f alpha| over all boxes (per-column
     // maxima, then one reduction per RHS)
-    const int64_t cm_waves = (int64_t)((pl->nboxes + kI8ColmaxBoxes - 1) / kI8ColmaxBoxes) * R;
-    hipLaunchKernelGGL(i8_colmax_kernel, dim3((unsigned)((cm_waves + wpb - 1) / wpb)), dim3(64 * wpb), 0, s,
-                       pl->n, pl->cplx, pl->nboxes, R, f, ldf, pl->box_starts.as<int32_t>(),
-                       pl->box_alpha.as<double>(), nmlz, pl->log_disp, st.colscale, st.nmdot);
-    hipLaunchKernelGGL(i8_sigma_kernel, dim3(R), dim3(256), 0, s, pl->nboxes, R, st.colscale,
-                       st.sigma);
-    const dim3 g(pl->nboxes), b(256);
+    const int rows = pl->cplx ? 2 * pl->n : pl->n;
+    const int KW = rows <= 64 ? 1 : (rows <= 128 ? 2 : 4);
+    const int cm_boxes = KW == 4 ? 4 : 8;  // boxes per wavefront of i8_colmax_kernel<KW>
+    const int64_t cm_waves = (int64_t)((pl->nboxes + cm_boxes - 1) / cm_boxes) * R;
+    const dim3 cg((unsigned)((cm_waves + wpb - 1) / wpb)), cb(64 * wpb);
+#define VL1_I8_COLMAX(KV)                                                                        \
+  hipLaunchKernelGGL(i8_colmax_kernel<KV>, cg, cb, 0, s, pl->n, pl->cplx, pl->nboxes, R, f, ldf, \
+                     pl->box_starts.as<int32_t>(), pl->box_alpha.as<double>(), nmlz,            \
+                     pl->log_disp, st.colscale, st.nmdot)
+    if (KW == 1)
+      VL1_I8_COLMAX(1);
+    else if (KW == 2)
+      VL1_I8_COLMAX(2);
+    else
+      VL1_I8_COLMAX(4);
+#undef VL1_I8_COLMAX
+    // maxima over the boxes in two stages (G partial maxima per RHS, then one)
+    const int G = std::max(1, std::min(256, pl->nboxes / 1024));
+    if (G > 1)
+      hipLaunchKernelGGL(i8_colmax_fold_kernel, dim3(G, R), dim3(256), 0, s, pl->nboxes, R,
+                         st.colscale);
+    hipLaunchKernelGGL(i8_sigma_kernel, dim3(R), dim3(256), 0, s, G > 1 ? G : pl->nboxes, R,
+                       st.colscale, st.sigma);
+    const int RR = i8_cols_round(rows, R);
+    const int tasks = RR * (pl->KP / 32) * 2;  // 16 K values of one column each
+    const int threads = tasks > 128 ? 256 : (tasks > 64 ? 128 : 64);
+    const size_t lds = sizeof(double) * ((size_t)(rows + 1) / 2 + (size_t)RR * (rows + 1));
+    const dim3 g(pl->nboxes), b(threads);
     switch (S) {
 #define VL1_I8_COLS(SV)                                                                          \
   case SV:                                                                                       \
-    hipLaunchKernelGGL(i8_split_cols_kernel<SV>, g, b, 0, s, pl->n, pl->cplx, pl->KP,           \
-                       pl->nboxes, R, f, ldf, pl->mode_point.as<int32_t>(),                       \
+    hipLaunchKernelGGL(i8_split_cols_kernel<SV>, g, b, lds, s, pl->n, pl->cplx, pl->KP,         \
+                       pl->nboxes, R, RR, f, ldf, pl->mode_point.as<int32_t>(),                   \
                        pl->box_starts.as<int32_t>(), pl->box_alpha.as<double>(), st.sigma,        \
                        st.X8);                                                                    \
     break;
@@ -1916,8 +1937,13 @@ static int contract_launch(vl1_plan* pl, int form, int S, TileSchedule* sch, int
       e = launch_i8(S, dim3(grid), dim3(64 * waves), i8_lds_bytes(S, prm.nslot), s, prm);
       if (e != hipSuccess) return fail(VL1_E_HIP, std::string("int8 split launch: ") + hipGetErrorString(e));
       if (error_est) {
-        hipLaunchKernelGGL(i8_rhs_error_kernel, dim3(Rc), dim3(256), 0, s, sch->n_tiles, sch->B, Rc,
-                           r0, st.est_part, error_est);
+        // two stages over the tiles (G partials per RHS, then one)
+        const int G = std::max(1, std::min(256, sch->n_tiles / 256));
+        if (G > 1)
+          hipLaunchKernelGGL(i8_est_fold_kernel, dim3(G, Rc), dim3(256), 0, s, sch->n_tiles, sch->B,
+                             Rc, st.est_part);
+        hipLaunchKernelGGL(i8_rhs_error_kernel, dim3(Rc), dim3(256), 0, s, G > 1 ? G : sch->n_tiles,
+                           G > 1 ? 1 : sch->B, Rc, r0, st.est_part, error_est);
         e = hipGetLastError();
         if (e != hipSuccess) return fail(VL1_E_HIP, std::string("error estimate launch: ") + hipGetErrorString(e));
       }

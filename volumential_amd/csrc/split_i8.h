@@ -113,17 +113,17 @@ __global__ void i8_split_rows_kernel(int MP, int KP, int n_blocks, int G,
 // column and, for the 2D log term, nmdot[box][r] = sum_sid nmlz[sid]
 // f[sbeg + sid].  A source box's particles are contiguous from
 // box_starts[box], so both read the column in place.  One wavefront per RHS
-// row and kI8ColmaxBoxes consecutive boxes (columns of up to 256 values: all
-// loads issued before the first reduction).  A box that is no list-1 source
-// has alpha 0 (the plan rejects a zero pair scaling) and gets 0 without
-// reading.
-constexpr int kI8ColmaxBoxes = 4;
+// row and 8 consecutive boxes (4 when KW = 4), a column taking KW loads of
+// 64 values (columns of up to 64 KW values; longer ones loop): all loads
+// issued before the first reduction.  A box that is no list-1 source has
+// alpha 0 (the plan rejects a zero pair scaling) and gets 0 without reading.
+template <int KW>
 __global__ void i8_colmax_kernel(int n, int cplx, int nboxes, int R, const double* __restrict__ f,
                                  int64_t ldf, const int32_t* __restrict__ box_starts,
                                  const double* __restrict__ box_alpha,
                                  const double* __restrict__ nmlz, int log_disp,
                                  double* __restrict__ colmax, double* __restrict__ nmdot) {
-  constexpr int NB = kI8ColmaxBoxes;
+  constexpr int NB = KW == 4 ? 4 : 8;  // boxes per wavefront (registers)
   const int ngrp = (nboxes + NB - 1) / NB;
   const int64_t w = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
@@ -141,17 +141,17 @@ __global__ void i8_colmax_kernel(int n, int cplx, int nboxes, int R, const doubl
     fb[b] = fr + (cplx ? 2 * bs : bs);
     mx[b] = 0.0;
   }
-  if (rows <= 256) {
-    double v[NB][4];
+  if (rows <= 64 * KW) {
+    double v[NB][KW];
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
+      for (int k = 0; k < KW; ++k)
         v[b][k] = (alpha[b] != 0.0 && lane + 64 * k < rows) ? fb[b][lane + 64 * k] : 0.0;
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) mx[b] = fmax(mx[b], fabs(v[b][k]));
+      for (int k = 0; k < KW; ++k) mx[b] = fmax(mx[b], fabs(v[b][k]));
   } else {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
@@ -173,6 +173,25 @@ __global__ void i8_colmax_kernel(int n, int cplx, int nboxes, int R, const doubl
       nmdot[(int64_t)box * R + r] = dot;
     }
   }
+}
+
+// Two-stage maxima over many boxes: block (g, r) folds column r of boxes g,
+// g + G, g + 2G, ... into colmax[g][r] (in place: no other block touches
+// those entries); then i8_sigma_kernel over the first G boxes.
+__global__ void __launch_bounds__(256) i8_colmax_fold_kernel(int nboxes, int R,
+                                                             double* __restrict__ colmax) {
+  __shared__ double red[256];
+  const int G = gridDim.x, g = blockIdx.x, r = blockIdx.y;
+  double mx = 0.0;
+  for (int64_t b = g + (int64_t)G * threadIdx.x; b < nboxes; b += (int64_t)G * blockDim.x)
+    mx = fmax(mx, colmax[b * R + r]);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) colmax[(int64_t)g * R + r] = red[0];
 }
 
 // sigma[r] = max over boxes of colmax[box][r]; one block per RHS.
@@ -201,15 +220,28 @@ __global__ void __launch_bounds__(256) i8_sigma_kernel(int nboxes, int R,
 // 16-byte digit words, consecutive threads writing consecutive 16-byte
 // pieces.  Boxes that are no source (alpha 0) are skipped: no step reads
 // their digits.
-constexpr int kI8ColsLds = 4096;  // doubles: offsets + one round of columns
+constexpr int kI8ColsLds = 4096;  // doubles at most: offsets + one round of columns
+
+// Columns per round of i8_split_cols_kernel: as many as kI8ColsLds holds, a
+// multiple of 4 where possible, so a round's digit words fill whole 128-byte
+// lines (4 columns x 32 bytes per K chunk); the block's LDS is
+// (rows + 1) / 2 + RR (rows + 1) doubles, its size 64 to 256 threads by the
+// round's work (small boxes take small blocks).
+inline int i8_cols_round(int rows, int R) {
+  const int cap = (kI8ColsLds - (rows + 1) / 2) / (rows + 1);  // >= 1: checked by the plan
+  const int c4 = cap >= 4 ? cap / 4 * 4 : cap;
+  return c4 < R ? c4 : R;
+}
+
 template <int S>
 __global__ void __launch_bounds__(256)
-    i8_split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, const double* __restrict__ f,
-                         int64_t ldf, const int32_t* __restrict__ mode_point,
+    i8_split_cols_kernel(int n, int cplx, int KP, int nboxes, int R, int RR,
+                         const double* __restrict__ f, int64_t ldf,
+                         const int32_t* __restrict__ mode_point,
                          const int32_t* __restrict__ box_starts,
                          const double* __restrict__ box_alpha, const double* __restrict__ sigma,
                          int8_t* __restrict__ X8) {
-  __shared__ double sf[kI8ColsLds];
+  extern __shared__ double sf[];
   const int box = blockIdx.x;
   const double alpha = box_alpha[box];
   if (alpha == 0.0) return;
@@ -218,10 +250,6 @@ __global__ void __launch_bounds__(256)
   const int LS = rows + 1;
   int* soff = reinterpret_cast<int*>(sf);  // rows ints
   double* sv = sf + (rows + 1) / 2;
-  const int cap = (kI8ColsLds - (rows + 1) / 2) / LS;  // >= 1: 1.5 rows + 1 <= kI8ColsLds
-  // columns per round; a multiple of 4 where possible, so a round's digit
-  // words fill whole 128-byte lines (4 columns x 32 bytes per K chunk)
-  const int RR = cap < R ? (cap >= 4 ? cap / 4 * 4 : cap) : R;
   const int64_t bs = box_starts[box];
   const int64_t fo = cplx ? 2 * bs : bs;
   const int32_t* mp = mode_point + (int64_t)box * n;
@@ -265,6 +293,48 @@ __global__ void __launch_bounds__(256)
       }
     }
     __syncthreads();
+  }
+}
+
+// Stage 1 of the error estimates over many tiles: block (g, r) folds the
+// partials of RHS r (columns b Rc + r, b < B) of tiles g, g + G, ... into
+// the entry of tile g, column r (in place: no other block touches those
+// entries); i8_rhs_error_kernel then reads tiles < G with B = 1.
+__global__ void __launch_bounds__(256) i8_est_fold_kernel(int n_tiles, int B, int Rc,
+                                                          double* __restrict__ part) {
+  __shared__ double red[4][256];
+  const int G = gridDim.x, g = blockIdx.x, r = blockIdx.y;
+  double em = 0.0, e2 = 0.0, ym = 0.0, y2 = 0.0;
+  const int64_t per = ((int64_t)n_tiles - g + G - 1) / G;  // tiles g, g + G, ...
+  for (int64_t i = threadIdx.x; i < per * B; i += blockDim.x) {
+    const int64_t tile = g + (i / B) * G;
+    const int c = (int)(i % B) * Rc + r;
+    const double* q = part + (tile * 64 + c) * 4;
+    em = fmax(em, q[0]);
+    e2 += q[1];
+    ym = fmax(ym, q[2]);
+    y2 += q[3];
+  }
+  red[0][threadIdx.x] = em;
+  red[1][threadIdx.x] = e2;
+  red[2][threadIdx.x] = ym;
+  red[3][threadIdx.x] = y2;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+      red[1][threadIdx.x] += red[1][threadIdx.x + o];
+      red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + o]);
+      red[3][threadIdx.x] += red[3][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double* q = part + ((int64_t)g * 64 + r) * 4;
+    q[0] = red[0][0];
+    q[1] = red[1][0];
+    q[2] = red[2][0];
+    q[3] = red[3][0];
   }
 }
 

@@ -213,6 +213,39 @@ def test_every_variant_matches_oracle(cuda):
     plan.set_variant(None, form="split", n_slices=5, n_rhs=4)
 
 
+def test_phase_aligned_variants_repeatable(cuda):
+    """Phase alignment (tiles start at the table block their XCD is on, read
+    from a word other workgroups rewrite while they run) only reorders a
+    tile's steps: repeated evaluations of a 16,384-box plan with it on agree
+    with the fixed-order variant to rounding, every time.  A workgroup whose
+    waves took different starting steps (each reading the word on its own)
+    mixed one step's densities with another's table block, now and then."""
+    print("case: phase-aligned variants, repeated, uniform 2D q=6 level 7, 16 RHS", flush=True)
+    tr = vtree.uniform_tree(2, 6, 7)
+    tables = synthetic_table_levels(2, 6, seed=3)
+    f = densities.gaussian_mixture_batch(tr.points, range(50, 66))
+    w = NearFieldWrangler(tr, tables[0])
+    plan = w.get_plan(w.target_kernels[0], tr.target_boxes, tr.neighbor_source_boxes_starts,
+                      tr.neighbor_source_boxes_lists)
+    fd = torch.as_tensor(f, device="cuda").contiguous()
+    for form in ("f64", "split"):
+        worst = 0.0
+        for v in (0, 1, 2, 3):
+            # fixed step order, same row tiling (the split kernel with one
+            # row tile per wave has its own product order)
+            plan.set_variant(4 | (v & 1), form=form, n_slices=5, n_rhs=16)
+            base = plan.eval(fd, form=form, n_slices=5)
+            plan.set_variant(v, form=form, n_slices=5, n_rhs=16)
+            for _ in range(12):
+                got = plan.eval(fd, form=form, n_slices=5)
+                d = ((got - base).abs().amax(dim=1) / base.abs().amax(dim=1)).max()
+                worst = max(worst, float(d))
+        print(f"form={form}: worst rel Linf vs fixed order over 48 evaluations {worst:.2e}",
+              flush=True)
+        assert worst <= 1e-13, (form, worst)
+        plan.set_variant(None, form=form, n_slices=5, n_rhs=16)
+
+
 @pytest.mark.parametrize("form,tol", [("f64", TOL64), ("split", TOLSPLIT)])
 def test_c2_style_adaptive_canonical(cuda, form, tol):
     """A small instance of the adaptive 3D workload: q=5, refined while the

@@ -429,17 +429,27 @@ __device__ inline int64_t out_index(const TileParams& p, int tb, int row) {
 // sorted block list, so a tile starts at the block its XCD's workgroups are
 // currently on: concurrently running workgroups then share one table block
 // in that XCD's L2 instead of spreading over all of them.  The phase word is
-// only a hint (plain racy stores); correctness does not depend on it.
+// only a hint (plain racy stores); correctness does not depend on its value,
+// but every wave of a workgroup must walk the steps in the same order (they
+// stage the density ring together): other workgroups rewrite the word while
+// this one starts, so one thread reads it and the rotation goes to the
+// other waves through LDS.  Called by every thread of the workgroup, in
+// workgroup-uniform control flow (it holds a barrier).
 __device__ inline int phase_start(const TileParams& p, int st0, int st1, int xcd) {
   const int nst = st1 - st0;
   if (nst <= 1 || !p.use_phase) return 0;
-  const int want = __builtin_nontemporal_load(p.phase + xcd);
-  int lo = st0, hi = st1;  // first step with block >= want (blocks sorted)
-  while (lo < hi) {
-    const int mid = (lo + hi) / 2;
-    if (p.step_block[mid] < want) lo = mid + 1; else hi = mid;
+  __shared__ int s_rot;
+  if (threadIdx.x == 0) {
+    const int want = __builtin_nontemporal_load(p.phase + xcd);
+    int lo = st0, hi = st1;  // first step with block >= want (blocks sorted)
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (p.step_block[mid] < want) lo = mid + 1; else hi = mid;
+    }
+    s_rot = lo < st1 ? lo - st0 : 0;
   }
-  return lo < st1 ? lo - st0 : 0;
+  __syncthreads();
+  return s_rot;
 }
 
 // XCD-aware tile order: consecutive workgroup ids go round-robin over the 8
@@ -506,6 +516,8 @@ __device__ __forceinline__ void static_for(F&& f) {
 // LDS: ring [2][S][64 columns][32] bf16, 16-byte vector v of column c stored
 // at v ^ ((c >> 1) & 3) (conflict-free b128 reads without padding); output
 // block [MP/16][4][4][64] doubles; column descriptors [2][64] x 3.
+// static LDS of the BF16 and FP64 tile kernels (the phase word), rounded up
+constexpr size_t kStaticLds = 16;
 inline size_t split_lds_bytes(int S, int MP) {
   return (size_t)2 * S * kCols * 32 * 2 + (size_t)(MP / 16) * 1024 * 8 + 6 * kCols * 8;
 }
@@ -1015,16 +1027,21 @@ struct TileSchedule {
   int64_t n_steps = 0;
 };
 
-// The dynamic-LDS limit of a tile kernel is raised to the full 160 KiB once
-// per device (bit d of `done`), so later launches -- including ones recorded
-// into a HIP graph -- make no attribute calls.
+// The dynamic-LDS limit of a tile kernel is raised to the full 160 KiB (less
+// the kernel's static LDS, the phase word's) once per device (bit d of
+// `done`), so later launches -- including ones recorded into a HIP graph --
+// make no attribute calls.
 static hipError_t allow_full_lds(const void* fn, std::atomic<unsigned>& done) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   const unsigned bit = 1u << (dev & 31);
   if (done.load(std::memory_order_acquire) & bit) return hipSuccess;
-  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  hipFuncAttributes fa{};
+  e = hipFuncGetAttributes(&fa, fn);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          160 * 1024 - (int)fa.sharedSizeBytes);
   if (e == hipSuccess) done.fetch_or(bit, std::memory_order_acq_rel);
   return e;
 }
@@ -1711,7 +1728,7 @@ static int plan_check(const vl1_plan* pl, int form, int n_rhs, const vl1_split_o
     if (pl->fused_split && pl->KP > 256)
       return fail(VL1_E_UNSUPPORTED,
                   "vl1_plan: fused Helmholtz split blocks beyond 256 columns take the FP64 form");
-    if (split_lds_bytes(S, pl->MP) > 160 * 1024)
+    if (split_lds_bytes(S, pl->MP) + kStaticLds > 160 * 1024)
       return fail(VL1_E_UNSUPPORTED, "vl1_plan: too many slices for the box size (LDS)");
   } else if (form == VL1_FORM_SPLIT_I8) {
     S = opts ? opts->n_slices : 6;

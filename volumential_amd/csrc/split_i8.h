@@ -27,6 +27,11 @@ typedef __attribute__((ext_vector_type(16))) int i32x16;
 
 constexpr int kI8MaxSlices = 6;
 constexpr int kI8Ring = 8;  // density ring slots (chunks in flight)
+#ifndef VL1_I8_PRODUCER_DEPTH
+#define VL1_I8_PRODUCER_DEPTH 2
+#endif
+// chunks whose global loads the producer wave keeps in flight (registers)
+constexpr int kI8ProducerDepth = VL1_I8_PRODUCER_DEPTH;
 
 // 1 / (127^2 254^d)
 __host__ __device__ constexpr double i8_weight(int d) {
@@ -444,13 +449,14 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
     if (lane < NS) sBlk[lane] = 0;
     int sbC[2], sbN[2], blkC = 0, blkN = 0;
     load_desc(0, sbN, blkN);
-    // two chunks' loads in flight: chunk c lives in buffer (c - D) % 2 and
-    // is loaded two iterations before its LDS stores; the loads are
-    // unconditional (a column without a source reads offset 0 and stores
-    // zeros), so no branch separates a load from its wait
-    i32x4 buf[2][2][S];
-    int buf_blk[2] = {0, 0};
-    bool live[2][2] = {{false, false}, {false, false}};
+    // NPB chunks' loads in flight: chunk c lives in register buffer
+    // (c - D) % NPB and is loaded NPB iterations before its LDS stores; the
+    // loads are unconditional (a column without a source reads offset 0 and
+    // stores zeros), so no branch separates a load from its wait
+    constexpr int NPB = kI8ProducerDepth;
+    i32x4 buf[NPB][2][S];
+    int buf_blk[NPB];
+    bool live[NPB][2];
     auto issue = [&](int c, auto pc) {
       constexpr int P = decltype(pc)::value;
       const int k = c / nkc, kc = c % nkc;
@@ -490,31 +496,31 @@ __global__ void __launch_bounds__(512) split_i8_kernel(const TileParams p) {
                                     col[m] * 16) = live[P][m] ? buf[P][m][j] : i32x4{0, 0, 0, 0};
     };
     using Z = std::integral_constant<int, 0>;
-    using O = std::integral_constant<int, 1>;
     const int pre = total < D ? total : D;
     for (int c = 0; c < pre; ++c) {
       issue(c, Z{});
       commit(c, Z{});
     }
-    if (D < total) issue(D, Z{});  // in flight across the first barrier
-    if (D + 1 < total) issue(D + 1, O{});
+    static_for<0, NPB>([&](auto qc) {  // in flight across the first barrier
+      if (D + decltype(qc)::value < total) issue(D + decltype(qc)::value, qc);
+    });
     __syncthreads();
     // iteration it stores chunk it + D (slot (it + D) % NS was last read in
-    // iteration it - 1) and issues the loads of chunk it + D + 2
+    // iteration it - 1) and issues the loads of chunk it + D + NPB
     auto step = [&](int it, auto pc) {
       const int c = it + D;
       if (c < total) {
         commit(c, pc);
-        if (c + 2 < total) issue(c + 2, pc);
+        if (c + NPB < total) issue(c + NPB, pc);
       }
       __syncthreads();
     };
     int it = 0;
-    for (; it + 1 < total; it += 2) {
-      step(it, Z{});
-      step(it + 1, O{});
-    }
-    if (it < total) step(it, Z{});
+    for (; it + NPB - 1 < total; it += NPB)
+      static_for<0, NPB>([&](auto qc) { step(it + decltype(qc)::value, qc); });
+    static_for<0, NPB - 1>([&](auto qc) {
+      if (it + decltype(qc)::value < total) step(it + decltype(qc)::value, qc);
+    });
     sDsp[lane] = dsum;
     __syncthreads();  // end of the step loop
   } else {

@@ -915,7 +915,10 @@ def run_hsplit(args):
     # point pair for the Horner series and log, plus 8 per complex MAC
     pt_pairs = pairs * n * n
     p2p_flops = pt_pairs * (8.0 * corr.nmax + 24.0 + 8.0 * R)
-    best = "f64"
+    # headline: the fastest fused form within 1e-10 of the fused FP64 form
+    key = _headline_split(forms)
+    ok = forms[key]["rel_l2_max"] <= SPLIT_TOL and forms[key]["rel_linf_max"] <= SPLIT_TOL
+    best = key if ok and forms[key]["ms"] < forms["f64"]["ms"] else "f64"
     extra = {"two_launch_ms": two_ms, "p2p_kernel_ms": p2p_ms,
              "p2p_declared_fp64_tflops": p2p_flops / (p2p_ms * 1e-3) / 1e12,
              "p2p_fp64_frac": p2p_flops / (p2p_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
@@ -961,20 +964,26 @@ def run_hsplit(args):
             "value": nb * n / dt / 1e9, "unit": "Gpts*RHS/s", "cores": cores, "kind": "port",
             "sample": f"{nb} target boxes x 1 RHS: C/OpenMP Laplace-table part (Re, Im) + "
                       f"numpy remainder sum ({dt:.1f} s)", **host_cpu()}
+    note = ("a complex block runs as the real 2n x 2n block: 8 n^2 flops per RHS, "
+            "the complex-MAC count")
+    if best == "f64":
+        dtype, label = "f64 (complex, fused)", "fused split plan f64"
+        roof = {"bound": "mfma", "achieved": flops / (forms[best]["kernel_ms"] * 1e-3) / 1e12,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": flops / (forms[best]["kernel_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                "traffic": None, "kernel": "dense_f64_kernel",
+                "kernel_ms": forms[best]["kernel_ms"], "flops_per_launch": flops}
+    else:
+        dtype, label, roof = _split_line_parts(forms, best, flops, args.slices)
+        label = "fused split plan " + label
+    roof["note"] = note
     return _line(
-        forms[best]["value"], forms[best]["ms"], args.steps, args.warmup, "f64 (complex, fused)",
+        forms[best]["value"], forms[best]["ms"], args.steps, args.warmup, dtype,
         "synthetic (16 seeded random-phase Gaussian beams, k=40; GPU-built Laplace 2D table)",
         {"workload": "2D Helmholtz k=40 split list 1, uniform level 7, q=6, 16 complex RHS",
          "boxes": tr.nboxes, "target_points": n_pts, "rhs": R, "list1_pairs": pairs,
-         "form": "fused split plan f64", "parallelism": "single GPU"},
-        {"bound": "mfma", "achieved": flops / (forms[best]["kernel_ms"] * 1e-3) / 1e12,
-         "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-         "frac": flops / (forms[best]["kernel_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
-         "traffic": None, "kernel": "dense_f64_kernel", "kernel_ms": forms[best]["kernel_ms"],
-         "flops_per_launch": flops,
-         "note": "a complex block runs as the real 2n x 2n block: 8 n^2 flops per RHS, "
-                 "the complex-MAC count"},
-        forms, extra)
+         "form": label, "parallelism": "single GPU"},
+        roof, forms, extra)
 
 
 def run_c5(args):

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 6
+#define VL1_ABI_VERSION 7
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -72,8 +72,14 @@ typedef enum vl1_kernel_kind {
 typedef enum vl1_scaling_mode {
   VL1_SCALING_CANONICAL = 0,  /* table level 0, scaling (h/H)^power [+ 2D log term] */
   VL1_SCALING_FIXED = 1,      /* table level 0, scaling 1 */
-  VL1_SCALING_PER_LEVEL = 2   /* table level from source level, scaling 1 */
+  VL1_SCALING_PER_LEVEL = 2,  /* table level from source level, scaling 1 */
+  /* table level 0, scaling level_scaling[source level] and displacement
+   * level_disp[source level] * mode_nmlz[0, s]: the caller's
+   * kernel_scaling_code / kernel_displacement_code (list1.py:389-404),
+   * evaluated per source-box level on the host */
+  VL1_SCALING_CUSTOM = 3
 } vl1_scaling_mode;
+#define VL1_MAX_LEVELS 32
 
 /* Last error message of the calling thread ("" if none). */
 const char* vl1_last_error(void);
@@ -179,6 +185,11 @@ typedef struct vl1_near_field_args {
   const int32_t* source_mode_ids;          /* per source particle */
   const int32_t* target_point_ids;         /* per target particle */
   int32_t* error_flags;                    /* optional [1], OR-ed: 1 = invalid case id */
+  /* VL1_SCALING_CUSTOM: device arrays [VL1_MAX_LEVELS] of finite values,
+   * indexed by source-box level (levels past the end take the last entry);
+   * log_displacement = 0 declares level_disp all zero (no displacement pass) */
+  const double* level_scaling;
+  const double* level_disp;
 } vl1_near_field_args;
 
 /* Overwrites result[target] for every target of the listed boxes. */
@@ -218,6 +229,10 @@ typedef struct vl1_plan_desc {
   int log_displacement;
   double case_encoding_bias;
   int boxes_per_tile;          /* 0 = choose */
+  /* VL1_SCALING_CUSTOM: host arrays [VL1_MAX_LEVELS] of finite values,
+   * indexed by source-box level */
+  const double* level_scaling;
+  const double* level_disp;
 } vl1_plan_desc;
 
 typedef struct vl1_split_opts {

@@ -60,8 +60,15 @@ def near_field(
     mode_coefs,
     result=None,
     bias=1e-10,
+    level_scaling=None,
+    level_disp=None,
 ):
-    """``mode``: "canonical", "fixed" or "per_level" (list1.py:361-437)."""
+    """``mode``: "canonical", "fixed", "per_level" or "custom"
+    (list1.py:361-437).  "custom" is the caller's code strings evaluated per
+    source level: table level 0, scaling ``level_scaling[level]``,
+    displacement ``level_disp[level] * mode_nmlz[0, sid]`` (``sid`` the
+    source's position in its box, as the kernel's loop index,
+    list1.py:1094-1112)."""
     coefs = np.atleast_2d(np.asarray(mode_coefs))
     R, npts = coefs.shape
     table = np.atleast_2d(np.asarray(table_data))
@@ -89,6 +96,8 @@ def near_field(
                 lev = 0
             elif mode == "fixed":
                 scal, lev = 1.0, 0
+            elif mode == "custom":
+                scal, lev = float(level_scaling[lvl]), 0
             else:
                 scal = 1.0
                 rel = lvl - table_starting_level
@@ -101,6 +110,8 @@ def near_field(
                 if mode == "canonical" and log_displacement:
                     disp = -0.5 / np.pi * scal * np.log(sext / table_root_extent) * nmlz[lev, :ns]
                     integ = integ + disp[:, None]
+                elif mode == "custom":
+                    integ = integ + (float(level_disp[lvl]) * nmlz[0, :ns])[:, None]
                 acc += coefs[:, sbeg : sbeg + ns] @ np.where(has, integ, 0.0)
                 continue
             if cid not in rows_cache:
@@ -112,6 +123,8 @@ def near_field(
             if mode == "canonical" and log_displacement:
                 disp = -0.5 / np.pi * scal * np.log(sext / table_root_extent) * nmlz[lev, :ns]
                 integ = integ + disp[:, None]
+            elif mode == "custom":
+                integ = integ + (float(level_disp[lvl]) * nmlz[0, :ns])[:, None]
             acc += coefs[:, sbeg : sbeg + ns] @ integ
         out[:, tbeg : tbeg + nt] = acc
     return out
@@ -182,7 +195,7 @@ def tree_groups(tree):
 
 def near_field_from_tree(tree, table_data, mode_nmlz, meta, mode_coefs, *, mode, scaling_power=2,
                          log_displacement=False, table_root_extent, table_starting_level=0,
-                         own_csr=True):
+                         own_csr=True, level_scaling=None, level_disp=None):
     """The kernel semantics on `tree`.  With `own_csr` (default) the list-1
     CSR of the tree's target boxes is recomputed by ``list1_csr_bruteforce``
     instead of taken from the tree builder."""
@@ -215,4 +228,6 @@ def near_field_from_tree(tree, table_data, mode_nmlz, meta, mode_coefs, *, mode,
         table_root_extent=table_root_extent,
         table_starting_level=table_starting_level,
         mode_coefs=mode_coefs,
+        level_scaling=level_scaling,
+        level_disp=level_disp,
     )

@@ -39,6 +39,8 @@ SPLIT_DERIV_SOURCE = 2
 SCALING_CANONICAL = 0
 SCALING_FIXED = 1
 SCALING_PER_LEVEL = 2
+SCALING_CUSTOM = 3
+MAX_LEVELS = 32
 
 FORM_F64 = 0
 FORM_SPLIT_BF16 = 1
@@ -148,6 +150,8 @@ class NearFieldArgs(C.Structure):
         ("source_mode_ids", vp),
         ("target_point_ids", vp),
         ("error_flags", vp),
+        ("level_scaling", vp),
+        ("level_disp", vp),
     ]
 
 
@@ -173,6 +177,8 @@ class PlanDesc(C.Structure):
         ("log_displacement", C.c_int),
         ("case_encoding_bias", f64),
         ("boxes_per_tile", C.c_int),
+        ("level_scaling", vp),
+        ("level_disp", vp),
     ]
 
 
@@ -286,7 +292,7 @@ def lib():
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 6:
+    if L.vl1_abi_version() != 7:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

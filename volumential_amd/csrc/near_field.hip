@@ -32,6 +32,11 @@
 namespace vl1 {
 
 struct NFParams {
+  // custom scaling and displacement factor per source level [VL1_MAX_LEVELS]
+  // (device); first in the struct: further down, the ROCm 7.2 compiler
+  // fails on the VALU kernel's load of them ("illegal instruction")
+  const double* lsc;
+  const double* ldisp;
   const double* centers;
   int nboxes;
   const uint8_t* levels;
@@ -140,6 +145,9 @@ __global__ void __launch_bounds__(kNFBlock) near_field_f64_kernel(const NFParams
           }
           scal = num / den;
           if (p.log_disp) dscale = -0.5 / M_PI * scal * log(sext / p.table_root_extent);
+        } else if (p.scaling_mode == VL1_SCALING_CUSTOM) {
+          scal = p.lsc[min(lvl, VL1_MAX_LEVELS - 1)];
+          dscale = p.ldisp[min(lvl, VL1_MAX_LEVELS - 1)];
         } else if (p.scaling_mode == VL1_SCALING_PER_LEVEL) {
           const int rel = lvl - p.table_starting_level;
           tlev = lvl < p.table_starting_level ? 0 : (rel < p.n_tables ? rel : p.n_tables - 1);
@@ -295,6 +303,9 @@ __device__ inline NFPair nf_pair(const NFParams& p, int li, const double (&ct)[3
     }
     d.scal = num / den;
     if (p.log_disp) d.dscale = -0.5 / M_PI * d.scal * log(sext / p.table_root_extent);
+  } else if (p.scaling_mode == VL1_SCALING_CUSTOM) {
+    d.scal = p.lsc[min(lvl, VL1_MAX_LEVELS - 1)];
+    d.dscale = p.ldisp[min(lvl, VL1_MAX_LEVELS - 1)];
   } else if (p.scaling_mode == VL1_SCALING_PER_LEVEL) {
     const int rel = lvl - p.table_starting_level;
     d.tlev = lvl < p.table_starting_level ? 0 : (rel < p.n_tables ? rel : p.n_tables - 1);
@@ -554,8 +565,10 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
     return fail(VL1_E_UNSUPPORTED, "vl1_near_field_f64: the 2D log displacement applies to real tables");
   if (a->n_tgt_boxes < 0 || a->n_rhs < 1 || a->aligned_nboxes < 1)
     return fail(VL1_E_ARG, "vl1_near_field_f64: bad sizes");
-  if (a->scaling_mode < 0 || a->scaling_mode > 2)
+  if (a->scaling_mode < 0 || a->scaling_mode > 3)
     return fail(VL1_E_ARG, "vl1_near_field_f64: bad scaling mode");
+  if (a->scaling_mode == VL1_SCALING_CUSTOM && (!a->level_scaling || !a->level_disp))
+    return fail(VL1_E_ARG, "vl1_near_field_f64: custom scaling needs level_scaling and level_disp");
   if (a->scaling_mode == VL1_SCALING_CANONICAL && (a->scaling_power < 0 || a->scaling_power > 3))
     return fail(VL1_E_ARG, "vl1_near_field_f64: bad scaling power");
   if (a->n_tgt_boxes == 0) return VL1_OK;
@@ -594,7 +607,16 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
   prm.n = pl->n;
   prm.scaling_mode = a->scaling_mode;
   prm.scaling_power = a->scaling_power;
+  prm.lsc = a->scaling_mode == VL1_SCALING_CUSTOM ? a->level_scaling : nullptr;
+  prm.ldisp = a->scaling_mode == VL1_SCALING_CUSTOM ? a->level_disp : nullptr;
   prm.log_disp = a->log_displacement ? 1 : 0;
+  if (a->scaling_mode == VL1_SCALING_CUSTOM) {
+    // the displacement term is on unless the caller declares it zero
+    prm.log_disp = a->log_displacement ? 1 : 0;
+    if (prm.log_disp && pl->is_complex)
+      return fail(VL1_E_UNSUPPORTED,
+                  "vl1_near_field_f64: a custom displacement applies to real tables");
+  }
   prm.root_extent = a->root_extent;
   prm.table_root_extent = a->table_root_extent;
   prm.table_starting_level = a->table_starting_level;

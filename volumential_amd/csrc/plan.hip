@@ -1350,8 +1350,18 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
   if (!d->box_centers || !d->box_levels || !d->box_starts || !d->box_counts || !d->target_boxes ||
       !d->nbr_starts || !d->nbr_lists || !d->local_ids)
     return fail(VL1_E_ARG, "vl1_plan_create: null host array");
-  if (d->scaling_mode < 0 || d->scaling_mode > 2)
+  if (d->scaling_mode < 0 || d->scaling_mode > 3)
     return fail(VL1_E_ARG, "vl1_plan_create: bad scaling mode");
+  bool custom_disp = false;
+  if (d->scaling_mode == VL1_SCALING_CUSTOM) {
+    if (!d->level_scaling || !d->level_disp)
+      return fail(VL1_E_ARG, "vl1_plan_create: custom scaling needs level_scaling and level_disp");
+    for (int l = 0; l < VL1_MAX_LEVELS; ++l) {
+      if (!std::isfinite(d->level_scaling[l]) || !std::isfinite(d->level_disp[l]))
+        return fail(VL1_E_ARG, "vl1_plan_create: custom scaling values must be finite");
+      custom_disp = custom_disp || d->level_disp[l] != 0.0;
+    }
+  }
   hipStream_t s = (hipStream_t)stream;
   const int n = pay->n, dim = pay->dim;
   // Helmholtz split: complex blocks over a real Laplace payload, the
@@ -1390,7 +1400,8 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
   pl->nboxes = d->n_boxes;
   pl->n_tgt = d->n_tgt_boxes;
   pl->n_points = d->n_points;
-  const bool want_log = d->scaling_mode == VL1_SCALING_CANONICAL && d->log_displacement;
+  const bool want_log =
+      (d->scaling_mode == VL1_SCALING_CANONICAL && d->log_displacement) || custom_disp;
   pl->log_disp = (want_log && !split) ? 1 : 0;  // split blocks carry the term themselves
   if (pl->log_disp && pay->has_missing) {
     // the blocks add the log term for every (s, t); the reference skips it
@@ -1529,6 +1540,9 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
         }
         alpha = num / den;
         if (pl->log_disp) disp = -0.5 / M_PI * alpha * log(sext / H);
+      } else if (d->scaling_mode == VL1_SCALING_CUSTOM) {
+        alpha = d->level_scaling[std::min(lvl, VL1_MAX_LEVELS - 1)];
+        disp = d->level_disp[std::min(lvl, VL1_MAX_LEVELS - 1)];
       } else if (d->scaling_mode == VL1_SCALING_PER_LEVEL) {
         const int rel = lvl - d->table_starting_level;
         lev = lvl < d->table_starting_level ? 0 : (rel < pay->n_tables ? rel : pay->n_tables - 1);

@@ -6,9 +6,11 @@ Same constructor and call convention as volumential's
 **list1_extra_kwargs)`` and ``__call__(queue, result=..., box_centers=...,
 ...)`` returning ``(result, event)``.  Arrays may be torch tensors (device or
 host) or numpy arrays; the evaluation always runs on the GPU through
-``vl1_near_field_f64``.  User scaling/displacement *code strings* are not
-supported (this library has no runtime code generation); the inferred,
-fixed-single-table and per-level-table policies are.
+``vl1_near_field_f64``.  All four scaling policies of the reference are
+supported: inferred, fixed single table, per-level tables, and the caller's
+``kernel_scaling_code`` / ``kernel_displacement_code`` strings (this library
+has no run-time code generation: the strings are evaluated per source-box
+level on the host, see ``scaling_code``).
 """
 
 from __future__ import annotations
@@ -22,6 +24,7 @@ import torch
 from . import _capi, gallery, orbit
 from .kernels import inferred_scaling, supports_inferred_scaling
 from .payload import DenseDevicePayload, DevicePayload, PayloadCache, fingerprint
+from .scaling_code import level_arrays
 
 CASE_ENCODING_BIAS_DEFAULT = 1.0e-10
 
@@ -127,18 +130,13 @@ class NearFieldFromCSR:
         self.extra_kwargs = dict(kwargs)
         self.extra_kwargs.setdefault("case_encoding_bias", CASE_ENCODING_BIAS_DEFAULT)
         self.extra_kwargs["case_encoding_bias"] = float(self.extra_kwargs["case_encoding_bias"])
-        if "kernel_scaling_code" in kwargs or "kernel_displacement_code" in kwargs:
-            raise NotImplementedError(
-                "custom kernel_scaling_code/kernel_displacement_code strings are not "
-                "supported; use per-level tables or inferred scaling"
-            )
         if "infer_kernel_scaling" not in self.extra_kwargs:
             self.extra_kwargs["infer_kernel_scaling"] = (
                 self.n_tables == 1 and supports_inferred_scaling(integral_kernel)
             )
-        if self.extra_kwargs["infer_kernel_scaling"] and not supports_inferred_scaling(
-            integral_kernel
-        ):
+        if (self.extra_kwargs["infer_kernel_scaling"]
+                and not supports_inferred_scaling(integral_kernel)
+                and "kernel_scaling_code" not in self.extra_kwargs):
             raise RuntimeError(
                 "infer_kernel_scaling is unsupported for kernel "
                 f"{integral_kernel!r}. For Helmholtz, K_k(h r) = h^{{-1}} K_{{k h}}(r): "
@@ -146,6 +144,17 @@ class NearFieldFromCSR:
                 "size, which a single fixed-k table cannot represent. Use multi-level "
                 "tables."
             )
+        # user rules replace the inferred ones; both strings are needed
+        # (list1.py:198-205)
+        if "kernel_scaling_code" in kwargs or "kernel_displacement_code" in kwargs:
+            self.extra_kwargs["infer_kernel_scaling"] = False
+            if not ("kernel_scaling_code" in kwargs and "kernel_displacement_code" in kwargs):
+                raise ValueError("kernel_scaling_code and kernel_displacement_code must be "
+                                 "given together")
+            # parse now: a malformed string fails at construction
+            level_arrays(kwargs["kernel_scaling_code"], kwargs["kernel_displacement_code"],
+                         2.0, 2.0)
+        self._custom_levels = {}
         if self.extra_kwargs.get("case_encoding_warn_tol") is not None:
             self.extra_kwargs["case_encoding_warn_tol"] = float(
                 self.extra_kwargs["case_encoding_warn_tol"])
@@ -164,6 +173,11 @@ class NearFieldFromCSR:
                 kname, "canonical_single_table", True, True, 0, p, logd,
                 "Uses table level 0 for all source-box levels with inferred exact scaling.",
             )
+        if "kernel_scaling_code" in self.extra_kwargs:
+            return KernelScalingPolicy(
+                kname, "custom_single_table", False, True, 0, 0, False,
+                "Uses user-provided scaling and displacement code with table level 0.",
+            )
         if self.n_tables == 1:
             return KernelScalingPolicy(
                 kname, "fixed_single_table", False, False, 0, 0, False,
@@ -180,7 +194,20 @@ class NearFieldFromCSR:
             "canonical_single_table": _capi.SCALING_CANONICAL,
             "fixed_single_table": _capi.SCALING_FIXED,
             "per_level_tables": _capi.SCALING_PER_LEVEL,
+            "custom_single_table": _capi.SCALING_CUSTOM,
         }[policy.mode]
+
+    def custom_level_arrays(self, root_extent, table_root_extent):
+        """``(level_scaling, level_disp)`` of the caller's code strings for
+        source boxes of each level (custom policy), or ``(None, None)``."""
+        if "kernel_scaling_code" not in self.extra_kwargs:
+            return None, None
+        key = (float(root_extent), float(table_root_extent))
+        if key not in self._custom_levels:
+            self._custom_levels[key] = level_arrays(
+                self.extra_kwargs["kernel_scaling_code"],
+                self.extra_kwargs["kernel_displacement_code"], *key)
+        return self._custom_levels[key]
 
     # ------------------------------------------------------------ payload
     def _dense_payload(self, kw):
@@ -354,6 +381,16 @@ class NearFieldFromCSR:
             target_point_ids=arrs["target_point_ids"].data_ptr(),
             error_flags=err.data_ptr(),
         )
+        lsc, ldisp = self.custom_level_arrays(kw["root_extent"], kw["table_root_extent"])
+        if lsc is not None:
+            key = (float(kw["root_extent"]), float(kw["table_root_extent"]), "device")
+            if key not in self._custom_levels:
+                self._custom_levels[key] = (torch.as_tensor(lsc, device="cuda"),
+                                            torch.as_tensor(ldisp, device="cuda"))
+            d_lsc, d_ldisp = self._custom_levels[key]
+            args.level_scaling = d_lsc.data_ptr()
+            args.level_disp = d_ldisp.data_ptr()
+            args.log_displacement = int(bool(np.any(ldisp != 0.0)))
         L = _capi.lib()
         _capi.check(
             L.vl1_near_field_f64(payload.handle, args, _capi.stream_handle()),

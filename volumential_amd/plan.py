@@ -44,14 +44,17 @@ class List1Plan:
     def __init__(self, tree, payload, *, scaling_mode, scaling_power=2, log_displacement=False,
                  table_root_extent, table_starting_level=0, case_encoding_bias=1e-10,
                  target_boxes=None, starts=None, lists=None, stream=None,
-                 helmholtz_split=None):
+                 helmholtz_split=None, level_scaling=None, level_disp=None):
         """``helmholtz_split``: ``{"k": k, "series_nmax": N}`` folds the
         Helmholtz split correction into the blocks of a real Laplace payload
         (``vl1_plan_create_helmholtz_split``); the plan then evaluates complex
         densities to the split's full list-1 value.  Optional keys
         ``deriv_kind`` (``_capi.SPLIT_DERIV_*``) and ``deriv_axis`` describe
         an axis derivative wrapper of the output kernel (the payload then
-        holds the wrapped Laplace table)."""
+        holds the wrapped Laplace table).  ``level_scaling`` /
+        ``level_disp`` (``_capi.MAX_LEVELS`` values each): the per-level
+        scaling and displacement factor of the custom policy
+        (``scaling_mode=_capi.SCALING_CUSTOM``)."""
         L = _capi.lib()
         tb = np.ascontiguousarray(tree.target_boxes if target_boxes is None else target_boxes,
                                   dtype=np.int32)
@@ -69,6 +72,14 @@ class List1Plan:
             ls=ls,
             ids=np.ascontiguousarray(tree.local_ids, np.int32),
         )
+        if int(scaling_mode) == _capi.SCALING_CUSTOM:
+            if level_scaling is None or level_disp is None:
+                raise ValueError("the custom scaling mode needs level_scaling and level_disp")
+            for name, arr in (("lsc", level_scaling), ("ldisp", level_disp)):
+                a = np.ascontiguousarray(arr, np.float64)
+                if a.shape != (_capi.MAX_LEVELS,):
+                    raise ValueError(f"level arrays must hold {_capi.MAX_LEVELS} values")
+                self._keep[name] = a
         k = self._keep
         desc = _capi.PlanDesc(
             dim=int(tree.dim),
@@ -91,6 +102,8 @@ class List1Plan:
             log_displacement=int(bool(log_displacement)),
             case_encoding_bias=float(case_encoding_bias),
             boxes_per_tile=0,
+            level_scaling=k["lsc"].ctypes.data if "lsc" in k else None,
+            level_disp=k["ldisp"].ctypes.data if "ldisp" in k else None,
         )
         handle = _capi.vp()
         if helmholtz_split is None:

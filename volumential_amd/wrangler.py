@@ -386,6 +386,7 @@ class NearFieldWrangler:
             np.frombuffer(key[1], dtype=np.int32), np.frombuffer(key[2], dtype=np.int32),
             np.frombuffer(key[3], dtype=np.int32), gallery.ENCODING_SHIFT,
             nf.extra_kwargs.get("case_encoding_warn_tol"))
+        lsc, ldisp = nf.custom_level_arrays(self.tree.root_extent, t0.source_box_extent)
         split_desc = None
         if fused_split:
             corr = self._correction(out_kernel)
@@ -403,6 +404,7 @@ class NearFieldWrangler:
             starts=np.frombuffer(key[2], dtype=np.int32),
             lists=np.frombuffer(key[3], dtype=np.int32),
             helmholtz_split=split_desc,
+            level_scaling=lsc, level_disp=ldisp,
         )
         self._plans[key] = plan
         while len(self._plans) > self.max_cached_plans:

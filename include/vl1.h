@@ -65,7 +65,12 @@ typedef enum vl1_kernel_kind {
    * dG/dd_a = G'(|d|) d_a/|d|, G' = 2D -(ik/4) H1^(1)(k|d|), 3D
    * e^{ik|d|}(ik|d| - 1)/(4 pi |d|^2); source = minus */
   VL1_KERNEL_HELMHOLTZ_TARGET_DERIV = 5,
-  VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV = 6
+  VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV = 6,
+  /* radial terms of the Helmholtz split beyond order 1 (real, table build
+   * only; expansion_wrangler_fpnd.py:144-210): |d|^p, and |d|^p log|d|
+   * (p > 0), with p = vl1_duffy_args.power */
+  VL1_KERNEL_POWER = 7,
+  VL1_KERNEL_POWER_LOG = 8
 } vl1_kernel_kind;
 
 /* list1.py:361-437 (get_kernel_scaling_policy) */
@@ -368,19 +373,23 @@ typedef struct vl1_duffy_args {
    * entries it names and adds nothing else. */
   const int64_t* chunk_start;    /* device [n_chunks + 1] */
   int64_t n_chunks;
+  int power;                     /* exponent p of VL1_KERNEL_POWER / _POWER_LOG (0..32) */
 } vl1_duffy_args;
 
 int vl1_duffy_build(const vl1_duffy_args* a, void* stream);
 
 /* ------------------------------------------------------------------ */
-/* Helmholtz split correction of list 1 (split order 1).               */
+/* Helmholtz split correction of list 1.                               */
 /* Replaces eval_direct_helmholtz_split_correction,                    */
 /* expansion_wrangler_fpnd.py:3491-3836, with the series remainder     */
 /* kernel of 212-312 and the self limit of 5293-5317.                  */
 /* ------------------------------------------------------------------ */
 typedef struct vl1_split_args {
   int dim;                 /* 2 or 3 */
-  int split_order;         /* 1 (the only order implemented) */
+  int split_order;         /* m >= 1: the remainder without the terms an order-m split
+                              tabulates (2D r^2n log r, 3D r^(2j-1), n, j < m;
+                              expansion_wrangler_fpnd.py:284-304); derivative
+                              wrappers take m = 1 only */
   int series_nmax;         /* series terms, 1..96 (expansion_wrangler_fpnd.py:4337-4389) */
   int n_tgt_boxes;
   int n_rhs;
@@ -401,6 +410,15 @@ typedef struct vl1_split_args {
   int64_t ld_result;             /* doubles */
   int deriv_kind;                /* vl1_split_deriv of the output kernel */
   int deriv_axis;                /* its axis */
+  /* Optional (NULL = the remainder series of k, split_order and
+   * series_nmax): a radial kernel given by its series coefficients
+   * (host, interleaved complex), evaluated instead of the remainder and
+   * at r = 0 for self pairs: 2D [c0, then (a_n, b_n) for n = 1..N],
+   * value c0 + sum_n r^2n (b_n - a_n log(r) / (2 pi)); 3D [c_n for
+   * n = 1..N], value sum_n c_n r^(n-1); N = series_nmax.  The 2D split of
+   * order m > 1 uses it for its log(h/H) term (the r^2n part with
+   * per-source log factors folded into the strengths, 3689-3800). */
+  const double* coefficients;
 } vl1_split_args;
 
 /* corr[t] = sum over the list-1 source points s of R(|x_t - y_s|) strength[s],

@@ -106,6 +106,17 @@ def laplace_source_derivative(dim, axis):
     return lambda *d: -g(*d)
 
 
+def radial_power(dim, p):
+    """|d|^p (no scaling constant): the term kernel the Helmholtz split of
+    order > 1 tabulates in 3D (expansion_wrangler_fpnd.py:144-175)."""
+    return lambda *d: sum(c * c for c in d) ** (0.5 * p)
+
+
+def radial_power_log(dim, p):
+    """|d|^p log|d|: the 2D term kernel of the same split (178-210)."""
+    return lambda *d: sum(c * c for c in d) ** (0.5 * p) * 0.5 * np.log(sum(c * c for c in d))
+
+
 def constant(dim):
     return lambda *d: np.ones_like(np.asarray(d[0], dtype=np.float64))
 

@@ -1,7 +1,8 @@
-"""Oracle: the Helmholtz split correction of list 1 (order 1), numpy.
+"""Oracle: the Helmholtz split correction of list 1, numpy.
 
-TEST INFRASTRUCTURE ONLY.  Restates, for ``helmholtz_split_order = 1`` on
-the base quadrature (the reference's default path):
+TEST INFRASTRUCTURE ONLY.  Restates, on the base quadrature (the
+reference's default path; orders above 1 through ``extracted_terms`` and
+``remainder(split_order=m)``):
 
 * ``expansion_wrangler_fpnd.py:212-312`` -- ``_HelmholtzSplitSeriesRemainderKernel``:
   the smooth remainder ``G_k - G_0`` as a truncated series,
@@ -80,8 +81,38 @@ def max_neighbor_distance(tree):
     return float(2.0 * np.sqrt(tree.dim) * tree.root_extent * 0.5 ** int(lv.min()))
 
 
-def remainder(dim, k, nmax, r):
-    """The series remainder kernel at distances ``r`` (any shape), complex."""
+def extracted_terms(dim, k, split_order):
+    """``_helmholtz_split_extra_terms`` (5052-5118): the non-smooth terms a
+    split of order m takes out of the remainder into tables, as
+    ``(kind, power, coefficient)``: 2D ``("power_log", 2n, -s_n / (2 pi))``
+    for n < m; 3D ``("power", 2j - 1, (ik)^(2j) / (4 pi (2j)!))`` for j < m."""
+    k = complex(k)
+    out = []
+    for j in range(1, int(split_order)):
+        if dim == 2:
+            s_n = ((-1) ** j) * (k * k / 4.0) ** j / (math.factorial(j) ** 2)
+            out.append(("power_log", 2 * j, -s_n / (2.0 * np.pi)))
+        else:
+            n = 2 * j
+            out.append(("power", n - 1, (1j * k) ** n / (4.0 * np.pi * math.factorial(n))))
+    return out
+
+
+def term_value(kind, power, r):
+    """``r^p`` or ``r^p log r`` (0 at r = 0)."""
+    r = np.asarray(r, dtype=np.float64)
+    rp = r ** power
+    if kind == "power":
+        return rp
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(r <= 1e-300, 0.0, rp * np.log(np.where(r > 0, r, 1.0)))
+
+
+def remainder(dim, k, nmax, r, split_order=1):
+    """The series remainder kernel at distances ``r`` (any shape), complex.
+    ``split_order`` m > 1 drops the terms ``extracted_terms`` names
+    (expansion_wrangler_fpnd.py:284-304): 2D the log parts of n < m, 3D
+    the even n <= 2(m - 1)."""
     k = complex(k)
     r = np.asarray(r, dtype=np.float64)
     out = np.zeros(r.shape, dtype=np.complex128)
@@ -98,9 +129,13 @@ def remainder(dim, k, nmax, r):
             c_log = -s_n / (2.0 * np.pi)
             c_pow = s_n * ((h_n - (lkh + EULER_GAMMA)) / (2.0 * np.pi) + 0.25j)
             rp = r ** (2 * n)
-            out += c_log * np.where(r <= 1e-300, 0.0, rp * logr) + c_pow * rp
+            if n >= split_order:
+                out += c_log * np.where(r <= 1e-300, 0.0, rp * logr)
+            out += c_pow * rp
         return out
     for n in range(1, nmax + 1):
+        if n % 2 == 0 and n <= 2 * (split_order - 1):
+            continue
         out += (1j * k) ** n / (4.0 * np.pi * math.factorial(n)) * r ** (n - 1)
     return out
 
@@ -163,10 +198,12 @@ def closed_form_difference(dim, k, r):
 
 
 def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=None,
-               deriv_axis=None, deriv_sign=1):
+               deriv_axis=None, deriv_sign=1, split_order=1, kernel=None):
     """P2P of the remainder over list 1: ``out[R, n_points]`` (complex);
     targets of unlisted boxes stay zero.  ``deriv_axis``: the axis of a
-    target (``deriv_sign = +1``) or source (``-1``) derivative wrapper."""
+    target (``deriv_sign = +1``) or source (``-1``) derivative wrapper.
+    ``kernel``: a radial function r -> complex used instead of the
+    remainder (its value at r = 0 for self pairs)."""
     st = np.atleast_2d(np.asarray(strengths, dtype=np.complex128))
     tb_list = tree.target_boxes if target_boxes is None else target_boxes
     starts = tree.neighbor_source_boxes_starts if starts is None else starts
@@ -184,8 +221,10 @@ def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=N
             dv = xt[:, :, None] - ys[:, None, :]  # [dim, nt, ns]
             r = np.sqrt(np.sum(dv ** 2, axis=0))
             same = (xt[:, :, None] == ys[:, None, :]).all(axis=0)
-            if deriv_axis is None:
-                rem = remainder(tree.dim, k, nmax, r)
+            if kernel is not None:
+                rem = np.asarray(kernel(r), dtype=np.complex128)
+            elif deriv_axis is None:
+                rem = remainder(tree.dim, k, nmax, r, split_order)
                 rem = np.where(same, self_limit(tree.dim, k), rem)
             else:
                 rem = deriv_sign * dv[deriv_axis] * remainder_radial_derivative(tree.dim, k, nmax, r)

@@ -168,3 +168,26 @@ def test_segmented_build_matches_per_entry(cuda, dim, q, kernel):
     b = duffy_values(t, ids, 8, 21, segmented=False)
     scale = np.maximum(1.0, np.abs(b))
     assert np.max(np.abs(a - b) / scale) <= 1e-13, np.max(np.abs(a - b) / scale)
+
+
+@pytest.mark.parametrize("dim,q,extent,kind,p", [(3, 3, 1.0, "power", 1), (3, 2, 2.0, "power", 3),
+                                                 (2, 4, 2.0, "power_log", 2),
+                                                 (2, 3, 0.5, "power_log", 4)])
+def test_radial_term_entries_match_oracle(cuda, dim, q, extent, kind, p):
+    """The Helmholtz split's term kernels (|d|^p, |d|^p log|d|): sampled
+    canonical entries vs the oracle's Duffy quadrature at the same rule."""
+    from volumential_amd.kernels import RadialPowerKernel, RadialPowerLogKernel
+
+    print(f"case: radial term table {kind} p={p} {dim}D q={q}", flush=True)
+    reg, rad = 12, 41
+    knl = RadialPowerKernel(dim, p) if kind == "power" else RadialPowerLogKernel(dim, p)
+    t = NearFieldInteractionTable(q, dim, knl, source_box_extent=extent)
+    t.build_table(DuffyBuildConfig(regular_quad_order=reg, radial_quad_order=rad))
+    ids, vals = t.get_reduced_table_data()
+    g = D.TableGeometry(dim, q, H=extent)
+    ref_k = D.radial_power(dim, p) if kind == "power" else D.radial_power_log(dim, p)
+    rng = np.random.default_rng(dim * 5 + p)
+    pick = np.unique(np.concatenate([[0, ids.size - 1], rng.integers(0, ids.size, 20)]))
+    for i in pick:
+        ref = g.entry(int(ids[i]), ref_k, reg, rad)
+        assert abs(vals[i] - ref) / max(1.0, abs(ref)) <= 1e-12, (ids[i], vals[i], ref)

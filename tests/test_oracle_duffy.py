@@ -172,3 +172,51 @@ def test_helmholtz_derivative_kernels_are_derivatives(dim, axis):
         fd = (base(*dp) - base(*dm)) / (2 * h)
         assert abs(gt(*d) - fd) <= 1e-7 * max(1.0, abs(fd)), (d, gt(*d), fd)
         assert gs(*d) == -gt(*d)
+
+
+def _gauss_entry(g, entry_id, kernel, m):
+    """A table entry by tensor Gauss-Legendre with m nodes per axis: exact
+    for a polynomial integrand of degree < 2m per axis, spectrally accurate
+    for a smooth one (target outside the source box)."""
+    n = g.n
+    case_id, s, t = entry_id // (n * n), (entry_id % (n * n)) // n, entry_id % n
+    x_t = g.target_point(t, case_id)
+    u, w = np.polynomial.legendre.leggauss(m)
+    y1, w1 = 0.5 * g.H * (u + 1.0), 0.5 * g.H * w
+    grids = np.meshgrid(*([y1] * g.dim), indexing="ij")
+    wts = np.prod(np.meshgrid(*([w1] * g.dim), indexing="ij"), axis=0)
+    vals = g.mode(s)(*grids) * kernel(*[x_t[i] - grids[i] for i in range(g.dim)])
+    return float(np.sum(vals * wts))
+
+
+@pytest.mark.parametrize("dim,q,p", [(2, 3, 2), (2, 4, 4), (3, 2, 2)])
+def test_radial_even_power_entries_are_polynomial_integrals(dim, q, p):
+    """|d|^p with p even is a polynomial: the Duffy entry (near-singular
+    cases included) equals Gauss-Legendre with enough nodes."""
+    g = D.TableGeometry(dim, q, H=2.0)
+    n = g.n
+    rng = np.random.default_rng(p + 10 * dim)
+    for e in rng.integers(0, len(g.vecs) * n * n, 12):
+        a = g.entry(int(e), D.radial_power(dim, p), 20, 61)
+        b = _gauss_entry(g, int(e), D.radial_power(dim, p), q + p)
+        assert abs(a - b) <= 1e-12 * max(1.0, abs(b)), (e, a, b)
+
+
+@pytest.mark.parametrize("dim,q,kern", [(3, 2, "power1"), (3, 3, "power3"), (2, 3, "powerlog2"),
+                                        (2, 4, "powerlog4")])
+def test_radial_term_entries_far_cases(dim, q, kern):
+    """Odd powers and the log terms are not polynomials: for a target outside
+    the source box (case vector with a component of size 6) the integrand
+    is smooth, and Duffy and 40-node Gauss-Legendre agree."""
+    p = int(kern[-1])
+    knl = D.radial_power(dim, p) if kern.startswith("power") and "log" not in kern else \
+        D.radial_power_log(dim, p)
+    g = D.TableGeometry(dim, q, H=1.0)
+    n = g.n
+    far = [c for c, v in enumerate(g.vecs) if np.max(np.abs(v)) == 6][:3]
+    for c in far:
+        for s, t in ((0, 0), (n - 1, n // 2)):
+            e = c * n * n + s * n + t
+            a = g.entry(e, knl, 20, 61)
+            b = _gauss_entry(g, e, knl, 40)
+            assert abs(a - b) <= 1e-11 * max(1.0, abs(b)), (kern, c, s, t, a, b)

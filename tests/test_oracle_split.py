@@ -173,3 +173,68 @@ def test_series_roundoff_estimate_tracks_cancellation():
         assert err / 100 <= est <= err * 100, (z, err, est)
         assert est > series_roundoff(3, z - 5.0, 1.0)
     assert series_roundoff(2, 40.0, 2 * math.sqrt(2) * 2 / 2**6) < 1e-15
+
+
+@pytest.mark.parametrize("dim,k,m", [(2, 5.0, 2), (2, 3.0 + 1.0j, 3), (3, 4.0, 2), (3, 2.0j, 3),
+                                     (3, 6.0, 4)])
+def test_higher_order_remainder_plus_terms_is_order_one(dim, k, m):
+    """A split of order m moves the terms ``extracted_terms`` names out of the
+    series: remainder_m + sum coeff * term == remainder_1 at every r, and
+    remainder_m equals the closed form G_k - G_0 minus those terms."""
+    r = np.linspace(0.0, 0.6, 31)
+    nmax = S.series_nmax(dim, k, 0.6)
+    terms = S.extracted_terms(dim, k, m)
+    assert len(terms) == m - 1
+    assert all(t[0] == ("power_log" if dim == 2 else "power") for t in terms)
+    assert [t[1] for t in terms] == ([2 * n for n in range(1, m)] if dim == 2 else
+                                     [2 * j - 1 for j in range(1, m)])
+    tab = sum(c * S.term_value(kind, p, r) for kind, p, c in terms)
+    r1 = S.remainder(dim, k, nmax, r)
+    rm = S.remainder(dim, k, nmax, r, split_order=m)
+    assert np.allclose(rm + tab, r1, rtol=1e-13, atol=1e-15)
+    rr = r[1:]
+    cf = S.closed_form_difference(dim, k, rr) - sum(c * S.term_value(kind, p, rr)
+                                                    for kind, p, c in terms)
+    assert np.allclose(rm[1:], cf, rtol=1e-10, atol=1e-12)
+
+
+def test_higher_order_extracted_coefficients():
+    """The extracted terms are the series' own: 2D -s_n/(2 pi) on r^2n log r
+    with s_n = (-1)^n (k^2/4)^n/(n!)^2; 3D (ik)^2j/(4 pi (2j)!) on r^(2j-1)."""
+    k = 3.0
+    (kind, p, c), = S.extracted_terms(2, k, 2)
+    assert (kind, p) == ("power_log", 2) and np.isclose(c, (k * k / 4.0) / (2.0 * np.pi))
+    (kind, p, c), (kind2, p2, c2) = S.extracted_terms(3, k, 3)
+    assert (kind, p, kind2, p2) == ("power", 1, "power", 3)
+    assert np.isclose(c, -(k * k) / (8.0 * np.pi)) and np.isclose(c2, k ** 4 / (96.0 * np.pi))
+
+
+def test_split_terms_match_oracle_restatement():
+    """The product's term list (split.split_terms) equals the oracle's."""
+    from volumential_amd.split import split_terms
+
+    for dim, k, m in ((2, 7.0, 4), (3, 2.5 + 0.5j, 4), (3, 1.0, 1)):
+        a, b = split_terms(dim, k, m), S.extracted_terms(dim, k, m)
+        assert [(x[0], x[1]) for x in a] == [(x[0], x[1]) for x in b]
+        assert np.allclose([x[2] for x in a], [x[2] for x in b], rtol=1e-15)
+
+
+def test_auto_split_order_ladder():
+    """``helmholtz_split_order="auto"`` (expansion_wrangler_fpnd.py:4228-4330):
+    rho = |k| h_max of the coarsest active box; default ladders 0.25 * 2^j
+    (real part of k) and 0.5 * 2^j (imaginary part) over orders 2..12."""
+    from volumential_amd.kernels import HelmholtzKernel
+    from volumential_amd.split import choose_auto_split_order
+
+    tr = vtree.uniform_tree(2, 3, 2)  # h_max = 0.5
+    pick = lambda k, **cfg: choose_auto_split_order(tr, [HelmholtzKernel(2, k)], cfg)
+    assert pick(0.4) == 2          # rho 0.2 <= 0.25
+    assert pick(0.9) == 3          # rho 0.45 in (0.25, 0.5]
+    assert pick(7.0) == 6          # rho 3.5 in (2, 4]
+    assert pick(1e4) == 12         # beyond the ladder: order_max
+    assert pick(1.2j) == 3         # imaginary: rho 0.6 in (0.5, 1]
+    assert pick(0.4 + 1.2j) == 3   # the larger of the two picks
+    assert pick(7.0, rho_thresholds=(0.5, 1.5, 3.0)) == 5
+    assert pick(0.9, orders=(2, 4, 6, 8), rho_thresholds=(0.5, 1.5, 3.0)) == 2
+    with pytest.raises(ValueError):
+        pick(1.0, order_min=5, order_max=3)

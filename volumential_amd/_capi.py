@@ -31,6 +31,8 @@ KERNEL_LAPLACE_TARGET_DERIV = 3
 KERNEL_LAPLACE_SOURCE_DERIV = 4
 KERNEL_HELMHOLTZ_TARGET_DERIV = 5
 KERNEL_HELMHOLTZ_SOURCE_DERIV = 6
+KERNEL_POWER = 7
+KERNEL_POWER_LOG = 8
 
 SPLIT_DERIV_NONE = 0
 SPLIT_DERIV_TARGET = 1
@@ -221,6 +223,7 @@ class DuffyArgs(C.Structure):
         ("deriv_axis", C.c_int),
         ("chunk_start", vp),
         ("n_chunks", i64),
+        ("power", C.c_int),
     ]
 
 
@@ -247,6 +250,7 @@ class SplitArgs(C.Structure):
         ("ld_result", i64),
         ("deriv_kind", C.c_int),
         ("deriv_axis", C.c_int),
+        ("coefficients", vp),
     ]
 
 

@@ -70,7 +70,9 @@ __device__ inline double lagrange_basis(double y, int m, int q, double cm, const
 // Laplace (da = component of d = target - source on the derivative axis):
 // target dG/dd_a = 2D -da/(2 pi r^2), 3D -da/(4 pi r^3); source = minus.
 // Axis derivatives of Helmholtz: G'(r) da / r with G' = 2D -(ik/4) H1^(1)(kr)
-// = (k/4)(Y1(kr) - i J1(kr)), 3D e^{ikr}(ikr - 1)/(4 pi r^2).
+// = (k/4)(Y1(kr) - i J1(kr)), 3D e^{ikr}(ikr - 1)/(4 pi r^2).  Radial terms
+// of the Helmholtz split: r^p and r^p log r, no scaling constant (the
+// exponent p travels in the k slot).
 __host__ __device__ constexpr bool kind_is_complex(int kind) {
   return kind == VL1_KERNEL_HELMHOLTZ || kind == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV ||
          kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV;
@@ -108,6 +110,11 @@ __device__ inline void kernel_value(double r2, double da, double k, double& re, 
     }
   } else if (KIND == VL1_KERNEL_CONSTANT) {
     re = 1.0;
+    im = 0.0;
+  } else if (KIND == VL1_KERNEL_POWER || KIND == VL1_KERNEL_POWER_LOG) {
+    // |d|^p (k carries p for these kinds), times log|d| for POWER_LOG
+    re = pow(r2, 0.5 * k);
+    if (KIND == VL1_KERNEL_POWER_LOG) re *= 0.5 * log(r2);
     im = 0.0;
   } else if (KIND == VL1_KERNEL_LAPLACE) {
     if (DIM == 2) {
@@ -449,8 +456,12 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
     return fail(VL1_E_ARG, "vl1_duffy_build: derivative axis out of range");
   if (a->chunk_start && (a->n_chunks < 1 || a->n_chunks > INT32_MAX))
     return fail(VL1_E_ARG, "vl1_duffy_build: bad chunk count");
+  const bool radial = a->kernel_kind == VL1_KERNEL_POWER || a->kernel_kind == VL1_KERNEL_POWER_LOG;
+  if (radial && (a->power < 0 || a->power > 32 ||
+                 (a->kernel_kind == VL1_KERNEL_POWER_LOG && a->power == 0)))
+    return fail(VL1_E_ARG, "vl1_duffy_build: power must be 0..32 (1..32 with the log)");
   DuffyParams prm{a->dim, a->quad_order, a->kernel_kind, deriv ? a->deriv_axis : -1,
-                  a->helmholtz_k, a->n_entries,
+                  radial ? (double)a->power : a->helmholtz_k, a->n_entries,
                   a->n_nodes, a->nodes_per_sign, a->source_box_extent, a->target_points,
                   a->decomposition, a->mode_axes, a->node_u, a->node_sign, a->node_jac,
                   a->interp_nodes, a->bary_w, a->result, a->chunk_start,
@@ -470,6 +481,8 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
       e = launch_duffy<2, VL1_KERNEL_HELMHOLTZ_TARGET_DERIV>(prm, s);
     else if (kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV)
       e = launch_duffy<2, VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_POWER) e = launch_duffy<2, VL1_KERNEL_POWER>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_LOG) e = launch_duffy<2, VL1_KERNEL_POWER_LOG>(prm, s);
     else return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: kernel kind");
   } else {
     if (kind == VL1_KERNEL_CONSTANT) e = launch_duffy<3, VL1_KERNEL_CONSTANT>(prm, s);
@@ -483,6 +496,8 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
       e = launch_duffy<3, VL1_KERNEL_HELMHOLTZ_TARGET_DERIV>(prm, s);
     else if (kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV)
       e = launch_duffy<3, VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_POWER) e = launch_duffy<3, VL1_KERNEL_POWER>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_LOG) e = launch_duffy<3, VL1_KERNEL_POWER_LOG>(prm, s);
     else return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: kernel kind");
   }
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("duffy launch: ") + hipGetErrorString(e));

@@ -1,4 +1,4 @@
-// Helmholtz split correction of list 1 (split order 1, base quadrature).
+// Helmholtz split correction of list 1 (base quadrature).
 //
 // With the split, list 1 of a Helmholtz (or Yukawa, k = i lambda) potential
 // is the Laplace-table evaluation plus an online point-to-point sum of the
@@ -178,8 +178,8 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   clear_error();
   if (!a) return fail(VL1_E_ARG, "vl1_split_correction: null argument");
   if (a->dim != 2 && a->dim != 3) return fail(VL1_E_ARG, "vl1_split_correction: dim must be 2 or 3");
-  if (a->split_order != 1)
-    return fail(VL1_E_UNSUPPORTED, "vl1_split_correction: only helmholtz_split_order = 1 is implemented");
+  if (a->split_order < 1 || 2 * (a->split_order - 1) > kSplitMaxTerms)
+    return fail(VL1_E_ARG, "vl1_split_correction: split_order must be 1..49");
   if (a->series_nmax < 1 || a->series_nmax > kSplitMaxTerms)
     return fail(VL1_E_ARG, "vl1_split_correction: series_nmax must be 1..96");
   if (a->n_tgt_boxes < 0 || a->n_rhs < 1 || a->n_points < 1 || a->max_box_points < 1)
@@ -195,10 +195,21 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   const bool deriv = a->deriv_kind != VL1_SPLIT_DERIV_NONE;
   if (deriv && (a->deriv_axis < 0 || a->deriv_axis >= a->dim))
     return fail(VL1_E_ARG, "vl1_split_correction: deriv_axis out of range");
+  if (deriv && (a->split_order != 1 || a->coefficients))
+    return fail(VL1_E_UNSUPPORTED,
+                "vl1_split_correction: derivative wrappers take split order 1 and the series of k");
   hipStream_t s = (hipStream_t)stream;
   const std::complex<double> k(a->k_re, a->k_im);
-  std::vector<double> coef = deriv ? split_derivative_coefficients(a->dim, k, a->series_nmax)
-                                   : split_coefficients(a->dim, k, a->series_nmax);
+  std::vector<double> coef;
+  if (a->coefficients) {
+    const int nc = a->dim == 2 ? 2 + 4 * a->series_nmax : 2 * a->series_nmax;
+    coef.assign(a->coefficients, a->coefficients + nc);
+    for (double c : coef)
+      if (!std::isfinite(c)) return fail(VL1_E_ARG, "vl1_split_correction: coefficients must be finite");
+  } else {
+    coef = deriv ? split_derivative_coefficients(a->dim, k, a->series_nmax)
+                 : split_coefficients(a->dim, k, a->series_nmax, a->split_order);
+  }
   if (coef.empty()) coef.assign(2, 0.0);  // 3D derivative with one term: R' = 0
   DevBuf dcoef;
   hipError_t e = upload(dcoef, coef.data(), coef.size() * sizeof(double), s);

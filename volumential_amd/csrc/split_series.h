@@ -1,5 +1,7 @@
-// Series form of the Helmholtz split remainder R = G_k - G_0 (split order 1),
-// shared by the point-to-point correction and the plan's split blocks.
+// Series form of the Helmholtz split remainder R = G_k - G_0, shared by the
+// point-to-point correction and the plan's split blocks.  A split of order
+// m > 1 drops the terms it tabulates (expansion_wrangler_fpnd.py:284-304):
+// 2D the log parts s_n of n < m, 3D the even n <= 2 (m - 1).
 //   2D  R = c0 + P_b(r^2) - log(r) / (2 pi) * P_a(r^2),
 //       P_a(u) = sum_{n=1..N} s_n u^n,  s_n = (-1)^n (k^2/4)^n / (n!)^2,
 //       P_b(u) = sum_n s_n ((H_n - log(k/2) - gamma) / (2 pi) + i/4) u^n,
@@ -99,7 +101,8 @@ __device__ inline void remainder_radial_derivative(double r2, const double* __re
 }
 
 // Series coefficients (interleaved complex), the reference's formulas.
-inline std::vector<double> split_coefficients(int dim, std::complex<double> k, int N) {
+inline std::vector<double> split_coefficients(int dim, std::complex<double> k, int N,
+                                              int order = 1) {
   using cd = std::complex<double>;
   std::vector<double> out;
   if (dim == 2) {
@@ -116,8 +119,9 @@ inline std::vector<double> split_coefficients(int dim, std::complex<double> k, i
       kp *= -k * k / 4.0;  // (-1)^n (k^2/4)^n
       const cd s = zero ? cd(0.0) : kp / (fact * fact);
       const cd b = s * ((harm - L) / (2.0 * M_PI) + cd(0.0, 0.25));
-      out.push_back(s.real());
-      out.push_back(s.imag());
+      const cd sl = n < order ? cd(0.0) : s;  // log part, tabulated below order m
+      out.push_back(sl.real());
+      out.push_back(sl.imag());
       out.push_back(b.real());
       out.push_back(b.imag());
     }
@@ -127,7 +131,8 @@ inline std::vector<double> split_coefficients(int dim, std::complex<double> k, i
     for (int n = 1; n <= N; ++n) {
       fact *= n;
       p *= ik;
-      const cd c = p / (4.0 * M_PI * fact);
+      const bool tabulated = n % 2 == 0 && n <= 2 * (order - 1);
+      const cd c = tabulated ? cd(0.0) : p / (4.0 * M_PI * fact);
       out.push_back(c.real());
       out.push_back(c.imag());
     }

@@ -17,6 +17,7 @@ Kernel conventions (sumpy expression times global scaling constant):
 * Axis derivatives of Helmholtz: target ``G'(|d|) d_a / |d|`` with
   ``G'`` = 2D ``-(ik/4) H1^(1)(k|d|)``, 3D ``e^{ik|d|}(ik|d| - 1) / (4 pi |d|^2)``;
   source = minus that
+* Radial split terms: ``|d|^p`` and ``|d|^p log|d|``
 """
 
 from __future__ import annotations
@@ -61,6 +62,50 @@ class HelmholtzKernel:
 
     def __repr__(self):
         return f"HelmKnl{self.dim}D"
+
+    def get_base_kernel(self):
+        return self
+
+
+@dataclass(frozen=True)
+class RadialPowerKernel:
+    """``|d|^power`` (real, no scaling constant): a term that the Helmholtz
+    split of order > 1 takes out of the online remainder into a table (3D
+    odd powers; ``expansion_wrangler_fpnd.py:144-175``)."""
+
+    dim: int
+    power: int
+    is_complex_valued = False
+    kind = _capi.KERNEL_POWER
+
+    def __post_init__(self):
+        if not 0 <= int(self.power) <= 32:
+            raise ValueError("power must be 0..32")
+
+    def __repr__(self):
+        return f"RadialPowerKernel(dim={self.dim}, power={self.power})"
+
+    def get_base_kernel(self):
+        return self
+
+
+@dataclass(frozen=True)
+class RadialPowerLogKernel:
+    """``|d|^power log|d|`` (real, no scaling constant, power > 0): the 2D
+    terms of the Helmholtz split of order > 1
+    (``expansion_wrangler_fpnd.py:178-210``)."""
+
+    dim: int
+    power: int
+    is_complex_valued = False
+    kind = _capi.KERNEL_POWER_LOG
+
+    def __post_init__(self):
+        if not 1 <= int(self.power) <= 32:
+            raise ValueError("power must be 1..32 for r**power * log(r)")
+
+    def __repr__(self):
+        return f"RadialPowerLogKernel(dim={self.dim}, power={self.power})"
 
     def get_base_kernel(self):
         return self

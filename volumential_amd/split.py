@@ -100,6 +100,81 @@ def self_limit(dim, k):
     return 0.25j - (np.log(0.5 * k) + EULER_GAMMA) / (2.0 * np.pi)
 
 
+def split_terms(dim, k, split_order):
+    """The non-smooth terms a split of order m tabulates, as ``(kind, power,
+    coefficient)`` (``_helmholtz_split_extra_terms``,
+    expansion_wrangler_fpnd.py:5052-5118): 2D ``("power_log", 2n,
+    -s_n / (2 pi))``, ``s_n = (-1)^n (k^2/4)^n / (n!)^2``, and 3D
+    ``("power", 2j - 1, (ik)^(2j) / (4 pi (2j)!))``, for n, j = 1..m-1."""
+    k = complex(k)
+    out = []
+    for j in range(1, int(split_order)):
+        if dim == 2:
+            s_n = ((-1) ** j) * (k * k / 4.0) ** j / (math.factorial(j) ** 2)
+            out.append(("power_log", 2 * j, complex(-s_n / (2.0 * math.pi))))
+        elif dim == 3:
+            n = 2 * j
+            out.append(("power", n - 1, complex((1j * k) ** n / (4.0 * math.pi * math.factorial(n)))))
+        else:
+            raise NotImplementedError("split terms exist only in 2D and 3D")
+    return out
+
+
+def _split_order_ladder(rho, thresholds, orders):
+    """``_select_split_order_from_rho`` (385-393)."""
+    for threshold, order in zip(thresholds, orders):
+        if rho <= float(threshold):
+            return int(order)
+    return int(orders[-1])
+
+
+def choose_auto_split_order(tree, kernels, config=None):
+    """``helmholtz_split_order="auto"``: ``_choose_auto_helmholtz_split_order``
+    (4228-4330).  rho = |Re k| h and |Im k| h for the coarsest active box
+    extent h; the order is the larger pick of two ladders, by default
+    ``0.25 * 2^j`` (real) and ``0.5 * 2^j`` (imaginary) for orders
+    ``order_min .. order_max`` (2..12)."""
+    cfg = dict(config or {})
+    order_min, order_max = int(cfg.get("order_min", 2)), int(cfg.get("order_max", 12))
+    if order_max < order_min:
+        raise ValueError("order_max must be >= order_min")
+    if "rho_thresholds" in cfg or "orders" in cfg:
+        th_re = tuple(cfg.get("rho_thresholds", (0.5, 1.5, 3.0)))
+        th_im = th_re
+        orders = tuple(cfg["orders"]) if "orders" in cfg else tuple(
+            range(order_min, order_min + len(th_re) + 1))
+    else:
+        b_re, b_im = float(cfg.get("rho_base_real", 0.25)), float(cfg.get("rho_base_imag", 0.5))
+        if b_re <= 0.0 or b_im <= 0.0:
+            raise ValueError("rho_base_real and rho_base_imag must be positive")
+        count = max(0, order_max - order_min)
+        th_re = tuple(b_re * 2.0 ** j for j in range(count))
+        th_im = tuple(b_im * 2.0 ** j for j in range(count))
+        orders = tuple(range(order_min, order_max + 1))
+    th_re = tuple(cfg.get("rho_thresholds_real", th_re))
+    th_im = tuple(cfg.get("rho_thresholds_imag", th_im))
+    if "orders" not in cfg:
+        if len(th_re) != len(th_im):
+            raise ValueError("rho_thresholds_real and rho_thresholds_imag must have the same "
+                             "length when orders are not provided")
+        orders = tuple(range(order_min, order_min + len(th_re) + 1))
+    if len(orders) != len(th_re) + 1:
+        raise ValueError("orders must have one more element than thresholds")
+    counts = np.asarray(tree.box_target_counts_nonchild)
+    active = np.flatnonzero(counts > 0)
+    if active.size == 0:
+        return order_min
+    h = float(tree.root_extent) * 0.5 ** int(np.asarray(tree.box_levels)[active].min())
+    kr = max((abs(complex(getattr(kn.get_base_kernel(), "k", 0.0)).real) for kn in kernels),
+             default=0.0)
+    ki = max((abs(complex(getattr(kn.get_base_kernel(), "k", 0.0)).imag) for kn in kernels),
+             default=0.0)
+    pick = max(_split_order_ladder(kr * h, th_re, orders), _split_order_ladder(ki * h, th_im, orders))
+    if "orders" not in cfg:
+        pick = max(order_min, min(order_max, pick))
+    return pick
+
+
 class HelmholtzSplitCorrection:
     """``corr = sum_{list 1} (G_k - G_0)(x_t - y_s) * strength_s`` on the GPU.
 
@@ -109,15 +184,20 @@ class HelmholtzSplitCorrection:
     ``deriv_axis``: an axis target (``_capi.SPLIT_DERIV_TARGET``) or source
     (``SPLIT_DERIV_SOURCE``) derivative wrapper of the output kernel
     (``expansion_wrangler_fpnd.py:3884-3957``): the pair value becomes
-    ``+-(x_a - y_a) R'(r)/r`` and self pairs give 0 (5293-5300)."""
+    ``+-(x_a - y_a) R'(r)/r`` and self pairs give 0 (5293-5300).
+    ``split_order`` m > 1: the remainder without the terms the split
+    tabulates (``split_terms``; derivative wrappers take m = 1)."""
 
     def __init__(self, tree, k, split_order=1, series_terms=None,
                  deriv_kind=_capi.SPLIT_DERIV_NONE, deriv_axis=0):
-        if int(split_order) != 1:
-            raise NotImplementedError("only helmholtz_split_order=1 is implemented")
+        if int(split_order) < 1:
+            raise ValueError("helmholtz_split_order must be >= 1")
+        if int(split_order) > 1 and int(deriv_kind) != _capi.SPLIT_DERIV_NONE:
+            raise NotImplementedError(
+                "helmholtz_split_order > 1 with a derivative wrapper is not implemented")
         self.dim = int(tree.dim)
         self.k = complex(k)
-        self.split_order = 1
+        self.split_order = int(split_order)
         self.deriv_kind = int(deriv_kind)
         self.deriv_axis = int(deriv_axis)
         if self.deriv_kind not in (_capi.SPLIT_DERIV_NONE, _capi.SPLIT_DERIV_TARGET,
@@ -150,10 +230,13 @@ class HelmholtzSplitCorrection:
         self._counts = torch.as_tensor(counts, device=dev)
         self.max_box_points = int(max(1, counts.max(initial=1)))
 
-    def __call__(self, strengths, target_boxes, starts, lists, out=None, stream=None):
+    def __call__(self, strengths, target_boxes, starts, lists, out=None, stream=None,
+                 coefficients=None):
         """``strengths[R, n_points]`` (or ``[n_points]``), complex; returns the
         correction with the same shape (targets of unlisted boxes zero, or
-        left as they are in ``out``)."""
+        left as they are in ``out``).  ``coefficients``: a radial kernel's
+        series coefficients (``vl1_split_args.coefficients`` layout, N =
+        ``self.nmax``) evaluated instead of the remainder."""
         st = strengths if isinstance(strengths, torch.Tensor) else torch.as_tensor(
             np.asarray(strengths))
         one = st.dim() == 1
@@ -194,6 +277,12 @@ class HelmholtzSplitCorrection:
             ld_strengths=2 * self.n_points, result=yr.data_ptr(), ld_result=2 * self.n_points,
             deriv_kind=self.deriv_kind, deriv_axis=self.deriv_axis,
         )
+        if coefficients is not None:
+            cf = np.ascontiguousarray(coefficients, np.float64)
+            want = 2 + 4 * self.nmax if self.dim == 2 else 2 * self.nmax
+            if cf.shape != (want,):
+                raise ValueError(f"coefficients must hold {want} values")
+            args.coefficients = cf.ctypes.data
         _capi.check(_capi.lib().vl1_split_correction(args, _capi.stream_handle(stream)),
                     "vl1_split_correction")
         return out[0] if one else out

@@ -19,11 +19,14 @@ Helmholtz split (``helmholtz_split``, 2253-2600 and 3491-3836): a Helmholtz
 output kernel backed by Laplace tables evaluates its table part with the
 Laplace scaling (real and imaginary parts of the density as two real
 densities), and ``eval_direct_helmholtz_split_correction`` adds the online
-remainder ``G_k - G_0`` over list 1 (``split.HelmholtzSplitCorrection``).
+remainder ``G_k - G_0`` over list 1 (``split.HelmholtzSplitCorrection``);
+at split order m > 1 also the tabulated terms (``split.split_terms``, each a
+plan over its term table).
 """
 
 from __future__ import annotations
 
+import logging
 from collections import OrderedDict
 
 import numpy as np
@@ -33,6 +36,8 @@ from . import _capi, gallery
 from .kernels import HelmholtzKernel, LaplaceKernel
 from .list1 import NearFieldFromCSR
 from .payload import prepare_table_data_and_entry_map
+
+logger = logging.getLogger(__name__)
 
 
 class TimingFuture:
@@ -82,6 +87,9 @@ class NearFieldWrangler:
         helmholtz_split=None,
         direct_method="plan",
         helmholtz_split_order=1,
+        helmholtz_split_auto_config=None,
+        helmholtz_split_term_tables=None,
+        helmholtz_split_term_build_config=None,
     ):
         """``direct_method``: how ``eval_direct`` evaluates list 1 --
         ``"plan"`` (default) through a plan cached per list-1 CSR, FP64
@@ -91,7 +99,17 @@ class NearFieldWrangler:
 
         ``helmholtz_split``: ``None`` enables the split for Helmholtz output
         kernels whose tables are Laplace tables (``expansion_wrangler_fpnd.py:
-        2482-2520``), ``True`` requires it, ``False`` disables it."""
+        2482-2520``), ``True`` requires it, ``False`` disables it.
+
+        ``helmholtz_split_order`` (2289-2310, 2520-2555): an integer m >= 1 or
+        ``"auto"`` (``helmholtz_split_auto_config``, ``split.choose_auto_split_order``).
+        An order m > 1 tabulates the non-smooth terms of the remainder (2D
+        r^2n log r, 3D r^(2j-1), n, j < m) and leaves the rest to the online
+        sum.  ``helmholtz_split_term_tables`` maps ``("power", p)`` /
+        ``("power_log", p)`` to tables (one per Laplace table level); missing
+        ones are built on the GPU with ``helmholtz_split_term_build_config``
+        (a ``DuffyBuildConfig``, default rule).  Several output kernels run at
+        order 1, as in the reference (2558-2568)."""
         if not getattr(tree, "sources_are_targets", False):
             raise ValueError(
                 "table-based near-field evaluation requires sources and targets "
@@ -138,17 +156,48 @@ class NearFieldWrangler:
         self._plans = OrderedDict()
         self.max_cached_plans = 8
         self._corrections = {}
-        self.helmholtz_split_order = helmholtz_split_order
+        self._term_tables = {}
+        self._term_plans = OrderedDict()
+        self._beta = {}
+        self.helmholtz_split_term_build_config = helmholtz_split_term_build_config
+        for key, tabs in (helmholtz_split_term_tables or {}).items():
+            kind, p = key
+            if kind not in ("power", "power_log"):
+                raise ValueError(f"unknown split term kind {kind!r}")
+            self._term_tables[(kind, int(p))] = self._as_list(tabs)
+        self.helmholtz_split_order = 1
         self.helmholtz_split = False
         if helmholtz_split is None or helmholtz_split:
             ok, why = self._split_support_status()
             if ok:
-                if int(helmholtz_split_order) != 1:
-                    raise NotImplementedError("only helmholtz_split_order=1 is implemented")
                 self.helmholtz_split = True
+                self.helmholtz_split_order = self._resolve_split_order(
+                    helmholtz_split_order, helmholtz_split_auto_config)
             elif helmholtz_split:
                 raise RuntimeError(f"helmholtz_split is not supported here: {why}")
         self._plain_only = set()  # kernels the plan route cannot take
+
+    def _resolve_split_order(self, order, auto_config):
+        """``expansion_wrangler_fpnd.py:2520-2568``."""
+        from .split import choose_auto_split_order
+
+        cfg = dict(auto_config or {})
+        if bool(cfg.get("enabled", False)) or (
+                isinstance(order, str) and order.strip().lower() == "auto"):
+            order = choose_auto_split_order(self.tree, self.target_kernels, cfg)
+            logger.info("Auto-selected helmholtz_split_order=%d", order)
+        order = int(order)
+        if order < 1:
+            raise ValueError("helmholtz_split_order must be >= 1")
+        if order > 1 and len(self.target_kernels) > 1:
+            logger.warning("split order %d requested with %d target kernels; clamping to order "
+                           "1 for multi-output split mode", order, len(self.target_kernels))
+            order = 1
+        if order > 1 and any(split_wrapper_chain(k)[1] != _capi.SPLIT_DERIV_NONE
+                             for k in self.target_kernels):
+            raise NotImplementedError(
+                "helmholtz_split_order > 1 with a derivative wrapper is not implemented")
+        return order
 
     def _split_support_status(self):
         """``_split_target_kernel_support_status`` /
@@ -498,18 +547,156 @@ class NearFieldWrangler:
         """The online list-1 remainder ``sum (G_k - G_0)(x_t - y_s) w_s`` per
         output kernel (``expansion_wrangler_fpnd.py:3491-3836``), with
         ``src_weights`` = density x quadrature weight (``[n_points]`` or
-        ``[R, n_points]``).  Zeros when the split is off.  ``src_func`` (the
-        smooth-quadrature path) is accepted and not used: this library
-        implements the base-quadrature path."""
+        ``[R, n_points]``).  Zeros when the split is off.  This library
+        implements the base-quadrature path.  At split order m > 1 the
+        remainder leaves out the tabulated terms, and their table
+        evaluations on ``src_func`` (the density values, required then, as in
+        the reference, 3580-3586) are added."""
         if not self.helmholtz_split:
             return self._finish(self.output_zeros()), TimingFuture()
+        if self.helmholtz_split_order > 1 and src_func is None:
+            raise RuntimeError("helmholtz_split_order > 1 requires source function samples "
+                               "(src_func) and smooth quadrature order >= q")
         w = src_weights if isinstance(src_weights, torch.Tensor) else torch.as_tensor(
             np.asarray(src_weights))
         pot = np.empty(len(self.target_kernels), dtype=object)
         for i, knl in enumerate(self.target_kernels):
             pot[i] = self._correction(knl)(w, target_boxes, neighbor_source_boxes_starts,
                                            neighbor_source_boxes_lists)
+            if self.helmholtz_split_order > 1:
+                pot[i] = pot[i] + self._split_terms_value(
+                    knl, target_boxes, neighbor_source_boxes_starts, neighbor_source_boxes_lists,
+                    src_func, w)
         return self._finish(pot), TimingFuture()
+
+    # ---------------------------------------------- split terms (order > 1)
+    def split_term_tables(self, knl, key):
+        """Tables of split term ``key`` = ``("power", p)`` / ``("power_log",
+        p)``: the caller's, or built on the GPU at the extents of ``knl``'s
+        Laplace tables (``_autobuild_helmholtz_split_term_tables``,
+        4700-4776)."""
+        if key not in self._term_tables:
+            from .kernels import RadialPowerKernel, RadialPowerLogKernel
+            from .table import DuffyBuildConfig, NearFieldInteractionTable
+
+            kind, p = key
+            dim = self.tree.dim
+            term = RadialPowerKernel(dim, p) if kind == "power" else RadialPowerLogKernel(dim, p)
+            cfg = self.helmholtz_split_term_build_config or DuffyBuildConfig()
+            built = []
+            for t in self.near_field_table[repr(knl)]:
+                tt = NearFieldInteractionTable(t.quad_order, dim, term,
+                                               source_box_extent=t.source_box_extent,
+                                               source_box_level=t.source_box_level)
+                built.append(tt.build_table(cfg))
+            self._term_tables[key] = built
+        tabs = self._term_tables[key]
+        lap = self.near_field_table[repr(knl)]
+        if len(tabs) != len(lap) or any(
+                not np.isclose(a.source_box_extent, b.source_box_extent) for a, b in zip(tabs, lap)):
+            raise RuntimeError(f"split term tables {key} do not match the Laplace table levels")
+        return tabs
+
+    def _term_plan(self, knl, key, target_boxes, starts, lists):
+        """A plan over term tables ``key``: one table scaled by
+        (h/H)^(d + p) (``_helmholtz_split_term_scaling_code``, 4777-4798),
+        or per-level tables unscaled."""
+        from .payload import DevicePayload
+        from .plan import List1Plan
+
+        ck = ("term",) + tuple(key) + _csr_key(knl, target_boxes, starts, lists)[1:]
+        if ck in self._term_plans:
+            self._term_plans.move_to_end(ck)
+            return self._term_plans[ck]
+        tabs = self.split_term_tables(knl, key)
+        t0 = tabs[0]
+        data, nmlz, _, _, _, _ = prepare_table_data_and_entry_map(tabs)
+        payload = DevicePayload(t0.dim, t0.quad_order, data, nmlz, t0.case_indices,
+                                t0.arithmetic_metadata())
+        kw = {}
+        if len(tabs) == 1:
+            e = self.tree.dim + int(key[1])
+            lev = np.arange(_capi.MAX_LEVELS)
+            h = float(self.tree.root_extent) * 0.5 ** lev
+            kw = dict(scaling_mode=_capi.SCALING_CUSTOM,
+                      level_scaling=(h / t0.source_box_extent) ** e,
+                      level_disp=np.zeros(_capi.MAX_LEVELS))
+        else:
+            kw = dict(scaling_mode=_capi.SCALING_PER_LEVEL)
+        plan = List1Plan(
+            self.tree, payload, table_root_extent=t0.source_box_extent,
+            table_starting_level=self.table_starting_level(tabs),
+            target_boxes=np.ascontiguousarray(_host(target_boxes), np.int32),
+            starts=np.ascontiguousarray(_host(starts), np.int32),
+            lists=np.ascontiguousarray(_host(lists), np.int32), **kw)
+        self._term_plans[ck] = plan
+        while len(self._term_plans) > self.max_cached_plans:
+            self._term_plans.popitem(last=False)
+        return plan
+
+    def _split_terms_value(self, knl, target_boxes, starts, lists, src_func, strengths):
+        """sum over the split's terms of coefficient x (term table on
+        ``src_func``), with, for a 2D log term from one table, its
+        log(h/H) part: the point-to-point sum of r^2n over the strengths
+        times log(h_s/H) of each source's box (``beta``, 3689-3800)."""
+        from .split import split_terms
+
+        base, _, _ = split_wrapper_chain(knl)
+        f = src_func if isinstance(src_func, torch.Tensor) else torch.as_tensor(
+            np.asarray(src_func))
+        f = f.to(device="cuda", dtype=torch.complex128)
+        one = f.dim() == 1
+        f2 = (f.unsqueeze(0) if one else f).contiguous()
+        r = int(f2.shape[0])
+        # real term tables: the real and imaginary parts as 2R real rows
+        stacked = torch.cat([f2.real, f2.imag], 0).contiguous()
+        rows = 2 * r
+        bulk = rows - rows % 64 if rows > 64 else rows  # plans take <= 64 rows or multiples
+        st = strengths.to(device="cuda", dtype=torch.complex128)
+        st = (st.unsqueeze(0) if st.dim() == 1 else st).contiguous()
+        total = torch.zeros_like(f2)
+        corr = self._correction(knl)
+        for kind, p, coeff in split_terms(self.tree.dim, base.k, self.helmholtz_split_order):
+            key = (kind, int(p))
+            plan = self._term_plan(knl, key, target_boxes, starts, lists)
+            tmp = torch.zeros_like(stacked)
+            plan.eval(stacked[:bulk], out=tmp[:bulk], form="f64")
+            if bulk < rows:
+                plan.eval(stacked[bulk:], out=tmp[bulk:], form="f64")
+            part = torch.complex(tmp[:r], tmp[r:])
+            if kind == "power_log" and len(self.split_term_tables(knl, key)) == 1:
+                n = p // 2
+                beta = self._log_alpha_per_point(self.split_term_tables(knl, key)[0])
+                pc = corr
+                if n > corr.nmax:  # a series long enough to hold r^2n
+                    from .split import HelmholtzSplitCorrection
+
+                    ck = ("beta", repr(knl), n)
+                    if ck not in self._corrections:
+                        self._corrections[ck] = HelmholtzSplitCorrection(
+                            self.tree, base.k, series_terms=n)
+                    pc = self._corrections[ck]
+                coefs = np.zeros(2 + 4 * pc.nmax)
+                coefs[2 + 4 * (n - 1) + 2] = 1.0  # b_n: r^2n, no log part
+                part = part + pc(st * beta[None, :], target_boxes, starts, lists,
+                                 coefficients=coefs)
+            total = total + complex(coeff) * part
+        return total[0] if one else total
+
+    def _log_alpha_per_point(self, table):
+        """log(h_b / H) of each point's box (``_get_helmholtz_split_log_alpha_per_source``,
+        4904-4974), as a complex CUDA vector."""
+        key = float(table.source_box_extent)
+        if key not in self._beta:
+            starts = np.asarray(self.tree.box_target_starts, np.int64)
+            counts = np.asarray(self.tree.box_target_counts_nonchild, np.int64)
+            levels = np.asarray(self.tree.box_levels, np.int64)
+            beta = np.zeros(int(np.asarray(self.tree.points).shape[1]))
+            for b in np.flatnonzero(counts > 0):
+                h = float(self.tree.root_extent) * 0.5 ** int(levels[b])
+                beta[starts[b]:starts[b] + counts[b]] = np.log(h / float(table.source_box_extent))
+            self._beta[key] = torch.as_tensor(beta, dtype=torch.complex128, device="cuda")
+        return self._beta[key]
 
     def eval_direct_split(self, target_boxes, neighbor_source_boxes_starts,
                           neighbor_source_boxes_lists, mode_coefs, form="f64", n_slices=5,
@@ -520,7 +707,8 @@ class NearFieldWrangler:
         ``mode_coefs * tree.weights``.  ``fused``: both in one plan
         contraction (``form`` "f64" or "split") where the plan takes it (2D:
         node ids in mode order); otherwise, or with ``fused=False``, the
-        table part (plan ``form``) plus the point-to-point correction."""
+        table part (plan ``form``) plus the point-to-point correction (and, at
+        split order m > 1, the tabulated terms: the fused plan is order 1)."""
         if not self.helmholtz_split:
             raise RuntimeError("eval_direct_split needs helmholtz_split enabled")
         mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
@@ -528,7 +716,8 @@ class NearFieldWrangler:
         mc = mc.to(device="cuda", dtype=torch.complex128)
         one = mc.dim() == 1
         mc2 = (mc.unsqueeze(0) if one else mc).contiguous()
-        if fused and all(len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels):
+        if fused and self.helmholtz_split_order == 1 and all(
+                len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels):
             # one contraction: complex blocks = table part + remainder quadrature
             # a split-form limit (slices vs the LDS budget) lets the fused
             # FP64 form (more accurate) stand in before two launches
@@ -555,7 +744,7 @@ class NearFieldWrangler:
         wts = torch.as_tensor(np.asarray(self.tree.weights, np.float64), device="cuda")
         corr, _ = self.eval_direct_helmholtz_split_correction(
             target_boxes, neighbor_source_boxes_starts, neighbor_source_boxes_lists,
-            mc2 * wts[None, :])
+            mc2 * wts[None, :], src_func=mc2)
         for i in range(len(pot)):
             pot[i] = pot[i] + corr[i]
             if one:

@@ -419,6 +419,16 @@ typedef struct vl1_split_args {
    * order m > 1 uses it for its log(h/H) term (the r^2n part with
    * per-source log factors folded into the strengths, 3689-3800). */
   const double* coefficients;
+  /* Optional (NULL = sources are the targets): a separate source set, the
+   * smooth-quadrature path (5116-5275): per box src_box_counts[b] sources
+   * from src_box_starts[b] (device), coordinates src_points [dim][n_src_points];
+   * strengths then index these sources ([n_rhs][ld_strengths], complex),
+   * and no pair is a self pair (r = 0 takes the series value at 0). */
+  const double* src_points;
+  const int32_t* src_box_starts;
+  const int32_t* src_box_counts;
+  int64_t n_src_points;
+  int max_src_box_points;        /* largest src_box_counts entry */
 } vl1_split_args;
 
 /* corr[t] = sum over the list-1 source points s of R(|x_t - y_s|) strength[s],

@@ -197,18 +197,77 @@ def closed_form_difference(dim, k, r):
     return 0.25j * hankel1(0, k * r) + np.log(r) / (2.0 * np.pi)
 
 
+def _bary_interp(src, tgt):
+    """Barycentric interpolation matrix (``_barycentric_interp_matrix``,
+    120-141), weights prod_k 1/(x_j - x_k)."""
+    w = np.array([1.0 / np.prod([xj - xk for xk in src if xk != xj]) for xj in src])
+    out = np.empty((len(tgt), len(src)))
+    for i, x in enumerate(tgt):
+        d = x - np.asarray(src)
+        if np.any(d == 0.0):
+            out[i] = (d == 0.0).astype(float)
+            continue
+        out[i] = (w / d) / np.sum(w / d)
+    return out
+
+
+def smooth_sources(tree, f, m):
+    """The smooth-quadrature sources of order m (5116-5275): per active box
+    the m^d Gauss-Legendre nodes, strengths = f interpolated from the box's
+    q^d nodes (tensor product, box-local node order from ``local_ids``) x
+    extent^d x the Gauss weights.  Returns ``(points [dim, n_src], starts,
+    counts, strengths [R, n_src])``."""
+    dim, q = tree.dim, tree.quad_order
+    f = np.atleast_2d(np.asarray(f, dtype=np.complex128))
+    xq = 0.5 * (np.polynomial.legendre.leggauss(q)[0] + 1.0)
+    xm, wm = np.polynomial.legendre.leggauss(m)
+    xm, wm = 0.5 * (xm + 1.0), 0.5 * wm
+    A = _bary_interp(xq, xm)
+    grids = np.meshgrid(*([xm] * dim), indexing="ij")
+    ref = np.stack([g.ravel() for g in grids])
+    wref = np.prod(np.meshgrid(*([wm] * dim), indexing="ij"), axis=0).ravel()
+    counts = np.asarray(tree.box_target_counts_nonchild)
+    bstarts = np.asarray(tree.box_target_starts)
+    starts = np.zeros(counts.size, np.int64)
+    scounts = np.zeros(counts.size, np.int64)
+    pts, st = [], []
+    cur = 0
+    for b in range(counts.size):
+        starts[b] = cur
+        if counts[b] == 0:
+            continue
+        h = tree.root_extent * 0.5 ** int(tree.box_levels[b])
+        lo = np.asarray(tree.box_centers)[:, b] - 0.5 * h
+        vals = np.empty((f.shape[0], q**dim), dtype=np.complex128)
+        ids = np.asarray(tree.local_ids)[bstarts[b]:bstarts[b] + counts[b]]
+        vals[:, ids] = f[:, bstarts[b]:bstarts[b] + counts[b]]
+        v = vals.reshape((f.shape[0],) + (q,) * dim)
+        if dim == 2:
+            v = np.einsum("ai,bj,rij->rab", A, A, v)
+        else:
+            v = np.einsum("ai,bj,ck,rijk->rabc", A, A, A, v)
+        pts.append(lo[:, None] + h * ref)
+        st.append(v.reshape(f.shape[0], -1) * (h**dim * wref)[None, :])
+        scounts[b] = m**dim
+        cur += m**dim
+    return np.concatenate(pts, axis=1), starts, scounts, np.concatenate(st, axis=1)
+
+
 def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=None,
-               deriv_axis=None, deriv_sign=1, split_order=1, kernel=None):
+               deriv_axis=None, deriv_sign=1, split_order=1, kernel=None, sources=None):
     """P2P of the remainder over list 1: ``out[R, n_points]`` (complex);
     targets of unlisted boxes stay zero.  ``deriv_axis``: the axis of a
     target (``deriv_sign = +1``) or source (``-1``) derivative wrapper.
     ``kernel``: a radial function r -> complex used instead of the
-    remainder (its value at r = 0 for self pairs)."""
+    remainder (its value at r = 0 for self pairs).  ``sources``: ``(points,
+    starts, counts)`` of a separate source set (``smooth_sources``); the
+    strengths then index it and no pair is a self pair (r = 0 takes the
+    series value at 0, which is the self limit)."""
     st = np.atleast_2d(np.asarray(strengths, dtype=np.complex128))
     tb_list = tree.target_boxes if target_boxes is None else target_boxes
     starts = tree.neighbor_source_boxes_starts if starts is None else starts
     lists = tree.neighbor_source_boxes_lists if lists is None else lists
-    out = np.zeros(st.shape, dtype=np.complex128)
+    out = np.zeros((st.shape[0], np.asarray(tree.points).shape[1]), dtype=np.complex128)
     pts = np.asarray(tree.points)
     bs, bc = tree.box_target_starts, tree.box_target_counts_nonchild
     for i, tb in enumerate(tb_list):
@@ -216,8 +275,12 @@ def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=N
         xt = pts[:, t0:t0 + nt]
         acc = np.zeros((st.shape[0], nt), dtype=np.complex128)
         for sb in lists[starts[i]:starts[i + 1]]:
-            s0, ns = int(bs[sb]), int(bc[sb])
-            ys = pts[:, s0:s0 + ns]
+            if sources is None:
+                s0, ns = int(bs[sb]), int(bc[sb])
+                ys = pts[:, s0:s0 + ns]
+            else:
+                s0, ns = int(sources[1][sb]), int(sources[2][sb])
+                ys = np.asarray(sources[0])[:, s0:s0 + ns]
             dv = xt[:, :, None] - ys[:, None, :]  # [dim, nt, ns]
             r = np.sqrt(np.sum(dv ** 2, axis=0))
             same = (xt[:, :, None] == ys[:, None, :]).all(axis=0)

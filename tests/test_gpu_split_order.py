@@ -9,9 +9,12 @@
   test_gpu_duffy pins) and the P2P oracle with the order-m series and the
   r^2n kernel -- to 1e-12 (rel L2 and Linf per RHS), orders 2 and 3, 2D
   and 3D.
+* The smooth-quadrature path (``helmholtz_split_smooth_quad_order`` > q):
+  the remainder over the density interpolated to finer per-box Gauss nodes
+  matches the oracle's restatement to 1e-12, orders 1 and 2.
 * Physics: against list 1 evaluated with directly built Helmholtz tables,
   order 2 is closer than order 1 (the tabulated terms are the non-smooth
-  ones the base quadrature integrates least well).
+  ones the base quadrature integrates least well), and so is "auto".
 """
 
 import numpy as np
@@ -42,7 +45,7 @@ def _setup(dim, q, level, k, m, R=3):
     return tr, knl, lap, w, f
 
 
-def _oracle(tr, knl, lap, w, f, m):
+def _oracle(tr, knl, lap, w, f, m, smooth=None):
     dim = tr.dim
     meta = orbit_ref.case_metadata(dim, tr.quad_order)
     kw = dict(table_root_extent=2.0, own_csr=False)
@@ -56,23 +59,55 @@ def _oracle(tr, knl, lap, w, f, m):
     tot = table_part(lap, mode="canonical", scaling_power=2, log_displacement=dim == 2)
     wts = np.asarray(tr.weights)[None, :]
     corr = w._correction(knl)
-    tot = tot + S.correction(tr, f * wts, knl.k, corr.nmax, split_order=m)
+    if smooth is None:
+        st, src = f * wts, None
+        beta_of = lambda per_box: per_box[_box_of_point(tr)]
+    else:
+        pts, starts, counts, st = S.smooth_sources(tr, f, smooth)
+        src = (pts, starts, counts)
+        beta_of = lambda per_box: np.repeat(per_box, counts)
+    tot = tot + S.correction(tr, st, knl.k, corr.nmax, split_order=m, sources=src)
     h = tr.root_extent * 0.5 ** np.arange(32)
     for kind, p, c in S.extracted_terms(dim, knl.k, m):
         tt = w.split_term_tables(knl, (kind, p))[0]
         part = table_part(tt, mode="custom", level_scaling=(h / 2.0) ** (dim + p),
                           level_disp=np.zeros(32))
         if kind == "power_log":
-            lv = np.asarray(tr.box_levels)
-            beta = np.zeros(f.shape[1])
-            for b in np.flatnonzero(np.asarray(tr.box_target_counts_nonchild) > 0):
-                s0 = tr.box_target_starts[b]
-                beta[s0:s0 + tr.box_target_counts_nonchild[b]] = np.log(
-                    tr.root_extent * 0.5 ** lv[b] / 2.0)
-            part = part + S.correction(tr, f * wts * beta[None, :], knl.k, corr.nmax,
-                                       kernel=lambda r, p=p: r ** p)
+            per_box = np.log(tr.root_extent * 0.5 ** np.asarray(tr.box_levels, float) / 2.0)
+            beta = beta_of(per_box)
+            part = part + S.correction(tr, st * beta[None, :], knl.k, corr.nmax,
+                                       kernel=lambda r, p=p: r ** p, sources=src)
         tot = tot + c * part
     return tot
+
+
+def _box_of_point(tr):
+    out = np.zeros(np.asarray(tr.points).shape[1], np.int64)
+    for b in np.flatnonzero(np.asarray(tr.box_target_counts_nonchild) > 0):
+        out[tr.box_target_starts[b]:tr.box_target_starts[b] + tr.box_target_counts_nonchild[b]] = b
+    return out
+
+
+@pytest.mark.parametrize("dim,q,level,k,m,ms", [(2, 4, 3, 8.0, 1, 6), (2, 4, 3, 8.0, 2, 7),
+                                                (3, 3, 2, 6.0, 1, 5), (3, 3, 2, 6.0, 2, 4)])
+def test_smooth_quadrature_path_matches_oracle(cuda, dim, q, level, k, m, ms):
+    """helmholtz_split_smooth_quad_order > q: the remainder (and the 2D
+    log(h/H) term) over the interpolated density on the finer nodes."""
+    print(f"case: smooth quadrature order {ms}, split order {m}, {dim}D q={q} k={k}", flush=True)
+    tr = vtree.uniform_tree(dim, q, level)
+    knl = HelmholtzKernel(dim, k)
+    lap = NearFieldInteractionTable(q, dim, LaplaceKernel(dim), source_box_extent=2.0)
+    lap.build_table(RULE)
+    w = NearFieldWrangler(tr, {repr(knl): [lap]}, target_kernels=[knl], helmholtz_split_order=m,
+                          helmholtz_split_term_build_config=RULE,
+                          helmholtz_split_smooth_quad_order=ms)
+    f = np.stack([densities.helmholtz_beams(tr.points, 80 + r, k=k, width=0.3) for r in range(2)])
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"))[0][0]
+    ref = _oracle(tr, knl, lap, w, f, m, smooth=ms)
+    l2, linf = rel_errors(got.cpu().numpy(), ref)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= 1e-12 and linf <= 1e-12
 
 
 @pytest.mark.parametrize("dim,q,level,k,m", [(3, 3, 2, 6.0, 2), (3, 3, 2, 6.0, 3),
@@ -117,3 +152,11 @@ def test_order_2_closer_to_helmholtz_tables(cuda, dim, q, level, k):
     print("rel L2 vs Helmholtz tables, orders 1-3:", ", ".join(f"{e:.2e}" for e in errs),
           flush=True)
     assert errs[1] < errs[0]
+    # "auto": the order from |k| h and, above q, the smooth quadrature
+    wa = NearFieldWrangler(tr, {repr(knl): [lap]}, target_kernels=[knl],
+                           helmholtz_split_order="auto")
+    got = wa.eval_direct_split(*args, f)[0][0].cpu().numpy()
+    print(f"auto: order {wa.helmholtz_split_order}, smooth order "
+          f"{wa.helmholtz_split_smooth_quad_order}, rel L2 {rel_errors(got, direct)[0]:.2e}",
+          flush=True)
+    assert rel_errors(got, direct)[0] < errs[0]

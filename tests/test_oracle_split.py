@@ -238,3 +238,55 @@ def test_auto_split_order_ladder():
     assert pick(0.9, orders=(2, 4, 6, 8), rho_thresholds=(0.5, 1.5, 3.0)) == 2
     with pytest.raises(ValueError):
         pick(1.0, order_min=5, order_max=3)
+
+
+@pytest.mark.parametrize("dim,q,m", [(2, 3, 5), (2, 4, 4), (3, 2, 4)])
+def test_smooth_sources_integrate_the_interpolant(dim, q, m):
+    """The smooth sources of order m carry the box interpolant: for f a
+    polynomial of degree < q per axis their strengths sum, per box, to the
+    box integral of f (= the q-point Gauss sum), and at m = q they are the
+    base nodes with weight x f."""
+    tr = vtree.uniform_tree(dim, q, 1)
+    P = np.asarray(tr.points)
+    f = (1.0 + 0.5 * P[0] - 0.25 * P[-1] ** (q - 1) + 0.1j * P[0] * P[-1] ** (q - 2))[None, :]
+    pts, starts, counts, st = S.smooth_sources(tr, f, m)
+    w = np.asarray(tr.weights)
+    for b in np.flatnonzero(np.asarray(tr.box_target_counts_nonchild) > 0):
+        s0, n = tr.box_target_starts[b], tr.box_target_counts_nonchild[b]
+        base = np.sum(f[0, s0:s0 + n] * w[s0:s0 + n])
+        assert np.isclose(np.sum(st[0, starts[b]:starts[b] + counts[b]]), base, rtol=1e-13)
+        h = tr.root_extent * 0.5 ** tr.box_levels[b]
+        c = np.asarray(tr.box_centers)[:, b]
+        seg = pts[:, starts[b]:starts[b] + counts[b]]
+        assert np.all(np.abs(seg - c[:, None]) < 0.5 * h)
+    if m == q:
+        assert np.allclose(np.sort(st[0].real), np.sort((f[0] * w).real), rtol=1e-13)
+
+
+def test_smooth_correction_at_base_order_is_base_correction():
+    """At m = q the smooth sources are the tree's own nodes: the correction
+    over them (r = 0 taking the series value, the self limit) equals the
+    base-quadrature correction."""
+    tr = vtree.uniform_tree(2, 3, 2)
+    P = np.asarray(tr.points)
+    f = np.exp(-np.sum(P**2, axis=0))[None, :] * (1.0 + 0.3j)
+    k = 5.0
+    nmax = S.series_nmax(2, k, 1.5)
+    base = S.correction(tr, f * np.asarray(tr.weights)[None, :], k, nmax)
+    pts, starts, counts, st = S.smooth_sources(tr, f, 3)
+    smooth = S.correction(tr, st, k, nmax, sources=(pts, starts, counts))
+    assert np.allclose(smooth, base, rtol=1e-12, atol=1e-15)
+
+
+def test_auto_smooth_quad_order_rule():
+    """``"auto"`` smooth order (2649-2741): q + (m - 1) per order, plus
+    ceil(rho_imag - 4) and ceil(0.5 (rho_real - 3)) above the hard
+    thresholds, at least q, at most smooth_quad_order_max."""
+    from volumential_amd.split import auto_smooth_quad_order as A
+
+    assert A(6, 1, 0.5, 0.0) == 6
+    assert A(6, 3, 0.5, 0.0) == 8
+    assert A(6, 2, 5.0, 0.0) == 6 + 1 + 1          # real boost ceil(0.5 * 2)
+    assert A(6, 2, 0.0, 6.5) == 6 + 1 + 3          # imaginary boost ceil(2.5)
+    assert A(6, 4, 0.0, 6.5, {"smooth_quad_order_max": 9}) == 9
+    assert A(6, 1, 0.0, 0.0, {"smooth_quad_order_min": 8}) == 8

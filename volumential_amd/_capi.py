@@ -251,6 +251,11 @@ class SplitArgs(C.Structure):
         ("deriv_kind", C.c_int),
         ("deriv_axis", C.c_int),
         ("coefficients", vp),
+        ("src_points", vp),
+        ("src_box_starts", vp),
+        ("src_box_counts", vp),
+        ("n_src_points", i64),
+        ("max_src_box_points", C.c_int),
     ]
 
 

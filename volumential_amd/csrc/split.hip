@@ -44,6 +44,10 @@ struct SplitParams {
   const int32_t* box_counts;
   const double* points;  // [dim][n_points]
   int64_t n_points;
+  const int32_t* src_starts;  // sources: the targets, or a separate set
+  const int32_t* src_counts;
+  const double* src_points;   // [dim][n_src]
+  int64_t n_src;
   const double* strengths;  // [n_rhs][lds] interleaved complex
   int64_t lds;
   double* result;  // [n_rhs][ldr] interleaved complex
@@ -85,14 +89,14 @@ __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const Spl
       for (int r = 0; r < RC; ++r) acc[r] = aci[r] = 0.0;
       for (int li = l0; li < l1; ++li) {
         const int sb = p.nbr_lists[li];
-        const int sbeg = p.box_starts[sb];
-        const int ns = p.box_counts[sb];
+        const int sbeg = p.src_starts[sb];
+        const int ns = p.src_counts[sb];
         for (int c0 = 0; c0 < ns; c0 += chunk) {
           const int nc = min(chunk, ns - c0);
           __syncthreads();  // previous chunk done with LDS (and coefficients staged)
           for (int i = tid; i < DIM * nc; i += kSplitBlock) {
             const int a = i / nc, j = i % nc;
-            sX[a * chunk + j] = p.points[(int64_t)a * p.n_points + sbeg + c0 + j];
+            sX[a * chunk + j] = p.src_points[(int64_t)a * p.n_src + sbeg + c0 + j];
           }
           for (int i = tid; i < nc * RC * 2; i += kSplitBlock) {
             const int c = i & 1, rj = i >> 1;
@@ -188,8 +192,16 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   if (!a->target_boxes || !a->nbr_starts || !a->nbr_lists || !a->box_starts || !a->box_counts ||
       !a->points || !a->strengths || !a->result)
     return fail(VL1_E_ARG, "vl1_split_correction: null device array");
-  if (a->n_rhs > 1 && (a->ld_strengths < 2 * a->n_points || a->ld_result < 2 * a->n_points))
+  const bool sep = a->src_points != nullptr;
+  if (sep && (!a->src_box_starts || !a->src_box_counts || a->n_src_points < 1 ||
+              a->max_src_box_points < 1))
+    return fail(VL1_E_ARG, "vl1_split_correction: incomplete separate source set");
+  const int64_t n_src = sep ? a->n_src_points : a->n_points;
+  if (a->n_rhs > 1 && (a->ld_strengths < 2 * n_src || a->ld_result < 2 * a->n_points))
     return fail(VL1_E_ARG, "vl1_split_correction: bad leading dimension");
+  if (sep && a->deriv_kind != VL1_SPLIT_DERIV_NONE)
+    return fail(VL1_E_UNSUPPORTED,
+                "vl1_split_correction: a separate source set takes no derivative wrapper");
   if (a->deriv_kind < VL1_SPLIT_DERIV_NONE || a->deriv_kind > VL1_SPLIT_DERIV_SOURCE)
     return fail(VL1_E_ARG, "vl1_split_correction: bad deriv_kind");
   const bool deriv = a->deriv_kind != VL1_SPLIT_DERIV_NONE;
@@ -230,8 +242,12 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   prm.box_counts = a->box_counts;
   prm.points = a->points;
   prm.n_points = a->n_points;
+  prm.src_starts = sep ? a->src_box_starts : a->box_starts;
+  prm.src_counts = sep ? a->src_box_counts : a->box_counts;
+  prm.src_points = sep ? a->src_points : a->points;
+  prm.n_src = n_src;
   prm.strengths = a->strengths;
-  prm.lds = a->n_rhs > 1 ? a->ld_strengths : 2 * a->n_points;
+  prm.lds = a->n_rhs > 1 ? a->ld_strengths : 2 * n_src;
   prm.result = a->result;
   prm.ldr = a->n_rhs > 1 ? a->ld_result : 2 * a->n_points;
   int nt = 1;
@@ -241,8 +257,9 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   // the remainder is evaluated once per (target, source) pair and RC-chunk;
   // RC <= 8 keeps the LDS small enough for several workgroups per CU
   const int RC = a->n_rhs == 1 ? 1 : (a->n_rhs <= 4 ? 4 : 8);
-  // source points staged per pass: the box size, rounded up to the parts
-  const int chunk = std::min(nt * P, (a->max_box_points + P - 1) / P * P);
+  // source points staged per pass: the source box size, rounded up to the parts
+  const int msrc = sep ? a->max_src_box_points : a->max_box_points;
+  const int chunk = std::min(nt * P, (msrc + P - 1) / P * P);
   prm.chunk = chunk;
   const size_t stage = (size_t)a->dim * chunk + (size_t)chunk * RC * 2;
   const size_t reduce = P > 1 ? (size_t)kSplitBlock * RC * 2 : 0;  // aliases the staging

@@ -160,19 +160,153 @@ def choose_auto_split_order(tree, kernels, config=None):
         orders = tuple(range(order_min, order_min + len(th_re) + 1))
     if len(orders) != len(th_re) + 1:
         raise ValueError("orders must have one more element than thresholds")
+    rho_re, rho_im = split_rho(tree, kernels)
+    pick = max(_split_order_ladder(rho_re, th_re, orders), _split_order_ladder(rho_im, th_im, orders))
+    if "orders" not in cfg:
+        pick = max(order_min, min(order_max, pick))
+    return pick
+
+
+def interp_matrix(src_nodes, tgt_nodes):
+    """Barycentric interpolation matrix from ``src_nodes`` to ``tgt_nodes``
+    (exact at coinciding nodes; ``_barycentric_interp_matrix``, 120-141)."""
+    from .quadrature import barycentric_weights
+
+    src = np.asarray(src_nodes, np.float64)
+    tgt = np.asarray(tgt_nodes, np.float64)
+    if src.size == 1:
+        return np.ones((tgt.size, 1))
+    w = barycentric_weights(src)
+    out = np.empty((tgt.size, src.size))
+    for i, x in enumerate(tgt):
+        d = x - src
+        hit = np.flatnonzero(d == 0.0)
+        if hit.size:
+            out[i] = 0.0
+            out[i, hit[0]] = 1.0
+            continue
+        t = w / d
+        out[i] = t / t.sum()
+    return out
+
+
+class SmoothSources:
+    """Sources of the smooth-quadrature correction path (5116-5275): per
+    active box the m^d tensor Gauss-Legendre nodes of order m (the smooth
+    quadrature order), strengths = the density interpolated from the box's
+    q^d nodes times extent^d times the Gauss weights.  Geometry is built
+    once per (tree, m); ``strengths`` interpolates on the GPU."""
+
+    def __init__(self, tree, m):
+        from .quadrature import gauss_legendre_01
+
+        dim, q = int(tree.dim), int(tree.quad_order)
+        self.dim, self.q, self.m = dim, q, int(m)
+        xq, _ = gauss_legendre_01(q)
+        xm, wm = gauss_legendre_01(self.m)
+        self.interp = interp_matrix(xq, xm)  # [m, q]
+        grids = np.meshgrid(*([xm] * dim), indexing="ij")
+        ref = np.stack([g.ravel() for g in grids])  # [dim, m^d]
+        wref = np.prod(np.meshgrid(*([wm] * dim), indexing="ij"), axis=0).ravel()
+        counts = np.asarray(tree.box_target_counts_nonchild, np.int64)
+        starts = np.asarray(tree.box_target_starts, np.int64)
+        levels = np.asarray(tree.box_levels, np.int64)
+        centers = np.asarray(tree.box_centers, np.float64)
+        ids = np.asarray(tree.local_ids, np.int64)
+        nq, nm = q**dim, self.m**dim
+        active = np.flatnonzero(counts > 0)
+        if np.any(counts[active] != nq):
+            raise ValueError(f"the smooth correction requires exactly {nq} sources per active box")
+        sstarts = np.zeros(counts.size, np.int32)
+        scounts = np.zeros(counts.size, np.int32)
+        pts = np.empty((dim, active.size * nm))
+        sw = np.empty(active.size * nm)
+        gather = np.empty((active.size, nq), np.int64)  # point of each box mode
+        for i, b in enumerate(active):
+            sstarts[b], scounts[b] = i * nm, nm
+            h = float(tree.root_extent) * 0.5 ** int(levels[b])
+            lo = centers[:, b] - 0.5 * h
+            pts[:, i * nm:(i + 1) * nm] = lo[:, None] + h * ref
+            sw[i * nm:(i + 1) * nm] = h**dim * wref
+            gather[i, ids[starts[b]:starts[b] + nq]] = np.arange(starts[b], starts[b] + nq)
+        dev = "cuda"
+        self.n_src = int(active.size * nm)
+        self.points = torch.as_tensor(pts, device=dev)
+        self.box_starts = torch.as_tensor(sstarts, device=dev)
+        self.box_counts = torch.as_tensor(scounts, device=dev)
+        self.weights = torch.as_tensor(sw, dtype=torch.complex128, device=dev)
+        self._gather = torch.as_tensor(gather.ravel(), device=dev)
+        self._n_active = int(active.size)
+        self._interp = torch.as_tensor(self.interp, dtype=torch.complex128, device=dev)
+        self._box_of_source = np.repeat(active, nm)
+
+    def strengths(self, f):
+        """``f[R, n_points]`` (complex, box-node values) -> ``[R, n_src]``."""
+        R = int(f.shape[0])
+        dim, q = self.dim, self.q
+        v = f[:, self._gather].reshape((R, self._n_active) + (q,) * dim)
+        A = self._interp
+        if dim == 2:
+            v = torch.matmul(torch.matmul(A, v), A.T)
+        else:
+            v = torch.einsum("ai,rbijk->rbajk", A, v)
+            v = torch.einsum("bj,rnajk->rnabk", A, v)
+            v = torch.einsum("ck,rnabk->rnabc", A, v)
+        return v.reshape(R, self.n_src) * self.weights[None, :]
+
+    def per_source(self, per_box):
+        """A per-box value array expanded to the sources (host array)."""
+        return np.asarray(per_box)[self._box_of_source]
+
+
+def auto_smooth_quad_order(q, split_order, rho_real, rho_imag, config=None):
+    """The smooth quadrature order of ``"auto"`` mode (2649-2741):
+    max(min, q + (m - 1) * per_order + boosts), boosts from |Im k| h above
+    the hard threshold (4) and |Re k| h above 3 (scale 0.5), at most
+    ``smooth_quad_order_max``."""
+    cfg = dict(config or {})
+    min_smooth = int(cfg.get("smooth_quad_order_min", q))
+    per = int(cfg.get("smooth_quad_order_per_order", 1))
+    per_hard = int(cfg.get("smooth_quad_order_per_order_hard", 1))
+    if per < 0 or per_hard < 0:
+        raise ValueError("smooth quadrature per-order increments must be non-negative")
+    hard_im = float(cfg.get("smooth_quad_order_hard_rho_imag", 4.0))
+    hard_re = float(cfg.get("smooth_quad_order_hard_rho_real", 3.0))
+    if hard_re <= 0.0:
+        raise ValueError("smooth_quad_order_hard_rho_real must be > 0")
+    add = per_hard if (rho_imag >= hard_im or rho_real >= hard_re) else per
+    b_start = float(cfg.get("smooth_quad_order_rho_boost_start", hard_im))
+    b_scale = float(cfg.get("smooth_quad_order_rho_boost_scale", 1.0))
+    if b_scale < 0.0:
+        raise ValueError("smooth_quad_order_rho_boost_scale must be >= 0")
+    boost = int(math.ceil(max(0.0, rho_imag - b_start) * b_scale))
+    if cfg.get("smooth_quad_order_rho_boost_cap") is not None:
+        boost = min(boost, int(cfg["smooth_quad_order_rho_boost_cap"]))
+    r_start = float(cfg.get("smooth_quad_order_real_boost_start", hard_re))
+    r_scale = float(cfg.get("smooth_quad_order_real_boost_scale", 0.5))
+    if r_scale < 0.0:
+        raise ValueError("smooth_quad_order_real_boost_scale must be >= 0")
+    rboost = int(math.ceil(max(0.0, rho_real - r_start) * r_scale))
+    if cfg.get("smooth_quad_order_real_boost_cap") is not None:
+        rboost = min(rboost, int(cfg["smooth_quad_order_real_boost_cap"]))
+    order = max(min_smooth, q + max(0, int(split_order) - 1) * add + boost + rboost)
+    if cfg.get("smooth_quad_order_max") is not None:
+        order = min(order, int(cfg["smooth_quad_order_max"]))
+    return order
+
+
+def split_rho(tree, kernels):
+    """(|Re k| h, |Im k| h) for the coarsest active box extent h (4228-4250)."""
     counts = np.asarray(tree.box_target_counts_nonchild)
     active = np.flatnonzero(counts > 0)
     if active.size == 0:
-        return order_min
+        return 0.0, 0.0
     h = float(tree.root_extent) * 0.5 ** int(np.asarray(tree.box_levels)[active].min())
     kr = max((abs(complex(getattr(kn.get_base_kernel(), "k", 0.0)).real) for kn in kernels),
              default=0.0)
     ki = max((abs(complex(getattr(kn.get_base_kernel(), "k", 0.0)).imag) for kn in kernels),
              default=0.0)
-    pick = max(_split_order_ladder(kr * h, th_re, orders), _split_order_ladder(ki * h, th_im, orders))
-    if "orders" not in cfg:
-        pick = max(order_min, min(order_max, pick))
-    return pick
+    return kr * h, ki * h
 
 
 class HelmholtzSplitCorrection:
@@ -231,25 +365,32 @@ class HelmholtzSplitCorrection:
         self.max_box_points = int(max(1, counts.max(initial=1)))
 
     def __call__(self, strengths, target_boxes, starts, lists, out=None, stream=None,
-                 coefficients=None):
+                 coefficients=None, sources=None):
         """``strengths[R, n_points]`` (or ``[n_points]``), complex; returns the
         correction with the same shape (targets of unlisted boxes zero, or
         left as they are in ``out``).  ``coefficients``: a radial kernel's
         series coefficients (``vl1_split_args.coefficients`` layout, N =
-        ``self.nmax``) evaluated instead of the remainder."""
+        ``self.nmax``) evaluated instead of the remainder.  ``sources``: a
+        ``SmoothSources`` set; ``strengths`` then index its sources
+        (``[R, sources.n_src]``) and the result still lives on the tree's
+        points."""
         st = strengths if isinstance(strengths, torch.Tensor) else torch.as_tensor(
             np.asarray(strengths))
         one = st.dim() == 1
         st = (st.unsqueeze(0) if one else st).to(device="cuda", dtype=torch.complex128)
         st = st.contiguous()
-        if st.shape[1] != self.n_points:
+        n_src = self.n_points if sources is None else int(sources.n_src)
+        if sources is not None and sources.dim != self.dim:
+            raise ValueError("source set of another dimension")
+        if st.shape[1] != n_src:
             raise ValueError("strengths have the wrong number of points")
+        shape = (int(st.shape[0]), self.n_points)
         if out is None:
-            out = torch.zeros_like(st)
+            out = torch.zeros(shape, dtype=torch.complex128, device="cuda")
         else:
             out = out.unsqueeze(0) if out.dim() == 1 else out
-            if out.shape != st.shape or out.dtype != torch.complex128 or not out.is_contiguous():
-                raise ValueError("out must be a contiguous complex128 tensor shaped like strengths")
+            if tuple(out.shape) != shape or out.dtype != torch.complex128 or not out.is_contiguous():
+                raise ValueError("out must be a contiguous complex128 tensor [R, n_points]")
         tb_h = np.ascontiguousarray(_host(target_boxes), np.int32)
         ns_h = np.ascontiguousarray(_host(starts), np.int32)
         nl_h = np.ascontiguousarray(_host(lists), np.int32)
@@ -274,9 +415,15 @@ class HelmholtzSplitCorrection:
             nbr_lists=nl.data_ptr() if nl.numel() else ns.data_ptr(),
             box_starts=self._starts.data_ptr(), box_counts=self._counts.data_ptr(),
             points=self._points.data_ptr(), strengths=sr.data_ptr(),
-            ld_strengths=2 * self.n_points, result=yr.data_ptr(), ld_result=2 * self.n_points,
+            ld_strengths=2 * n_src, result=yr.data_ptr(), ld_result=2 * self.n_points,
             deriv_kind=self.deriv_kind, deriv_axis=self.deriv_axis,
         )
+        if sources is not None:
+            args.src_points = sources.points.data_ptr()
+            args.src_box_starts = sources.box_starts.data_ptr()
+            args.src_box_counts = sources.box_counts.data_ptr()
+            args.n_src_points = n_src
+            args.max_src_box_points = int(sources.m) ** self.dim
         if coefficients is not None:
             cf = np.ascontiguousarray(coefficients, np.float64)
             want = 2 + 4 * self.nmax if self.dim == 2 else 2 * self.nmax

@@ -90,6 +90,7 @@ class NearFieldWrangler:
         helmholtz_split_auto_config=None,
         helmholtz_split_term_tables=None,
         helmholtz_split_term_build_config=None,
+        helmholtz_split_smooth_quad_order=None,
     ):
         """``direct_method``: how ``eval_direct`` evaluates list 1 --
         ``"plan"`` (default) through a plan cached per list-1 CSR, FP64
@@ -109,7 +110,13 @@ class NearFieldWrangler:
         ``("power_log", p)`` to tables (one per Laplace table level); missing
         ones are built on the GPU with ``helmholtz_split_term_build_config``
         (a ``DuffyBuildConfig``, default rule).  Several output kernels run at
-        order 1, as in the reference (2558-2568)."""
+        order 1, as in the reference (2558-2568).
+
+        ``helmholtz_split_smooth_quad_order`` (2303-2306, 2649-2741): the
+        tensor Gauss order of the online remainder's sources; above q the
+        correction runs on those finer per-box nodes with the density
+        interpolated to them (given ``src_func``, 3562-3597).  Default: q, or
+        in ``"auto"`` mode the reference's rule (``split.auto_smooth_quad_order``)."""
         if not getattr(tree, "sources_are_targets", False):
             raise ValueError(
                 "table-based near-field evaluation requires sources and targets "
@@ -167,12 +174,17 @@ class NearFieldWrangler:
             self._term_tables[(kind, int(p))] = self._as_list(tabs)
         self.helmholtz_split_order = 1
         self.helmholtz_split = False
+        self.helmholtz_split_smooth_quad_order = None
+        self._smooth_sources = {}
         if helmholtz_split is None or helmholtz_split:
             ok, why = self._split_support_status()
             if ok:
                 self.helmholtz_split = True
                 self.helmholtz_split_order = self._resolve_split_order(
                     helmholtz_split_order, helmholtz_split_auto_config)
+                self.helmholtz_split_smooth_quad_order = self._resolve_smooth_order(
+                    helmholtz_split_smooth_quad_order, helmholtz_split_order,
+                    helmholtz_split_auto_config)
             elif helmholtz_split:
                 raise RuntimeError(f"helmholtz_split is not supported here: {why}")
         self._plain_only = set()  # kernels the plan route cannot take
@@ -198,6 +210,33 @@ class NearFieldWrangler:
             raise NotImplementedError(
                 "helmholtz_split_order > 1 with a derivative wrapper is not implemented")
         return order
+
+    def _resolve_smooth_order(self, smooth_order, order_arg, auto_config):
+        """``expansion_wrangler_fpnd.py:2649-2747``."""
+        from .split import auto_smooth_quad_order, split_rho
+
+        cfg = dict(auto_config or {})
+        auto = bool(cfg.get("enabled", False)) or (
+            isinstance(order_arg, str) and order_arg.strip().lower() == "auto")
+        if smooth_order is None:
+            if auto:
+                rho_re, rho_im = split_rho(self.tree, self.target_kernels)
+                smooth_order = auto_smooth_quad_order(
+                    int(self.tree.quad_order), self.helmholtz_split_order, rho_re, rho_im, cfg)
+            else:
+                smooth_order = int(self.tree.quad_order)
+        smooth_order = int(smooth_order)
+        if smooth_order < 1:
+            raise ValueError("helmholtz_split_smooth_quad_order must be >= 1")
+        return smooth_order
+
+    def smooth_sources(self, m):
+        """The smooth-quadrature source set of order ``m`` (cached)."""
+        from .split import SmoothSources
+
+        if m not in self._smooth_sources:
+            self._smooth_sources[m] = SmoothSources(self.tree, m)
+        return self._smooth_sources[m]
 
     def _split_support_status(self):
         """``_split_target_kernel_support_status`` /
@@ -554,19 +593,33 @@ class NearFieldWrangler:
         the reference, 3580-3586) are added."""
         if not self.helmholtz_split:
             return self._finish(self.output_zeros()), TimingFuture()
-        if self.helmholtz_split_order > 1 and src_func is None:
+        q = int(self.tree.quad_order)
+        ms = self.helmholtz_split_smooth_quad_order
+        smooth = ms is not None and ms > q and src_func is not None
+        if self.helmholtz_split_order > 1 and not (
+                smooth or (ms == q and src_func is not None)):
             raise RuntimeError("helmholtz_split_order > 1 requires source function samples "
                                "(src_func) and smooth quadrature order >= q")
         w = src_weights if isinstance(src_weights, torch.Tensor) else torch.as_tensor(
             np.asarray(src_weights))
+        sources, st = None, w
+        if smooth:  # interpolated density on the finer per-box nodes (5188-5275)
+            sources = self.smooth_sources(ms)
+            f = src_func if isinstance(src_func, torch.Tensor) else torch.as_tensor(
+                np.asarray(src_func))
+            f = f.to(device="cuda", dtype=torch.complex128)
+            one = f.dim() == 1
+            st = sources.strengths(f.unsqueeze(0) if one else f)
+            if one:
+                st = st[0]
         pot = np.empty(len(self.target_kernels), dtype=object)
         for i, knl in enumerate(self.target_kernels):
-            pot[i] = self._correction(knl)(w, target_boxes, neighbor_source_boxes_starts,
-                                           neighbor_source_boxes_lists)
+            pot[i] = self._correction(knl)(st, target_boxes, neighbor_source_boxes_starts,
+                                           neighbor_source_boxes_lists, sources=sources)
             if self.helmholtz_split_order > 1:
                 pot[i] = pot[i] + self._split_terms_value(
                     knl, target_boxes, neighbor_source_boxes_starts, neighbor_source_boxes_lists,
-                    src_func, w)
+                    src_func, st, sources)
         return self._finish(pot), TimingFuture()
 
     # ---------------------------------------------- split terms (order > 1)
@@ -634,7 +687,8 @@ class NearFieldWrangler:
             self._term_plans.popitem(last=False)
         return plan
 
-    def _split_terms_value(self, knl, target_boxes, starts, lists, src_func, strengths):
+    def _split_terms_value(self, knl, target_boxes, starts, lists, src_func, strengths,
+                           sources=None):
         """sum over the split's terms of coefficient x (term table on
         ``src_func``), with, for a 2D log term from one table, its
         log(h/H) part: the point-to-point sum of r^2n over the strengths
@@ -666,7 +720,7 @@ class NearFieldWrangler:
             part = torch.complex(tmp[:r], tmp[r:])
             if kind == "power_log" and len(self.split_term_tables(knl, key)) == 1:
                 n = p // 2
-                beta = self._log_alpha_per_point(self.split_term_tables(knl, key)[0])
+                beta = self._log_alpha(self.split_term_tables(knl, key)[0], sources)
                 pc = corr
                 if n > corr.nmax:  # a series long enough to hold r^2n
                     from .split import HelmholtzSplitCorrection
@@ -679,22 +733,27 @@ class NearFieldWrangler:
                 coefs = np.zeros(2 + 4 * pc.nmax)
                 coefs[2 + 4 * (n - 1) + 2] = 1.0  # b_n: r^2n, no log part
                 part = part + pc(st * beta[None, :], target_boxes, starts, lists,
-                                 coefficients=coefs)
+                                 coefficients=coefs, sources=sources)
             total = total + complex(coeff) * part
         return total[0] if one else total
 
-    def _log_alpha_per_point(self, table):
-        """log(h_b / H) of each point's box (``_get_helmholtz_split_log_alpha_per_source``,
-        4904-4974), as a complex CUDA vector."""
-        key = float(table.source_box_extent)
+    def _log_alpha(self, table, sources=None):
+        """log(h_b / H) of each source's box (``_get_helmholtz_split_log_alpha_per_source``,
+        4904-4974): per tree point, or per source of a smooth set; a complex
+        CUDA vector."""
+        key = (float(table.source_box_extent), None if sources is None else sources.m)
         if key not in self._beta:
-            starts = np.asarray(self.tree.box_target_starts, np.int64)
-            counts = np.asarray(self.tree.box_target_counts_nonchild, np.int64)
             levels = np.asarray(self.tree.box_levels, np.int64)
-            beta = np.zeros(int(np.asarray(self.tree.points).shape[1]))
-            for b in np.flatnonzero(counts > 0):
-                h = float(self.tree.root_extent) * 0.5 ** int(levels[b])
-                beta[starts[b]:starts[b] + counts[b]] = np.log(h / float(table.source_box_extent))
+            per_box = np.log(float(self.tree.root_extent) * 0.5 ** levels
+                             / float(table.source_box_extent))
+            if sources is not None:
+                beta = sources.per_source(per_box)
+            else:
+                starts = np.asarray(self.tree.box_target_starts, np.int64)
+                counts = np.asarray(self.tree.box_target_counts_nonchild, np.int64)
+                beta = np.zeros(int(np.asarray(self.tree.points).shape[1]))
+                for b in np.flatnonzero(counts > 0):
+                    beta[starts[b]:starts[b] + counts[b]] = per_box[b]
             self._beta[key] = torch.as_tensor(beta, dtype=torch.complex128, device="cuda")
         return self._beta[key]
 
@@ -708,7 +767,8 @@ class NearFieldWrangler:
         contraction (``form`` "f64" or "split") where the plan takes it (2D:
         node ids in mode order); otherwise, or with ``fused=False``, the
         table part (plan ``form``) plus the point-to-point correction (and, at
-        split order m > 1, the tabulated terms: the fused plan is order 1)."""
+        split order m > 1, the tabulated terms: the fused plan is order 1 on
+        the base quadrature)."""
         if not self.helmholtz_split:
             raise RuntimeError("eval_direct_split needs helmholtz_split enabled")
         mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
@@ -716,7 +776,8 @@ class NearFieldWrangler:
         mc = mc.to(device="cuda", dtype=torch.complex128)
         one = mc.dim() == 1
         mc2 = (mc.unsqueeze(0) if one else mc).contiguous()
-        if fused and self.helmholtz_split_order == 1 and all(
+        base_quad = (self.helmholtz_split_smooth_quad_order or 0) <= int(self.tree.quad_order)
+        if fused and self.helmholtz_split_order == 1 and base_quad and all(
                 len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels):
             # one contraction: complex blocks = table part + remainder quadrature
             # a split-form limit (slices vs the LDS budget) lets the fused

@@ -24,8 +24,9 @@
  *                        (table part and split correction in one plan)
  *   vl1_duffy_build      nearfield_potential_table.py:2034-2213,2328-2463
  *                        (duffy_invariant_fused_table kernel and its driver)
- *   vl1_split_correction expansion_wrangler_fpnd.py:3491-3836, 212-312
- *                        (Helmholtz split correction, order 1)
+ *   vl1_split_correction expansion_wrangler_fpnd.py:3491-3836, 212-312,
+ *                        5116-5275 (Helmholtz split correction, any split
+ *                        order, base or smooth quadrature)
  *   vl1_orbit_rows_host  orbit_arithmetic.py:566-634 (host twin of the
  *                        device compact-row function, for CPU-side checks)
  */

@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const Spl
                            : 0.0;
           }
           __syncthreads();
-          if (tact) {
+          if (tact && part < P) {  // threads past P * nt (nt not dividing 256) idle
             for (int j = part; j < nc; j += P) {
               double r2 = 0.0, da = 0.0;
 #pragma unroll
@@ -250,8 +250,9 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   prm.lds = a->n_rhs > 1 ? a->ld_strengths : 2 * n_src;
   prm.result = a->result;
   prm.ldr = a->n_rhs > 1 ? a->ld_result : 2 * a->n_points;
-  int nt = 1;
-  while (nt < a->max_box_points && nt < kSplitBlock) nt *= 2;
+  // a lane per target (the box's targets, at most a block), P = 256 / nt
+  // source parts: 36 targets use 7 parts = 252 of the 256 threads
+  const int nt = std::min(a->max_box_points, kSplitBlock);
   prm.nt = nt;
   const int P = kSplitBlock / nt;
   // the remainder is evaluated once per (target, source) pair and RC-chunk;

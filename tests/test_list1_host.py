@@ -104,3 +104,49 @@ def test_split_support_with_derivative_wrappers():
     assert split_wrapper_chain(helm) == (helm, _capi.SPLIT_DERIV_NONE, 0)
     assert split_wrapper_chain(AxisTargetDerivative(1, helm)) == (helm, _capi.SPLIT_DERIV_TARGET, 1)
     assert split_wrapper_chain(AxisSourceDerivative(0, helm)) == (helm, _capi.SPLIT_DERIV_SOURCE, 0)
+
+
+def test_split_order_options():
+    """helmholtz_split_order / _smooth_quad_order resolution
+    (expansion_wrangler_fpnd.py:2520-2568, 2649-2747): integers >= 1,
+    "auto" through the rho ladder (and the smooth order rule), clamped to 1
+    with several output kernels, refused above 1 under a derivative
+    wrapper; the smooth order defaults to q outside "auto"; order > 1 asks
+    for the density values (src_func) in the correction."""
+    from volumential_amd import tree as vtree
+    from volumential_amd.kernels import AxisTargetDerivative, HelmholtzKernel, LaplaceKernel
+    from volumential_amd.table import NearFieldInteractionTable
+    from volumential_amd.wrangler import NearFieldWrangler
+
+    tr = vtree.uniform_tree(2, 3, 2)  # coarsest active box h = 0.5
+    lap = NearFieldInteractionTable(3, 2, LaplaceKernel(2), source_box_extent=2.0)
+    helm = HelmholtzKernel(2, 7.0)
+
+    def make(knls=(helm,), **kw):
+        return NearFieldWrangler(tr, {repr(k): [lap] for k in knls}, target_kernels=list(knls),
+                                 **kw)
+
+    w = make(helmholtz_split_order=3)
+    assert (w.helmholtz_split_order, w.helmholtz_split_smooth_quad_order) == (3, 3)
+    w = make(helmholtz_split_order="auto")
+    assert w.helmholtz_split_order == 6                 # rho = 3.5 in (2, 4]
+    assert w.helmholtz_split_smooth_quad_order == 3 + 5 + 1  # + (m-1), + ceil(0.5 (3.5 - 3))
+    w = make(helmholtz_split_order=2, helmholtz_split_smooth_quad_order=5)
+    assert w.helmholtz_split_smooth_quad_order == 5
+    with pytest.raises(ValueError):
+        make(helmholtz_split_order=0)
+    two = (helm, HelmholtzKernel(2, 3.0))
+    assert make(two, helmholtz_split_order=4).helmholtz_split_order == 1
+    dk = AxisTargetDerivative(0, helm)
+    dt = NearFieldInteractionTable(3, 2, AxisTargetDerivative(0, LaplaceKernel(2)),
+                                   source_box_extent=2.0)
+    assert NearFieldWrangler(tr, {repr(dk): [dt]}, target_kernels=[dk]).helmholtz_split
+    with pytest.raises(NotImplementedError):
+        NearFieldWrangler(tr, {repr(dk): [dt]}, target_kernels=[dk], helmholtz_split_order=2)
+    w = make(helmholtz_split_order=2)
+    with pytest.raises(RuntimeError, match="src_func"):
+        w.eval_direct_helmholtz_split_correction(
+            tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists,
+            np.zeros(tr.n_points, complex))
+    # no split: the order is irrelevant (1)
+    assert make(helmholtz_split=False, helmholtz_split_order=5).helmholtz_split_order == 1

@@ -1858,21 +1858,21 @@ static int stage_launch(vl1_plan* pl, int form, int S, const double* f, int64_t 
     // column scale per RHS: the largest |f alpha| over all boxes (per-column
     // maxima, then one reduction per RHS)
     const int rows = pl->cplx ? 2 * pl->n : pl->n;
-    const int KW = rows <= 64 ? 1 : (rows <= 128 ? 2 : 4);
-    const int cm_boxes = KW == 4 ? 4 : 8;  // boxes per wavefront of i8_colmax_kernel<KW>
-    const int64_t cm_waves = (int64_t)((pl->nboxes + cm_boxes - 1) / cm_boxes) * R;
-    const dim3 cg((unsigned)((cm_waves + wpb - 1) / wpb)), cb(64 * wpb);
-#define VL1_I8_COLMAX(KV)                                                                        \
-  hipLaunchKernelGGL(i8_colmax_kernel<KV>, cg, cb, 0, s, pl->n, pl->cplx, pl->nboxes, R, f, ldf, \
-                     pl->box_starts.as<int32_t>(), pl->box_alpha.as<double>(), nmlz,            \
-                     pl->log_disp, st.colscale, st.nmdot)
-    if (KW == 1)
-      VL1_I8_COLMAX(1);
-    else if (KW == 2)
-      VL1_I8_COLMAX(2);
-    else
-      VL1_I8_COLMAX(4);
-#undef VL1_I8_COLMAX
+    if (rows <= 128) {
+      // small boxes: a lane per (box, RHS) (2D Helmholtz forest: 2.37 -> 1.3 ms)
+      const int64_t cm_lanes = ((int64_t)pl->nboxes + 63) / 64 * 64 * R;
+      hipLaunchKernelGGL(i8_colmax_lanes_kernel, dim3((unsigned)((cm_lanes + 255) / 256)),
+                         dim3(256), 0, s, pl->n, pl->cplx, pl->nboxes, R, f, ldf,
+                         pl->box_starts.as<int32_t>(), pl->box_alpha.as<double>(), nmlz,
+                         pl->log_disp, st.colscale, st.nmdot);
+    } else {
+      // large boxes: four boxes per wavefront, whole-wave rows
+      const int64_t cm_waves = (int64_t)((pl->nboxes + 3) / 4) * R;
+      hipLaunchKernelGGL(i8_colmax_kernel<4>, dim3((unsigned)((cm_waves + wpb - 1) / wpb)),
+                         dim3(64 * wpb), 0, s, pl->n, pl->cplx, pl->nboxes, R, f, ldf,
+                         pl->box_starts.as<int32_t>(), pl->box_alpha.as<double>(), nmlz,
+                         pl->log_disp, st.colscale, st.nmdot);
+    }
     // maxima over the boxes in two stages (G partial maxima per RHS, then one)
     const int G = std::max(1, std::min(256, pl->nboxes / 1024));
     if (G > 1)

@@ -122,6 +122,7 @@ __global__ void i8_split_rows_kernel(int MP, int KP, int n_blocks, int G,
 // 64 values (columns of up to 64 KW values; longer ones loop): all loads
 // issued before the first reduction.  A box that is no list-1 source has
 // alpha 0 (the plan rejects a zero pair scaling) and gets 0 without reading.
+// Boxes of at most 128 values per density take i8_colmax_lanes_kernel.
 template <int KW>
 __global__ void i8_colmax_kernel(int n, int cplx, int nboxes, int R, const double* __restrict__ f,
                                  int64_t ldf, const int32_t* __restrict__ box_starts,
@@ -178,6 +179,48 @@ __global__ void i8_colmax_kernel(int n, int cplx, int nboxes, int R, const doubl
       nmdot[(int64_t)box * R + r] = dot;
     }
   }
+}
+
+// Column maxima, one lane per (box, RHS): a lane streams its box's `rows`
+// values (16-byte loads where the box starts 16-byte aligned) and keeps the
+// maximum (and the 2D log term's dot with the normalisers) in registers --
+// no cross-lane reduction, every lane busy whatever the box size.  The lanes
+// of a wavefront take consecutive boxes of one RHS, so a load instruction's
+// addresses advance box by box and the lines each lane touches are reused
+// by its next loads from L1.
+__global__ void __launch_bounds__(256)
+    i8_colmax_lanes_kernel(int n, int cplx, int nboxes, int R, const double* __restrict__ f,
+                           int64_t ldf, const int32_t* __restrict__ box_starts,
+                           const double* __restrict__ box_alpha, const double* __restrict__ nmlz,
+                           int log_disp, double* __restrict__ colmax, double* __restrict__ nmdot) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nbp = ((int64_t)nboxes + 63) / 64 * 64;  // boxes padded to whole waves
+  if (gid >= nbp * R) return;
+  const int r = (int)(gid / nbp), box = (int)(gid % nbp);
+  if (box >= nboxes) return;
+  const double alpha = box_alpha[box];
+  double mx = 0.0, dot = 0.0;
+  if (alpha != 0.0) {
+    const int rows = cplx ? 2 * n : n;
+    const int64_t bs = box_starts[box];
+    const double* fb = f + (int64_t)r * ldf + (cplx ? 2 * bs : bs);
+    int q = 0;
+    if ((reinterpret_cast<uintptr_t>(fb) & 15) == 0) {
+#pragma unroll 8
+      for (; q + 1 < rows; q += 2) {
+        const double2 v = *reinterpret_cast<const double2*>(fb + q);
+        mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+        if (log_disp) dot = fma(nmlz[q], v.x, fma(nmlz[q + 1], v.y, dot));
+      }
+    }
+    for (; q < rows; ++q) {
+      const double v = fb[q];
+      mx = fmax(mx, fabs(v));
+      if (log_disp) dot = fma(nmlz[q], v, dot);
+    }
+  }
+  colmax[(int64_t)box * R + r] = mx * fabs(alpha);
+  nmdot[(int64_t)box * R + r] = dot;
 }
 
 // Two-stage maxima over many boxes: block (g, r) folds column r of boxes g,

@@ -710,6 +710,7 @@ class NearFieldWrangler:
         st = (st.unsqueeze(0) if st.dim() == 1 else st).contiguous()
         total = torch.zeros_like(f2)
         corr = self._correction(knl)
+        beta_terms = []  # (n, coefficient) of the 2D log terms from one table
         for kind, p, coeff in split_terms(self.tree.dim, base.k, self.helmholtz_split_order):
             key = (kind, int(p))
             plan = self._term_plan(knl, key, target_boxes, starts, lists)
@@ -717,24 +718,29 @@ class NearFieldWrangler:
             plan.eval(stacked[:bulk], out=tmp[:bulk], form="f64")
             if bulk < rows:
                 plan.eval(stacked[bulk:], out=tmp[bulk:], form="f64")
-            part = torch.complex(tmp[:r], tmp[r:])
+            total = total + complex(coeff) * torch.complex(tmp[:r], tmp[r:])
             if kind == "power_log" and len(self.split_term_tables(knl, key)) == 1:
-                n = p // 2
-                beta = self._log_alpha(self.split_term_tables(knl, key)[0], sources)
-                pc = corr
-                if n > corr.nmax:  # a series long enough to hold r^2n
-                    from .split import HelmholtzSplitCorrection
+                beta_terms.append((p // 2, complex(coeff), key))
+        if beta_terms:
+            # one P2P for all log terms: sum_n coeff_n r^2n over the strengths
+            # times log(h_s/H) (the same factor for every term: one table extent)
+            n_hi = max(n for n, _, _ in beta_terms)
+            pc = corr
+            if n_hi > corr.nmax:  # a series long enough to hold r^2n
+                from .split import HelmholtzSplitCorrection
 
-                    ck = ("beta", repr(knl), n)
-                    if ck not in self._corrections:
-                        self._corrections[ck] = HelmholtzSplitCorrection(
-                            self.tree, base.k, series_terms=n)
-                    pc = self._corrections[ck]
-                coefs = np.zeros(2 + 4 * pc.nmax)
-                coefs[2 + 4 * (n - 1) + 2] = 1.0  # b_n: r^2n, no log part
-                part = part + pc(st * beta[None, :], target_boxes, starts, lists,
-                                 coefficients=coefs, sources=sources)
-            total = total + complex(coeff) * part
+                ck = ("beta", repr(knl), n_hi)
+                if ck not in self._corrections:
+                    self._corrections[ck] = HelmholtzSplitCorrection(
+                        self.tree, base.k, series_terms=n_hi)
+                pc = self._corrections[ck]
+            coefs = np.zeros(2 + 4 * pc.nmax)
+            for n, c, _ in beta_terms:
+                coefs[2 + 4 * (n - 1) + 2] = c.real  # b_n: r^2n, no log part
+                coefs[2 + 4 * (n - 1) + 3] = c.imag
+            beta = self._log_alpha(self.split_term_tables(knl, beta_terms[0][2])[0], sources)
+            total = total + pc(st * beta[None, :], target_boxes, starts, lists,
+                               coefficients=coefs, sources=sources)
         return total[0] if one else total
 
     def _log_alpha(self, table, sources=None):

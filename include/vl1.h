@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 7
+#define VL1_ABI_VERSION 8
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -254,7 +254,8 @@ typedef struct vl1_split_opts {
 
 int vl1_plan_create(vl1_plan** out, const vl1_payload* p, const vl1_plan_desc* d, void* stream);
 
-/* Helmholtz split folded into the plan (split order 1): complex blocks per
+/* Helmholtz split folded into the plan (any split order, base or smooth
+ * quadrature of the remainder): complex blocks per
  * (case, source level) holding the Laplace table part (canonical scaling and
  * the 2D log term folded in) plus the Gauss quadrature of the remainder
  * G_k - G_0 on the source nodes -- the table part of eval_direct
@@ -283,6 +284,25 @@ typedef struct vl1_helmholtz_split_blocks {
   const int32_t* case_vecs;   /* [n_cases][dim] list-1 gallery vectors (host) */
   int deriv_kind;             /* vl1_split_deriv: derivative wrapper of the output kernel */
   int deriv_axis;             /* its axis (deriv_kind != VL1_SPLIT_DERIV_NONE) */
+  /* Split order m (0 or 1: order 1).  m > 1 (no derivative wrapper): the
+   * remainder without the tabulated terms, plus each term's table scaled by
+   * coefficient x (h/H)^(d + p) and, for a 2D log term, coefficient x
+   * log(h/H) x the quadrature of r^p (expansion_wrangler_fpnd.py:3560-3836,
+   * 4552-5118).  Term tables share the Laplace payload's compact layout. */
+  int split_order;
+  int n_terms;
+  int64_t n_term_entries;       /* compact entries per term table (= the payload's) */
+  const double* term_tables;    /* host [n_terms][n_term_entries] */
+  const int32_t* term_powers;   /* host [n_terms] p */
+  const int32_t* term_logs;     /* host [n_terms] 1: r^p log r, 0: r^p */
+  const double* term_coefs;     /* host [n_terms] complex (re, im) */
+  /* Smooth quadrature of the remainder (5116-5275): 0 = the base nodes;
+   * m_s > 0 = per source box the m_s^d Gauss nodes, with the source mode's
+   * Lagrange basis evaluated there (smooth_interp[j][i] = l_i(node_j)) */
+  int smooth_order;
+  const double* smooth_nodes;   /* host [smooth_order] on [0, 1] */
+  const double* smooth_weights; /* host [smooth_order], summing to 1 */
+  const double* smooth_interp;  /* host [smooth_order][q] */
 } vl1_helmholtz_split_blocks;
 
 int vl1_plan_create_helmholtz_split(vl1_plan** out, const vl1_payload* laplace,

@@ -9,9 +9,13 @@
   test_gpu_duffy pins) and the P2P oracle with the order-m series and the
   r^2n kernel -- to 1e-12 (rel L2 and Linf per RHS), orders 2 and 3, 2D
   and 3D.
+  Both routes: the two-launch one (table plans + point-to-point sums) and
+  the fused plan, whose complex blocks carry the order-m remainder, the
+  term tables and the log parts (FP64 form 1e-12, int8 split form 1e-10).
 * The smooth-quadrature path (``helmholtz_split_smooth_quad_order`` > q):
   the remainder over the density interpolated to finer per-box Gauss nodes
-  matches the oracle's restatement to 1e-12, orders 1 and 2.
+  matches the oracle's restatement to 1e-12, orders 1 and 2, both routes
+  (the fused int8 split form to 1e-10).
 * Physics: against list 1 evaluated with directly built Helmholtz tables,
   order 2 is closer than order 1 (the tabulated terms are the non-smooth
   ones the base quadrature integrates least well), and so is "auto".
@@ -88,12 +92,17 @@ def _box_of_point(tr):
     return out
 
 
+ROUTES = [(False, "f64", 1e-12), (True, "f64", 1e-12), (True, "split_i8", 1e-10)]
+
+
+@pytest.mark.parametrize("fused,form,tol", ROUTES)
 @pytest.mark.parametrize("dim,q,level,k,m,ms", [(2, 4, 3, 8.0, 1, 6), (2, 4, 3, 8.0, 2, 7),
                                                 (3, 3, 2, 6.0, 1, 5), (3, 3, 2, 6.0, 2, 4)])
-def test_smooth_quadrature_path_matches_oracle(cuda, dim, q, level, k, m, ms):
+def test_smooth_quadrature_path_matches_oracle(cuda, dim, q, level, k, m, ms, fused, form, tol):
     """helmholtz_split_smooth_quad_order > q: the remainder (and the 2D
     log(h/H) term) over the interpolated density on the finer nodes."""
-    print(f"case: smooth quadrature order {ms}, split order {m}, {dim}D q={q} k={k}", flush=True)
+    print(f"case: smooth quadrature order {ms}, split order {m}, {dim}D q={q} k={k}, "
+          f"fused={fused} {form}", flush=True)
     tr = vtree.uniform_tree(dim, q, level)
     knl = HelmholtzKernel(dim, k)
     lap = NearFieldInteractionTable(q, dim, LaplaceKernel(dim), source_box_extent=2.0)
@@ -103,29 +112,31 @@ def test_smooth_quadrature_path_matches_oracle(cuda, dim, q, level, k, m, ms):
                           helmholtz_split_smooth_quad_order=ms)
     f = np.stack([densities.helmholtz_beams(tr.points, 80 + r, k=k, width=0.3) for r in range(2)])
     args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
-    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"))[0][0]
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"), form=form,
+                              fused=fused)[0][0]
     ref = _oracle(tr, knl, lap, w, f, m, smooth=ms)
     l2, linf = rel_errors(got.cpu().numpy(), ref)
     print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
-    assert l2 <= 1e-12 and linf <= 1e-12
+    assert l2 <= tol and linf <= tol
 
 
+@pytest.mark.parametrize("fused,form,tol", ROUTES)
 @pytest.mark.parametrize("dim,q,level,k,m", [(3, 3, 2, 6.0, 2), (3, 3, 2, 6.0, 3),
                                              (2, 4, 3, 12.0, 2), (2, 4, 3, 12.0, 3)])
-def test_order_m_matches_oracle(cuda, dim, q, level, k, m):
-    print(f"case: split order {m}, {dim}D q={q} level {level} k={k}", flush=True)
+def test_order_m_matches_oracle(cuda, dim, q, level, k, m, fused, form, tol):
+    print(f"case: split order {m}, {dim}D q={q} level {level} k={k}, fused={fused} {form}",
+          flush=True)
     tr, knl, lap, w, f = _setup(dim, q, level, k, m)
     args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
-    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"), fused=False)[0][0]
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"), form=form,
+                              fused=fused)[0][0]
     ref = _oracle(tr, knl, lap, w, f, m)
     l2, linf = rel_errors(got.cpu().numpy(), ref)
     print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
-    assert l2 <= 1e-12 and linf <= 1e-12
-    # the fused route declines order m > 1 and takes the same one (the step
-    # order of phase-aligned tiles may differ: equal to rounding)
-    again = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"))[0][0]
-    l2, linf = rel_errors(again.cpu().numpy(), got.cpu().numpy())
-    assert l2 <= 1e-13 and linf <= 1e-13
+    assert l2 <= tol and linf <= tol
+    if fused:  # the fused plan was taken (no fall-back to two launches)
+        key = [k for k in w._plans if k[-1]]
+        assert key, "the fused split plan was not built"
 
 
 @pytest.mark.parametrize("dim,q,level,k", [(3, 4, 2, 10.0), (2, 4, 3, 8.0)])

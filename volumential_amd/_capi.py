@@ -194,6 +194,17 @@ class HelmholtzSplitBlocks(C.Structure):
         ("case_vecs", vp),
         ("deriv_kind", C.c_int),
         ("deriv_axis", C.c_int),
+        ("split_order", C.c_int),
+        ("n_terms", C.c_int),
+        ("n_term_entries", C.c_int64),
+        ("term_tables", vp),
+        ("term_powers", vp),
+        ("term_logs", vp),
+        ("term_coefs", vp),
+        ("smooth_order", C.c_int),
+        ("smooth_nodes", vp),
+        ("smooth_weights", vp),
+        ("smooth_interp", vp),
     ]
 
 
@@ -301,7 +312,7 @@ def lib():
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 7:
+    if L.vl1_abi_version() != 8:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

@@ -140,6 +140,102 @@ __global__ void expand_dense_blocks_kernel(int n, int n_cases, int cplx, int MP,
 // part (3457-3489), so one contraction evaluates both.  With an axis
 // derivative wrapper (deriv_axis >= 0, 3884-3957) T is the wrapped Laplace
 // table (signed entries, sp_mask) and R(|d|) becomes deriv_sign d_a R'(r)/r.
+// Non-table part of fused split blocks beyond split order 1 or with the
+// smooth quadrature, per (case-level block b, source mode s, target node t),
+// complex:
+//   sum over the quadrature nodes y_j of R_m(x_t - y_j) W_j L_s(y_j)
+//   + sum over the tabulated terms of coeff [ (h/H)^(d+p) T_p(case, s, t)
+//       + (2D log terms) log(h/H) sum_j |x_t - y_j|^p W_j L_s(y_j) ].
+// The nodes are the source box's own (ms = 0: y_j = its node s, L = 1, W =
+// its Gauss weight) or the ms^d smooth nodes with the Lagrange basis of
+// mode s there.  Target / source offsets are those of expand_split_blocks_kernel.
+constexpr int kSplitMaxBlockTerms = 16;
+
+template <int DIM>
+__global__ void split_extra_kernel(int q, int n, int n_blocks, const int32_t* __restrict__ blk_case,
+                                   const double* __restrict__ blk_h, double H,
+                                   const int32_t* __restrict__ case_vecs,
+                                   const uint8_t* __restrict__ perm, const int8_t* __restrict__ sign,
+                                   const uint8_t* __restrict__ group,
+                                   const uint16_t* __restrict__ ranks,
+                                   const int32_t* __restrict__ offs, int64_t n_entries,
+                                   const double* __restrict__ unodes,
+                                   const double* __restrict__ uweights,
+                                   const double* __restrict__ coef, int N, int ms,
+                                   const double* __restrict__ snodes,
+                                   const double* __restrict__ sweights,
+                                   const double* __restrict__ sinterp, int n_terms,
+                                   const double* __restrict__ term_tables,
+                                   const int32_t* __restrict__ term_powers,
+                                   const int32_t* __restrict__ term_logs,
+                                   const double* __restrict__ term_coefs,
+                                   double2* __restrict__ extra) {
+  constexpr int kMaxTerms = kSplitMaxBlockTerms;
+  const int64_t total = (int64_t)n_blocks * n * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int ss = (int)(i % n), tt = (int)((i / n) % n), b = (int)(i / ((int64_t)n * n));
+    const int c = blk_case[b];
+    const double h = blk_h[b];
+    const int32_t* vec = case_vecs + c * DIM;
+    int m = 0;
+    for (int a = 0; a < DIM; ++a) m = max(m, abs((int)vec[a]));
+    const double size = m == 0 ? 1.0 : 0.5 * (m - 2);
+    double xt[DIM];
+    int sa[DIM];
+    {
+      int sr = ss, tr = tt;
+      for (int a = DIM - 1; a >= 0; --a) {
+        sa[a] = sr % q;
+        xt[a] = 0.25 * vec[a] * h + size * h * (unodes[tr % q] - 0.5);
+        sr /= q;
+        tr /= q;
+      }
+    }
+    double rr = 0.0, ri = 0.0;           // remainder quadrature
+    double lq[kMaxTerms];                // quadrature of r^p for the log terms
+    for (int k = 0; k < n_terms && k < kMaxTerms; ++k) lq[k] = 0.0;
+    const int nodes = ms > 0 ? (DIM == 2 ? ms * ms : ms * ms * ms) : 1;
+    for (int j = 0; j < nodes; ++j) {
+      double r2 = 0.0, wl = 1.0;
+      int jr = j;
+      for (int a = DIM - 1; a >= 0; --a) {
+        double y, wa;
+        if (ms > 0) {
+          const int ja = jr % ms;
+          jr /= ms;
+          y = h * (snodes[ja] - 0.5);
+          wa = sweights[ja] * h * sinterp[ja * q + sa[a]];
+        } else {
+          y = h * (unodes[sa[a]] - 0.5);
+          wa = uweights[sa[a]] * h;
+        }
+        const double d = xt[a] - y;
+        r2 = fma(d, d, r2);
+        wl *= wa;
+      }
+      double vr, vi;
+      remainder_value<DIM>(r2, coef, N, vr, vi);
+      rr = fma(wl, vr, rr);
+      ri = fma(wl, vi, ri);
+      for (int k = 0; k < n_terms && k < kMaxTerms; ++k)
+        if (term_logs[k]) lq[k] = fma(wl, pow(r2, 0.5 * term_powers[k]), lq[k]);
+    }
+    if (n_terms > 0) {
+      const int64_t rel = compact_rel_row(DIM, q, perm + c * DIM, sign + c * DIM, group + c * DIM, ss, tt);
+      const int64_t at = offs[ranks[c]] + rel;
+      const double lg = log(h / H);
+      for (int k = 0; k < n_terms && k < kMaxTerms; ++k) {
+        double v = pow(h / H, (double)(DIM + term_powers[k])) * term_tables[(int64_t)k * n_entries + at];
+        if (term_logs[k]) v = fma(lg, lq[k], v);
+        rr = fma(term_coefs[2 * k], v, rr);
+        ri = fma(term_coefs[2 * k + 1], v, ri);
+      }
+    }
+    extra[i] = make_double2(rr, ri);
+  }
+}
+
 __global__ void expand_split_blocks_kernel(int dim, int q, int n, int sp_mask, int deriv_axis,
                                            double deriv_sign, int MP, int KP, int n_blocks,
                                            int n_rg, int rg_size,
@@ -158,6 +254,7 @@ __global__ void expand_split_blocks_kernel(int dim, int q, int n, int sp_mask, i
                                            const double* __restrict__ unodes,
                                            const double* __restrict__ uweights,
                                            const double* __restrict__ coef, int N,
+                                           const double2* __restrict__ extra,
                                            double* __restrict__ M) {
   const int64_t total = (int64_t)n_blocks * MP * KP;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -175,6 +272,14 @@ __global__ void expand_split_blocks_kernel(int dim, int q, int n, int sp_mask, i
       double tv = table[offs[ranks[c]] + rel];
       if (entry_sign(dim, q, perm + c * dim, sign + c * dim, sp_mask, ss, tt) < 0) tv = -tv;
       double re = blk_alpha[b] * tv + blk_disp[b] * nmlz[ss];
+      if (extra) {  // split order > 1 or smooth quadrature: split_extra_kernel
+        const double2 x = extra[((int64_t)b * n + tt) * n + ss];
+        const bool tre = t < n, sre = s < n;
+        re += x.x;
+        v = (tre == sre) ? re : (tre ? -x.y : x.y);
+        M[i] = v;
+        continue;
+      }
       const double h = blk_h[b];
       const int32_t* vec = case_vecs + c * dim;
       int m = 0;
@@ -1383,6 +1488,32 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
       return fail(VL1_E_ARG, "Helmholtz split plans: series_nmax must be 1..96");
     if (!sp->unit_nodes || !sp->unit_weights || !sp->case_vecs)
       return fail(VL1_E_ARG, "Helmholtz split plans: null host array");
+    const int order = sp->split_order < 1 ? 1 : sp->split_order;
+    if (2 * (order - 1) > kSplitMaxTerms)
+      return fail(VL1_E_ARG, "Helmholtz split plans: split_order must be 1..49");
+    if ((order > 1 || sp->smooth_order > 0) && sp->deriv_kind != VL1_SPLIT_DERIV_NONE)
+      return fail(VL1_E_UNSUPPORTED,
+                  "Helmholtz split plans: derivative wrappers take split order 1 and the base "
+                  "quadrature");
+    if (sp->n_terms != order - 1)
+      return fail(VL1_E_ARG, "Helmholtz split plans: n_terms must be split_order - 1");
+    if (sp->n_terms > kSplitMaxBlockTerms)
+      return fail(VL1_E_UNSUPPORTED, "Helmholtz split plans: at most " +
+                                         std::to_string(kSplitMaxBlockTerms) +
+                                         " tabulated terms (split order 17)");
+    if (sp->n_terms > 0 && (!sp->term_tables || !sp->term_powers || !sp->term_logs ||
+                            !sp->term_coefs || sp->n_term_entries != pay->n_entries))
+      return fail(VL1_E_ARG,
+                  "Helmholtz split plans: term tables must be given in the payload's compact "
+                  "layout (n_term_entries = its entry count)");
+    for (int k = 0; k < sp->n_terms; ++k)
+      if (sp->term_powers[k] < 0 || sp->term_powers[k] > 2 * kSplitMaxTerms ||
+          (sp->term_logs[k] != 0 && sp->term_logs[k] != 1) || (sp->term_logs[k] && dim != 2))
+        return fail(VL1_E_ARG, "Helmholtz split plans: bad term power or log flag");
+    if (sp->smooth_order < 0 || sp->smooth_order > 64 ||
+        (sp->smooth_order > 0 && (!sp->smooth_nodes || !sp->smooth_weights || !sp->smooth_interp)))
+      return fail(VL1_E_ARG, "Helmholtz split plans: smooth_order must be 0..64 with its nodes, "
+                             "weights and interpolation matrix");
   }
   auto pl = new vl1_plan();
   pl->payload = pay;
@@ -1616,10 +1747,11 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
       for (size_t k = 0; k < vecs.size(); ++k) vecs[k] = sp->case_vecs[k];
       const bool deriv = sp->deriv_kind != VL1_SPLIT_DERIV_NONE;
       const std::complex<double> kk(sp->k_re, sp->k_im);
+      const int order = sp->split_order < 1 ? 1 : sp->split_order;
       std::vector<double> coef = deriv ? split_derivative_coefficients(dim, kk, sp->series_nmax)
-                                       : split_coefficients(dim, kk, sp->series_nmax);
+                                       : split_coefficients(dim, kk, sp->series_nmax, order);
       if (coef.empty()) coef.assign(2, 0.0);  // 3D derivative with one term: R' = 0
-      DevBuf dal, ddi, dh, dvec, dun, duw, dco;
+      DevBuf dal, ddi, dh, dvec, dun, duw, dco, dext;
       e = upload(dal, pl->blk_alpha.data(), pl->blk_alpha.size() * 8, s);
       if (e == hipSuccess) e = upload(ddi, pl->blk_disp.data(), pl->blk_disp.size() * 8, s);
       if (e == hipSuccess) e = upload(dh, pl->blk_h.data(), pl->blk_h.size() * 8, s);
@@ -1627,6 +1759,39 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
       if (e == hipSuccess) e = upload(dun, sp->unit_nodes, (size_t)pay->q * 8, s);
       if (e == hipSuccess) e = upload(duw, sp->unit_weights, (size_t)pay->q * 8, s);
       if (e == hipSuccess) e = upload(dco, coef.data(), coef.size() * 8, s);
+      if (e == hipSuccess && (order > 1 || sp->smooth_order > 0)) {
+        // per (block, target, source) the remainder of order m, the terms'
+        // tables and the 2D log parts, complex
+        DevBuf dtt, dtp, dtl, dtc, dsn, dsw, dsi;
+        const int nt = sp->n_terms, ms = sp->smooth_order;
+        e = dext.alloc((size_t)pl->n_blocks * n * n * sizeof(double2));
+        if (e == hipSuccess && nt > 0) {
+          e = upload(dtt, sp->term_tables, (size_t)nt * pay->n_entries * 8, s);
+          if (e == hipSuccess) e = upload(dtp, sp->term_powers, (size_t)nt * 4, s);
+          if (e == hipSuccess) e = upload(dtl, sp->term_logs, (size_t)nt * 4, s);
+          if (e == hipSuccess) e = upload(dtc, sp->term_coefs, (size_t)nt * 16, s);
+        }
+        if (e == hipSuccess && ms > 0) {
+          e = upload(dsn, sp->smooth_nodes, (size_t)ms * 8, s);
+          if (e == hipSuccess) e = upload(dsw, sp->smooth_weights, (size_t)ms * 8, s);
+          if (e == hipSuccess) e = upload(dsi, sp->smooth_interp, (size_t)ms * pay->q * 8, s);
+        }
+        if (e == hipSuccess) {
+          const int64_t total = (int64_t)pl->n_blocks * n * n;
+          const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+          auto kern = dim == 2 ? split_extra_kernel<2> : split_extra_kernel<3>;
+          hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, pay->q, n, pl->n_blocks,
+                             dcase.as<int32_t>(), dh.as<double>(), H, dvec.as<int32_t>(),
+                             dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
+                             pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(), pay->n_entries,
+                             dun.as<double>(), duw.as<double>(), dco.as<double>(),
+                             sp->series_nmax, ms, dsn.as<double>(), dsw.as<double>(),
+                             dsi.as<double>(), nt, dtt.as<double>(), dtp.as<int32_t>(),
+                             dtl.as<int32_t>(), dtc.as<double>(), dext.as<double2>());
+          e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);  // before the uploads are freed
+      }
       if (e == hipSuccess) {
         const int64_t total = (int64_t)pl->n_blocks * pl->MP * pl->KP;
         const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
@@ -1638,7 +1803,8 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
                            dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
                            pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(),
                            pay->table.as<double>(), pay->nmlz.as<double>(), dun.as<double>(),
-                           duw.as<double>(), dco.as<double>(), sp->series_nmax, pl->M.as<double>());
+                           duw.as<double>(), dco.as<double>(), sp->series_nmax,
+                           dext.ptr ? dext.as<double2>() : nullptr, pl->M.as<double>());
         e = hipGetLastError();
       }
       if (e == hipSuccess) e = hipStreamSynchronize(s);  // before the uploads are freed

@@ -51,7 +51,13 @@ class List1Plan:
         densities to the split's full list-1 value.  Optional keys
         ``deriv_kind`` (``_capi.SPLIT_DERIV_*``) and ``deriv_axis`` describe
         an axis derivative wrapper of the output kernel (the payload then
-        holds the wrapped Laplace table).  ``level_scaling`` /
+        holds the wrapped Laplace table).  ``split_order`` (m > 1) with
+        ``terms``, a list of ``(compact_table, power, is_log, coefficient)``
+        -- the m - 1 tabulated terms, tables in the payload's compact layout
+        -- and ``smooth`` = ``(nodes, weights, interp)`` (the smooth
+        quadrature's per-axis rule on [0, 1] and the Lagrange basis of the
+        box nodes there, ``interp[j][i] = l_i(node_j)``) fold the order-m
+        split and the smooth quadrature into the blocks.  ``level_scaling`` /
         ``level_disp`` (``_capi.MAX_LEVELS`` values each): the per-level
         scaling and displacement factor of the custom policy
         (``scaling_mode=_capi.SCALING_CUSTOM``)."""
@@ -121,6 +127,8 @@ class List1Plan:
             k["split_nodes"] = np.ascontiguousarray(xi, np.float64)
             k["split_weights"] = np.ascontiguousarray(wi, np.float64)
             k["case_vecs"] = np.ascontiguousarray(np.asarray(vecs, np.int32))
+            terms = list(helmholtz_split.get("terms", ()))
+            smooth = helmholtz_split.get("smooth")
             sp = _capi.HelmholtzSplitBlocks(
                 k_re=sk.real, k_im=sk.imag, series_nmax=int(helmholtz_split["series_nmax"]),
                 unit_nodes=k["split_nodes"].ctypes.data,
@@ -128,7 +136,34 @@ class List1Plan:
                 case_vecs=k["case_vecs"].ctypes.data,
                 deriv_kind=int(helmholtz_split.get("deriv_kind", _capi.SPLIT_DERIV_NONE)),
                 deriv_axis=int(helmholtz_split.get("deriv_axis", 0)),
+                split_order=int(helmholtz_split.get("split_order", 1)),
+                n_terms=len(terms),
             )
+            if terms:
+                k["term_tables"] = np.ascontiguousarray(
+                    np.stack([np.asarray(t[0], np.float64).ravel() for t in terms]))
+                k["term_powers"] = np.ascontiguousarray([int(t[1]) for t in terms], np.int32)
+                k["term_logs"] = np.ascontiguousarray([int(bool(t[2])) for t in terms], np.int32)
+                k["term_coefs"] = np.ascontiguousarray(
+                    [[complex(t[3]).real, complex(t[3]).imag] for t in terms], np.float64)
+                sp.n_term_entries = int(k["term_tables"].shape[1])
+                sp.term_tables = k["term_tables"].ctypes.data
+                sp.term_powers = k["term_powers"].ctypes.data
+                sp.term_logs = k["term_logs"].ctypes.data
+                sp.term_coefs = k["term_coefs"].ctypes.data
+            if smooth is not None:
+                nodes, weights, interp = smooth
+                k["smooth_nodes"] = np.ascontiguousarray(nodes, np.float64)
+                k["smooth_weights"] = np.ascontiguousarray(weights, np.float64)
+                k["smooth_interp"] = np.ascontiguousarray(interp, np.float64)
+                ms = k["smooth_nodes"].size
+                if (k["smooth_weights"].shape != (ms,)
+                        or k["smooth_interp"].shape != (ms, payload.quad_order)):
+                    raise ValueError("smooth quadrature: nodes, weights and interp disagree")
+                sp.smooth_order = ms
+                sp.smooth_nodes = k["smooth_nodes"].ctypes.data
+                sp.smooth_weights = k["smooth_weights"].ctypes.data
+                sp.smooth_interp = k["smooth_interp"].ctypes.data
             _capi.check(
                 L.vl1_plan_create_helmholtz_split(_capi.C.byref(handle), payload.handle, desc, sp,
                                                   _capi.stream_handle(stream)),

@@ -479,7 +479,27 @@ class NearFieldWrangler:
         if fused_split:
             corr = self._correction(out_kernel)
             split_desc = {"k": corr.k, "series_nmax": corr.nmax,
-                          "deriv_kind": corr.deriv_kind, "deriv_axis": corr.deriv_axis}
+                          "deriv_kind": corr.deriv_kind, "deriv_axis": corr.deriv_axis,
+                          "split_order": self.helmholtz_split_order}
+            if self.helmholtz_split_order > 1:
+                from .split import split_terms
+
+                base, _, _ = split_wrapper_chain(out_kernel)
+                terms = []
+                for kind, p, coeff in split_terms(self.tree.dim, base.k,
+                                                  self.helmholtz_split_order):
+                    tabs = self.split_term_tables(out_kernel, (kind, int(p)))
+                    tdata = prepare_table_data_and_entry_map(tabs)[0]
+                    terms.append((np.asarray(tdata[0]), int(p), kind == "power_log", coeff))
+                split_desc["terms"] = terms
+            ms = self.helmholtz_split_smooth_quad_order or 0
+            if ms > int(self.tree.quad_order):
+                from .quadrature import gauss_legendre_01
+                from .split import interp_matrix
+
+                xq, _ = gauss_legendre_01(int(self.tree.quad_order))
+                xm, wm = gauss_legendre_01(ms)
+                split_desc["smooth"] = (xm, wm, interp_matrix(xq, xm))
         plan = List1Plan(
             self.tree, payload,
             scaling_mode=nf._mode_code(pol),
@@ -773,8 +793,8 @@ class NearFieldWrangler:
         contraction (``form`` "f64" or "split") where the plan takes it (2D:
         node ids in mode order); otherwise, or with ``fused=False``, the
         table part (plan ``form``) plus the point-to-point correction (and, at
-        split order m > 1, the tabulated terms: the fused plan is order 1 on
-        the base quadrature)."""
+        split order m > 1, the tabulated terms).  The fused plan folds split
+        orders up to 17 and the smooth quadrature into its blocks."""
         if not self.helmholtz_split:
             raise RuntimeError("eval_direct_split needs helmholtz_split enabled")
         mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
@@ -782,8 +802,7 @@ class NearFieldWrangler:
         mc = mc.to(device="cuda", dtype=torch.complex128)
         one = mc.dim() == 1
         mc2 = (mc.unsqueeze(0) if one else mc).contiguous()
-        base_quad = (self.helmholtz_split_smooth_quad_order or 0) <= int(self.tree.quad_order)
-        if fused and self.helmholtz_split_order == 1 and base_quad and all(
+        if fused and self.helmholtz_split_order <= 17 and all(
                 len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels):
             # one contraction: complex blocks = table part + remainder quadrature
             # a split-form limit (slices vs the LDS budget) lets the fused

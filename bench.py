@@ -848,6 +848,56 @@ def run_c1(args):
         forms, extra)
 
 
+def _hsplit_auto(args, tr, knl, lap, fd, args_csr):
+    """``helmholtz_split_order="auto"`` on the same tree and densities: the
+    order from |k| h and the smooth quadrature order by the reference's rule.
+    Whole ``eval_direct_split`` calls (host work included, synchronized),
+    the fused plan (one contraction: remainder, term tables and log parts in
+    its blocks) against the two-launch route (table plans per term + the
+    point-to-point sums); error of each fused form against the two-launch
+    result."""
+    import torch
+
+    from volumential_amd.wrangler import NearFieldWrangler
+
+    w = NearFieldWrangler(tr, {repr(knl): [lap]}, target_kernels=[knl],
+                          helmholtz_split_order="auto")
+    t0 = time.perf_counter()
+    ref = w.eval_direct_split(*args_csr, fd, fused=False)[0][0]
+    torch.cuda.synchronize()
+    setup_two = time.perf_counter() - t0
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t1) / args.steps * 1e3
+
+    out = {"split_order": w.helmholtz_split_order,
+           "smooth_quad_order": w.helmholtz_split_smooth_quad_order,
+           "two_launch_ms": timed(lambda: w.eval_direct_split(*args_csr, fd, fused=False)),
+           "two_launch_first_call_s": setup_two}
+    for form in ("f64", "split_i8"):
+        t0 = time.perf_counter()
+        got = w.eval_direct_split(*args_csr, fd, form=form)[0][0]
+        torch.cuda.synchronize()
+        if form == "f64":
+            out["fused_plan_build_s"] = time.perf_counter() - t0
+        d = (got - ref).abs().pow(2).sum(dim=1).sqrt() / ref.abs().pow(2).sum(dim=1).sqrt()
+        out[f"fused_{form}_ms"] = timed(lambda: w.eval_direct_split(*args_csr, fd, form=form))
+        out[f"fused_{form}_rel_l2_vs_two_launch"] = float(d.max())
+    out["speedup_fused_split_i8_vs_two_launch"] = out["two_launch_ms"] / out["fused_split_i8_ms"]
+    log(f"auto split order {out['split_order']}, smooth order {out['smooth_quad_order']}: "
+        f"two launches {out['two_launch_ms']:.3f} ms, fused f64 {out['fused_f64_ms']:.3f} ms "
+        f"({out['fused_f64_rel_l2_vs_two_launch']:.2e}), fused int8 "
+        f"{out['fused_split_i8_ms']:.3f} ms ({out['fused_split_i8_rel_l2_vs_two_launch']:.2e})")
+    return out
+
+
 def run_hsplit(args):
     """2D Helmholtz k=40 on [-1,1]^2 with the reference's default split:
     uniform level 7 (16,384 leaves), q=6 (589,824 points), 16 complex
@@ -911,6 +961,7 @@ def run_hsplit(args):
     log(f"fused f64 {forms['f64']['ms']:.3f} ms, fused split {forms['split']['ms']:.3f} ms, "
         f"two launches {two_ms:.3f} ms (P2P kernel {p2p_ms:.3f} ms); "
         f"rel L2 vs two-launch: f64 {fused['f64']:.2e}, split {fused['split']:.2e}")
+    auto = _hsplit_auto(args, tr, knl, lap, fd, args_csr)
     # P2P roofline: FP64 VALU, declared 8 N + 24 flop per (target, source)
     # point pair for the Horner series and log, plus 8 per complex MAC
     pt_pairs = pairs * n * n
@@ -924,7 +975,8 @@ def run_hsplit(args):
              "p2p_fp64_frac": p2p_flops / (p2p_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
              "series_terms": corr.nmax,
              "rel_l2_vs_two_launch": fused,
-             "speedup_fused_vs_two_launch": two_ms / forms[best]["ms"]}
+             "speedup_fused_vs_two_launch": two_ms / forms[best]["ms"],
+             "auto_split_order": auto}
     if not args.no_cpu_baseline:
         # oracle on a sample of target boxes: C/OpenMP table part (Re, Im) +
         # numpy remainder sum; scaled to points x RHS per second

@@ -27,7 +27,7 @@ import torch
 
 from oracle import list1_ref, orbit_ref
 from oracle import split_ref as S
-from tests.gpu_helpers import rel_errors
+from tests.gpu_helpers import adaptive, rel_errors
 from volumential_amd import densities, tree as vtree
 from volumential_amd.kernels import HelmholtzKernel, LaplaceKernel
 from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
@@ -137,6 +137,33 @@ def test_order_m_matches_oracle(cuda, dim, q, level, k, m, fused, form, tol):
     if fused:  # the fused plan was taken (no fall-back to two launches)
         key = [k for k in w._plans if k[-1]]
         assert key, "the fused split plan was not built"
+
+
+@pytest.mark.parametrize("fused,form,tol", ROUTES)
+@pytest.mark.parametrize("dim,q,max_level,k,m,ms", [(2, 4, 4, 8.0, 3, None), (2, 4, 4, 8.0, 2, 6),
+                                                    (3, 3, 4, 5.0, 2, None), (3, 3, 4, 5.0, 2, 5)])
+def test_adaptive_trees_match_oracle(cuda, dim, q, max_level, k, m, ms, fused, form, tol):
+    """Source boxes on several levels: the term scaling (h/H)^(d+p), the 2D
+    log(h/H) factor and the remainder geometry change with the source
+    level (the fused plan's blocks are per (case, source level))."""
+    print(f"case: adaptive {dim}D q={q} max level {max_level} k={k}, split order {m}, "
+          f"smooth {ms}, fused={fused} {form}", flush=True)
+    tr = adaptive(dim, q, max_level=max_level)
+    assert len(np.unique(np.asarray(tr.box_levels)[tr.neighbor_source_boxes_lists])) >= 3
+    knl = HelmholtzKernel(dim, k)
+    lap = NearFieldInteractionTable(q, dim, LaplaceKernel(dim), source_box_extent=2.0)
+    lap.build_table(RULE)
+    w = NearFieldWrangler(tr, {repr(knl): [lap]}, target_kernels=[knl], helmholtz_split_order=m,
+                          helmholtz_split_term_build_config=RULE,
+                          helmholtz_split_smooth_quad_order=ms)
+    f = np.stack([densities.helmholtz_beams(tr.points, 60 + r, k=k, width=0.3) for r in range(3)])
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"), form=form,
+                              fused=fused)[0][0]
+    ref = _oracle(tr, knl, lap, w, f, m, smooth=ms)
+    l2, linf = rel_errors(got.cpu().numpy(), ref)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= tol and linf <= tol
 
 
 @pytest.mark.parametrize("dim,q,level,k", [(3, 4, 2, 10.0), (2, 4, 3, 8.0)])

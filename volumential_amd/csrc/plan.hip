@@ -1224,6 +1224,10 @@ struct vl1_plan {
   // and per source box the pair scaling alpha and the 2D log-term factor
   std::map<int, DevBuf*> A8_by_slices;
   DevBuf rho, box_alpha, box_disp;
+  // streaming column maxima (no 2D log term): per point the index of its
+  // source box's scaling in alpha_lut (255: no list-1 source)
+  DevBuf pt_class, alpha_lut;
+  bool stream_colmax = false;
   std::map<int, TileSchedule*> schedules;  // by 2 Rc + (Morton-only order)
   std::map<int, int> tuned;                 // variant by (form, slices, Rc), from vl1_plan_tune
   int forced_variant = -1;                  // candidate under test in vl1_plan_tune
@@ -1725,7 +1729,46 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
     bdisp[pl->pair_src[k]] = pl->pair_disp[k];
   }
 
+  // per point the class of its source box's scaling, for the streaming
+  // column maxima: distinct scalings (bitwise) get classes 0..254; more of
+  // them, points shared by two source boxes or the 2D log term (whose dots
+  // the per-box pass computes) keep the per-box pass
+  std::vector<uint8_t> pcls;
+  std::vector<double> lut(256, 0.0);
+  bool stream_ok = !pl->log_disp;
+  if (stream_ok) {
+    pcls.assign((size_t)d->n_points, 255);
+    int n_lut = 0;
+    for (int b = 0; b < d->n_boxes && stream_ok; ++b) {
+      if (balpha[b] == 0.0) continue;  // no list-1 source
+      int idx = -1;
+      for (int k = 0; k < n_lut; ++k)
+        if (std::memcmp(&lut[k], &balpha[b], sizeof(double)) == 0) idx = k;
+      if (idx < 0) {
+        if (n_lut == 255) {
+          stream_ok = false;
+          break;
+        }
+        idx = n_lut;
+        lut[n_lut++] = balpha[b];
+      }
+      for (int k = 0; k < n; ++k) {
+        const int64_t pt = (int64_t)d->box_starts[b] + k;  // in range: checked above
+        if (pcls[pt] != 255) {
+          stream_ok = false;
+          break;
+        }
+        pcls[pt] = (uint8_t)idx;
+      }
+    }
+  }
+
   hipError_t e = upload(pl->mode_point, mode_point.data(), mode_point.size() * 4, s);
+  if (e == hipSuccess && stream_ok) {
+    e = upload(pl->pt_class, pcls.data(), pcls.size(), s);
+    if (e == hipSuccess) e = upload(pl->alpha_lut, lut.data(), lut.size() * 8, s);
+    pl->stream_colmax = e == hipSuccess;
+  }
   if (e == hipSuccess) {
     const int32_t zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     e = upload(pl->phase, zeros, sizeof(zeros), s);
@@ -2024,7 +2067,18 @@ static int stage_launch(vl1_plan* pl, int form, int S, const double* f, int64_t 
     // column scale per RHS: the largest |f alpha| over all boxes (per-column
     // maxima, then one reduction per RHS)
     const int rows = pl->cplx ? 2 * pl->n : pl->n;
-    if (rows <= 128) {
+    int n_partial = pl->nboxes;  // maxima to fold: per box, or per value chunk
+    const int64_t n_values = pl->n_points * (pl->cplx ? 2 : 1);
+    const int64_t n_chunks = (n_values + kI8StreamChunk - 1) / kI8StreamChunk;
+    if (pl->stream_colmax && n_chunks <= pl->nboxes) {
+      // flat streams of the density rows (partials per 2048-value chunk fit
+      // the per-box maxima's space)
+      n_partial = (int)n_chunks;
+      hipLaunchKernelGGL(i8_colmax_stream_kernel,
+                         dim3((unsigned)n_chunks, (unsigned)((R + kI8StreamRows - 1) / kI8StreamRows)),
+                         dim3(256), 0, s, n_values, pl->cplx ? 1 : 0, R, f, ldf,
+                         pl->pt_class.as<uint8_t>(), pl->alpha_lut.as<double>(), st.colscale);
+    } else if (rows <= 128) {
       // small boxes: a lane per (box, RHS) (2D Helmholtz forest: 2.37 -> 1.3 ms)
       const int64_t cm_lanes = ((int64_t)pl->nboxes + 63) / 64 * 64 * R;
       hipLaunchKernelGGL(i8_colmax_lanes_kernel, dim3((unsigned)((cm_lanes + 255) / 256)),
@@ -2040,11 +2094,11 @@ static int stage_launch(vl1_plan* pl, int form, int S, const double* f, int64_t 
                          pl->log_disp, st.colscale, st.nmdot);
     }
     // maxima over the boxes in two stages (G partial maxima per RHS, then one)
-    const int G = std::max(1, std::min(256, pl->nboxes / 1024));
+    const int G = std::max(1, std::min(256, n_partial / 1024));
     if (G > 1)
-      hipLaunchKernelGGL(i8_colmax_fold_kernel, dim3(G, R), dim3(256), 0, s, pl->nboxes, R,
+      hipLaunchKernelGGL(i8_colmax_fold_kernel, dim3(G, R), dim3(256), 0, s, n_partial, R,
                          st.colscale);
-    hipLaunchKernelGGL(i8_sigma_kernel, dim3(R), dim3(256), 0, s, G > 1 ? G : pl->nboxes, R,
+    hipLaunchKernelGGL(i8_sigma_kernel, dim3(R), dim3(256), 0, s, G > 1 ? G : n_partial, R,
                        st.colscale, st.sigma);
     const int RR = i8_cols_round(rows, R);
     const int tasks = RR * (pl->KP / 32) * 2;  // 16 K values of one column each

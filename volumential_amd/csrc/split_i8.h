@@ -223,6 +223,61 @@ __global__ void __launch_bounds__(256)
   nmdot[(int64_t)box * R + r] = dot;
 }
 
+// Density staging, pass 1, streaming form (no 2D log term): partial[g][r] =
+// the largest |f[r][v]| alpha(v) over the kI8StreamChunk values of chunk g,
+// alpha(v) = alpha_lut[pt_class[v >> cshift]] (class 255: a point of no
+// list-1 source box, alpha 0; cshift 1 for interleaved complex values).  The
+// per-RHS scale needs only these maxima, so the rows are read as flat
+// streams -- each load instruction covers 512 consecutive bytes, with no
+// per-box structure -- kI8StreamRows RHS rows per block.  Same products
+// |f| alpha as the per-box form, so the same maxima.
+constexpr int kI8StreamChunk = 2048;  // values per block: 256 threads x 8
+constexpr int kI8StreamRows = 8;      // RHS rows per block
+
+__global__ void __launch_bounds__(256)
+    i8_colmax_stream_kernel(int64_t n_values, int cshift, int R, const double* __restrict__ f,
+                            int64_t ldf, const uint8_t* __restrict__ pt_class,
+                            const double* __restrict__ alpha_lut, double* __restrict__ partial) {
+  __shared__ double red[kI8StreamRows][4];
+  constexpr int PER = kI8StreamChunk / 256;
+  const int64_t v0 = (int64_t)blockIdx.x * kI8StreamChunk + threadIdx.x;
+  const int r0 = blockIdx.y * kI8StreamRows;
+  const int nr = R - r0 < kI8StreamRows ? R - r0 : kI8StreamRows;
+  double a[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int64_t v = v0 + 256 * k;
+    a[k] = v < n_values ? alpha_lut[pt_class[v >> cshift]] : 0.0;
+  }
+  double mx[kI8StreamRows];
+#pragma unroll
+  for (int rr = 0; rr < kI8StreamRows; ++rr) {
+    mx[rr] = 0.0;
+    if (rr < nr) {
+      const double* row = f + (int64_t)(r0 + rr) * ldf;
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int64_t v = v0 + 256 * k;
+        if (v < n_values) mx[rr] = fmax(mx[rr], fabs(row[v]) * a[k]);
+      }
+    }
+  }
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+#pragma unroll
+  for (int rr = 0; rr < kI8StreamRows; ++rr) {
+    double m = mx[rr];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if (lane == 0) red[rr][wave] = m;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nr) {
+    const int rr = threadIdx.x;
+    partial[(int64_t)blockIdx.x * R + r0 + rr] =
+        fmax(fmax(red[rr][0], red[rr][1]), fmax(red[rr][2], red[rr][3]));
+  }
+}
+
 // Two-stage maxima over many boxes: block (g, r) folds column r of boxes g,
 // g + G, g + 2G, ... into colmax[g][r] (in place: no other block touches
 // those entries); then i8_sigma_kernel over the first G boxes.

@@ -166,6 +166,57 @@ def test_adaptive_trees_match_oracle(cuda, dim, q, max_level, k, m, ms, fused, f
     assert l2 <= tol and linf <= tol
 
 
+@pytest.mark.parametrize("tree_kind,m", [("uniform", 2), ("uniform", 3), ("adaptive", 3)])
+def test_table_beta_mode_matches_oracle(cuda, tree_kind, m):
+    """power_log_single_table_beta_mode="table" (3757-3779): the log(h/H)
+    part of each 2D r^2n log r term is the r^2n term table on the density
+    times log(h_s/H), here against the oracle's custom-scaling list 1 over
+    the same r^2n table (1e-12).  At q = 4 the base quadrature integrates
+    r^2n times a basis polynomial exactly for n <= 2, so the default "p2p"
+    mode agrees with it to the tables' build accuracy (1e-10); the fused
+    plan, which sums the p2p form, is not taken."""
+    dim, q, k = 2, 4, 8.0
+    print(f"case: table beta mode, 2D q={q} {tree_kind} k={k}, split order {m}", flush=True)
+    tr = vtree.uniform_tree(dim, q, 3) if tree_kind == "uniform" else adaptive(dim, q)
+    knl = HelmholtzKernel(dim, k)
+    lap = NearFieldInteractionTable(q, dim, LaplaceKernel(dim), source_box_extent=2.0)
+    lap.build_table(RULE)
+    kw = dict(target_kernels=[knl], helmholtz_split_order=m,
+              helmholtz_split_term_build_config=RULE)
+    w = NearFieldWrangler(tr, {repr(knl): [lap]}, helmholtz_split_auto_config={
+        "power_log_single_table_beta_mode": "table"}, **kw)
+    f = np.stack([densities.helmholtz_beams(tr.points, 40 + r, k=k, width=0.3) for r in range(2)])
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"))[0][0].cpu().numpy()
+    assert not any(key[-1] for key in w._plans), "the fused plan sums the p2p form"
+    # oracle: the p2p oracle minus its log(h/H) P2P parts plus the table forms
+    meta = orbit_ref.case_metadata(dim, q)
+    kwo = dict(table_root_extent=2.0, own_csr=False)
+    tot = _oracle(tr, knl, lap, w, f, m)
+    wts = np.asarray(tr.weights)[None, :]
+    corr = w._correction(knl)
+    h = tr.root_extent * 0.5 ** np.arange(32)
+    per_box = np.log(tr.root_extent * 0.5 ** np.asarray(tr.box_levels, float) / 2.0)
+    beta = per_box[_box_of_point(tr)]
+    for kind, p, c in S.extracted_terms(dim, knl.k, m):
+        tot = tot - c * S.correction(tr, f * wts * beta[None, :], knl.k, corr.nmax,
+                                     kernel=lambda r, p=p: r ** p)
+        pt = w.split_term_tables(knl, ("power", p))[0]
+        part = [list1_ref.near_field_from_tree(
+            tr, pt.compact_values()[None], pt.mode_normalizers[None], meta, comp, mode="custom",
+            level_scaling=(h / 2.0) ** (dim + p) * np.log(h / 2.0), level_disp=np.zeros(32),
+            **kwo) for comp in (f.real, f.imag)]
+        tot = tot + c * (part[0] + 1j * part[1])
+    l2, linf = rel_errors(got, tot)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= 1e-12 and linf <= 1e-12
+    wp = NearFieldWrangler(tr, {repr(knl): [lap]}, **kw)
+    p2p = wp.eval_direct_split(*args, torch.as_tensor(f, device="cuda"), fused=False)[0][0]
+    l2, linf = rel_errors(got, p2p.cpu().numpy())
+    print(f"vs the p2p mode: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= 1e-10 and linf <= 1e-10
+
+
 @pytest.mark.parametrize("dim,q,level,k", [(3, 4, 2, 10.0), (2, 4, 3, 8.0)])
 def test_order_2_closer_to_helmholtz_tables(cuda, dim, q, level, k):
     print(f"case: split orders 1-3 vs Helmholtz tables, {dim}D q={q} level {level} k={k}",

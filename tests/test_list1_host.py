@@ -150,3 +150,11 @@ def test_split_order_options():
             np.zeros(tr.n_points, complex))
     # no split: the order is irrelevant (1)
     assert make(helmholtz_split=False, helmholtz_split_order=5).helmholtz_split_order == 1
+    # the 2D log term's beta mode (3709-3786): "p2p" by default, "table", else ValueError
+    assert make(helmholtz_split_order=2).helmholtz_split_beta_mode == "p2p"
+    cfg = {"power_log_single_table_beta_mode": " Table "}
+    assert make(helmholtz_split_order=2,
+                helmholtz_split_auto_config=cfg).helmholtz_split_beta_mode == "table"
+    with pytest.raises(ValueError, match="beta_mode"):
+        make(helmholtz_split_order=2,
+             helmholtz_split_auto_config={"power_log_single_table_beta_mode": "quad"})

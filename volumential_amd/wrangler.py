@@ -116,7 +116,13 @@ class NearFieldWrangler:
         tensor Gauss order of the online remainder's sources; above q the
         correction runs on those finer per-box nodes with the density
         interpolated to them (given ``src_func``, 3562-3597).  Default: q, or
-        in ``"auto"`` mode the reference's rule (``split.auto_smooth_quad_order``)."""
+        in ``"auto"`` mode the reference's rule (``split.auto_smooth_quad_order``).
+
+        ``helmholtz_split_auto_config["power_log_single_table_beta_mode"]``
+        (2297-2301, 3709-3786): how the log(h/H) part of a 2D r^2n log r term
+        from one table is summed -- ``"p2p"`` (default: a point-to-point sum
+        of r^2n over the strengths times log(h_s/H)) or ``"table"`` (the r^2n
+        term table applied to the density times log(h_s/H))."""
         if not getattr(tree, "sources_are_targets", False):
             raise ValueError(
                 "table-based near-field evaluation requires sources and targets "
@@ -173,6 +179,11 @@ class NearFieldWrangler:
                 raise ValueError(f"unknown split term kind {kind!r}")
             self._term_tables[(kind, int(p))] = self._as_list(tabs)
         self.helmholtz_split_order = 1
+        self.helmholtz_split_beta_mode = str(
+            dict(helmholtz_split_auto_config or {}).get("power_log_single_table_beta_mode",
+                                                        "p2p")).strip().lower()
+        if self.helmholtz_split_beta_mode not in ("p2p", "table"):
+            raise ValueError("power_log_single_table_beta_mode must be 'table' or 'p2p'")
         self.helmholtz_split = False
         self.helmholtz_split_smooth_quad_order = None
         self._smooth_sources = {}
@@ -670,14 +681,17 @@ class NearFieldWrangler:
             raise RuntimeError(f"split term tables {key} do not match the Laplace table levels")
         return tabs
 
-    def _term_plan(self, knl, key, target_boxes, starts, lists):
+    def _term_plan(self, knl, key, target_boxes, starts, lists, log_factor=False):
         """A plan over term tables ``key``: one table scaled by
         (h/H)^(d + p) (``_helmholtz_split_term_scaling_code``, 4777-4798),
-        or per-level tables unscaled."""
+        or per-level tables unscaled.  ``log_factor``: one table scaled by
+        (h/H)^(d + p) log(h/H) -- the "table" form of a 2D log term's
+        log(h/H) part (3757-3779), the factor of the source box's level."""
         from .payload import DevicePayload
         from .plan import List1Plan
 
-        ck = ("term",) + tuple(key) + _csr_key(knl, target_boxes, starts, lists)[1:]
+        ck = (("term", bool(log_factor)) + tuple(key)
+              + _csr_key(knl, target_boxes, starts, lists)[1:])
         if ck in self._term_plans:
             self._term_plans.move_to_end(ck)
             return self._term_plans[ck]
@@ -691,9 +705,13 @@ class NearFieldWrangler:
             e = self.tree.dim + int(key[1])
             lev = np.arange(_capi.MAX_LEVELS)
             h = float(self.tree.root_extent) * 0.5 ** lev
-            kw = dict(scaling_mode=_capi.SCALING_CUSTOM,
-                      level_scaling=(h / t0.source_box_extent) ** e,
+            scal = (h / t0.source_box_extent) ** e
+            if log_factor:
+                scal = scal * np.log(h / t0.source_box_extent)
+            kw = dict(scaling_mode=_capi.SCALING_CUSTOM, level_scaling=scal,
                       level_disp=np.zeros(_capi.MAX_LEVELS))
+        elif log_factor:
+            raise RuntimeError("the log(h/H) factor applies to single-table terms only")
         else:
             kw = dict(scaling_mode=_capi.SCALING_PER_LEVEL)
         plan = List1Plan(
@@ -712,7 +730,9 @@ class NearFieldWrangler:
         """sum over the split's terms of coefficient x (term table on
         ``src_func``), with, for a 2D log term from one table, its
         log(h/H) part: the point-to-point sum of r^2n over the strengths
-        times log(h_s/H) of each source's box (``beta``, 3689-3800)."""
+        times log(h_s/H) of each source's box (``beta``, 3689-3800), or in
+        the "table" beta mode the r^2n term table on ``src_func`` times
+        log(h_s/H) (3757-3779)."""
         from .split import split_terms
 
         base, _, _ = split_wrapper_chain(knl)
@@ -740,7 +760,16 @@ class NearFieldWrangler:
                 plan.eval(stacked[bulk:], out=tmp[bulk:], form="f64")
             total = total + complex(coeff) * torch.complex(tmp[:r], tmp[r:])
             if kind == "power_log" and len(self.split_term_tables(knl, key)) == 1:
-                beta_terms.append((p // 2, complex(coeff), key))
+                if self.helmholtz_split_beta_mode == "table":
+                    bplan = self._term_plan(knl, ("power", int(p)), target_boxes, starts, lists,
+                                            log_factor=True)
+                    tmp = torch.zeros_like(stacked)
+                    bplan.eval(stacked[:bulk], out=tmp[:bulk], form="f64")
+                    if bulk < rows:
+                        bplan.eval(stacked[bulk:], out=tmp[bulk:], form="f64")
+                    total = total + complex(coeff) * torch.complex(tmp[:r], tmp[r:])
+                else:
+                    beta_terms.append((p // 2, complex(coeff), key))
         if beta_terms:
             # one P2P for all log terms: sum_n coeff_n r^2n over the strengths
             # times log(h_s/H) (the same factor for every term: one table extent)
@@ -802,7 +831,11 @@ class NearFieldWrangler:
         mc = mc.to(device="cuda", dtype=torch.complex128)
         one = mc.dim() == 1
         mc2 = (mc.unsqueeze(0) if one else mc).contiguous()
-        if fused and self.helmholtz_split_order <= 17 and all(
+        # the fused blocks sum a 2D log term's log(h/H) part on the quadrature
+        # nodes (the "p2p" beta mode)
+        beta_ok = (self.helmholtz_split_beta_mode == "p2p" or self.tree.dim != 2
+                   or self.helmholtz_split_order == 1)
+        if fused and beta_ok and self.helmholtz_split_order <= 17 and all(
                 len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels):
             # one contraction: complex blocks = table part + remainder quadrature
             # a split-form limit (slices vs the LDS budget) lets the fused

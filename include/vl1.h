@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 8
+#define VL1_ABI_VERSION 9
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -450,6 +450,13 @@ typedef struct vl1_split_args {
   const int32_t* src_box_counts;
   int64_t n_src_points;
   int max_src_box_points;        /* largest src_box_counts entry */
+  /* 1: G_k - G_0 from the closed forms per pair instead of the series (the
+   * reference's helmholtz_split_order1_legacy_subtraction, 3806-3822: two
+   * P2P sums subtracted), self pairs taking the r = 0 limit (3824-3834);
+   * split order 1, no derivative wrapper, coefficients or separate source
+   * set.  2D (i/4) H0^(1)(kr) + log(r) / (2 pi), H0 to 2e-12 absolute; 3D
+   * (e^{ikr} - 1) / (4 pi r). */
+  int legacy_subtraction;
 } vl1_split_args;
 
 /* corr[t] = sum over the list-1 source points s of R(|x_t - y_s|) strength[s],

@@ -254,7 +254,8 @@ def smooth_sources(tree, f, m):
 
 
 def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=None,
-               deriv_axis=None, deriv_sign=1, split_order=1, kernel=None, sources=None):
+               deriv_axis=None, deriv_sign=1, split_order=1, kernel=None, sources=None,
+               legacy=False):
     """P2P of the remainder over list 1: ``out[R, n_points]`` (complex);
     targets of unlisted boxes stay zero.  ``deriv_axis``: the axis of a
     target (``deriv_sign = +1``) or source (``-1``) derivative wrapper.
@@ -262,7 +263,9 @@ def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=N
     remainder (its value at r = 0 for self pairs).  ``sources``: ``(points,
     starts, counts)`` of a separate source set (``smooth_sources``); the
     strengths then index it and no pair is a self pair (r = 0 takes the
-    series value at 0, which is the self limit)."""
+    series value at 0, which is the self limit).  ``legacy``: the closed
+    form ``closed_form_difference`` off the diagonal and ``self_limit`` on
+    it (the legacy subtraction, expansion_wrangler_fpnd.py:3806-3834)."""
     st = np.atleast_2d(np.asarray(strengths, dtype=np.complex128))
     tb_list = tree.target_boxes if target_boxes is None else target_boxes
     starts = tree.neighbor_source_boxes_starts if starts is None else starts
@@ -286,6 +289,9 @@ def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=N
             same = (xt[:, :, None] == ys[:, None, :]).all(axis=0)
             if kernel is not None:
                 rem = np.asarray(kernel(r), dtype=np.complex128)
+            elif legacy:
+                rem = closed_form_difference(tree.dim, complex(k), np.where(same, 1.0, r))
+                rem = np.where(same, self_limit(tree.dim, k), rem)
             elif deriv_axis is None:
                 rem = remainder(tree.dim, k, nmax, r, split_order)
                 rem = np.where(same, self_limit(tree.dim, k), rem)

@@ -31,7 +31,8 @@ from volumential_amd import densities, tree as vtree
 from volumential_amd import _capi
 from volumential_amd.kernels import (AxisSourceDerivative, AxisTargetDerivative, HelmholtzKernel,
                                      LaplaceKernel)
-from volumential_amd.split import HelmholtzSplitCorrection, max_neighbor_distance, series_nmax
+from volumential_amd.split import (HelmholtzSplitCorrection, max_neighbor_distance, series_nmax,
+                                   series_roundoff)
 from volumential_amd.table import DuffyBuildConfig, NearFieldInteractionTable
 from volumential_amd.wrangler import NearFieldWrangler
 
@@ -289,3 +290,62 @@ def test_derivative_split_matches_oracle(cuda, dim, q, tree_kind, kind, axis, k)
         assert any(key[-1] for key in w._plans), "the fused plan was not used"
         l2, linf = rel_errors(fused[0].cpu().numpy(), ref)
         assert l2 <= tol and linf <= tol, (form, l2, linf)
+
+
+@pytest.mark.parametrize("dim,q,level,k,n_rhs", [
+    (2, 4, 3, 10.0, 3), (2, 4, 2, 12.0, 2), (2, 6, 3, 4.0 + 1.0j, 4), (2, 3, 2, 14.0j, 2),
+    (3, 3, 2, 5.0, 1), (3, 4, 2, 2.0 + 0.5j, 5), (3, 2, 2, 4.0j, 8)])
+def test_legacy_subtraction_matches_oracle(cuda, dim, q, level, k, n_rhs):
+    """helmholtz_split_order1_legacy_subtraction (3806-3834): G_k - G_0 from
+    the closed forms per pair (2D the device H0^(1): power series below
+    |kr| ~ 12, Hankel's asymptotic expansion above -- level 2 at k = 12
+    and 14i reaches |kr| ~ 17 and 20) vs the oracle's scipy closed forms:
+    rel <= 1e-11 per RHS (H0 to 2e-12 absolute); and vs the series path
+    where the series is accurate (<= 1e-11)."""
+    print(f"case: legacy subtraction {dim}D q={q} L{level} k={k} R={n_rhs}", flush=True)
+    tr = vtree.uniform_tree(dim, q, level)
+    st = _strengths(tr, n_rhs, 30 + dim + q)
+    corr = HelmholtzSplitCorrection(tr, k, legacy_subtraction=True)
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = corr(torch.as_tensor(st, device="cuda"), *args).cpu().numpy()
+    ref = S.correction(tr, st, k, corr.nmax, legacy=True)
+    l2, linf = rel_errors(got, ref)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= 1e-11 and linf <= 1e-11
+    if series_roundoff(dim, k, S.max_neighbor_distance(tr)) < 1e-13:
+        ser = HelmholtzSplitCorrection(tr, k)(torch.as_tensor(st, device="cuda"), *args)
+        l2, linf = rel_errors(got, ser.cpu().numpy())
+        print(f"vs the series path: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+        assert l2 <= 1e-11 and linf <= 1e-11
+
+
+def test_legacy_subtraction_wrangler(cuda):
+    """eval_direct_split with the legacy subtraction: Laplace-table part +
+    the closed-form correction (two launches; the fused plan is not taken)
+    vs the oracle's sum."""
+    print("case: legacy subtraction through eval_direct_split, 2D q=4 adaptive k=6+0.5i",
+          flush=True)
+    tr = adaptive(2, 4)
+    k = 6.0 + 0.5j
+    knl = HelmholtzKernel(2, k)
+    lap = NearFieldInteractionTable(4, 2, LaplaceKernel(2), source_box_extent=2.0)
+    lap.build_table(DuffyBuildConfig(regular_quad_order=14, radial_quad_order=41))
+    w = NearFieldWrangler(tr, {repr(knl): [lap]}, target_kernels=[knl],
+                          helmholtz_split_order1_legacy_subtraction=True)
+    f = np.stack([densities.helmholtz_beams(tr.points, 20 + r, k=6.0, width=0.3)
+                  for r in range(3)])
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"))[0][0].cpu().numpy()
+    assert not any(key[-1] for key in w._plans), "the fused plan sums the series form"
+    from oracle import list1_ref
+
+    meta = orbit_ref.case_metadata(2, 4)
+    part = [list1_ref.near_field_from_tree(
+        tr, lap.compact_values()[None], lap.mode_normalizers[None], meta, comp,
+        mode="canonical", scaling_power=2, log_displacement=True, table_root_extent=2.0,
+        own_csr=False) for comp in (f.real, f.imag)]
+    ref = part[0] + 1j * part[1] + S.correction(
+        tr, f * np.asarray(tr.weights)[None, :], k, w._correction(knl).nmax, legacy=True)
+    l2, linf = rel_errors(got, ref)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= 1e-11 and linf <= 1e-11

@@ -290,3 +290,20 @@ def test_auto_smooth_quad_order_rule():
     assert A(6, 2, 0.0, 6.5) == 6 + 1 + 3          # imaginary boost ceil(2.5)
     assert A(6, 4, 0.0, 6.5, {"smooth_quad_order_max": 9}) == 9
     assert A(6, 1, 0.0, 0.0, {"smooth_quad_order_min": 8}) == 8
+
+
+@pytest.mark.parametrize("dim,k", [(2, 7.0), (2, 3.0 + 1.0j), (2, 4.0j), (3, 5.0), (3, 2.0 + 0.5j)])
+def test_legacy_correction_equals_series_correction(dim, k):
+    """The legacy subtraction (closed forms off the diagonal, the self limit
+    on it; expansion_wrangler_fpnd.py:3806-3834) and the series path sum
+    the same function G_k - G_0: equal to the series' accuracy.  A wrong
+    sign or factor in either closed form, or a missing diagonal term,
+    fails it."""
+    tr = vtree.uniform_tree(dim, 3, 2)
+    nmax = S.series_nmax(dim, k, S.max_neighbor_distance(tr))
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((2, tr.n_points)) + 1j * rng.standard_normal((2, tr.n_points))
+    ser = S.correction(tr, a, k, nmax)
+    leg = S.correction(tr, a, k, nmax, legacy=True)
+    err = np.abs(leg - ser).max() / np.abs(ser).max()
+    assert err < 1e-12

@@ -267,6 +267,7 @@ class SplitArgs(C.Structure):
         ("src_box_counts", vp),
         ("n_src_points", i64),
         ("max_src_box_points", C.c_int),
+        ("legacy_subtraction", C.c_int),
     ]
 
 
@@ -312,7 +313,7 @@ def lib():
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 8:
+    if L.vl1_abi_version() != 9:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

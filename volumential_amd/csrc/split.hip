@@ -23,6 +23,7 @@
 #include <cmath>
 #include <complex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -32,8 +33,13 @@ namespace vl1 {
 
 constexpr int kSplitBlock = 256;
 
+// Pair value: the remainder series, its axis derivative, or the closed
+// forms of G_k - G_0 (legacy subtraction).
+constexpr int kPairSeries = 0, kPairDeriv = 1, kPairLegacy = 2;
+
 struct SplitParams {
   int dim, n_rhs, nt, n_terms, chunk, ncoef;
+  double k_re, k_im;   // wave number (legacy subtraction)
   int deriv_axis;      // derivative wrapper (DERIV kernels): axis and sign
   double deriv_sign;   // +1 target derivative, -1 source derivative
   const double* coef;  // split_coefficients / split_derivative_coefficients
@@ -54,8 +60,9 @@ struct SplitParams {
   int64_t ldr;
 };
 
-template <int DIM, int RC, bool DERIV>
+template <int DIM, int RC, int MODE>
 __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const SplitParams p) {
+  constexpr bool DERIV = MODE == kPairDeriv;
   extern __shared__ double smem[];
   const int N = p.n_terms;
   const int ncoef = p.ncoef;
@@ -124,6 +131,13 @@ __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const Spl
                   vr *= f;
                   vi *= f;
                 }
+              } else if constexpr (MODE == kPairLegacy) {
+                // closed forms off the diagonal; a self pair takes the
+                // series at r = 0, the limit the reference adds back
+                if (r2 > 0.0)
+                  legacy_difference<DIM>(r2, p.k_re, p.k_im, vr, vi);
+                else
+                  remainder_value<DIM>(r2, sC, N, vr, vi);
               } else {
                 remainder_value<DIM>(r2, sC, N, vr, vi);
               }
@@ -164,9 +178,9 @@ __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const Spl
   }
 }
 
-template <int DIM, int RC, bool DERIV>
+template <int DIM, int RC, int MODE>
 static hipError_t launch_split(const SplitParams& prm, int grid, size_t smem, hipStream_t s) {
-  auto fn = split_correction_kernel<DIM, RC, DERIV>;
+  auto fn = split_correction_kernel<DIM, RC, MODE>;
   hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)smem);
   if (e != hipSuccess) return e;
@@ -210,6 +224,15 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   if (deriv && (a->split_order != 1 || a->coefficients))
     return fail(VL1_E_UNSUPPORTED,
                 "vl1_split_correction: derivative wrappers take split order 1 and the series of k");
+  if (a->legacy_subtraction != 0 && a->legacy_subtraction != 1)
+    return fail(VL1_E_ARG, "vl1_split_correction: legacy_subtraction must be 0 or 1");
+  const bool legacy = a->legacy_subtraction == 1;
+  if (legacy && (deriv || sep || a->coefficients || a->split_order != 1))
+    return fail(VL1_E_UNSUPPORTED,
+                "vl1_split_correction: the legacy subtraction takes split order 1 on the base "
+                "quadrature, no derivative wrapper and no coefficients");
+  if (legacy && a->k_im < 0.0)
+    return fail(VL1_E_ARG, "vl1_split_correction: the legacy subtraction needs Im k >= 0");
   hipStream_t s = (hipStream_t)stream;
   const std::complex<double> k(a->k_re, a->k_im);
   std::vector<double> coef;
@@ -232,6 +255,8 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   prm.n_rhs = a->n_rhs;
   prm.n_terms = a->series_nmax;
   prm.ncoef = (int)coef.size();
+  prm.k_re = a->k_re;
+  prm.k_im = a->k_im;
   prm.deriv_axis = deriv ? a->deriv_axis : -1;
   prm.deriv_sign = a->deriv_kind == VL1_SPLIT_DERIV_SOURCE ? -1.0 : 1.0;
   prm.coef = dcoef.as<double>();
@@ -267,21 +292,22 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   const size_t smem = sizeof(double) * ((size_t)coef.size() + std::max(stage, reduce));
   if (smem > 160 * 1024) return fail(VL1_E_UNSUPPORTED, "vl1_split_correction: LDS budget exceeded");
   const int grid = a->n_tgt_boxes;
-  if (a->dim == 2) {
-    e = deriv ? (RC == 1   ? launch_split<2, 1, true>(prm, grid, smem, s)
-                 : RC == 4 ? launch_split<2, 4, true>(prm, grid, smem, s)
-                           : launch_split<2, 8, true>(prm, grid, smem, s))
-              : (RC == 1   ? launch_split<2, 1, false>(prm, grid, smem, s)
-                 : RC == 4 ? launch_split<2, 4, false>(prm, grid, smem, s)
-                           : launch_split<2, 8, false>(prm, grid, smem, s));
-  } else {
-    e = deriv ? (RC == 1   ? launch_split<3, 1, true>(prm, grid, smem, s)
-                 : RC == 4 ? launch_split<3, 4, true>(prm, grid, smem, s)
-                           : launch_split<3, 8, true>(prm, grid, smem, s))
-              : (RC == 1   ? launch_split<3, 1, false>(prm, grid, smem, s)
-                 : RC == 4 ? launch_split<3, 4, false>(prm, grid, smem, s)
-                           : launch_split<3, 8, false>(prm, grid, smem, s));
-  }
+  const int mode = deriv ? kPairDeriv : (legacy ? kPairLegacy : kPairSeries);
+  auto pick = [&](auto dimc, auto modec) {
+    constexpr int D = decltype(dimc)::value, M = decltype(modec)::value;
+    return RC == 1   ? launch_split<D, 1, M>(prm, grid, smem, s)
+           : RC == 4 ? launch_split<D, 4, M>(prm, grid, smem, s)
+                     : launch_split<D, 8, M>(prm, grid, smem, s);
+  };
+  using D2 = std::integral_constant<int, 2>;
+  using D3 = std::integral_constant<int, 3>;
+  using MS = std::integral_constant<int, kPairSeries>;
+  using MD = std::integral_constant<int, kPairDeriv>;
+  using ML = std::integral_constant<int, kPairLegacy>;
+  if (a->dim == 2)
+    e = mode == kPairDeriv ? pick(D2{}, MD{}) : (mode == kPairLegacy ? pick(D2{}, ML{}) : pick(D2{}, MS{}));
+  else
+    e = mode == kPairDeriv ? pick(D3{}, MD{}) : (mode == kPairLegacy ? pick(D3{}, ML{}) : pick(D3{}, MS{}));
   // the coefficient buffer is freed when dcoef leaves scope: wait for the launch
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_split_correction: ") + hipGetErrorString(e));

@@ -21,6 +21,96 @@ namespace vl1 {
 constexpr int kSplitMaxTerms = 96;
 constexpr double kEulerGamma = 0.5772156649015329;
 
+// ---- Closed forms of G_k - G_0 (the legacy subtraction, 3806-3822) ----
+struct dcplx {
+  double re, im;
+};
+__device__ inline dcplx cmul(dcplx a, dcplx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ inline dcplx cdiv(dcplx a, dcplx b) {  // Smith's algorithm
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double t = b.im / b.re, d = b.re + b.im * t;
+    return {(a.re + a.im * t) / d, (a.im - a.re * t) / d};
+  }
+  const double t = b.re / b.im, d = b.re * t + b.im;
+  return {(a.re * t + a.im) / d, (a.im * t - a.re) / d};
+}
+
+// H0^(1)(z) for Im z >= 0, z != 0.  For |z| >= 8, Hankel's asymptotic
+// expansion (DLMF 10.17.5, nu = 0: term_k = term_{k-1} i (-(2k-1)^2) /
+// (8 k z)) cut at its smallest term, taken when 4 x that term x
+// |sqrt(2 / (pi z)) e^{iz}| is below the power series' rounding scale
+// 2e-16 e^|z| / sqrt(2 pi |z|); otherwise the power series of J0 and Y0
+// (DLMF 10.8.1-2): J0 = sum_m t_m, t_m = (-z^2/4)^m / (m!)^2, Y0 = (2/pi)
+// ((log(z/2) + gamma) J0 - sum_{m>=1} H_m t_m).  Absolute error below
+// 2e-12 over the upper half plane (worst near |z| = 12 on the real axis).
+__device__ inline dcplx hankel1_0(dcplx z) {
+  const double az = hypot(z.re, z.im);
+  const double arg = atan2(z.im, z.re);
+  if (az >= 8.0) {
+    dcplx s = {1.0, 0.0}, term = {1.0, 0.0};
+    double tmin = 1.0;
+    for (int k = 1; k < 64; ++k) {
+      const double c = -(double)((2 * k - 1) * (2 * k - 1)) / (8.0 * k);
+      const dcplx nt = cdiv(dcplx{-c * term.im, c * term.re}, z);  // term * (i c) / z
+      const double an = hypot(nt.re, nt.im);
+      if (an > tmin) break;
+      term = nt;
+      tmin = an;
+      s.re += term.re;
+      s.im += term.im;
+      if (an < 1e-17) break;
+    }
+    const double pref = sqrt(2.0 / (M_PI * az)) * exp(-z.im);
+    if (4.0 * tmin * pref < 2e-16 * exp(az) / sqrt(2.0 * M_PI * az)) {
+      const double ph = z.re - 0.25 * M_PI - 0.5 * arg;
+      return cmul(dcplx{pref * cos(ph), pref * sin(ph)}, s);
+    }
+  }
+  const dcplx q = {0.25 * (z.re * z.re - z.im * z.im), 0.5 * z.re * z.im};
+  const double aq = 0.25 * az * az;
+  dcplx t = {1.0, 0.0}, J = {1.0, 0.0}, S = {0.0, 0.0};
+  double H = 0.0;
+  for (int m = 1; m < 100; ++m) {
+    const double inv = -1.0 / ((double)m * m);
+    t = cmul(t, dcplx{q.re * inv, q.im * inv});
+    H += 1.0 / m;
+    J.re += t.re;
+    J.im += t.im;
+    S.re -= t.re * H;
+    S.im -= t.im * H;
+    if ((double)m * m > aq && hypot(t.re, t.im) * (1.0 + H) < 1e-18 * fmax(hypot(J.re, J.im), 1.0))
+      break;
+  }
+  const dcplx L = {log(0.5 * az) + kEulerGamma, arg};
+  dcplx Y = cmul(L, J);
+  Y.re = (Y.re + S.re) * (2.0 / M_PI);
+  Y.im = (Y.im + S.im) * (2.0 / M_PI);
+  return {J.re - Y.im, J.im + Y.re};
+}
+
+// G_k(r) - G_0(r) for r > 0 from the closed forms: 3D (e^{ikr} - 1) / (4 pi
+// r); 2D (i/4) H0^(1)(kr) + log(r) / (2 pi).  k = 0 gives 0.
+template <int DIM>
+__device__ inline void legacy_difference(double r2, double kr, double ki, double& vr,
+                                         double& vi) {
+  const double r = sqrt(r2);
+  if (kr == 0.0 && ki == 0.0) {
+    vr = vi = 0.0;
+    return;
+  }
+  if (DIM == 3) {
+    const double e = exp(-ki * r), w = 0.25 / (M_PI * r);
+    vr = (e * cos(kr * r) - 1.0) * w;
+    vi = e * sin(kr * r) * w;
+  } else {
+    const dcplx h = hankel1_0(dcplx{kr * r, ki * r});
+    vr = -0.25 * h.im + 0.5 / M_PI * log(r);
+    vi = 0.25 * h.re;
+  }
+}
+
 // Coefficient layout (interleaved complex): 2D c0, then (s_n, b_n) for
 // n = 1..N; 3D c_n for n = 1..N.
 template <int DIM>

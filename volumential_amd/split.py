@@ -320,12 +320,24 @@ class HelmholtzSplitCorrection:
     (``expansion_wrangler_fpnd.py:3884-3957``): the pair value becomes
     ``+-(x_a - y_a) R'(r)/r`` and self pairs give 0 (5293-5300).
     ``split_order`` m > 1: the remainder without the terms the split
-    tabulates (``split_terms``; derivative wrappers take m = 1)."""
+    tabulates (``split_terms``; derivative wrappers take m = 1).
+    ``legacy_subtraction``: ``G_k - G_0`` per pair from the closed forms
+    (Hankel / exponential) instead of the series, the reference's
+    ``helmholtz_split_order1_legacy_subtraction`` (3806-3822; order 1, no
+    derivative wrapper, base quadrature), self pairs taking the r = 0
+    limit (3824-3834)."""
 
     def __init__(self, tree, k, split_order=1, series_terms=None,
-                 deriv_kind=_capi.SPLIT_DERIV_NONE, deriv_axis=0):
+                 deriv_kind=_capi.SPLIT_DERIV_NONE, deriv_axis=0, legacy_subtraction=False):
         if int(split_order) < 1:
             raise ValueError("helmholtz_split_order must be >= 1")
+        self.legacy = bool(legacy_subtraction)
+        if self.legacy and (int(split_order) != 1 or int(deriv_kind) != _capi.SPLIT_DERIV_NONE):
+            raise NotImplementedError(
+                "the legacy subtraction is implemented for split order 1 without a derivative "
+                "wrapper")
+        if self.legacy and complex(k).imag < 0:
+            raise ValueError("the legacy subtraction needs Im k >= 0")
         if int(split_order) > 1 and int(deriv_kind) != _capi.SPLIT_DERIV_NONE:
             raise NotImplementedError(
                 "helmholtz_split_order > 1 with a derivative wrapper is not implemented")
@@ -345,7 +357,7 @@ class HelmholtzSplitCorrection:
         # the series cancels badly at large |k| r: say so instead of
         # returning a silently inaccurate remainder
         self.roundoff = series_roundoff(self.dim, self.k, r_max)
-        if self.roundoff > 1e-10:
+        if self.roundoff > 1e-10 and not self.legacy:
             logger.warning(
                 "Helmholtz split remainder: |k| r_max = %.3g; the series loses digits to "
                 "cancellation (estimated relative error %.1e of the remainder). Refine the "
@@ -417,7 +429,12 @@ class HelmholtzSplitCorrection:
             points=self._points.data_ptr(), strengths=sr.data_ptr(),
             ld_strengths=2 * n_src, result=yr.data_ptr(), ld_result=2 * self.n_points,
             deriv_kind=self.deriv_kind, deriv_axis=self.deriv_axis,
+            legacy_subtraction=int(self.legacy),
         )
+        if self.legacy and (sources is not None or coefficients is not None):
+            raise NotImplementedError(
+                "the legacy subtraction runs on the base quadrature with the closed forms of "
+                "G_k - G_0 only")
         if sources is not None:
             args.src_points = sources.points.data_ptr()
             args.src_box_starts = sources.box_starts.data_ptr()

@@ -91,6 +91,7 @@ class NearFieldWrangler:
         helmholtz_split_term_tables=None,
         helmholtz_split_term_build_config=None,
         helmholtz_split_smooth_quad_order=None,
+        helmholtz_split_order1_legacy_subtraction=False,
     ):
         """``direct_method``: how ``eval_direct`` evaluates list 1 --
         ``"plan"`` (default) through a plan cached per list-1 CSR, FP64
@@ -122,7 +123,13 @@ class NearFieldWrangler:
         (2297-2301, 3709-3786): how the log(h/H) part of a 2D r^2n log r term
         from one table is summed -- ``"p2p"`` (default: a point-to-point sum
         of r^2n over the strengths times log(h_s/H)) or ``"table"`` (the r^2n
-        term table applied to the density times log(h_s/H))."""
+        term table applied to the density times log(h_s/H)).
+
+        ``helmholtz_split_order1_legacy_subtraction`` (2313-2316, 3806-3822):
+        at split order 1 the online correction sums ``G_k - G_0`` from the
+        closed forms (2D Hankel, 3D exponential) instead of the series; self
+        pairs take the r = 0 limit.  Base quadrature, no derivative wrapper;
+        the fused split plan (series quadrature in its blocks) is not taken."""
         if not getattr(tree, "sources_are_targets", False):
             raise ValueError(
                 "table-based near-field evaluation requires sources and targets "
@@ -185,6 +192,8 @@ class NearFieldWrangler:
         if self.helmholtz_split_beta_mode not in ("p2p", "table"):
             raise ValueError("power_log_single_table_beta_mode must be 'table' or 'p2p'")
         self.helmholtz_split = False
+        self.helmholtz_split_order1_legacy_subtraction = bool(
+            helmholtz_split_order1_legacy_subtraction)
         self.helmholtz_split_smooth_quad_order = None
         self._smooth_sources = {}
         if helmholtz_split is None or helmholtz_split:
@@ -198,6 +207,16 @@ class NearFieldWrangler:
                     helmholtz_split_auto_config)
             elif helmholtz_split:
                 raise RuntimeError(f"helmholtz_split is not supported here: {why}")
+        if self.helmholtz_split and self._legacy():
+            if any(split_wrapper_chain(k)[1] != _capi.SPLIT_DERIV_NONE
+                   for k in self.target_kernels):
+                raise NotImplementedError(
+                    "helmholtz_split_order1_legacy_subtraction with a derivative wrapper is not "
+                    "implemented")
+            if (self.helmholtz_split_smooth_quad_order or 0) > int(self.tree.quad_order):
+                raise NotImplementedError(
+                    "helmholtz_split_order1_legacy_subtraction runs on the base quadrature "
+                    "(smooth quadrature order q)")
         self._plain_only = set()  # kernels the plan route cannot take
 
     def _resolve_split_order(self, order, auto_config):
@@ -608,8 +627,12 @@ class NearFieldWrangler:
             base, kind, axis = split_wrapper_chain(knl)
             self._corrections[key] = HelmholtzSplitCorrection(
                 self.tree, base.k, split_order=self.helmholtz_split_order,
-                deriv_kind=kind, deriv_axis=axis)
+                deriv_kind=kind, deriv_axis=axis, legacy_subtraction=self._legacy())
         return self._corrections[key]
+
+    def _legacy(self):
+        """The legacy subtraction applies at split order 1 only (3573-3578)."""
+        return self.helmholtz_split_order1_legacy_subtraction and self.helmholtz_split_order == 1
 
     def eval_direct_helmholtz_split_correction(self, target_boxes, neighbor_source_boxes_starts,
                                                neighbor_source_boxes_lists, src_weights,
@@ -835,7 +858,7 @@ class NearFieldWrangler:
         # nodes (the "p2p" beta mode)
         beta_ok = (self.helmholtz_split_beta_mode == "p2p" or self.tree.dim != 2
                    or self.helmholtz_split_order == 1)
-        if fused and beta_ok and self.helmholtz_split_order <= 17 and all(
+        if fused and beta_ok and not self._legacy() and self.helmholtz_split_order <= 17 and all(
                 len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels):
             # one contraction: complex blocks = table part + remainder quadrature
             # a split-form limit (slices vs the LDS budget) lets the fused

@@ -106,6 +106,36 @@ def test_split_support_with_derivative_wrappers():
     assert split_wrapper_chain(AxisSourceDerivative(0, helm)) == (helm, _capi.SPLIT_DERIV_SOURCE, 0)
 
 
+def test_auto_split_imaginary_coverage_warning(caplog):
+    """"auto" with |Im k| h above rho_imag_split_max (default 8) warns and
+    keeps the split (expansion_wrangler_fpnd.py:2550-2568)."""
+    import logging
+
+    from volumential_amd import tree as vtree
+    from volumential_amd.kernels import HelmholtzKernel, LaplaceKernel
+    from volumential_amd.table import NearFieldInteractionTable
+    from volumential_amd.wrangler import NearFieldWrangler
+
+    tr = vtree.uniform_tree(2, 3, 2)  # coarsest active box h = 0.5
+    lap = NearFieldInteractionTable(3, 2, LaplaceKernel(2), source_box_extent=2.0)
+    yuk = HelmholtzKernel(2, 20.0j)  # rho_imag = 10
+    with caplog.at_level(logging.WARNING, logger="volumential_amd.wrangler"):
+        w = NearFieldWrangler(tr, {repr(yuk): [lap]}, target_kernels=[yuk],
+                              helmholtz_split_order="auto")
+    assert w.helmholtz_split and "exceeds configured split coverage" in caplog.text
+    caplog.clear()
+    with caplog.at_level(logging.WARNING, logger="volumential_amd.wrangler"):
+        NearFieldWrangler(tr, {repr(yuk): [lap]}, target_kernels=[yuk],
+                          helmholtz_split_order="auto",
+                          helmholtz_split_auto_config={"rho_imag_split_max": 12.0})
+    assert "split coverage" not in caplog.text
+    with caplog.at_level(logging.WARNING, logger="volumential_amd.wrangler"):
+        NearFieldWrangler(tr, {repr(yuk): [lap]}, target_kernels=[yuk],
+                          helmholtz_split_order="auto",
+                          helmholtz_split_auto_config={"disable_split_if_outside_coverage": True})
+    assert "Keeping split mode enabled" in caplog.text
+
+
 def test_split_order_options():
     """helmholtz_split_order / _smooth_quad_order resolution
     (expansion_wrangler_fpnd.py:2520-2568, 2649-2747): integers >= 1,

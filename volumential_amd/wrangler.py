@@ -220,12 +220,15 @@ class NearFieldWrangler:
         self._plain_only = set()  # kernels the plan route cannot take
 
     def _resolve_split_order(self, order, auto_config):
-        """``expansion_wrangler_fpnd.py:2520-2568``."""
-        from .split import choose_auto_split_order
+        """``expansion_wrangler_fpnd.py:2520-2568``; in "auto" mode an
+        imaginary rho above ``rho_imag_split_max`` (default 8) is reported,
+        and the split stays on (2550-2568)."""
+        from .split import choose_auto_split_order, split_rho
 
         cfg = dict(auto_config or {})
-        if bool(cfg.get("enabled", False)) or (
-                isinstance(order, str) and order.strip().lower() == "auto"):
+        auto = bool(cfg.get("enabled", False)) or (
+            isinstance(order, str) and order.strip().lower() == "auto")
+        if auto:
             order = choose_auto_split_order(self.tree, self.target_kernels, cfg)
             logger.info("Auto-selected helmholtz_split_order=%d", order)
         order = int(order)
@@ -235,6 +238,19 @@ class NearFieldWrangler:
             logger.warning("split order %d requested with %d target kernels; clamping to order "
                            "1 for multi-output split mode", order, len(self.target_kernels))
             order = 1
+        if auto:
+            rho_im = split_rho(self.tree, self.target_kernels)[1]
+            max_im = float(cfg.get("rho_imag_split_max", 8.0))
+            if rho_im > max_im:
+                if bool(cfg.get("disable_split_if_outside_coverage", False)):
+                    logger.warning(
+                        "Imaginary rho %.3g exceeds configured split coverage %.3g. Keeping "
+                        "split mode enabled because direct fallback requires matching direct "
+                        "near-field tables.", rho_im, max_im)
+                else:
+                    logger.warning(
+                        "Imaginary rho %.3g exceeds configured split coverage %.3g; continuing "
+                        "with split at configured max order", rho_im, max_im)
         if order > 1 and any(split_wrapper_chain(k)[1] != _capi.SPLIT_DERIV_NONE
                              for k in self.target_kernels):
             raise NotImplementedError(

@@ -122,8 +122,10 @@ def test_smooth_quadrature_path_matches_oracle(cuda, dim, q, level, k, m, ms, fu
 
 @pytest.mark.parametrize("fused,form,tol", ROUTES)
 @pytest.mark.parametrize("dim,q,level,k,m", [(3, 3, 2, 6.0, 2), (3, 3, 2, 6.0, 3),
-                                             (2, 4, 3, 12.0, 2), (2, 4, 3, 12.0, 3)])
+                                             (2, 4, 3, 12.0, 2), (2, 4, 3, 12.0, 3),
+                                             (3, 5, 2, 4.0, 2)])
 def test_order_m_matches_oracle(cuda, dim, q, level, k, m, fused, form, tol):
+    """3D q = 5: complex blocks of 250 rows, cut into two row groups."""
     print(f"case: split order {m}, {dim}D q={q} level {level} k={k}, fused={fused} {form}",
           flush=True)
     tr, knl, lap, w, f = _setup(dim, q, level, k, m)

@@ -152,7 +152,7 @@ __global__ void expand_dense_blocks_kernel(int n, int n_cases, int cplx, int MP,
 constexpr int kSplitMaxBlockTerms = 16;
 
 template <int DIM>
-__global__ void split_extra_kernel(int q, int n, int n_blocks, const int32_t* __restrict__ blk_case,
+__global__ void split_extra_kernel(int q, int n, int n_keys, const int32_t* __restrict__ blk_case,
                                    const double* __restrict__ blk_h, double H,
                                    const int32_t* __restrict__ case_vecs,
                                    const uint8_t* __restrict__ perm, const int8_t* __restrict__ sign,
@@ -171,7 +171,7 @@ __global__ void split_extra_kernel(int q, int n, int n_blocks, const int32_t* __
                                    const double* __restrict__ term_coefs,
                                    double2* __restrict__ extra) {
   constexpr int kMaxTerms = kSplitMaxBlockTerms;
-  const int64_t total = (int64_t)n_blocks * n * n;
+  const int64_t total = (int64_t)n_keys * n * n;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int ss = (int)(i % n), tt = (int)((i / n) % n), b = (int)(i / ((int64_t)n * n));
@@ -1803,11 +1803,12 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
       if (e == hipSuccess) e = upload(duw, sp->unit_weights, (size_t)pay->q * 8, s);
       if (e == hipSuccess) e = upload(dco, coef.data(), coef.size() * 8, s);
       if (e == hipSuccess && (order > 1 || sp->smooth_order > 0)) {
-        // per (block, target, source) the remainder of order m, the terms'
-        // tables and the 2D log parts, complex
+        // per (case, level) block -- n_keys of them, before the cut into
+        // row groups -- and (target, source) the remainder of order m, the
+        // terms' tables and the 2D log parts, complex
         DevBuf dtt, dtp, dtl, dtc, dsn, dsw, dsi;
         const int nt = sp->n_terms, ms = sp->smooth_order;
-        e = dext.alloc((size_t)pl->n_blocks * n * n * sizeof(double2));
+        e = dext.alloc((size_t)pl->n_keys * n * n * sizeof(double2));
         if (e == hipSuccess && nt > 0) {
           e = upload(dtt, sp->term_tables, (size_t)nt * pay->n_entries * 8, s);
           if (e == hipSuccess) e = upload(dtp, sp->term_powers, (size_t)nt * 4, s);
@@ -1820,10 +1821,10 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
           if (e == hipSuccess) e = upload(dsi, sp->smooth_interp, (size_t)ms * pay->q * 8, s);
         }
         if (e == hipSuccess) {
-          const int64_t total = (int64_t)pl->n_blocks * n * n;
+          const int64_t total = (int64_t)pl->n_keys * n * n;
           const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
           auto kern = dim == 2 ? split_extra_kernel<2> : split_extra_kernel<3>;
-          hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, pay->q, n, pl->n_blocks,
+          hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, pay->q, n, pl->n_keys,
                              dcase.as<int32_t>(), dh.as<double>(), H, dvec.as<int32_t>(),
                              dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
                              pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(), pay->n_entries,

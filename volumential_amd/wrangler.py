@@ -852,8 +852,8 @@ class NearFieldWrangler:
         return self._beta[key]
 
     def eval_direct_split(self, target_boxes, neighbor_source_boxes_starts,
-                          neighbor_source_boxes_lists, mode_coefs, form="f64", n_slices=5,
-                          fused=True):
+                          neighbor_source_boxes_lists, mode_coefs, form="f64", n_slices=None,
+                          fused=True, tol=1e-10):
         """List 1 as ``drive_volume_fmm`` sums it under the split
         (``volume_fmm.py:983-1026``): the Laplace-table part of
         ``mode_coefs[R, n_points]`` plus the correction with strengths
@@ -862,9 +862,13 @@ class NearFieldWrangler:
         node ids in mode order); otherwise, or with ``fused=False``, the
         table part (plan ``form``) plus the point-to-point correction (and, at
         split order m > 1, the tabulated terms).  The fused plan folds split
-        orders up to 17 and the smooth quadrature into its blocks."""
+        orders up to 17 and the smooth quadrature into its blocks.
+        ``form="split_i8"`` is accuracy-guarded on both routes as in
+        ``eval_direct_plan`` (``n_slices=None``: 5 slices, then 6, then the
+        FP64 form for a density whose estimate exceeds ``tol``)."""
         if not self.helmholtz_split:
             raise RuntimeError("eval_direct_split needs helmholtz_split enabled")
+        self.last_split_report = []
         mc = mode_coefs if isinstance(mode_coefs, torch.Tensor) else torch.as_tensor(
             np.asarray(mode_coefs))
         mc = mc.to(device="cuda", dtype=torch.complex128)
@@ -887,10 +891,10 @@ class NearFieldWrangler:
                                              neighbor_source_boxes_lists, fused_split=True)
                         r = int(mc2.shape[0])
                         bulk = r - r % 64 if r > 64 else r
-                        plan.eval(mc2[:bulk], out=pot[i][:bulk], form=fform, n_slices=n_slices)
+                        self._plan_eval(plan, mc2[:bulk], pot[i][:bulk], fform, n_slices, tol)
                         if bulk < r:
-                            plan.eval(mc2[bulk:], out=pot[i][bulk:], form=fform,
-                                      n_slices=n_slices)
+                            self._plan_eval(plan, mc2[bulk:].contiguous(), pot[i][bulk:], fform,
+                                            n_slices, tol)
                         if one:
                             pot[i] = pot[i][0]
                     return self._finish(pot), TimingFuture()
@@ -898,7 +902,7 @@ class NearFieldWrangler:
                     pass  # box size / node order / LDS beyond the fused plan
         pot, _ = self.eval_direct_plan(target_boxes, neighbor_source_boxes_starts,
                                        neighbor_source_boxes_lists, mc2, form=form,
-                                       n_slices=n_slices)
+                                       n_slices=n_slices, tol=tol)
         wts = torch.as_tensor(np.asarray(self.tree.weights, np.float64), device="cuda")
         corr, _ = self.eval_direct_helmholtz_split_correction(
             target_boxes, neighbor_source_boxes_starts, neighbor_source_boxes_lists,

@@ -168,6 +168,46 @@ def test_adaptive_trees_match_oracle(cuda, dim, q, max_level, k, m, ms, fused, f
     assert l2 <= tol and linf <= tol
 
 
+@pytest.mark.parametrize("form,tol", [("f64", 1e-12), ("split_i8", 1e-10)])
+@pytest.mark.parametrize("ms", [None, 6])
+def test_fused_per_level_tables_match_oracle(cuda, ms, form, tol):
+    """Split order 1 over per-level Laplace tables (no scaling, no log term):
+    the fused plan's blocks read each source level's own table, on the base
+    or the smooth quadrature, vs the oracle's per-level list 1 plus its
+    remainder sum."""
+    print(f"case: fused split over per-level tables, adaptive 2D q=4, smooth {ms}, {form}",
+          flush=True)
+    from tests.gpu_helpers import synthetic_table_levels
+
+    tr = adaptive(2, 4)
+    tables = synthetic_table_levels(2, 4, n_tables=tr.nlevels, seed=9)
+    k = 6.0 + 0.5j
+    knl = HelmholtzKernel(2, k)
+    w = NearFieldWrangler(tr, {repr(knl): tables}, target_kernels=[knl],
+                          helmholtz_split_smooth_quad_order=ms)
+    f = np.stack([densities.helmholtz_beams(tr.points, 50 + r, k=6.0, width=0.3)
+                  for r in range(3)])
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"), form=form)[0][0]
+    assert any(key[-1] for key in w._plans), "the fused plan was not built"
+    meta = orbit_ref.case_metadata(2, 4)
+    data = np.stack([t.compact_values() for t in tables])
+    nm = np.stack([t.mode_normalizers for t in tables])
+    part = [list1_ref.near_field_from_tree(
+        tr, data, nm, meta, comp, mode="per_level", table_root_extent=2.0,
+        table_starting_level=0, own_csr=False) for comp in (f.real, f.imag)]
+    nmax = w._correction(knl).nmax
+    if ms is None:
+        corr = S.correction(tr, f * np.asarray(tr.weights)[None, :], k, nmax)
+    else:
+        pts, starts, counts, st = S.smooth_sources(tr, f, ms)
+        corr = S.correction(tr, st, k, nmax, sources=(pts, starts, counts))
+    ref = part[0] + 1j * part[1] + corr
+    l2, linf = rel_errors(got.cpu().numpy(), ref)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= tol and linf <= tol
+
+
 @pytest.mark.parametrize("tree_kind,m", [("uniform", 2), ("uniform", 3), ("adaptive", 3)])
 def test_table_beta_mode_matches_oracle(cuda, tree_kind, m):
     """power_log_single_table_beta_mode="table" (3757-3779): the log(h/H)

@@ -240,6 +240,7 @@ __global__ void expand_split_blocks_kernel(int dim, int q, int n, int sp_mask, i
                                            double deriv_sign, int MP, int KP, int n_blocks,
                                            int n_rg, int rg_size,
                                            const int32_t* __restrict__ blk_case,
+                                           const int32_t* __restrict__ blk_lev, int64_t n_entries,
                                            const double* __restrict__ blk_alpha,
                                            const double* __restrict__ blk_disp,
                                            const double* __restrict__ blk_h,
@@ -269,7 +270,8 @@ __global__ void expand_split_blocks_kernel(int dim, int q, int n, int sp_mask, i
       const int ss = s % n, tt = t % n;
       const int64_t rel =
           compact_rel_row(dim, q, perm + c * dim, sign + c * dim, group + c * dim, ss, tt);
-      double tv = table[offs[ranks[c]] + rel];
+      // the block's table level (0 for one table)
+      double tv = table[(int64_t)blk_lev[b] * n_entries + offs[ranks[c]] + rel];
       if (entry_sign(dim, q, perm + c * dim, sign + c * dim, sp_mask, ss, tt) < 0) tv = -tv;
       double re = blk_alpha[b] * tv + blk_disp[b] * nmlz[ss];
       if (extra) {  // split order > 1 or smooth quadrature: split_extra_kernel
@@ -1486,8 +1488,13 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
     if (pay->dense_kind)
       return fail(VL1_E_UNSUPPORTED,
                   "Helmholtz split plans take an arithmetic-orbit payload, not the dense kind");
-    if (d->scaling_mode != VL1_SCALING_CANONICAL)
-      return fail(VL1_E_ARG, "Helmholtz split plans use the canonical single-table scaling");
+    // the canonical single-table scaling, or per-level tables (no scaling,
+    // no log term) at split order 1 (the term tables are single-table)
+    if (d->scaling_mode != VL1_SCALING_CANONICAL &&
+        !(d->scaling_mode == VL1_SCALING_PER_LEVEL && sp->n_terms == 0))
+      return fail(VL1_E_ARG,
+                  "Helmholtz split plans use the canonical single-table scaling, or per-level "
+                  "tables at split order 1");
     if (sp->series_nmax < 1 || sp->series_nmax > kSplitMaxTerms)
       return fail(VL1_E_ARG, "Helmholtz split plans: series_nmax must be 1..96");
     if (!sp->unit_nodes || !sp->unit_weights || !sp->case_vecs)
@@ -1842,7 +1849,8 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
         hipLaunchKernelGGL(expand_split_blocks_kernel, dim3(grid), dim3(256), 0, s, dim, pay->q, n,
                            pay->sign_mask, deriv ? sp->deriv_axis : -1,
                            sp->deriv_kind == VL1_SPLIT_DERIV_SOURCE ? -1.0 : 1.0, pl->MP, pl->KP,
-                           pl->n_blocks, pl->n_rg, pl->rg_size, dcase.as<int32_t>(), dal.as<double>(),
+                           pl->n_blocks, pl->n_rg, pl->rg_size, dcase.as<int32_t>(),
+                           dlev.as<int32_t>(), pay->n_entries, dal.as<double>(),
                            ddi.as<double>(), dh.as<double>(), dvec.as<int32_t>(),
                            dperm.as<uint8_t>(), dsign.as<int8_t>(), dgroup.as<uint8_t>(),
                            pay->ranks.as<uint16_t>(), pay->offs.as<int32_t>(),

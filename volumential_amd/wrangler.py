@@ -862,7 +862,8 @@ class NearFieldWrangler:
         node ids in mode order); otherwise, or with ``fused=False``, the
         table part (plan ``form``) plus the point-to-point correction (and, at
         split order m > 1, the tabulated terms).  The fused plan folds split
-        orders up to 17 and the smooth quadrature into its blocks.
+        orders up to 17 (one table) and the smooth quadrature into its
+        blocks; at order 1 it also takes per-level tables.
         ``form="split_i8"`` is accuracy-guarded on both routes as in
         ``eval_direct_plan`` (``n_slices=None``: 5 slices, then 6, then the
         FP64 form for a density whose estimate exceeds ``tol``)."""
@@ -878,8 +879,12 @@ class NearFieldWrangler:
         # nodes (the "p2p" beta mode)
         beta_ok = (self.helmholtz_split_beta_mode == "p2p" or self.tree.dim != 2
                    or self.helmholtz_split_order == 1)
-        if fused and beta_ok and not self._legacy() and self.helmholtz_split_order <= 17 and all(
-                len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels):
+        # per-level Laplace tables fold in at split order 1 (the term tables of
+        # higher orders are single-table)
+        tables_ok = self.helmholtz_split_order == 1 or all(
+            len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels)
+        if (fused and beta_ok and tables_ok and not self._legacy()
+                and self.helmholtz_split_order <= 17):
             # one contraction: complex blocks = table part + remainder quadrature
             # a split-form limit (slices vs the LDS budget) lets the fused
             # FP64 form (more accurate) stand in before two launches

@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 9
+#define VL1_ABI_VERSION 10
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -71,7 +71,15 @@ typedef enum vl1_kernel_kind {
    * only; expansion_wrangler_fpnd.py:144-210): |d|^p, and |d|^p log|d|
    * (p > 0), with p = vl1_duffy_args.power */
   VL1_KERNEL_POWER = 7,
-  VL1_KERNEL_POWER_LOG = 8
+  VL1_KERNEL_POWER_LOG = 8,
+  /* axis derivatives of the radial terms (deriv_axis, power p): target
+   * d/dd_a |d|^p = p |d|^(p-2) d_a and d/dd_a |d|^p log|d| = d_a |d|^(p-2)
+   * (p log|d| + 1); source = minus (the wrapped split terms,
+   * expansion_wrangler_fpnd.py:5018-5051) */
+  VL1_KERNEL_POWER_TARGET_DERIV = 9,
+  VL1_KERNEL_POWER_SOURCE_DERIV = 10,
+  VL1_KERNEL_POWER_LOG_TARGET_DERIV = 11,
+  VL1_KERNEL_POWER_LOG_SOURCE_DERIV = 12
 } vl1_kernel_kind;
 
 /* list1.py:361-437 (get_kernel_scaling_policy) */

@@ -117,6 +117,20 @@ def radial_power_log(dim, p):
     return lambda *d: sum(c * c for c in d) ** (0.5 * p) * 0.5 * np.log(sum(c * c for c in d))
 
 
+def radial_power_target_derivative(dim, axis, p):
+    """``d/dd_axis |d|^p = p |d|^(p-2) d_axis`` (the split term under a target
+    derivative wrapper, expansion_wrangler_fpnd.py:5018-5035)."""
+    return lambda *d: p * sum(c * c for c in d) ** (0.5 * p - 1.0) * d[axis]
+
+
+def radial_power_log_target_derivative(dim, axis, p):
+    """``d/dd_axis |d|^p log|d| = d_axis |d|^(p-2) (p log|d| + 1)`` (5037-5050)."""
+    def g(*d):
+        r2 = sum(c * c for c in d)
+        return d[axis] * r2 ** (0.5 * p - 1.0) * (0.5 * p * np.log(r2) + 1.0)
+    return g
+
+
 def constant(dim):
     return lambda *d: np.ones_like(np.asarray(d[0], dtype=np.float64))
 

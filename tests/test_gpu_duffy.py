@@ -191,3 +191,32 @@ def test_radial_term_entries_match_oracle(cuda, dim, q, extent, kind, p):
     for i in pick:
         ref = g.entry(int(ids[i]), ref_k, reg, rad)
         assert abs(vals[i] - ref) / max(1.0, abs(ref)) <= 1e-12, (ids[i], vals[i], ref)
+
+
+@pytest.mark.parametrize("dim,q,extent,kind,p,wrap,axis", [
+    (3, 3, 1.0, "power", 1, "target", 0), (3, 2, 2.0, "power", 3, "source", 2),
+    (2, 4, 2.0, "power_log", 2, "target", 1), (2, 3, 0.5, "power_log", 4, "source", 0)])
+def test_radial_term_derivative_entries_match_oracle(cuda, dim, q, extent, kind, p, wrap, axis):
+    """Axis derivatives of the split's term kernels (the order > 1 split under
+    a derivative wrapper): sampled canonical entries vs the oracle's Duffy
+    quadrature of the closed-form derivative at the same rule."""
+    from volumential_amd.kernels import (AxisSourceDerivative, AxisTargetDerivative,
+                                         RadialPowerKernel, RadialPowerLogKernel)
+
+    print(f"case: radial term derivative table {kind} p={p} {wrap} axis {axis} {dim}D q={q}",
+          flush=True)
+    reg, rad = 12, 41
+    inner = RadialPowerKernel(dim, p) if kind == "power" else RadialPowerLogKernel(dim, p)
+    knl = (AxisTargetDerivative if wrap == "target" else AxisSourceDerivative)(axis, inner)
+    t = NearFieldInteractionTable(q, dim, knl, source_box_extent=extent)
+    t.build_table(DuffyBuildConfig(regular_quad_order=reg, radial_quad_order=rad))
+    ids, vals = t.get_reduced_table_data()
+    g = D.TableGeometry(dim, q, H=extent)
+    der = (D.radial_power_target_derivative if kind == "power" else
+           D.radial_power_log_target_derivative)(dim, axis, p)
+    sgn = 1.0 if wrap == "target" else -1.0
+    rng = np.random.default_rng(dim * 7 + p)
+    pick = np.unique(np.concatenate([[0, ids.size - 1], rng.integers(0, ids.size, 20)]))
+    for i in pick:
+        ref = sgn * g.entry(int(ids[i]), der, reg, rad)
+        assert abs(vals[i] - ref) / max(1.0, abs(ref)) <= 1e-12, (ids[i], vals[i], ref)

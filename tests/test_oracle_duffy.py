@@ -220,3 +220,36 @@ def test_radial_term_entries_far_cases(dim, q, kern):
             a = g.entry(e, knl, 20, 61)
             b = _gauss_entry(g, e, knl, 40)
             assert abs(a - b) <= 1e-11 * max(1.0, abs(b)), (kern, c, s, t, a, b)
+
+
+@pytest.mark.parametrize("dim,p,log", [(2, 2, True), (2, 4, True), (3, 1, False), (3, 3, False),
+                                       (2, 2, False)])
+def test_radial_term_derivatives_are_derivatives(dim, p, log):
+    """The wrapped split terms' closed forms (d/dd_a |d|^p, d/dd_a |d|^p
+    log|d|) equal central differences of the terms themselves."""
+    base = D.radial_power_log(dim, p) if log else D.radial_power(dim, p)
+    rng = np.random.default_rng(7 * dim + p)
+    for axis in range(dim):
+        der = (D.radial_power_log_target_derivative if log else
+               D.radial_power_target_derivative)(dim, axis, p)
+        for _ in range(5):
+            d = rng.uniform(-1.5, 1.5, dim)
+            h = 1e-5
+            dp, dm = d.copy(), d.copy()
+            dp[axis] += h
+            dm[axis] -= h
+            fd = (base(*dp) - base(*dm)) / (2 * h)
+            assert abs(der(*d) - fd) <= 1e-8 * max(1.0, abs(fd)), (axis, d, der(*d), fd)
+
+
+@pytest.mark.parametrize("dim,q,p", [(2, 3, 2), (2, 3, 4), (3, 2, 2)])
+def test_radial_even_power_derivative_entries_are_polynomial_integrals(dim, q, p):
+    """d/dd_a |d|^p = p |d|^(p-2) d_a is a polynomial for even p: the Duffy
+    entry equals Gauss-Legendre with enough nodes, near cases included."""
+    g = D.TableGeometry(dim, q, H=2.0)
+    rng = np.random.default_rng(p + 3 * dim)
+    knl = D.radial_power_target_derivative(dim, dim - 1, p)
+    for e in rng.integers(0, len(g.vecs) * g.n * g.n, 10):
+        a = g.entry(int(e), knl, 20, 61)
+        b = _gauss_entry(g, int(e), knl, q + p)
+        assert abs(a - b) <= 1e-12 * max(1.0, abs(b)), (e, a, b)

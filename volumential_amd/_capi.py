@@ -33,6 +33,10 @@ KERNEL_HELMHOLTZ_TARGET_DERIV = 5
 KERNEL_HELMHOLTZ_SOURCE_DERIV = 6
 KERNEL_POWER = 7
 KERNEL_POWER_LOG = 8
+KERNEL_POWER_TARGET_DERIV = 9
+KERNEL_POWER_SOURCE_DERIV = 10
+KERNEL_POWER_LOG_TARGET_DERIV = 11
+KERNEL_POWER_LOG_SOURCE_DERIV = 12
 
 SPLIT_DERIV_NONE = 0
 SPLIT_DERIV_TARGET = 1
@@ -313,7 +317,7 @@ def lib():
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 9:
+    if L.vl1_abi_version() != 10:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

@@ -77,9 +77,14 @@ __host__ __device__ constexpr bool kind_is_complex(int kind) {
   return kind == VL1_KERNEL_HELMHOLTZ || kind == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV ||
          kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV;
 }
+__host__ __device__ constexpr bool kind_is_power_deriv(int kind) {
+  return kind == VL1_KERNEL_POWER_TARGET_DERIV || kind == VL1_KERNEL_POWER_SOURCE_DERIV ||
+         kind == VL1_KERNEL_POWER_LOG_TARGET_DERIV || kind == VL1_KERNEL_POWER_LOG_SOURCE_DERIV;
+}
 __host__ __device__ constexpr bool kind_is_deriv(int kind) {
   return kind == VL1_KERNEL_LAPLACE_TARGET_DERIV || kind == VL1_KERNEL_LAPLACE_SOURCE_DERIV ||
-         kind == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV || kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV;
+         kind == VL1_KERNEL_HELMHOLTZ_TARGET_DERIV || kind == VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV ||
+         kind_is_power_deriv(kind);
 }
 template <int DIM, int KIND>
 __device__ inline void kernel_value(double r2, double da, double k, double& re, double& im) {
@@ -110,6 +115,15 @@ __device__ inline void kernel_value(double r2, double da, double k, double& re, 
     }
   } else if (KIND == VL1_KERNEL_CONSTANT) {
     re = 1.0;
+    im = 0.0;
+  } else if (kind_is_power_deriv(KIND)) {
+    // d_a |d|^(p-2) times p, or times (p log|d| + 1) with the log (k carries p)
+    constexpr bool tgt =
+        KIND == VL1_KERNEL_POWER_TARGET_DERIV || KIND == VL1_KERNEL_POWER_LOG_TARGET_DERIV;
+    constexpr bool lg =
+        KIND == VL1_KERNEL_POWER_LOG_TARGET_DERIV || KIND == VL1_KERNEL_POWER_LOG_SOURCE_DERIV;
+    const double base = da * pow(r2, 0.5 * k - 1.0);
+    re = (tgt ? 1.0 : -1.0) * base * (lg ? fma(0.5 * k, log(r2), 1.0) : k);
     im = 0.0;
   } else if (KIND == VL1_KERNEL_POWER || KIND == VL1_KERNEL_POWER_LOG) {
     // |d|^p (k carries p for these kinds), times log|d| for POWER_LOG
@@ -456,10 +470,12 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
     return fail(VL1_E_ARG, "vl1_duffy_build: derivative axis out of range");
   if (a->chunk_start && (a->n_chunks < 1 || a->n_chunks > INT32_MAX))
     return fail(VL1_E_ARG, "vl1_duffy_build: bad chunk count");
-  const bool radial = a->kernel_kind == VL1_KERNEL_POWER || a->kernel_kind == VL1_KERNEL_POWER_LOG;
-  if (radial && (a->power < 0 || a->power > 32 ||
-                 (a->kernel_kind == VL1_KERNEL_POWER_LOG && a->power == 0)))
-    return fail(VL1_E_ARG, "vl1_duffy_build: power must be 0..32 (1..32 with the log)");
+  const bool radial = a->kernel_kind == VL1_KERNEL_POWER || a->kernel_kind == VL1_KERNEL_POWER_LOG ||
+                      kind_is_power_deriv(a->kernel_kind);
+  const bool needs_pos = a->kernel_kind != VL1_KERNEL_POWER;  // log terms, derivatives
+  if (radial && (a->power < 0 || a->power > 32 || (needs_pos && a->power == 0)))
+    return fail(VL1_E_ARG,
+                "vl1_duffy_build: power must be 0..32 (1..32 with the log or a derivative)");
   DuffyParams prm{a->dim, a->quad_order, a->kernel_kind, deriv ? a->deriv_axis : -1,
                   radial ? (double)a->power : a->helmholtz_k, a->n_entries,
                   a->n_nodes, a->nodes_per_sign, a->source_box_extent, a->target_points,
@@ -483,6 +499,14 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
       e = launch_duffy<2, VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV>(prm, s);
     else if (kind == VL1_KERNEL_POWER) e = launch_duffy<2, VL1_KERNEL_POWER>(prm, s);
     else if (kind == VL1_KERNEL_POWER_LOG) e = launch_duffy<2, VL1_KERNEL_POWER_LOG>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_TARGET_DERIV)
+      e = launch_duffy<2, VL1_KERNEL_POWER_TARGET_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_SOURCE_DERIV)
+      e = launch_duffy<2, VL1_KERNEL_POWER_SOURCE_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_LOG_TARGET_DERIV)
+      e = launch_duffy<2, VL1_KERNEL_POWER_LOG_TARGET_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_LOG_SOURCE_DERIV)
+      e = launch_duffy<2, VL1_KERNEL_POWER_LOG_SOURCE_DERIV>(prm, s);
     else return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: kernel kind");
   } else {
     if (kind == VL1_KERNEL_CONSTANT) e = launch_duffy<3, VL1_KERNEL_CONSTANT>(prm, s);
@@ -498,6 +522,14 @@ extern "C" int vl1_duffy_build(const vl1_duffy_args* a, void* stream) {
       e = launch_duffy<3, VL1_KERNEL_HELMHOLTZ_SOURCE_DERIV>(prm, s);
     else if (kind == VL1_KERNEL_POWER) e = launch_duffy<3, VL1_KERNEL_POWER>(prm, s);
     else if (kind == VL1_KERNEL_POWER_LOG) e = launch_duffy<3, VL1_KERNEL_POWER_LOG>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_TARGET_DERIV)
+      e = launch_duffy<3, VL1_KERNEL_POWER_TARGET_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_SOURCE_DERIV)
+      e = launch_duffy<3, VL1_KERNEL_POWER_SOURCE_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_LOG_TARGET_DERIV)
+      e = launch_duffy<3, VL1_KERNEL_POWER_LOG_TARGET_DERIV>(prm, s);
+    else if (kind == VL1_KERNEL_POWER_LOG_SOURCE_DERIV)
+      e = launch_duffy<3, VL1_KERNEL_POWER_LOG_SOURCE_DERIV>(prm, s);
     else return fail(VL1_E_UNSUPPORTED, "vl1_duffy_build: kernel kind");
   }
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("duffy launch: ") + hipGetErrorString(e));

@@ -89,7 +89,7 @@ def duffy_values(table, entry_ids, regular_order, radial_order, stream=None,
         deriv_axis=int(getattr(table.integral_knl, "axis", -1)),
         chunk_start=d_starts.data_ptr() if order is not None else None,
         n_chunks=int(starts.size - 1) if order is not None else 0,
-        power=int(getattr(table.integral_knl, "power", 0)),
+        power=int(getattr(table.integral_knl.get_base_kernel(), "power", 0)),
     )
     st = stream if stream is not None else torch.cuda.current_stream()
     ev0 = torch.cuda.Event(enable_timing=True)

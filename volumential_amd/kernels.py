@@ -121,9 +121,12 @@ class AxisTargetDerivative:
 
     @property
     def kind(self):
-        """Table-builder kernel id (one derivative of Laplace or Helmholtz)."""
+        """Table-builder kernel id (one derivative of Laplace, Helmholtz or a
+        radial split term)."""
         return _derivative_kind(self, _capi.KERNEL_LAPLACE_TARGET_DERIV,
-                                _capi.KERNEL_HELMHOLTZ_TARGET_DERIV)
+                                _capi.KERNEL_HELMHOLTZ_TARGET_DERIV,
+                                _capi.KERNEL_POWER_TARGET_DERIV,
+                                _capi.KERNEL_POWER_LOG_TARGET_DERIV)
 
     @property
     def dim(self):
@@ -150,7 +153,9 @@ class AxisSourceDerivative:
     @property
     def kind(self):
         return _derivative_kind(self, _capi.KERNEL_LAPLACE_SOURCE_DERIV,
-                                _capi.KERNEL_HELMHOLTZ_SOURCE_DERIV)
+                                _capi.KERNEL_HELMHOLTZ_SOURCE_DERIV,
+                                _capi.KERNEL_POWER_SOURCE_DERIV,
+                                _capi.KERNEL_POWER_LOG_SOURCE_DERIV)
 
     @property
     def dim(self):
@@ -170,12 +175,18 @@ class AxisSourceDerivative:
 _DERIVATIVES = (AxisTargetDerivative, AxisSourceDerivative)
 
 
-def _derivative_kind(kernel, laplace_kind, helmholtz_kind):
+def _derivative_kind(kernel, laplace_kind, helmholtz_kind, power_kind, power_log_kind):
     inner = kernel.inner_kernel
     if isinstance(inner, LaplaceKernel):
         return laplace_kind
     if isinstance(inner, HelmholtzKernel):
         return helmholtz_kind
+    if isinstance(inner, RadialPowerKernel):
+        if int(inner.power) < 1:
+            raise ValueError("the derivative of |d|^0 is zero: no table")
+        return power_kind
+    if isinstance(inner, RadialPowerLogKernel):
+        return power_log_kind
     raise NotImplementedError(f"no table builder for {kernel!r} (nested derivatives)")
 
 

@@ -417,8 +417,8 @@ typedef struct vl1_split_args {
   int dim;                 /* 2 or 3 */
   int split_order;         /* m >= 1: the remainder without the terms an order-m split
                               tabulates (2D r^2n log r, 3D r^(2j-1), n, j < m;
-                              expansion_wrangler_fpnd.py:284-304); derivative
-                              wrappers take m = 1 only */
+                              expansion_wrangler_fpnd.py:284-304), also under a
+                              derivative wrapper */
   int series_nmax;         /* series terms, 1..96 (expansion_wrangler_fpnd.py:4337-4389) */
   int n_tgt_boxes;
   int n_rhs;
@@ -446,7 +446,10 @@ typedef struct vl1_split_args {
    * value c0 + sum_n r^2n (b_n - a_n log(r) / (2 pi)); 3D [c_n for
    * n = 1..N], value sum_n c_n r^(n-1); N = series_nmax.  The 2D split of
    * order m > 1 uses it for its log(h/H) term (the r^2n part with
-   * per-source log factors folded into the strengths, 3689-3800). */
+   * per-source log factors folded into the strengths, 3689-3800).  With a
+   * derivative wrapper they give R'(r)/r instead: 2D [(a_n, b_n) for n =
+   * 1..N], R'/r = sum_n r^(2n-2) (b_n - a_n log(r) / (2 pi)); 3D [c_m for
+   * m = 0..N-2], R'/r = sum_m c_m r^(m-1). */
   const double* coefficients;
   /* Optional (NULL = sources are the targets): a separate source set, the
    * smooth-quadrature path (5116-5275): per box src_box_counts[b] sources

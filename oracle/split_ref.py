@@ -140,9 +140,10 @@ def remainder(dim, k, nmax, r, split_order=1):
     return out
 
 
-def remainder_radial_derivative(dim, k, nmax, r):
+def remainder_radial_derivative(dim, k, nmax, r, split_order=1):
     """``R'(r) / r`` of the truncated series, term by term (0 at r = 0,
-    where every derivative-wrapped pair value ``d_a R'(r)/r`` vanishes)."""
+    where every derivative-wrapped pair value ``d_a R'(r)/r`` vanishes);
+    ``split_order`` m > 1 drops the derivatives of the tabulated terms."""
     k = complex(k)
     r = np.asarray(r, dtype=np.float64)
     out = np.zeros(r.shape, dtype=np.complex128)
@@ -159,9 +160,13 @@ def remainder_radial_derivative(dim, k, nmax, r):
             c_log = -s_n / (2.0 * np.pi)
             c_pow = s_n * ((h_n - (lkh + EULER_GAMMA)) / (2.0 * np.pi) + 0.25j)
             # d/dr [c_log r^2n log r + c_pow r^2n] / r
+            if n < split_order:
+                c_log = 0.0
             out += rs ** (2 * n - 2) * (c_log * (2 * n * logr + 1.0) + 2 * n * c_pow)
     else:
         for n in range(2, nmax + 1):
+            if n % 2 == 0 and n <= 2 * (split_order - 1):
+                continue
             out += (n - 1) * (1j * k) ** n / (4.0 * np.pi * math.factorial(n)) * rs ** (n - 3)
     return np.where(pos, out, 0.0)
 
@@ -255,7 +260,7 @@ def smooth_sources(tree, f, m):
 
 def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=None,
                deriv_axis=None, deriv_sign=1, split_order=1, kernel=None, sources=None,
-               legacy=False):
+               legacy=False, kernel_derivative=None):
     """P2P of the remainder over list 1: ``out[R, n_points]`` (complex);
     targets of unlisted boxes stay zero.  ``deriv_axis``: the axis of a
     target (``deriv_sign = +1``) or source (``-1``) derivative wrapper.
@@ -265,7 +270,9 @@ def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=N
     strengths then index it and no pair is a self pair (r = 0 takes the
     series value at 0, which is the self limit).  ``legacy``: the closed
     form ``closed_form_difference`` off the diagonal and ``self_limit`` on
-    it (the legacy subtraction, expansion_wrangler_fpnd.py:3806-3834)."""
+    it (the legacy subtraction, expansion_wrangler_fpnd.py:3806-3834).
+    ``kernel_derivative``: with ``deriv_axis``, a radial ``r -> K'(r)/r``
+    used instead of the remainder's (self pairs 0)."""
     st = np.atleast_2d(np.asarray(strengths, dtype=np.complex128))
     tb_list = tree.target_boxes if target_boxes is None else target_boxes
     starts = tree.neighbor_source_boxes_starts if starts is None else starts
@@ -296,7 +303,10 @@ def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=N
                 rem = remainder(tree.dim, k, nmax, r, split_order)
                 rem = np.where(same, self_limit(tree.dim, k), rem)
             else:
-                rem = deriv_sign * dv[deriv_axis] * remainder_radial_derivative(tree.dim, k, nmax, r)
+                rad = (remainder_radial_derivative(tree.dim, k, nmax, r, split_order)
+                       if kernel_derivative is None else
+                       np.asarray(kernel_derivative(np.where(same, 1.0, r)), np.complex128))
+                rem = deriv_sign * dv[deriv_axis] * rad
                 rem = np.where(same, 0.0, rem)
             acc += st[:, s0:s0 + ns] @ rem.T
         out[:, t0:t0 + nt] = acc

@@ -291,3 +291,146 @@ def test_order_2_closer_to_helmholtz_tables(cuda, dim, q, level, k):
           f"{wa.helmholtz_split_smooth_quad_order}, rel L2 {rel_errors(got, direct)[0]:.2e}",
           flush=True)
     assert rel_errors(got, direct)[0] < errs[0]
+
+
+@pytest.mark.parametrize("form,tol", [("f64", 1e-12), ("split_i8", 1e-10)])
+@pytest.mark.parametrize("dim,q,tree_kind,wrap,axis,k,m", [
+    (2, 4, "uniform", "target", 1, 8.0, 2), (2, 4, "adaptive", "source", 0, 6.0 + 0.5j, 3),
+    (3, 3, "uniform", "target", 2, 5.0, 2), (3, 3, "uniform", "source", 0, 4.0 + 0.5j, 3)])
+def test_derivative_wrapper_order_m_matches_oracle(cuda, dim, q, tree_kind, wrap, axis, k, m,
+                                                   form, tol):
+    """Split order m > 1 under an axis derivative wrapper (5018-5051): the
+    wrapped Laplace table (power 1) + the derivative of the order-m
+    remainder + each wrapped term table scaled by (h/H)^(d+p-1) + in 2D the
+    log(h/H) part as the derivative P2P of r^2n -- vs the oracle assembled
+    from the same pieces."""
+    from oracle import cpu as ocpu
+    from volumential_amd.kernels import AxisSourceDerivative, AxisTargetDerivative
+
+    print(f"case: {wrap} derivative axis {axis}, split order {m}, {dim}D q={q} {tree_kind} "
+          f"k={k} {form}", flush=True)
+    tr = (vtree.uniform_tree(dim, q, 3 if dim == 2 else 2) if tree_kind == "uniform"
+          else adaptive(dim, q))
+    cls = AxisTargetDerivative if wrap == "target" else AxisSourceDerivative
+    sgn = 1 if wrap == "target" else -1
+    hk = cls(axis, HelmholtzKernel(dim, k))
+    dtab = NearFieldInteractionTable(q, dim, cls(axis, LaplaceKernel(dim)), source_box_extent=2.0)
+    dtab.build_table(RULE)
+    w = NearFieldWrangler(tr, {repr(hk): [dtab]}, target_kernels=[hk], helmholtz_split_order=m,
+                          helmholtz_split_term_build_config=RULE)
+    f = np.stack([densities.helmholtz_beams(tr.points, 30 + r, k=4.0, width=0.3)
+                  for r in range(2)])
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"), form=form)[0][0]
+    power = [0] * dim
+    power[axis] = 1
+    meta = orbit_ref.case_metadata(dim, q, (axis,), power)
+    part = ocpu.near_field(tr, dtab.compact_values(), dtab.mode_normalizers, meta,
+                           np.concatenate([f.real, f.imag]), mode="canonical", scaling_power=1,
+                           table_root_extent=2.0)
+    ref = part[:2] + 1j * part[2:]
+    st = f * np.asarray(tr.weights)[None, :]
+    nmax = w._correction(hk).nmax
+    ref = ref + S.correction(tr, st, k, nmax, deriv_axis=axis, deriv_sign=sgn, split_order=m)
+    h = tr.root_extent * 0.5 ** np.arange(32)
+    per_box = np.log(tr.root_extent * 0.5 ** np.asarray(tr.box_levels, float) / 2.0)
+    beta = per_box[_box_of_point(tr)]
+    for kind, p, c in S.extracted_terms(dim, k, m):
+        tt = w.split_term_tables(hk, (kind, p))[0]
+        tp = [list1_ref.near_field_from_tree(
+            tr, tt.compact_values()[None], tt.mode_normalizers[None], meta, comp, mode="custom",
+            level_scaling=(h / 2.0) ** (dim + p - 1), level_disp=np.zeros(32),
+            table_root_extent=2.0, own_csr=False) for comp in (f.real, f.imag)]
+        term = tp[0] + 1j * tp[1]
+        if kind == "power_log":
+            term = term + S.correction(tr, st * beta[None, :], k, nmax, deriv_axis=axis,
+                                       deriv_sign=sgn,
+                                       kernel_derivative=lambda r, p=p: p * r ** (p - 2.0))
+        ref = ref + c * term
+    l2, linf = rel_errors(got.cpu().numpy(), ref)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= tol and linf <= tol
+
+
+@pytest.mark.parametrize("dim,q,level,k", [(3, 4, 2, 10.0), (2, 4, 3, 8.0)])
+def test_derivative_order_2_closer_to_helmholtz_tables(cuda, dim, q, level, k):
+    """Physics of the wrapped split: against list 1 through directly built
+    Helmholtz target-derivative tables, order 2 is closer than order 1 (a
+    wrong term scaling power or sign would not be)."""
+    from volumential_amd.kernels import AxisTargetDerivative
+
+    print(f"case: derivative split orders 1-3 vs Helmholtz derivative tables, {dim}D q={q}",
+          flush=True)
+    tr = vtree.uniform_tree(dim, q, level)
+    axis = dim - 1
+    hk = AxisTargetDerivative(axis, HelmholtzKernel(dim, k))
+    ext = tr.root_extent / 2 ** level
+    helm = NearFieldInteractionTable(q, dim, hk, source_box_extent=ext, source_box_level=level)
+    helm.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=61))
+    f = torch.as_tensor(np.stack([densities.helmholtz_beams(tr.points, 95 + r, k=k, width=0.3)
+                                  for r in range(2)]), device="cuda")
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    wd = NearFieldWrangler(tr, {repr(hk): [helm]}, target_kernels=[hk], helmholtz_split=False)
+    direct = wd.eval_direct_batched(*args, f)[0][0].cpu().numpy()
+    lap = NearFieldInteractionTable(q, dim, AxisTargetDerivative(axis, LaplaceKernel(dim)),
+                                    source_box_extent=2.0)
+    lap.build_table(DuffyBuildConfig(regular_quad_order=20, radial_quad_order=61))
+    errs = []
+    for m in (1, 2, 3):
+        w = NearFieldWrangler(tr, {repr(hk): [lap]}, target_kernels=[hk], helmholtz_split_order=m)
+        got = w.eval_direct_split(*args, f, fused=False)[0][0].cpu().numpy()
+        errs.append(rel_errors(got, direct)[0])
+    print("rel L2 vs Helmholtz derivative tables, orders 1-3:",
+          ", ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert errs[1] < errs[0] and errs[2] < errs[0]
+
+
+@pytest.mark.parametrize("m", [1, 2])
+def test_derivative_wrapper_smooth_quadrature_matches_oracle(cuda, m):
+    """The smooth-quadrature path under a derivative wrapper: the derivative
+    remainder (and at m = 2 the log part's derivative P2P) over the
+    interpolated density on finer nodes, coincident node pairs giving 0
+    (the wrapped chain's limit), vs the oracle."""
+    from oracle import cpu as ocpu
+    from volumential_amd.kernels import AxisTargetDerivative
+
+    dim, q, k, ms, axis = 2, 4, 6.0, 7, 0
+    print(f"case: target derivative, smooth order {ms}, split order {m}, 2D q={q}", flush=True)
+    tr = vtree.uniform_tree(dim, q, 3)
+    hk = AxisTargetDerivative(axis, HelmholtzKernel(dim, k))
+    dtab = NearFieldInteractionTable(q, dim, AxisTargetDerivative(axis, LaplaceKernel(dim)),
+                                     source_box_extent=2.0)
+    dtab.build_table(RULE)
+    w = NearFieldWrangler(tr, {repr(hk): [dtab]}, target_kernels=[hk], helmholtz_split_order=m,
+                          helmholtz_split_term_build_config=RULE,
+                          helmholtz_split_smooth_quad_order=ms)
+    f = np.stack([densities.helmholtz_beams(tr.points, 60 + r, k=k, width=0.3) for r in range(2)])
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"))[0][0].cpu().numpy()
+    power = [0] * dim
+    power[axis] = 1
+    meta = orbit_ref.case_metadata(dim, q, (axis,), power)
+    part = ocpu.near_field(tr, dtab.compact_values(), dtab.mode_normalizers, meta,
+                           np.concatenate([f.real, f.imag]), mode="canonical", scaling_power=1,
+                           table_root_extent=2.0)
+    ref = part[:2] + 1j * part[2:]
+    pts, starts, counts, st = S.smooth_sources(tr, f, ms)
+    src = (pts, starts, counts)
+    nmax = w._correction(hk).nmax
+    ref = ref + S.correction(tr, st, k, nmax, deriv_axis=axis, split_order=m, sources=src)
+    h = tr.root_extent * 0.5 ** np.arange(32)
+    beta = np.repeat(np.log(tr.root_extent * 0.5 ** np.asarray(tr.box_levels, float) / 2.0),
+                     counts)
+    for kind, p, c in S.extracted_terms(dim, k, m):
+        tt = w.split_term_tables(hk, (kind, p))[0]
+        tp = [list1_ref.near_field_from_tree(
+            tr, tt.compact_values()[None], tt.mode_normalizers[None], meta, comp, mode="custom",
+            level_scaling=(h / 2.0) ** (dim + p - 1), level_disp=np.zeros(32),
+            table_root_extent=2.0, own_csr=False) for comp in (f.real, f.imag)]
+        term = tp[0] + 1j * tp[1] + S.correction(
+            tr, st * beta[None, :], k, nmax, deriv_axis=axis, sources=src,
+            kernel_derivative=lambda r, p=p: p * r ** (p - 2.0))
+        ref = ref + c * term
+    l2, linf = rel_errors(got, ref)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= 1e-12 and linf <= 1e-12

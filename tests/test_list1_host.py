@@ -140,9 +140,9 @@ def test_split_order_options():
     """helmholtz_split_order / _smooth_quad_order resolution
     (expansion_wrangler_fpnd.py:2520-2568, 2649-2747): integers >= 1,
     "auto" through the rho ladder (and the smooth order rule), clamped to 1
-    with several output kernels, refused above 1 under a derivative
-    wrapper; the smooth order defaults to q outside "auto"; order > 1 asks
-    for the density values (src_func) in the correction."""
+    with several output kernels, kept above 1 under a derivative wrapper;
+    the smooth order defaults to q outside "auto"; order > 1 asks for the
+    density values (src_func) in the correction."""
     from volumential_amd import tree as vtree
     from volumential_amd.kernels import AxisTargetDerivative, HelmholtzKernel, LaplaceKernel
     from volumential_amd.table import NearFieldInteractionTable
@@ -171,8 +171,9 @@ def test_split_order_options():
     dt = NearFieldInteractionTable(3, 2, AxisTargetDerivative(0, LaplaceKernel(2)),
                                    source_box_extent=2.0)
     assert NearFieldWrangler(tr, {repr(dk): [dt]}, target_kernels=[dk]).helmholtz_split
-    with pytest.raises(NotImplementedError):
-        NearFieldWrangler(tr, {repr(dk): [dt]}, target_kernels=[dk], helmholtz_split_order=2)
+    # order > 1 under a derivative wrapper (the wrapped terms, 5018-5051)
+    assert NearFieldWrangler(tr, {repr(dk): [dt]}, target_kernels=[dk],
+                             helmholtz_split_order=2).helmholtz_split_order == 2
     w = make(helmholtz_split_order=2)
     with pytest.raises(RuntimeError, match="src_func"):
         w.eval_direct_helmholtz_split_correction(

@@ -307,3 +307,23 @@ def test_legacy_correction_equals_series_correction(dim, k):
     leg = S.correction(tr, a, k, nmax, legacy=True)
     err = np.abs(leg - ser).max() / np.abs(ser).max()
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("dim,k,m", [(2, 7.0, 2), (2, 3.0 + 1.0j, 3), (3, 5.0, 2), (3, 2.0j, 3)])
+def test_higher_order_derivative_plus_term_derivatives_is_order_one(dim, k, m):
+    """Under a derivative wrapper: R_m'(r)/r plus each extracted term's
+    coefficient times its radial derivative over r (2D r^2n log r ->
+    r^(2n-2) (2n log r + 1), 3D r^p -> p r^(p-2)) is R_1'(r)/r; and the
+    order-1 derivative matches the closed form."""
+    r = np.linspace(0.05, 1.5, 31)
+    nmax = 40
+    tot = S.remainder_radial_derivative(dim, k, nmax, r, split_order=m)
+    for kind, p, c in S.extracted_terms(dim, k, m):
+        if kind == "power_log":
+            tot = tot + c * r ** (p - 2) * (p * np.log(r) + 1.0)
+        else:
+            tot = tot + c * p * r ** (p - 2.0)
+    one = S.remainder_radial_derivative(dim, k, nmax, r)
+    np.testing.assert_allclose(tot, one, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(one, S.closed_form_radial_derivative(dim, k, r), rtol=1e-9,
+                               atol=1e-12)

@@ -213,17 +213,12 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   const int64_t n_src = sep ? a->n_src_points : a->n_points;
   if (a->n_rhs > 1 && (a->ld_strengths < 2 * n_src || a->ld_result < 2 * a->n_points))
     return fail(VL1_E_ARG, "vl1_split_correction: bad leading dimension");
-  if (sep && a->deriv_kind != VL1_SPLIT_DERIV_NONE)
-    return fail(VL1_E_UNSUPPORTED,
-                "vl1_split_correction: a separate source set takes no derivative wrapper");
   if (a->deriv_kind < VL1_SPLIT_DERIV_NONE || a->deriv_kind > VL1_SPLIT_DERIV_SOURCE)
     return fail(VL1_E_ARG, "vl1_split_correction: bad deriv_kind");
   const bool deriv = a->deriv_kind != VL1_SPLIT_DERIV_NONE;
   if (deriv && (a->deriv_axis < 0 || a->deriv_axis >= a->dim))
     return fail(VL1_E_ARG, "vl1_split_correction: deriv_axis out of range");
-  if (deriv && (a->split_order != 1 || a->coefficients))
-    return fail(VL1_E_UNSUPPORTED,
-                "vl1_split_correction: derivative wrappers take split order 1 and the series of k");
+
   if (a->legacy_subtraction != 0 && a->legacy_subtraction != 1)
     return fail(VL1_E_ARG, "vl1_split_correction: legacy_subtraction must be 0 or 1");
   const bool legacy = a->legacy_subtraction == 1;
@@ -237,12 +232,14 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   const std::complex<double> k(a->k_re, a->k_im);
   std::vector<double> coef;
   if (a->coefficients) {
-    const int nc = a->dim == 2 ? 2 + 4 * a->series_nmax : 2 * a->series_nmax;
+    // value layout, or with a derivative wrapper the R'(r)/r layout
+    const int nc = deriv ? (a->dim == 2 ? 4 * a->series_nmax : 2 * (a->series_nmax - 1))
+                         : (a->dim == 2 ? 2 + 4 * a->series_nmax : 2 * a->series_nmax);
     coef.assign(a->coefficients, a->coefficients + nc);
     for (double c : coef)
       if (!std::isfinite(c)) return fail(VL1_E_ARG, "vl1_split_correction: coefficients must be finite");
   } else {
-    coef = deriv ? split_derivative_coefficients(a->dim, k, a->series_nmax)
+    coef = deriv ? split_derivative_coefficients(a->dim, k, a->series_nmax, a->split_order)
                  : split_coefficients(a->dim, k, a->series_nmax, a->split_order);
   }
   if (coef.empty()) coef.assign(2, 0.0);  // 3D derivative with one term: R' = 0

@@ -231,8 +231,11 @@ inline std::vector<double> split_coefficients(int dim, std::complex<double> k, i
 }
 
 // Coefficients of remainder_radial_derivative, from those above.
-inline std::vector<double> split_derivative_coefficients(int dim, std::complex<double> k, int N) {
-  const std::vector<double> c = split_coefficients(dim, k, N);
+inline std::vector<double> split_derivative_coefficients(int dim, std::complex<double> k, int N,
+                                                         int order = 1) {
+  // term by term from the value series of the same order (the terms an
+  // order-m split tabulates are already dropped there)
+  const std::vector<double> c = split_coefficients(dim, k, N, order);
   std::vector<double> out;
   if (dim == 2) {
     for (int n = 1; n <= N; ++n) {

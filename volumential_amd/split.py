@@ -320,7 +320,7 @@ class HelmholtzSplitCorrection:
     (``expansion_wrangler_fpnd.py:3884-3957``): the pair value becomes
     ``+-(x_a - y_a) R'(r)/r`` and self pairs give 0 (5293-5300).
     ``split_order`` m > 1: the remainder without the terms the split
-    tabulates (``split_terms``; derivative wrappers take m = 1).
+    tabulates (``split_terms``), also under a derivative wrapper.
     ``legacy_subtraction``: ``G_k - G_0`` per pair from the closed forms
     (Hankel / exponential) instead of the series, the reference's
     ``helmholtz_split_order1_legacy_subtraction`` (3806-3822; order 1, no
@@ -338,9 +338,6 @@ class HelmholtzSplitCorrection:
                 "wrapper")
         if self.legacy and complex(k).imag < 0:
             raise ValueError("the legacy subtraction needs Im k >= 0")
-        if int(split_order) > 1 and int(deriv_kind) != _capi.SPLIT_DERIV_NONE:
-            raise NotImplementedError(
-                "helmholtz_split_order > 1 with a derivative wrapper is not implemented")
         self.dim = int(tree.dim)
         self.k = complex(k)
         self.split_order = int(split_order)
@@ -443,7 +440,10 @@ class HelmholtzSplitCorrection:
             args.max_src_box_points = int(sources.m) ** self.dim
         if coefficients is not None:
             cf = np.ascontiguousarray(coefficients, np.float64)
-            want = 2 + 4 * self.nmax if self.dim == 2 else 2 * self.nmax
+            if self.deriv_kind == _capi.SPLIT_DERIV_NONE:
+                want = 2 + 4 * self.nmax if self.dim == 2 else 2 * self.nmax
+            else:  # the R'(r)/r layout
+                want = 4 * self.nmax if self.dim == 2 else 2 * (self.nmax - 1)
             if cf.shape != (want,):
                 raise ValueError(f"coefficients must hold {want} values")
             args.coefficients = cf.ctypes.data

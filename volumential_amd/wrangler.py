@@ -251,10 +251,6 @@ class NearFieldWrangler:
                     logger.warning(
                         "Imaginary rho %.3g exceeds configured split coverage %.3g; continuing "
                         "with split at configured max order", rho_im, max_im)
-        if order > 1 and any(split_wrapper_chain(k)[1] != _capi.SPLIT_DERIV_NONE
-                             for k in self.target_kernels):
-            raise NotImplementedError(
-                "helmholtz_split_order > 1 with a derivative wrapper is not implemented")
         return order
 
     def _resolve_smooth_order(self, smooth_order, order_arg, auto_config):
@@ -697,14 +693,20 @@ class NearFieldWrangler:
         """Tables of split term ``key`` = ``("power", p)`` / ``("power_log",
         p)``: the caller's, or built on the GPU at the extents of ``knl``'s
         Laplace tables (``_autobuild_helmholtz_split_term_tables``,
-        4700-4776)."""
+        4700-4776); under ``knl``'s derivative wrapper for a wrapped output
+        kernel."""
+        from .kernels import RadialPowerKernel, RadialPowerLogKernel
+
+        kind, p = key
+        dim = self.tree.dim
+        # the term under the output kernel's derivative wrapper (5018-5051)
+        term = _rewrap(knl, RadialPowerKernel(dim, p) if kind == "power"
+                       else RadialPowerLogKernel(dim, p))
+        if split_wrapper_chain(knl)[1] != _capi.SPLIT_DERIV_NONE:
+            key = (kind, p, repr(term))  # the caller's tables are the bare terms'
         if key not in self._term_tables:
-            from .kernels import RadialPowerKernel, RadialPowerLogKernel
             from .table import DuffyBuildConfig, NearFieldInteractionTable
 
-            kind, p = key
-            dim = self.tree.dim
-            term = RadialPowerKernel(dim, p) if kind == "power" else RadialPowerLogKernel(dim, p)
             cfg = self.helmholtz_split_term_build_config or DuffyBuildConfig()
             built = []
             for t in self.near_field_table[repr(knl)]:
@@ -741,7 +743,9 @@ class NearFieldWrangler:
                                 t0.arithmetic_metadata())
         kw = {}
         if len(tabs) == 1:
-            e = self.tree.dim + int(key[1])
+            # (h/H)^(d + p), one power less under a derivative wrapper
+            n_deriv = int(split_wrapper_chain(knl)[1] != _capi.SPLIT_DERIV_NONE)
+            e = self.tree.dim + int(key[1]) - n_deriv
             lev = np.arange(_capi.MAX_LEVELS)
             h = float(self.tree.root_extent) * 0.5 ** lev
             scal = (h / t0.source_box_extent) ** e
@@ -820,12 +824,19 @@ class NearFieldWrangler:
                 ck = ("beta", repr(knl), n_hi)
                 if ck not in self._corrections:
                     self._corrections[ck] = HelmholtzSplitCorrection(
-                        self.tree, base.k, series_terms=n_hi)
+                        self.tree, base.k, series_terms=n_hi, deriv_kind=corr.deriv_kind,
+                        deriv_axis=corr.deriv_axis)
                 pc = self._corrections[ck]
-            coefs = np.zeros(2 + 4 * pc.nmax)
-            for n, c, _ in beta_terms:
-                coefs[2 + 4 * (n - 1) + 2] = c.real  # b_n: r^2n, no log part
-                coefs[2 + 4 * (n - 1) + 3] = c.imag
+            if pc.deriv_kind == _capi.SPLIT_DERIV_NONE:
+                coefs = np.zeros(2 + 4 * pc.nmax)
+                for n, c, _ in beta_terms:
+                    coefs[2 + 4 * (n - 1) + 2] = c.real  # b_n: r^2n, no log part
+                    coefs[2 + 4 * (n - 1) + 3] = c.imag
+            else:  # R'(r)/r of c r^2n: 2n c r^(2n-2)
+                coefs = np.zeros(4 * pc.nmax)
+                for n, c, _ in beta_terms:
+                    coefs[4 * (n - 1) + 2] = 2 * n * c.real
+                    coefs[4 * (n - 1) + 3] = 2 * n * c.imag
             beta = self._log_alpha(self.split_term_tables(knl, beta_terms[0][2])[0], sources)
             total = total + pc(st * beta[None, :], target_boxes, starts, lists,
                                coefficients=coefs, sources=sources)
@@ -879,11 +890,16 @@ class NearFieldWrangler:
         # nodes (the "p2p" beta mode)
         beta_ok = (self.helmholtz_split_beta_mode == "p2p" or self.tree.dim != 2
                    or self.helmholtz_split_order == 1)
+        # derivative wrappers: the fused blocks at order 1 on the base nodes
+        deriv_ok = all(split_wrapper_chain(k)[1] == _capi.SPLIT_DERIV_NONE
+                       for k in self.target_kernels) or (
+            self.helmholtz_split_order == 1
+            and (self.helmholtz_split_smooth_quad_order or 0) <= int(self.tree.quad_order))
         # per-level Laplace tables fold in at split order 1 (the term tables of
         # higher orders are single-table)
         tables_ok = self.helmholtz_split_order == 1 or all(
             len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels)
-        if (fused and beta_ok and tables_ok and not self._legacy()
+        if (fused and beta_ok and tables_ok and deriv_ok and not self._legacy()
                 and self.helmholtz_split_order <= 17):
             # one contraction: complex blocks = table part + remainder quadrature
             # a split-form limit (slices vs the LDS budget) lets the fused

@@ -463,10 +463,10 @@ typedef struct vl1_split_args {
   int max_src_box_points;        /* largest src_box_counts entry */
   /* 1: G_k - G_0 from the closed forms per pair instead of the series (the
    * reference's helmholtz_split_order1_legacy_subtraction, 3806-3822: two
-   * P2P sums subtracted), self pairs taking the r = 0 limit (3824-3834);
-   * split order 1, no derivative wrapper, coefficients or separate source
-   * set.  2D (i/4) H0^(1)(kr) + log(r) / (2 pi), H0 to 2e-12 absolute; 3D
-   * (e^{ikr} - 1) / (4 pi r). */
+   * P2P sums subtracted), self pairs taking the r = 0 limit (3824-3834), 0
+   * under a derivative wrapper; split order 1, no coefficients.  2D (i/4)
+   * H0^(1)(kr) + log(r) / (2 pi), H0 to 2e-12 absolute; 3D (e^{ikr} - 1) /
+   * (4 pi r); derivatives d_a (G_k - G_0)'(r) / r with H1^(1) in 2D. */
   int legacy_subtraction;
 } vl1_split_args;
 

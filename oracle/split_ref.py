@@ -296,16 +296,20 @@ def correction(tree, strengths, k, nmax, target_boxes=None, starts=None, lists=N
             same = (xt[:, :, None] == ys[:, None, :]).all(axis=0)
             if kernel is not None:
                 rem = np.asarray(kernel(r), dtype=np.complex128)
-            elif legacy:
+            elif legacy and deriv_axis is None:
                 rem = closed_form_difference(tree.dim, complex(k), np.where(same, 1.0, r))
                 rem = np.where(same, self_limit(tree.dim, k), rem)
             elif deriv_axis is None:
                 rem = remainder(tree.dim, k, nmax, r, split_order)
                 rem = np.where(same, self_limit(tree.dim, k), rem)
             else:
-                rad = (remainder_radial_derivative(tree.dim, k, nmax, r, split_order)
-                       if kernel_derivative is None else
-                       np.asarray(kernel_derivative(np.where(same, 1.0, r)), np.complex128))
+                if kernel_derivative is not None:
+                    rad = np.asarray(kernel_derivative(np.where(same, 1.0, r)), np.complex128)
+                elif legacy:
+                    rad = closed_form_radial_derivative(tree.dim, complex(k),
+                                                        np.where(same, 1.0, r))
+                else:
+                    rad = remainder_radial_derivative(tree.dim, k, nmax, r, split_order)
                 rem = deriv_sign * dv[deriv_axis] * rad
                 rem = np.where(same, 0.0, rem)
             acc += st[:, s0:s0 + ns] @ rem.T

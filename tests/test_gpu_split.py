@@ -349,3 +349,65 @@ def test_legacy_subtraction_wrangler(cuda):
     l2, linf = rel_errors(got, ref)
     print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
     assert l2 <= 1e-11 and linf <= 1e-11
+
+
+@pytest.mark.parametrize("dim,q,level,k,kind,axis", [
+    (2, 4, 3, 10.0, "target", 0), (2, 4, 2, 12.0, "source", 1), (2, 3, 2, 3.0 + 1.0j, "target", 1),
+    (2, 3, 2, 14.0j, "source", 0), (3, 3, 2, 5.0, "target", 2), (3, 4, 2, 2.0 + 0.5j, "source", 1)])
+def test_legacy_subtraction_derivative_matches_oracle(cuda, dim, q, level, k, kind, axis):
+    """The legacy subtraction under an axis derivative wrapper: +-(x_a - y_a)
+    (G_k - G_0)'(r)/r from the closed forms (2D the device H1^(1), series or
+    asymptotic expansion; 3D the exponential), self pairs 0, vs the oracle's
+    scipy closed forms (rel <= 1e-10 per RHS), and vs the series
+    derivative where the series is accurate."""
+    print(f"case: legacy subtraction {kind} derivative axis {axis} {dim}D q={q} L{level} k={k}",
+          flush=True)
+    tr = vtree.uniform_tree(dim, q, level)
+    st = _strengths(tr, 3, 40 + dim + q)
+    dk, sgn = _deriv_args(kind)
+    corr = HelmholtzSplitCorrection(tr, k, deriv_kind=dk, deriv_axis=axis,
+                                    legacy_subtraction=True)
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = corr(torch.as_tensor(st, device="cuda"), *args).cpu().numpy()
+    ref = S.correction(tr, st, k, corr.nmax, deriv_axis=axis, deriv_sign=sgn, legacy=True)
+    l2, linf = rel_errors(got, ref)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= 1e-10 and linf <= 1e-10
+    if series_roundoff(dim, k, S.max_neighbor_distance(tr)) < 1e-13:
+        ser = HelmholtzSplitCorrection(tr, k, deriv_kind=dk, deriv_axis=axis)(
+            torch.as_tensor(st, device="cuda"), *args)
+        l2, linf = rel_errors(got, ser.cpu().numpy())
+        print(f"vs the series path: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+        assert l2 <= 1e-10 and linf <= 1e-10
+
+
+def test_legacy_subtraction_smooth_quadrature(cuda):
+    """The legacy subtraction on the smooth quadrature (q = 3 nodes, 4 smooth
+    nodes per axis: none shared) vs the oracle: the Laplace-table part plus
+    the closed-form sum over the interpolated density."""
+    print("case: legacy subtraction, smooth order 4, 2D q=3 level 3 k=5+0.5i", flush=True)
+    tr = vtree.uniform_tree(2, 3, 3)
+    k = 5.0 + 0.5j
+    knl = HelmholtzKernel(2, k)
+    lap = NearFieldInteractionTable(3, 2, LaplaceKernel(2), source_box_extent=2.0)
+    lap.build_table(DuffyBuildConfig(regular_quad_order=14, radial_quad_order=41))
+    w = NearFieldWrangler(tr, {repr(knl): [lap]}, target_kernels=[knl],
+                          helmholtz_split_order1_legacy_subtraction=True,
+                          helmholtz_split_smooth_quad_order=4)
+    f = np.stack([densities.helmholtz_beams(tr.points, 70 + r, k=5.0, width=0.3)
+                  for r in range(2)])
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"))[0][0].cpu().numpy()
+    from oracle import list1_ref
+
+    meta = orbit_ref.case_metadata(2, 3)
+    part = [list1_ref.near_field_from_tree(
+        tr, lap.compact_values()[None], lap.mode_normalizers[None], meta, comp,
+        mode="canonical", scaling_power=2, log_displacement=True, table_root_extent=2.0,
+        own_csr=False) for comp in (f.real, f.imag)]
+    pts, starts, counts, st = S.smooth_sources(tr, f, 4)
+    ref = part[0] + 1j * part[1] + S.correction(tr, st, k, w._correction(knl).nmax, legacy=True,
+                                                sources=(pts, starts, counts))
+    l2, linf = rel_errors(got, ref)
+    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= 1e-11 and linf <= 1e-11

@@ -189,14 +189,15 @@ def test_split_order_options():
     with pytest.raises(ValueError, match="beta_mode"):
         make(helmholtz_split_order=2,
              helmholtz_split_auto_config={"power_log_single_table_beta_mode": "quad"})
-    # the legacy subtraction (3573-3578, 3806-3822): order 1 only, base
-    # quadrature, no derivative wrapper
+    # the legacy subtraction (3573-3578, 3806-3822): order 1 only
     w = make(helmholtz_split_order1_legacy_subtraction=True)
     assert w._legacy()
     assert not make(helmholtz_split_order=2,
                     helmholtz_split_order1_legacy_subtraction=True)._legacy()
-    with pytest.raises(NotImplementedError):
+    # q = 3 and 5 Gauss nodes share the midpoint (5125-5138); 4 does not
+    with pytest.raises(ValueError, match="shares nodes"):
         make(helmholtz_split_order1_legacy_subtraction=True, helmholtz_split_smooth_quad_order=5)
-    with pytest.raises(NotImplementedError):
-        NearFieldWrangler(tr, {repr(dk): [dt]}, target_kernels=[dk],
-                          helmholtz_split_order1_legacy_subtraction=True)
+    assert make(helmholtz_split_order1_legacy_subtraction=True,
+                helmholtz_split_smooth_quad_order=4)._legacy()
+    assert NearFieldWrangler(tr, {repr(dk): [dt]}, target_kernels=[dk],
+                             helmholtz_split_order1_legacy_subtraction=True)._legacy()

@@ -327,3 +327,17 @@ def test_higher_order_derivative_plus_term_derivatives_is_order_one(dim, k, m):
     np.testing.assert_allclose(tot, one, rtol=1e-12, atol=1e-13)
     np.testing.assert_allclose(one, S.closed_form_radial_derivative(dim, k, r), rtol=1e-9,
                                atol=1e-12)
+
+
+@pytest.mark.parametrize("dim,k", [(2, 4.0), (2, 2.0 + 1.0j), (3, 3.0), (3, 1.0 + 0.5j)])
+def test_legacy_derivative_correction_equals_series_derivative(dim, k):
+    """Under a derivative wrapper the legacy closed form d_a (G_k - G_0)'/r and
+    the series derivative sum the same function (self pairs 0 in both)."""
+    tr = vtree.uniform_tree(dim, 3, 2)
+    nmax = S.series_nmax(dim, k, S.max_neighbor_distance(tr))
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((2, tr.n_points)) + 1j * rng.standard_normal((2, tr.n_points))
+    for axis, sgn in ((0, 1), (dim - 1, -1)):
+        ser = S.correction(tr, a, k, nmax, deriv_axis=axis, deriv_sign=sgn)
+        leg = S.correction(tr, a, k, nmax, deriv_axis=axis, deriv_sign=sgn, legacy=True)
+        assert np.abs(leg - ser).max() / np.abs(ser).max() < 1e-11

@@ -35,7 +35,7 @@ constexpr int kSplitBlock = 256;
 
 // Pair value: the remainder series, its axis derivative, or the closed
 // forms of G_k - G_0 (legacy subtraction).
-constexpr int kPairSeries = 0, kPairDeriv = 1, kPairLegacy = 2;
+constexpr int kPairSeries = 0, kPairDeriv = 1, kPairLegacy = 2, kPairLegacyDeriv = 3;
 
 struct SplitParams {
   int dim, n_rhs, nt, n_terms, chunk, ncoef;
@@ -62,7 +62,7 @@ struct SplitParams {
 
 template <int DIM, int RC, int MODE>
 __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const SplitParams p) {
-  constexpr bool DERIV = MODE == kPairDeriv;
+  constexpr bool DERIV = MODE == kPairDeriv || MODE == kPairLegacyDeriv;
   extern __shared__ double smem[];
   const int N = p.n_terms;
   const int ncoef = p.ncoef;
@@ -126,7 +126,10 @@ __global__ void __launch_bounds__(kSplitBlock) split_correction_kernel(const Spl
                 // +-(x_a - y_a) R'(r) / r; the self pair (r = 0) gives 0
                 vr = vi = 0.0;
                 if (r2 > 0.0) {
-                  remainder_radial_derivative<DIM>(r2, sC, N, vr, vi);
+                  if constexpr (MODE == kPairLegacyDeriv)
+                    legacy_radial_derivative<DIM>(r2, p.k_re, p.k_im, vr, vi);
+                  else
+                    remainder_radial_derivative<DIM>(r2, sC, N, vr, vi);
                   const double f = p.deriv_sign * da;
                   vr *= f;
                   vi *= f;
@@ -222,10 +225,10 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   if (a->legacy_subtraction != 0 && a->legacy_subtraction != 1)
     return fail(VL1_E_ARG, "vl1_split_correction: legacy_subtraction must be 0 or 1");
   const bool legacy = a->legacy_subtraction == 1;
-  if (legacy && (deriv || sep || a->coefficients || a->split_order != 1))
+  if (legacy && (a->coefficients || a->split_order != 1))
     return fail(VL1_E_UNSUPPORTED,
-                "vl1_split_correction: the legacy subtraction takes split order 1 on the base "
-                "quadrature, no derivative wrapper and no coefficients");
+                "vl1_split_correction: the legacy subtraction takes split order 1 and no "
+                "coefficients");
   if (legacy && a->k_im < 0.0)
     return fail(VL1_E_ARG, "vl1_split_correction: the legacy subtraction needs Im k >= 0");
   hipStream_t s = (hipStream_t)stream;
@@ -289,7 +292,8 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   const size_t smem = sizeof(double) * ((size_t)coef.size() + std::max(stage, reduce));
   if (smem > 160 * 1024) return fail(VL1_E_UNSUPPORTED, "vl1_split_correction: LDS budget exceeded");
   const int grid = a->n_tgt_boxes;
-  const int mode = deriv ? kPairDeriv : (legacy ? kPairLegacy : kPairSeries);
+  const int mode = deriv ? (legacy ? kPairLegacyDeriv : kPairDeriv)
+                         : (legacy ? kPairLegacy : kPairSeries);
   auto pick = [&](auto dimc, auto modec) {
     constexpr int D = decltype(dimc)::value, M = decltype(modec)::value;
     return RC == 1   ? launch_split<D, 1, M>(prm, grid, smem, s)
@@ -301,10 +305,16 @@ extern "C" int vl1_split_correction(const vl1_split_args* a, void* stream) {
   using MS = std::integral_constant<int, kPairSeries>;
   using MD = std::integral_constant<int, kPairDeriv>;
   using ML = std::integral_constant<int, kPairLegacy>;
-  if (a->dim == 2)
-    e = mode == kPairDeriv ? pick(D2{}, MD{}) : (mode == kPairLegacy ? pick(D2{}, ML{}) : pick(D2{}, MS{}));
-  else
-    e = mode == kPairDeriv ? pick(D3{}, MD{}) : (mode == kPairLegacy ? pick(D3{}, ML{}) : pick(D3{}, MS{}));
+  using MLD = std::integral_constant<int, kPairLegacyDeriv>;
+  auto by_mode = [&](auto dimc) {
+    switch (mode) {
+      case kPairDeriv: return pick(dimc, MD{});
+      case kPairLegacy: return pick(dimc, ML{});
+      case kPairLegacyDeriv: return pick(dimc, MLD{});
+      default: return pick(dimc, MS{});
+    }
+  };
+  e = a->dim == 2 ? by_mode(D2{}) : by_mode(D3{});
   // the coefficient buffer is freed when dcoef leaves scope: wait for the launch
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return fail(VL1_E_HIP, std::string("vl1_split_correction: ") + hipGetErrorString(e));

@@ -90,6 +90,89 @@ __device__ inline dcplx hankel1_0(dcplx z) {
   return {J.re - Y.im, J.im + Y.re};
 }
 
+// H1^(1)(z) for Im z >= 0, z != 0, by the same rule as hankel1_0: the
+// asymptotic expansion (nu = 1: term_k = term_{k-1} i (4 - (2k-1)^2) /
+// (8 k z), phase z - 3 pi / 4) where it is the more accurate, otherwise
+// J1 = (z/2) sum_m t_m, t_m = (-z^2/4)^m / (m! (m+1)!), and Y1 = (2/pi)
+// log(z/2) J1 - 2 / (pi z) - (z / (2 pi)) sum_m (psi(m+1) + psi(m+2)) t_m
+// (DLMF 10.8.1-2).  Error below 1e-12 (absolute, relative where |H1| > 1)
+// over the upper half plane in a float64 restatement against scipy.
+__device__ inline dcplx hankel1_1(dcplx z) {
+  const double az = hypot(z.re, z.im);
+  const double arg = atan2(z.im, z.re);
+  if (az >= 8.0) {
+    dcplx s = {1.0, 0.0}, term = {1.0, 0.0};
+    double tmin = 1.0;
+    for (int k = 1; k < 64; ++k) {
+      const double c = (4.0 - (double)((2 * k - 1) * (2 * k - 1))) / (8.0 * k);
+      const dcplx nt = cdiv(dcplx{-c * term.im, c * term.re}, z);  // term * (i c) / z
+      const double an = hypot(nt.re, nt.im);
+      if (an > tmin) break;
+      term = nt;
+      tmin = an;
+      s.re += term.re;
+      s.im += term.im;
+      if (an < 1e-17) break;
+    }
+    const double pref = sqrt(2.0 / (M_PI * az)) * exp(-z.im);
+    if (4.0 * tmin * pref < 2e-16 * exp(az) / sqrt(2.0 * M_PI * az)) {
+      const double ph = z.re - 0.75 * M_PI - 0.5 * arg;
+      return cmul(dcplx{pref * cos(ph), pref * sin(ph)}, s);
+    }
+  }
+  const dcplx q = {0.25 * (z.re * z.re - z.im * z.im), 0.5 * z.re * z.im};
+  const double aq = 0.25 * az * az;
+  dcplx t = {1.0, 0.0}, J = {1.0, 0.0};
+  dcplx S = {1.0 - 2.0 * kEulerGamma, 0.0};  // psi(1) + psi(2)
+  double H = 0.0;
+  for (int m = 1; m < 100; ++m) {
+    const double inv = -1.0 / ((double)m * (m + 1));
+    t = cmul(t, dcplx{q.re * inv, q.im * inv});
+    H += 1.0 / m;
+    const double psi = 2.0 * (H - kEulerGamma) + 1.0 / (m + 1);
+    J.re += t.re;
+    J.im += t.im;
+    S.re += psi * t.re;
+    S.im += psi * t.im;
+    if ((double)m * m > aq &&
+        4.0 * hypot(t.re, t.im) * (1.0 + H) < 1e-18 * fmax(hypot(J.re, J.im), 1.0))
+      break;
+  }
+  const dcplx hz = {0.5 * z.re, 0.5 * z.im};
+  const dcplx J1 = cmul(hz, J);
+  const dcplx L = {log(0.5 * az), arg};
+  const dcplx LJ = cmul(L, J1);
+  const dcplx iz = cdiv(dcplx{1.0, 0.0}, z);
+  const dcplx zS = cmul(hz, S);
+  const dcplx Y1 = {(2.0 * LJ.re - 2.0 * iz.re - zS.re) / M_PI,
+                    (2.0 * LJ.im - 2.0 * iz.im - zS.im) / M_PI};
+  return {J1.re - Y1.im, J1.im + Y1.re};
+}
+
+// (G_k - G_0)'(r) / r for r > 0 from the closed forms: 3D [e^{ikr} (ikr -
+// 1) + 1] / (4 pi r^3); 2D [-(ik/4) H1^(1)(kr) + 1 / (2 pi r)] / r.  k = 0
+// gives 0.
+template <int DIM>
+__device__ inline void legacy_radial_derivative(double r2, double kr, double ki, double& gr,
+                                                double& gi) {
+  const double r = sqrt(r2);
+  if (kr == 0.0 && ki == 0.0) {
+    gr = gi = 0.0;
+    return;
+  }
+  if (DIM == 3) {
+    const double e = exp(-ki * r), c = cos(kr * r), sn = sin(kr * r);
+    const double A = -ki * r - 1.0, B = kr * r;  // ikr - 1
+    const double w = 0.25 / (M_PI * r2 * r);
+    gr = (e * (c * A - sn * B) + 1.0) * w;
+    gi = e * (sn * A + c * B) * w;
+  } else {
+    const dcplx h = hankel1_1(dcplx{kr * r, ki * r});
+    gr = (0.25 * (ki * h.re + kr * h.im) + 0.5 / (M_PI * r)) / r;
+    gi = 0.25 * (ki * h.im - kr * h.re) / r;
+  }
+}
+
 // G_k(r) - G_0(r) for r > 0 from the closed forms: 3D (e^{ikr} - 1) / (4 pi
 // r); 2D (i/4) H0^(1)(kr) + log(r) / (2 pi).  k = 0 gives 0.
 template <int DIM>

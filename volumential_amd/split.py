@@ -323,19 +323,17 @@ class HelmholtzSplitCorrection:
     tabulates (``split_terms``), also under a derivative wrapper.
     ``legacy_subtraction``: ``G_k - G_0`` per pair from the closed forms
     (Hankel / exponential) instead of the series, the reference's
-    ``helmholtz_split_order1_legacy_subtraction`` (3806-3822; order 1, no
-    derivative wrapper, base quadrature), self pairs taking the r = 0
-    limit (3824-3834)."""
+    ``helmholtz_split_order1_legacy_subtraction`` (3806-3822; order 1),
+    self pairs taking the r = 0 limit (3824-3834; 0 under a derivative
+    wrapper)."""
 
     def __init__(self, tree, k, split_order=1, series_terms=None,
                  deriv_kind=_capi.SPLIT_DERIV_NONE, deriv_axis=0, legacy_subtraction=False):
         if int(split_order) < 1:
             raise ValueError("helmholtz_split_order must be >= 1")
         self.legacy = bool(legacy_subtraction)
-        if self.legacy and (int(split_order) != 1 or int(deriv_kind) != _capi.SPLIT_DERIV_NONE):
-            raise NotImplementedError(
-                "the legacy subtraction is implemented for split order 1 without a derivative "
-                "wrapper")
+        if self.legacy and int(split_order) != 1:
+            raise NotImplementedError("the legacy subtraction is the split of order 1")
         if self.legacy and complex(k).imag < 0:
             raise ValueError("the legacy subtraction needs Im k >= 0")
         self.dim = int(tree.dim)
@@ -428,10 +426,9 @@ class HelmholtzSplitCorrection:
             deriv_kind=self.deriv_kind, deriv_axis=self.deriv_axis,
             legacy_subtraction=int(self.legacy),
         )
-        if self.legacy and (sources is not None or coefficients is not None):
+        if self.legacy and coefficients is not None:
             raise NotImplementedError(
-                "the legacy subtraction runs on the base quadrature with the closed forms of "
-                "G_k - G_0 only")
+                "the legacy subtraction sums the closed forms of G_k - G_0 only")
         if sources is not None:
             args.src_points = sources.points.data_ptr()
             args.src_box_starts = sources.box_starts.data_ptr()

@@ -126,10 +126,12 @@ class NearFieldWrangler:
         term table applied to the density times log(h_s/H)).
 
         ``helmholtz_split_order1_legacy_subtraction`` (2313-2316, 3806-3822):
-        at split order 1 the online correction sums ``G_k - G_0`` from the
-        closed forms (2D Hankel, 3D exponential) instead of the series; self
-        pairs take the r = 0 limit.  Base quadrature, no derivative wrapper;
-        the fused split plan (series quadrature in its blocks) is not taken."""
+        at split order 1 the online correction sums ``G_k - G_0`` (or its
+        axis derivative) from the closed forms (2D Hankel, 3D exponential)
+        instead of the series; self pairs take the r = 0 limit; a smooth
+        quadrature order whose nodes meet the box nodes is refused
+        (5125-5138).  The fused split plan (series quadrature in its blocks)
+        is not taken."""
         if not getattr(tree, "sources_are_targets", False):
             raise ValueError(
                 "table-based near-field evaluation requires sources and targets "
@@ -208,15 +210,19 @@ class NearFieldWrangler:
             elif helmholtz_split:
                 raise RuntimeError(f"helmholtz_split is not supported here: {why}")
         if self.helmholtz_split and self._legacy():
-            if any(split_wrapper_chain(k)[1] != _capi.SPLIT_DERIV_NONE
-                   for k in self.target_kernels):
-                raise NotImplementedError(
-                    "helmholtz_split_order1_legacy_subtraction with a derivative wrapper is not "
-                    "implemented")
-            if (self.helmholtz_split_smooth_quad_order or 0) > int(self.tree.quad_order):
-                raise NotImplementedError(
-                    "helmholtz_split_order1_legacy_subtraction runs on the base quadrature "
-                    "(smooth quadrature order q)")
+            ms = self.helmholtz_split_smooth_quad_order or 0
+            if ms > int(self.tree.quad_order):
+                # the closed forms are singular where a smooth node meets a
+                # box node (5125-5138)
+                from .quadrature import gauss_legendre_01
+
+                xq, _ = gauss_legendre_01(int(self.tree.quad_order))
+                xm, _ = gauss_legendre_01(ms)
+                if np.any(np.abs(xm[:, None] - xq[None, :]) < 1.0e-15):
+                    raise ValueError(
+                        "helmholtz split smooth quadrature order shares nodes with source "
+                        "quadrature; choose a non-overlapping order to avoid singular "
+                        "Helmholtz/Laplace evaluations")
         self._plain_only = set()  # kernels the plan route cannot take
 
     def _resolve_split_order(self, order, auto_config):

@@ -272,9 +272,13 @@ int vl1_plan_create(vl1_plan** out, const vl1_payload* p, const vl1_plan_desc* d
  * `laplace` is a real scalar Laplace payload, d->scaling_mode canonical; the
  * plan then evaluates complex densities like a complex-table plan.  In 2D
  * the box-local node ids must be in mode order (VL1_E_UNSUPPORTED).
- * An axis target / source derivative wrapper (3884-3957, order 1) takes
- * the wrapped Laplace table's signed payload (canonical power 1, no log
- * term) and the remainder's axis derivative +-(x_a - y_a) R'(r) / r. */
+ * An axis target / source derivative wrapper (3884-3957) takes the wrapped
+ * Laplace table's signed payload (canonical power 1, no log term) and the
+ * remainder's axis derivative +-(x_a - y_a) R'(r) / r; at m > 1 the term
+ * tables are the wrapped terms' (same signed layout), scaled by
+ * (h/H)^(d + p - 1), and a 2D log term's part is log(h/H) x the quadrature
+ * of +-(x_a - y_a) p r^(p-2) (5018-5051).  Per-level payloads
+ * (d->scaling_mode per-level) are taken at split order 1. */
 /* Derivative wrapper of a split output kernel (expansion_wrangler_fpnd.py:
  * 3884-3957): none, AxisTargetDerivative (d/dx_a), AxisSourceDerivative
  * (d/dy_a = -d/dx_a). */
@@ -292,7 +296,7 @@ typedef struct vl1_helmholtz_split_blocks {
   const int32_t* case_vecs;   /* [n_cases][dim] list-1 gallery vectors (host) */
   int deriv_kind;             /* vl1_split_deriv: derivative wrapper of the output kernel */
   int deriv_axis;             /* its axis (deriv_kind != VL1_SPLIT_DERIV_NONE) */
-  /* Split order m (0 or 1: order 1).  m > 1 (no derivative wrapper): the
+  /* Split order m (0 or 1: order 1).  m > 1: the
    * remainder without the tabulated terms, plus each term's table scaled by
    * coefficient x (h/H)^(d + p) and, for a 2D log term, coefficient x
    * log(h/H) x the quadrature of r^p (expansion_wrangler_fpnd.py:3560-3836,

@@ -321,7 +321,10 @@ def test_derivative_wrapper_order_m_matches_oracle(cuda, dim, q, tree_kind, wrap
     f = np.stack([densities.helmholtz_beams(tr.points, 30 + r, k=4.0, width=0.3)
                   for r in range(2)])
     args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
-    got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"), form=form)[0][0]
+    fd = torch.as_tensor(f, device="cuda")
+    got = w.eval_direct_split(*args, fd, form=form)[0][0]
+    assert any(key[-1] for key in w._plans), "the fused plan was not built"
+    two = w.eval_direct_split(*args, fd, form=form, fused=False)[0][0]
     power = [0] * dim
     power[axis] = 1
     meta = orbit_ref.case_metadata(dim, q, (axis,), power)
@@ -347,9 +350,10 @@ def test_derivative_wrapper_order_m_matches_oracle(cuda, dim, q, tree_kind, wrap
                                        deriv_sign=sgn,
                                        kernel_derivative=lambda r, p=p: p * r ** (p - 2.0))
         ref = ref + c * term
-    l2, linf = rel_errors(got.cpu().numpy(), ref)
-    print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
-    assert l2 <= tol and linf <= tol
+    for route, val in (("fused", got), ("two launches", two)):
+        l2, linf = rel_errors(val.cpu().numpy(), ref)
+        print(f"{route} vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+        assert l2 <= tol and linf <= tol
 
 
 @pytest.mark.parametrize("dim,q,level,k", [(3, 4, 2, 10.0), (2, 4, 3, 8.0)])
@@ -407,6 +411,7 @@ def test_derivative_wrapper_smooth_quadrature_matches_oracle(cuda, m):
     f = np.stack([densities.helmholtz_beams(tr.points, 60 + r, k=k, width=0.3) for r in range(2)])
     args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
     got = w.eval_direct_split(*args, torch.as_tensor(f, device="cuda"))[0][0].cpu().numpy()
+    assert any(key[-1] for key in w._plans), "the fused plan was not built"
     power = [0] * dim
     power[axis] = 1
     meta = orbit_ref.case_metadata(dim, q, (axis,), power)

@@ -169,8 +169,14 @@ __global__ void split_extra_kernel(int q, int n, int n_keys, const int32_t* __re
                                    const int32_t* __restrict__ term_powers,
                                    const int32_t* __restrict__ term_logs,
                                    const double* __restrict__ term_coefs,
+                                   int deriv_axis, double deriv_sign, int sp_mask,
                                    double2* __restrict__ extra) {
   constexpr int kMaxTerms = kSplitMaxBlockTerms;
+  // under an axis derivative wrapper (deriv_axis >= 0): the remainder's
+  // +-(x_a - y_a) R'(r)/r (coef in the R'/r layout, self pairs 0), the
+  // wrapped term tables' signed entries at (h/H)^(d + p - 1), and the log
+  // parts' derivative +-(x_a - y_a) p r^(p-2)
+  const bool deriv = deriv_axis >= 0;
   const int64_t total = (int64_t)n_keys * n * n;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -197,7 +203,7 @@ __global__ void split_extra_kernel(int q, int n, int n_keys, const int32_t* __re
     for (int k = 0; k < n_terms && k < kMaxTerms; ++k) lq[k] = 0.0;
     const int nodes = ms > 0 ? (DIM == 2 ? ms * ms : ms * ms * ms) : 1;
     for (int j = 0; j < nodes; ++j) {
-      double r2 = 0.0, wl = 1.0;
+      double r2 = 0.0, wl = 1.0, da = 0.0;
       int jr = j;
       for (int a = DIM - 1; a >= 0; --a) {
         double y, wa;
@@ -213,20 +219,35 @@ __global__ void split_extra_kernel(int q, int n, int n_keys, const int32_t* __re
         const double d = xt[a] - y;
         r2 = fma(d, d, r2);
         wl *= wa;
+        if (a == deriv_axis) da = d;
       }
-      double vr, vi;
-      remainder_value<DIM>(r2, coef, N, vr, vi);
+      double vr = 0.0, vi = 0.0;
+      if (!deriv) {
+        remainder_value<DIM>(r2, coef, N, vr, vi);
+      } else if (r2 > 0.0) {
+        remainder_radial_derivative<DIM>(r2, coef, N, vr, vi);
+        vr *= deriv_sign * da;
+        vi *= deriv_sign * da;
+      }
       rr = fma(wl, vr, rr);
       ri = fma(wl, vi, ri);
-      for (int k = 0; k < n_terms && k < kMaxTerms; ++k)
-        if (term_logs[k]) lq[k] = fma(wl, pow(r2, 0.5 * term_powers[k]), lq[k]);
+      for (int k = 0; k < n_terms && k < kMaxTerms; ++k) {
+        if (!term_logs[k]) continue;
+        const int pw = term_powers[k];
+        const double tv = !deriv ? pow(r2, 0.5 * pw)
+                                 : (r2 > 0.0 ? deriv_sign * da * pw * pow(r2, 0.5 * pw - 1.0) : 0.0);
+        lq[k] = fma(wl, tv, lq[k]);
+      }
     }
     if (n_terms > 0) {
       const int64_t rel = compact_rel_row(DIM, q, perm + c * DIM, sign + c * DIM, group + c * DIM, ss, tt);
       const int64_t at = offs[ranks[c]] + rel;
       const double lg = log(h / H);
+      const double tsg =
+          entry_sign(DIM, q, perm + c * DIM, sign + c * DIM, sp_mask, ss, tt) < 0 ? -1.0 : 1.0;
       for (int k = 0; k < n_terms && k < kMaxTerms; ++k) {
-        double v = pow(h / H, (double)(DIM + term_powers[k])) * term_tables[(int64_t)k * n_entries + at];
+        double v = tsg * pow(h / H, (double)(DIM + term_powers[k] - (deriv ? 1 : 0))) *
+                   term_tables[(int64_t)k * n_entries + at];
         if (term_logs[k]) v = fma(lg, lq[k], v);
         rr = fma(term_coefs[2 * k], v, rr);
         ri = fma(term_coefs[2 * k + 1], v, ri);
@@ -1502,10 +1523,6 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
     const int order = sp->split_order < 1 ? 1 : sp->split_order;
     if (2 * (order - 1) > kSplitMaxTerms)
       return fail(VL1_E_ARG, "Helmholtz split plans: split_order must be 1..49");
-    if ((order > 1 || sp->smooth_order > 0) && sp->deriv_kind != VL1_SPLIT_DERIV_NONE)
-      return fail(VL1_E_UNSUPPORTED,
-                  "Helmholtz split plans: derivative wrappers take split order 1 and the base "
-                  "quadrature");
     if (sp->n_terms != order - 1)
       return fail(VL1_E_ARG, "Helmholtz split plans: n_terms must be split_order - 1");
     if (sp->n_terms > kSplitMaxBlockTerms)
@@ -1798,7 +1815,7 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
       const bool deriv = sp->deriv_kind != VL1_SPLIT_DERIV_NONE;
       const std::complex<double> kk(sp->k_re, sp->k_im);
       const int order = sp->split_order < 1 ? 1 : sp->split_order;
-      std::vector<double> coef = deriv ? split_derivative_coefficients(dim, kk, sp->series_nmax)
+      std::vector<double> coef = deriv ? split_derivative_coefficients(dim, kk, sp->series_nmax, order)
                                        : split_coefficients(dim, kk, sp->series_nmax, order);
       if (coef.empty()) coef.assign(2, 0.0);  // 3D derivative with one term: R' = 0
       DevBuf dal, ddi, dh, dvec, dun, duw, dco, dext;
@@ -1838,7 +1855,9 @@ static int plan_create(vl1_plan** out, const vl1_payload* pay, const vl1_plan_de
                              dun.as<double>(), duw.as<double>(), dco.as<double>(),
                              sp->series_nmax, ms, dsn.as<double>(), dsw.as<double>(),
                              dsi.as<double>(), nt, dtt.as<double>(), dtp.as<int32_t>(),
-                             dtl.as<int32_t>(), dtc.as<double>(), dext.as<double2>());
+                             dtl.as<int32_t>(), dtc.as<double>(), deriv ? sp->deriv_axis : -1,
+                             sp->deriv_kind == VL1_SPLIT_DERIV_SOURCE ? -1.0 : 1.0,
+                             pay->sign_mask, dext.as<double2>());
           e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(s);  // before the uploads are freed

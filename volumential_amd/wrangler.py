@@ -896,16 +896,11 @@ class NearFieldWrangler:
         # nodes (the "p2p" beta mode)
         beta_ok = (self.helmholtz_split_beta_mode == "p2p" or self.tree.dim != 2
                    or self.helmholtz_split_order == 1)
-        # derivative wrappers: the fused blocks at order 1 on the base nodes
-        deriv_ok = all(split_wrapper_chain(k)[1] == _capi.SPLIT_DERIV_NONE
-                       for k in self.target_kernels) or (
-            self.helmholtz_split_order == 1
-            and (self.helmholtz_split_smooth_quad_order or 0) <= int(self.tree.quad_order))
         # per-level Laplace tables fold in at split order 1 (the term tables of
         # higher orders are single-table)
         tables_ok = self.helmholtz_split_order == 1 or all(
             len(self.near_field_table[repr(k)]) == 1 for k in self.target_kernels)
-        if (fused and beta_ok and tables_ok and deriv_ok and not self._legacy()
+        if (fused and beta_ok and tables_ok and not self._legacy()
                 and self.helmholtz_split_order <= 17):
             # one contraction: complex blocks = table part + remainder quadrature
             # a split-form limit (slices vs the LDS budget) lets the fused

@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define VL1_ABI_VERSION 10
+#define VL1_ABI_VERSION 11
 
 typedef enum vl1_status {
   VL1_OK = 0,
@@ -204,6 +204,13 @@ typedef struct vl1_near_field_args {
    * log_displacement = 0 declares level_disp all zero (no displacement pass) */
   const double* level_scaling;
   const double* level_disp;
+  /* Optional (NULL = potential kind 1): potential kind 2, the inverse-type
+   * (hypersingular) potentials (list1.py:101-109, 586-595, 1097-1143): each
+   * target adds mode_coefs[target] x exterior_nmlz[lev][target mode] x the
+   * target box's scaling (its level's table and scaling, by the rules of
+   * the source boxes).  Device [n_tables][n]; real tables without a
+   * displacement (VL1_E_UNSUPPORTED otherwise). */
+  const double* exterior_nmlz;
 } vl1_near_field_args;
 
 /* Overwrites result[target] for every target of the listed boxes. */

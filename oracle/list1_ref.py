@@ -62,6 +62,7 @@ def near_field(
     bias=1e-10,
     level_scaling=None,
     level_disp=None,
+    exterior_nmlz=None,
 ):
     """``mode``: "canonical", "fixed", "per_level" or "custom"
     (list1.py:361-437).  "custom" is the caller's code strings evaluated per
@@ -126,6 +127,24 @@ def near_field(
             elif mode == "custom":
                 integ = integ + (float(level_disp[lvl]) * nmlz[0, :ns])[:, None]
             acc += coefs[:, sbeg : sbeg + ns] @ integ
+        if exterior_nmlz is not None:
+            # potential kind 2 (list1.py:1097-1105, 1141-1143): the target's
+            # density times its exterior normaliser, scaled as its own box
+            lvl_t = int(box_levels[tb])
+            text = root_extent / 2.0**lvl_t
+            if mode == "canonical":
+                scal_t, lev_t = (text / table_root_extent) ** scaling_power, 0
+            elif mode == "fixed":
+                scal_t, lev_t = 1.0, 0
+            elif mode == "custom":
+                scal_t, lev_t = float(level_scaling[lvl_t]), 0
+            else:
+                rel = lvl_t - table_starting_level
+                scal_t = 1.0
+                lev_t = 0 if lvl_t < table_starting_level else (rel if rel < n_tables else
+                                                                 n_tables - 1)
+            ext = np.atleast_2d(np.asarray(exterior_nmlz))[lev_t, tmodes] * scal_t
+            acc += coefs[:, tbeg : tbeg + nt] * ext[None, :]
         out[:, tbeg : tbeg + nt] = acc
     return out
 
@@ -195,7 +214,7 @@ def tree_groups(tree):
 
 def near_field_from_tree(tree, table_data, mode_nmlz, meta, mode_coefs, *, mode, scaling_power=2,
                          log_displacement=False, table_root_extent, table_starting_level=0,
-                         own_csr=True, level_scaling=None, level_disp=None):
+                         own_csr=True, level_scaling=None, level_disp=None, exterior_nmlz=None):
     """The kernel semantics on `tree`.  With `own_csr` (default) the list-1
     CSR of the tree's target boxes is recomputed by ``list1_csr_bruteforce``
     instead of taken from the tree builder."""
@@ -230,4 +249,5 @@ def near_field_from_tree(tree, table_data, mode_nmlz, meta, mode_coefs, *, mode,
         mode_coefs=mode_coefs,
         level_scaling=level_scaling,
         level_disp=level_disp,
+        exterior_nmlz=exterior_nmlz,
     )

@@ -20,7 +20,7 @@ def test_header_symbols_exported():
     assert declared == set(_capi.EXPORTED_SYMBOLS)
     for name in declared:
         assert hasattr(L, name), name
-    assert L.vl1_abi_version() == 10
+    assert L.vl1_abi_version() == 11
 
 
 def test_status_mapping():

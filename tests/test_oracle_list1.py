@@ -233,3 +233,43 @@ def test_oracle_csr_uniform_counts(dim, level):
     pairs = {(int(t), int(s)) for i, t in enumerate(tr.target_boxes)
              for s in lists[starts[i]:starts[i + 1]]}
     assert all((s, t) in pairs for t, s in pairs)
+
+
+@pytest.mark.parametrize("mode", ["canonical", "per_level"])
+def test_oracle_exterior_term(mode):
+    """Potential kind 2 (list1.py:1097-1143): with exterior normalisers 0
+    the oracle is the kind-1 sum; over a zero table it is each target's
+    density times exterior_nmlz[its box's table level, its mode] times its
+    box's scaling, on an adaptive tree with targets on several levels."""
+    from oracle import list1_ref
+    from tests.gpu_helpers import adaptive, synthetic_table_levels
+
+    tr = adaptive(3, 2)
+    n_tables = tr.nlevels if mode == "per_level" else 1
+    tables = synthetic_table_levels(3, 2, n_tables=n_tables, seed=4)
+    meta = orbit_ref.case_metadata(3, 2)
+    data = np.stack([t.compact_values() for t in tables])
+    nm = np.stack([t.mode_normalizers for t in tables])
+    rng = np.random.default_rng(2)
+    ext = rng.standard_normal((n_tables, tr.n_q_points))
+    f = rng.standard_normal((2, tr.n_points))
+    kw = dict(mode=mode, scaling_power=2, table_root_extent=2.0, own_csr=False)
+    one = list1_ref.near_field_from_tree(tr, data, nm, meta, f, **kw)
+    z = list1_ref.near_field_from_tree(tr, data, nm, meta, f, exterior_nmlz=np.zeros_like(ext),
+                                       **kw)
+    assert np.array_equal(one, z)
+    got = list1_ref.near_field_from_tree(tr, np.zeros_like(data), nm, meta, f, exterior_nmlz=ext,
+                                         **kw)
+    exp = np.zeros_like(f)
+    lv = np.asarray(tr.box_levels)
+    assert len(np.unique(lv[tr.target_boxes])) >= 2
+    for tb in tr.target_boxes:
+        s0, c = int(tr.box_target_starts[tb]), int(tr.box_target_counts_nonchild[tb])
+        ids = np.asarray(tr.local_ids[s0:s0 + c])
+        h = tr.root_extent / 2.0 ** lv[tb]
+        if mode == "canonical":
+            e = ext[0, ids] * (h / 2.0) ** 2
+        else:
+            e = ext[min(int(lv[tb]), n_tables - 1), ids]
+        exp[:, s0:s0 + c] = f[:, s0:s0 + c] * e[None, :]
+    np.testing.assert_allclose(got, exp, rtol=1e-15, atol=0)

@@ -158,6 +158,7 @@ class NearFieldArgs(C.Structure):
         ("error_flags", vp),
         ("level_scaling", vp),
         ("level_disp", vp),
+        ("exterior_nmlz", vp),
     ]
 
 
@@ -317,7 +318,7 @@ def lib():
     L.vl1_csr_builder_destroy.argtypes = [vp]
     L.vl1_orbit_rows_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
     L.vl1_orbit_signs_host.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, i64, vp]
-    if L.vl1_abi_version() != 10:
+    if L.vl1_abi_version() != 11:
         raise RuntimeError("libvl1.so ABI version mismatch; rebuild the library")
     _lib = L
     return L

@@ -557,6 +557,55 @@ static hipError_t launch_nf(const NFParams& prm, int grid, size_t smem, hipStrea
 
 using namespace vl1;
 
+// Potential kind 2 (list1.py:1097-1143): y[t] += f[t] x exterior_nmlz[lev_t][mode_t] x
+// the target box's scaling, after the list-1 sums.  One thread per (RHS,
+// target box, box-local target); real tables, no displacement.
+__global__ void exterior_term_kernel(const NFParams p, const double* __restrict__ ext,
+                                     int64_t ld_f, int64_t ld_y, int n_tgt) {
+  const int64_t total = (int64_t)p.n_rhs * n_tgt * p.n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(i % p.n);
+    const int tbox = (int)((i / p.n) % n_tgt);
+    const int r = (int)(i / ((int64_t)p.n * n_tgt));
+    const int tb = p.target_boxes[tbox];
+    if (t >= min(p.tgt_counts[tb], p.n)) continue;
+    const int lvl = p.levels[tb];
+    const double text = p.root_extent * (1.0 / (double)(1 << lvl));
+    double scal = 1.0;
+    int tlev = 0;
+    if (p.scaling_mode == VL1_SCALING_CANONICAL) {
+      double num = 1.0, den = 1.0;
+      for (int k = 0; k < p.scaling_power; ++k) {
+        num *= text;
+        den *= p.table_root_extent;
+      }
+      scal = num / den;
+    } else if (p.scaling_mode == VL1_SCALING_CUSTOM) {
+      scal = p.lsc[min(lvl, VL1_MAX_LEVELS - 1)];
+    } else if (p.scaling_mode == VL1_SCALING_PER_LEVEL) {
+      const int rel = lvl - p.table_starting_level;
+      tlev = lvl < p.table_starting_level ? 0 : (rel < p.n_tables ? rel : p.n_tables - 1);
+    }
+    const int64_t pt = (int64_t)p.tgt_starts[tb] + t;
+    const double e = ext[(int64_t)tlev * p.n + p.tgt_ids[pt]] * scal;
+    p.y[(int64_t)r * ld_y + pt] += p.f[(int64_t)r * ld_f + pt] * e;
+  }
+}
+
+static int exterior_pass(const NFParams& prm, const vl1_near_field_args* a, hipStream_t s) {
+  if (!a->exterior_nmlz) return VL1_OK;
+  const int64_t total = (int64_t)a->n_rhs * a->n_tgt_boxes * prm.n;
+  const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(exterior_term_kernel, dim3(grid), dim3(256), 0, s, prm, a->exterior_nmlz,
+                     a->n_rhs > 1 ? a->ld_coefs : 0, a->n_rhs > 1 ? a->ld_result : 0,
+                     a->n_tgt_boxes);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(VL1_E_HIP, std::string("exterior term launch: ") + hipGetErrorString(e));
+  return VL1_OK;
+}
+
 extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_args* a,
                                   void* stream) {
   clear_error();
@@ -580,6 +629,11 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
     return fail(VL1_E_ARG, "vl1_near_field_f64: null device array");
   if (a->n_rhs > 1 && (a->ld_coefs < 1 || a->ld_result < 1))
     return fail(VL1_E_ARG, "vl1_near_field_f64: bad leading dimension");
+  if (a->exterior_nmlz &&
+      (pl->is_complex || (a->log_displacement && (a->scaling_mode == VL1_SCALING_CANONICAL ||
+                                                  a->scaling_mode == VL1_SCALING_CUSTOM))))
+    return fail(VL1_E_UNSUPPORTED,
+                "vl1_near_field_f64: potential kind 2 takes real tables without a displacement");
 
   NFParams prm{};
   prm.centers = a->box_centers;
@@ -681,7 +735,7 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
       return fail(VL1_E_HIP, std::string("near_field_mfma launch: ") + hipGetErrorString(e));
-    return VL1_OK;
+    return exterior_pass(prm, a, st);
   }
   const size_t fixed =
       sizeof(double) * ((size_t)pl->n * RC * vw + pl->n + (P > 1 ? kNFBlock * RC * vw : 0)) +
@@ -706,5 +760,5 @@ extern "C" int vl1_near_field_f64(const vl1_payload* pl, const vl1_near_field_ar
 #undef VL1_NF_LAUNCH
   if (e != hipSuccess)
     return fail(VL1_E_HIP, std::string("near_field_f64 launch: ") + hipGetErrorString(e));
-  return VL1_OK;
+  return exterior_pass(prm, a, s);
 }

@@ -101,10 +101,13 @@ class NearFieldFromCSR:
     default_name = "near_field_from_csr"
 
     def __init__(self, integral_kernel, table_data_shapes, potential_kind=1, **kwargs):
-        if potential_kind != 1:
-            raise NotImplementedError(
-                "potential_kind=2 (exterior normalisers) is not part of this library"
-            )
+        """``potential_kind`` (list1.py:101-109): 1, the volume potentials; 2,
+        the inverse-type (hypersingular) potentials, which add each
+        target's density times its exterior mode normaliser scaled as its
+        box (``exterior_mode_nmlz_combined`` at call time; real tables
+        without a displacement term)."""
+        if potential_kind not in (1, 2):
+            raise ValueError("Unsupported potential kind %d" % potential_kind)
         self.integral_kernel = integral_kernel
         self.potential_kind = potential_kind
         self.n_tables = int(table_data_shapes["n_tables"])
@@ -381,6 +384,16 @@ class NearFieldFromCSR:
             target_point_ids=arrs["target_point_ids"].data_ptr(),
             error_flags=err.data_ptr(),
         )
+        if self.potential_kind == 2:
+            ext = kw.get("exterior_mode_nmlz_combined")
+            if ext is None:
+                raise ValueError("potential_kind=2 needs exterior_mode_nmlz_combined")
+            d_ext = _to_device(ext, torch.float64).reshape(-1).contiguous()
+            if payload.is_complex or d_ext.numel() != self.n_tables * self.n_q_points:
+                raise ValueError("exterior_mode_nmlz_combined must hold n_tables x n_q_points "
+                                 "real values (potential_kind=2 takes real tables)")
+            args.exterior_nmlz = d_ext.data_ptr()
+            self._keep_ext = d_ext
         lsc, ldisp = self.custom_level_arrays(kw["root_extent"], kw["table_root_extent"])
         if lsc is not None:
             key = (float(kw["root_extent"]), float(kw["table_root_extent"]), "device")

@@ -92,6 +92,7 @@ class NearFieldWrangler:
         helmholtz_split_term_build_config=None,
         helmholtz_split_smooth_quad_order=None,
         helmholtz_split_order1_legacy_subtraction=False,
+        potential_kind=1,
     ):
         """``direct_method``: how ``eval_direct`` evaluates list 1 --
         ``"plan"`` (default) through a plan cached per list-1 CSR, FP64
@@ -131,7 +132,14 @@ class NearFieldWrangler:
         instead of the series; self pairs take the r = 0 limit; a smooth
         quadrature order whose nodes meet the box nodes is refused
         (5125-5138).  The fused split plan (series quadrature in its blocks)
-        is not taken."""
+        is not taken.
+
+        ``potential_kind`` (2257, 2407, 3329): 2 for the inverse-type
+        potentials, whose list 1 adds the exterior mode normalisers of the
+        tables; evaluated by the plain route (``NearFieldFromCSR``)."""
+        if potential_kind not in (1, 2):
+            raise ValueError("Unsupported potential kind %d" % potential_kind)
+        self.potential_kind = int(potential_kind)
         if not getattr(tree, "sources_are_targets", False):
             raise ValueError(
                 "table-based near-field evaluation requires sources and targets "
@@ -376,7 +384,8 @@ class NearFieldWrangler:
                     "Insufficient list of tables: the coarsest level mesh cells at level "
                     f"{min_lev} are not covered."
                 )
-        if self.direct_method == "plan" and kname not in self._plain_only:
+        if (self.direct_method == "plan" and kname not in self._plain_only
+                and self.potential_kind == 1):
             key = _csr_key(out_kernel, target_boxes, starts, lists) + (False,)
             if key not in self._plans:  # host layout checks, once per CSR
                 _validate_layout(self.tree, np.asarray(_host(target_boxes), dtype=np.int64),
@@ -390,7 +399,7 @@ class NearFieldWrangler:
             if kname not in self._plain_only:
                 return self._eval_plan(out_pot, out_kernel, target_boxes, starts, lists,
                                        mode_coefs)
-        data, nmlz, _, _, _, info = prepare_table_data_and_entry_map(tables)
+        data, nmlz, ext, _, _, info = prepare_table_data_and_entry_map(tables)
         shapes = {
             "n_tables": len(tables),
             "n_q_points": t0.n_q_points,
@@ -403,6 +412,7 @@ class NearFieldWrangler:
         key = (kname, len(tables))
         if key not in self._evaluators:
             self._evaluators[key] = NearFieldFromCSR(self._policy_kernel(out_kernel), shapes,
+                                                     potential_kind=self.potential_kind,
                                                      **self.list1_extra_kwargs)
         nf = self._evaluators[key]
         tb = np.asarray(_host(target_boxes), dtype=np.int64)
@@ -423,6 +433,7 @@ class NearFieldWrangler:
             encoding_base=gallery.ENCODING_BASE,
             encoding_shift=gallery.ENCODING_SHIFT,
             mode_nmlz_combined=nmlz,
+            exterior_mode_nmlz_combined=ext,
             neighbor_source_boxes_starts=starts,
             root_extent=tree.root_extent,
             neighbor_source_boxes_lists=lists,
@@ -496,6 +507,10 @@ class NearFieldWrangler:
         from .payload import DevicePayload
         from .plan import List1Plan
 
+        if self.potential_kind != 1:
+            raise NotImplementedError(
+                "plans evaluate potential kind 1; potential_kind=2 runs through eval_direct "
+                "(the plain route)")
         key = _csr_key(out_kernel, target_boxes, neighbor_source_boxes_starts,
                        neighbor_source_boxes_lists) + (bool(fused_split),)
         if key in self._plans:

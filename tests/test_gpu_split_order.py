@@ -439,3 +439,32 @@ def test_derivative_wrapper_smooth_quadrature_matches_oracle(cuda, m):
     l2, linf = rel_errors(got, ref)
     print(f"vs oracle: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
     assert l2 <= 1e-12 and linf <= 1e-12
+
+
+def test_derivative_wrapper_table_beta_mode(cuda):
+    """The "table" beta mode under a derivative wrapper: the log(h/H) part
+    of each 2D r^2n log r term through the wrapped r^2n table (scaled by
+    (h/H)^(d+2n-1) log(h/H)).  At q = 4 the base quadrature of d_a r^2n
+    times a basis polynomial is exact for n <= 2, so it equals the p2p
+    mode to the tables' build accuracy."""
+    from volumential_amd.kernels import AxisTargetDerivative
+
+    print("case: table beta mode under a target derivative, 2D q=4 adaptive, order 3",
+          flush=True)
+    tr = adaptive(2, 4)
+    hk = AxisTargetDerivative(1, HelmholtzKernel(2, 6.0))
+    dtab = NearFieldInteractionTable(4, 2, AxisTargetDerivative(1, LaplaceKernel(2)),
+                                     source_box_extent=2.0)
+    dtab.build_table(RULE)
+    kw = dict(target_kernels=[hk], helmholtz_split_order=3, helmholtz_split_term_build_config=RULE)
+    f = torch.as_tensor(np.stack([densities.helmholtz_beams(tr.points, 11 + r, k=6.0, width=0.3)
+                                  for r in range(2)]), device="cuda")
+    args = (tr.target_boxes, tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists)
+    wt = NearFieldWrangler(tr, {repr(hk): [dtab]}, helmholtz_split_auto_config={
+        "power_log_single_table_beta_mode": "table"}, **kw)
+    got = wt.eval_direct_split(*args, f)[0][0].cpu().numpy()
+    wp = NearFieldWrangler(tr, {repr(hk): [dtab]}, **kw)
+    p2p = wp.eval_direct_split(*args, f, fused=False)[0][0].cpu().numpy()
+    l2, linf = rel_errors(got, p2p)
+    print(f"table vs p2p mode: rel L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+    assert l2 <= 1e-10 and linf <= 1e-10

@@ -466,7 +466,8 @@ typedef struct vl1_split_args {
    * smooth-quadrature path (5116-5275): per box src_box_counts[b] sources
    * from src_box_starts[b] (device), coordinates src_points [dim][n_src_points];
    * strengths then index these sources ([n_rhs][ld_strengths], complex),
-   * and no pair is a self pair (r = 0 takes the series value at 0). */
+   * and no pair is a self pair (r = 0 takes the series value at 0, or 0
+   * under a derivative wrapper). */
   const double* src_points;
   const int32_t* src_box_starts;
   const int32_t* src_box_counts;
